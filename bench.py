@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Throughput of the 2D DPD step on one MI355X.
+
+Workload: the plain method at scale -- N = 1,048,576 beads at rho = 3
+(L = 591.2067), 50% W, 20% precursor P, 30% H-T-T lipid beads (104,858
+lipids), precursor chemistry on every step (P -> T next to a lipid tail
+p = 0.005, T-T bond form p = 0.01 / break p = 0.001), dt = 0.01, harmonic
+bonds, no warm-up ramp; synthetic random soup, all state resident in HBM.
+
+One step = integrate + cell sort + chemistry + pair forces + bonded forces
+for every bead.  W untimed steps, then K timed steps replayed from a hipGraph,
+with a device synchronisation on both sides of the timed region.  The pair
+kernel is then timed alone with HIP events on the library's stream for the
+roofline; the reference engine (compiled from its source) is timed on the host
+cores on a bounded sample of the same physics as the CPU baseline.
+"""
+import argparse
+import ctypes as C
+import importlib
+import json
+import math
+import os
+import re
+import shutil
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "2D DPD particle-steps/s, N=1M ρ=3; batched-ensemble & adjoint vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def config2_params(pm, N, nsub, chem=True):
+    p = pm.Params()
+    p.update({
+        "L": repr(pm.L_for_N(N)), "rho": "3", "dt": "0.01", "lambda": "0.65", "gamma": "4.5", "kT": "1",
+        "k_bond": "100", "r0": "0.5", "k_angle": "0", "lip_nH": "1", "lip_nT": "2", "f_lip": "0.3", "c_N": "0.2",
+        "f_Ka": "0", "f_Kb": "0", "a_W_W": "25", "a_H_H": "25", "a_T_T": "25", "a_W_H": "25", "a_W_T": "80",
+        "a_H_T": "80", "a_N_W": "25", "a_N_H": "25", "a_N_N": "25", "a_N_T": "50", "warmup": "0", "chem": "0",
+        "pchem": "1" if chem else "0", "n_chem": "1", "p_conv": "0.005", "r_conv": "0.7", "p_form": "0.01",
+        "r_form": "0.6", "p_break": "0.001", "nsub": str(nsub), "seed": "1",
+    })
+    return p.finalize()
+
+
+class HipEvents:
+    """hipEvent timing through the HIP runtime (the library's stream)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def new(self):
+        e = C.c_void_p()
+        rc = self.hip.hipEventCreate(C.byref(e))
+        if rc:
+            raise RuntimeError(f"hipEventCreate failed ({rc})")
+        return e
+
+    def ms(self, a, b):
+        self.hip.hipEventSynchronize(b)
+        t = C.c_float()
+        self.hip.hipEventElapsedTime(C.byref(t), a, b)
+        return t.value
+
+
+def time_pair_kernel(e, reps=50):
+    """Average duration (s) of one pair-kernel launch, from HIP events recorded
+    on the stream the kernel runs on."""
+    ev = HipEvents()
+    a, b = ev.new(), ev.new()
+    stream = C.c_void_p(e.stream)
+    e.lib.oc_launch_pair(e.h, 3)
+    ev.hip.hipEventRecord(a, stream)
+    rc = e.lib.oc_launch_pair(e.h, reps)
+    ev.hip.hipEventRecord(b, stream)
+    if rc:
+        raise RuntimeError(e.lib.oc_last_error().decode())
+    ms = ev.ms(a, b)
+    ev.hip.hipEventDestroy(a)
+    ev.hip.hipEventDestroy(b)
+    return ms * 1e-3 / reps
+
+
+def cpu_baseline(pm, N, nsub, target_s=12.0):
+    """The reference engine (oracle/_ref, built from cel/engine/oercel.c with its
+    own -O3 flags) on the host cores, same physics with P as its type N and
+    chemistry off (it has no precursor chemistry).  Falls back to the oracle
+    restatement when the reference binaries were not built."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    p = config2_params(pm, N, nsub, chem=False)
+    threads = max(1, min(16, os.cpu_count() or 1, int(math.floor(p.L)) // 2))
+    binp = oracle.ref_binary(fast=True)
+    if os.path.exists(binp):
+        ov = {"L": repr(p.L), "dt": "0.01", "chem": "0", "f_lip": "0.3", "lip_nH": "1", "lip_nT": "2",
+              "f_Ka": "0", "f_Kb": "0", "c_N": "0.2", "a_W_H": "25", "a_H_T": "80", "a_W_T": "80", "a_N_T": "50",
+              "a_N_W": "25", "a_N_H": "25", "k_angle": "0", "warmup": "0", "snap_every": "0", "stats_every": "0",
+              "ckpt_every": "0", "log_every": "0", "events": "0", "threads": str(threads), "nsub": str(nsub),
+              "seed": "1"}
+        tmp = tempfile.mkdtemp(prefix="ocref_")
+        try:
+            def run(steps):
+                out = oracle.run_ref_cli(os.path.join(tmp, f"r{steps}"), dict(ov, steps=str(steps)), fast=True)
+                m = re.search(r"timing: (\d+) steps, ([0-9.]+) ms/step", out)
+                return float(m.group(2))
+            ms = run(10)
+            steps = int(max(20, min(3000, target_s / (ms * 1e-3))))
+            ms = run(steps)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        return {"value": N / (ms * 1e-3), "unit": "particle-steps/s", "cores": threads, "kind": "reference",
+                "sample": f"reference CLI {os.path.basename(binp)} (cel/engine/oercel.c, -O3), N={N}, same physics "
+                          f"with P as type N, chemistry off, {steps} steps after setup, {threads} OpenMP threads, "
+                          f"{ms:.3f} ms/step"}
+    # restatement fallback
+    cp = p.to_c()
+    sysm = oracle.System(cp, N)
+    sysm.start(oracle.soup(cp, N))
+    t0 = time.perf_counter()
+    sysm.step(5)
+    dt = (time.perf_counter() - t0) / 5
+    return {"value": N / dt, "unit": "particle-steps/s", "cores": os.cpu_count() or 1, "kind": "port",
+            "sample": f"oracle restatement, N={N}, chemistry off, 5 steps"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--N", type=int, default=1048576)
+    ap.add_argument("--nsub", type=int, default=4)
+    ap.add_argument("--pair-block", type=int, default=0)
+    ap.add_argument("--tile-beads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-graph", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the step on a single MI355X "
+              "(multi-GPU domain decomposition is a design note, not implemented).", file=sys.stderr)
+        sys.exit(2)
+
+    pm = importlib.import_module("welcome-youtube_amd.params")
+    eng = importlib.import_module("welcome-youtube_amd.engine")
+    p = config2_params(pm, a.N, a.nsub)
+    N = p.N
+    e = eng.Engine(p, pair_block=a.pair_block, tile_beads=a.tile_beads)
+    e.start()
+    e.step(a.warmup, graph=not a.no_graph)
+    e.step_count()  # device synchronised
+    t0 = time.perf_counter()
+    e.step(a.steps, graph=not a.no_graph)  # returns after a stream synchronisation
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    st = e.stats()
+    ev = e.events()
+    pair_s = time_pair_kernel(e)
+    ncell = e.cells()[0].size - 1
+    bytes_pair = N * (16 + 8 + 4 + 16) + 4 * (ncell + 1)
+    achieved = bytes_pair / pair_s / 1e9
+    e.close()
+
+    cpu = None if a.no_cpu else cpu_baseline(pm, N, a.nsub)
+    out = {
+        "metric": METRIC,
+        "value": N * a.steps / sec,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * sec / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain 2D DPD step at scale: N=1,048,576, rho=3, L=591.2067, 50% W / 20% P / 30% H-T-T "
+                        "lipid beads, precursor chemistry every step, dt=0.01, k_angle=0, warmup=0",
+            "N": N, "L": p.L, "nsub": a.nsub, "graph": not a.no_graph,
+            "state": "fp64 positions/velocities/forces, fp32 pair arithmetic",
+            "events_in_timed_region": int(len(ev)), "T_kin_end": st["T_kin"],
+            "momentum_end": [st["px"], st["py"]],
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "k_pair", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "avg_launch_us": pair_s * 1e6, "algorithmic_bytes_per_launch": bytes_pair,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

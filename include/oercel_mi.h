@@ -1,0 +1,196 @@
+/*
+ * oercel_mi.h -- C ABI of the MI355X (gfx950) 2D DPD step library.
+ *
+ * The reference engine (cel/engine/oercel.c) has no FFI: its step is a set of
+ * static C functions called from main()'s loop (cel/engine/oercel.c:2300-2362)
+ * and, in its tests, in-process via "#define main ...; #include"
+ * (cel/engine/tests/forcetest.c:3-5).  This header is the boundary a caller of
+ * that step binds instead: plain C types, pointers and sizes, integer return
+ * codes (0 = ok) plus oc_last_error(), one engine handle per set of replicas,
+ * one HIP stream per handle, called from one host thread.
+ *
+ * A single system is an ensemble of one replica; every entry point that takes
+ * a replica index accepts 0 for it.
+ *
+ * State exchanged with the caller is in gid (permanent bead id) order, the
+ * order the reference's snapshots and checkpoints use
+ * (cel/engine/oercel.c:1830-1936); inside the library it lives in cell-sorted
+ * slot order, re-sorted every step.
+ */
+#ifndef OERCEL_MI_H
+#define OERCEL_MI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OC_ABI_VERSION 1
+#define OC_NTYPES 7
+#define OC_MAXMLEN 12
+
+/* bead types, cel/engine/oercel.c:47-49.  In the precursor chemistry the
+ * precursor monomer P is type 3 (the reference's N row of a_ij). */
+enum { OC_W = 0, OC_H = 1, OC_T = 2, OC_N = 3, OC_KA = 4, OC_KB = 5, OC_Z = 6 };
+
+/* error codes */
+enum {
+    OC_OK = 0,
+    OC_EINVAL = 1,   /* bad argument or parameter */
+    OC_ENOMEM = 2,   /* device or host allocation failed */
+    OC_EHIP = 3,     /* HIP runtime error */
+    OC_EDEVICE = 4,  /* sticky device-side error (event overflow, broken chain, ...) */
+    OC_ESTATE = 5    /* call made in the wrong state (e.g. step before any state) */
+};
+
+/* event kinds of the precursor chemistry (no counterpart in the reference) */
+enum { OC_EV_CONV = 0, OC_EV_FORM = 1, OC_EV_BREAK = 2 };
+
+/*
+ * Parameters of one replica: the numeric subset of the reference's Params
+ * (cel/engine/oercel.c:178-221) that the DPD step reads, after the ini/key=value
+ * rules of set_param/finalize_params (cel/engine/oercel.c:338-395, 578-619) have
+ * been applied by the host (the Python package mirrors those rules).
+ * Lipids are linear chains of lip_nH heads then lip_nT tails
+ * (cel/engine/oercel.c:485-499).
+ */
+typedef struct oc_params {
+    /* physics */
+    double L, rho, dt, lambda, gamma, kT;
+    double k_bond, r0, k_angle;
+    double a[8][8];            /* symmetric repulsion matrix; row/column 7 unused */
+    double stir_A;             /* Kolmogorov stirring amplitude (0 = off) */
+    double kT_amp, kT_period;  /* temperature cycle kT(t) = kT + kT_amp sin(2 pi t / kT_period) */
+    int32_t stir_n;
+    int32_t nsub;              /* y sub-cells per unit cell, 1..8 (performance knob) */
+    int32_t lip_nH, lip_nT;
+    int64_t warmup;            /* repulsion ramp length (steps), cel/engine/oercel.c:1122-1129 */
+    int64_t warm_end;          /* step at which the ramp ends (0 = none) */
+    uint64_t seed;
+    /* soup composition (init only), cel/engine/oercel.c:1774-1814 */
+    double f_lip, f_Ka, f_Kb, c_N;
+    /* precursor chemistry: P -> T near a lipid tail, T-T bond form / break */
+    int32_t chem;              /* 1 = on */
+    int32_t n_chem;            /* check interval (steps); probabilities are per check */
+    double p_conv, r_conv;     /* P within r_conv of a lipid T becomes a free T */
+    double p_form, r_form;     /* two free T within r_form bond */
+    double p_break;            /* a chemistry bond breaks */
+    /* time axis: t(step) = t_anchor + (step - step_anchor) * dt */
+    double t_anchor;
+    int64_t step_anchor;
+} oc_params;
+
+/*
+ * Per-bead state in gid order.  Any pointer may be NULL in oc_get_state (that
+ * field is skipped); oc_set_state needs all but fx/fy (NULL = zero forces).
+ *   vx, vy : half-step velocity vh = v - dt/2 f (cel/engine/oercel.c:2207-2211)
+ *   mpos   : position in the lipid (0 = first head), -1 = not a lipid bead
+ *   nxt    : gid of the next bead of the lipid chain, -1 = none
+ *   bond   : gid of the chemistry-bond partner, -1 = none (only for mpos = -1)
+ */
+typedef struct oc_state {
+    double *x, *y, *vx, *vy, *fx, *fy;
+    uint8_t *type;
+    int8_t *mpos;
+    int32_t *nxt;
+    int32_t *bond;
+} oc_state;
+
+/* one chemistry event; kind OC_EV_CONV has gid_b = gid_a */
+typedef struct oc_event {
+    int64_t step;
+    int32_t kind;
+    uint32_t gid_a, gid_b;
+    int32_t pad;
+} oc_event;
+
+/* stats of one replica (cel/engine/oercel.c:2053-2071 kinetic_T/count_types,
+ * plus an energy the reference does not compute) */
+typedef struct oc_stats {
+    int64_t step;
+    double time;
+    double T_kin;       /* sum (vh + dt/2 f)^2 / (2N - 2) */
+    double px, py;      /* total momentum of the full velocity */
+    double E_kin;       /* 1/2 sum v^2 */
+    double E_pair;      /* sum_{i<j} a_ij/2 (1 - r)^2 at the current warm-up ramp */
+    double E_bond;      /* harmonic bonds + angle terms */
+    double max_bond;    /* running max bond length since the last oc_stats call */
+    int64_t count[8];   /* beads per type */
+    int64_t n_lipids;   /* beads with mpos == 0 */
+    int64_t n_bonds;    /* chemistry bonds */
+} oc_stats;
+
+typedef struct oc_engine oc_engine;
+
+const char *oc_last_error(void);
+int oc_abi_version(void);
+
+/* Engine over nrep replicas; p[r] and N[r] describe replica r.  Allocates all
+ * device memory up front (nothing is allocated by oc_step).
+ * Replaces sys_alloc (cel/engine/oercel.c:714-759) for one or many systems. */
+int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engine **out);
+void oc_destroy(oc_engine *h);
+
+/* Tuning knobs (0 = library default): pair-kernel workgroup size (64/128/256)
+ * and target i-beads per pair tile. */
+int oc_set_tuning(oc_engine *h, int32_t pair_block, int32_t tile_beads);
+
+/* Host-side soup of replica r in gid order, exactly as init_soup +
+ * set_velocities (cel/engine/oercel.c:1757-1814) with rng_seed(seed)
+ * (cel/engine/oercel.c:2193-2195); velocities are full velocities v.
+ * Needs no GPU. */
+int oc_soup_host(const oc_params *p, int32_t N, oc_state *out);
+
+/* Fresh start of every replica from full velocities v (gid order arrays are
+ * concatenated over replicas): v~ = v, sort, forces at step 0 (warm-up ramp
+ * starts), vh = v - dt/2 f.  cel/engine/oercel.c:2196-2212. */
+int oc_start(oc_engine *h, const oc_state *s);
+
+/* Load / read replica r's state (gid order).  oc_set_state mirrors a restart
+ * from a checkpoint (cel/engine/oercel.c:1942-2010): vh and f as stored. */
+int oc_set_state(oc_engine *h, int32_t r, const oc_state *s, int64_t step);
+int oc_get_state(oc_engine *h, int32_t r, oc_state *s);
+
+/* n full steps of every replica (integrate, sort, chemistry, pair forces,
+ * bonded forces): the loop body of cel/engine/oercel.c:2300-2362.
+ * use_graph = 1 captures a step pair in a hipGraph and replays it. */
+int oc_step(oc_engine *h, int64_t n, int32_t use_graph);
+
+/* Single phases on the current state, for parity tests. */
+int oc_sort(oc_engine *h);                       /* sort_cells, :828-933 */
+int oc_forces(oc_engine *h, int64_t step, int32_t with_bonded); /* pair_forces (+ bonded_forces), :1198-1216, :1317-1343 */
+int oc_chem(oc_engine *h, int64_t step);         /* precursor chemistry at this step (no counterpart) */
+
+/* Cell list of replica r: cstart[ncell+1] (slot indices relative to the
+ * replica) and gid_of_slot[N]; ncell = floor(L) * floor(L) * nsub, column-major
+ * (cel/engine/oercel.c:739-745, 779-786). */
+int oc_get_cells(oc_engine *h, int32_t r, int32_t *cstart, uint32_t *gid_of_slot, int32_t *ncell);
+
+/* Chemistry events of replica r not yet returned, sorted by (step, kind,
+ * gid_a, gid_b).  n_out receives how many are pending; the first min(cap, n)
+ * are copied to ev and removed (ev = NULL or cap = 0 only counts). */
+int oc_get_events(oc_engine *h, int32_t r, oc_event *ev, int64_t cap, int64_t *n_out);
+
+int oc_get_stats(oc_engine *h, int32_t r, oc_stats *out);
+
+/* Current step counter of replica r. */
+int oc_get_step(oc_engine *h, int32_t r, int64_t *step);
+
+/* Launch only the pair-force kernel reps times on the handle's stream, for
+ * the current state and step, without synchronising: lets a caller time that
+ * kernel alone with events on oc_stream(). */
+int oc_launch_pair(oc_engine *h, int32_t reps);
+
+/* sizeof the ABI structs, for binding checks: out[0..3] = oc_params,
+ * oc_state, oc_event, oc_stats (n = room in out); returns how many it wrote. */
+int oc_abi_sizes(int32_t *out, int32_t n);
+
+/* HIP stream of the handle (hipStream_t as void*), for callers that overlap
+ * their own work or time kernels with events on it. */
+void *oc_stream(oc_engine *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

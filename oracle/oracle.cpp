@@ -1,0 +1,817 @@
+/*
+ * oracle.cpp -- TEST INFRASTRUCTURE ONLY.  CPU (FP64) restatement of the 2D DPD
+ * step of the reference engine cel/engine/oercel.c, used as the checker of the
+ * HIP library.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg may load it; the product never does.
+ *
+ * Every function cites the reference lines it restates.  Parts with no
+ * reference counterpart (the precursor chemistry, the energy, the full-shell
+ * traversal order) are this project's own definitions and say so.
+ *
+ * Pinning (what checks this file, see tests/):
+ *   hashes, soup init, cell keys + stable sort : bit-exact against golden
+ *       vectors written by the reference itself (tests/golden), and the
+ *       feed known-answer count 1138 (cel/engine/README.md:519-520);
+ *   pair forces   : golden reference forces (<= 1e-10 rel), O(N^2) brute force,
+ *       sum F = 0, noise moments;
+ *   bonded forces : golden reference forces, central finite differences of an
+ *       independently written energy;
+ *   integrator + full step : golden reference trajectories (20 steps);
+ *   reductions    : closed forms on synthetic inputs;
+ *   precursor chemistry (or_chem) : PARITY UNPINNED against the reference (it has
+ *       no such chemistry); pinned only by brute-force neighbour search and
+ *       binomial rate checks in the tests.
+ *
+ * Built with -ffp-contract=off so that the arithmetic is exactly the written
+ * expression order (the HIP library is built the same way for its FP64 parts).
+ */
+#include <math.h>
+#include <omp.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../include/oercel_mi.h"
+
+#define OR_API extern "C" __attribute__((visibility("default")))
+
+static const double SQRT12 = 3.4641016151377545870548926830117;
+
+/* ------------------------------------------------------------------ hashes */
+
+/* splitmix64 finaliser, cel/engine/oercel.c:123-128 */
+static inline uint64_t mix13(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+/* per-step pair-noise key, cel/engine/oercel.c:130-133 */
+static inline uint64_t step_seed(uint64_t seed, int64_t step) {
+    return mix13(mix13(seed ^ 0x5851f42d4c957f2dULL) ^ mix13((uint64_t)step + 0x632be59bd9b4e019ULL));
+}
+static inline uint64_t pair_hash(uint64_t key, uint32_t gi, uint32_t gj) {
+    const uint64_t k = gi < gj ? (((uint64_t)gi << 32) | gj) : (((uint64_t)gj << 32) | gi);
+    return mix13((k * 0x9e3779b97f4a7c15ULL) ^ key);
+}
+/* symmetric pair noise on [-sqrt3, sqrt3), cel/engine/oercel.c:135-141 */
+static inline double pair_theta(uint64_t sseed, uint32_t gi, uint32_t gj) {
+    return (double)(int64_t)pair_hash(sseed, gi, gj) * (SQRT12 / 18446744073709551616.0);
+}
+/* the same deviate from the top 32 bits of the hash: the form the device
+ * computes in FP32 (cel/engine_gpu/oercel_gpu.cu:246-251 uses the same cut) */
+static inline double pair_theta32(uint64_t sseed, uint32_t gi, uint32_t gj) {
+    return (double)(int32_t)(uint32_t)(pair_hash(sseed, gi, gj) >> 32) * (SQRT12 / 4294967296.0);
+}
+/* counter-based per-bead uniform, cel/engine/oercel.c:819-826 */
+static inline uint64_t hash_key(uint64_t seed, int64_t step, uint64_t salt) {
+    return mix13(seed ^ mix13((uint64_t)step * 0x9e3779b97f4a7c15ULL + salt));
+}
+static inline double hash_u01(uint64_t key, uint32_t g) {
+    return (double)(mix13(key ^ ((uint64_t)g * 0xd1b54a32d192ed03ULL)) >> 11) * 0x1.0p-53;
+}
+/* symmetric per-pair uniform for bond formation (own definition; same hash
+ * shape as pair_theta, keyed by a chemistry salt) */
+static inline double pair_u01(uint64_t key, uint32_t gi, uint32_t gj) {
+    return (double)(pair_hash(key, gi, gj) >> 11) * 0x1.0p-53;
+}
+
+/* chemistry salts (own; the reference uses 11-14, its GPU port 21-25 and 40+) */
+enum { SALT_CONV = 31, SALT_FORM = 32, SALT_BREAK = 33 };
+
+OR_API uint64_t or_mix13(uint64_t z) { return mix13(z); }
+OR_API uint64_t or_step_seed(uint64_t seed, int64_t step) { return step_seed(seed, step); }
+OR_API double or_pair_theta(uint64_t s, uint32_t gi, uint32_t gj) { return pair_theta(s, gi, gj); }
+OR_API double or_pair_theta32(uint64_t s, uint32_t gi, uint32_t gj) { return pair_theta32(s, gi, gj); }
+OR_API uint64_t or_hash_key(uint64_t seed, int64_t step, uint64_t salt) { return hash_key(seed, step, salt); }
+OR_API double or_hash_u01(uint64_t key, uint32_t g) { return hash_u01(key, g); }
+OR_API double or_pair_u01(uint64_t key, uint32_t gi, uint32_t gj) { return pair_u01(key, gi, gj); }
+
+/* Known-answer count of the uniform food supply on pure water
+ * (cel/engine/oercel.c:1713-1753 with feed_cap = 0 and no decay): a bead
+ * converts W -> N at the first check whose hash_u01(hash_key(seed, step, 14),
+ * gid) < p_feed, so the total is the number of gids with any success. */
+OR_API int64_t or_feed_count(uint64_t seed, int32_t N, double p_feed, int64_t n_chem, int64_t steps) {
+    std::vector<uint8_t> done(N, 0);
+    int64_t tot = 0;
+    for (int64_t st = n_chem; st <= steps; st += n_chem) {
+        const uint64_t kf = hash_key(seed, st, 14);
+        for (int32_t g = 0; g < N; g++)
+            if (!done[g] && hash_u01(kf, (uint32_t)g) < p_feed) { done[g] = 1; tot++; }
+    }
+    return tot;
+}
+
+/* ------------------------------------------------------------------ serial RNG (init only) */
+
+/* xoshiro256**, splitmix seeding, Box-Muller: cel/engine/oercel.c:143-174 */
+struct Rng { uint64_t s[4]; };
+static inline uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+static uint64_t rng_next(Rng *r) {
+    uint64_t *s = r->s;
+    const uint64_t result = rotl64(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl64(s[3], 45);
+    return result;
+}
+static void rng_seed(Rng *r, uint64_t seed) {
+    uint64_t z = seed;
+    for (int k = 0; k < 4; k++) {
+        z += 0x9e3779b97f4a7c15ULL;
+        r->s[k] = mix13(z);
+    }
+}
+static double rng_unif(Rng *r) { return (double)(rng_next(r) >> 11) * 0x1.0p-53; }
+static double rng_normal(Rng *r) {
+    double u1 = rng_unif(r), u2 = rng_unif(r);
+    if (u1 < 1e-300) u1 = 1e-300;
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
+/* ------------------------------------------------------------------ geometry */
+
+struct Geo {
+    int N, ncx, ncy, ncell;
+    double L, icsx, icsy, csx, csy;
+};
+/* cells: ncx = floor(L) columns, ncy = floor(L)*nsub sub-cells, column-major,
+ * cel/engine/oercel.c:739-745 */
+static Geo make_geo(const oc_params *p, int N) {
+    Geo g;
+    g.N = N;
+    g.L = p->L;
+    g.ncx = (int)floor(p->L);
+    g.ncy = (int)floor(p->L) * p->nsub;
+    g.ncell = g.ncx * g.ncy;
+    g.csx = p->L / g.ncx;
+    g.csy = p->L / g.ncy;
+    g.icsx = g.ncx / p->L;
+    g.icsy = g.ncy / p->L;
+    return g;
+}
+/* cel/engine/oercel.c:761-786 */
+static inline void wrap1(double *q, double L) {
+    double v = *q;
+    if (v < 0.0) {
+        v += L;
+        if (v >= L) v -= L;
+    } else if (v >= L) {
+        v -= L;
+        if (v >= L || v < 0.0) v = fmod(v, L);
+    }
+    *q = v;
+}
+static inline double minimg(double d, double L) {
+    const double h = 0.5 * L;
+    if (d > h) d -= L;
+    else if (d < -h) d += L;
+    return d;
+}
+/* the cell key of sort_cells, cel/engine/oercel.c:860-866 */
+static inline int cell_key(const Geo &g, double x, double y) {
+    int cx = (int)(x * g.icsx), cy = (int)(y * g.icsy);
+    cx = cx >= g.ncx ? g.ncx - 1 : cx;
+    cy = cy >= g.ncy ? g.ncy - 1 : cy;
+    return cx * g.ncy + cy;
+}
+
+OR_API void or_cell_keys(const oc_params *p, int32_t N, const double *x, const double *y, int32_t *key) {
+    const Geo g = make_geo(p, N);
+    for (int i = 0; i < N; i++) key[i] = cell_key(g, x[i], y[i]);
+}
+
+/* Stable counting sort by key: src[new slot] = old slot, cstart[ncell+1].
+ * The unique stable order that sort_cells produces, cel/engine/oercel.c:828-933. */
+OR_API void or_sort(int32_t N, int32_t ncell, const int32_t *key, int32_t *src, int32_t *cstart) {
+    std::vector<int32_t> cnt(ncell + 1, 0);
+    for (int i = 0; i < N; i++) cnt[key[i]]++;
+    int run = 0;
+    for (int c = 0; c < ncell; c++) {
+        cstart[c] = run;
+        run += cnt[c];
+    }
+    cstart[ncell] = run;
+    std::vector<int32_t> fill(cstart, cstart + ncell);
+    for (int i = 0; i < N; i++) src[fill[key[i]]++] = i;
+}
+
+/* ------------------------------------------------------------------ parameters of one step */
+
+/* warm-up ramp of the repulsion, cel/engine/oercel.c:1122-1129 */
+static double a_eff(const oc_params *p, int i, int j, int64_t step) {
+    const double a = p->a[i][j];
+    if (step >= p->warm_end || p->warmup <= 0 || a <= 25.0) return a;
+    const double f = 1.0 - (double)(p->warm_end - step) / (double)p->warmup;
+    return 25.0 + (a - 25.0) * (f < 0 ? 0 : f);
+}
+/* cel/engine/oercel.c:622-624 */
+static inline double kT_at(const oc_params *p, double t) {
+    return p->kT_amp == 0.0 ? p->kT : p->kT + p->kT_amp * sin(2.0 * M_PI * t / p->kT_period);
+}
+static inline double time_of(const oc_params *p, int64_t step) {
+    return p->t_anchor + (double)(step - p->step_anchor) * p->dt;
+}
+
+/* ------------------------------------------------------------------ pair forces */
+
+struct PairPar {
+    double a[64];
+    double gamma, sdt;
+    uint64_t sseed;
+};
+/* cel/engine/oercel.c:1198-1205 */
+static PairPar pair_par(const oc_params *p, int64_t step, double t) {
+    PairPar pp;
+    memset(&pp, 0, sizeof pp);
+    for (int i = 0; i < OC_NTYPES; i++)
+        for (int j = 0; j < OC_NTYPES; j++) pp.a[8 * i + j] = a_eff(p, i, j, step);
+    pp.gamma = p->gamma;
+    pp.sdt = sqrt(2.0 * p->gamma * kT_at(p, t)) / sqrt(p->dt);
+    pp.sseed = step_seed(p->seed, step);
+    return pp;
+}
+
+/* one pair term on bead i from j (dx = x_i - x_j image), the arithmetic of
+ * pair_seg, cel/engine/oercel.c:968-979; returns the force on i */
+static inline void pair_term(const PairPar &pp, int theta32, double dx, double dy, double dvx, double dvy, int ti, int tj,
+                             uint32_t gi, uint32_t gj, double *fx, double *fy, double *e, double *sabs) {
+    const double r2 = dx * dx + dy * dy;
+    if (!(r2 < 1.0 && r2 > 1e-24)) return;
+    const double r = sqrt(r2);
+    const double ir = 1.0 / r;
+    const double w = 1.0 - r;
+    const double rv = (dx * dvx + dy * dvy) * ir;
+    const double th = theta32 ? pair_theta32(pp.sseed, gi, gj) : pair_theta(pp.sseed, gi, gj);
+    const double a = pp.a[8 * ti + tj];
+    const double fm = w * (a - pp.gamma * w * rv + pp.sdt * th) * ir;
+    *fx += fm * dx;
+    *fy += fm * dy;
+    if (e) *e += 0.25 * a * w * w; /* half of a/2 (1-r)^2: every pair is visited from both sides */
+    if (sabs) *sabs += fabs(fm) * r;
+}
+
+/* Stirring force minus its mean over NBSUM fixed blocks,
+ * cel/engine/oercel.c:1131-1196 (sin(2 pi u) through libm here). */
+static void stir_init(const oc_params *p, int N, const double *y, double *fx, double *fy) {
+    for (int i = 0; i < N; i++) { fx[i] = 0.0; fy[i] = 0.0; }
+    if (p->stir_A == 0.0) return;
+    const double A = p->stir_A, kn = (double)p->stir_n / p->L;
+    double tot = 0.0;
+    for (int b = 0; b < 64; b++) {
+        const int lo = (int)(((long)b * N) / 64), hi = (int)(((long)(b + 1) * N) / 64);
+        double sum = 0.0;
+        for (int i = lo; i < hi; i++) {
+            const double f = A * sin(2.0 * M_PI * (kn * y[i]));
+            fx[i] = f;
+            sum += f;
+        }
+        tot += sum;
+    }
+    const double mean = tot / N;
+    for (int i = 0; i < N; i++) fx[i] -= mean;
+}
+
+/*
+ * Full-shell pair forces on slot-ordered, cell-sorted arrays (own traversal:
+ * the reference visits each pair once from a half shell, pair_column,
+ * cel/engine/oercel.c:1054-1120; the device and this oracle visit both
+ * orientations).  For bead i in column cx the candidates are, per column
+ * cx-1, cx, cx+1, the sub-cells floor((y_i-1)/csy) .. floor((y_i+1)/csy)
+ * (the window expressions of cel/engine/oercel.c:1073-1074), each piece with
+ * the periodic image shift of its column / rows.  Forces are added to fx, fy
+ * (which the caller initialises, e.g. with stir_init).
+ */
+OR_API void or_pair_forces(const oc_params *p, int32_t N, const double *x, const double *y, const double *vtx,
+                           const double *vty, const uint8_t *type, const uint32_t *gid, const int32_t *cstart,
+                           int64_t step, int32_t theta32, int32_t init_f, double *fx, double *fy, double *epot,
+                           double *sabs) {
+    const Geo g = make_geo(p, N);
+    const PairPar pp = pair_par(p, step, time_of(p, step));
+    if (init_f) stir_init(p, N, y, fx, fy);
+    const int ncx = g.ncx, ncy = g.ncy;
+    const double L = g.L;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int cx = 0; cx < ncx; cx++) {
+        for (int i = cstart[cx * ncy]; i < cstart[(cx + 1) * ncy]; i++) {
+            const double xi = x[i], yi = y[i];
+            const int khi = (int)((yi + 1.0) * g.icsy);
+            const int klo = (int)((yi - 1.0) * g.icsy + ncy) - ncy;
+            double ax = 0.0, ay = 0.0, e = 0.0, sa = 0.0;
+            for (int dc = -1; dc <= 1; dc++) {
+                int col = cx + dc;
+                double sx = 0.0;
+                if (col < 0) { col += ncx; sx = -L; }
+                else if (col >= ncx) { col -= ncx; sx = L; }
+                const int base = col * ncy;
+                /* pieces: rows below 0 (image -L), rows 0..ncy-1, rows >= ncy (image +L) */
+                int k0[3], k1[3];
+                double sy[3];
+                int np = 0;
+                if (klo < 0) { k0[np] = klo + ncy; k1[np] = ncy - 1; sy[np] = -L; np++; }
+                k0[np] = klo < 0 ? 0 : klo; k1[np] = khi >= ncy ? ncy - 1 : khi; sy[np] = 0.0; np++;
+                if (khi >= ncy) { k0[np] = 0; k1[np] = khi - ncy; sy[np] = L; np++; }
+                for (int q = 0; q < np; q++) {
+                    for (int j = cstart[base + k0[q]]; j < cstart[base + k1[q] + 1]; j++) {
+                        const double dx = (xi - x[j]) - sx, dy = (yi - y[j]) - sy[q];
+                        pair_term(pp, theta32, dx, dy, vtx[i] - vtx[j], vty[i] - vty[j], type[i], type[j], gid[i], gid[j],
+                                  &ax, &ay, epot ? &e : nullptr, sabs ? &sa : nullptr);
+                    }
+                }
+            }
+            fx[i] += ax;
+            fy[i] += ay;
+            if (epot) epot[i] = e;
+            if (sabs) sabs[i] = sa;
+        }
+    }
+}
+
+/* O(N^2) brute force with minimum image (cel/engine/tests/forcetest.c:91-103
+ * computes the same sum this way); any order of arrays. */
+OR_API void or_pair_brute(const oc_params *p, int32_t N, const double *x, const double *y, const double *vtx,
+                          const double *vty, const uint8_t *type, const uint32_t *gid, int64_t step, int32_t theta32,
+                          double *fx, double *fy) {
+    const PairPar pp = pair_par(p, step, time_of(p, step));
+    const double L = p->L;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < N; i++) {
+        double ax = 0.0, ay = 0.0;
+        for (int j = 0; j < N; j++) {
+            if (j == i) continue;
+            const double dx = minimg(x[i] - x[j], L), dy = minimg(y[i] - y[j], L);
+            pair_term(pp, theta32, dx, dy, vtx[i] - vtx[j], vty[i] - vty[j], type[i], type[j], gid[i], gid[j], &ax,
+                      &ay, nullptr, nullptr);
+        }
+        fx[i] = ax;
+        fy[i] = ay;
+    }
+}
+
+/* ------------------------------------------------------------------ bonded forces */
+
+/* one harmonic bond i -> j (d = x_j - x_i, minimum image):
+ * chain_forces_linear's bond loop, cel/engine/oercel.c:1280-1289 */
+static inline void bond_term(double L, double kb, double r0, int i, int j, const double *x, const double *y, double *fx,
+                             double *fy, double *mb, double *eb, double *bx, double *by, double *bl) {
+    const double dx = minimg(x[j] - x[i], L), dy = minimg(y[j] - y[i], L);
+    const double r = sqrt(dx * dx + dy * dy);
+    if (bx) { *bx = dx; *by = dy; *bl = r; }
+    if (r > *mb) *mb = r;
+    const double f = -kb * (r - r0) / r;
+    fx[j] += f * dx;
+    fy[j] += f * dy;
+    fx[i] -= f * dx;
+    fy[i] -= f * dy;
+    if (eb) *eb += 0.5 * kb * (r - r0) * (r - r0);
+}
+
+/*
+ * Bonded forces on slot-ordered arrays, added to fx, fy.
+ * Lipids (mpos == 0 heads, chain through lnk): bonds then angle terms,
+ * chain_forces_linear, cel/engine/oercel.c:1267-1308, walked as in
+ * bonded_forces, cel/engine/oercel.c:1317-1343.
+ * Chemistry bonds (mpos < 0, lnk = partner): one harmonic bond with k_bond and
+ * r0, applied once from the bead with the smaller gid (own definition).
+ * Returns the max bond length through *maxbond (max with its input).
+ */
+OR_API void or_bonded(const oc_params *p, int32_t N, const double *x, const double *y, const int8_t *mpos,
+                      const int32_t *lnk, const uint32_t *gid, double *fx, double *fy, double *maxbond, double *ebond) {
+    const int ml = p->lip_nH + p->lip_nT;
+    const double L = p->L, kb = p->k_bond, r0 = p->r0, ka = p->k_angle;
+    double mb = *maxbond, eb = 0.0;
+    for (int h = 0; h < N; h++) {
+        if (mpos[h] == 0) {
+            int sl[OC_MAXMLEN];
+            double bx[OC_MAXMLEN], by[OC_MAXMLEN], bl[OC_MAXMLEN];
+            int n = 0;
+            for (int d = h; d >= 0 && n < ml; d = lnk[d]) sl[n++] = d;
+            for (int b = 0; b < n - 1; b++)
+                bond_term(L, kb, r0, sl[b], sl[b + 1], x, y, fx, fy, &mb, ebond ? &eb : nullptr, &bx[b], &by[b], &bl[b]);
+            if (ka != 0.0) {
+                /* cel/engine/oercel.c:1290-1306 */
+                for (int b = 0; b < n - 2; b++) {
+                    const int i = sl[b], j = sl[b + 1], k = sl[b + 2];
+                    const double ax = bx[b], ay = by[b], cxv = bx[b + 1], cyv = by[b + 1];
+                    const double la = bl[b], lc = bl[b + 1];
+                    const double ilac = 1.0 / (la * lc);
+                    const double cs = (ax * cxv + ay * cyv) * ilac;
+                    const double dUax = -ka * (cxv * ilac - cs * ax / (la * la));
+                    const double dUay = -ka * (cyv * ilac - cs * ay / (la * la));
+                    const double dUcx = -ka * (ax * ilac - cs * cxv / (lc * lc));
+                    const double dUcy = -ka * (ay * ilac - cs * cyv / (lc * lc));
+                    fx[i] += dUax;
+                    fy[i] += dUay;
+                    fx[j] += -dUax + dUcx;
+                    fy[j] += -dUay + dUcy;
+                    fx[k] -= dUcx;
+                    fy[k] -= dUcy;
+                    eb += ka * (1.0 - cs);
+                }
+            }
+        } else if (mpos[h] < 0 && lnk[h] >= 0 && gid[h] < gid[lnk[h]]) {
+            bond_term(L, kb, r0, h, lnk[h], x, y, fx, fy, &mb, ebond ? &eb : nullptr, nullptr, nullptr, nullptr);
+        }
+    }
+    *maxbond = mb;
+    if (ebond) *ebond = eb;
+}
+
+/* ------------------------------------------------------------------ integrator */
+
+/* Groot-Warren update with stored half-step velocity, cel/engine/oercel.c:2303-2334 */
+OR_API void or_integrate(const oc_params *p, int32_t N, double *x, double *y, double *vx, double *vy, const double *fx,
+                         const double *fy, double *vtx, double *vty) {
+    const double dt = p->dt, dt2 = dt * dt, cvt = (0.5 + p->lambda) * dt, L = p->L;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; i++) {
+        const double ddx = dt * vx[i] + dt2 * fx[i], ddy = dt * vy[i] + dt2 * fy[i];
+        double xi = x[i] + ddx, yi = y[i] + ddy;
+        xi = xi < 0.0 ? xi + L : xi;
+        yi = yi < 0.0 ? yi + L : yi;
+        xi = xi >= L ? xi - L : xi;
+        yi = yi >= L ? yi - L : yi;
+        x[i] = xi;
+        y[i] = yi;
+        vtx[i] = vx[i] + cvt * fx[i];
+        vty[i] = vy[i] + cvt * fy[i];
+        vx[i] += dt * fx[i];
+        vy[i] += dt * fy[i];
+    }
+}
+
+/* ------------------------------------------------------------------ soup init */
+
+/* init_soup + set_velocities, cel/engine/oercel.c:1757-1814, after
+ * rng_seed(seed) (cel/engine/oercel.c:2195); gid order = creation order;
+ * nxt as gid; velocities are the full v.  Returns 0 or -1 (composition > N). */
+OR_API int or_soup(const oc_params *p, int32_t N, double *x, double *y, double *vx, double *vy, uint8_t *type,
+                   int8_t *mpos, int32_t *nxt) {
+    Rng rng;
+    rng_seed(&rng, p->seed);
+    const int ml = p->lip_nH + p->lip_nT;
+    const double L = p->L;
+    const long nlip = lround(p->f_lip * N / ml);
+    const long nKa = lround(p->f_Ka * N), nKb = lround(p->f_Kb * N), nN = lround(p->c_N * N);
+    if (nlip * ml + nKa + nKb + nN > N) return -1;
+    int gi = 0;
+    for (long m = 0; m < nlip; m++) {
+        const double hx = L * rng_unif(&rng), hy = L * rng_unif(&rng);
+        const double phi = 2.0 * M_PI * rng_unif(&rng);
+        for (int b = 0; b < ml; b++) {
+            x[gi] = hx + b * p->r0 * cos(phi);
+            y[gi] = hy + b * p->r0 * sin(phi);
+            wrap1(&x[gi], L);
+            wrap1(&y[gi], L);
+            type[gi] = b < p->lip_nH ? OC_H : OC_T;
+            mpos[gi] = (int8_t)b;
+            nxt[gi] = b + 1 < ml ? gi + 1 : -1;
+            gi++;
+        }
+    }
+    long cnt = 0;
+    for (; gi < N; gi++, cnt++) {
+        x[gi] = L * rng_unif(&rng);
+        y[gi] = L * rng_unif(&rng);
+        mpos[gi] = -1;
+        nxt[gi] = -1;
+        if (cnt < nKa) type[gi] = OC_KA;
+        else if (cnt < nKa + nKb) type[gi] = OC_KB;
+        else if (cnt < nKa + nKb + nN) type[gi] = OC_N;
+        else type[gi] = OC_W;
+    }
+    double px = 0, py = 0;
+    for (int i = 0; i < N; i++) {
+        vx[i] = rng_normal(&rng) * sqrt(p->kT);
+        vy[i] = rng_normal(&rng) * sqrt(p->kT);
+        px += vx[i];
+        py += vy[i];
+    }
+    px /= N;
+    py /= N;
+    double k2 = 0;
+    for (int i = 0; i < N; i++) {
+        vx[i] -= px;
+        vy[i] -= py;
+        k2 += vx[i] * vx[i] + vy[i] * vy[i];
+    }
+    const double sc = sqrt(p->kT * (2.0 * N - 2.0) / k2);
+    for (int i = 0; i < N; i++) {
+        vx[i] *= sc;
+        vy[i] *= sc;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ precursor chemistry (own definition) */
+
+/*
+ * One chemistry check at `step` (only if chem && step % n_chem == 0), on a
+ * cell-sorted state.  All decisions read the state at the start of the check:
+ *   break : a chemistry bond (mpos < 0, lnk >= 0) breaks if
+ *           hash_u01(hash_key(seed, step, 33), min gid) < p_break;
+ *   conv  : a P bead (type 3) becomes a free T if
+ *           hash_u01(hash_key(seed, step, 31), gid) < p_conv and some lipid T
+ *           (type T, mpos >= 0) lies within r < r_conv;
+ *   form  : a free T (type T, mpos < 0, lnk < 0) proposes to the free T within
+ *           r < r_form with the smallest pair_u01(hash_key(seed, step, 32))
+ *           below p_form (ties: smaller gid); mutual proposals bond.
+ * Distances are FP64 minimum-image, r^2 < rc*rc.  Events are returned sorted by
+ * (kind, gid_a, gid_b).  Returns the number of events (<= cap copied).
+ * PARITY UNPINNED against the reference, which has no such chemistry.
+ */
+OR_API int64_t or_chem(const oc_params *p, int32_t N, int64_t step, const double *x, const double *y, uint8_t *type,
+                       const int8_t *mpos, int32_t *lnk, const uint32_t *gid, const int32_t *cstart, oc_event *ev,
+                       int64_t cap) {
+    if (!p->chem || p->n_chem < 1 || step % p->n_chem != 0) return 0;
+    const Geo g = make_geo(p, N);
+    const double L = g.L;
+    const uint64_t kc = hash_key(p->seed, step, SALT_CONV), kf = hash_key(p->seed, step, SALT_FORM),
+                   kb = hash_key(p->seed, step, SALT_BREAK);
+    const double rc2 = p->r_conv * p->r_conv, rf2 = p->r_form * p->r_form;
+    std::vector<uint8_t> conv(N, 0), brk(N, 0);
+    std::vector<int32_t> best(N, -1);
+    /* neighbour visitor over the cell window of bead i (cutoff <= 1) */
+    auto visit = [&](int i, auto &&fn) {
+        const int cx = cell_key(g, x[i], y[i]) / g.ncy;
+        const int khi = (int)((y[i] + 1.0) * g.icsy);
+        const int klo = (int)((y[i] - 1.0) * g.icsy + g.ncy) - g.ncy;
+        for (int dc = -1; dc <= 1; dc++) {
+            const int col = (cx + dc + g.ncx) % g.ncx;
+            for (int k = klo; k <= khi; k++) {
+                const int kk = (k % g.ncy + g.ncy) % g.ncy;
+                const int c = col * g.ncy + kk;
+                for (int j = cstart[c]; j < cstart[c + 1]; j++) {
+                    if (j == i) continue;
+                    const double dx = minimg(x[j] - x[i], L), dy = minimg(y[j] - y[i], L);
+                    if (!fn(j, dx * dx + dy * dy)) return;
+                }
+            }
+        }
+    };
+    for (int i = 0; i < N; i++) {
+        if (mpos[i] < 0 && lnk[i] >= 0 && gid[i] < gid[lnk[i]] && hash_u01(kb, gid[i]) < p->p_break) brk[i] = 1;
+        if (type[i] == OC_N && hash_u01(kc, gid[i]) < p->p_conv) {
+            bool found = false;
+            visit(i, [&](int j, double r2) {
+                if (type[j] == OC_T && mpos[j] >= 0 && r2 < rc2) { found = true; return false; }
+                return true;
+            });
+            conv[i] = found;
+        }
+        if (type[i] == OC_T && mpos[i] < 0 && lnk[i] < 0) {
+            double bu = 2.0;
+            uint32_t bg = 0xffffffffu;
+            int bj = -1;
+            visit(i, [&](int j, double r2) {
+                if (type[j] == OC_T && mpos[j] < 0 && lnk[j] < 0 && r2 < rf2) {
+                    const double u = pair_u01(kf, gid[i], gid[j]);
+                    if (u < p->p_form && (u < bu || (u == bu && gid[j] < bg))) { bu = u; bg = gid[j]; bj = j; }
+                }
+                return true;
+            });
+            best[i] = bj;
+        }
+    }
+    std::vector<oc_event> out;
+    for (int i = 0; i < N; i++) {
+        if (brk[i]) {
+            const int j = lnk[i];
+            out.push_back({step, OC_EV_BREAK, std::min(gid[i], gid[j]), std::max(gid[i], gid[j]), 0});
+        }
+        if (conv[i]) out.push_back({step, OC_EV_CONV, gid[i], gid[i], 0});
+        if (best[i] >= 0 && best[best[i]] == i && gid[i] < gid[best[i]])
+            out.push_back({step, OC_EV_FORM, gid[i], gid[best[i]], 0});
+    }
+    /* apply */
+    for (int i = 0; i < N; i++) {
+        if (brk[i]) { const int j = lnk[i]; lnk[i] = -1; lnk[j] = -1; }
+    }
+    for (int i = 0; i < N; i++) {
+        if (conv[i]) type[i] = OC_T;
+        if (best[i] >= 0 && best[best[i]] == i && gid[i] < gid[best[i]]) { lnk[i] = best[i]; lnk[best[i]] = i; }
+    }
+    std::sort(out.begin(), out.end(), [](const oc_event &a, const oc_event &b) {
+        if (a.step != b.step) return a.step < b.step;
+        if (a.kind != b.kind) return a.kind < b.kind;
+        if (a.gid_a != b.gid_a) return a.gid_a < b.gid_a;
+        return a.gid_b < b.gid_b;
+    });
+    const int64_t n = (int64_t)out.size();
+    for (int64_t k = 0; k < n && k < cap; k++) ev[k] = out[k];
+    return n;
+}
+
+/* ------------------------------------------------------------------ system (composition of the above) */
+
+struct OrSys {
+    oc_params p;
+    Geo g;
+    int N;
+    std::vector<double> x, y, vx, vy, fx, fy, vtx, vty;
+    std::vector<uint8_t> type;
+    std::vector<uint32_t> gid;
+    std::vector<int8_t> mpos;
+    std::vector<int32_t> lnk, cstart, key, src;
+    int64_t step;
+    double maxbond;
+    std::vector<oc_event> ev;
+};
+
+OR_API OrSys *or_create(const oc_params *p, int32_t N) {
+    OrSys *s = new OrSys();
+    s->p = *p;
+    s->g = make_geo(p, N);
+    s->N = N;
+    s->x.assign(N, 0); s->y.assign(N, 0); s->vx.assign(N, 0); s->vy.assign(N, 0);
+    s->fx.assign(N, 0); s->fy.assign(N, 0); s->vtx.assign(N, 0); s->vty.assign(N, 0);
+    s->type.assign(N, 0); s->gid.assign(N, 0); s->mpos.assign(N, -1); s->lnk.assign(N, -1);
+    s->cstart.assign(s->g.ncell + 1, 0);
+    s->key.assign(N, 0); s->src.assign(N, 0);
+    s->step = 0;
+    s->maxbond = 0;
+    return s;
+}
+OR_API void or_destroy(OrSys *s) { delete s; }
+
+/* sort_cells on the system: keys, stable order, gather every slot array and
+ * remap the links through dest (cel/engine/oercel.c:897-932) */
+static void sys_sort(OrSys *s) {
+    const int N = s->N;
+    or_cell_keys(&s->p, N, s->x.data(), s->y.data(), s->key.data());
+    or_sort(N, s->g.ncell, s->key.data(), s->src.data(), s->cstart.data());
+    std::vector<int32_t> dest(N);
+    for (int d = 0; d < N; d++) dest[s->src[d]] = d;
+    auto gather = [&](auto &v) {
+        auto w = v;
+        for (int d = 0; d < N; d++) w[d] = v[s->src[d]];
+        v.swap(w);
+    };
+    gather(s->x); gather(s->y); gather(s->vx); gather(s->vy); gather(s->vtx); gather(s->vty);
+    gather(s->fx); gather(s->fy); gather(s->type); gather(s->gid); gather(s->mpos);
+    std::vector<int32_t> l2(N);
+    for (int d = 0; d < N; d++) {
+        const int32_t o = s->lnk[s->src[d]];
+        l2[d] = o < 0 ? -1 : dest[o];
+    }
+    s->lnk.swap(l2);
+}
+
+static void sys_forces(OrSys *s, int64_t step, int theta32) {
+    or_pair_forces(&s->p, s->N, s->x.data(), s->y.data(), s->vtx.data(), s->vty.data(), s->type.data(), s->gid.data(),
+                   s->cstart.data(), step, theta32, 1, s->fx.data(), s->fy.data(), nullptr, nullptr);
+    or_bonded(&s->p, s->N, s->x.data(), s->y.data(), s->mpos.data(), s->lnk.data(), s->gid.data(), s->fx.data(),
+              s->fy.data(), &s->maxbond, nullptr);
+}
+
+static void load_gid_order(OrSys *s, const oc_state *st) {
+    const int N = s->N;
+    for (int g = 0; g < N; g++) {
+        s->x[g] = st->x[g]; s->y[g] = st->y[g];
+        s->vx[g] = st->vx[g]; s->vy[g] = st->vy[g];
+        s->fx[g] = st->fx ? st->fx[g] : 0.0; s->fy[g] = st->fy ? st->fy[g] : 0.0;
+        s->type[g] = st->type[g]; s->gid[g] = (uint32_t)g; s->mpos[g] = st->mpos[g];
+        const int32_t nx = st->mpos[g] >= 0 ? st->nxt[g] : (st->bond ? st->bond[g] : -1);
+        s->lnk[g] = nx;
+    }
+}
+
+/* fresh start, cel/engine/oercel.c:2196-2212: v~ = v, sort, forces at step 0, vh = v - dt/2 f */
+OR_API int or_start(OrSys *s, const oc_state *st, int32_t theta32) {
+    load_gid_order(s, st);
+    s->vtx = s->vx;
+    s->vty = s->vy;
+    s->step = s->p.step_anchor;
+    sys_sort(s);
+    sys_forces(s, s->step, theta32);
+    for (int i = 0; i < s->N; i++) {
+        s->vx[i] -= 0.5 * s->p.dt * s->fx[i];
+        s->vy[i] -= 0.5 * s->p.dt * s->fy[i];
+    }
+    return 0;
+}
+
+/* restart-style load (vh and f as given) + sort; the predicted velocity is
+ * vh rounded to FP32, the value the device keeps after oc_set_state */
+OR_API int or_set_state(OrSys *s, const oc_state *st, int64_t step) {
+    load_gid_order(s, st);
+    for (int i = 0; i < s->N; i++) { s->vtx[i] = (double)(float)s->vx[i]; s->vty[i] = (double)(float)s->vy[i]; }
+    s->step = step;
+    sys_sort(s);
+    return 0;
+}
+
+OR_API void or_get_state(OrSys *s, oc_state *st) {
+    const int N = s->N;
+    for (int d = 0; d < N; d++) {
+        const uint32_t g = s->gid[d];
+        if (st->x) st->x[g] = s->x[d];
+        if (st->y) st->y[g] = s->y[d];
+        if (st->vx) st->vx[g] = s->vx[d];
+        if (st->vy) st->vy[g] = s->vy[d];
+        if (st->fx) st->fx[g] = s->fx[d];
+        if (st->fy) st->fy[g] = s->fy[d];
+        if (st->type) st->type[g] = s->type[d];
+        if (st->mpos) st->mpos[g] = s->mpos[d];
+        const int32_t l = s->lnk[d];
+        const int32_t lg = l < 0 ? -1 : (int32_t)s->gid[l];
+        if (st->nxt) st->nxt[g] = s->mpos[d] >= 0 ? lg : -1;
+        if (st->bond) st->bond[g] = s->mpos[d] < 0 ? lg : -1;
+    }
+}
+
+/* n steps: integrate, sort, chemistry, pair + bonded forces.  The reference
+ * runs chemistry between integrate and sort on the previous cell index
+ * (cel/engine/oercel.c:2337-2346); types and bonds do not move beads, so
+ * running it on the fresh cell list makes the same decisions. */
+OR_API void or_step(OrSys *s, int64_t n, int32_t theta32) {
+    for (int64_t k = 0; k < n; k++) {
+        const int64_t ns = s->step + 1;
+        or_integrate(&s->p, s->N, s->x.data(), s->y.data(), s->vx.data(), s->vy.data(), s->fx.data(), s->fy.data(),
+                     s->vtx.data(), s->vty.data());
+        sys_sort(s);
+        if (s->p.chem) {
+            std::vector<oc_event> tmp(s->N * 2 + 16);
+            const int64_t ne = or_chem(&s->p, s->N, ns, s->x.data(), s->y.data(), s->type.data(), s->mpos.data(),
+                                       s->lnk.data(), s->gid.data(), s->cstart.data(), tmp.data(), (int64_t)tmp.size());
+            s->ev.insert(s->ev.end(), tmp.begin(), tmp.begin() + ne);
+        }
+        sys_forces(s, ns, theta32);
+        s->step = ns;
+    }
+}
+
+/* one chemistry check on the loaded (sorted) state, events appended */
+OR_API int64_t or_sys_chem(OrSys *s, int64_t step) {
+    std::vector<oc_event> tmp(s->N * 2 + 16);
+    const int64_t ne = or_chem(&s->p, s->N, step, s->x.data(), s->y.data(), s->type.data(), s->mpos.data(),
+                               s->lnk.data(), s->gid.data(), s->cstart.data(), tmp.data(), (int64_t)tmp.size());
+    s->ev.insert(s->ev.end(), tmp.begin(), tmp.begin() + ne);
+    return ne;
+}
+
+/* pair + bonded forces of the loaded state at `step` (the predicted velocity
+ * is whatever the last set_state/step left: vh after set_state) */
+OR_API void or_sys_forces(OrSys *s, int64_t step, int32_t theta32, int32_t with_bonded) {
+    or_pair_forces(&s->p, s->N, s->x.data(), s->y.data(), s->vtx.data(), s->vty.data(), s->type.data(), s->gid.data(),
+                   s->cstart.data(), step, theta32, 1, s->fx.data(), s->fy.data(), nullptr, nullptr);
+    if (with_bonded)
+        or_bonded(&s->p, s->N, s->x.data(), s->y.data(), s->mpos.data(), s->lnk.data(), s->gid.data(), s->fx.data(),
+                  s->fy.data(), &s->maxbond, nullptr);
+}
+
+OR_API int64_t or_get_events(OrSys *s, oc_event *ev, int64_t cap) {
+    const int64_t n = (int64_t)s->ev.size();
+    for (int64_t k = 0; k < n && k < cap; k++) ev[k] = s->ev[k];
+    s->ev.clear();
+    return n;
+}
+
+OR_API void or_get_cells(OrSys *s, int32_t *cstart, uint32_t *gid_of_slot) {
+    if (cstart) memcpy(cstart, s->cstart.data(), sizeof(int32_t) * (s->g.ncell + 1));
+    if (gid_of_slot) memcpy(gid_of_slot, s->gid.data(), sizeof(uint32_t) * s->N);
+}
+
+/* kinetic_T and count_types, cel/engine/oercel.c:2053-2071, plus energies */
+OR_API void or_stats(OrSys *s, oc_stats *o) {
+    const int N = s->N;
+    const double h = 0.5 * s->p.dt;
+    double k2 = 0, sx = 0, sy = 0;
+    memset(o, 0, sizeof *o);
+    for (int i = 0; i < N; i++) {
+        const double vx = s->vx[i] + h * s->fx[i], vy = s->vy[i] + h * s->fy[i];
+        k2 += vx * vx + vy * vy;
+        sx += vx;
+        sy += vy;
+        o->count[s->type[i]]++;
+        o->n_lipids += s->mpos[i] == 0;
+        o->n_bonds += (s->mpos[i] < 0 && s->lnk[i] >= 0 && s->gid[i] < s->gid[s->lnk[i]]);
+    }
+    o->step = s->step;
+    o->time = time_of(&s->p, s->step);
+    o->T_kin = k2 / (2.0 * N - 2.0);
+    o->px = sx;
+    o->py = sy;
+    o->E_kin = 0.5 * k2;
+    std::vector<double> ep(N), fx(N, 0.0), fy(N, 0.0);
+    or_pair_forces(&s->p, N, s->x.data(), s->y.data(), s->vtx.data(), s->vty.data(), s->type.data(), s->gid.data(),
+                   s->cstart.data(), s->step, 1, 0, fx.data(), fy.data(), ep.data(), nullptr);
+    double e = 0;
+    for (int i = 0; i < N; i++) e += ep[i];
+    o->E_pair = e;
+    double mb = 0, eb = 0;
+    or_bonded(&s->p, N, s->x.data(), s->y.data(), s->mpos.data(), s->lnk.data(), s->gid.data(), fx.data(), fy.data(), &mb,
+              &eb);
+    o->E_bond = eb;
+    o->max_bond = s->maxbond;
+    s->maxbond = 0;
+}
+
+OR_API int64_t or_get_step(OrSys *s) { return s->step; }

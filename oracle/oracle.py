@@ -1,0 +1,378 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrappers of the CPU oracle (liboracle.so,
+a restatement of cel/engine/oercel.c's step) and of the reference engine itself
+compiled from its source (oracle/_ref/libocref.so, oercel_ref*).  Imported only
+by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg, as the
+checker -- never by the product.
+"""
+import ctypes as C
+import importlib
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+if REPO not in sys.path:
+    sys.path.insert(0, REPO)
+abi = importlib.import_module("welcome-youtube_amd.abi")
+
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+REF_SO = os.path.join(REF_DIR, "libocref.so")
+REF_SRC = "/root/reference/cel/engine/oercel.c"
+
+P = C.POINTER
+vp = C.c_void_p
+dp = P(C.c_double)
+_OR = None
+_REF = None
+
+
+def build(ref=True, quiet=True):
+    """Compile the oracle (and, where the reference source exists, oracle/_ref)."""
+    out = subprocess.run(["make", "-C", HERE, "liboracle.so"], capture_output=True, text=True)
+    if out.returncode:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if ref and os.path.exists(REF_SRC):
+        out = subprocess.run(["make", "-C", HERE, "ref"], capture_output=True, text=True)
+        if out.returncode:
+            raise RuntimeError("oracle/_ref build failed:\n" + out.stdout + out.stderr)
+
+
+def lib():
+    global _OR
+    if _OR is None:
+        if not os.path.exists(ORACLE_SO):
+            build(ref=False)
+        L = C.CDLL(ORACLE_SO)
+        sig = {
+            "or_mix13": (C.c_uint64, [C.c_uint64]),
+            "or_step_seed": (C.c_uint64, [C.c_uint64, C.c_int64]),
+            "or_pair_theta": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint32]),
+            "or_pair_theta32": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint32]),
+            "or_hash_key": (C.c_uint64, [C.c_uint64, C.c_int64, C.c_uint64]),
+            "or_hash_u01": (C.c_double, [C.c_uint64, C.c_uint32]),
+            "or_pair_u01": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint32]),
+            "or_feed_count": (C.c_int64, [C.c_uint64, C.c_int32, C.c_double, C.c_int64, C.c_int64]),
+            "or_cell_keys": (None, [P(abi.oc_params), C.c_int32, dp, dp, P(C.c_int32)]),
+            "or_sort": (None, [C.c_int32, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
+            "or_pair_forces": (None, [P(abi.oc_params), C.c_int32, dp, dp, dp, dp, P(C.c_uint8), P(C.c_uint32),
+                                      P(C.c_int32), C.c_int64, C.c_int32, C.c_int32, dp, dp, dp, dp]),
+            "or_pair_brute": (None, [P(abi.oc_params), C.c_int32, dp, dp, dp, dp, P(C.c_uint8), P(C.c_uint32),
+                                     C.c_int64, C.c_int32, dp, dp]),
+            "or_bonded": (None, [P(abi.oc_params), C.c_int32, dp, dp, P(C.c_int8), P(C.c_int32), P(C.c_uint32), dp, dp,
+                                 dp, dp]),
+            "or_integrate": (None, [P(abi.oc_params), C.c_int32, dp, dp, dp, dp, dp, dp, dp, dp]),
+            "or_soup": (C.c_int, [P(abi.oc_params), C.c_int32, dp, dp, dp, dp, P(C.c_uint8), P(C.c_int8),
+                                  P(C.c_int32)]),
+            "or_chem": (C.c_int64, [P(abi.oc_params), C.c_int32, C.c_int64, dp, dp, P(C.c_uint8), P(C.c_int8),
+                                    P(C.c_int32), P(C.c_uint32), P(C.c_int32), P(abi.oc_event), C.c_int64]),
+            "or_create": (vp, [P(abi.oc_params), C.c_int32]),
+            "or_destroy": (None, [vp]),
+            "or_start": (C.c_int, [vp, P(abi.oc_state), C.c_int32]),
+            "or_set_state": (C.c_int, [vp, P(abi.oc_state), C.c_int64]),
+            "or_get_state": (None, [vp, P(abi.oc_state)]),
+            "or_step": (None, [vp, C.c_int64, C.c_int32]),
+            "or_get_events": (C.c_int64, [vp, P(abi.oc_event), C.c_int64]),
+            "or_sys_chem": (C.c_int64, [vp, C.c_int64]),
+            "or_sys_forces": (None, [vp, C.c_int64, C.c_int32, C.c_int32]),
+            "or_get_cells": (None, [vp, P(C.c_int32), P(C.c_uint32)]),
+            "or_stats": (None, [vp, P(abi.oc_stats)]),
+            "or_get_step": (C.c_int64, [vp]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype, f.argtypes = r, a
+        _OR = L
+    return _OR
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference engine's own functions (oracle/_ref/libocref.so)."""
+    global _REF
+    if _REF is None:
+        if not ref_available():
+            raise RuntimeError("oracle/_ref is not built (needs the reference source in this container)")
+        L = C.CDLL(REF_SO)
+        sig = {
+            "ref_mix13": (C.c_uint64, [C.c_uint64]),
+            "ref_step_seed": (C.c_uint64, [C.c_uint64, C.c_int64]),
+            "ref_pair_theta": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint32]),
+            "ref_hash_key": (C.c_uint64, [C.c_uint64, C.c_int64, C.c_uint64]),
+            "ref_hash_u01": (C.c_double, [C.c_uint64, C.c_uint32]),
+            "ref_soup": (C.c_int, [P(abi.oc_params), C.c_int, dp, dp, dp, dp, P(C.c_uint8), P(C.c_int8),
+                                   P(C.c_int32)]),
+            "ref_sort": (C.c_int, [P(abi.oc_params), C.c_int, C.c_int, dp, dp, P(C.c_int32), P(C.c_int32)]),
+            "ref_forces": (C.c_int, [P(abi.oc_params), C.c_int, C.c_int, dp, dp, dp, dp, P(C.c_uint8), P(C.c_int8),
+                                     P(C.c_int32), C.c_int64, C.c_double, C.c_int, C.c_int, dp, dp, dp]),
+            "ref_kinetic_T": (C.c_double, [C.c_int, C.c_double, dp, dp, dp, dp, dp, dp]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype, f.argtypes = r, a
+        _REF = L
+    return _REF
+
+
+def _p(a, t):
+    return a.ctypes.data_as(P(t)) if a is not None else P(t)()
+
+
+def d(a):
+    return _p(a, C.c_double)
+
+
+# ---------------------------------------------------------------------- oracle helpers
+
+def soup(prm, N):
+    """Oracle soup (gid order): dict of numpy arrays with full velocities."""
+    cp = prm.to_c() if hasattr(prm, "to_c") else prm
+    s = {"x": np.zeros(N), "y": np.zeros(N), "vx": np.zeros(N), "vy": np.zeros(N),
+         "type": np.zeros(N, np.uint8), "mpos": np.zeros(N, np.int8), "nxt": np.zeros(N, np.int32)}
+    rc = lib().or_soup(C.byref(cp), N, d(s["x"]), d(s["y"]), d(s["vx"]), d(s["vy"]), _p(s["type"], C.c_uint8),
+                       _p(s["mpos"], C.c_int8), _p(s["nxt"], C.c_int32))
+    if rc:
+        raise ValueError("soup composition exceeds N")
+    s["bond"] = np.full(N, -1, np.int32)
+    return s
+
+
+def ref_soup(prm, N):
+    cp = prm.to_c() if hasattr(prm, "to_c") else prm
+    s = {"x": np.zeros(N), "y": np.zeros(N), "vx": np.zeros(N), "vy": np.zeros(N),
+         "type": np.zeros(N, np.uint8), "mpos": np.zeros(N, np.int8), "nxt": np.zeros(N, np.int32)}
+    ref().ref_soup(C.byref(cp), N, d(s["x"]), d(s["y"]), d(s["vx"]), d(s["vy"]), _p(s["type"], C.c_uint8),
+                   _p(s["mpos"], C.c_int8), _p(s["nxt"], C.c_int32))
+    return s
+
+
+def cell_keys(cp, x, y):
+    k = np.zeros(len(x), np.int32)
+    lib().or_cell_keys(C.byref(cp), len(x), d(x), d(y), _p(k, C.c_int32))
+    return k
+
+
+def ncell(cp):
+    import math
+    n = int(math.floor(cp.L))
+    return n * n * cp.nsub
+
+
+def sort(cp, x, y):
+    """(src, cstart) of the stable counting sort of positions given in any order."""
+    k = cell_keys(cp, x, y)
+    nc = ncell(cp)
+    src = np.zeros(len(x), np.int32)
+    cs = np.zeros(nc + 1, np.int32)
+    lib().or_sort(len(x), nc, _p(k, C.c_int32), _p(src, C.c_int32), _p(cs, C.c_int32))
+    return src, cs
+
+
+def ref_sort(cp, x, y, threads=2):
+    nc = ncell(cp)
+    src = np.zeros(len(x), np.int32)
+    cs = np.zeros(nc + 1, np.int32)
+    ref().ref_sort(C.byref(cp), len(x), threads, d(x), d(y), _p(src, C.c_int32), _p(cs, C.c_int32))
+    return src, cs
+
+
+def pair_forces_gid(cp, s, step, theta32=0, energy=False, sabs=False):
+    """Full-shell oracle pair forces (+ stirring init) of a gid-order state
+    (keys x, y, vtx, vty, type); returned in gid order."""
+    N = len(s["x"])
+    src, cs = sort(cp, s["x"], s["y"])
+    xs, ys = s["x"][src].copy(), s["y"][src].copy()
+    vxs, vys = s["vtx"][src].copy(), s["vty"][src].copy()
+    ts = s["type"][src].astype(np.uint8).copy()
+    gs = src.astype(np.uint32)
+    fx, fy = np.zeros(N), np.zeros(N)
+    ep = np.zeros(N) if energy else None
+    sa = np.zeros(N) if sabs else None
+    lib().or_pair_forces(C.byref(cp), N, d(xs), d(ys), d(vxs), d(vys), _p(ts, C.c_uint8), _p(gs, C.c_uint32),
+                         _p(cs, C.c_int32), step, theta32, 1, d(fx), d(fy), d(ep), d(sa))
+    out = {"fx": np.zeros(N), "fy": np.zeros(N)}
+    out["fx"][src] = fx
+    out["fy"][src] = fy
+    if energy:
+        out["epot"] = np.zeros(N)
+        out["epot"][src] = ep
+    if sabs:
+        out["sabs"] = np.zeros(N)
+        out["sabs"][src] = sa
+    return out
+
+
+def pair_brute(cp, s, step, theta32=0):
+    N = len(s["x"])
+    fx, fy = np.zeros(N), np.zeros(N)
+    g = np.arange(N, dtype=np.uint32)
+    ts = s["type"].astype(np.uint8)
+    lib().or_pair_brute(C.byref(cp), N, d(s["x"]), d(s["y"]), d(s["vtx"]), d(s["vty"]), _p(ts, C.c_uint8),
+                        _p(g, C.c_uint32), step, theta32, d(fx), d(fy))
+    return fx, fy
+
+
+def bonded_gid(cp, s):
+    """Oracle bonded forces of a gid-order state (links nxt/bond as gid);
+    (fx, fy, maxbond, energy)."""
+    N = len(s["x"])
+    lnk = np.where(s["mpos"] >= 0, s["nxt"], s.get("bond", np.full(N, -1, np.int32))).astype(np.int32)
+    g = np.arange(N, dtype=np.uint32)
+    fx, fy = np.zeros(N), np.zeros(N)
+    mb, eb = C.c_double(0.0), C.c_double(0.0)
+    lib().or_bonded(C.byref(cp), N, d(s["x"]), d(s["y"]), _p(s["mpos"].astype(np.int8), C.c_int8),
+                    _p(lnk, C.c_int32), _p(g, C.c_uint32), d(fx), d(fy), C.byref(mb), C.byref(eb))
+    return fx, fy, mb.value, eb.value
+
+
+def ref_forces_gid(cp, s, step, t, with_pair=True, with_bonded=False, threads=2):
+    N = len(s["x"])
+    fx, fy = np.zeros(N), np.zeros(N)
+    mb = C.c_double(0.0)
+    ref().ref_forces(C.byref(cp), N, threads, d(s["x"]), d(s["y"]), d(s["vtx"]), d(s["vty"]),
+                     _p(s["type"].astype(np.uint8), C.c_uint8), _p(s["mpos"].astype(np.int8), C.c_int8),
+                     _p(s["nxt"].astype(np.int32), C.c_int32), step, t, int(with_pair), int(with_bonded), d(fx), d(fy),
+                     C.byref(mb))
+    return fx, fy, mb.value
+
+
+class System:
+    """Multi-step oracle system (handle in liboracle.so), gid-order IO."""
+
+    def __init__(self, cp, N):
+        self.cp = cp
+        self.N = N
+        self.h = lib().or_create(C.byref(cp), N)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_destroy(self.h)
+            self.h = None
+
+    def start(self, s, theta32=1):
+        st = abi.make_state(s)
+        lib().or_start(self.h, C.byref(st), theta32)
+
+    def set_state(self, s, step):
+        st = abi.make_state(s)
+        lib().or_set_state(self.h, C.byref(st), step)
+
+    def get_state(self):
+        N = self.N
+        s = {"x": np.zeros(N), "y": np.zeros(N), "vx": np.zeros(N), "vy": np.zeros(N), "fx": np.zeros(N),
+             "fy": np.zeros(N), "type": np.zeros(N, np.uint8), "mpos": np.zeros(N, np.int8),
+             "nxt": np.zeros(N, np.int32), "bond": np.zeros(N, np.int32)}
+        st = abi.make_state(s)
+        lib().or_get_state(self.h, C.byref(st))
+        return s
+
+    def step(self, n=1, theta32=1):
+        lib().or_step(self.h, n, theta32)
+
+    def chem(self, step):
+        return lib().or_sys_chem(self.h, step)
+
+    def forces(self, step, theta32=1, bonded=True):
+        lib().or_sys_forces(self.h, step, theta32, 1 if bonded else 0)
+
+    def events(self):
+        ev = np.zeros(4 * self.N + 64, dtype=abi.event_dtype())
+        n = lib().or_get_events(self.h, ev.ctypes.data_as(P(abi.oc_event)), len(ev))
+        return ev[:n].copy()
+
+    def cells(self):
+        cs = np.zeros(ncell(self.cp) + 1, np.int32)
+        g = np.zeros(self.N, np.uint32)
+        lib().or_get_cells(self.h, _p(cs, C.c_int32), _p(g, C.c_uint32))
+        return cs, g
+
+    def stats(self):
+        o = abi.oc_stats()
+        lib().or_stats(self.h, C.byref(o))
+        return o
+
+    @property
+    def step_count(self):
+        return lib().or_get_step(self.h)
+
+
+def read_ckpt(path):
+    """Reference checkpoint OCKP v2 (cel/engine/oercel.c:1886-1936): slot-ordered
+    arrays and header, as a dict."""
+    with open(path, "rb") as f:
+        b = f.read()
+    assert b[:4] == b"OCKP", "not a checkpoint"
+    ver, N, ml, nH = np.frombuffer(b, "<i4", 4, 4)
+    o = 20 + (4 if ver == 3 else 0)
+    hdr = {}
+    for k, t in (("step", "<i8"), ("time", "<f8"), ("L", "<f8"), ("seed", "<u8")):
+        hdr[k] = np.frombuffer(b, t, 1, o)[0]
+        o += 8
+    o += 32  # xoshiro state
+    for k, t in (("maxbond", "<f8"), ("warm_end", "<i8"), ("t_anchor", "<f8"), ("step_anchor", "<i8"),
+                 ("dt_anchor", "<f8"), ("soup_seed", "<u8")):
+        hdr[k] = np.frombuffer(b, t, 1, o)[0]
+        o += 8
+    s = {}
+    for k, t, sz in (("gid", "<u4", 4), ("type", "u1", 1), ("mpos", "i1", 1), ("nxt", "<i4", 4), ("x", "<f8", 8),
+                     ("y", "<f8", 8), ("vx", "<f8", 8), ("vy", "<f8", 8), ("fx", "<f8", 8), ("fy", "<f8", 8)):
+        s[k] = np.frombuffer(b, t, N, o).copy()
+        o += sz * N
+    assert b[o:o + 4] == b"END!"
+    s.update(hdr)
+    s["N"] = int(N)
+    return s
+
+
+def ckpt_gid_order(ck):
+    """Slot-ordered checkpoint arrays -> gid order (nxt as gid)."""
+    N = ck["N"]
+    g = ck["gid"].astype(np.int64)
+    out = {}
+    for k in ("x", "y", "vx", "vy", "fx", "fy", "type", "mpos"):
+        a = np.empty_like(ck[k])
+        a[g] = ck[k]
+        out[k] = a
+    nx = np.full(N, -1, np.int32)
+    has = ck["nxt"] >= 0
+    nx[g[has]] = ck["gid"][ck["nxt"][has]].astype(np.int32)
+    out["nxt"] = nx
+    out["bond"] = np.full(N, -1, np.int32)
+    return out
+
+
+def ref_binary(fast=False):
+    """Path of the reference CLI: the bit-reproducible build, or (fast=True) the
+    optimised build for this host's vector ISA."""
+    if not fast:
+        return os.path.join(REF_DIR, "oercel_ref")
+    flags = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = line
+                    break
+    except OSError:
+        pass
+    v4 = all(f" {x}" in flags for x in ("avx512f", "avx512dq", "avx512bw", "avx512vl", "avx512cd"))
+    v3 = all(f" {x}" in flags for x in ("avx2", "fma", "bmi2"))
+    name = "oercel_ref_v4" if v4 else ("oercel_ref_v3" if v3 else "oercel_ref")
+    return os.path.join(REF_DIR, name)
+
+
+def run_ref_cli(outdir, overrides, fast=False, timeout=600):
+    """Run the reference engine CLI with an empty ini and key=value overrides
+    (cel/engine/oercel.c:2125-2160); returns its stdout."""
+    os.makedirs(outdir, exist_ok=True)
+    cmd = [ref_binary(fast), "/dev/null", outdir] + [f"{k}={v}" for k, v in overrides.items()]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    if r.returncode:
+        raise RuntimeError(f"reference run failed ({r.returncode}): {r.stderr[-2000:]}")
+    return r.stdout

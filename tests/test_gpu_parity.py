@@ -1,0 +1,294 @@
+"""Parity of the HIP step (through the C ABI) with the CPU oracle.
+
+Bars: cell lists, integration and chemistry events bit-exact on identical
+inputs; pair forces within 1e-5 of the oracle relative to the per-bead force
+scale sum_j |f_ij| (and to max |F|), with the same per-pair noise (the oracle
+draws theta from the same 32-bit cut of the same hash).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import mkparams
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def soup_state(oracle, p, rng, vt_noise=0.3):
+    s = oracle.soup(p.to_c(), p.N)
+    s["vtx"] = s["vx"] + vt_noise * rng.standard_normal(p.N)
+    s["vty"] = s["vy"].copy()
+    return s
+
+
+def device_forces(eng, p, s, step, bonded):
+    """forces of state s on the device: vh := predicted velocity, f := 0"""
+    e = eng.Engine(p)
+    st = {k: s[k] for k in ("x", "y", "type", "mpos", "nxt")}
+    st["vx"], st["vy"] = s["vtx"], s["vty"]
+    st["bond"] = s.get("bond", np.full(p.N, -1, np.int32))
+    e.set_state(st, step=0)
+    e.forces(step, bonded=bonded)
+    out = e.get_state()
+    e.close()
+    return out
+
+
+def check_forces(fx, fy, ofx, ofy, sabs, what):
+    d = np.hypot(fx - ofx, fy - ofy)
+    scale = np.maximum(sabs, 1e-12)
+    rel = (d / scale).max()
+    glob = d.max() / np.hypot(ofx, ofy).max()
+    say(f"{what}: max |dF|/sum|f_ij| = {rel:.3e}, max|dF|/max|F| = {glob:.3e}, max|dF| = {d.max():.3e}")
+    assert rel < FORCE_TOL and glob < FORCE_TOL
+
+
+@pytest.mark.parametrize("L,nsub", [(9.0, 1), (12.5, 4), (23.7, 2), (31.3, 8)])
+def test_pair_forces_vs_oracle(oracle, eng, rng, L, nsub):
+    say(f"case: pair forces L={L} nsub={nsub}")
+    p = mkparams(L, nsub=nsub, f_lip=0.15, c_N=0.1, seed=99)
+    s = soup_state(oracle, p, rng)
+    # the device keeps the predicted velocity in FP32: give the oracle the same values
+    s["vtx"] = s["vtx"].astype(np.float32).astype(np.float64)
+    s["vty"] = s["vty"].astype(np.float32).astype(np.float64)
+    out = device_forces(eng, p, s, 12345, bonded=False)
+    ref = oracle.pair_forces_gid(p.to_c(), s, 12345, theta32=1, sabs=True)
+    check_forces(out["fx"], out["fy"], ref["fx"], ref["fy"], ref["sabs"], f"pair L={L} nsub={nsub}")
+    # full shell, pairwise antisymmetric: total force vanishes to rounding
+    assert abs(out["fx"].sum()) < 1e-9 * np.abs(out["fx"]).sum() + 1e-9
+    assert abs(out["fy"].sum()) < 1e-9 * np.abs(out["fy"]).sum() + 1e-9
+
+
+def test_bonded_forces_vs_oracle(oracle, eng, rng):
+    say("case: bonded forces L=16")
+    p = mkparams(16.0, f_lip=0.3, k_angle=10.0, seed=5)
+    s = soup_state(oracle, p, rng)
+    lip = s["mpos"] >= 0
+    s["x"][lip] = (s["x"][lip] + 0.15 * rng.standard_normal(lip.sum())) % p.L
+    s["y"][lip] = (s["y"][lip] + 0.15 * rng.standard_normal(lip.sum())) % p.L
+    a = device_forces(eng, p, s, 7, bonded=True)
+    b = device_forces(eng, p, s, 7, bonded=False)
+    fx, fy, mb, eb = oracle.bonded_gid(p.to_c(), s)
+    dx = (a["fx"] - b["fx"]) - fx
+    dy = (a["fy"] - b["fy"]) - fy
+    say(f"bonded: max|dF| = {max(abs(dx).max(), abs(dy).max()):.3e} (max|F| {abs(fx).max():.1f})")
+    assert max(abs(dx).max(), abs(dy).max()) < 1e-9 * max(1.0, abs(fx).max())
+
+
+def oracle_after_steps(oracle, p, n, rng=None, boost=None):
+    """An oracle system started from the soup and run n steps; returns the
+    gid-order state (x, y, vh, f, ...)."""
+    sysm = oracle.System(p.to_c(), p.N)
+    s0 = oracle.soup(p.to_c(), p.N)
+    sysm.start(s0)
+    sysm.step(n)
+    st = sysm.get_state()
+    if boost is not None:
+        idx = rng.choice(p.N, size=boost[0], replace=False)
+        st["vx"][idx] *= boost[1]
+        st["vy"][idx] *= boost[1]
+    return st
+
+
+@pytest.mark.parametrize("nsub,boost", [(1, None), (4, None), (4, (40, 60.0)), (2, (5, 150.0))])
+def test_step_sort_bit_exact(oracle, eng, rng, nsub, boost):
+    """One full step from an identical state: integration, cell list and
+    permutation bit-exact; forces within tolerance."""
+    say(f"case: step/sort nsub={nsub} boost={boost}")
+    p = mkparams(20.3, nsub=nsub, f_lip=0.2, seed=3)
+    st = oracle_after_steps(oracle, p, 5, rng, boost)
+    e = eng.Engine(p)
+    e.set_state(st, step=5)
+    o = oracle.System(p.to_c(), p.N)
+    o.set_state(st, 5)
+    e.step(1, graph=False)
+    o.step(1)
+    cs_d, g_d = e.cells()
+    cs_o, g_o = o.cells()
+    assert np.array_equal(cs_d, cs_o), "cstart differs"
+    assert np.array_equal(g_d, g_o), "slot order differs"
+    a, b = e.get_state(), o.get_state()
+    for k in ("x", "y", "vx", "vy"):
+        assert np.array_equal(a[k], b[k]), f"{k} not bit-exact"
+    for k in ("type", "mpos", "nxt", "bond"):
+        assert np.array_equal(a[k], b[k]), k
+    d = np.hypot(a["fx"] - b["fx"], a["fy"] - b["fy"]).max() / np.hypot(b["fx"], b["fy"]).max()
+    say(f"step forces: max|dF|/max|F| = {d:.3e}")
+    assert d < FORCE_TOL
+    assert e.step_count() == 6
+    e.close()
+
+
+def test_sort_matches_reference_order(oracle, eng, rng):
+    """The device's cell list of a loaded state equals the stable counting sort
+    (oracle restatement of sort_cells) of the same positions."""
+    p = mkparams(17.9, nsub=3, seed=11)
+    st = oracle_after_steps(oracle, p, 2)
+    e = eng.Engine(p)
+    e.set_state(st, step=2)
+    e.sort()
+    cs_d, g_d = e.cells()
+    src, cs_o = oracle.sort(p.to_c(), st["x"], st["y"])
+    assert np.array_equal(cs_d, cs_o) and np.array_equal(g_d, src.astype(np.uint32))
+    e.close()
+
+
+def chem_state(oracle, p, rng):
+    st = oracle_after_steps(oracle, p, 3)
+    N = p.N
+    free = np.flatnonzero((st["type"] == 3))
+    # turn some P into free T, and bond some of those pairwise
+    conv = rng.choice(free, size=len(free) // 3, replace=False)
+    st["type"][conv] = 2
+    half = len(conv) // 4
+    a, b = conv[:half], conv[half:2 * half]
+    st["bond"] = np.full(N, -1, np.int32)
+    st["bond"][a] = b
+    st["bond"][b] = a
+    return st
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_chemistry_events_bit_exact(oracle, eng, rng, seed):
+    say(f"case: chemistry seed={seed}")
+    p = mkparams(24.0, f_lip=0.3, c_N=0.2, pchem=1, n_chem=1, p_conv=0.5, p_form=0.5, p_break=0.3, seed=seed)
+    st = chem_state(oracle, p, rng)
+    e = eng.Engine(p)
+    e.set_state(st, step=3)
+    o = oracle.System(p.to_c(), p.N)
+    o.set_state(st, 3)
+    e.chem(4)
+    o.chem(4)
+    ev_d, ev_o = e.events(), o.events()
+    say(f"events: device {len(ev_d)}, oracle {len(ev_o)}")
+    assert len(ev_o) > 20
+    assert np.array_equal(ev_d[["step", "kind", "gid_a", "gid_b"]], ev_o[["step", "kind", "gid_a", "gid_b"]])
+    a, b = e.get_state(), o.get_state()
+    for k in ("type", "bond", "nxt", "mpos"):
+        assert np.array_equal(a[k], b[k]), k
+    e.close()
+
+
+def test_chemistry_in_steps_matches_oracle(oracle, eng):
+    """Chemistry inside oc_step: the events of the first step (identical
+    inputs) match exactly."""
+    p = mkparams(20.0, f_lip=0.3, c_N=0.2, pchem=1, n_chem=1, p_conv=0.3, p_form=0.3, p_break=0.1, seed=4)
+    st = oracle_after_steps(oracle, p, 2)
+    e = eng.Engine(p)
+    e.set_state(st, step=2)
+    o = oracle.System(p.to_c(), p.N)
+    o.set_state(st, 2)
+    e.step(1, graph=False)
+    o.step(1)
+    ev_d, ev_o = e.events(), o.events()
+    assert np.array_equal(ev_d[["step", "kind", "gid_a", "gid_b"]], ev_o[["step", "kind", "gid_a", "gid_b"]])
+    e.close()
+
+
+def test_graph_equals_eager_and_deterministic(eng):
+    p = mkparams(24.0, f_lip=0.3, c_N=0.2, pchem=1, n_chem=1, seed=8)
+    runs = []
+    for graph in (False, True, True):
+        e = eng.Engine(p)
+        e.start()
+        e.step(21, graph=graph)
+        runs.append((e.get_state(), e.events()))
+        e.close()
+    for k in runs[0][0]:
+        assert np.array_equal(runs[0][0][k], runs[1][0][k]), k
+        assert np.array_equal(runs[1][0][k], runs[2][0][k]), k
+    assert np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_ensemble_equals_single(eng):
+    """Replicas of one handle are bit-identical to separate single runs."""
+    ps = [mkparams(L, nsub=ns, f_lip=fl, a_W_T=aw, c_N=0.2, pchem=1, n_chem=1, seed=sd)
+          for L, ns, fl, aw, sd in ((14.2, 4, 0.1, 70, 1), (26.1, 2, 0.3, 95, 2), (9.3, 1, 0.2, 80, 3))]
+    ens = eng.Engine(ps)
+    ens.start()
+    ens.step(12)
+    for r, p in enumerate(ps):
+        e = eng.Engine(p)
+        e.start()
+        e.step(12)
+        a, b = ens.get_state(r), e.get_state()
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (r, k)
+        assert np.array_equal(ens.events(r), e.events())
+        e.close()
+    ens.close()
+
+
+def test_trajectory_close_to_oracle(oracle, eng):
+    """50 steps from the same soup stay within FP32-force round-off of the
+    FP64 oracle (chaos grows the difference slowly over 0.5 time units)."""
+    p = mkparams(18.0, f_lip=0.2, seed=21)
+    s0 = oracle.soup(p.to_c(), p.N)
+    e = eng.Engine(p)
+    e.start([s0])
+    e.step(50)
+    o = oracle.System(p.to_c(), p.N)
+    o.start(s0)
+    o.step(50)
+    a, b = e.get_state(), o.get_state()
+    dx = np.abs(((a["x"] - b["x"]) + p.L / 2) % p.L - p.L / 2).max()
+    dv = np.abs(a["vx"] - b["vx"]).max()
+    say(f"trajectory 50 steps: max|dx| = {dx:.3e}, max|dv| = {dv:.3e}")
+    assert dx < 1e-6 and dv < 1e-4
+    e.close()
+
+
+def test_stats_vs_oracle(oracle, eng):
+    p = mkparams(16.0, f_lip=0.3, k_angle=5.0, seed=2)
+    st = oracle_after_steps(oracle, p, 4)
+    e = eng.Engine(p)
+    e.set_state(st, step=4)
+    o = oracle.System(p.to_c(), p.N)
+    o.set_state(st, 4)
+    a, b = e.stats(), o.stats()
+    say(f"stats: T {a['T_kin']:.9f} vs {b.T_kin:.9f}; Epair {a['E_pair']:.6f} vs {b.E_pair:.6f}")
+    assert abs(a["T_kin"] - b.T_kin) < 1e-12 * b.T_kin
+    assert abs(a["px"] - b.px) < 1e-10 and abs(a["py"] - b.py) < 1e-10
+    assert abs(a["E_pair"] - b.E_pair) < 1e-5 * abs(b.E_pair)
+    assert abs(a["E_bond"] - b.E_bond) < 1e-9 * abs(b.E_bond)
+    assert a["count"] == list(b.count)[:7]
+    assert a["n_lipids"] == b.n_lipids
+    e.close()
+
+
+@pytest.mark.parametrize("f_lip", [0.0, 0.1])
+def test_momentum_and_temperature_config1(eng, f_lip):
+    """The CPU configuration (N = 12,000, rho 3, dt 0.01; with f_lip = 0.1 its
+    400 H-T-T lipids): total momentum stays < 1e-6 and <T> is within 1% of kT,
+    averaged after t = 30 as the reference's own soup check does
+    (cel/engine/tests/validate.sh:126)."""
+    pm = __import__("conftest").params_mod()
+    L = pm.L_for_N(12000)
+    p = mkparams(L, f_lip=f_lip, seed=1)
+    assert p.N == 12000
+    e = eng.Engine(p)
+    e.start()
+    s = e.stats()
+    if f_lip:
+        assert s["n_lipids"] == 400 and s["count"][0] == 10800
+    e.step(3000)
+    e.stats()
+    Ts, pmax = [], 0.0
+    for k in range(60):
+        e.step(50)
+        s = e.stats()
+        Ts.append(s["T_kin"])
+        pmax = max(pmax, abs(s["px"]), abs(s["py"]))
+    T = float(np.mean(Ts))
+    say(f"config 1 f_lip={f_lip}: <T> = {T:.4f} +- {np.std(Ts) / np.sqrt(len(Ts)):.4f}, max|P| = {pmax:.2e}")
+    assert abs(T - 1.0) < 0.01
+    assert pmax < 1e-6
+    e.close()

@@ -1,0 +1,852 @@
+// oc_capi.hip -- host side of the C ABI declared in include/oercel_mi.h:
+// engine handle (device buffers, block tables, pair tiles), state load/store in
+// gid order, the step launch sequence, hipGraph capture, stats and events.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "oc_kernels.hpp"
+
+using namespace oc;
+
+#define OC_API extern "C" __attribute__((visibility("default")))
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) return fail(OC_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct oc_engine {
+    int nrep = 0;
+    std::vector<oc_params> prm;
+    std::vector<RepInfo> ri;
+    int64_t Ntot = 0, Ctot = 0;
+    std::vector<int32_t> bblk, cblk, rblk;
+    std::vector<Tile> tiles;
+    int pair_block = 256, tile_beads = 0;
+    int per_xcd = 0;
+    // device
+    RepInfo *d_ri = nullptr;
+    RepStep *d_rs = nullptr;
+    int32_t *d_bblk = nullptr, *d_cblk = nullptr, *d_rblk = nullptr;
+    Tile *d_tiles = nullptr;
+    StateP st[2];
+    int cur = 0;
+    double2 *d_f = nullptr;
+    int32_t *d_key = nullptr, *d_src = nullptr, *d_dest = nullptr, *d_count = nullptr, *d_cs[2] = {nullptr, nullptr};
+    int32_t *d_bsum = nullptr;
+    int nscan = 0;
+    double *d_stir = nullptr, *d_epot = nullptr, *d_ebond = nullptr, *d_part = nullptr, *d_statout = nullptr;
+    uint8_t *d_flag = nullptr;
+    int32_t *d_best = nullptr;
+    Event *d_ev = nullptr;
+    unsigned long long *d_evn = nullptr;
+    int64_t evcap = 0;
+    int record = 1;
+    uint32_t *d_err = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    bool have_state = false, any_stir = false, any_chem = false;
+    std::vector<std::vector<oc_event>> evq;
+    std::vector<void *> allocs;
+    bool events_dropped = false;
+};
+
+template <class T>
+static int dalloc(oc_engine *h, T **p, size_t n) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return fail(OC_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    h->allocs.push_back(q);
+    *p = (T *)q;
+    return 0;
+}
+
+static int grid_of(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+// --------------------------------------------------------------------------- replica constants
+
+static int make_repinfo(const oc_params &p, int32_t N, RepInfo &R) {
+    memset(&R, 0, sizeof R);
+    if (!(p.L >= 8.0)) return fail(OC_EINVAL, "L must be >= 8");
+    if (p.nsub < 1 || p.nsub > 8) return fail(OC_EINVAL, "nsub must be 1..8");
+    if (p.dt <= 0) return fail(OC_EINVAL, "dt must be > 0");
+    if (N < 2 || N >= (1 << kGidBits)) return fail(OC_EINVAL, "N must be in [2, 2^29)");
+    if (p.lip_nH < 1 || p.lip_nT < 1 || p.lip_nH + p.lip_nT > OC_MAXMLEN)
+        return fail(OC_EINVAL, "lipid length must be 2..12 with >= 1 head and tail");
+    if (p.chem && (p.r_conv > 1.0 || p.r_form > 1.0)) return fail(OC_EINVAL, "r_conv and r_form must be <= 1");
+    R.N = N;
+    R.ncx = (int)floor(p.L);
+    R.ncy = (int)floor(p.L) * p.nsub;
+    R.ncell = R.ncx * R.ncy;
+    R.mlen = p.lip_nH + p.lip_nT;
+    R.nH = p.lip_nH;
+    R.chem = p.chem;
+    R.n_chem = p.n_chem < 1 ? 1 : p.n_chem;
+    R.has_stir = p.stir_A != 0.0;
+    R.L = p.L;
+    R.halfL = 0.5 * p.L;
+    R.icsx = R.ncx / p.L;
+    R.icsy = R.ncy / p.L;
+    R.csx = p.L / R.ncx;
+    R.csy = p.L / R.ncy;
+    R.dt = p.dt;
+    R.dt2 = p.dt * p.dt;
+    R.cvt = (0.5 + p.lambda) * p.dt;
+    R.hdt = 0.5 * p.dt;
+    R.gamma = p.gamma;
+    R.kT = p.kT;
+    R.kb = p.k_bond;
+    R.r0 = p.r0;
+    R.ka = p.k_angle;
+    R.stir_A = p.stir_A;
+    R.stir_kn = (double)p.stir_n / p.L;
+    R.kT_amp = p.kT_amp;
+    R.kT_period = p.kT_period;
+    R.t_anchor = p.t_anchor;
+    R.step_anchor = p.step_anchor;
+    R.warmup = p.warmup;
+    R.warm_end = p.warm_end;
+    R.seed = p.seed;
+    R.p_conv = p.p_conv;
+    R.rc2 = p.r_conv * p.r_conv;
+    R.p_form = p.p_form;
+    R.rf2 = p.r_form * p.r_form;
+    R.p_break = p.p_break;
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++) R.a[8 * i + j] = (i < OC_NTYPES && j < OC_NTYPES) ? p.a[i][j] : 0.0;
+    return 0;
+}
+
+// pair tiles: rows per tile from the expected beads per row
+static int build_tiles(oc_engine *h) {
+    h->tiles.clear();
+    const int target = h->tile_beads > 0 ? h->tile_beads : (h->pair_block * 4) / 5;
+    for (int r = 0; r < h->nrep; r++) {
+        const RepInfo &R = h->ri[r];
+        const double per_row = (double)R.N / ((double)R.ncx * R.ncy);
+        const int M = (int)ceil(R.icsy) + 1;
+        int rows = (int)lround(target / std::max(per_row, 1e-9));
+        rows = std::max(1, std::min(rows, R.ncy));
+        rows = std::min(rows, 512 - 2 * M - 1);
+        if (rows < 1) return fail(OC_EINVAL, "nsub too large for the pair tile buffer");
+        const int nt = (R.ncy + rows - 1) / rows;
+        for (int cx = 0; cx < R.ncx; cx++)
+            for (int t = 0; t < nt; t++) {
+                Tile T;
+                T.rep = r;
+                T.cx = cx;
+                T.k0 = (int)(((int64_t)t * R.ncy) / nt);
+                T.k1 = (int)(((int64_t)(t + 1) * R.ncy) / nt);
+                h->tiles.push_back(T);
+            }
+    }
+    h->per_xcd = (int)((h->tiles.size() + 7) / 8);
+    if (h->d_tiles) {
+        (void)hipFree(h->d_tiles);
+        h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), (void *)h->d_tiles), h->allocs.end());
+        h->d_tiles = nullptr;
+    }
+    if (int rc = dalloc(h, &h->d_tiles, h->tiles.size())) return rc;
+    HIPCHK(hipMemcpy(h->d_tiles, h->tiles.data(), h->tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static void drop_graphs(oc_engine *h) {
+    for (int k = 0; k < 2; k++) {
+        if (h->gexec[k]) (void)hipGraphExecDestroy(h->gexec[k]);
+        if (h->graph[k]) (void)hipGraphDestroy(h->graph[k]);
+        h->gexec[k] = nullptr;
+        h->graph[k] = nullptr;
+    }
+}
+
+// --------------------------------------------------------------------------- launches
+
+static void launch_pair(oc_engine *h, bool energy) {
+    const int c = h->cur;
+    PairArgs A;
+    A.ri = h->d_ri;
+    A.rs = h->d_rs;
+    A.tiles = h->d_tiles;
+    A.ntiles = (int)h->tiles.size();
+    A.per_xcd = h->per_xcd;
+    A.cs = h->d_cs[c];
+    A.xy = h->st[c].xy;
+    A.vt = h->st[c].vt;
+    A.gt = h->st[c].gt;
+    A.stir = h->any_stir ? h->d_stir : nullptr;
+    A.f = h->d_f;
+    A.epot = h->d_epot;
+    A.err = h->d_err;
+    const dim3 grid(8 * h->per_xcd);
+    switch (h->pair_block) {
+    case 64:
+        if (energy) k_pair<64, 256, true><<<grid, 64, 0, h->stream>>>(A);
+        else k_pair<64, 256, false><<<grid, 64, 0, h->stream>>>(A);
+        break;
+    case 128:
+        if (energy) k_pair<128, 512, true><<<grid, 128, 0, h->stream>>>(A);
+        else k_pair<128, 512, false><<<grid, 128, 0, h->stream>>>(A);
+        break;
+    default:
+        if (energy) k_pair<256, 1024, true><<<grid, 256, 0, h->stream>>>(A);
+        else k_pair<256, 1024, false><<<grid, 256, 0, h->stream>>>(A);
+        break;
+    }
+}
+
+static void launch_stir(oc_engine *h) {
+    if (!h->any_stir) return;
+    const int c = h->cur;
+    const int nrb = (int)h->rblk.size();
+    k_stir_partial<<<nrb, 256, 0, h->stream>>>(h->d_ri, h->d_rblk, h->st[c].xy, h->d_stir, h->d_part);
+    k_stir_final<<<h->nrep, 256, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_part);
+}
+
+static void launch_forces(oc_engine *h, bool with_bonded) {
+    launch_stir(h);
+    launch_pair(h, false);
+    if (with_bonded) {
+        const int c = h->cur;
+        k_bonded<false><<<(int)h->bblk.size(), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy,
+                                                                           h->st[c].gt, h->st[c].mp, h->st[c].lnk,
+                                                                           h->d_f, h->d_ebond, h->d_err);
+    }
+}
+
+static void launch_chem(oc_engine *h) {
+    if (!h->any_chem) return;
+    const int c = h->cur;
+    const int nb = (int)h->bblk.size();
+    k_chem_decide<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->d_cs[c], h->st[c].xy, h->st[c].gt,
+                                                   h->st[c].mp, h->st[c].lnk, h->d_flag, h->d_best);
+    k_chem_apply<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].gt, h->st[c].lnk, h->d_flag,
+                                                  h->d_best, h->d_ev, h->d_evn, h->evcap, h->record, h->d_err);
+}
+
+// sort (with integration when integ): keys, scan, gather, permute; flips cur
+static void launch_sort(oc_engine *h, bool integ) {
+    const int c = h->cur, n = c ^ 1;
+    const int nb = (int)h->bblk.size();
+    if (integ)
+        k_keys<true><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy, h->st[c].v, h->d_f,
+                                                       h->d_key, h->d_count);
+    else
+        k_keys<false><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy, h->st[c].v, h->d_f,
+                                                        h->d_key, h->d_count);
+    k_scan_reduce<<<h->nscan, 256, 0, h->stream>>>(h->d_count, h->Ctot, h->d_bsum);
+    k_scan_top<<<1, 1024, 0, h->stream>>>(h->d_bsum, h->nscan);
+    k_scan_down<<<h->nscan, 256, 0, h->stream>>>(h->d_count, h->Ctot, h->d_bsum, h->d_cs[n]);
+    k_gather<<<(int)h->cblk.size(), kCellBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_cblk, h->d_key, h->d_cs[c],
+                                                               h->d_cs[n], h->d_src, h->d_dest, h->d_err);
+    if (integ)
+        k_permute<true><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->d_src, h->d_dest, h->st[c], h->st[n],
+                                                          h->d_f);
+    else
+        k_permute<false><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->d_src, h->d_dest, h->st[c], h->st[n],
+                                                           h->d_f);
+    h->cur = n;
+}
+
+static void launch_step_params(oc_engine *h, int64_t delta, int64_t set_step) {
+    k_step_params<<<h->nrep, 64, 0, h->stream>>>(h->d_ri, h->d_rs, delta, set_step);
+}
+
+// one full step of every replica: cel/engine/oercel.c:2300-2362
+static void enqueue_step(oc_engine *h) {
+    launch_step_params(h, 1, -1);
+    launch_sort(h, true);
+    launch_chem(h);
+    launch_forces(h, true);
+}
+
+static int check_err(oc_engine *h) {
+    uint32_t e = 0;
+    HIPCHK(hipMemcpyAsync(&e, h->d_err, sizeof e, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (e & ERR_EVOVF) h->events_dropped = true;
+    if (e & ~ERR_EVOVF) {
+        std::string m = "device error:";
+        if (e & ERR_SORT) m += " cell gather count mismatch (sort)";
+        if (e & ERR_CHAIN) m += " broken lipid chain";
+        if (e & ERR_TILE) m += " pair tile window outside its staged rows";
+        return fail(OC_EDEVICE, m);
+    }
+    return 0;
+}
+
+// --------------------------------------------------------------------------- host-side soup (init)
+
+namespace {
+struct HRng {
+    uint64_t s[4];
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    // xoshiro256** seeded by splitmix64, cel/engine/oercel.c:143-174
+    explicit HRng(uint64_t seed) {
+        uint64_t z = seed;
+        for (int k = 0; k < 4; k++) {
+            z += 0x9e3779b97f4a7c15ULL;
+            s[k] = mix13(z);
+        }
+    }
+    uint64_t next() {
+        const uint64_t out = rotl(s[1] * 5, 7) * 9;
+        const uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return out;
+    }
+    double unif() { return (double)(next() >> 11) * 0x1.0p-53; }
+    double normal() {
+        double u1 = unif();
+        const double u2 = unif();
+        if (u1 < 1e-300) u1 = 1e-300;
+        return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+    }
+};
+double wrap_box(double v, double L) {
+    // wrap1, cel/engine/oercel.c:761-772
+    if (v < 0.0) {
+        v += L;
+        if (v >= L) v -= L;
+    } else if (v >= L) {
+        v -= L;
+        if (v >= L || v < 0.0) v = fmod(v, L);
+    }
+    return v;
+}
+}  // namespace
+
+OC_API int oc_soup_host(const oc_params *p, int32_t N, oc_state *o) {
+    if (!p || !o || !o->x || !o->y || !o->vx || !o->vy || !o->type || !o->mpos || !o->nxt)
+        return fail(OC_EINVAL, "oc_soup_host: missing output array");
+    if (N < 2) return fail(OC_EINVAL, "N must be >= 2");
+    const int ml = p->lip_nH + p->lip_nT;
+    const double L = p->L;
+    // composition, cel/engine/oercel.c:1775-1780
+    const long nlip = lround(p->f_lip * N / ml);
+    const long nKa = lround(p->f_Ka * N), nKb = lround(p->f_Kb * N), nN = lround(p->c_N * N);
+    if (nlip * ml + nKa + nKb + nN > N) return fail(OC_EINVAL, "soup composition exceeds N");
+    HRng rng(p->seed);
+    int g = 0;
+    for (long m = 0; m < nlip; m++) {
+        // straight template chain from a random head position and angle
+        const double hx = L * rng.unif(), hy = L * rng.unif();
+        const double phi = 2.0 * M_PI * rng.unif();
+        for (int b = 0; b < ml; b++, g++) {
+            o->x[g] = wrap_box(hx + b * p->r0 * cos(phi), L);
+            o->y[g] = wrap_box(hy + b * p->r0 * sin(phi), L);
+            o->type[g] = b < p->lip_nH ? OC_H : OC_T;
+            o->mpos[g] = (int8_t)b;
+            o->nxt[g] = b + 1 < ml ? g + 1 : -1;
+            if (o->bond) o->bond[g] = -1;
+        }
+    }
+    for (long k = 0; g < N; g++, k++) {
+        o->x[g] = L * rng.unif();
+        o->y[g] = L * rng.unif();
+        o->type[g] = k < nKa ? OC_KA : (k < nKa + nKb ? OC_KB : (k < nKa + nKb + nN ? OC_N : OC_W));
+        o->mpos[g] = -1;
+        o->nxt[g] = -1;
+        if (o->bond) o->bond[g] = -1;
+    }
+    // Maxwell-Boltzmann velocities, net momentum removed, scaled to exact kT
+    double px = 0, py = 0;
+    const double skT = sqrt(p->kT);
+    for (int i = 0; i < N; i++) {
+        o->vx[i] = rng.normal() * skT;
+        o->vy[i] = rng.normal() * skT;
+        px += o->vx[i];
+        py += o->vy[i];
+    }
+    px /= N;
+    py /= N;
+    double k2 = 0;
+    for (int i = 0; i < N; i++) {
+        o->vx[i] -= px;
+        o->vy[i] -= py;
+        k2 += o->vx[i] * o->vx[i] + o->vy[i] * o->vy[i];
+    }
+    const double sc = sqrt(p->kT * (2.0 * N - 2.0) / k2);
+    for (int i = 0; i < N; i++) {
+        o->vx[i] *= sc;
+        o->vy[i] *= sc;
+    }
+    if (o->fx)
+        for (int i = 0; i < N; i++) o->fx[i] = 0.0;
+    if (o->fy)
+        for (int i = 0; i < N; i++) o->fy[i] = 0.0;
+    return 0;
+}
+
+// --------------------------------------------------------------------------- create / destroy
+
+OC_API const char *oc_last_error(void) { return g_err.c_str(); }
+OC_API int oc_abi_version(void) { return OC_ABI_VERSION; }
+
+OC_API void oc_destroy(oc_engine *h) {
+    if (!h) return;
+    drop_graphs(h);
+    for (void *p : h->allocs) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engine **out) {
+    if (!p || !N || !out || nrep < 1) return fail(OC_EINVAL, "oc_create: bad arguments");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(OC_EHIP, "no HIP device");
+    oc_engine *h = new oc_engine();
+    h->nrep = nrep;
+    h->prm.assign(p, p + nrep);
+    h->ri.resize(nrep);
+    int64_t off = 0, coff = 0;
+    for (int r = 0; r < nrep; r++) {
+        if (int rc = make_repinfo(p[r], N[r], h->ri[r])) {
+            delete h;
+            return rc;
+        }
+        RepInfo &R = h->ri[r];
+        R.off = (int32_t)off;
+        R.coff = (int32_t)coff;
+        off += R.N;
+        coff += R.ncell;
+        if (off >= INT32_MAX || coff >= INT32_MAX) {
+            delete h;
+            return fail(OC_EINVAL, "too many beads or cells for 32-bit slot indices");
+        }
+        R.bblk0 = (int32_t)h->bblk.size();
+        for (int b = 0; b < grid_of(R.N, kBeadBlock); b++) h->bblk.push_back(r);
+        R.cblk0 = (int32_t)h->cblk.size();
+        for (int b = 0; b < grid_of(R.ncell, kCellBlock); b++) h->cblk.push_back(r);
+        R.rblk0 = (int32_t)h->rblk.size();
+        for (int b = 0; b < grid_of(R.N, kRedChunk); b++) h->rblk.push_back(r);
+        h->any_stir |= R.has_stir != 0;
+        h->any_chem |= R.chem != 0;
+    }
+    h->Ntot = off;
+    h->Ctot = coff;
+    h->nscan = grid_of(h->Ctot, kScanChunk);
+    h->evq.resize(nrep);
+    h->evcap = std::min<int64_t>(std::max<int64_t>(1 << 20, 4 * h->Ntot), 8 << 20);
+    int rc = 0;
+    const size_t Nt = (size_t)h->Ntot, Ct = (size_t)h->Ctot;
+    rc = rc ? rc : dalloc(h, &h->d_ri, nrep);
+    rc = rc ? rc : dalloc(h, &h->d_rs, nrep);
+    rc = rc ? rc : dalloc(h, &h->d_bblk, h->bblk.size());
+    rc = rc ? rc : dalloc(h, &h->d_cblk, h->cblk.size());
+    rc = rc ? rc : dalloc(h, &h->d_rblk, h->rblk.size());
+    for (int k = 0; k < 2; k++) {
+        rc = rc ? rc : dalloc(h, &h->st[k].xy, Nt);
+        rc = rc ? rc : dalloc(h, &h->st[k].v, Nt);
+        rc = rc ? rc : dalloc(h, &h->st[k].vt, Nt);
+        rc = rc ? rc : dalloc(h, &h->st[k].gt, Nt);
+        rc = rc ? rc : dalloc(h, &h->st[k].mp, Nt);
+        rc = rc ? rc : dalloc(h, &h->st[k].lnk, Nt);
+        rc = rc ? rc : dalloc(h, &h->d_cs[k], Ct + 1);
+    }
+    rc = rc ? rc : dalloc(h, &h->d_f, Nt);
+    rc = rc ? rc : dalloc(h, &h->d_key, Nt);
+    rc = rc ? rc : dalloc(h, &h->d_src, Nt);
+    rc = rc ? rc : dalloc(h, &h->d_dest, Nt);
+    rc = rc ? rc : dalloc(h, &h->d_count, Ct);
+    rc = rc ? rc : dalloc(h, &h->d_bsum, h->nscan);
+    rc = rc ? rc : dalloc(h, &h->d_epot, Nt);
+    rc = rc ? rc : dalloc(h, &h->d_ebond, Nt);
+    rc = rc ? rc : dalloc(h, &h->d_part, h->rblk.size() * 16);
+    rc = rc ? rc : dalloc(h, &h->d_statout, (size_t)nrep * 16);
+    rc = rc ? rc : dalloc(h, &h->d_flag, Nt);
+    rc = rc ? rc : dalloc(h, &h->d_best, Nt);
+    rc = rc ? rc : dalloc(h, &h->d_ev, (size_t)h->evcap);
+    rc = rc ? rc : dalloc(h, &h->d_evn, 1);
+    rc = rc ? rc : dalloc(h, &h->d_err, 1);
+    if (h->any_stir) rc = rc ? rc : dalloc(h, &h->d_stir, Nt);
+    if (rc) {
+        oc_destroy(h);
+        return rc;
+    }
+    if (hipStreamCreate(&h->stream) != hipSuccess) {
+        oc_destroy(h);
+        return fail(OC_EHIP, "hipStreamCreate failed");
+    }
+    std::vector<RepStep> rs(nrep);
+    memset(rs.data(), 0, rs.size() * sizeof(RepStep));
+    for (int r = 0; r < nrep; r++) rs[r].step = p[r].step_anchor;
+    hipError_t e = hipSuccess;
+    e = e ? e : hipMemcpy(h->d_ri, h->ri.data(), nrep * sizeof(RepInfo), hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(h->d_rs, rs.data(), nrep * sizeof(RepStep), hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(h->d_bblk, h->bblk.data(), h->bblk.size() * 4, hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(h->d_cblk, h->cblk.data(), h->cblk.size() * 4, hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(h->d_rblk, h->rblk.data(), h->rblk.size() * 4, hipMemcpyHostToDevice);
+    e = e ? e : hipMemset(h->d_count, 0, Ct * 4);
+    e = e ? e : hipMemset(h->d_evn, 0, sizeof(unsigned long long));
+    e = e ? e : hipMemset(h->d_err, 0, 4);
+    e = e ? e : hipMemset(h->d_ebond, 0, Nt * sizeof(double));
+    e = e ? e : hipMemset(h->d_epot, 0, Nt * sizeof(double));
+    e = e ? e : hipMemset(h->d_f, 0, Nt * sizeof(double2));
+    for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(h->st[k].lnk, 0xff, Nt * 4);
+    if (e != hipSuccess) {
+        oc_destroy(h);
+        return fail(OC_EHIP, std::string("setup copy failed: ") + hipGetErrorString(e));
+    }
+    if ((rc = build_tiles(h))) {
+        oc_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return 0;
+}
+
+OC_API int oc_set_tuning(oc_engine *h, int32_t pair_block, int32_t tile_beads) {
+    if (!h) return fail(OC_EINVAL, "null engine");
+    if (pair_block != 0 && pair_block != 64 && pair_block != 128 && pair_block != 256)
+        return fail(OC_EINVAL, "pair_block must be 64, 128 or 256");
+    h->pair_block = pair_block ? pair_block : 256;
+    h->tile_beads = tile_beads > 0 ? tile_beads : 0;
+    drop_graphs(h);
+    return build_tiles(h);
+}
+
+OC_API void *oc_stream(oc_engine *h) { return h ? (void *)h->stream : nullptr; }
+
+OC_API int oc_abi_sizes(int32_t *out, int32_t n) {
+    const int32_t sz[4] = {(int32_t)sizeof(oc_params), (int32_t)sizeof(oc_state), (int32_t)sizeof(oc_event),
+                           (int32_t)sizeof(oc_stats)};
+    int k = 0;
+    for (; out && k < n && k < 4; k++) out[k] = sz[k];
+    return k;
+}
+
+// --------------------------------------------------------------------------- state in / out
+
+// Load replica r from gid-order host arrays: stable counting sort by cell on
+// the host (the order sort_cells gives a gid-ordered input), upload in slot order.
+static int load_replica(oc_engine *h, int r, const oc_state *s, size_t base, bool vt_from_v) {
+    const RepInfo &R = h->ri[r];
+    const int N = R.N;
+    if (!s->x || !s->y || !s->vx || !s->vy || !s->type || !s->mpos) return fail(OC_EINVAL, "state: missing array");
+    std::vector<int32_t> key(N), cnt(R.ncell + 1, 0), src(N), dest(N);
+    for (int g = 0; g < N; g++) {
+        const double x = s->x[base + g], y = s->y[base + g];
+        if (!(x >= 0.0 && x < R.L && y >= 0.0 && y < R.L)) return fail(OC_EINVAL, "state: position outside [0, L)");
+        int cx = (int)(x * R.icsx), cy = (int)(y * R.icsy);
+        cx = cx >= R.ncx ? R.ncx - 1 : cx;
+        cy = cy >= R.ncy ? R.ncy - 1 : cy;
+        key[g] = cx * R.ncy + cy;
+        cnt[key[g]]++;
+    }
+    std::vector<int32_t> cs(R.ncell + 1);
+    int run = 0;
+    for (int c = 0; c < R.ncell; c++) {
+        cs[c] = run;
+        run += cnt[c];
+    }
+    cs[R.ncell] = run;
+    std::vector<int32_t> fill(cs.begin(), cs.end() - 1);
+    for (int g = 0; g < N; g++) {
+        const int d = fill[key[g]]++;
+        src[d] = g;
+        dest[g] = d;
+    }
+    std::vector<double2> xy(N), v(N), f(N);
+    std::vector<float2> vt(N);
+    std::vector<uint32_t> gt(N);
+    std::vector<int8_t> mp(N);
+    std::vector<int32_t> lnk(N), csg(R.ncell + 1);
+    for (int d = 0; d < N; d++) {
+        const int g = src[d];
+        const size_t k = base + g;
+        const int ty = s->type[k];
+        if (ty < 0 || ty >= OC_NTYPES) return fail(OC_EINVAL, "state: bad type");
+        xy[d] = make_double2(s->x[k], s->y[k]);
+        v[d] = make_double2(s->vx[k], s->vy[k]);
+        f[d] = make_double2(s->fx ? s->fx[k] : 0.0, s->fy ? s->fy[k] : 0.0);
+        vt[d] = make_float2((float)s->vx[k], (float)s->vy[k]);
+        gt[d] = (uint32_t)g | ((uint32_t)ty << kGidBits);
+        mp[d] = s->mpos[k];
+        int32_t partner = -1;
+        if (s->mpos[k] >= 0) partner = s->nxt ? s->nxt[k] : -1;
+        else if (s->bond) partner = s->bond[k];
+        if (partner >= N) return fail(OC_EINVAL, "state: link gid out of range");
+        lnk[d] = partner >= 0 ? R.off + dest[partner] : -1;
+    }
+    for (int c = 0; c <= R.ncell; c++) csg[c] = cs[c] + R.off;
+    (void)vt_from_v;
+    const int c = h->cur;
+    const size_t o = R.off;
+    HIPCHK(hipMemcpy(h->st[c].xy + o, xy.data(), N * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->st[c].v + o, v.data(), N * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->st[c].vt + o, vt.data(), N * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->st[c].gt + o, gt.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->st[c].mp + o, mp.data(), N * sizeof(int8_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->st[c].lnk + o, lnk.data(), N * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_f + o, f.data(), N * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_cs[c] + R.coff, csg.data(), (R.ncell + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int set_step_fields(oc_engine *h, int r, int64_t step) {
+    RepStep rs;
+    HIPCHK(hipMemcpy(&rs, h->d_rs + r, sizeof rs, hipMemcpyDeviceToHost));
+    rs.step = step;
+    rs.maxbond_bits = 0;
+    HIPCHK(hipMemcpy(h->d_rs + r, &rs, sizeof rs, hipMemcpyHostToDevice));
+    return 0;
+}
+
+OC_API int oc_start(oc_engine *h, const oc_state *s) {
+    if (!h || !s) return fail(OC_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    size_t base = 0;
+    for (int r = 0; r < h->nrep; r++) {
+        if (int rc = load_replica(h, r, s, base, true)) return rc;
+        if (int rc = set_step_fields(h, r, h->prm[r].step_anchor)) return rc;
+        base += h->ri[r].N;
+    }
+    drop_graphs(h);
+    h->have_state = true;
+    // forces at the start step, then vh = v - dt/2 f (cel/engine/oercel.c:2203-2211)
+    launch_step_params(h, 0, -1);
+    launch_forces(h, true);
+    k_half_kick<<<(int)h->bblk.size(), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->st[h->cur].v, h->d_f);
+    HIPCHK(hipGetLastError());
+    return check_err(h);
+}
+
+OC_API int oc_set_state(oc_engine *h, int32_t r, const oc_state *s, int64_t step) {
+    if (!h || !s || r < 0 || r >= h->nrep) return fail(OC_EINVAL, "bad arguments");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (int rc = load_replica(h, r, s, 0, false)) return rc;
+    if (int rc = set_step_fields(h, r, step)) return rc;
+    launch_step_params(h, 0, -1);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->have_state = true;
+    return 0;
+}
+
+OC_API int oc_get_state(oc_engine *h, int32_t r, oc_state *s) {
+    if (!h || !s || r < 0 || r >= h->nrep) return fail(OC_EINVAL, "bad arguments");
+    const RepInfo &R = h->ri[r];
+    const int N = R.N, c = h->cur;
+    const size_t o = R.off;
+    std::vector<double2> xy(N), v(N), f(N);
+    std::vector<uint32_t> gt(N);
+    std::vector<int8_t> mp(N);
+    std::vector<int32_t> lnk(N);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(xy.data(), h->st[c].xy + o, N * sizeof(double2), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(v.data(), h->st[c].v + o, N * sizeof(double2), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(f.data(), h->d_f + o, N * sizeof(double2), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(gt.data(), h->st[c].gt + o, N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(mp.data(), h->st[c].mp + o, N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lnk.data(), h->st[c].lnk + o, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int d = 0; d < N; d++) {
+        const uint32_t g = gt[d] & kGidMask;
+        if (s->x) s->x[g] = xy[d].x;
+        if (s->y) s->y[g] = xy[d].y;
+        if (s->vx) s->vx[g] = v[d].x;
+        if (s->vy) s->vy[g] = v[d].y;
+        if (s->fx) s->fx[g] = f[d].x;
+        if (s->fy) s->fy[g] = f[d].y;
+        if (s->type) s->type[g] = (uint8_t)(gt[d] >> kGidBits);
+        if (s->mpos) s->mpos[g] = mp[d];
+        const int32_t l = lnk[d];
+        const int32_t lg = l < 0 ? -1 : (int32_t)(gt[l - R.off] & kGidMask);
+        if (s->nxt) s->nxt[g] = mp[d] >= 0 ? lg : -1;
+        if (s->bond) s->bond[g] = mp[d] < 0 ? lg : -1;
+    }
+    return 0;
+}
+
+// --------------------------------------------------------------------------- stepping
+
+static int capture_graph(oc_engine *h, int parity) {
+    const int saved = h->cur;
+    h->cur = parity;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(h);
+    enqueue_step(h);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    h->cur = saved;
+    if (e != hipSuccess) return fail(OC_EHIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+    h->graph[parity] = g;
+    HIPCHK(hipGraphInstantiate(&h->gexec[parity], g, nullptr, nullptr, 0));
+    return 0;
+}
+
+OC_API int oc_step(oc_engine *h, int64_t n, int32_t use_graph) {
+    if (!h) return fail(OC_EINVAL, "null engine");
+    if (!h->have_state) return fail(OC_ESTATE, "no state: call oc_start or oc_set_state first");
+    if (n <= 0) return 0;
+    int64_t k = 0;
+    if (use_graph && n >= 2) {
+        const int p = h->cur;
+        if (!h->gexec[p])
+            if (int rc = capture_graph(h, p)) return rc;
+        for (; k + 2 <= n; k += 2) HIPCHK(hipGraphLaunch(h->gexec[p], h->stream));
+    }
+    for (; k < n; k++) enqueue_step(h);
+    HIPCHK(hipGetLastError());
+    return check_err(h);
+}
+
+OC_API int oc_sort(oc_engine *h) {
+    if (!h || !h->have_state) return fail(OC_ESTATE, "no state");
+    launch_step_params(h, 0, -1);
+    launch_sort(h, false);
+    HIPCHK(hipGetLastError());
+    return check_err(h);
+}
+
+OC_API int oc_forces(oc_engine *h, int64_t step, int32_t with_bonded) {
+    if (!h || !h->have_state) return fail(OC_ESTATE, "no state");
+    launch_step_params(h, 0, step);
+    launch_forces(h, with_bonded != 0);
+    HIPCHK(hipGetLastError());
+    return check_err(h);
+}
+
+OC_API int oc_launch_pair(oc_engine *h, int32_t reps) {
+    if (!h || !h->have_state) return fail(OC_ESTATE, "no state");
+    for (int k = 0; k < reps; k++) launch_pair(h, false);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+OC_API int oc_chem(oc_engine *h, int64_t step) {
+    if (!h || !h->have_state) return fail(OC_ESTATE, "no state");
+    launch_step_params(h, 0, step);
+    launch_chem(h);
+    HIPCHK(hipGetLastError());
+    return check_err(h);
+}
+
+OC_API int oc_get_step(oc_engine *h, int32_t r, int64_t *step) {
+    if (!h || !step || r < 0 || r >= h->nrep) return fail(OC_EINVAL, "bad arguments");
+    RepStep rs;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(&rs, h->d_rs + r, sizeof rs, hipMemcpyDeviceToHost));
+    *step = rs.step;
+    return 0;
+}
+
+// --------------------------------------------------------------------------- cells, events, stats
+
+OC_API int oc_get_cells(oc_engine *h, int32_t r, int32_t *cstart, uint32_t *gid_of_slot, int32_t *ncell) {
+    if (!h || r < 0 || r >= h->nrep) return fail(OC_EINVAL, "bad arguments");
+    const RepInfo &R = h->ri[r];
+    if (ncell) *ncell = R.ncell;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int c = h->cur;
+    if (cstart) {
+        HIPCHK(hipMemcpy(cstart, h->d_cs[c] + R.coff, (R.ncell + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (int k = 0; k <= R.ncell; k++) cstart[k] -= R.off;
+    }
+    if (gid_of_slot) {
+        HIPCHK(hipMemcpy(gid_of_slot, h->st[c].gt + R.off, R.N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (int k = 0; k < R.N; k++) gid_of_slot[k] &= kGidMask;
+    }
+    return 0;
+}
+
+static int drain_events(oc_engine *h) {
+    unsigned long long n = 0;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(&n, h->d_evn, sizeof n, hipMemcpyDeviceToHost));
+    const int64_t m = std::min<int64_t>((int64_t)n, h->evcap);
+    if (m > 0) {
+        std::vector<Event> ev(m);
+        HIPCHK(hipMemcpy(ev.data(), h->d_ev, m * sizeof(Event), hipMemcpyDeviceToHost));
+        for (const Event &e : ev) {
+            oc_event o;
+            o.step = e.step;
+            o.kind = e.kind;
+            o.gid_a = e.gid_a;
+            o.gid_b = e.gid_b;
+            o.pad = 0;
+            h->evq[e.rep].push_back(o);
+        }
+    }
+    const unsigned long long z = 0;
+    HIPCHK(hipMemcpy(h->d_evn, &z, sizeof z, hipMemcpyHostToDevice));
+    return 0;
+}
+
+OC_API int oc_get_events(oc_engine *h, int32_t r, oc_event *ev, int64_t cap, int64_t *n_out) {
+    if (!h || r < 0 || r >= h->nrep) return fail(OC_EINVAL, "bad arguments");
+    if (int rc = drain_events(h)) return rc;
+    std::vector<oc_event> &q = h->evq[r];
+    std::sort(q.begin(), q.end(), [](const oc_event &a, const oc_event &b) {
+        if (a.step != b.step) return a.step < b.step;
+        if (a.kind != b.kind) return a.kind < b.kind;
+        if (a.gid_a != b.gid_a) return a.gid_a < b.gid_a;
+        return a.gid_b < b.gid_b;
+    });
+    const int64_t n = (int64_t)q.size();
+    if (n_out) *n_out = n;
+    const int64_t m = ev ? std::min<int64_t>(n, std::max<int64_t>(cap, 0)) : 0;
+    for (int64_t k = 0; k < m; k++) ev[k] = q[k];
+    q.erase(q.begin(), q.begin() + m);
+    if (h->events_dropped) {
+        h->events_dropped = false;
+        return fail(OC_EDEVICE, "event buffer overflowed: some events were dropped");
+    }
+    return 0;
+}
+
+OC_API int oc_get_stats(oc_engine *h, int32_t r, oc_stats *out) {
+    if (!h || !out || r < 0 || r >= h->nrep) return fail(OC_EINVAL, "bad arguments");
+    if (!h->have_state) return fail(OC_ESTATE, "no state");
+    const int c = h->cur;
+    launch_pair(h, true);
+    k_bonded<true><<<(int)h->bblk.size(), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy,
+                                                                      h->st[c].gt, h->st[c].mp, h->st[c].lnk, h->d_f,
+                                                                      h->d_ebond, h->d_err);
+    k_stats_partial<<<(int)h->rblk.size(), 256, 0, h->stream>>>(h->d_ri, h->d_rblk, h->st[c].v, h->d_f, h->st[c].gt,
+                                                               h->st[c].mp, h->st[c].lnk, h->d_epot, h->d_ebond,
+                                                               h->d_part);
+    k_stats_final<<<h->nrep, 256, 0, h->stream>>>(h->d_ri, h->d_part, h->d_statout);
+    HIPCHK(hipGetLastError());
+    if (int rc = check_err(h)) return rc;
+    double v[16];
+    RepStep rs;
+    HIPCHK(hipMemcpy(v, h->d_statout + (size_t)r * 16, sizeof v, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&rs, h->d_rs + r, sizeof rs, hipMemcpyDeviceToHost));
+    const RepInfo &R = h->ri[r];
+    memset(out, 0, sizeof *out);
+    out->step = rs.step;
+    out->time = R.t_anchor + (double)(rs.step - R.step_anchor) * R.dt;
+    out->T_kin = v[0] / (2.0 * R.N - 2.0);
+    out->px = v[1];
+    out->py = v[2];
+    out->E_kin = 0.5 * v[0];
+    out->E_pair = v[3];
+    out->E_bond = v[4];
+    double mb;
+    memcpy(&mb, &rs.maxbond_bits, sizeof mb);
+    out->max_bond = mb;
+    for (int k = 0; k < OC_NTYPES; k++) out->count[k] = (int64_t)v[5 + k];
+    out->n_lipids = (int64_t)v[12];
+    out->n_bonds = (int64_t)v[13];
+    rs.maxbond_bits = 0;
+    HIPCHK(hipMemcpy(h->d_rs + r, &rs, sizeof rs, hipMemcpyHostToDevice));
+    return 0;
+}

@@ -1,0 +1,103 @@
+// oc_common.hpp -- structures shared by the gfx950 kernels and the host side
+// of the C ABI.  All arithmetic that must be bit-identical to the FP64
+// restatement of cel/engine/oercel.c (integration, cell keys, chemistry
+// distances, bonded forces) is written as plain expressions; the library is
+// compiled with -ffp-contract=off so none of them is fused.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/oercel_mi.h"
+
+namespace oc {
+
+constexpr int kGidBits = 29;                      // gt = gid | type << 29 (N < 2^29 per replica)
+constexpr uint32_t kGidMask = (1u << kGidBits) - 1u;
+constexpr int kBeadBlock = 256;                   // beads per block of the per-bead kernels
+constexpr int kCellBlock = 256;                   // cells per block of the per-cell kernels
+constexpr int kScanChunk = 2048;                  // cells per block of the scan
+constexpr int kRedChunk = 2048;                   // beads per partial of the stats reduction
+
+// sticky device error bits
+enum : uint32_t { ERR_EVOVF = 1u, ERR_CHAIN = 2u, ERR_SORT = 4u, ERR_TILE = 8u };
+
+// Replica constants (one system of the ensemble).
+struct RepInfo {
+    int32_t N, off;           // beads, first global slot
+    int32_t ncx, ncy, ncell, coff;  // cells (column-major), first global cell
+    int32_t mlen, nH;
+    int32_t chem, n_chem;
+    int32_t has_stir, bblk0, cblk0, rblk0;  // stirring on; first block of the bead / cell / reduction tables
+    double L, halfL, icsx, icsy, csx, csy;
+    double dt, dt2, cvt, hdt;       // dt, dt^2, (1/2 + lambda) dt, dt/2
+    double gamma, kT, kb, r0, ka;
+    double stir_A, stir_kn, kT_amp, kT_period;
+    double t_anchor;
+    int64_t step_anchor, warmup, warm_end;
+    uint64_t seed;
+    double p_conv, rc2, p_form, rf2, p_break;
+    double a[64];
+};
+
+// Per-step values of one replica, written on the device by k_step_params.
+struct RepStep {
+    int64_t step;             // step whose forces are (being) computed
+    double t;
+    uint64_t sseed;           // pair-noise key of this step
+    uint64_t kconv, kform, kbreak;  // chemistry keys
+    float a_eff[64];          // repulsion at this step's warm-up ramp
+    float gamma, sdt;         // gamma, sigma(t)/sqrt(dt)
+    int32_t Rx, Ry;           // sort window radii (cells) of the last integrate
+    int32_t chem_now;
+    int32_t pad;
+    double stir_mean;
+    uint64_t maxbond_bits;    // running max bond length (bits of a positive double)
+};
+
+struct Event {
+    int64_t step;
+    int32_t kind;
+    uint32_t gid_a, gid_b;
+    int32_t rep;
+};
+
+// Pair tile: replica, column, sub-cell rows [k0, k1).
+struct Tile {
+    int32_t rep, cx, k0, k1;
+};
+
+// --------------------------------------------------------------------------- hashes
+// splitmix64 finaliser and the keys of cel/engine/oercel.c:123-141, 819-826
+__host__ __device__ inline uint64_t mix13(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t step_seed(uint64_t seed, int64_t step) {
+    return mix13(mix13(seed ^ 0x5851f42d4c957f2dULL) ^ mix13((uint64_t)step + 0x632be59bd9b4e019ULL));
+}
+__host__ __device__ inline uint64_t hash_key(uint64_t seed, int64_t step, uint64_t salt) {
+    return mix13(seed ^ mix13((uint64_t)step * 0x9e3779b97f4a7c15ULL + salt));
+}
+__host__ __device__ inline double hash_u01(uint64_t key, uint32_t g) {
+    return (double)(mix13(key ^ ((uint64_t)g * 0xd1b54a32d192ed03ULL)) >> 11) * 0x1.0p-53;
+}
+__host__ __device__ inline uint64_t pair_hash(uint64_t key, uint32_t gi, uint32_t gj) {
+    const uint32_t lo = gi < gj ? gi : gj, hi = gi < gj ? gj : gi;
+    const uint64_t k = ((uint64_t)lo << 32) | hi;
+    return mix13((k * 0x9e3779b97f4a7c15ULL) ^ key);
+}
+__host__ __device__ inline double pair_u01(uint64_t key, uint32_t gi, uint32_t gj) {
+    return (double)(pair_hash(key, gi, gj) >> 11) * 0x1.0p-53;
+}
+
+// chemistry salts (distinct from the reference's 11-14)
+enum : uint64_t { SALT_CONV = 31, SALT_FORM = 32, SALT_BREAK = 33 };
+
+__host__ __device__ inline double minimg(double d, double L, double h) {
+    if (d > h) d -= L;
+    else if (d < -h) d += L;
+    return d;
+}
+
+}  // namespace oc

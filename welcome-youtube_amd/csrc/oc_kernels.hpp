@@ -1,0 +1,827 @@
+// oc_kernels.hip -- gfx950 kernels of the 2D DPD step.
+//
+// Step order (one launch sequence per step for all replicas at once):
+//   k_step_params  per-replica step counter, noise key, warm-up ramp, sigma(t)
+//   k_keys         integrated position -> new cell key, atomic histogram,
+//                  sort window radius (max cell displacement)
+//   k_scan_*       exclusive scan of the histogram -> new cell starts
+//   k_gather       one thread per new cell collects its beads from the old
+//                  cells within the radius in old-slot order: the stable order
+//   k_permute      gather of the state into the new order, integrating on the
+//                  way (the integrate of cel/engine/oercel.c:2303-2334 fused
+//                  into the scatter of sort_cells, :897-921)
+//   k_chem_*       precursor chemistry (own definition)
+//   k_pair         full-shell DPD pair forces, LDS-staged tiles
+//   k_bonded       lipid chains and chemistry bonds
+#include "oc_common.hpp"
+
+namespace oc {
+
+// --------------------------------------------------------------------------- helpers
+
+__device__ __forceinline__ int cell_x(const RepInfo &R, double x) {
+    int c = (int)(x * R.icsx);
+    return c >= R.ncx ? R.ncx - 1 : c;
+}
+__device__ __forceinline__ int cell_y(const RepInfo &R, double y) {
+    int c = (int)(y * R.icsy);
+    return c >= R.ncy ? R.ncy - 1 : c;
+}
+
+// Groot-Warren update of one bead, cel/engine/oercel.c:2319-2332 (FP64,
+// unfused: identical bits to the CPU restatement)
+__device__ __forceinline__ void integrate1(const RepInfo &R, double2 p, double2 v, double2 f, double2 &po, double2 &vo,
+                                           float2 &vt) {
+    const double ddx = R.dt * v.x + R.dt2 * f.x, ddy = R.dt * v.y + R.dt2 * f.y;
+    double xi = p.x + ddx, yi = p.y + ddy;
+    xi = xi < 0.0 ? xi + R.L : xi;
+    yi = yi < 0.0 ? yi + R.L : yi;
+    xi = xi >= R.L ? xi - R.L : xi;
+    yi = yi >= R.L ? yi - R.L : yi;
+    po = make_double2(xi, yi);
+    vt = make_float2((float)(v.x + R.cvt * f.x), (float)(v.y + R.cvt * f.y));
+    vo = make_double2(v.x + R.dt * f.x, v.y + R.dt * f.y);
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int w = __shfl_up(v, o, 64);
+        if (lane >= o) v += w;
+    }
+    return v;
+}
+
+// --------------------------------------------------------------------------- per-step parameters
+
+__global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs,
+                                                    int64_t delta, int64_t set_step) {
+    const int r = blockIdx.x, k = threadIdx.x;
+    const RepInfo &R = ri[r];
+    RepStep &S = rs[r];
+    const int64_t st = set_step >= 0 ? set_step : S.step + delta;
+    // a_eff, cel/engine/oercel.c:1122-1129: one matrix entry per lane
+    const double a = R.a[k];
+    double ae = a;
+    if (!(st >= R.warm_end || R.warmup <= 0 || a <= 25.0)) {
+        const double fr = 1.0 - (double)(R.warm_end - st) / (double)R.warmup;
+        ae = 25.0 + (a - 25.0) * (fr < 0 ? 0 : fr);
+    }
+    S.a_eff[k] = (float)ae;
+    if (k != 0) return;
+    S.step = st;
+    const double t = R.t_anchor + (double)(st - R.step_anchor) * R.dt;
+    S.t = t;
+    S.sseed = step_seed(R.seed, st);
+    S.kconv = hash_key(R.seed, st, SALT_CONV);
+    S.kform = hash_key(R.seed, st, SALT_FORM);
+    S.kbreak = hash_key(R.seed, st, SALT_BREAK);
+    // kT_at, cel/engine/oercel.c:622-624; sdt as in pair_forces, :1204
+    const double kTt = R.kT_amp == 0.0 ? R.kT : R.kT + R.kT_amp * sin(2.0 * M_PI * t / R.kT_period);
+    S.sdt = (float)(sqrt(2.0 * R.gamma * kTt) / sqrt(R.dt));
+    S.gamma = (float)R.gamma;
+    S.chem_now = (R.chem && R.n_chem > 0 && (st % R.n_chem) == 0) ? 1 : 0;
+    S.Rx = 0;
+    S.Ry = 0;
+}
+
+// --------------------------------------------------------------------------- sort
+
+// New cell key of every bead (of its integrated position when INTEG) and the
+// histogram; the largest periodic cell displacement goes to Rx/Ry.
+template <bool INTEG>
+__global__ __launch_bounds__(kBeadBlock) void k_keys(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs,
+                                                     const int32_t *__restrict__ bblk, const double2 *__restrict__ xy,
+                                                     const double2 *__restrict__ v, const double2 *__restrict__ f,
+                                                     int32_t *__restrict__ key, int32_t *__restrict__ count) {
+    const int r = bblk[blockIdx.x];
+    const RepInfo &R = ri[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    int dx = 0, dy = 0;
+    if (loc < R.N) {
+        const int s = R.off + loc;
+        const double2 p = xy[s];
+        const int ox = cell_x(R, p.x), oy = cell_y(R, p.y);
+        double2 q = p;
+        if (INTEG) {
+            double2 vo;
+            float2 vt;
+            integrate1(R, p, v[s], f[s], q, vo, vt);
+        }
+        const int cx = cell_x(R, q.x), cy = cell_y(R, q.y);
+        const int c = R.coff + cx * R.ncy + cy;
+        key[s] = c;
+        atomicAdd(&count[c], 1);
+        dx = abs(cx - ox);
+        dx = min(dx, R.ncx - dx);
+        dy = abs(cy - oy);
+        dy = min(dy, R.ncy - dy);
+    }
+    dx = wave_max(dx);
+    dy = wave_max(dy);
+    // one word per replica: most waves find the maximum already there, so read
+    // before the (serialising) same-address atomic
+    if ((threadIdx.x & 63) == 0) {
+        if (dx > __hip_atomic_load(&rs[r].Rx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&rs[r].Rx, dx);
+        if (dy > __hip_atomic_load(&rs[r].Ry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&rs[r].Ry, dy);
+    }
+}
+
+// exclusive scan of count[0..C) -> cstart[0..C]; count is cleared on the way
+__global__ __launch_bounds__(256) void k_scan_reduce(const int32_t *__restrict__ count, int64_t C,
+                                                     int32_t *__restrict__ bsum) {
+    __shared__ int red[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * 8;
+    int s = 0;
+    for (int k = 0; k < 8; k++)
+        if (base + k < C) s += count[base + k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(int32_t *__restrict__ bsum, int nb) {
+    __shared__ int wsum[16];
+    const int per = (nb + 1023) / 1024;
+    const int b0 = threadIdx.x * per;
+    int loc = 0;
+    for (int k = 0; k < per; k++)
+        if (b0 + k < nb) loc += bsum[b0 + k];
+    const int inc = wave_incl_scan(loc);
+    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); w++) woff += wsum[w];
+    int run = woff + inc - loc;
+    for (int k = 0; k < per; k++)
+        if (b0 + k < nb) {
+            const int c = bsum[b0 + k];
+            bsum[b0 + k] = run;
+            run += c;
+        }
+}
+
+__global__ __launch_bounds__(256) void k_scan_down(int32_t *__restrict__ count, int64_t C,
+                                                   const int32_t *__restrict__ bsum, int32_t *__restrict__ cstart) {
+    __shared__ int wsum[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * 8;
+    int c8[8];
+    int loc = 0;
+    for (int k = 0; k < 8; k++) {
+        c8[k] = base + k < C ? count[base + k] : 0;
+        loc += c8[k];
+    }
+    const int inc = wave_incl_scan(loc);
+    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); w++) woff += wsum[w];
+    int run = bsum[blockIdx.x] + woff + inc - loc;
+    for (int k = 0; k < 8; k++) {
+        if (base + k < C) {
+            cstart[base + k] = run;
+            count[base + k] = 0;
+            run += c8[k];
+            if (base + k == C - 1) cstart[C] = run;
+        }
+    }
+}
+
+// Piece list of the periodic window [c - R, c + R] of 0..n-1 in ascending order.
+__device__ __forceinline__ int win_pieces(int c, int R, int n, int *a, int *b) {
+    if (2 * R + 1 >= n) { a[0] = 0; b[0] = n - 1; return 1; }
+    if (c - R < 0) { a[0] = 0; b[0] = c + R; a[1] = c - R + n; b[1] = n - 1; return 2; }
+    if (c + R >= n) { a[0] = 0; b[0] = c + R - n; a[1] = c - R; b[1] = n - 1; return 2; }
+    a[0] = c - R; b[0] = c + R;
+    return 1;
+}
+
+// One thread per new cell: scan the old cells within (Rx, Ry) in increasing
+// old-cell index (= increasing old slot) and collect the beads whose key is
+// this cell.  The result is the stable counting-sort order of
+// cel/engine/oercel.c:828-933.
+__global__ __launch_bounds__(kCellBlock) void k_gather(const RepInfo *__restrict__ ri, const RepStep *__restrict__ rs,
+                                                       const int32_t *__restrict__ cblk,
+                                                       const int32_t *__restrict__ key,
+                                                       const int32_t *__restrict__ cs_old,
+                                                       const int32_t *__restrict__ cs_new, int32_t *__restrict__ src,
+                                                       int32_t *__restrict__ dest, uint32_t *__restrict__ err) {
+    const int r = cblk[blockIdx.x];
+    const RepInfo &R = ri[r];
+    const int lc = (blockIdx.x - R.cblk0) * kCellBlock + threadIdx.x;
+    if (lc >= R.ncell) return;
+    const int c = R.coff + lc;
+    int out = cs_new[c];
+    const int end = cs_new[c + 1];
+    if (out == end) return;
+    const int cx = lc / R.ncy, cy = lc - cx * R.ncy;
+    int ca[2], cb[2], ra[2], rb[2];
+    const int nca = win_pieces(cx, rs[r].Rx, R.ncx, ca, cb);
+    const int nra = win_pieces(cy, rs[r].Ry, R.ncy, ra, rb);
+    for (int pc = 0; pc < nca; pc++)
+        for (int col = ca[pc]; col <= cb[pc]; col++) {
+            const int cbase = R.coff + col * R.ncy;
+            for (int pr = 0; pr < nra; pr++) {
+                const int s1 = cs_old[cbase + rb[pr] + 1];
+                for (int s = cs_old[cbase + ra[pr]]; s < s1; s++)
+                    if (key[s] == c) {
+                        src[out] = s;
+                        dest[s] = out;
+                        if (++out == end) return;
+                    }
+            }
+        }
+    atomicOr(err, ERR_SORT);
+}
+
+struct StateP {
+    double2 *xy, *v;
+    float2 *vt;
+    uint32_t *gt;
+    int8_t *mp;
+    int32_t *lnk;
+};
+
+// Gather into the new slot order; when INTEG the bead is integrated on the way
+// (same arithmetic as k_keys, so the new position is the one that was keyed).
+template <bool INTEG>
+__global__ __launch_bounds__(kBeadBlock) void k_permute(const RepInfo *__restrict__ ri, const int32_t *__restrict__ bblk,
+                                                        const int32_t *__restrict__ src,
+                                                        const int32_t *__restrict__ dest, StateP in, StateP out,
+                                                        const double2 *__restrict__ f) {
+    const int r = bblk[blockIdx.x];
+    const RepInfo &R = ri[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    if (loc >= R.N) return;
+    const int d = R.off + loc;
+    const int s = src[d];
+    double2 p = in.xy[s], vv = in.v[s];
+    float2 vt;
+    if (INTEG) {
+        double2 po, vo;
+        integrate1(R, p, vv, f[s], po, vo, vt);
+        p = po;
+        vv = vo;
+    } else {
+        vt = in.vt[s];
+    }
+    out.xy[d] = p;
+    out.v[d] = vv;
+    out.vt[d] = vt;
+    out.gt[d] = in.gt[s];
+    out.mp[d] = in.mp[s];
+    const int32_t l = in.lnk[s];
+    out.lnk[d] = l >= 0 ? dest[l] : -1;
+}
+
+// --------------------------------------------------------------------------- pair forces
+
+constexpr int kMaxRows = 512;  // rows per staged column (tile rows + 2 * margin + 1)
+constexpr float kTheta32 = (float)(3.4641016151377545870548926830117 / 4294967296.0);  // sqrt(12) / 2^32
+
+__device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float vyi, uint32_t gi, const float *arow,
+                                         double2 pj, float2 vj, uint32_t gtj, double sxs, double sys, float gam,
+                                         float sdt, uint64_t sseed, double &ax, double &ay, float &e, bool energy) {
+    // FP64 image difference, then FP32 force arithmetic.  Both orientations
+    // of a pair run the same operations on negated operands, so
+    // f_ji = -f_ij exactly.
+    const float dx = (float)((xi - pj.x) - sxs), dy = (float)((yi - pj.y) - sys);
+    const float r2 = dx * dx + dy * dy;
+    if (r2 < 1.0f && r2 > 1e-24f) {
+        const float ir = __builtin_amdgcn_rsqf(r2);
+        const float r = r2 * ir;
+        const float w = 1.0f - r;
+        const float dvx = vxi - vj.x, dvy = vyi - vj.y;
+        const float rv = (dx * dvx + dy * dvy) * ir;
+        const uint32_t gj = gtj & kGidMask;
+        const uint64_t h = pair_hash(sseed, gi, gj);
+        // theta on [-sqrt3, sqrt3) from the top 32 bits of the hash
+        const float th = (float)(int32_t)(uint32_t)(h >> 32) * kTheta32;
+        const float a = arow[gtj >> kGidBits];
+        const float fm = w * (a - gam * w * rv + sdt * th) * ir;
+        ax += (double)(fm * dx);
+        ay += (double)(fm * dy);
+        if (energy) e += 0.25f * a * w * w;
+    }
+}
+
+struct PairArgs {
+    const RepInfo *ri;
+    const RepStep *rs;
+    const Tile *tiles;
+    int ntiles, per_xcd;
+    const int32_t *cs;
+    const double2 *xy;
+    const float2 *vt;
+    const uint32_t *gt;
+    const double *stir;
+    double2 *f;
+    double *epot;
+    uint32_t *err;
+};
+
+// Full-shell pair forces of the i-beads of one tile (a column, rows [k0,k1)).
+// The candidate beads -- columns cx-1..cx+1, rows k0-M..k1-1+M -- are staged
+// in LDS once per tile; each lane then runs its own y-window per column, the
+// window expressions of cel/engine/oercel.c:1073-1074.  Forces are summed in
+// FP64 per bead, no atomics: the result is deterministic.
+template <int BLOCK, int CAP, bool ENERGY>
+__global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
+    __shared__ double2 s_xy[CAP];
+    __shared__ float2 s_vt[CAP];
+    __shared__ uint32_t s_gt[CAP];
+    __shared__ int32_t s_row[3][kMaxRows + 1];
+    __shared__ float s_a[64];
+    __shared__ int32_t s_piece[3][3][3];  // [col][piece A/B/C] = {first slot, length, lds start}
+    __shared__ int32_t s_colbase[4];
+    __shared__ int32_t s_col[3];
+
+    // XCD-aware tile order: consecutive workgroups go to different XCDs
+    // round-robin, so give each XCD a contiguous run of tiles (neighbouring
+    // columns share their staged beads through that XCD's L2).
+    const int t = (blockIdx.x & 7) * A.per_xcd + (blockIdx.x >> 3);
+    if (t >= A.ntiles) return;
+    const Tile T = A.tiles[t];
+    const RepInfo &R = A.ri[T.rep];
+    const RepStep &S = A.rs[T.rep];
+    const int ncy = R.ncy, ncx = R.ncx;
+    const int M = (int)ceil(R.icsy) + 1;  // staging margin in rows
+    const int kb0 = T.k0 - M;             // first staged (unwrapped) row
+    const int nrow = T.k1 - T.k0 + 2 * M + 1;
+    const int32_t *cs = A.cs;
+
+    if (threadIdx.x < 64) s_a[threadIdx.x] = S.a_eff[threadIdx.x];
+    if (threadIdx.x == 0) {
+        int base = 0;
+        for (int c = 0; c < 3; c++) {
+            int col = T.cx - 1 + c;
+            col = col < 0 ? col + ncx : (col >= ncx ? col - ncx : col);
+            s_col[c] = col;
+            const int cb = R.coff + col * ncy;
+            s_colbase[c] = base;
+            const int lo = kb0, hi = T.k1 - 1 + M;  // staged rows, inclusive
+            // piece A: rows lo..-1 (wrapped to the top of the column)
+            int fA = 0, nA = 0;
+            if (lo < 0) { fA = cs[cb + lo + ncy]; nA = cs[cb + ncy] - fA; }
+            // piece B: rows max(lo,0)..min(hi,ncy-1)
+            const int b0 = lo < 0 ? 0 : lo, b1 = hi >= ncy ? ncy - 1 : hi;
+            const int fB = cs[cb + b0], nB = cs[cb + b1 + 1] - fB;
+            // piece C: rows ncy..hi (wrapped to the bottom)
+            int fC = 0, nC = 0;
+            if (hi >= ncy) { fC = cs[cb]; nC = cs[cb + hi - ncy + 1] - fC; }
+            s_piece[c][0][0] = fA; s_piece[c][0][1] = nA; s_piece[c][0][2] = base;
+            s_piece[c][1][0] = fB; s_piece[c][1][1] = nB; s_piece[c][1][2] = base + nA;
+            s_piece[c][2][0] = fC; s_piece[c][2][1] = nC; s_piece[c][2][2] = base + nA + nB;
+            base += nA + nB + nC;
+        }
+        s_colbase[3] = base;
+    }
+    __syncthreads();
+    const int total = s_colbase[3];
+    const bool staged = total <= CAP && nrow <= kMaxRows;
+
+    // row offsets: s_row[c][k - kb0] = LDS (staged) or slot index of the first
+    // bead of unwrapped row k of column c
+    for (int e = threadIdx.x; e < 3 * nrow && nrow <= kMaxRows; e += BLOCK) {
+        const int c = e / nrow, k = kb0 + (e - c * nrow);
+        const int cb = R.coff + s_col[c] * ncy;
+        int off;
+        if (k < 0) off = s_piece[c][0][2] + (cs[cb + ncy + k] - s_piece[c][0][0]);
+        else if (k < ncy) off = s_piece[c][1][2] + (cs[cb + k] - s_piece[c][1][0]);
+        else off = s_piece[c][2][2] + (cs[cb + k - ncy] - s_piece[c][2][0]);
+        s_row[c][e - c * nrow] = off;
+    }
+    if (staged) {
+        for (int e = threadIdx.x; e < total; e += BLOCK) {
+            int c = 0, pc = 0;
+            for (int q = 0; q < 9; q++) {
+                const int cc = q / 3, pp = q - cc * 3;
+                if (e >= s_piece[cc][pp][2] && e < s_piece[cc][pp][2] + s_piece[cc][pp][1]) { c = cc; pc = pp; }
+            }
+            const int slot = s_piece[c][pc][0] + (e - s_piece[c][pc][2]);
+            s_xy[e] = A.xy[slot];
+            s_vt[e] = A.vt[slot];
+            s_gt[e] = A.gt[slot];
+        }
+    }
+    __syncthreads();
+    if (nrow > kMaxRows) {  // the host sizes tiles so that this cannot happen
+        if (threadIdx.x == 0) atomicOr(A.err, ERR_TILE);
+        return;
+    }
+
+    const double L = R.L;
+    const double sxc[3] = {T.cx == 0 ? -L : 0.0, 0.0, T.cx == ncx - 1 ? L : 0.0};
+    const float gam = S.gamma, sdt = S.sdt;
+    const uint64_t sseed = S.sseed;
+    const int ibeg = s_row[1][T.k0 - kb0], iend = s_row[1][T.k1 - kb0];
+    const int slot0 = cs[R.coff + T.cx * ncy + T.k0];
+    // A tile whose candidates overflow the LDS buffer (a dense clump) runs the
+    // same loop on global memory; s_row offsets are then rebased per piece.
+    auto run = [&](const double2 *XY, const float2 *VT, const uint32_t *GT, const bool stg) {
+        for (int ii = ibeg + threadIdx.x; ii < iend; ii += BLOCK) {
+            const int slot = slot0 + (ii - ibeg);
+            const int is = stg ? ii : slot;
+            const double2 pi = XY[is];
+            const float2 vi = VT[is];
+            const uint32_t gti = GT[is];
+            const uint32_t gi = gti & kGidMask;
+            const float *arow = s_a + 8 * (gti >> kGidBits);
+            int khi = (int)((pi.y + 1.0) * R.icsy);
+            int klo = (int)((pi.y - 1.0) * R.icsy + ncy) - ncy;
+            if (klo < kb0 || khi > T.k1 - 1 + M) {  // cannot happen for rows k0..k1-1 (margin M); guard
+                atomicOr(A.err, ERR_TILE);
+                klo = max(klo, kb0);
+                khi = min(khi, T.k1 - 1 + M);
+            }
+            double ax = 0.0, ay = 0.0;
+            float e = 0.0f;
+            for (int c = 0; c < 3; c++) {
+                // window split at the periodic seams: rows < 0 (image -L, staged piece 0),
+                // 0..ncy-1 (piece 1), >= ncy (image +L, piece 2)
+                for (int q = 0; q < 3; q++) {
+                    int pa, pb;
+                    double sy;
+                    if (q == 0) { pa = klo; pb = min(khi, -1); sy = -L; }
+                    else if (q == 1) { pa = max(klo, 0); pb = min(khi, ncy - 1); sy = 0.0; }
+                    else { pa = max(klo, ncy); pb = khi; sy = L; }
+                    if (pa > pb) continue;
+                    const int adj = stg ? 0 : s_piece[c][q][0] - s_piece[c][q][2];
+                    const int ja = s_row[c][pa - kb0] + adj, jb = s_row[c][pb + 1 - kb0] + adj;
+                    for (int j = ja; j < jb; j++)
+                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], sxc[c], sy, gam, sdt, sseed,
+                                 ax, ay, e, ENERGY);
+                }
+            }
+            if (ENERGY) {
+                A.epot[slot] = (double)e;
+            } else {
+                double bx = 0.0;
+                if (A.stir) bx = A.stir[slot] - S.stir_mean;
+                A.f[slot] = make_double2(bx + ax, ay);
+            }
+        }
+    };
+    if (staged) run(s_xy, s_vt, s_gt, true);
+    else run(A.xy, A.vt, A.gt, false);
+}
+
+// --------------------------------------------------------------------------- bonded forces
+
+// harmonic bond i -> j, the arithmetic of cel/engine/oercel.c:1280-1289
+template <bool WF>
+__device__ __forceinline__ void bond1(const RepInfo &R, int i, int j, const double2 *xy, double2 *f, double &mb,
+                                      double &eb, double &bx, double &by, double &bl) {
+    const double2 pi = xy[i], pj = xy[j];
+    const double dx = minimg(pj.x - pi.x, R.L, R.halfL), dy = minimg(pj.y - pi.y, R.L, R.halfL);
+    const double r = sqrt(dx * dx + dy * dy);
+    bx = dx; by = dy; bl = r;
+    if (r > mb) mb = r;
+    const double fb = -R.kb * (r - R.r0) / r;
+    if (WF) {
+        f[j].x += fb * dx;
+        f[j].y += fb * dy;
+        f[i].x -= fb * dx;
+        f[i].y -= fb * dy;
+    }
+    eb += 0.5 * R.kb * (r - R.r0) * (r - R.r0);
+}
+
+// One thread per bead: a lipid head walks its chain (bonds, then angle terms,
+// chain_forces_linear, cel/engine/oercel.c:1267-1308); the smaller-gid bead of
+// a chemistry bond applies that bond.  Every bead belongs to at most one of
+// them, so the writes never collide.
+template <bool ENERGY>
+__global__ __launch_bounds__(kBeadBlock) void k_bonded(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs,
+                                                       const int32_t *__restrict__ bblk,
+                                                       const double2 *__restrict__ xy, const uint32_t *__restrict__ gt,
+                                                       const int8_t *__restrict__ mp, const int32_t *__restrict__ lnk,
+                                                       double2 *__restrict__ f, double *__restrict__ ebond,
+                                                       uint32_t *__restrict__ err) {
+    const int r = bblk[blockIdx.x];
+    const RepInfo &R = ri[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    double mb = 0.0, eb = 0.0;
+    bool owner = false;
+    if (loc < R.N) {
+        const int s = R.off + loc;
+        const int m = mp[s];
+        if (m == 0) {
+            owner = true;
+            int sl[OC_MAXMLEN];
+            double bx[OC_MAXMLEN], by[OC_MAXMLEN], bl[OC_MAXMLEN];
+            int n = 0;
+            for (int d = s; d >= 0 && n < R.mlen; d = lnk[d]) sl[n++] = d;
+            if (n != R.mlen) atomicOr(err, ERR_CHAIN);
+            for (int b = 0; b < n - 1; b++) bond1<!ENERGY>(R, sl[b], sl[b + 1], xy, f, mb, eb, bx[b], by[b], bl[b]);
+            const double ka = R.ka;
+            if (ka != 0.0) {
+                for (int b = 0; b < n - 2; b++) {
+                    const int i = sl[b], j = sl[b + 1], k = sl[b + 2];
+                    const double ax = bx[b], ay = by[b], cxv = bx[b + 1], cyv = by[b + 1];
+                    const double la = bl[b], lc = bl[b + 1];
+                    const double ilac = 1.0 / (la * lc);
+                    const double csn = (ax * cxv + ay * cyv) * ilac;
+                    const double dUax = -ka * (cxv * ilac - csn * ax / (la * la));
+                    const double dUay = -ka * (cyv * ilac - csn * ay / (la * la));
+                    const double dUcx = -ka * (ax * ilac - csn * cxv / (lc * lc));
+                    const double dUcy = -ka * (ay * ilac - csn * cyv / (lc * lc));
+                    if (!ENERGY) {
+                        f[i].x += dUax;
+                        f[i].y += dUay;
+                        f[j].x += -dUax + dUcx;
+                        f[j].y += -dUay + dUcy;
+                        f[k].x -= dUcx;
+                        f[k].y -= dUcy;
+                    }
+                    eb += ka * (1.0 - csn);
+                }
+            }
+        } else if (m < 0) {
+            const int p = lnk[s];
+            if (p >= 0 && (gt[s] & kGidMask) < (gt[p] & kGidMask)) {
+                owner = true;
+                double bx, by, bl;
+                bond1<!ENERGY>(R, s, p, xy, f, mb, eb, bx, by, bl);
+            }
+        }
+        if (ENERGY) ebond[s] = owner ? eb : 0.0;
+    }
+    if (!ENERGY) {
+        uint64_t bits = (uint64_t)__double_as_longlong(mb);
+        bits = wave_max_u64(bits);
+        unsigned long long *mbw = (unsigned long long *)&rs[r].maxbond_bits;
+        if ((threadIdx.x & 63) == 0 && bits > __hip_atomic_load(mbw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(mbw, (unsigned long long)bits);
+    }
+}
+
+// --------------------------------------------------------------------------- precursor chemistry
+
+// Visit the beads within the cell window of bead i (cutoff <= 1) until fn
+// returns false.  Distances: FP64 minimum image.
+template <typename Fn>
+__device__ __forceinline__ void visit_nbrs(const RepInfo &R, const int32_t *cs, const double2 *xy, int i, double2 p,
+                                           Fn fn) {
+    const int cx = cell_x(R, p.x);
+    const int khi = (int)((p.y + 1.0) * R.icsy);
+    const int klo = (int)((p.y - 1.0) * R.icsy + R.ncy) - R.ncy;
+    for (int dc = -1; dc <= 1; dc++) {
+        int col = cx + dc;
+        col = col < 0 ? col + R.ncx : (col >= R.ncx ? col - R.ncx : col);
+        for (int k = klo; k <= khi; k++) {
+            const int kk = k < 0 ? k + R.ncy : (k >= R.ncy ? k - R.ncy : k);
+            const int c = R.coff + col * R.ncy + kk;
+            for (int j = cs[c]; j < cs[c + 1]; j++) {
+                if (j == i) continue;
+                const double2 q = xy[j];
+                const double dx = minimg(q.x - p.x, R.L, R.halfL), dy = minimg(q.y - p.y, R.L, R.halfL);
+                if (!fn(j, dx * dx + dy * dy)) return;
+            }
+        }
+    }
+}
+
+enum : uint8_t { CF_BREAK = 1, CF_CONV = 2 };
+
+// Decisions from the state at the start of the check (own definition):
+// break of chemistry bonds, P -> T conversion next to a lipid tail, and the
+// bond-formation proposal of every free T.
+__global__ __launch_bounds__(kBeadBlock) void k_chem_decide(const RepInfo *__restrict__ ri,
+                                                            const RepStep *__restrict__ rs,
+                                                            const int32_t *__restrict__ bblk,
+                                                            const int32_t *__restrict__ cs,
+                                                            const double2 *__restrict__ xy,
+                                                            const uint32_t *__restrict__ gt,
+                                                            const int8_t *__restrict__ mp,
+                                                            const int32_t *__restrict__ lnk,
+                                                            uint8_t *__restrict__ flag, int32_t *__restrict__ best) {
+    const int r = bblk[blockIdx.x];
+    const RepStep &S = rs[r];
+    if (!S.chem_now) return;
+    const RepInfo &R = ri[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    if (loc >= R.N) return;
+    const int s = R.off + loc;
+    const uint32_t g = gt[s] & kGidMask, ty = gt[s] >> kGidBits;
+    const int m = mp[s], l = lnk[s];
+    uint8_t fl = 0;
+    int32_t b = -1;
+    if (m < 0 && l >= 0 && g < (gt[l] & kGidMask) && hash_u01(S.kbreak, g) < R.p_break) fl |= CF_BREAK;
+    if (ty == OC_N && hash_u01(S.kconv, g) < R.p_conv) {
+        bool found = false;
+        visit_nbrs(R, cs, xy, s, xy[s], [&](int j, double r2) {
+            if ((gt[j] >> kGidBits) == OC_T && mp[j] >= 0 && r2 < R.rc2) { found = true; return false; }
+            return true;
+        });
+        if (found) fl |= CF_CONV;
+    }
+    if (ty == OC_T && m < 0 && l < 0) {
+        double bu = 2.0;
+        uint32_t bg = 0xffffffffu;
+        visit_nbrs(R, cs, xy, s, xy[s], [&](int j, double r2) {
+            const uint32_t gj = gt[j];
+            if ((gj >> kGidBits) == OC_T && mp[j] < 0 && lnk[j] < 0 && r2 < R.rf2) {
+                const uint32_t gg = gj & kGidMask;
+                const double u = pair_u01(S.kform, g, gg);
+                if (u < R.p_form && (u < bu || (u == bu && gg < bg))) { bu = u; bg = gg; b = j; }
+            }
+            return true;
+        });
+    }
+    flag[s] = fl;
+    best[s] = b;
+}
+
+__device__ __forceinline__ void push_event(Event *ev, unsigned long long *evn, int64_t cap, uint32_t *err, int64_t step,
+                                           int kind, uint32_t a, uint32_t b, int rep) {
+    const unsigned long long k = atomicAdd(evn, 1ull);
+    if ((int64_t)k < cap) {
+        Event e;
+        e.step = step; e.kind = kind; e.gid_a = a; e.gid_b = b; e.rep = rep;
+        ev[k] = e;
+    } else {
+        atomicOr(err, ERR_EVOVF);
+    }
+}
+
+__global__ __launch_bounds__(kBeadBlock) void k_chem_apply(const RepInfo *__restrict__ ri,
+                                                           const RepStep *__restrict__ rs,
+                                                           const int32_t *__restrict__ bblk, uint32_t *__restrict__ gt,
+                                                           int32_t *__restrict__ lnk, const uint8_t *__restrict__ flag,
+                                                           const int32_t *__restrict__ best, Event *__restrict__ ev,
+                                                           unsigned long long *__restrict__ evn, int64_t evcap,
+                                                           int record, uint32_t *__restrict__ err) {
+    const int r = bblk[blockIdx.x];
+    const RepStep &S = rs[r];
+    if (!S.chem_now) return;
+    const RepInfo &R = ri[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    if (loc >= R.N) return;
+    const int s = R.off + loc;
+    const uint32_t g = gt[s] & kGidMask;
+    const uint8_t fl = flag[s];
+    if (fl & CF_BREAK) {
+        const int p = lnk[s];
+        const uint32_t gp = gt[p] & kGidMask;
+        lnk[s] = -1;
+        lnk[p] = -1;
+        if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_BREAK, min(g, gp), max(g, gp), r);
+    }
+    if (fl & CF_CONV) {
+        gt[s] = g | ((uint32_t)OC_T << kGidBits);
+        if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_CONV, g, g, r);
+    }
+    const int b = best[s];
+    if (b >= 0 && best[b] == s) {
+        const uint32_t gb = gt[b] & kGidMask;
+        if (g < gb) {
+            lnk[s] = b;
+            lnk[b] = s;
+            if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_FORM, g, gb, r);
+        }
+    }
+}
+
+// --------------------------------------------------------------------------- reductions
+
+constexpr int kNStat = 16;  // k2, px, py, epair, ebond, count[7], lipids, bonds, stir, spare
+
+// Fixed-partition FP64 sums: chunk b of a replica covers beads
+// [b*kRedChunk, (b+1)*kRedChunk); a thread sums a strided subset in order,
+// then a fixed tree.  The result depends only on the slot order.
+__global__ __launch_bounds__(256) void k_stats_partial(const RepInfo *__restrict__ ri, const int32_t *__restrict__ rblk,
+                                                       const double2 *__restrict__ v, const double2 *__restrict__ f,
+                                                       const uint32_t *__restrict__ gt, const int8_t *__restrict__ mp,
+                                                       const int32_t *__restrict__ lnk,
+                                                       const double *__restrict__ epot,
+                                                       const double *__restrict__ ebond, double *__restrict__ part) {
+    __shared__ double red[kNStat][256];
+    const int r = rblk[blockIdx.x];
+    const RepInfo &R = ri[r];
+    const int b = blockIdx.x - R.rblk0;
+    double acc[kNStat];
+    for (int k = 0; k < kNStat; k++) acc[k] = 0.0;
+    for (int q = 0; q < kRedChunk / 256; q++) {
+        const int loc = b * kRedChunk + q * 256 + threadIdx.x;
+        if (loc >= R.N) break;
+        const int s = R.off + loc;
+        const double2 vv = v[s], ff = f[s];
+        const double vx = vv.x + R.hdt * ff.x, vy = vv.y + R.hdt * ff.y;
+        acc[0] += vx * vx + vy * vy;
+        acc[1] += vx;
+        acc[2] += vy;
+        acc[3] += epot[s];
+        acc[4] += ebond[s];
+        const uint32_t ty = gt[s] >> kGidBits;
+        for (int k = 0; k < 7; k++) acc[5 + k] += ty == (uint32_t)k ? 1.0 : 0.0;
+        acc[12] += mp[s] == 0 ? 1.0 : 0.0;
+        const int l = lnk[s];
+        acc[13] += (mp[s] < 0 && l >= 0 && (gt[s] & kGidMask) < (gt[l] & kGidMask)) ? 1.0 : 0.0;
+    }
+    for (int k = 0; k < kNStat; k++) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < kNStat; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < kNStat) part[(size_t)blockIdx.x * kNStat + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_stats_final(const RepInfo *__restrict__ ri, const double *__restrict__ part,
+                                                     double *__restrict__ out) {
+    __shared__ double red[256];
+    const RepInfo &R = ri[blockIdx.x];
+    const int nb = (R.N + kRedChunk - 1) / kRedChunk;
+    for (int k = 0; k < kNStat; k++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += 256) s += part[(size_t)(R.rblk0 + b) * kNStat + k];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[(size_t)blockIdx.x * kNStat + k] = red[0];
+        __syncthreads();
+    }
+}
+
+// stirring body force f_x = A sin(2 pi n y / L) and its per-replica sum
+// (force_init, cel/engine/oercel.c:1161-1196)
+__global__ __launch_bounds__(256) void k_stir_partial(const RepInfo *__restrict__ ri, const int32_t *__restrict__ rblk,
+                                                      const double2 *__restrict__ xy, double *__restrict__ stir,
+                                                      double *__restrict__ part) {
+    __shared__ double red[256];
+    const int r = rblk[blockIdx.x];
+    const RepInfo &R = ri[r];
+    const int b = blockIdx.x - R.rblk0;
+    double acc = 0.0;
+    for (int q = 0; q < kRedChunk / 256; q++) {
+        const int loc = b * kRedChunk + q * 256 + threadIdx.x;
+        if (loc >= R.N) break;
+        const int s = R.off + loc;
+        const double fs = R.has_stir ? R.stir_A * sin(2.0 * M_PI * (R.stir_kn * xy[s].y)) : 0.0;
+        stir[s] = fs;
+        acc += fs;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_stir_final(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs,
+                                                    const double *__restrict__ part) {
+    __shared__ double red[256];
+    const RepInfo &R = ri[blockIdx.x];
+    const int nb = (R.N + kRedChunk - 1) / kRedChunk;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) s += part[R.rblk0 + b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rs[blockIdx.x].stir_mean = red[0] / R.N;
+}
+
+// vh = v - dt/2 f after the first forces of a fresh start (cel/engine/oercel.c:2207-2211)
+__global__ __launch_bounds__(kBeadBlock) void k_half_kick(const RepInfo *__restrict__ ri,
+                                                          const int32_t *__restrict__ bblk, double2 *__restrict__ v,
+                                                          const double2 *__restrict__ f) {
+    const int r = bblk[blockIdx.x];
+    const RepInfo &R = ri[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    if (loc >= R.N) return;
+    const int s = R.off + loc;
+    double2 vv = v[s];
+    const double2 ff = f[s];
+    vv.x -= 0.5 * R.dt * ff.x;
+    vv.y -= 0.5 * R.dt * ff.y;
+    v[s] = vv;
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace oc
