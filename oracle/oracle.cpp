@@ -686,6 +686,7 @@ static void load_gid_order(OrSys *s, const oc_state *st) {
 /* fresh start, cel/engine/oercel.c:2196-2212: v~ = v, sort, forces at step 0, vh = v - dt/2 f */
 OR_API int or_start(OrSys *s, const oc_state *st, int32_t theta32) {
     load_gid_order(s, st);
+    s->p.warm_end = s->p.step_anchor + s->p.warmup; /* g_warm_end = warmup, cel/engine/oercel.c:2197 */
     s->vtx = s->vx;
     s->vty = s->vy;
     s->step = s->p.step_anchor;

@@ -1,0 +1,161 @@
+"""Write the golden fixtures of tests/golden from the reference engine itself.
+
+Run in a container that holds the reference source (the fixtures are committed;
+tests only read them):
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Every vector comes from the reference compiled from cel/engine/oercel.c
+(oracle/_ref: the static step functions behind ref_shim.c, and the reference
+CLI).  The reference is built without FMA contraction (its own bitwise mode),
+so the integer/ordering parts and the soup bits are exact references for the
+restatement, and the FP64 forces are references to ~1e-13.
+
+Fixtures (numpy .npz / JSON, small systems):
+  hashes.npz        mix13, step_seed, pair_theta, hash_key, hash_u01 on inputs
+  soups.npz         init_soup + set_velocities (gid order) of three soups
+  sorts.npz         sort_cells permutation + cell starts (nsub 1, 3, 4)
+  forces.npz        pair_forces (plain; stirring + kT cycle) and bonded_forces
+  traj.npz          the CLI's checkpoint after 20 steps from a fresh soup
+  params.json       the reference's --params table: keys, defaults, a_ij
+"""
+import ctypes as C
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+import oracle  # noqa: E402
+import importlib  # noqa: E402
+
+pm = importlib.import_module("welcome-youtube_amd.params")
+
+# soups: (name, L, nsub, overrides)
+SOUPS = [
+    ("forcetest", 12.5, 4, dict(f_lip=0.15, f_Ka=0.02, f_Kb=0.02, c_N=0.1, seed=99)),
+    ("cfg1like", 20.0, 4, dict(f_lip=0.1, lip_nT=2, c_N=0.0, f_Ka=0.0, seed=1)),
+    ("precursor", 16.0, 2, dict(f_lip=0.3, lip_nT=2, c_N=0.2, f_Ka=0.0, seed=7)),
+]
+
+
+def mk(L, nsub, **kw):
+    p = pm.Params()
+    d = {"L": repr(L), "nsub": str(nsub)}
+    d.update({k: str(v) for k, v in kw.items()})
+    p.update(d)
+    return p.finalize()
+
+
+def hashes(R):
+    rng = np.random.default_rng(2024)
+    z = rng.integers(0, 2**63, 64, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, 64, dtype=np.uint64)
+    seeds = z[:16]
+    steps = rng.integers(0, 10**7, 16)
+    gi = rng.integers(0, 2**29, 64).astype(np.uint32)
+    gj = rng.integers(0, 2**29, 64).astype(np.uint32)
+    salts = rng.integers(0, 64, 16).astype(np.uint64)
+    out = {
+        "z": z, "mix13": np.array([R.ref_mix13(int(v)) for v in z], np.uint64),
+        "seeds": seeds, "steps": steps,
+        "step_seed": np.array([R.ref_step_seed(int(s), int(t)) for s, t in zip(seeds, steps)], np.uint64),
+        "gi": gi, "gj": gj,
+        "pair_theta": np.array([R.ref_pair_theta(int(z[k % 64]), int(gi[k]), int(gj[k])) for k in range(64)]),
+        "salts": salts,
+        "hash_key": np.array([R.ref_hash_key(int(s), int(t), int(a)) for s, t, a in zip(seeds, steps, salts)],
+                             np.uint64),
+    }
+    keys = out["hash_key"]
+    out["hash_u01"] = np.array([R.ref_hash_u01(int(keys[k % 16]), int(gi[k])) for k in range(64)])
+    return out
+
+
+def main():
+    if not oracle.ref_available():
+        sys.exit("oracle/_ref is not built: make -C oracle ref")
+    R = oracle.ref()
+    np.savez_compressed(os.path.join(HERE, "hashes.npz"), **hashes(R))
+
+    soups, sorts, forces = {}, {}, {}
+    rng = np.random.default_rng(7)
+    for name, L, nsub, kw in SOUPS:
+        p = mk(L, nsub, **kw)
+        cp = p.to_c()
+        s = oracle.ref_soup(cp, p.N)
+        for k, v in s.items():
+            soups[f"{name}_{k}"] = v
+        soups[f"{name}_L"] = np.array(L)
+        # sort: moved positions, several nsub
+        x = (s["x"] + 0.4 * rng.standard_normal(p.N)) % L
+        y = (s["y"] + 0.4 * rng.standard_normal(p.N)) % L
+        for ns in (1, 3, 4):
+            cp.nsub = ns
+            src, cs = oracle.ref_sort(cp, x, y)
+            sorts[f"{name}_n{ns}_src"] = src
+            sorts[f"{name}_n{ns}_cstart"] = cs
+        sorts[f"{name}_x"], sorts[f"{name}_y"] = x, y
+        cp.nsub = nsub
+        # pair forces at step 12345, plain and with stirring + temperature cycle
+        vtx = s["vx"] + 0.3 * rng.standard_normal(p.N)
+        vty = s["vy"].copy()
+        st = dict(s, vtx=vtx, vty=vty)
+        forces[f"{name}_vtx"], forces[f"{name}_vty"] = vtx, vty
+        fx, fy, _ = oracle.ref_forces_gid(cp, st, 12345, cp.t_anchor + 12345 * cp.dt)
+        forces[f"{name}_fx"], forces[f"{name}_fy"] = fx, fy
+        cpe = mk(L, nsub, stir_A=0.7, stir_n=2, kT_amp=0.3, kT_period=10.0, **kw).to_c()
+        fx, fy, _ = oracle.ref_forces_gid(cpe, st, 12345, 2.5)
+        forces[f"{name}_env_fx"], forces[f"{name}_env_fy"] = fx, fy
+        # bonded forces of distorted lipids, with angle terms
+        xb, yb = s["x"].copy(), s["y"].copy()
+        lip = s["mpos"] >= 0
+        xb[lip] = (xb[lip] + 0.15 * rng.standard_normal(lip.sum())) % L
+        yb[lip] = (yb[lip] + 0.15 * rng.standard_normal(lip.sum())) % L
+        sb = dict(st, x=xb, y=yb)
+        fx, fy, mb = oracle.ref_forces_gid(cp, sb, 0, 0.0, with_pair=False, with_bonded=True)
+        forces[f"{name}_bx"], forces[f"{name}_by"] = xb, yb
+        forces[f"{name}_bfx"], forces[f"{name}_bfy"], forces[f"{name}_maxbond"] = fx, fy, np.array(mb)
+    np.savez_compressed(os.path.join(HERE, "soups.npz"), **soups)
+    np.savez_compressed(os.path.join(HERE, "sorts.npz"), **sorts)
+    np.savez_compressed(os.path.join(HERE, "forces.npz"), **forces)
+
+    # trajectory: the reference CLI, fresh soup, 20 steps, warm-up ramp active
+    traj = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, ov in (("lipid", dict(L="16", dt="0.01", f_lip="0.2", lip_nT="2", c_N="0", f_Ka="0", k_angle="10",
+                                        warmup="50", seed="5", nsub="4")),
+                         ("water", dict(L="14.5", dt="0.03", f_lip="0", c_N="0", f_Ka="0", warmup="0", seed="9",
+                                        nsub="2"))):
+            ov = dict(ov, chem="0", steps="20", snap_every="0", stats_every="0", ckpt_every="0", log_every="0",
+                      events="0", threads="2")
+            oracle.run_ref_cli(os.path.join(tmp, name), ov)
+            ck = oracle.read_ckpt(os.path.join(tmp, name, "ckpt.bin"))
+            g = oracle.ckpt_gid_order(ck)
+            for k in ("x", "y", "vx", "vy", "fx", "fy", "type", "mpos", "nxt"):
+                traj[f"{name}_{k}"] = g[k]
+            traj[f"{name}_slot_gid"] = ck["gid"]
+            traj[f"{name}_step"] = np.array(ck["step"])
+            traj[f"{name}_maxbond"] = np.array(ck["maxbond"])
+            traj[f"{name}_overrides"] = np.array(json.dumps(ov))
+    np.savez_compressed(os.path.join(HERE, "traj.npz"), **traj)
+
+    # parameter table: keys and defaults only (the reference's --params)
+    out = subprocess.run([oracle.ref_binary(), "--params"], capture_output=True, text=True, check=True).stdout
+    keys = re.findall(r"^\| `([^`]+)` \| ([^|]*) \|", out, re.M)
+    amat = re.findall(r"^\| \*\*(\w+)\*\* \|(.*)\|$", out, re.M)
+    tab = {"defaults": {k: (v.strip() if v.strip() != "(empty)" else "") for k, v in keys},
+           "order": [k for k, _ in keys],
+           "amatrix": {n: [float(x) for x in row.split("|") if x.strip()] for n, row in amat}}
+    with open(os.path.join(HERE, "params.json"), "w") as f:
+        json.dump(tab, f, indent=1)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,196 @@
+"""Checks of the oracle that do not need the reference: brute force, finite
+differences, closed forms, known-answer counts, and an independent numpy
+statement of the precursor chemistry.  CPU only."""
+import numpy as np
+import pytest
+
+from conftest import mkparams
+
+
+def test_feed_known_answer(oracle):
+    """Uniform food supply on water, L=48 (N=6912), seed 7, p_feed 0.003,
+    n_chem 10, 600 steps: 1138 conversions (cel/engine/README.md:519-520,
+    cel/engine/tests/validate.sh:133-134).  Depends only on hash_key/hash_u01."""
+    assert oracle.lib().or_feed_count(7, 6912, 0.003, 10, 600) == 1138
+
+
+def test_noise_moments_and_symmetry(oracle):
+    L = oracle.lib()
+    ss = L.or_step_seed(99, 12345)
+    th = np.array([L.or_pair_theta(ss, i, j) for i in range(400) for j in range(i + 1, i + 300)])
+    assert abs(th.mean()) < 0.01 and abs(th.var() - 1.0) < 0.01
+    assert th.min() >= -np.sqrt(3) and th.max() < np.sqrt(3)
+    assert L.or_pair_theta(ss, 5, 9) == L.or_pair_theta(ss, 9, 5)
+    # the device's 32-bit cut of the same deviate
+    d = max(abs(L.or_pair_theta(ss, i, j) - L.or_pair_theta32(ss, i, j)) for i in range(50) for j in range(50, 90))
+    assert d <= np.sqrt(12) / 2**32 * 1.0001
+
+
+@pytest.mark.parametrize("L,nsub", [(9.0, 1), (12.5, 4), (13.7, 3), (10.2, 8)])
+def test_cell_list_forces_equal_brute_force(oracle, L, nsub):
+    """Full-shell cell-list traversal finds every pair (O(N^2) minimum image)."""
+    p = mkparams(L, nsub=nsub, f_lip=0.15, c_N=0.1, seed=3)
+    rng = np.random.default_rng(1)
+    s = oracle.soup(p.to_c(), p.N)
+    s["vtx"] = s["vx"] + 0.3 * rng.standard_normal(p.N)
+    s["vty"] = s["vy"]
+    a = oracle.pair_forces_gid(p.to_c(), s, 77, theta32=0)
+    bx, by = oracle.pair_brute(p.to_c(), s, 77, theta32=0)
+    F = np.hypot(bx, by).max()
+    assert np.abs(a["fx"] - bx).max() < 1e-12 * F and np.abs(a["fy"] - by).max() < 1e-12 * F
+    assert abs(a["fx"].sum()) < 1e-9 and abs(a["fy"].sum()) < 1e-9
+
+
+def bonded_energy(x, y, mpos, nxt, bond, L, kb, r0, ka):
+    """Independent bonded energy: harmonic bonds and U = ka (1 - cos) angles."""
+    def mi(d):
+        return (d + L / 2) % L - L / 2
+    U = 0.0
+    for h in np.flatnonzero(mpos == 0):
+        ch = [h]
+        while nxt[ch[-1]] >= 0:
+            ch.append(nxt[ch[-1]])
+        v = [np.array([mi(x[ch[b + 1]] - x[ch[b]]), mi(y[ch[b + 1]] - y[ch[b]])]) for b in range(len(ch) - 1)]
+        for d in v:
+            U += 0.5 * kb * (np.linalg.norm(d) - r0) ** 2
+        for b in range(len(v) - 1):
+            U += ka * (1.0 - v[b] @ v[b + 1] / (np.linalg.norm(v[b]) * np.linalg.norm(v[b + 1])))
+    for i in np.flatnonzero((mpos < 0) & (bond > np.arange(len(x)))):
+        j = bond[i]
+        d = np.array([mi(x[j] - x[i]), mi(y[j] - y[i])])
+        U += 0.5 * kb * (np.linalg.norm(d) - r0) ** 2
+    return U
+
+
+def test_bonded_forces_finite_differences(oracle):
+    p = mkparams(12.0, f_lip=0.3, k_angle=10.0, seed=4)
+    rng = np.random.default_rng(2)
+    s = oracle.soup(p.to_c(), p.N)
+    lip = s["mpos"] >= 0
+    s["x"][lip] = (s["x"][lip] + 0.15 * rng.standard_normal(lip.sum())) % p.L
+    s["y"][lip] = (s["y"][lip] + 0.15 * rng.standard_normal(lip.sum())) % p.L
+    # two chemistry bonds between non-lipid beads
+    free = np.flatnonzero(~lip)[:4]
+    s["bond"][free[0]], s["bond"][free[1]] = free[1], free[0]
+    s["bond"][free[2]], s["bond"][free[3]] = free[3], free[2]
+    s["x"][free[1]] = (s["x"][free[0]] + 0.4) % p.L
+    s["y"][free[1]] = s["y"][free[0]]
+    fx, fy, mb, eb = oracle.bonded_gid(p.to_c(), s)
+    args = (s["mpos"], s["nxt"], s["bond"], p.L, p.k_bond, p.r0, p.k_angle)
+    assert abs(eb - bonded_energy(s["x"], s["y"], *args)) < 1e-9 * max(1.0, eb)
+    h = 1e-6
+    worst = 0.0
+    for i in list(np.flatnonzero(lip)[:60]) + list(free):
+        for arr, fa in ((s["x"], fx), (s["y"], fy)):
+            q0 = arr[i]
+            arr[i] = q0 + h
+            up = bonded_energy(s["x"], s["y"], *args)
+            arr[i] = q0 - h
+            um = bonded_energy(s["x"], s["y"], *args)
+            arr[i] = q0
+            worst = max(worst, abs(-(up - um) / (2 * h) - fa[i]))
+    assert worst < 1e-5 * max(1.0, np.abs(fx).max())
+    assert abs(fx.sum()) < 1e-10 and abs(fy.sum()) < 1e-10
+
+
+def test_ballistic_motion_without_forces(oracle):
+    """With every a_ij = 0, gamma = 0 and no bonds the force vanishes and the
+    integrator must move every bead on a straight line: x_n = x_0 + n dt v."""
+    p = mkparams(11.0, f_lip=0.0, gamma=0.0, a_W_W=0, seed=6)
+    cp = p.to_c()
+    s = oracle.soup(cp, p.N)
+    sysm = oracle.System(cp, p.N)
+    sysm.start(s)
+    sysm.step(40)
+    o = sysm.get_state()
+    L = p.L
+    exp_x = (s["x"] + 40 * p.dt * s["vx"]) % L
+    d = np.abs((o["x"] - exp_x + L / 2) % L - L / 2).max()
+    assert d < 1e-11 and np.array_equal(o["vx"], s["vx"]) and np.all(o["fx"] == 0)
+
+
+def chem_numpy(p, st, step, oracle):
+    """Independent statement of the precursor chemistry rules on gid-order
+    arrays (all-pairs distances), returning sorted (kind, a, b) tuples."""
+    L = oracle.lib()
+    N = len(st["x"])
+    x, y = st["x"], st["y"]
+    dx = x[None, :] - x[:, None]
+    dy = y[None, :] - y[:, None]
+    dx = np.where(dx > p.L / 2, dx - p.L, np.where(dx < -p.L / 2, dx + p.L, dx))
+    dy = np.where(dy > p.L / 2, dy - p.L, np.where(dy < -p.L / 2, dy + p.L, dy))
+    r2 = dx * dx + dy * dy
+    np.fill_diagonal(r2, np.inf)
+    t, mp, bd = st["type"], st["mpos"], st["bond"]
+    kc, kf, kb = (L.or_hash_key(p.seed, step, s) for s in (31, 32, 33))
+    ev = []
+    lipT = (t == 2) & (mp >= 0)
+    for i in range(N):
+        if mp[i] < 0 and bd[i] >= 0 and i < bd[i] and L.or_hash_u01(kb, i) < p.p_break:
+            ev.append((2, i, bd[i]))
+        if t[i] == 3 and L.or_hash_u01(kc, i) < p.p_conv and np.any(lipT & (r2[i] < p.r_conv * p.r_conv)):
+            ev.append((0, i, i))
+    free = (t == 2) & (mp < 0) & (bd < 0)
+    best = np.full(N, -1)
+    for i in np.flatnonzero(free):
+        cand = [(L.or_pair_u01(kf, i, j), j) for j in np.flatnonzero(free & (r2[i] < p.r_form * p.r_form))]
+        cand = [c for c in cand if c[0] < p.p_form]
+        if cand:
+            best[i] = min(cand)[1]
+    for i in range(N):
+        j = best[i]
+        if j >= 0 and best[j] == i and i < j:
+            ev.append((1, i, j))
+    return sorted(ev)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_chemistry_oracle_vs_numpy(oracle, seed):
+    """The oracle's cell-list chemistry equals an all-pairs numpy statement
+    of the same rules (events and their order)."""
+    p = mkparams(14.0, f_lip=0.3, c_N=0.2, pchem=1, n_chem=1, p_conv=0.6, p_form=0.6, p_break=0.4, seed=seed)
+    cp = p.to_c()
+    rng = np.random.default_rng(seed)
+    sysm = oracle.System(cp, p.N)
+    sysm.start(oracle.soup(cp, p.N))
+    st = sysm.get_state()
+    P = np.flatnonzero(st["type"] == 3)
+    conv = rng.choice(P, size=len(P) // 2, replace=False)
+    st["type"][conv] = 2
+    q = len(conv) // 4
+    a, b = conv[:q], conv[q:2 * q]
+    st["bond"][:] = -1
+    st["bond"][a], st["bond"][b] = b, a
+    sysm.set_state(st, 0)
+    sysm.chem(5)
+    ev = sysm.events()
+    got = sorted((int(e["kind"]), int(e["gid_a"]), int(e["gid_b"])) for e in ev)
+    exp = chem_numpy(p, st, 5, oracle)
+    assert len(exp) > 10 and got == exp
+
+
+def test_stats_closed_forms(oracle):
+    p = mkparams(12.0, f_lip=0.2, seed=2)
+    cp = p.to_c()
+    sysm = oracle.System(cp, p.N)
+    sysm.start(oracle.soup(cp, p.N))
+    sysm.step(3)
+    s = sysm.get_state()
+    o = sysm.stats()
+    vx = s["vx"] + 0.5 * p.dt * s["fx"]
+    vy = s["vy"] + 0.5 * p.dt * s["fy"]
+    T = (vx @ vx + vy @ vy) / (2 * p.N - 2)
+    assert abs(o.T_kin - T) < 1e-12 * T
+    assert abs(o.px - vx.sum()) < 1e-10 and abs(o.py - vy.sum()) < 1e-10
+    # pair energy: all pairs, a_ij/2 (1 - r)^2
+    x, y = s["x"], s["y"]
+    dx = (x[None, :] - x[:, None] + p.L / 2) % p.L - p.L / 2
+    dy = (y[None, :] - y[:, None] + p.L / 2) % p.L - p.L / 2
+    r = np.hypot(dx, dy)
+    A = np.array(p.a)[s["type"][:, None], s["type"][None, :]]
+    iu = np.triu_indices(p.N, 1)
+    rr, aa = r[iu], A[iu]
+    m = rr < 1
+    E = (0.5 * aa[m] * (1 - rr[m]) ** 2).sum()
+    assert abs(o.E_pair - E) < 1e-9 * E
+    assert o.count[0] + o.count[1] + o.count[2] == p.N

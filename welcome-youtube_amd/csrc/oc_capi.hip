@@ -618,8 +618,12 @@ OC_API int oc_start(oc_engine *h, const oc_state *s) {
     for (int r = 0; r < h->nrep; r++) {
         if (int rc = load_replica(h, r, s, base, true)) return rc;
         if (int rc = set_step_fields(h, r, h->prm[r].step_anchor)) return rc;
+        // a fresh start ramps the repulsion in over `warmup` steps
+        // (g_warm_end = warmup, cel/engine/oercel.c:2197)
+        h->ri[r].warm_end = h->prm[r].step_anchor + h->prm[r].warmup;
         base += h->ri[r].N;
     }
+    HIPCHK(hipMemcpy(h->d_ri, h->ri.data(), h->nrep * sizeof(RepInfo), hipMemcpyHostToDevice));
     drop_graphs(h);
     h->have_state = true;
     // forces at the start step, then vh = v - dt/2 f (cel/engine/oercel.c:2203-2211)
