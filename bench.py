@@ -131,6 +131,66 @@ def cpu_baseline(pm, N, nsub, target_s=12.0):
             "sample": f"oracle restatement, N={N}, chemistry off, 5 steps"}
 
 
+def ensemble_params(pm, nrep, seed=0):
+    """Batched-ensemble workload: N log-uniform in [2048, 65536] at rho = 3
+    (L = sqrt(N/3)), lipid fraction U[0.05, 0.4], a_WT U[60, 100], distinct
+    seeds, the physics and chemistry of the plain workload."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(nrep):
+        N = int(round(math.exp(rng.uniform(math.log(2048), math.log(65536)))))
+        p = config2_params(pm, N, 4)
+        p.update({"f_lip": repr(float(rng.uniform(0.05, 0.4))), "a_W_T": repr(float(rng.uniform(60, 100))),
+                  "seed": str(1000 + r)})
+        out.append(p.finalize())
+    return out
+
+
+def bench_ensemble(a, pm, eng):
+    """Throughput of R replicas stepped together by one engine (one launch per
+    phase for all replicas, hipGraph replay) vs. the same replicas stepped one
+    after another by single-replica engines (also graph replay)."""
+    ps = ensemble_params(pm, a.replicas)
+    if a.tune:
+        at = importlib.import_module("welcome-youtube_amd.autotune")
+        ps, blk, tb = at.tune_ensemble(ps)
+    else:
+        blk, tb = a.pair_block, a.tile_beads
+    Ntot = sum(p.N for p in ps)
+    e = eng.Engine(ps, pair_block=blk, tile_beads=tb)
+    e.start()
+    e.step(a.warmup)
+    e.step_count()
+    t0 = time.perf_counter()
+    e.step(a.steps)
+    batched = time.perf_counter() - t0
+    e.close()
+    singles = [eng.Engine(p) for p in ps]
+    for s in singles:
+        s.start()
+        s.step(a.warmup)
+    t0 = time.perf_counter()
+    for s in singles:
+        s.step(a.steps)
+    seq = time.perf_counter() - t0
+    for s in singles:
+        s.close()
+    out = {
+        "metric": "batched-ensemble particle-steps/s (vs a sequential loop of single-replica engines)",
+        "value": Ntot * a.steps / batched, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * batched / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"batched ensemble, {a.replicas} replicas, N log-uniform [2048, 65536], rho=3, "
+                               f"f_lip U[0.05,0.4], a_WT U[60,100], precursor chemistry on",
+                   "N_total": Ntot, "pair_block": blk, "tile_beads": tb,
+                   "nsub": sorted({p.nsub for p in ps})},
+        "sequential_value": Ntot * a.steps / seq,
+        "speedup_vs_sequential": seq / batched,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +202,9 @@ def main():
     ap.add_argument("--tile-beads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--workload", choices=("plain", "ensemble"), default="plain")
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--tune", action="store_true", help="autotune nsub / pair launch shape first")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the step on a single MI355X "
@@ -150,6 +213,8 @@ def main():
 
     pm = importlib.import_module("welcome-youtube_amd.params")
     eng = importlib.import_module("welcome-youtube_amd.engine")
+    if a.workload == "ensemble":
+        return bench_ensemble(a, pm, eng)
     p = config2_params(pm, a.N, a.nsub)
     N = p.N
     e = eng.Engine(p, pair_block=a.pair_block, tile_beads=a.tile_beads)

@@ -816,3 +816,276 @@ OR_API void or_stats(OrSys *s, oc_stats *o) {
 }
 
 OR_API int64_t or_get_step(OrSys *s) { return s->step; }
+
+/* ------------------------------------------------------------------ adjoint (own definition, FP64 reference gradient) */
+
+/*
+ * Gradient of a smooth radial-distribution loss over a window of steps with
+ * respect to theta = (a_WW, a_WH, a_WT, a_HH, a_HT, a_TT, gamma), the noise
+ * stream held fixed (no reference counterpart; BASELINE adjoint window).
+ *
+ * Window: start state (x0, vh0, v~0) loaded in the system; f0 = F(x0, v~0;
+ * theta) is recomputed, then nsteps steps of the Groot-Warren update
+ * (or_integrate + sort + forces; chemistry must be off).
+ * RDF of type pair (ta, tb) on nbins bins of [0, rmax): a Gaussian kernel of
+ * width `width` per pair, cut at 4 widths,
+ *   g_b(x) = C_b sum_{i<j} phi(r_ij - c_b),  C_b = dr * norm / (pi (r_{b+1}^2 - r_b^2)),
+ *   norm = 2 / (n_a rho_a) for ta == tb, 1 / (n_a rho_b) otherwise,
+ * averaged over steps 1..nsteps; loss = sum_b (gbar_b - target_b)^2.
+ * Reverse sweep (f_n = F(x_n, v~_n)):
+ *   X_n += Jx^T F_n ; T_n = Jv^T F_n ; theta_bar += Jtheta^T F_n
+ *   X_{n-1} = X_n + dL/dx_{n-1};  V_{n-1} = dt X_n + T_n + V_n;
+ *   F_{n-1} = dt^2 X_n + (1/2 + lambda) dt T_n + dt V_n
+ * with adjoints carried through the sort permutations.  Pair Jacobians use
+ * f_ij = m e, m = w (a - gamma w (e.u) + s theta_ij), s = sqrt(2 gamma kT / dt);
+ * bonds f = -k (r - r0)/r d.  Angle terms, stirring, the kT cycle and the
+ * warm-up ramp must be off.
+ */
+struct RdfSpec {
+    int32_t ta, tb, nbins, pad;
+    double rmax, width;
+};
+
+static inline int theta_index(int a, int b) {
+    if (a > b) { const int t = a; a = b; b = t; }
+    if (b > 2) return -1;
+    static const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    return idx[a][b];
+}
+
+static void rdf_norm(const RdfSpec &sp, int na, int nb, double L, std::vector<double> &C) {
+    const double dr = sp.rmax / sp.nbins, area = L * L;
+    const double norm = sp.ta == sp.tb ? 2.0 / (na * (na / area)) : 1.0 / (na * (nb / area));
+    C.resize(sp.nbins);
+    for (int b = 0; b < sp.nbins; b++) {
+        const double r0 = b * dr, r1 = (b + 1) * dr;
+        C[b] = dr * norm / (M_PI * (r1 * r1 - r0 * r0));
+    }
+}
+
+/* all-pairs smooth RDF (and, with gw != null, X += dL/dx for bin weights gw) */
+static void rdf_pass(const OrSys *s, const RdfSpec &sp, const std::vector<double> &C, double *g, const double *gw,
+                     double *X) {
+    const int N = s->N;
+    const double L = s->p.L, dr = sp.rmax / sp.nbins, w = sp.width, cut = sp.rmax + 4.0 * w;
+    const double pref = 1.0 / (sqrt(2.0 * M_PI) * w);
+    for (int i = 0; i < N; i++) {
+        const int ti = s->type[i];
+        if (ti != sp.ta && ti != sp.tb) continue;
+        for (int j = 0; j < N; j++) {
+            if (j == i) continue;
+            const int tj = s->type[j];
+            const bool pr = (ti == sp.ta && tj == sp.tb) || (ti == sp.tb && tj == sp.ta);
+            if (!pr) continue;
+            const double dx = minimg(s->x[i] - s->x[j], L), dy = minimg(s->y[i] - s->y[j], L);
+            const double r = sqrt(dx * dx + dy * dy);
+            if (r >= cut || r == 0.0) continue;
+            const int b0 = std::max(0, (int)floor((r - 4.0 * w) / dr)), b1 = std::min(sp.nbins - 1, (int)floor((r + 4.0 * w) / dr));
+            double dsum = 0.0;
+            for (int b = b0; b <= b1; b++) {
+                const double u = r - (b + 0.5) * dr;
+                if (fabs(u) > 4.0 * w) continue;
+                const double ph = pref * exp(-0.5 * u * u / (w * w));
+                /* each unordered pair once for g (ta == tb: i < j) */
+                if (g && (sp.ta != sp.tb ? (ti == sp.ta) : (i < j))) g[b] += C[b] * ph;
+                if (gw) dsum += gw[b] * C[b] * (-u / (w * w)) * ph;
+            }
+            if (gw) {
+                /* d/dx_i of the pair's terms (counted once per unordered pair) */
+                X[2 * i] += dsum * dx / r;
+                X[2 * i + 1] += dsum * dy / r;
+            }
+        }
+    }
+}
+
+/* J^T products of the pair + bond forces at the current state for force adjoint F:
+ * X += Jx^T F, T = Jv^T F, th += Jtheta^T F */
+static void force_adjoint(const OrSys *s, int64_t step, int theta32, const double *F, double *X, double *T, double *th) {
+    const int N = s->N;
+    const oc_params *p = &s->p;
+    const Geo &g = s->g;
+    const PairPar pp = pair_par(p, step, time_of(p, step));
+    const int ncx = g.ncx, ncy = g.ncy;
+    const double L = g.L, sdg = pp.sdt / (2.0 * p->gamma);
+    for (int i = 0; i < 2 * N; i++) T[i] = 0.0;
+    for (int cx = 0; cx < ncx; cx++) {
+        for (int i = s->cstart[cx * ncy]; i < s->cstart[(cx + 1) * ncy]; i++) {
+            const double xi = s->x[i], yi = s->y[i];
+            const int khi = (int)((yi + 1.0) * g.icsy);
+            const int klo = (int)((yi - 1.0) * g.icsy + ncy) - ncy;
+            double xa = 0, ya = 0, ta = 0, tb = 0, tha[7] = {0, 0, 0, 0, 0, 0, 0};
+            for (int dc = -1; dc <= 1; dc++) {
+                int col = cx + dc;
+                double sx = 0.0;
+                if (col < 0) { col += ncx; sx = -L; }
+                else if (col >= ncx) { col -= ncx; sx = L; }
+                for (int k = klo; k <= khi; k++) {
+                    const int kk = k < 0 ? k + ncy : (k >= ncy ? k - ncy : k);
+                    const double sy = k < 0 ? -L : (k >= ncy ? L : 0.0);
+                    const int c = col * ncy + kk;
+                    for (int j = s->cstart[c]; j < s->cstart[c + 1]; j++) {
+                        const double dx = (xi - s->x[j]) - sx, dy = (yi - s->y[j]) - sy;
+                        const double r2 = dx * dx + dy * dy;
+                        if (!(r2 < 1.0 && r2 > 1e-24)) continue;
+                        const double r = sqrt(r2), w = 1.0 - r;
+                        const double ex = dx / r, ey = dy / r;
+                        const double ux = s->vtx[i] - s->vtx[j], uy = s->vty[i] - s->vty[j];
+                        const double eu = ex * ux + ey * uy;
+                        const double th = theta32 ? pair_theta32(pp.sseed, s->gid[i], s->gid[j])
+                                                  : pair_theta(pp.sseed, s->gid[i], s->gid[j]);
+                        const int ti = s->type[i], tj = s->type[j];
+                        const double a = pp.a[8 * ti + tj];
+                        const double m = w * (a - pp.gamma * w * eu + pp.sdt * th);
+                        const double dmr = -(a + pp.sdt * th) + 2.0 * pp.gamma * w * eu;
+                        const double dmu = -pp.gamma * w * w;
+                        const double lx = F[2 * i] - F[2 * j], ly = F[2 * i + 1] - F[2 * j + 1];
+                        const double el = ex * lx + ey * ly;
+                        const double pux = ux - eu * ex, puy = uy - eu * ey; /* P u */
+                        const double plx = lx - el * ex, ply = ly - el * ey; /* P lambda */
+                        xa += (dmr * ex + dmu * pux / r) * el + m * plx / r;
+                        ya += (dmr * ey + dmu * puy / r) * el + m * ply / r;
+                        ta += dmu * el * ex;
+                        tb += dmu * el * ey;
+                        const int q = theta_index(ti, tj);
+                        if (q >= 0) tha[q] += 0.5 * w * el;
+                        tha[6] += 0.5 * el * (-w * w * eu + sdg * w * th);
+                    }
+                }
+            }
+            X[2 * i] += xa;
+            X[2 * i + 1] += ya;
+            T[2 * i] = ta;
+            T[2 * i + 1] = tb;
+            for (int q = 0; q < 7; q++) th[q] += tha[q];
+        }
+    }
+    /* harmonic bonds (lipid chains and chemistry bonds), d = x_j - x_i */
+    auto bond_adj = [&](int i, int j) {
+        const double dx = minimg(s->x[j] - s->x[i], L), dy = minimg(s->y[j] - s->y[i], L);
+        const double r = sqrt(dx * dx + dy * dy);
+        const double gg = -p->k_bond * (r - p->r0) / r, gp = -p->k_bond * p->r0 / (r * r);
+        const double ex = dx / r, ey = dy / r;
+        const double lx = F[2 * j] - F[2 * i], ly = F[2 * j + 1] - F[2 * i + 1];
+        const double el = ex * lx + ey * ly;
+        const double bx = gg * lx + gp * r * el * ex, by = gg * ly + gp * r * el * ey;
+        X[2 * j] += bx;
+        X[2 * j + 1] += by;
+        X[2 * i] -= bx;
+        X[2 * i + 1] -= by;
+    };
+    const int ml = p->lip_nH + p->lip_nT;
+    for (int h = 0; h < N; h++) {
+        if (s->mpos[h] == 0) {
+            int prev = h, n = 1;
+            for (int d = s->lnk[h]; d >= 0 && n < ml; d = s->lnk[d], n++) { bond_adj(prev, d); prev = d; }
+        } else if (s->mpos[h] < 0 && s->lnk[h] >= 0 && s->gid[h] < s->gid[s->lnk[h]]) {
+            bond_adj(h, s->lnk[h]);
+        }
+    }
+}
+
+struct TapeStep {
+    std::vector<double> x, y, vtx, vty;
+    std::vector<uint8_t> type;
+    std::vector<uint32_t> gid;
+    std::vector<int8_t> mpos;
+    std::vector<int32_t> lnk, cstart, src;
+};
+
+static void tape_load(OrSys *s, const TapeStep &t) {
+    s->x = t.x; s->y = t.y; s->vtx = t.vtx; s->vty = t.vty; s->type = t.type; s->gid = t.gid;
+    s->mpos = t.mpos; s->lnk = t.lnk; s->cstart = t.cstart;
+}
+
+/* Window gradient on the loaded state (which the call advances by nsteps).
+ * out: loss, grad[7], gmean[nbins].  Returns 0, or -1 if the physics is not
+ * covered (angles, stirring, kT cycle, ramp or chemistry on). */
+OR_API int or_window_grad(OrSys *s, int32_t nsteps, const RdfSpec *spec, const double *target, int32_t theta32,
+                          double *loss, double *grad, double *gmean) {
+    const oc_params *p = &s->p;
+    if (p->k_angle != 0.0 || p->stir_A != 0.0 || p->kT_amp != 0.0 || p->chem || s->step < p->warm_end) return -1;
+    const RdfSpec sp = *spec;
+    const int N = s->N, nb = sp.nbins;
+    int na = 0, nbb = 0;
+    for (int i = 0; i < N; i++) { na += s->type[i] == sp.ta; nbb += s->type[i] == sp.tb; }
+    std::vector<double> C;
+    rdf_norm(sp, na, nbb, p->L, C);
+    /* forward with tape: state 0 (after recomputing f0), then steps 1..n */
+    std::vector<TapeStep> tape(nsteps + 1);
+    auto record = [&](int n) {
+        TapeStep &t = tape[n];
+        t.x = s->x; t.y = s->y; t.vtx = s->vtx; t.vty = s->vty; t.type = s->type; t.gid = s->gid;
+        t.mpos = s->mpos; t.lnk = s->lnk; t.cstart = s->cstart; t.src = s->src;
+    };
+    sys_forces(s, s->step, theta32);
+    record(0);
+    std::vector<double> gacc(nb, 0.0);
+    const int64_t step0 = s->step;
+    for (int n = 1; n <= nsteps; n++) {
+        or_step(s, 1, theta32);
+        record(n);
+        rdf_pass(s, sp, C, gacc.data(), nullptr, nullptr);
+    }
+    double Lsum = 0.0;
+    std::vector<double> gw(nb);
+    for (int b = 0; b < nb; b++) {
+        gmean[b] = gacc[b] / nsteps;
+        const double d = gmean[b] - target[b];
+        Lsum += d * d;
+        gw[b] = 2.0 * d / nsteps;
+    }
+    *loss = Lsum;
+    /* reverse sweep; the live system is restored to the window end afterwards */
+    OrSys saved = *s;
+    std::vector<double> X(2 * N, 0.0), V(2 * N, 0.0), F(2 * N, 0.0), T(2 * N, 0.0);
+    std::vector<double> th(7, 0.0);
+    const double dt = p->dt, c = (0.5 + p->lambda) * dt;
+    tape_load(s, tape[nsteps]);
+    rdf_pass(s, sp, C, nullptr, gw.data(), X.data());
+    for (int n = nsteps; n >= 1; n--) {
+        tape_load(s, tape[n]);
+        force_adjoint(s, step0 + n, theta32, F.data(), X.data(), T.data(), th.data());
+        std::vector<double> Xp(2 * N), Vp(2 * N), Fp(2 * N);
+        for (int k = 0; k < 2 * N; k++) {
+            Xp[k] = X[k];
+            Vp[k] = dt * X[k] + T[k] + V[k];
+            Fp[k] = dt * dt * X[k] + c * T[k] + dt * V[k];
+        }
+        const std::vector<int32_t> &src = tape[n].src;
+        for (int d = 0; d < N; d++) {
+            const int o = src[d];
+            X[2 * o] = Xp[2 * d]; X[2 * o + 1] = Xp[2 * d + 1];
+            V[2 * o] = Vp[2 * d]; V[2 * o + 1] = Vp[2 * d + 1];
+            F[2 * o] = Fp[2 * d]; F[2 * o + 1] = Fp[2 * d + 1];
+        }
+        if (n - 1 >= 1) {
+            tape_load(s, tape[n - 1]);
+            rdf_pass(s, sp, C, nullptr, gw.data(), X.data());
+        }
+    }
+    tape_load(s, tape[0]);
+    force_adjoint(s, step0, theta32, F.data(), X.data(), T.data(), th.data());
+    for (int q = 0; q < 7; q++) grad[q] = th[q];
+    *s = saved;
+    return 0;
+}
+
+/* smooth RDF of the loaded state (for targets) */
+OR_API void or_rdf(OrSys *s, const RdfSpec *spec, double *g) {
+    const RdfSpec sp = *spec;
+    int na = 0, nbb = 0;
+    for (int i = 0; i < s->N; i++) { na += s->type[i] == sp.ta; nbb += s->type[i] == sp.tb; }
+    std::vector<double> C;
+    rdf_norm(sp, na, nbb, s->p.L, C);
+    for (int b = 0; b < sp.nbins; b++) g[b] = 0.0;
+    rdf_pass(s, sp, C, g, nullptr, nullptr);
+}
+
+/* change the pair parameters of the loaded system (a_ij symmetric 8x8, gamma) */
+OR_API void or_set_pair_params(OrSys *s, const double *a64, double gamma) {
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++) s->p.a[i][j] = a64[8 * i + j];
+    s->p.gamma = gamma;
+}
+

@@ -25,6 +25,17 @@ REF_SRC = "/root/reference/cel/engine/oercel.c"
 
 P = C.POINTER
 vp = C.c_void_p
+
+
+class RdfSpec(C.Structure):
+    """Smooth RDF of the type pair (ta, tb): nbins bins on [0, rmax), Gaussian
+    kernel of the given width per pair."""
+    _fields_ = [("ta", C.c_int32), ("tb", C.c_int32), ("nbins", C.c_int32), ("pad", C.c_int32),
+                ("rmax", C.c_double), ("width", C.c_double)]
+
+
+def rdf_spec(ta=2, tb=2, nbins=64, rmax=3.0, width=None):
+    return RdfSpec(ta, tb, nbins, 0, rmax, width if width is not None else 0.5 * rmax / nbins)
 dp = P(C.c_double)
 _OR = None
 _REF = None
@@ -81,6 +92,9 @@ def lib():
             "or_get_cells": (None, [vp, P(C.c_int32), P(C.c_uint32)]),
             "or_stats": (None, [vp, P(abi.oc_stats)]),
             "or_get_step": (C.c_int64, [vp]),
+            "or_window_grad": (C.c_int, [vp, C.c_int32, P(RdfSpec), dp, C.c_int32, dp, dp, dp]),
+            "or_rdf": (None, [vp, P(RdfSpec), dp]),
+            "or_set_pair_params": (None, [vp, dp, C.c_double]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -296,6 +310,30 @@ class System:
         o = abi.oc_stats()
         lib().or_stats(self.h, C.byref(o))
         return o
+
+    def rdf(self, spec):
+        g = np.zeros(spec.nbins)
+        lib().or_rdf(self.h, C.byref(spec), d(g))
+        return g
+
+    def set_pair_params(self, a, gamma):
+        """a: 7x7 (or 8x8) symmetric repulsion matrix."""
+        a64 = np.zeros((8, 8))
+        a = np.asarray(a, dtype=np.float64)
+        a64[:a.shape[0], :a.shape[1]] = a
+        lib().or_set_pair_params(self.h, d(a64.ravel().copy()), float(gamma))
+
+    def window_grad(self, nsteps, spec, target, theta32=1):
+        """(loss, grad[7] over a_WW, a_WH, a_WT, a_HH, a_HT, a_TT, gamma, mean RDF);
+        advances the system by nsteps."""
+        g = np.zeros(7)
+        gm = np.zeros(spec.nbins)
+        loss = C.c_double()
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        rc = lib().or_window_grad(self.h, int(nsteps), C.byref(spec), d(t), int(theta32), C.byref(loss), d(g), d(gm))
+        if rc:
+            raise ValueError("window gradient needs k_angle = 0, no stirring / kT cycle / warm-up / chemistry")
+        return loss.value, g, gm
 
     @property
     def step_count(self):

@@ -191,18 +191,19 @@ static void launch_pair(oc_engine *h, bool energy) {
     A.epot = h->d_epot;
     A.err = h->d_err;
     const dim3 grid(8 * h->per_xcd);
+    // CAP (staged candidates) is 4x the block; the lists hold 24 pairs per lane
     switch (h->pair_block) {
     case 64:
-        if (energy) k_pair<64, 256, true><<<grid, 64, 0, h->stream>>>(A);
-        else k_pair<64, 256, false><<<grid, 64, 0, h->stream>>>(A);
+        if (energy) k_pair<64, 256, 24, true><<<grid, 64, 0, h->stream>>>(A);
+        else k_pair<64, 256, 24, false><<<grid, 64, 0, h->stream>>>(A);
         break;
     case 128:
-        if (energy) k_pair<128, 512, true><<<grid, 128, 0, h->stream>>>(A);
-        else k_pair<128, 512, false><<<grid, 128, 0, h->stream>>>(A);
+        if (energy) k_pair<128, 512, 24, true><<<grid, 128, 0, h->stream>>>(A);
+        else k_pair<128, 512, 24, false><<<grid, 128, 0, h->stream>>>(A);
         break;
     default:
-        if (energy) k_pair<256, 1024, true><<<grid, 256, 0, h->stream>>>(A);
-        else k_pair<256, 1024, false><<<grid, 256, 0, h->stream>>>(A);
+        if (energy) k_pair<256, 1024, 24, true><<<grid, 256, 0, h->stream>>>(A);
+        else k_pair<256, 1024, 24, false><<<grid, 256, 0, h->stream>>>(A);
         break;
     }
 }
@@ -445,7 +446,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     h->Ctot = coff;
     h->nscan = grid_of(h->Ctot, kScanChunk);
     h->evq.resize(nrep);
-    h->evcap = std::min<int64_t>(std::max<int64_t>(1 << 20, 4 * h->Ntot), 8 << 20);
+    h->evcap = std::min<int64_t>(std::max<int64_t>(1 << 16, 4 * h->Ntot), 8 << 20);
     int rc = 0;
     const size_t Nt = (size_t)h->Ntot, Ct = (size_t)h->Ctot;
     rc = rc ? rc : dalloc(h, &h->d_ri, nrep);

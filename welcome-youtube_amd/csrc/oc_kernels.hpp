@@ -337,8 +337,9 @@ struct PairArgs {
 // in LDS once per tile; each lane then runs its own y-window per column, the
 // window expressions of cel/engine/oercel.c:1073-1074.  Forces are summed in
 // FP64 per bead, no atomics: the result is deterministic.
-template <int BLOCK, int CAP, bool ENERGY>
+template <int BLOCK, int CAP, int CAPL, bool ENERGY>
 __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
+    __shared__ uint16_t s_lst[CAPL][BLOCK];  // per-lane list of interacting candidates
     __shared__ double2 s_xy[CAP];
     __shared__ float2 s_vt[CAP];
     __shared__ uint32_t s_gt[CAP];
@@ -430,6 +431,14 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     const int slot0 = cs[R.coff + T.cx * ncy + T.k0];
     // A tile whose candidates overflow the LDS buffer (a dense clump) runs the
     // same loop on global memory; s_row offsets are then rebased per piece.
+    // Each lane owns one i-bead at a time.  Staged tiles run two phases: the
+    // cutoff test over all candidates appends the interacting ones to the
+    // lane's LDS list (u16 code: LDS index | column << 10 | piece << 12), then
+    // the force and noise arithmetic runs over that list only, so a wavefront
+    // executes it ~max(pairs) times instead of ~max(candidates) times.  A list
+    // that fills up spills to in-place evaluation.  Tiles whose candidates
+    // overflow the LDS buffer (a dense clump) run one pass on global memory,
+    // with s_row offsets rebased per piece.
     auto run = [&](const double2 *XY, const float2 *VT, const uint32_t *GT, const bool stg) {
         for (int ii = ibeg + threadIdx.x; ii < iend; ii += BLOCK) {
             const int slot = slot0 + (ii - ibeg);
@@ -448,6 +457,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             }
             double ax = 0.0, ay = 0.0;
             float e = 0.0f;
+            int cnt = 0;
             for (int c = 0; c < 3; c++) {
                 // window split at the periodic seams: rows < 0 (image -L, staged piece 0),
                 // 0..ncy-1 (piece 1), >= ncy (image +L, piece 2)
@@ -460,10 +470,30 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                     if (pa > pb) continue;
                     const int adj = stg ? 0 : s_piece[c][q][0] - s_piece[c][q][2];
                     const int ja = s_row[c][pa - kb0] + adj, jb = s_row[c][pb + 1 - kb0] + adj;
-                    for (int j = ja; j < jb; j++)
-                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], sxc[c], sy, gam, sdt, sseed,
-                                 ax, ay, e, ENERGY);
+                    if (!stg) {
+                        for (int j = ja; j < jb; j++)
+                            pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], sxc[c], sy, gam, sdt,
+                                     sseed, ax, ay, e, ENERGY);
+                        continue;
+                    }
+                    for (int j = ja; j < jb; j++) {
+                        const double2 pj = XY[j];
+                        const float dx = (float)((pi.x - pj.x) - sxc[c]), dy = (float)((pi.y - pj.y) - sy);
+                        const float r2 = dx * dx + dy * dy;
+                        if (r2 < 1.0f && r2 > 1e-24f) {
+                            if (cnt < CAPL) s_lst[cnt++][threadIdx.x] = (uint16_t)(j | (c << 10) | (q << 12));
+                            else pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, pj, VT[j], GT[j], sxc[c], sy, gam, sdt,
+                                          sseed, ax, ay, e, ENERGY);
+                        }
+                    }
                 }
+            }
+            for (int k = 0; k < cnt; k++) {
+                const int code = s_lst[k][threadIdx.x];
+                const int j = code & 1023, c = (code >> 10) & 3, q = code >> 12;
+                const double sy = q == 0 ? -L : (q == 2 ? L : 0.0);
+                pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], sxc[c], sy, gam, sdt, sseed, ax, ay,
+                         e, ENERGY);
             }
             if (ENERGY) {
                 A.epot[slot] = (double)e;
