@@ -191,6 +191,64 @@ def bench_ensemble(a, pm, eng):
     print(json.dumps(out), flush=True)
 
 
+def adjoint_params(pm, N=65536, seed=1, **over):
+    """Adjoint window workload: N = 65,536 at rho = 3 (L = 147.8017), 30% H-T-T
+    lipid beads, 70% W, harmonic bonds only, chemistry off, dt = 0.01."""
+    p = config2_params(pm, N, 4, chem=False)
+    p.update({"c_N": "0", "seed": str(seed)})
+    p.update({k: str(v) for k, v in over.items()})
+    return p.finalize()
+
+
+def bench_adjoint(a, pm, eng):
+    """Cost of a 200-step window gradient (forward with checkpoints every 20
+    steps + the adjoint sweep with recomputation) against 200 plain steps."""
+    abi = importlib.import_module("welcome-youtube_amd.abi")
+    spec = abi.rdf_spec(2, 2, 64, 3.0)
+    nwin = a.window
+    # target: the tail-tail RDF of a seed run at a_TT = 20, a_WT = 90
+    pt = adjoint_params(pm, seed=2, a_T_T=20, a_W_T=90)
+    with eng.Engine(pt) as et:
+        et.start()
+        et.step(a.equil)
+        target = et.window_forward(nwin, spec)[0]
+    p = adjoint_params(pm)
+    e = eng.Engine(p)
+    e.start()
+    e.step(a.equil)
+    st0 = e.get_state()
+    s0 = e.step_count()
+    e.step(nwin)  # warm the graph
+    e.set_state(st0, step=s0)
+    e.step_count()
+    t0 = time.perf_counter()
+    e.step(nwin)
+    t_fwd = time.perf_counter() - t0
+    e.set_state(st0, step=s0)
+    e.window_forward(nwin, spec)  # warm buffers
+    e.set_state(st0, step=s0)
+    t0 = time.perf_counter()
+    gbar = e.window_forward(nwin, spec)
+    t1 = time.perf_counter()
+    dl = 2.0 * (gbar - target[None, :])
+    grad = e.window_backward(dl)[0]
+    t2 = time.perf_counter()
+    loss = float(((gbar[0] - target) ** 2).sum())
+    e.close()
+    out = {
+        "metric": "adjoint window: gradient cost vs plain forward steps",
+        "value": (t2 - t0) / t_fwd, "unit": "x forward", "n_gpus": 1, "steps": nwin, "warmup": a.equil,
+        "ms_per_step": 1e3 * (t2 - t0) / nwin, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "adjoint window: N=65,536, rho=3, 30% H-T-T lipids, k_angle=0, chemistry off, "
+                               f"{nwin}-step window, checkpoint every 20, tail-tail RDF 64 bins on [0,3), target "
+                               "from a seed run at a_TT=20, a_WT=90", "N": p.N},
+        "forward_ms": 1e3 * t_fwd, "window_forward_ms": 1e3 * (t1 - t0), "window_backward_ms": 1e3 * (t2 - t1),
+        "loss": loss, "grad": {n: float(g) for n, g in zip(eng.Engine.THETA_NAMES, grad)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,7 +260,9 @@ def main():
     ap.add_argument("--tile-beads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--workload", choices=("plain", "ensemble"), default="plain")
+    ap.add_argument("--workload", choices=("plain", "ensemble", "adjoint"), default="plain")
+    ap.add_argument("--window", type=int, default=200)
+    ap.add_argument("--equil", type=int, default=1000)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--tune", action="store_true", help="autotune nsub / pair launch shape first")
     a = ap.parse_args()
@@ -215,6 +275,8 @@ def main():
     eng = importlib.import_module("welcome-youtube_amd.engine")
     if a.workload == "ensemble":
         return bench_ensemble(a, pm, eng)
+    if a.workload == "adjoint":
+        return bench_adjoint(a, pm, eng)
     p = config2_params(pm, a.N, a.nsub)
     N = p.N
     e = eng.Engine(p, pair_block=a.pair_block, tile_beads=a.tile_beads)

@@ -136,6 +136,12 @@ void oc_destroy(oc_engine *h);
  * and target i-beads per pair tile. */
 int oc_set_tuning(oc_engine *h, int32_t pair_block, int32_t tile_beads);
 
+/* Pair-kernel variant: lists = 1 tests every candidate first and evaluates
+ * the listed interacting pairs in a second pass; 0 (default, faster on
+ * gfx950) evaluates in one pass.  Same forces up to the FP64 rounding of the per-bead sums (pairs a
+ * full list spills are summed in another order). */
+int oc_set_pair_variant(oc_engine *h, int32_t lists);
+
 /* Host-side soup of replica r in gid order, exactly as init_soup +
  * set_velocities (cel/engine/oercel.c:1757-1814) with rng_seed(seed)
  * (cel/engine/oercel.c:2193-2195); velocities are full velocities v.
@@ -176,6 +182,43 @@ int oc_get_stats(oc_engine *h, int32_t r, oc_stats *out);
 
 /* Current step counter of replica r. */
 int oc_get_step(oc_engine *h, int32_t r, int64_t *step);
+
+/* Smooth radial distribution of the type pair (ta, tb): nbins bins on
+ * [0, rmax), each pair spread over the bins by a triweight kernel
+ * 35/(32 w) (1 - (u/w)^2)^3 of half-width w = width. */
+typedef struct oc_rdf_spec {
+    int32_t ta, tb, nbins, pad;
+    double rmax, width;
+} oc_rdf_spec;
+
+/* Replace replica r's repulsion matrix (8x8 row-major, symmetric) and gamma;
+ * sigma = sqrt(2 gamma kT) follows.  No counterpart in the reference (its
+ * parameters are fixed for a run). */
+int oc_set_pair_params(oc_engine *h, int32_t r, const double *a64, double gamma);
+
+/* Forward half of the window gradient: recompute f at the current state, run
+ * nsteps steps of every replica (checkpoint every ckpt_every steps) and return
+ * the smooth RDF of spec averaged over steps 1..nsteps, gbar[r*nbins + b].
+ * The engine ends at the window's last step.  Needs k_angle = 0, chemistry,
+ * stirring and the kT cycle off, and the warm-up ramp over. */
+int oc_window_forward(oc_engine *h, int32_t nsteps, int32_t ckpt_every, const oc_rdf_spec *spec, double *gbar);
+
+/* Reverse half: given dL/dgbar (nrep * nbins) of any loss of the last forward
+ * window, grad[r*7 + q] = dL/dtheta_q of replica r over q = (a_WW, a_WH,
+ * a_WT, a_HH, a_HT, a_TT, gamma), the noise stream held fixed.  Recomputes the
+ * window from its checkpoints; the engine stays at the window's last step. */
+int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad);
+
+/* Window gradient of every replica (no counterpart in the reference): with the
+ * noise stream fixed and chemistry off, recompute f at the current state, run
+ * nsteps steps (checkpoint every ckpt_every steps, recomputed in the reverse
+ * sweep), average the smooth RDF of spec over steps 1..nsteps into gbar, and
+ * return loss_r = sum_b (gbar_rb - target_rb)^2 and its gradient
+ * grad[r*7 + q] over q = (a_WW, a_WH, a_WT, a_HH, a_HT, a_TT, gamma).
+ * target_stride = nbins for per-replica targets, 0 for one shared target.
+ * The engine ends at the window's last step.  Needs k_angle = 0. */
+int oc_window_grad(oc_engine *h, int32_t nsteps, int32_t ckpt_every, const oc_rdf_spec *spec, const double *target,
+                   int32_t target_stride, double *loss, double *grad, double *gbar);
 
 /* Launch only the pair-force kernel reps times on the handle's stream, for
  * the current state and step, without synchronising: lets a caller time that

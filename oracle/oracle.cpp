@@ -827,9 +827,9 @@ OR_API int64_t or_get_step(OrSys *s) { return s->step; }
  * Window: start state (x0, vh0, v~0) loaded in the system; f0 = F(x0, v~0;
  * theta) is recomputed, then nsteps steps of the Groot-Warren update
  * (or_integrate + sort + forces; chemistry must be off).
- * RDF of type pair (ta, tb) on nbins bins of [0, rmax): a Gaussian kernel of
- * width `width` per pair, cut at 4 widths,
- *   g_b(x) = C_b sum_{i<j} phi(r_ij - c_b),  C_b = dr * norm / (pi (r_{b+1}^2 - r_b^2)),
+ * RDF of type pair (ta, tb) on nbins bins of [0, rmax): a triweight kernel
+ * K of half-width `width` per pair,
+ *   g_b(x) = C_b sum_{i<j} K(r_ij - c_b),  C_b = dr * norm / (pi (r_{b+1}^2 - r_b^2)),
  *   norm = 2 / (n_a rho_a) for ta == tb, 1 / (n_a rho_b) otherwise,
  * averaged over steps 1..nsteps; loss = sum_b (gbar_b - target_b)^2.
  * Reverse sweep (f_n = F(x_n, v~_n)):
@@ -863,12 +863,14 @@ static void rdf_norm(const RdfSpec &sp, int na, int nb, double L, std::vector<do
     }
 }
 
-/* all-pairs smooth RDF (and, with gw != null, X += dL/dx for bin weights gw) */
+/* all-pairs smooth RDF (and, with gw != null, X += dL/dx for bin weights gw).
+ * Bin kernel: triweight K(u) = 35/(32 h) (1 - (u/h)^2)^3 on |u| < h
+ * (h = spec width; integrates to 1, twice differentiable at its edge). */
 static void rdf_pass(const OrSys *s, const RdfSpec &sp, const std::vector<double> &C, double *g, const double *gw,
                      double *X) {
     const int N = s->N;
-    const double L = s->p.L, dr = sp.rmax / sp.nbins, w = sp.width, cut = sp.rmax + 4.0 * w;
-    const double pref = 1.0 / (sqrt(2.0 * M_PI) * w);
+    const double L = s->p.L, dr = sp.rmax / sp.nbins, h = sp.width, cut = sp.rmax + h;
+    const double pref = 35.0 / (32.0 * h);
     for (int i = 0; i < N; i++) {
         const int ti = s->type[i];
         if (ti != sp.ta && ti != sp.tb) continue;
@@ -880,15 +882,15 @@ static void rdf_pass(const OrSys *s, const RdfSpec &sp, const std::vector<double
             const double dx = minimg(s->x[i] - s->x[j], L), dy = minimg(s->y[i] - s->y[j], L);
             const double r = sqrt(dx * dx + dy * dy);
             if (r >= cut || r == 0.0) continue;
-            const int b0 = std::max(0, (int)floor((r - 4.0 * w) / dr)), b1 = std::min(sp.nbins - 1, (int)floor((r + 4.0 * w) / dr));
+            const int b0 = std::max(0, (int)floor((r - h) / dr)), b1 = std::min(sp.nbins - 1, (int)floor((r + h) / dr));
             double dsum = 0.0;
             for (int b = b0; b <= b1; b++) {
-                const double u = r - (b + 0.5) * dr;
-                if (fabs(u) > 4.0 * w) continue;
-                const double ph = pref * exp(-0.5 * u * u / (w * w));
+                const double t = (r - (b + 0.5) * dr) / h;
+                if (fabs(t) >= 1.0) continue;
+                const double q = 1.0 - t * t;
                 /* each unordered pair once for g (ta == tb: i < j) */
-                if (g && (sp.ta != sp.tb ? (ti == sp.ta) : (i < j))) g[b] += C[b] * ph;
-                if (gw) dsum += gw[b] * C[b] * (-u / (w * w)) * ph;
+                if (g && (sp.ta != sp.tb ? (ti == sp.ta) : (i < j))) g[b] += C[b] * pref * q * q * q;
+                if (gw) dsum += gw[b] * C[b] * pref * 3.0 * q * q * (-2.0 * t / h);
             }
             if (gw) {
                 /* d/dx_i of the pair's terms (counted once per unordered pair) */

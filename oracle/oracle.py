@@ -28,14 +28,14 @@ vp = C.c_void_p
 
 
 class RdfSpec(C.Structure):
-    """Smooth RDF of the type pair (ta, tb): nbins bins on [0, rmax), Gaussian
-    kernel of the given width per pair."""
+    """Smooth RDF of the type pair (ta, tb): nbins bins on [0, rmax), triweight
+    kernel of the given half-width per pair."""
     _fields_ = [("ta", C.c_int32), ("tb", C.c_int32), ("nbins", C.c_int32), ("pad", C.c_int32),
                 ("rmax", C.c_double), ("width", C.c_double)]
 
 
 def rdf_spec(ta=2, tb=2, nbins=64, rmax=3.0, width=None):
-    return RdfSpec(ta, tb, nbins, 0, rmax, width if width is not None else 0.5 * rmax / nbins)
+    return RdfSpec(ta, tb, nbins, 0, rmax, width if width is not None else rmax / nbins)
 dp = P(C.c_double)
 _OR = None
 _REF = None

@@ -60,6 +60,17 @@ class oc_stats(C.Structure):
     ]
 
 
+class oc_rdf_spec(C.Structure):
+    _fields_ = [("ta", C.c_int32), ("tb", C.c_int32), ("nbins", C.c_int32), ("pad", C.c_int32),
+                ("rmax", C.c_double), ("width", C.c_double)]
+
+
+def rdf_spec(ta=2, tb=2, nbins=64, rmax=3.0, width=None):
+    """Smooth RDF of the type pair (ta, tb) on nbins bins of [0, rmax); the
+    triweight kernel's half-width defaults to one bin."""
+    return oc_rdf_spec(ta, tb, nbins, 0, rmax, width if width is not None else rmax / nbins)
+
+
 EVENT_DTYPE = None  # numpy dtype, filled lazily
 
 
@@ -98,6 +109,7 @@ def _declare(lib):
         "oc_create": (C.c_int, [P(oc_params), P(C.c_int32), C.c_int32, P(vp)]),
         "oc_destroy": (None, [vp]),
         "oc_set_tuning": (C.c_int, [vp, C.c_int32, C.c_int32]),
+        "oc_set_pair_variant": (C.c_int, [vp, C.c_int32]),
         "oc_soup_host": (C.c_int, [P(oc_params), C.c_int32, P(oc_state)]),
         "oc_start": (C.c_int, [vp, P(oc_state)]),
         "oc_set_state": (C.c_int, [vp, C.c_int32, P(oc_state), C.c_int64]),
@@ -113,6 +125,11 @@ def _declare(lib):
         "oc_stream": (vp, [vp]),
         "oc_launch_pair": (C.c_int, [vp, C.c_int32]),
         "oc_abi_sizes": (C.c_int, [P(C.c_int32), C.c_int32]),
+        "oc_set_pair_params": (C.c_int, [vp, C.c_int32, P(C.c_double), C.c_double]),
+        "oc_window_forward": (C.c_int, [vp, C.c_int32, C.c_int32, P(oc_rdf_spec), P(C.c_double)]),
+        "oc_window_backward": (C.c_int, [vp, P(C.c_double), P(C.c_double)]),
+        "oc_window_grad": (C.c_int, [vp, C.c_int32, C.c_int32, P(oc_rdf_spec), P(C.c_double), C.c_int32,
+                                     P(C.c_double), P(C.c_double), P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

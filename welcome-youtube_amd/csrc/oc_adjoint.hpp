@@ -1,0 +1,318 @@
+// oc_adjoint.hpp -- gfx950 kernels of the window gradient (no counterpart in
+// the reference): d loss / d (a_WW, a_WH, a_WT, a_HH, a_HT, a_TT, gamma) for a
+// smooth radial-distribution loss, the noise stream held fixed.
+//
+// Reverse sweep over the Groot-Warren step (f_n = F(x_n, v~_n)):
+//   X_n += Jx^T F_n ; T_n = Jv^T F_n ; theta_bar += Jtheta^T F_n
+//   X_{n-1} = X_n + dL/dx_{n-1} ; V_{n-1} = dt X_n + T_n + V_n ;
+//   F_{n-1} = dt^2 X_n + (1/2 + lambda) dt T_n + dt V_n
+// carried back through each step's sort permutation.  Pair arithmetic is FP32
+// (as the forward), per-bead sums and the parameter gradients FP64.
+#pragma once
+#include "oc_kernels.hpp"
+
+namespace oc {
+
+struct RdfArgs {
+    int32_t ta, tb, nbins;
+    float rmax, width, dr, cut;
+    int32_t Rc;  // columns searched on each side
+};
+
+constexpr double kFix = 4294967296.0;  // fixed-point scale of the RDF histogram (2^32)
+
+// Visit the (ta, tb) partners of bead i within `cut`, window of +-Rc columns.
+template <typename Fn>
+__device__ __forceinline__ void visit_wide(const RepInfo &R, const int32_t *cs, const double2 *xy, int i, double2 p,
+                                           float cut, int Rc, Fn fn) {
+    const int cx = (int)(p.x * R.icsx) >= R.ncx ? R.ncx - 1 : (int)(p.x * R.icsx);
+    const int khi = (int)((p.y + cut) * R.icsy);
+    const int klo = (int)((p.y - cut) * R.icsy + 4 * R.ncy) - 4 * R.ncy;
+    for (int dc = -Rc; dc <= Rc; dc++) {
+        int col = cx + dc;
+        double sx = 0.0;
+        if (col < 0) { col += R.ncx; sx = -R.L; }
+        else if (col >= R.ncx) { col -= R.ncx; sx = R.L; }
+        for (int k = klo; k <= khi; k++) {
+            const int kk = k < 0 ? k + R.ncy : (k >= R.ncy ? k - R.ncy : k);
+            const double sy = k < 0 ? -R.L : (k >= R.ncy ? R.L : 0.0);
+            const int c = R.coff + col * R.ncy + kk;
+            for (int j = cs[c]; j < cs[c + 1]; j++) {
+                if (j == i) continue;
+                const double2 q = xy[j];
+                fn(j, (float)((p.x - q.x) - sx), (float)((p.y - q.y) - sy));
+            }
+        }
+    }
+}
+
+// Smooth RDF histogram of the current state: per block an LDS fixed-point
+// histogram (integer atomics: order independent), written as a partial.
+__global__ __launch_bounds__(kBeadBlock) void k_rdf(const RepInfo *__restrict__ ri, const int32_t *__restrict__ bblk,
+                                                    const int32_t *__restrict__ cs, const double2 *__restrict__ xy,
+                                                    const uint32_t *__restrict__ gt, RdfArgs A,
+                                                    unsigned long long *__restrict__ part) {
+    __shared__ unsigned long long h[256];
+    for (int b = threadIdx.x; b < A.nbins; b += blockDim.x) h[b] = 0ull;
+    __syncthreads();
+    const int r = bblk[blockIdx.x];
+    const RepInfo &R = ri[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    if (loc < R.N) {
+        const int i = R.off + loc;
+        const uint32_t gti = gt[i], ti = gti >> kGidBits, gi = gti & kGidMask;
+        if ((int)ti == A.ta || (int)ti == A.tb) {
+            const float pref = 35.0f / (32.0f * A.width), iw = 1.0f / A.width;
+            visit_wide(R, cs, xy, i, xy[i], A.cut, A.Rc, [&](int j, float dx, float dy) {
+                const uint32_t gtj = gt[j], tj = gtj >> kGidBits;
+                bool own;
+                if (A.ta == A.tb) own = (int)tj == A.ta && gi < (gtj & kGidMask);
+                else own = (int)ti == A.ta && (int)tj == A.tb;
+                if (!own) return;
+                const float r2 = dx * dx + dy * dy;
+                if (r2 >= A.cut * A.cut || r2 == 0.0f) return;
+                const float rr = sqrtf(r2);
+                const int b0 = max(0, (int)floorf((rr - A.width) / A.dr));
+                const int b1 = min(A.nbins - 1, (int)floorf((rr + A.width) / A.dr));
+                for (int b = b0; b <= b1; b++) {
+                    const float t = (rr - (b + 0.5f) * A.dr) * iw;
+                    if (fabsf(t) >= 1.0f) continue;
+                    const float q = 1.0f - t * t;
+                    atomicAdd(&h[b], (unsigned long long)((double)(pref * q * q * q) * kFix + 0.5));
+                }
+            });
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < A.nbins; b += blockDim.x) part[(size_t)blockIdx.x * A.nbins + b] = h[b];
+}
+
+// window accumulator += sum of the block partials of each replica
+__global__ void k_rdf_reduce(const RepInfo *__restrict__ ri, int nbins, const unsigned long long *__restrict__ part,
+                             unsigned long long *__restrict__ acc) {
+    const int r = blockIdx.x;
+    const RepInfo &R = ri[r];
+    const int nb = (R.N + kBeadBlock - 1) / kBeadBlock;
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+        unsigned long long s = 0;
+        for (int k = 0; k < nb; k++) s += part[(size_t)(R.bblk0 + k) * nbins + b];
+        acc[(size_t)r * nbins + b] += s;
+    }
+}
+
+// X_i += d loss / d x_i of the RDF at the current state; gw[r][b] = dL/dg_b * C_b
+__global__ __launch_bounds__(kBeadBlock) void k_rdf_adj(const RepInfo *__restrict__ ri,
+                                                        const int32_t *__restrict__ bblk,
+                                                        const int32_t *__restrict__ cs, const double2 *__restrict__ xy,
+                                                        const uint32_t *__restrict__ gt, RdfArgs A,
+                                                        const double *__restrict__ gw, double2 *__restrict__ X) {
+    __shared__ float w_s[256];
+    const int r = bblk[blockIdx.x];
+    for (int b = threadIdx.x; b < A.nbins; b += blockDim.x) w_s[b] = (float)gw[(size_t)r * A.nbins + b];
+    __syncthreads();
+    const RepInfo &R = ri[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    if (loc >= R.N) return;
+    const int i = R.off + loc;
+    const uint32_t ti = gt[i] >> kGidBits;
+    if ((int)ti != A.ta && (int)ti != A.tb) return;
+    const float pref = 35.0f / (32.0f * A.width), iw = 1.0f / A.width;
+    double ax = 0.0, ay = 0.0;
+    visit_wide(R, cs, xy, i, xy[i], A.cut, A.Rc, [&](int j, float dx, float dy) {
+        const uint32_t tj = gt[j] >> kGidBits;
+        const bool pr = ((int)ti == A.ta && (int)tj == A.tb) || ((int)ti == A.tb && (int)tj == A.ta);
+        if (!pr) return;
+        const float r2 = dx * dx + dy * dy;
+        if (r2 >= A.cut * A.cut || r2 == 0.0f) return;
+        const float rr = sqrtf(r2);
+        const int b0 = max(0, (int)floorf((rr - A.width) / A.dr));
+        const int b1 = min(A.nbins - 1, (int)floorf((rr + A.width) / A.dr));
+        float ds = 0.0f;
+        for (int b = b0; b <= b1; b++) {
+            const float t = (rr - (b + 0.5f) * A.dr) * iw;
+            if (fabsf(t) >= 1.0f) continue;
+            const float q = 1.0f - t * t;
+            ds += w_s[b] * pref * 3.0f * q * q * (-2.0f * t * iw);
+        }
+        ax += (double)(ds * dx / rr);
+        ay += (double)(ds * dy / rr);
+    });
+    X[i].x += ax;
+    X[i].y += ay;
+}
+
+__device__ __forceinline__ int theta_index(uint32_t a, uint32_t b) {
+    if (a > b) { const uint32_t t = a; a = b; b = t; }
+    if (b > 2) return -1;
+    return a == 0 ? (int)b : (a == 1 ? 2 + (int)b : 5);
+}
+
+// J^T products of the pair forces at state n for the force adjoint Fa:
+// X += Jx^T Fa (FP64 sums), T = Jv^T Fa, per-gid theta gradient += (half of
+// each pair: every pair is visited from both beads).
+__global__ __launch_bounds__(kBeadBlock) void k_pair_adj(const RepInfo *__restrict__ ri,
+                                                         const RepStep *__restrict__ rs,
+                                                         const int32_t *__restrict__ bblk,
+                                                         const int32_t *__restrict__ cs,
+                                                         const double2 *__restrict__ xy, const float2 *__restrict__ vt,
+                                                         const uint32_t *__restrict__ gt,
+                                                         const double2 *__restrict__ Fa, double2 *__restrict__ X,
+                                                         double2 *__restrict__ T, double *__restrict__ thb) {
+    const int r = bblk[blockIdx.x];
+    const RepInfo &R = ri[r];
+    const RepStep &S = rs[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    if (loc >= R.N) return;
+    const int i = R.off + loc;
+    const double2 pi = xy[i];
+    const float2 vi = vt[i];
+    const uint32_t gti = gt[i], ti = gti >> kGidBits, gi = gti & kGidMask;
+    const double2 Fi = Fa[i];
+    const float gam = S.gamma, sdt = S.sdt, sdg = S.sdt / (2.0f * S.gamma);
+    const int ncy = R.ncy, ncx = R.ncx;
+    const int cx = (int)(pi.x * R.icsx) >= ncx ? ncx - 1 : (int)(pi.x * R.icsx);
+    const int khi = (int)((pi.y + 1.0) * R.icsy);
+    const int klo = (int)((pi.y - 1.0) * R.icsy + ncy) - ncy;
+    double xa = 0.0, ya = 0.0, ta = 0.0, tb = 0.0;
+    double th[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int dc = -1; dc <= 1; dc++) {
+        int col = cx + dc;
+        double sx = 0.0;
+        if (col < 0) { col += ncx; sx = -R.L; }
+        else if (col >= ncx) { col -= ncx; sx = R.L; }
+        for (int k = klo; k <= khi; k++) {
+            const int kk = k < 0 ? k + ncy : (k >= ncy ? k - ncy : k);
+            const double sy = k < 0 ? -R.L : (k >= ncy ? R.L : 0.0);
+            const int c = R.coff + col * ncy + kk;
+            for (int j = cs[c]; j < cs[c + 1]; j++) {
+                const double2 pj = xy[j];
+                const float dx = (float)((pi.x - pj.x) - sx), dy = (float)((pi.y - pj.y) - sy);
+                const float r2 = dx * dx + dy * dy;
+                if (!(r2 < 1.0f && r2 > 1e-24f)) continue;
+                const float ir = __builtin_amdgcn_rsqf(r2);
+                const float rr = r2 * ir, w = 1.0f - rr;
+                const float ex = dx * ir, ey = dy * ir;
+                const float2 vj = vt[j];
+                const float ux = vi.x - vj.x, uy = vi.y - vj.y;
+                const float eu = ex * ux + ey * uy;
+                const uint32_t gtj = gt[j], tj = gtj >> kGidBits;
+                const uint64_t hsh = pair_hash(S.sseed, gi, gtj & kGidMask);
+                const float thn = (float)(int32_t)(uint32_t)(hsh >> 32) * kTheta32;
+                const float a = S.a_eff[8 * ti + tj];
+                const float m = w * (a - gam * w * eu + sdt * thn);
+                const float dmr = -(a + sdt * thn) + 2.0f * gam * w * eu;
+                const float dmu = -gam * w * w;
+                const double2 Fj = Fa[j];
+                const float lx = (float)(Fi.x - Fj.x), ly = (float)(Fi.y - Fj.y);
+                const float el = ex * lx + ey * ly;
+                const float pux = ux - eu * ex, puy = uy - eu * ey;
+                const float plx = lx - el * ex, ply = ly - el * ey;
+                xa += (double)((dmr * ex + dmu * pux * ir) * el + m * plx * ir);
+                ya += (double)((dmr * ey + dmu * puy * ir) * el + m * ply * ir);
+                ta += (double)(dmu * el * ex);
+                tb += (double)(dmu * el * ey);
+                const double ha = (double)(0.5f * w * el);
+                const int q = theta_index(ti, tj);
+                if (q == 0) th[0] += ha;
+                else if (q == 1) th[1] += ha;
+                else if (q == 2) th[2] += ha;
+                else if (q == 3) th[3] += ha;
+                else if (q == 4) th[4] += ha;
+                else if (q == 5) th[5] += ha;
+                th[6] += (double)(0.5f * el * (-w * w * eu + sdg * w * thn));
+            }
+        }
+    }
+    X[i].x += xa;
+    X[i].y += ya;
+    T[i] = make_double2(ta, tb);
+    double *o = thb + (size_t)(R.off + gi) * 8;
+    for (int q = 0; q < 7; q++) o[q] += th[q];
+}
+
+// bond terms of the force adjoint (FP64): lipid chains from their head, a
+// chemistry bond from its smaller-gid bead
+__global__ __launch_bounds__(kBeadBlock) void k_bond_adj(const RepInfo *__restrict__ ri,
+                                                         const int32_t *__restrict__ bblk,
+                                                         const double2 *__restrict__ xy,
+                                                         const uint32_t *__restrict__ gt,
+                                                         const int8_t *__restrict__ mp, const int32_t *__restrict__ lnk,
+                                                         const double2 *__restrict__ Fa, double2 *__restrict__ X) {
+    const int r = bblk[blockIdx.x];
+    const RepInfo &R = ri[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    if (loc >= R.N) return;
+    const int s = R.off + loc;
+    auto adj = [&](int i, int j) {
+        const double2 p = xy[i], q = xy[j];
+        const double dx = minimg(q.x - p.x, R.L, R.halfL), dy = minimg(q.y - p.y, R.L, R.halfL);
+        const double rr = sqrt(dx * dx + dy * dy);
+        const double g = -R.kb * (rr - R.r0) / rr, gp = -R.kb * R.r0 / (rr * rr);
+        const double ex = dx / rr, ey = dy / rr;
+        const double lx = Fa[j].x - Fa[i].x, ly = Fa[j].y - Fa[i].y;
+        const double el = ex * lx + ey * ly;
+        const double bx = g * lx + gp * rr * el * ex, by = g * ly + gp * rr * el * ey;
+        X[j].x += bx;
+        X[j].y += by;
+        X[i].x -= bx;
+        X[i].y -= by;
+    };
+    const int m = mp[s];
+    if (m == 0) {
+        int prev = s, n = 1;
+        for (int d = lnk[s]; d >= 0 && n < R.mlen; d = lnk[d], n++) {
+            adj(prev, d);
+            prev = d;
+        }
+    } else if (m < 0) {
+        const int p = lnk[s];
+        if (p >= 0 && (gt[s] & kGidMask) < (gt[p] & kGidMask)) adj(s, p);
+    }
+}
+
+// one reverse step: new adjoints in the order of step n, scattered to the
+// order of step n-1 through the sort map src (new slot d <- old slot src[d])
+__global__ __launch_bounds__(kBeadBlock) void k_adj_step(const RepInfo *__restrict__ ri,
+                                                         const int32_t *__restrict__ bblk,
+                                                         const int32_t *__restrict__ src,
+                                                         const double2 *__restrict__ X, const double2 *__restrict__ V,
+                                                         const double2 *__restrict__ F, const double2 *__restrict__ T,
+                                                         double2 *__restrict__ Xo, double2 *__restrict__ Vo,
+                                                         double2 *__restrict__ Fo) {
+    const int r = bblk[blockIdx.x];
+    const RepInfo &R = ri[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    if (loc >= R.N) return;
+    const int d = R.off + loc;
+    const int o = src[d];
+    const double2 x = X[d], v = V[d], f = F[d], t = T[d];
+    Xo[o] = x;
+    Vo[o] = make_double2(R.dt * x.x + t.x + v.x, R.dt * x.y + t.y + v.y);
+    Fo[o] = make_double2(R.dt2 * x.x + R.cvt * t.x + R.dt * v.x, R.dt2 * x.y + R.cvt * t.y + R.dt * v.y);
+}
+
+// per-replica sum of the per-gid theta gradients (fixed partition, FP64)
+__global__ __launch_bounds__(256) void k_theta_reduce(const RepInfo *__restrict__ ri, const double *__restrict__ thb,
+                                                      double *__restrict__ out) {
+    __shared__ double red[256];
+    const RepInfo &R = ri[blockIdx.x];
+    for (int q = 0; q < 7; q++) {
+        double s = 0.0;
+        for (int g = threadIdx.x; g < R.N; g += 256) s += thb[(size_t)(R.off + g) * 8 + q];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[(size_t)blockIdx.x * 8 + q] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_f64(double *p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+}  // namespace oc

@@ -7,7 +7,7 @@
 #include <string>
 #include <vector>
 
-#include "oc_kernels.hpp"
+#include "oc_adjoint.hpp"
 
 using namespace oc;
 
@@ -34,6 +34,7 @@ struct oc_engine {
     std::vector<int32_t> bblk, cblk, rblk;
     std::vector<Tile> tiles;
     int pair_block = 256, tile_beads = 0;
+    bool pair_lists = false;
     int per_xcd = 0;
     // device
     RepInfo *d_ri = nullptr;
@@ -45,6 +46,7 @@ struct oc_engine {
     double2 *d_f = nullptr;
     int32_t *d_key = nullptr, *d_src = nullptr, *d_dest = nullptr, *d_count = nullptr, *d_cs[2] = {nullptr, nullptr};
     int32_t *d_bsum = nullptr;
+    int2 *d_bmax = nullptr;
     int nscan = 0;
     double *d_stir = nullptr, *d_epot = nullptr, *d_ebond = nullptr, *d_part = nullptr, *d_statout = nullptr;
     uint8_t *d_flag = nullptr;
@@ -61,6 +63,7 @@ struct oc_engine {
     std::vector<std::vector<oc_event>> evq;
     std::vector<void *> allocs;
     bool events_dropped = false;
+    struct Adj *adj = nullptr;
 };
 
 template <class T>
@@ -132,14 +135,14 @@ static int make_repinfo(const oc_params &p, int32_t N, RepInfo &R) {
 // pair tiles: rows per tile from the expected beads per row
 static int build_tiles(oc_engine *h) {
     h->tiles.clear();
-    const int target = h->tile_beads > 0 ? h->tile_beads : (h->pair_block * 4) / 5;
+    const int target = h->tile_beads > 0 ? h->tile_beads : (h->pair_block * 9) / 10;
     for (int r = 0; r < h->nrep; r++) {
         const RepInfo &R = h->ri[r];
         const double per_row = (double)R.N / ((double)R.ncx * R.ncy);
         const int M = (int)ceil(R.icsy) + 1;
         int rows = (int)lround(target / std::max(per_row, 1e-9));
         rows = std::max(1, std::min(rows, R.ncy));
-        rows = std::min(rows, 512 - 2 * M - 1);
+        rows = std::min(rows, pair_max_rows(h->pair_block) - 2 * M - 1);
         if (rows < 1) return fail(OC_EINVAL, "nsub too large for the pair tile buffer");
         const int nt = (R.ncy + rows - 1) / rows;
         for (int cx = 0; cx < R.ncx; cx++)
@@ -191,21 +194,26 @@ static void launch_pair(oc_engine *h, bool energy) {
     A.epot = h->d_epot;
     A.err = h->d_err;
     const dim3 grid(8 * h->per_xcd);
-    // CAP (staged candidates) is 4x the block; the lists hold 24 pairs per lane
+    // CAP (staged candidates) is 3.5x the block; `lists` picks the two-phase
+    // variant with 16 listed pairs per lane, or the single pass
+    const bool L2 = h->pair_lists;
+#define OC_PAIR(B, C, CL)                                                                        \
+    do {                                                                                         \
+        if (energy) k_pair<B, C, CL, true><<<grid, B, 0, h->stream>>>(A);                         \
+        else k_pair<B, C, CL, false><<<grid, B, 0, h->stream>>>(A);                               \
+    } while (0)
     switch (h->pair_block) {
     case 64:
-        if (energy) k_pair<64, 256, 24, true><<<grid, 64, 0, h->stream>>>(A);
-        else k_pair<64, 256, 24, false><<<grid, 64, 0, h->stream>>>(A);
+        if (L2) OC_PAIR(64, 224, 16); else OC_PAIR(64, 224, 0);
         break;
     case 128:
-        if (energy) k_pair<128, 512, 24, true><<<grid, 128, 0, h->stream>>>(A);
-        else k_pair<128, 512, 24, false><<<grid, 128, 0, h->stream>>>(A);
+        if (L2) OC_PAIR(128, 448, 16); else OC_PAIR(128, 448, 0);
         break;
     default:
-        if (energy) k_pair<256, 1024, 24, true><<<grid, 256, 0, h->stream>>>(A);
-        else k_pair<256, 1024, 24, false><<<grid, 256, 0, h->stream>>>(A);
+        if (L2) OC_PAIR(256, 896, 16); else OC_PAIR(256, 896, 0);
         break;
     }
+#undef OC_PAIR
 }
 
 static void launch_stir(oc_engine *h) {
@@ -242,13 +250,13 @@ static void launch_sort(oc_engine *h, bool integ) {
     const int c = h->cur, n = c ^ 1;
     const int nb = (int)h->bblk.size();
     if (integ)
-        k_keys<true><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy, h->st[c].v, h->d_f,
-                                                       h->d_key, h->d_count);
+        k_keys<true><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->st[c].xy, h->st[c].v, h->d_f, h->d_key,
+                                                       h->d_count, h->d_bmax);
     else
-        k_keys<false><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy, h->st[c].v, h->d_f,
-                                                        h->d_key, h->d_count);
+        k_keys<false><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->st[c].xy, h->st[c].v, h->d_f,
+                                                        h->d_key, h->d_count, h->d_bmax);
     k_scan_reduce<<<h->nscan, 256, 0, h->stream>>>(h->d_count, h->Ctot, h->d_bsum);
-    k_scan_top<<<1, 1024, 0, h->stream>>>(h->d_bsum, h->nscan);
+    k_scan_top<<<1, 1024, 0, h->stream>>>(h->d_bsum, h->nscan, h->d_ri, h->d_rs, h->nrep, h->d_bmax);
     k_scan_down<<<h->nscan, 256, 0, h->stream>>>(h->d_count, h->Ctot, h->d_bsum, h->d_cs[n]);
     k_gather<<<(int)h->cblk.size(), kCellBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_cblk, h->d_key, h->d_cs[c],
                                                                h->d_cs[n], h->d_src, h->d_dest, h->d_err);
@@ -262,7 +270,7 @@ static void launch_sort(oc_engine *h, bool integ) {
 }
 
 static void launch_step_params(oc_engine *h, int64_t delta, int64_t set_step) {
-    k_step_params<<<h->nrep, 64, 0, h->stream>>>(h->d_ri, h->d_rs, delta, set_step);
+    k_step_params<<<h->nrep, 64, 0, h->stream>>>(h->d_ri, h->d_rs, delta, set_step, nullptr);
 }
 
 // one full step of every replica: cel/engine/oercel.c:2300-2362
@@ -401,8 +409,11 @@ OC_API int oc_soup_host(const oc_params *p, int32_t N, oc_state *o) {
 OC_API const char *oc_last_error(void) { return g_err.c_str(); }
 OC_API int oc_abi_version(void) { return OC_ABI_VERSION; }
 
+static void adj_free(oc_engine *h);
+
 OC_API void oc_destroy(oc_engine *h) {
     if (!h) return;
+    adj_free(h);
     drop_graphs(h);
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -469,6 +480,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     rc = rc ? rc : dalloc(h, &h->d_dest, Nt);
     rc = rc ? rc : dalloc(h, &h->d_count, Ct);
     rc = rc ? rc : dalloc(h, &h->d_bsum, h->nscan);
+    rc = rc ? rc : dalloc(h, &h->d_bmax, h->bblk.size());
     rc = rc ? rc : dalloc(h, &h->d_epot, Nt);
     rc = rc ? rc : dalloc(h, &h->d_ebond, Nt);
     rc = rc ? rc : dalloc(h, &h->d_part, h->rblk.size() * 16);
@@ -526,6 +538,13 @@ OC_API int oc_set_tuning(oc_engine *h, int32_t pair_block, int32_t tile_beads) {
 }
 
 OC_API void *oc_stream(oc_engine *h) { return h ? (void *)h->stream : nullptr; }
+
+OC_API int oc_set_pair_variant(oc_engine *h, int32_t lists) {
+    if (!h) return fail(OC_EINVAL, "null engine");
+    h->pair_lists = lists != 0;
+    drop_graphs(h);
+    return 0;
+}
 
 OC_API int oc_abi_sizes(int32_t *out, int32_t n) {
     const int32_t sz[4] = {(int32_t)sizeof(oc_params), (int32_t)sizeof(oc_state), (int32_t)sizeof(oc_event),
@@ -854,4 +873,324 @@ OC_API int oc_get_stats(oc_engine *h, int32_t r, oc_stats *out) {
     rs.maxbond_bits = 0;
     HIPCHK(hipMemcpy(h->d_rs + r, &rs, sizeof rs, hipMemcpyHostToDevice));
     return 0;
+}
+
+// --------------------------------------------------------------------------- window gradient
+
+OC_API int oc_set_pair_params(oc_engine *h, int32_t r, const double *a64, double gamma) {
+    if (!h || !a64 || r < 0 || r >= h->nrep) return fail(OC_EINVAL, "bad arguments");
+    if (!(gamma > 0.0)) return fail(OC_EINVAL, "gamma must be > 0");
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++) {
+            const double v = (i < OC_NTYPES && j < OC_NTYPES) ? a64[8 * i + j] : 0.0;
+            if (v != ((i < OC_NTYPES && j < OC_NTYPES) ? a64[8 * j + i] : 0.0))
+                return fail(OC_EINVAL, "repulsion matrix must be symmetric");
+            h->prm[r].a[i][j] = v;
+            h->ri[r].a[8 * i + j] = v;
+        }
+    h->prm[r].gamma = gamma;
+    h->ri[r].gamma = gamma;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(h->d_ri + r, &h->ri[r], sizeof(RepInfo), hipMemcpyHostToDevice));
+    // the step parameters (a_eff, sigma) of the current step follow at once
+    launch_step_params(h, 0, -1);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+struct StateSnap {
+    double2 *xy = nullptr, *v = nullptr, *f = nullptr;
+    float2 *vt = nullptr;
+    uint32_t *gt = nullptr;
+    int8_t *mp = nullptr;
+    int32_t *lnk = nullptr, *cs = nullptr, *src = nullptr;
+    RepStep *rs = nullptr;
+};
+
+struct Adj {
+    int K = 0, nseg = 0, nbins = 0;
+    std::vector<StateSnap> tape, ckpt;
+    StateSnap end;
+    double2 *X[2] = {nullptr, nullptr}, *V[2] = {nullptr, nullptr}, *F[2] = {nullptr, nullptr}, *T = nullptr;
+    double *thb = nullptr, *gw = nullptr, *grad = nullptr;
+    unsigned long long *rpart = nullptr, *racc = nullptr;
+    int64_t *base = nullptr;
+    std::vector<void *> mem;
+    // the last forward window
+    bool ready = false;
+    int nsteps = 0;
+    RdfArgs RA;
+    std::vector<double> Cb;
+};
+
+static void adj_free(oc_engine *h) {
+    if (!h->adj) return;
+    for (void *p : h->adj->mem) (void)hipFree(p);
+    delete h->adj;
+    h->adj = nullptr;
+}
+
+template <class T>
+static int aalloc(Adj *a, T **p, size_t n) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return fail(OC_ENOMEM, std::string("hipMalloc failed (window gradient): ") + hipGetErrorString(e));
+    a->mem.push_back(q);
+    *p = (T *)q;
+    return 0;
+}
+
+static int snap_alloc(oc_engine *h, Adj *a, StateSnap &s, bool full, bool with_src) {
+    const size_t N = h->Ntot, C = h->Ctot + 1;
+    int rc = 0;
+    rc = rc ? rc : aalloc(a, &s.xy, N);
+    rc = rc ? rc : aalloc(a, &s.vt, N);
+    rc = rc ? rc : aalloc(a, &s.gt, N);
+    rc = rc ? rc : aalloc(a, &s.mp, N);
+    rc = rc ? rc : aalloc(a, &s.lnk, N);
+    rc = rc ? rc : aalloc(a, &s.cs, C);
+    if (with_src) rc = rc ? rc : aalloc(a, &s.src, N);
+    if (full) {
+        rc = rc ? rc : aalloc(a, &s.v, N);
+        rc = rc ? rc : aalloc(a, &s.f, N);
+        rc = rc ? rc : aalloc(a, &s.rs, h->nrep);
+    }
+    return rc;
+}
+
+// copy the engine's current state into (save) or out of (restore) a snapshot
+static int snap_copy(oc_engine *h, StateSnap &s, bool save, bool with_src) {
+    const size_t N = h->Ntot, C = h->Ctot + 1;
+    const int c = h->cur;
+    auto cp = [&](void *eng, void *snp, size_t bytes) -> hipError_t {
+        return save ? hipMemcpyAsync(snp, eng, bytes, hipMemcpyDeviceToDevice, h->stream)
+                    : hipMemcpyAsync(eng, snp, bytes, hipMemcpyDeviceToDevice, h->stream);
+    };
+    HIPCHK(cp(h->st[c].xy, s.xy, N * sizeof(double2)));
+    HIPCHK(cp(h->st[c].vt, s.vt, N * sizeof(float2)));
+    HIPCHK(cp(h->st[c].gt, s.gt, N * sizeof(uint32_t)));
+    HIPCHK(cp(h->st[c].mp, s.mp, N));
+    HIPCHK(cp(h->st[c].lnk, s.lnk, N * sizeof(int32_t)));
+    HIPCHK(cp(h->d_cs[c], s.cs, C * sizeof(int32_t)));
+    if (with_src && save) HIPCHK(hipMemcpyAsync(s.src, h->d_src, N * sizeof(int32_t), hipMemcpyDeviceToDevice, h->stream));
+    if (s.v) {
+        HIPCHK(cp(h->st[c].v, s.v, N * sizeof(double2)));
+        HIPCHK(cp(h->d_f, s.f, N * sizeof(double2)));
+        HIPCHK(cp(h->d_rs, s.rs, h->nrep * sizeof(RepStep)));
+    }
+    return 0;
+}
+
+static int adj_setup(oc_engine *h, int K, int nseg, int nbins) {
+    if (h->adj && h->adj->K == K && h->adj->nseg == nseg && h->adj->nbins == nbins) return 0;
+    adj_free(h);
+    Adj *a = new Adj();
+    h->adj = a;
+    a->K = K;
+    a->nseg = nseg;
+    a->nbins = nbins;
+    const size_t N = h->Ntot;
+    int rc = 0;
+    a->tape.resize(K + 1);
+    for (auto &t : a->tape) rc = rc ? rc : snap_alloc(h, a, t, false, true);
+    a->ckpt.resize(nseg);
+    for (auto &t : a->ckpt) rc = rc ? rc : snap_alloc(h, a, t, true, false);
+    rc = rc ? rc : snap_alloc(h, a, a->end, true, false);
+    for (int k = 0; k < 2; k++) {
+        rc = rc ? rc : aalloc(a, &a->X[k], N);
+        rc = rc ? rc : aalloc(a, &a->V[k], N);
+        rc = rc ? rc : aalloc(a, &a->F[k], N);
+    }
+    rc = rc ? rc : aalloc(a, &a->T, N);
+    rc = rc ? rc : aalloc(a, &a->thb, N * 8);
+    rc = rc ? rc : aalloc(a, &a->gw, (size_t)h->nrep * nbins);
+    rc = rc ? rc : aalloc(a, &a->grad, (size_t)h->nrep * 8);
+    rc = rc ? rc : aalloc(a, &a->rpart, h->bblk.size() * (size_t)nbins);
+    rc = rc ? rc : aalloc(a, &a->racc, (size_t)h->nrep * nbins);
+    rc = rc ? rc : aalloc(a, &a->base, h->nrep);
+    if (rc) adj_free(h);
+    return rc;
+}
+
+static void zero_f64(oc_engine *h, double *p, size_t n) {
+    k_zero_f64<<<grid_of((int64_t)n, 256), 256, 0, h->stream>>>(p, (int64_t)n);
+}
+
+OC_API int oc_window_forward(oc_engine *h, int32_t nsteps, int32_t ckpt_every, const oc_rdf_spec *spec,
+                             double *gbar) {
+    if (!h || !spec || nsteps < 1) return fail(OC_EINVAL, "bad arguments");
+    if (!h->have_state) return fail(OC_ESTATE, "no state");
+    if (spec->nbins < 1 || spec->nbins > 256 || !(spec->rmax > 0) || !(spec->width > 0))
+        return fail(OC_EINVAL, "rdf spec: 1..256 bins, rmax > 0, width > 0");
+    if (spec->ta < 0 || spec->ta >= OC_NTYPES || spec->tb < 0 || spec->tb >= OC_NTYPES)
+        return fail(OC_EINVAL, "rdf spec: bad types");
+    const float cut = (float)(spec->rmax + spec->width);
+    int Rc = 0;
+    for (int r = 0; r < h->nrep; r++) {
+        const RepInfo &R = h->ri[r];
+        const oc_params &P = h->prm[r];
+        if (P.k_angle != 0.0 || P.stir_A != 0.0 || P.kT_amp != 0.0 || P.chem)
+            return fail(OC_EINVAL, "window gradient needs k_angle = 0 and no stirring, kT cycle or chemistry");
+        Rc = std::max(Rc, (int)ceil(cut * R.icsx));
+    }
+    for (int r = 0; r < h->nrep; r++)
+        if (2 * Rc + 1 > h->ri[r].ncx || 2.0 * cut + 2.0 > h->ri[r].L)
+            return fail(OC_EINVAL, "box too small for the RDF cutoff");
+    const int K = ckpt_every > 0 ? std::min(ckpt_every, nsteps) : nsteps;
+    const int nseg = (nsteps + K - 1) / K;
+    const int nb = spec->nbins;
+    if (int rc = adj_setup(h, K, nseg, nb)) return rc;
+    Adj *a = h->adj;
+    a->ready = false;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<RepStep> rs0(h->nrep);
+    HIPCHK(hipMemcpy(rs0.data(), h->d_rs, h->nrep * sizeof(RepStep), hipMemcpyDeviceToHost));
+    std::vector<int64_t> base(h->nrep);
+    for (int r = 0; r < h->nrep; r++) {
+        base[r] = rs0[r].step;
+        if (rs0[r].step < h->ri[r].warm_end) return fail(OC_EINVAL, "window gradient inside the warm-up ramp");
+    }
+    HIPCHK(hipMemcpy(a->base, base.data(), h->nrep * sizeof(int64_t), hipMemcpyHostToDevice));
+    // RDF normalisation per replica from its type counts
+    std::vector<uint32_t> gth(h->Ntot);
+    HIPCHK(hipMemcpy(gth.data(), h->st[h->cur].gt, h->Ntot * 4, hipMemcpyDeviceToHost));
+    a->Cb.assign((size_t)h->nrep * nb, 0.0);
+    const double dr = spec->rmax / nb;
+    for (int r = 0; r < h->nrep; r++) {
+        const RepInfo &R = h->ri[r];
+        int64_t na = 0, nbb = 0;
+        for (int k = 0; k < R.N; k++) {
+            const uint32_t t = gth[R.off + k] >> kGidBits;
+            na += (int)t == spec->ta;
+            nbb += (int)t == spec->tb;
+        }
+        if (na == 0 || nbb == 0) return fail(OC_EINVAL, "rdf spec: a replica has no bead of the pair's types");
+        const double area = R.L * R.L;
+        const double norm = spec->ta == spec->tb ? 2.0 / (na * (na / area)) : 1.0 / (na * (nbb / area));
+        for (int b = 0; b < nb; b++) {
+            const double r0 = b * dr, r1 = (b + 1) * dr;
+            a->Cb[(size_t)r * nb + b] = dr * norm / (M_PI * (r1 * r1 - r0 * r0));
+        }
+    }
+    RdfArgs &RA = a->RA;
+    RA.ta = spec->ta;
+    RA.tb = spec->tb;
+    RA.nbins = nb;
+    RA.rmax = (float)spec->rmax;
+    RA.width = (float)spec->width;
+    RA.dr = (float)dr;
+    RA.cut = cut;
+    RA.Rc = Rc;
+    const int nbk = (int)h->bblk.size();
+    // f0 = F(x0, v~0; theta), then nsteps steps, checkpoints at the segment starts
+    drop_graphs(h);
+    launch_step_params(h, 0, -1);
+    launch_forces(h, true);
+    HIPCHK(hipMemsetAsync(a->racc, 0, (size_t)h->nrep * nb * sizeof(unsigned long long), h->stream));
+    for (int n = 0; n < nsteps; n++) {
+        if (n % K == 0)
+            if (int rc = snap_copy(h, a->ckpt[n / K], true, false)) return rc;
+        enqueue_step(h);
+        const int c = h->cur;
+        k_rdf<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->d_cs[c], h->st[c].xy, h->st[c].gt, RA, a->rpart);
+        k_rdf_reduce<<<h->nrep, 64, 0, h->stream>>>(h->d_ri, nb, a->rpart, a->racc);
+    }
+    if (int rc = snap_copy(h, a->end, true, false)) return rc;
+    HIPCHK(hipGetLastError());
+    if (int rc = check_err(h)) return rc;
+    std::vector<unsigned long long> acc((size_t)h->nrep * nb);
+    HIPCHK(hipMemcpy(acc.data(), a->racc, acc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    if (gbar)
+        for (size_t k = 0; k < acc.size(); k++) gbar[k] = a->Cb[k] * ((double)acc[k] / kFix) / nsteps;
+    a->nsteps = nsteps;
+    a->ready = true;
+    return 0;
+}
+
+OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad) {
+    if (!h || !dL_dgbar) return fail(OC_EINVAL, "bad arguments");
+    Adj *a = h->adj;
+    if (!a || !a->ready) return fail(OC_ESTATE, "no forward window to differentiate");
+    const int nb = a->nbins, K = a->K, nseg = a->nseg, nsteps = a->nsteps;
+    std::vector<double> gw((size_t)h->nrep * nb);
+    for (size_t k = 0; k < gw.size(); k++) gw[k] = dL_dgbar[k] / nsteps * a->Cb[k];
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(a->gw, gw.data(), gw.size() * sizeof(double), hipMemcpyHostToDevice));
+    const RdfArgs RA = a->RA;
+    const int nbk = (int)h->bblk.size();
+    const size_t N = h->Ntot;
+    zero_f64(h, a->thb, N * 8);
+    int ab = 0;  // adjoint buffer parity
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+        const int n0 = seg * K, n1 = std::min(n0 + K, nsteps), len = n1 - n0;
+        // recompute the segment from its checkpoint (bit-identical), recording each state
+        if (int rc = snap_copy(h, a->ckpt[seg], false, false)) return rc;
+        if (int rc = snap_copy(h, a->tape[0], true, false)) return rc;
+        for (int m = 1; m <= len; m++) {
+            enqueue_step(h);
+            if (int rc = snap_copy(h, a->tape[m], true, true)) return rc;
+        }
+        if (seg == nseg - 1) {
+            zero_f64(h, (double *)a->X[ab], 2 * N);
+            zero_f64(h, (double *)a->V[ab], 2 * N);
+            zero_f64(h, (double *)a->F[ab], 2 * N);
+            StateSnap &t = a->tape[len];
+            k_rdf_adj<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, t.cs, t.xy, t.gt, RA, a->gw, a->X[ab]);
+        }
+        for (int m = len; m >= 1; m--) {
+            StateSnap &t = a->tape[m];
+            k_step_params<<<h->nrep, 64, 0, h->stream>>>(h->d_ri, h->d_rs, n0 + m, -1, a->base);
+            k_pair_adj<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, t.cs, t.xy, t.vt, t.gt,
+                                                         a->F[ab], a->X[ab], a->T, a->thb);
+            k_bond_adj<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, t.xy, t.gt, t.mp, t.lnk, a->F[ab],
+                                                         a->X[ab]);
+            k_adj_step<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, t.src, a->X[ab], a->V[ab], a->F[ab],
+                                                         a->T, a->X[ab ^ 1], a->V[ab ^ 1], a->F[ab ^ 1]);
+            ab ^= 1;
+            if (n0 + m - 1 >= 1) {
+                StateSnap &tp = a->tape[m - 1];
+                k_rdf_adj<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, tp.cs, tp.xy, tp.gt, RA, a->gw,
+                                                            a->X[ab]);
+            }
+        }
+    }
+    // the window's first force, f0 = F(x0, v~0; theta), carries parameter gradient too
+    {
+        StateSnap &t = a->tape[0];
+        k_step_params<<<h->nrep, 64, 0, h->stream>>>(h->d_ri, h->d_rs, 0, -1, a->base);
+        k_pair_adj<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, t.cs, t.xy, t.vt, t.gt, a->F[ab],
+                                                     a->X[ab], a->T, a->thb);
+    }
+    k_theta_reduce<<<h->nrep, 256, 0, h->stream>>>(h->d_ri, a->thb, a->grad);
+    // the engine continues from the window end
+    if (int rc = snap_copy(h, a->end, false, false)) return rc;
+    HIPCHK(hipGetLastError());
+    if (int rc = check_err(h)) return rc;
+    std::vector<double> g8((size_t)h->nrep * 8);
+    HIPCHK(hipMemcpy(g8.data(), a->grad, g8.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (grad)
+        for (int r = 0; r < h->nrep; r++)
+            for (int q = 0; q < 7; q++) grad[(size_t)r * 7 + q] = g8[(size_t)r * 8 + q];
+    return 0;
+}
+
+OC_API int oc_window_grad(oc_engine *h, int32_t nsteps, int32_t ckpt_every, const oc_rdf_spec *spec,
+                          const double *target, int32_t target_stride, double *loss, double *grad, double *gbar) {
+    if (!h || !spec || !target) return fail(OC_EINVAL, "bad arguments");
+    const int nb = spec->nbins;
+    std::vector<double> g((size_t)h->nrep * std::max(nb, 1));
+    if (int rc = oc_window_forward(h, nsteps, ckpt_every, spec, g.data())) return rc;
+    std::vector<double> dl(g.size());
+    for (int r = 0; r < h->nrep; r++) {
+        double Ls = 0.0;
+        for (int b = 0; b < nb; b++) {
+            const size_t k = (size_t)r * nb + b;
+            const double d = g[k] - target[(size_t)r * target_stride + b];
+            Ls += d * d;
+            dl[k] = 2.0 * d;
+        }
+        if (loss) loss[r] = Ls;
+    }
+    if (gbar) std::copy(g.begin(), g.end(), gbar);
+    return oc_window_backward(h, dl.data(), grad);
 }

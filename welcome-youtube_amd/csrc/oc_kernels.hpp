@@ -66,11 +66,13 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 // --------------------------------------------------------------------------- per-step parameters
 
 __global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs,
-                                                    int64_t delta, int64_t set_step) {
+                                                    int64_t delta, int64_t set_step,
+                                                    const int64_t *__restrict__ base) {
     const int r = blockIdx.x, k = threadIdx.x;
     const RepInfo &R = ri[r];
     RepStep &S = rs[r];
-    const int64_t st = set_step >= 0 ? set_step : S.step + delta;
+    // step: base[r] + delta (per-replica base), or set_step, or the current + delta
+    const int64_t st = base ? base[r] + delta : (set_step >= 0 ? set_step : S.step + delta);
     // a_eff, cel/engine/oercel.c:1122-1129: one matrix entry per lane
     const double a = R.a[k];
     double ae = a;
@@ -99,12 +101,15 @@ __global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ 
 // --------------------------------------------------------------------------- sort
 
 // New cell key of every bead (of its integrated position when INTEG) and the
-// histogram; the largest periodic cell displacement goes to Rx/Ry.
+// histogram; each block stores its largest periodic cell displacement
+// (x, y) in bmax[block] (no same-address atomics), reduced per replica by
+// k_scan_top.
 template <bool INTEG>
-__global__ __launch_bounds__(kBeadBlock) void k_keys(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs,
-                                                     const int32_t *__restrict__ bblk, const double2 *__restrict__ xy,
-                                                     const double2 *__restrict__ v, const double2 *__restrict__ f,
-                                                     int32_t *__restrict__ key, int32_t *__restrict__ count) {
+__global__ __launch_bounds__(kBeadBlock) void k_keys(const RepInfo *__restrict__ ri, const int32_t *__restrict__ bblk,
+                                                     const double2 *__restrict__ xy, const double2 *__restrict__ v,
+                                                     const double2 *__restrict__ f, int32_t *__restrict__ key,
+                                                     int32_t *__restrict__ count, int2 *__restrict__ bmax) {
+    __shared__ int red[2][kBeadBlock / 64];
     const int r = bblk[blockIdx.x];
     const RepInfo &R = ri[r];
     const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
@@ -130,11 +135,18 @@ __global__ __launch_bounds__(kBeadBlock) void k_keys(const RepInfo *__restrict__
     }
     dx = wave_max(dx);
     dy = wave_max(dy);
-    // one word per replica: most waves find the maximum already there, so read
-    // before the (serialising) same-address atomic
     if ((threadIdx.x & 63) == 0) {
-        if (dx > __hip_atomic_load(&rs[r].Rx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&rs[r].Rx, dx);
-        if (dy > __hip_atomic_load(&rs[r].Ry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&rs[r].Ry, dy);
+        red[0][threadIdx.x >> 6] = dx;
+        red[1][threadIdx.x >> 6] = dy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int mx = 0, my = 0;
+        for (int w = 0; w < kBeadBlock / 64; w++) {
+            mx = max(mx, red[0][w]);
+            my = max(my, red[1][w]);
+        }
+        bmax[blockIdx.x] = make_int2(mx, my);
     }
 }
 
@@ -152,7 +164,11 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const int32_t *__restrict__
     if (threadIdx.x == 0) bsum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ __launch_bounds__(1024) void k_scan_top(int32_t *__restrict__ bsum, int nb) {
+// Scan of the block sums (one block), and the per-replica sort window radius
+// from the k_keys block maxima.
+__global__ __launch_bounds__(1024) void k_scan_top(int32_t *__restrict__ bsum, int nb, const RepInfo *__restrict__ ri,
+                                                   RepStep *__restrict__ rs, int nrep,
+                                                   const int2 *__restrict__ bmax) {
     __shared__ int wsum[16];
     const int per = (nb + 1023) / 1024;
     const int b0 = threadIdx.x * per;
@@ -171,6 +187,24 @@ __global__ __launch_bounds__(1024) void k_scan_top(int32_t *__restrict__ bsum, i
             bsum[b0 + k] = run;
             run += c;
         }
+    // one wavefront per replica (16 in flight)
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int r = wv; r < nrep; r += 16) {
+        const RepInfo &R = ri[r];
+        const int nblk = (R.N + kBeadBlock - 1) / kBeadBlock;
+        int mx = 0, my = 0;
+        for (int b = lane; b < nblk; b += 64) {
+            const int2 m = bmax[R.bblk0 + b];
+            mx = max(mx, m.x);
+            my = max(my, m.y);
+        }
+        mx = wave_max(mx);
+        my = wave_max(my);
+        if (lane == 0) {
+            rs[r].Rx = mx;
+            rs[r].Ry = my;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_scan_down(int32_t *__restrict__ count, int64_t C,
@@ -288,7 +322,6 @@ __global__ __launch_bounds__(kBeadBlock) void k_permute(const RepInfo *__restric
 
 // --------------------------------------------------------------------------- pair forces
 
-constexpr int kMaxRows = 512;  // rows per staged column (tile rows + 2 * margin + 1)
 constexpr float kTheta32 = (float)(3.4641016151377545870548926830117 / 4294967296.0);  // sqrt(12) / 2^32
 
 __device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float vyi, uint32_t gi, const float *arow,
@@ -337,9 +370,13 @@ struct PairArgs {
 // in LDS once per tile; each lane then runs its own y-window per column, the
 // window expressions of cel/engine/oercel.c:1073-1074.  Forces are summed in
 // FP64 per bead, no atomics: the result is deterministic.
+// rows per staged column (tile rows + 2 * margin + 1) of a pair tile
+__host__ __device__ constexpr int pair_max_rows(int block) { return 2 * block; }
+
 template <int BLOCK, int CAP, int CAPL, bool ENERGY>
 __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
-    __shared__ uint16_t s_lst[CAPL][BLOCK];  // per-lane list of interacting candidates
+    constexpr int kMaxRows = pair_max_rows(BLOCK);
+    __shared__ uint16_t s_lst[CAPL > 0 ? CAPL : 1][BLOCK];  // per-lane list of interacting candidates
     __shared__ double2 s_xy[CAP];
     __shared__ float2 s_vt[CAP];
     __shared__ uint32_t s_gt[CAP];
@@ -405,16 +442,14 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
         s_row[c][e - c * nrow] = off;
     }
     if (staged) {
-        for (int e = threadIdx.x; e < total; e += BLOCK) {
-            int c = 0, pc = 0;
-            for (int q = 0; q < 9; q++) {
-                const int cc = q / 3, pp = q - cc * 3;
-                if (e >= s_piece[cc][pp][2] && e < s_piece[cc][pp][2] + s_piece[cc][pp][1]) { c = cc; pc = pp; }
+        for (int q = 0; q < 9; q++) {
+            const int cc = q / 3, pp = q - cc * 3;
+            const int first = s_piece[cc][pp][0], len = s_piece[cc][pp][1], dst = s_piece[cc][pp][2];
+            for (int e = threadIdx.x; e < len; e += BLOCK) {
+                s_xy[dst + e] = A.xy[first + e];
+                s_vt[dst + e] = A.vt[first + e];
+                s_gt[dst + e] = A.gt[first + e];
             }
-            const int slot = s_piece[c][pc][0] + (e - s_piece[c][pc][2]);
-            s_xy[e] = A.xy[slot];
-            s_vt[e] = A.vt[slot];
-            s_gt[e] = A.gt[slot];
         }
     }
     __syncthreads();
@@ -431,15 +466,18 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     const int slot0 = cs[R.coff + T.cx * ncy + T.k0];
     // A tile whose candidates overflow the LDS buffer (a dense clump) runs the
     // same loop on global memory; s_row offsets are then rebased per piece.
-    // Each lane owns one i-bead at a time.  Staged tiles run two phases: the
-    // cutoff test over all candidates appends the interacting ones to the
-    // lane's LDS list (u16 code: LDS index | column << 10 | piece << 12), then
-    // the force and noise arithmetic runs over that list only, so a wavefront
-    // executes it ~max(pairs) times instead of ~max(candidates) times.  A list
-    // that fills up spills to in-place evaluation.  Tiles whose candidates
-    // overflow the LDS buffer (a dense clump) run one pass on global memory,
-    // with s_row offsets rebased per piece.
+    // Each lane owns one i-bead at a time.  With CAPL > 0, staged tiles run two
+    // phases: the cutoff test over all candidates appends the interacting ones
+    // to the lane's LDS list (u16 code: LDS index | column << 10 | piece << 12),
+    // then the force and noise arithmetic runs over that list only, so a
+    // wavefront executes it ~max(pairs per lane) times instead of
+    // ~max(candidates per lane) times; a full list spills to in-place
+    // evaluation.  With CAPL = 0 a single pass evaluates in place.  Tiles whose
+    // candidates overflow the LDS buffer (a dense clump) run the single pass on
+    // global memory, with s_row offsets rebased per piece.  The next
+    // candidate's position is loaded one iteration ahead.
     auto run = [&](const double2 *XY, const float2 *VT, const uint32_t *GT, const bool stg) {
+        const bool lists = CAPL > 0 && stg;
         for (int ii = ibeg + threadIdx.x; ii < iend; ii += BLOCK) {
             const int slot = slot0 + (ii - ibeg);
             const int is = stg ? ii : slot;
@@ -468,32 +506,56 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                     else if (q == 1) { pa = max(klo, 0); pb = min(khi, ncy - 1); sy = 0.0; }
                     else { pa = max(klo, ncy); pb = khi; sy = L; }
                     if (pa > pb) continue;
+                    const double sx = sxc[c];
                     const int adj = stg ? 0 : s_piece[c][q][0] - s_piece[c][q][2];
                     const int ja = s_row[c][pa - kb0] + adj, jb = s_row[c][pb + 1 - kb0] + adj;
-                    if (!stg) {
+                    if (ja >= jb) continue;
+                    if (!lists) {
                         for (int j = ja; j < jb; j++)
-                            pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], sxc[c], sy, gam, sdt,
-                                     sseed, ax, ay, e, ENERGY);
+                            pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], sx, sy, gam, sdt, sseed,
+                                     ax, ay, e, ENERGY);
                         continue;
                     }
-                    for (int j = ja; j < jb; j++) {
-                        const double2 pj = XY[j];
-                        const float dx = (float)((pi.x - pj.x) - sxc[c]), dy = (float)((pi.y - pj.y) - sy);
-                        const float r2 = dx * dx + dy * dy;
-                        if (r2 < 1.0f && r2 > 1e-24f) {
-                            if (cnt < CAPL) s_lst[cnt++][threadIdx.x] = (uint16_t)(j | (c << 10) | (q << 12));
-                            else pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, pj, VT[j], GT[j], sxc[c], sy, gam, sdt,
-                                          sseed, ax, ay, e, ENERGY);
+                    const uint16_t tag = (uint16_t)((c << 10) | (q << 12));
+                    double2 pn = XY[ja];
+                    if (sx == 0.0 && sy == 0.0) {
+                        for (int j = ja; j < jb; j++) {
+                            const double2 pj = pn;
+                            if (j + 1 < jb) pn = XY[j + 1];
+                            const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
+                            const float r2 = dx * dx + dy * dy;
+                            if (r2 < 1.0f && r2 > 1e-24f) {
+                                if (cnt < CAPL) s_lst[cnt++ % (CAPL > 0 ? CAPL : 1)][threadIdx.x] = (uint16_t)j | tag;
+                                else pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, pj, VT[j], GT[j], sx, sy, gam, sdt,
+                                              sseed, ax, ay, e, ENERGY);
+                            }
+                        }
+                    } else {
+                        for (int j = ja; j < jb; j++) {
+                            const double2 pj = pn;
+                            if (j + 1 < jb) pn = XY[j + 1];
+                            const float dx = (float)((pi.x - pj.x) - sx), dy = (float)((pi.y - pj.y) - sy);
+                            const float r2 = dx * dx + dy * dy;
+                            if (r2 < 1.0f && r2 > 1e-24f) {
+                                if (cnt < CAPL) s_lst[cnt++ % (CAPL > 0 ? CAPL : 1)][threadIdx.x] = (uint16_t)j | tag;
+                                else pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, pj, VT[j], GT[j], sx, sy, gam, sdt,
+                                              sseed, ax, ay, e, ENERGY);
+                            }
                         }
                     }
                 }
             }
-            for (int k = 0; k < cnt; k++) {
-                const int code = s_lst[k][threadIdx.x];
-                const int j = code & 1023, c = (code >> 10) & 3, q = code >> 12;
-                const double sy = q == 0 ? -L : (q == 2 ? L : 0.0);
-                pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], sxc[c], sy, gam, sdt, sseed, ax, ay,
-                         e, ENERGY);
+            if (lists) {
+                int code = cnt > 0 ? s_lst[0][threadIdx.x] : 0;
+                for (int k = 0; k < cnt; k++) {
+                    const int cur = code;
+                    if (k + 1 < cnt) code = s_lst[(k + 1) % (CAPL > 0 ? CAPL : 1)][threadIdx.x];
+                    const int j = cur & 1023, c = (cur >> 10) & 3, q = cur >> 12;
+                    const double sy = q == 0 ? -L : (q == 2 ? L : 0.0);
+                    const double sx = c == 0 ? sxc[0] : (c == 1 ? 0.0 : sxc[2]);
+                    pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], sx, sy, gam, sdt, sseed, ax, ay,
+                             e, ENERGY);
+                }
             }
             if (ENERGY) {
                 A.epot[slot] = (double)e;
@@ -601,7 +663,10 @@ __global__ __launch_bounds__(kBeadBlock) void k_bonded(const RepInfo *__restrict
 // --------------------------------------------------------------------------- precursor chemistry
 
 // Visit the beads within the cell window of bead i (cutoff <= 1) until fn
-// returns false.  Distances: FP64 minimum image.
+// returns false.  Within one column the window's rows are contiguous cells,
+// hence one contiguous slot range per periodic piece (at most two per column);
+// the slot loop is unrolled so independent candidate loads overlap.
+// Distances: FP64 minimum image (the oracle's arithmetic).
 template <typename Fn>
 __device__ __forceinline__ void visit_nbrs(const RepInfo &R, const int32_t *cs, const double2 *xy, int i, double2 p,
                                            Fn fn) {
@@ -611,13 +676,20 @@ __device__ __forceinline__ void visit_nbrs(const RepInfo &R, const int32_t *cs, 
     for (int dc = -1; dc <= 1; dc++) {
         int col = cx + dc;
         col = col < 0 ? col + R.ncx : (col >= R.ncx ? col - R.ncx : col);
-        for (int k = klo; k <= khi; k++) {
-            const int kk = k < 0 ? k + R.ncy : (k >= R.ncy ? k - R.ncy : k);
-            const int c = R.coff + col * R.ncy + kk;
-            for (int j = cs[c]; j < cs[c + 1]; j++) {
+        const int cb = R.coff + col * R.ncy;
+        // the window spans fewer rows than the column (L >= 8), so at most
+        // one seam: rows wrapped from the top or from the bottom
+        int ra[2], rb[2], np = 0;
+        if (klo < 0) { ra[np] = klo + R.ncy; rb[np] = R.ncy - 1; np++; }
+        ra[np] = max(klo, 0); rb[np] = min(khi, R.ncy - 1); np++;
+        if (khi >= R.ncy && np < 2) { ra[np] = 0; rb[np] = khi - R.ncy; np++; }
+        for (int q = 0; q < np; q++) {
+            const int j0 = cs[cb + ra[q]], j1 = cs[cb + rb[q] + 1];
+#pragma unroll 4
+            for (int j = j0; j < j1; j++) {
                 if (j == i) continue;
-                const double2 q = xy[j];
-                const double dx = minimg(q.x - p.x, R.L, R.halfL), dy = minimg(q.y - p.y, R.L, R.halfL);
+                const double2 qq = xy[j];
+                const double dx = minimg(qq.x - p.x, R.L, R.halfL), dy = minimg(qq.y - p.y, R.L, R.halfL);
                 if (!fn(j, dx * dx + dy * dy)) return;
             }
         }

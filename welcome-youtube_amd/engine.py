@@ -55,7 +55,7 @@ class Engine:
     """One engine handle over one or more replicas (a single system is an
     ensemble of one)."""
 
-    def __init__(self, params, N=None, pair_block=0, tile_beads=0):
+    def __init__(self, params, N=None, pair_block=0, tile_beads=0, pair_lists=False):
         plist = params if isinstance(params, (list, tuple)) else [params]
         self.params = [p if isinstance(p, Params) else p for p in plist]
         for p in self.params:
@@ -75,6 +75,8 @@ class Engine:
         self.h = h
         if pair_block or tile_beads:
             self.set_tuning(pair_block, tile_beads)
+        if pair_lists:
+            self._chk(self.lib.oc_set_pair_variant(self.h, 1))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -172,6 +174,63 @@ class Engine:
         d = {k: getattr(o, k) for k, _ in abi.oc_stats._fields_ if k != "count"}
         d["count"] = list(o.count)[:abi.NTYPES]
         return d
+
+    # ------------------------------------------------------------------ window gradient
+    THETA_NAMES = ("a_W_W", "a_W_H", "a_W_T", "a_H_H", "a_H_T", "a_T_T", "gamma")
+    THETA_IJ = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
+
+    def set_pair_params(self, a, gamma, r=0):
+        """Replace replica r's repulsion matrix (7x7 or 8x8, symmetric) and gamma."""
+        a64 = np.zeros((8, 8))
+        a = np.asarray(a, dtype=np.float64)
+        a64[:a.shape[0], :a.shape[1]] = a
+        a64 = np.ascontiguousarray(a64)
+        self._chk(self.lib.oc_set_pair_params(self.h, int(r), a64.ctypes.data_as(C.POINTER(C.c_double)),
+                                              float(gamma)))
+
+    def set_theta(self, theta, r=0):
+        """theta = (a_WW, a_WH, a_WT, a_HH, a_HT, a_TT, gamma) of replica r; the
+        other a_ij keep their values."""
+        a = np.array(self.params[r].a, dtype=np.float64)
+        for q, (i, j) in enumerate(self.THETA_IJ):
+            a[i][j] = a[j][i] = float(theta[q])
+        self.set_pair_params(a, float(theta[6]), r)
+        p = self.params[r]
+        for q, (i, j) in enumerate(self.THETA_IJ):
+            p.a[i][j] = p.a[j][i] = float(theta[q])
+        p.gamma = float(theta[6])
+
+    def theta(self, r=0):
+        p = self.params[r]
+        return np.array([p.a[i][j] for i, j in self.THETA_IJ] + [p.gamma])
+
+    def window_forward(self, nsteps, spec, ckpt_every=20):
+        """Run an nsteps window (checkpoints every ckpt_every steps); returns the
+        window-mean smooth RDF per replica, shape (nrep, nbins)."""
+        g = np.zeros((self.nrep, spec.nbins))
+        self._chk(self.lib.oc_window_forward(self.h, int(nsteps), int(ckpt_every), C.byref(spec),
+                                             g.ctypes.data_as(C.POINTER(C.c_double))))
+        return g
+
+    def window_backward(self, dL_dgbar):
+        """dL/dtheta (nrep, 7) of the last forward window for dL/dgbar (nrep, nbins)."""
+        dl = np.ascontiguousarray(dL_dgbar, dtype=np.float64)
+        g = np.zeros((self.nrep, 7))
+        self._chk(self.lib.oc_window_backward(self.h, dl.ctypes.data_as(C.POINTER(C.c_double)),
+                                              g.ctypes.data_as(C.POINTER(C.c_double))))
+        return g
+
+    def window_grad(self, nsteps, spec, target, ckpt_every=20):
+        """L2 RDF loss per replica, its gradient (nrep, 7) and the mean RDF."""
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        stride = spec.nbins if t.ndim == 2 else 0
+        loss = np.zeros(self.nrep)
+        g = np.zeros((self.nrep, 7))
+        gb = np.zeros((self.nrep, spec.nbins))
+        P = C.POINTER(C.c_double)
+        self._chk(self.lib.oc_window_grad(self.h, int(nsteps), int(ckpt_every), C.byref(spec), t.ctypes.data_as(P),
+                                          stride, loss.ctypes.data_as(P), g.ctypes.data_as(P), gb.ctypes.data_as(P)))
+        return loss, g, gb
 
     def step_count(self, r=0):
         s = C.c_int64()
