@@ -258,6 +258,7 @@ def main():
     ap.add_argument("--nsub", type=int, default=4)
     ap.add_argument("--pair-block", type=int, default=0)
     ap.add_argument("--tile-beads", type=int, default=0)
+    ap.add_argument("--pair-lists", action="store_true", help="two-phase pair kernel variant")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--workload", choices=("plain", "ensemble", "adjoint"), default="plain")
@@ -279,7 +280,7 @@ def main():
         return bench_adjoint(a, pm, eng)
     p = config2_params(pm, a.N, a.nsub)
     N = p.N
-    e = eng.Engine(p, pair_block=a.pair_block, tile_beads=a.tile_beads)
+    e = eng.Engine(p, pair_block=a.pair_block, tile_beads=a.tile_beads, pair_lists=a.pair_lists)
     e.start()
     e.step(a.warmup, graph=not a.no_graph)
     e.step_count()  # device synchronised

@@ -75,13 +75,10 @@ def test_checkpoint_interval_does_not_change_gradient(oracle, eng):
     e.close()
 
 
-@pytest.mark.parametrize("nsteps,tol", [(20, 1e-3), (200, None)])
-def test_gradient_vs_central_differences(oracle, eng, nsteps, tol):
+def test_gradient_vs_central_differences(oracle, eng):
     """Central differences (eps = 1e-3, as the adjoint workload states) of the
-    device's own forward window with the noise fixed.  Over 20 steps they agree
-    to 1e-3 relative to the largest component.  Over a 200-step window (two
-    time units) the FP32 trajectories' chaotic sensitivity dominates the
-    differences; the agreement is measured and printed, and bounded loosely."""
+    device's own forward window with the noise fixed, over 20 steps."""
+    nsteps = 20
     p, st, s0 = system(oracle, L=16.0, seed=7)
     abi = importlib.import_module("welcome-youtube_amd.abi")
     spec = abi.rdf_spec(2, 2, 64, 3.0)
@@ -110,7 +107,32 @@ def test_gradient_vs_central_differences(oracle, eng, nsteps, tol):
     rel = np.abs(gad - fd) / np.abs(fd).max()
     say(f"nsteps={nsteps} eps={eps:g}: adjoint {gad}\n   fd {fd}\n   rel {rel} (max {rel.max():.2e})")
     e.close()
-    assert rel.max() < (tol if tol else 0.25)
+    assert rel.max() < 1e-3
+
+
+def test_long_window_matches_fp64_oracle(oracle, eng):
+    """200-step window (the adjoint workload's length).  Central differences
+    at eps = 1e-3 are outside the loss's linear range over this chaotic window
+    (even in FP64 they differ by tens of percent, while eps = 1e-7 agrees with
+    the FP64 adjoint to ~1e-6: tests/test_oracle_math.py), so the device
+    gradient is compared with the FP64 oracle's adjoint of the same window; the
+    FP32 and FP64 trajectories separate chaotically, which bounds the
+    agreement."""
+    p, st, s0 = system(oracle, L=16.0, seed=7)
+    abi = importlib.import_module("welcome-youtube_amd.abi")
+    spec = abi.rdf_spec(2, 2, 64, 3.0)
+    ospec = oracle.rdf_spec(2, 2, 64, 3.0)
+    o = oracle.System(p.to_c(), p.N)
+    o.set_state(st, s0)
+    tgt = o.rdf(ospec) * 1.05
+    lo, go, _ = o.window_grad(200, ospec, tgt, theta32=1)
+    e = eng.Engine(p)
+    e.set_state(st, step=s0)
+    ld, gd, _ = e.window_grad(200, spec, tgt, ckpt_every=20)
+    rel = np.abs(gd[0] - go) / np.abs(go).max()
+    say(f"200 steps: device {gd[0]}\n   oracle {go}\n   rel {rel} (max {rel.max():.2e}); loss {ld[0]:.6f} vs {lo:.6f}")
+    e.close()
+    assert rel.max() < 0.05
 
 
 def test_autograd_wrapper(oracle, eng):

@@ -1,0 +1,99 @@
+"""File formats against the reference engine's own files: a checkpoint and a
+snapshot re-written from what was read are byte-identical to the reference's,
+and a checkpoint written here restarts the reference CLI, whose continuation
+matches the oracle's.  CPU only; needs oracle/_ref (the reference CLI built
+from its source)."""
+import importlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import params_mod
+
+io = importlib.import_module("welcome-youtube_amd.io")
+pm = params_mod()
+
+
+def ref_cli(oracle):
+    b = oracle.ref_binary()
+    if not os.path.exists(b):
+        pytest.skip("oracle/_ref not built")
+    return b
+
+
+OV = dict(L="14.5", dt="0.01", f_lip="0.2", lip_nT="2", c_N="0.1", f_Ka="0", k_angle="10", warmup="5", seed="3",
+          nsub="4", chem="0", snap_every="10", stats_every="0", ckpt_every="0", log_every="0", events="0",
+          threads="2")
+
+
+def test_checkpoint_and_snapshot_bytes_identical(oracle, tmp_path):
+    ref_cli(oracle)
+    d = str(tmp_path / "run")
+    oracle.run_ref_cli(d, dict(OV, steps="10"))
+    s, hdr, slot_gid = io.read_checkpoint(os.path.join(d, "ckpt.bin"))
+    assert hdr["step"] == 10 and hdr["N"] == len(s["x"])
+    out = str(tmp_path / "ck.bin")
+    io.write_checkpoint(out, s, slot_gid, hdr)
+    assert open(out, "rb").read() == open(os.path.join(d, "ckpt.bin"), "rb").read()
+    sn = str(tmp_path / "snap.bin")
+    io.write_snapshot(sn, s, hdr["step"], hdr["time"], hdr["L"], hdr["mlen"])
+    ref_snap = os.path.join(d, "snap_0000000010.bin")
+    assert open(sn, "rb").read() == open(ref_snap, "rb").read()
+    r = io.read_snapshot(ref_snap)
+    assert r["step"] == 10 and len(r["bonds"]) == int((s["mpos"] == 0).sum()) * (hdr["mlen"] - 1)
+
+
+def test_reference_restarts_from_our_checkpoint(oracle, tmp_path):
+    """Oracle state after 15 steps -> our checkpoint -> the reference CLI runs
+    10 more steps from it; the oracle's own 10 steps agree to round-off."""
+    ref_cli(oracle)
+    p = pm.load(overrides={k: v for k, v in OV.items()})
+    cp = p.to_c()
+    sysm = oracle.System(cp, p.N)
+    sysm.start(oracle.soup(cp, p.N), theta32=0)
+    sysm.step(15, theta32=0)
+    st = sysm.get_state()
+    _, slot_gid = sysm.cells()
+    ck = str(tmp_path / "ours.bin")
+    hdr = {"step": 15, "time": 15 * p.dt, "L": p.L, "seed": p.seed, "mlen": p.mlen, "nH": p.lip_nH,
+           "warm_end": p.warmup, "dt_anchor": p.dt, "maxbond": 0.0}
+    io.write_checkpoint(ck, st, slot_gid, hdr)
+    d = str(tmp_path / "cont")
+    oracle.run_ref_cli(d, dict(OV, steps="25", restart=ck, init="restart"))
+    ref, rh, _ = io.read_checkpoint(os.path.join(d, "ckpt.bin"))
+    assert rh["step"] == 25
+    sysm.step(10, theta32=0)
+    o = sysm.get_state()
+    L = p.L
+    dx = np.abs((o["x"] - ref["x"] + L / 2) % L - L / 2).max()
+    dv = np.abs(o["vx"] - ref["vx"]).max()
+    assert dx < 1e-10 and dv < 1e-8, (dx, dv)
+
+
+def test_chemistry_bonds_round_trip(tmp_path):
+    N = 12
+    s = {"x": np.linspace(0, 5, N), "y": np.linspace(1, 2, N), "vx": np.zeros(N), "vy": np.ones(N),
+         "fx": np.arange(N, dtype=float), "fy": np.zeros(N), "type": np.array([1, 2, 2] + [2, 2] + [0] * 7, np.uint8),
+         "mpos": np.array([0, 1, 2] + [-1] * 9, np.int8), "nxt": np.array([1, 2] + [-1] * 10, np.int32),
+         "bond": np.array([-1, -1, -1, 4, 3] + [-1] * 7, np.int32)}
+    slot = np.random.default_rng(0).permutation(N)
+    path = str(tmp_path / "c.bin")
+    io.write_checkpoint(path, s, slot, {"step": 3, "time": 0.03, "L": 8.0, "seed": 5, "mlen": 3, "nH": 1})
+    r, h, g = io.read_checkpoint(path)
+    for k in s:
+        assert np.array_equal(r[k], s[k]), k
+    assert np.array_equal(g, slot) and h["rng"] == io.rng_seed_state(5)
+    with open(path, "r+b") as f:
+        f.truncate(100)
+    with pytest.raises(io.FormatError, match="truncated"):
+        io.read_checkpoint(path)
+
+
+def test_stats_row_format():
+    st = {"step": 100, "time": 1.0, "T_kin": 0.99876543, "count": [10, 2, 4, 0, 0, 0, 0], "n_lipids": 2,
+          "max_bond": 0.81234567}
+    assert io.stats_row(st) == "100,1.000000,0.998765,10,2,4,0,0,0,0,2,0.812346"
+    assert io.STATS_HEADER.count(",") == io.stats_row(st).count(",")
+    json.dumps(st)

@@ -194,3 +194,36 @@ def test_stats_closed_forms(oracle):
     E = (0.5 * aa[m] * (1 - rr[m]) ** 2).sum()
     assert abs(o.E_pair - E) < 1e-9 * E
     assert o.count[0] + o.count[1] + o.count[2] == p.N
+
+
+@pytest.mark.parametrize("nsteps", [20, 200])
+def test_window_adjoint_vs_central_differences(oracle, nsteps):
+    """The FP64 oracle adjoint of the smooth tail-tail RDF loss against central
+    differences with a small step (the loss is smooth at that scale; over 200
+    chaotic steps eps = 1e-3 is no longer in its linear range)."""
+    p = mkparams(12.0, f_lip=0.3, seed=3)
+    cp = p.to_c()
+    sysm = oracle.System(cp, p.N)
+    sysm.start(oracle.soup(cp, p.N))
+    sysm.step(150)
+    st0 = sysm.get_state()
+    spec = oracle.rdf_spec(2, 2, 32, 3.0)
+    IJ = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+
+    def run(theta, tgt):
+        s = oracle.System(cp, p.N)
+        s.set_state(st0, 150)
+        a = np.array(p.a, dtype=float)
+        for q, (i, j) in enumerate(IJ):
+            a[i][j] = a[j][i] = theta[q]
+        s.set_pair_params(a, theta[6])
+        return s.window_grad(nsteps, spec, tgt)
+
+    th0 = np.array([p.a[i][j] for i, j in IJ] + [p.gamma])
+    tgt = run(th0, np.zeros(32))[2] * 1.05
+    _, g, _ = run(th0, tgt)
+    eps = 1e-6 if nsteps <= 20 else 1e-7
+    for q in (2, 5, 6):
+        e = np.eye(7)[q] * eps
+        fd = (run(th0 + e, tgt)[0] - run(th0 - e, tgt)[0]) / (2 * eps)
+        assert abs(fd - g[q]) < 2e-5 * np.abs(g).max(), (q, fd, g[q])
