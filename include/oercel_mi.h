@@ -220,9 +220,10 @@ int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad);
 int oc_window_grad(oc_engine *h, int32_t nsteps, int32_t ckpt_every, const oc_rdf_spec *spec, const double *target,
                    int32_t target_stride, double *loss, double *grad, double *gbar);
 
-/* Launch only the pair-force kernel reps times on the handle's stream, for
- * the current state and step, without synchronising: lets a caller time that
- * kernel alone with events on oc_stream(). */
+/* Launch only the force kernel (pair forces with the bonded terms, as the
+ * step runs it) reps times on the handle's stream, for the current state and
+ * step, without synchronising: lets a caller time that kernel alone with
+ * events on oc_stream(). */
 int oc_launch_pair(oc_engine *h, int32_t reps);
 
 /* sizeof the ABI structs, for binding checks: out[0..3] = oc_params,

@@ -31,26 +31,31 @@ struct oc_engine {
     std::vector<oc_params> prm;
     std::vector<RepInfo> ri;
     int64_t Ntot = 0, Ctot = 0;
-    std::vector<int32_t> bblk, cblk, rblk;
-    std::vector<Tile> tiles;
+    std::vector<int32_t> bblk, rblk;
+    std::vector<Tile> tiles;   // pair-force tiles
+    std::vector<Tile> stiles;  // sort tiles (rows of new cells, one thread each)
     int pair_block = 256, tile_beads = 0;
     bool pair_lists = false;
     int per_xcd = 0;
     // device
     RepInfo *d_ri = nullptr;
     RepStep *d_rs = nullptr;
-    int32_t *d_bblk = nullptr, *d_cblk = nullptr, *d_rblk = nullptr;
-    Tile *d_tiles = nullptr;
+    int32_t *d_bblk = nullptr, *d_rblk = nullptr;
+    Tile *d_tiles = nullptr, *d_stiles = nullptr;
     StateP st[2];
     int cur = 0;
     double2 *d_f = nullptr;
-    int32_t *d_key = nullptr, *d_src = nullptr, *d_dest = nullptr, *d_count = nullptr, *d_cs[2] = {nullptr, nullptr};
-    int32_t *d_bsum = nullptr;
+    int32_t *d_key = nullptr, *d_src = nullptr, *d_cs[2] = {nullptr, nullptr};
+    // sort: per tile bead count, first new slot, per-tile bead list; per bead
+    // its (tile, rank); per cell its rank offset within the tile
+    int32_t *d_tcount = nullptr, *d_toff = nullptr, *d_tsrc = nullptr, *d_tslot = nullptr, *d_locst = nullptr;
+    uint32_t *d_moved = nullptr;  // a bead moved by more than one cell this step
+    bool record_src = false;      // the sort writes src (new slot -> old slot) for the adjoint's tape
     int2 *d_bmax = nullptr;
-    int nscan = 0;
     double *d_stir = nullptr, *d_epot = nullptr, *d_ebond = nullptr, *d_part = nullptr, *d_statout = nullptr;
     uint8_t *d_flag = nullptr;
     int32_t *d_best = nullptr;
+    int32_t *d_act = nullptr, *d_actn = nullptr;  // per-chunk lists of beads with a chemistry decision
     Event *d_ev = nullptr;
     unsigned long long *d_evn = nullptr;
     int64_t evcap = 0;
@@ -177,11 +182,18 @@ static void drop_graphs(oc_engine *h) {
 
 // --------------------------------------------------------------------------- launches
 
-static void launch_pair(oc_engine *h, bool energy) {
+// pair forces (+ bonded terms when bonded, in the same pass), or the pair
+// energies when energy
+static void launch_pair(oc_engine *h, bool energy, bool bonded = false) {
     const int c = h->cur;
     PairArgs A;
     A.ri = h->d_ri;
     A.rs = h->d_rs;
+    A.rsw = h->d_rs;
+    A.mp = h->st[c].mp;
+    A.lnk = h->st[c].lnk;
+    A.prv = h->st[c].prv;
+    A.bonded = bonded && !energy;
     A.tiles = h->d_tiles;
     A.ntiles = (int)h->tiles.size();
     A.per_xcd = h->per_xcd;
@@ -226,46 +238,52 @@ static void launch_stir(oc_engine *h) {
 
 static void launch_forces(oc_engine *h, bool with_bonded) {
     launch_stir(h);
-    launch_pair(h, false);
-    if (with_bonded) {
-        const int c = h->cur;
-        k_bonded<false><<<(int)h->bblk.size(), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy,
-                                                                           h->st[c].gt, h->st[c].mp, h->st[c].lnk,
-                                                                           h->d_f, h->d_ebond, h->d_err);
-    }
+    launch_pair(h, false, with_bonded);
+}
+
+static void launch_build_prv(oc_engine *h) {
+    const int c = h->cur;
+    k_build_prv<<<(int)h->bblk.size(), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->st[c].mp, h->st[c].lnk,
+                                                                  h->st[c].prv);
 }
 
 static void launch_chem(oc_engine *h) {
     if (!h->any_chem) return;
     const int c = h->cur;
-    const int nb = (int)h->bblk.size();
-    k_chem_decide<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->d_cs[c], h->st[c].xy, h->st[c].gt,
-                                                   h->st[c].mp, h->st[c].lnk, h->d_flag, h->d_best);
-    k_chem_apply<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].gt, h->st[c].lnk, h->d_flag,
-                                                  h->d_best, h->d_ev, h->d_evn, h->evcap, h->record, h->d_err);
+    const int nk = (int)h->rblk.size();
+    k_chem_decide<<<nk, kChemThreads, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_rblk, h->d_cs[c], h->st[c].xy,
+                                                      h->st[c].gt, h->st[c].mp, h->st[c].lnk, h->d_flag, h->d_best,
+                                                      h->d_act, h->d_actn);
+    k_chem_apply<<<nk, 64, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_rblk, h->st[c].gt, h->st[c].lnk, h->d_flag,
+                                           h->d_best, h->d_act, h->d_actn, h->d_ev, h->d_evn, h->evcap, h->record,
+                                           h->d_err);
 }
 
-// sort (with integration when integ): keys, scan, gather, permute; flips cur
-static void launch_sort(oc_engine *h, bool integ) {
+// sort (with integration when integ): keys, tile sort, tile offsets,
+// placement; flips cur.  step_delta advances every replica's step counter and
+// per-step values (in k_tscan; no later kernel of the sort reads them).
+static void launch_sort(oc_engine *h, bool integ, int step_delta) {
     const int c = h->cur, n = c ^ 1;
-    const int nb = (int)h->bblk.size();
+    const int nb = (int)h->bblk.size(), nt = (int)h->stiles.size();
     if (integ)
         k_keys<true><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->st[c].xy, h->st[c].v, h->d_f, h->d_key,
-                                                       h->d_count, h->d_bmax);
+                                                       h->d_bmax, h->d_moved);
     else
         k_keys<false><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->st[c].xy, h->st[c].v, h->d_f,
-                                                        h->d_key, h->d_count, h->d_bmax);
-    k_scan_reduce<<<h->nscan, 256, 0, h->stream>>>(h->d_count, h->Ctot, h->d_bsum);
-    k_scan_top<<<1, 1024, 0, h->stream>>>(h->d_bsum, h->nscan, h->d_ri, h->d_rs, h->nrep, h->d_bmax);
-    k_scan_down<<<h->nscan, 256, 0, h->stream>>>(h->d_count, h->Ctot, h->d_bsum, h->d_cs[n]);
-    k_gather<<<(int)h->cblk.size(), kCellBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_cblk, h->d_key, h->d_cs[c],
-                                                               h->d_cs[n], h->d_src, h->d_dest, h->d_err);
+                                                        h->d_key, h->d_bmax, h->d_moved);
+    k_tsort<<<nt, kSortRows, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_cs[c], h->d_key, h->d_bmax, h->d_moved,
+                                             h->d_tcount, h->d_locst, h->d_tsrc, h->d_tslot, h->d_err);
+    k_tscan<<<1, 1024, 0, h->stream>>>(h->d_ri, h->d_rs, h->nrep, h->d_tcount, nt, h->d_toff, h->d_moved,
+                                       step_delta);
+    int32_t *src = h->record_src ? h->d_src : nullptr;
     if (integ)
-        k_permute<true><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->d_src, h->d_dest, h->st[c], h->st[n],
-                                                          h->d_f);
+        k_place<true><<<nt, kSortRows, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,
+                                                       h->d_tsrc, h->d_tslot, h->d_cs[n], src, h->st[c],
+                                                       h->st[n], h->d_f, h->d_err);
     else
-        k_permute<false><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->d_src, h->d_dest, h->st[c], h->st[n],
-                                                           h->d_f);
+        k_place<false><<<nt, kSortRows, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,
+                                                        h->d_tsrc, h->d_tslot, h->d_cs[n], src, h->st[c],
+                                                        h->st[n], h->d_f, h->d_err);
     h->cur = n;
 }
 
@@ -275,8 +293,7 @@ static void launch_step_params(oc_engine *h, int64_t delta, int64_t set_step) {
 
 // one full step of every replica: cel/engine/oercel.c:2300-2362
 static void enqueue_step(oc_engine *h) {
-    launch_step_params(h, 1, -1);
-    launch_sort(h, true);
+    launch_sort(h, true, 1);
     launch_chem(h);
     launch_forces(h, true);
 }
@@ -288,7 +305,7 @@ static int check_err(oc_engine *h) {
     if (e & ERR_EVOVF) h->events_dropped = true;
     if (e & ~ERR_EVOVF) {
         std::string m = "device error:";
-        if (e & ERR_SORT) m += " cell gather count mismatch (sort)";
+        if (e & ERR_SORT) m += " sort tile overflow or bead count mismatch (sort)";
         if (e & ERR_CHAIN) m += " broken lipid chain";
         if (e & ERR_TILE) m += " pair tile window outside its staged rows";
         return fail(OC_EDEVICE, m);
@@ -446,8 +463,21 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
         }
         R.bblk0 = (int32_t)h->bblk.size();
         for (int b = 0; b < grid_of(R.N, kBeadBlock); b++) h->bblk.push_back(r);
-        R.cblk0 = (int32_t)h->cblk.size();
-        for (int b = 0; b < grid_of(R.ncell, kCellBlock); b++) h->cblk.push_back(r);
+        // sort tiles of about 192 beads (at most kSortRows rows)
+        {
+            const double per_row = (double)R.N / ((double)R.ncx * R.ncy);
+            const int rows = std::max(1, std::min(kSortRows, (int)lround(192.0 / std::max(per_row, 1e-9))));
+            const int nt = (R.ncy + rows - 1) / rows;
+            for (int cx = 0; cx < R.ncx; cx++)
+                for (int k = 0; k < nt; k++) {
+                    Tile T;
+                    T.rep = r;
+                    T.cx = cx;
+                    T.k0 = (int)(((int64_t)k * R.ncy) / nt);
+                    T.k1 = (int)(((int64_t)(k + 1) * R.ncy) / nt);
+                    h->stiles.push_back(T);
+                }
+        }
         R.rblk0 = (int32_t)h->rblk.size();
         for (int b = 0; b < grid_of(R.N, kRedChunk); b++) h->rblk.push_back(r);
         h->any_stir |= R.has_stir != 0;
@@ -455,7 +485,10 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     }
     h->Ntot = off;
     h->Ctot = coff;
-    h->nscan = grid_of(h->Ctot, kScanChunk);
+    if ((int64_t)h->stiles.size() << kSortCapLog >= INT32_MAX) {
+        delete h;
+        return fail(OC_EINVAL, "too many sort tiles for 32-bit indices");
+    }
     h->evq.resize(nrep);
     h->evcap = std::min<int64_t>(std::max<int64_t>(1 << 16, 4 * h->Ntot), 8 << 20);
     int rc = 0;
@@ -463,7 +496,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     rc = rc ? rc : dalloc(h, &h->d_ri, nrep);
     rc = rc ? rc : dalloc(h, &h->d_rs, nrep);
     rc = rc ? rc : dalloc(h, &h->d_bblk, h->bblk.size());
-    rc = rc ? rc : dalloc(h, &h->d_cblk, h->cblk.size());
+    rc = rc ? rc : dalloc(h, &h->d_stiles, h->stiles.size());
     rc = rc ? rc : dalloc(h, &h->d_rblk, h->rblk.size());
     for (int k = 0; k < 2; k++) {
         rc = rc ? rc : dalloc(h, &h->st[k].xy, Nt);
@@ -472,14 +505,18 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
         rc = rc ? rc : dalloc(h, &h->st[k].gt, Nt);
         rc = rc ? rc : dalloc(h, &h->st[k].mp, Nt);
         rc = rc ? rc : dalloc(h, &h->st[k].lnk, Nt);
+        rc = rc ? rc : dalloc(h, &h->st[k].prv, Nt);
         rc = rc ? rc : dalloc(h, &h->d_cs[k], Ct + 1);
     }
     rc = rc ? rc : dalloc(h, &h->d_f, Nt);
     rc = rc ? rc : dalloc(h, &h->d_key, Nt);
     rc = rc ? rc : dalloc(h, &h->d_src, Nt);
-    rc = rc ? rc : dalloc(h, &h->d_dest, Nt);
-    rc = rc ? rc : dalloc(h, &h->d_count, Ct);
-    rc = rc ? rc : dalloc(h, &h->d_bsum, h->nscan);
+    rc = rc ? rc : dalloc(h, &h->d_tcount, h->stiles.size());
+    rc = rc ? rc : dalloc(h, &h->d_toff, h->stiles.size());
+    rc = rc ? rc : dalloc(h, &h->d_tsrc, h->stiles.size() << kSortCapLog);
+    rc = rc ? rc : dalloc(h, &h->d_tslot, Nt);
+    rc = rc ? rc : dalloc(h, &h->d_locst, Ct);
+    rc = rc ? rc : dalloc(h, &h->d_moved, 1);
     rc = rc ? rc : dalloc(h, &h->d_bmax, h->bblk.size());
     rc = rc ? rc : dalloc(h, &h->d_epot, Nt);
     rc = rc ? rc : dalloc(h, &h->d_ebond, Nt);
@@ -487,6 +524,8 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     rc = rc ? rc : dalloc(h, &h->d_statout, (size_t)nrep * 16);
     rc = rc ? rc : dalloc(h, &h->d_flag, Nt);
     rc = rc ? rc : dalloc(h, &h->d_best, Nt);
+    rc = rc ? rc : dalloc(h, &h->d_act, h->rblk.size() * (size_t)kRedChunk);
+    rc = rc ? rc : dalloc(h, &h->d_actn, h->rblk.size());
     rc = rc ? rc : dalloc(h, &h->d_ev, (size_t)h->evcap);
     rc = rc ? rc : dalloc(h, &h->d_evn, 1);
     rc = rc ? rc : dalloc(h, &h->d_err, 1);
@@ -506,15 +545,16 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     e = e ? e : hipMemcpy(h->d_ri, h->ri.data(), nrep * sizeof(RepInfo), hipMemcpyHostToDevice);
     e = e ? e : hipMemcpy(h->d_rs, rs.data(), nrep * sizeof(RepStep), hipMemcpyHostToDevice);
     e = e ? e : hipMemcpy(h->d_bblk, h->bblk.data(), h->bblk.size() * 4, hipMemcpyHostToDevice);
-    e = e ? e : hipMemcpy(h->d_cblk, h->cblk.data(), h->cblk.size() * 4, hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(h->d_stiles, h->stiles.data(), h->stiles.size() * sizeof(Tile), hipMemcpyHostToDevice);
     e = e ? e : hipMemcpy(h->d_rblk, h->rblk.data(), h->rblk.size() * 4, hipMemcpyHostToDevice);
-    e = e ? e : hipMemset(h->d_count, 0, Ct * 4);
     e = e ? e : hipMemset(h->d_evn, 0, sizeof(unsigned long long));
     e = e ? e : hipMemset(h->d_err, 0, 4);
+    e = e ? e : hipMemset(h->d_moved, 0, 4);
     e = e ? e : hipMemset(h->d_ebond, 0, Nt * sizeof(double));
     e = e ? e : hipMemset(h->d_epot, 0, Nt * sizeof(double));
     e = e ? e : hipMemset(h->d_f, 0, Nt * sizeof(double2));
     for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(h->st[k].lnk, 0xff, Nt * 4);
+    for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(h->st[k].prv, 0xff, Nt * 4);
     if (e != hipSuccess) {
         oc_destroy(h);
         return fail(OC_EHIP, std::string("setup copy failed: ") + hipGetErrorString(e));
@@ -646,6 +686,7 @@ OC_API int oc_start(oc_engine *h, const oc_state *s) {
     HIPCHK(hipMemcpy(h->d_ri, h->ri.data(), h->nrep * sizeof(RepInfo), hipMemcpyHostToDevice));
     drop_graphs(h);
     h->have_state = true;
+    launch_build_prv(h);
     // forces at the start step, then vh = v - dt/2 f (cel/engine/oercel.c:2203-2211)
     launch_step_params(h, 0, -1);
     launch_forces(h, true);
@@ -659,6 +700,7 @@ OC_API int oc_set_state(oc_engine *h, int32_t r, const oc_state *s, int64_t step
     HIPCHK(hipStreamSynchronize(h->stream));
     if (int rc = load_replica(h, r, s, 0, false)) return rc;
     if (int rc = set_step_fields(h, r, step)) return rc;
+    launch_build_prv(h);
     launch_step_params(h, 0, -1);
     HIPCHK(hipStreamSynchronize(h->stream));
     h->have_state = true;
@@ -735,7 +777,7 @@ OC_API int oc_step(oc_engine *h, int64_t n, int32_t use_graph) {
 OC_API int oc_sort(oc_engine *h) {
     if (!h || !h->have_state) return fail(OC_ESTATE, "no state");
     launch_step_params(h, 0, -1);
-    launch_sort(h, false);
+    launch_sort(h, false, 0);
     HIPCHK(hipGetLastError());
     return check_err(h);
 }
@@ -750,7 +792,7 @@ OC_API int oc_forces(oc_engine *h, int64_t step, int32_t with_bonded) {
 
 OC_API int oc_launch_pair(oc_engine *h, int32_t reps) {
     if (!h || !h->have_state) return fail(OC_ESTATE, "no state");
-    for (int k = 0; k < reps; k++) launch_pair(h, false);
+    for (int k = 0; k < reps; k++) launch_pair(h, false, true);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1125,11 +1167,17 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
         const int n0 = seg * K, n1 = std::min(n0 + K, nsteps), len = n1 - n0;
         // recompute the segment from its checkpoint (bit-identical), recording each state
         if (int rc = snap_copy(h, a->ckpt[seg], false, false)) return rc;
+        launch_build_prv(h);
         if (int rc = snap_copy(h, a->tape[0], true, false)) return rc;
+        h->record_src = true;
         for (int m = 1; m <= len; m++) {
             enqueue_step(h);
-            if (int rc = snap_copy(h, a->tape[m], true, true)) return rc;
+            if (int rc = snap_copy(h, a->tape[m], true, true)) {
+                h->record_src = false;
+                return rc;
+            }
         }
+        h->record_src = false;
         if (seg == nseg - 1) {
             zero_f64(h, (double *)a->X[ab], 2 * N);
             zero_f64(h, (double *)a->V[ab], 2 * N);
@@ -1164,6 +1212,7 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
     k_theta_reduce<<<h->nrep, 256, 0, h->stream>>>(h->d_ri, a->thb, a->grad);
     // the engine continues from the window end
     if (int rc = snap_copy(h, a->end, false, false)) return rc;
+    launch_build_prv(h);
     HIPCHK(hipGetLastError());
     if (int rc = check_err(h)) return rc;
     std::vector<double> g8((size_t)h->nrep * 8);

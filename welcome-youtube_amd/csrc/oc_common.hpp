@@ -14,8 +14,6 @@ namespace oc {
 constexpr int kGidBits = 29;                      // gt = gid | type << 29 (N < 2^29 per replica)
 constexpr uint32_t kGidMask = (1u << kGidBits) - 1u;
 constexpr int kBeadBlock = 256;                   // beads per block of the per-bead kernels
-constexpr int kCellBlock = 256;                   // cells per block of the per-cell kernels
-constexpr int kScanChunk = 2048;                  // cells per block of the scan
 constexpr int kRedChunk = 2048;                   // beads per partial of the stats reduction
 
 // sticky device error bits
@@ -27,7 +25,7 @@ struct RepInfo {
     int32_t ncx, ncy, ncell, coff;  // cells (column-major), first global cell
     int32_t mlen, nH;
     int32_t chem, n_chem;
-    int32_t has_stir, bblk0, cblk0, rblk0;  // stirring on; first block of the bead / cell / reduction tables
+    int32_t has_stir, bblk0, rblk0, pad0;  // stirring on; first block of the bead / reduction tables
     double L, halfL, icsx, icsy, csx, csy;
     double dt, dt2, cvt, hdt;       // dt, dt^2, (1/2 + lambda) dt, dt/2
     double gamma, kT, kb, r0, ka;

@@ -2,17 +2,19 @@
 //
 // Step order (one launch sequence per step for all replicas at once):
 //   k_step_params  per-replica step counter, noise key, warm-up ramp, sigma(t)
-//   k_keys         integrated position -> new cell key, atomic histogram,
-//                  sort window radius (max cell displacement)
-//   k_scan_*       exclusive scan of the histogram -> new cell starts
-//   k_gather       one thread per new cell collects its beads from the old
-//                  cells within the radius in old-slot order: the stable order
-//   k_permute      gather of the state into the new order, integrating on the
-//                  way (the integrate of cel/engine/oercel.c:2303-2334 fused
-//                  into the scatter of sort_cells, :897-921)
+//   k_keys         integrated position -> new cell key, block maxima of the
+//                  cell displacement
+//   k_tsort        per sort tile: the beads of each new cell, collected from
+//                  the old cells within the radius in old-slot order (the
+//                  stable order), numbered within the tile
+//   k_tscan        tile offsets (+ the step's per-replica values)
+//   k_place        new cell starts, and the gather of the state into the new
+//                  order, integrating on the way (the integrate of
+//                  cel/engine/oercel.c:2303-2334 fused into the scatter of
+//                  sort_cells, :897-921)
 //   k_chem_*       precursor chemistry (own definition)
-//   k_pair         full-shell DPD pair forces, LDS-staged tiles
-//   k_bonded       lipid chains and chemistry bonds
+//   k_pair         full-shell DPD pair forces, LDS-staged tiles, with the
+//                  bonded terms of each bead (gather form)
 #include "oc_common.hpp"
 
 namespace oc {
@@ -65,13 +67,10 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 
 // --------------------------------------------------------------------------- per-step parameters
 
-__global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs,
-                                                    int64_t delta, int64_t set_step,
-                                                    const int64_t *__restrict__ base) {
-    const int r = blockIdx.x, k = threadIdx.x;
-    const RepInfo &R = ri[r];
-    RepStep &S = rs[r];
-    // step: base[r] + delta (per-replica base), or set_step, or the current + delta
+// Per-step values of replica r, computed by one wavefront (lane k: a_eff[k]).
+// step = base[r] + delta (per-replica base), or set_step, or the current + delta.
+__device__ __forceinline__ void step_params_wave(const RepInfo &R, RepStep &S, int r, int k, int64_t delta,
+                                                 int64_t set_step, const int64_t *base) {
     const int64_t st = base ? base[r] + delta : (set_step >= 0 ? set_step : S.step + delta);
     // a_eff, cel/engine/oercel.c:1122-1129: one matrix entry per lane
     const double a = R.a[k];
@@ -94,21 +93,26 @@ __global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ 
     S.sdt = (float)(sqrt(2.0 * R.gamma * kTt) / sqrt(R.dt));
     S.gamma = (float)R.gamma;
     S.chem_now = (R.chem && R.n_chem > 0 && (st % R.n_chem) == 0) ? 1 : 0;
-    S.Rx = 0;
-    S.Ry = 0;
+}
+
+__global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs,
+                                                    int64_t delta, int64_t set_step,
+                                                    const int64_t *__restrict__ base) {
+    const int r = blockIdx.x;
+    step_params_wave(ri[r], rs[r], r, threadIdx.x, delta, set_step, base);
 }
 
 // --------------------------------------------------------------------------- sort
 
-// New cell key of every bead (of its integrated position when INTEG) and the
-// histogram; each block stores its largest periodic cell displacement
-// (x, y) in bmax[block] (no same-address atomics), reduced per replica by
-// k_scan_top.
+// New cell key of every bead (of its integrated position when INTEG); each
+// block stores its largest periodic cell displacement (x, y) in bmax[block]
+// (no atomics), and a bead that moved by more than one cell sets *moved (a
+// rare event: the sort then widens its window to the replica's maximum).
 template <bool INTEG>
 __global__ __launch_bounds__(kBeadBlock) void k_keys(const RepInfo *__restrict__ ri, const int32_t *__restrict__ bblk,
                                                      const double2 *__restrict__ xy, const double2 *__restrict__ v,
                                                      const double2 *__restrict__ f, int32_t *__restrict__ key,
-                                                     int32_t *__restrict__ count, int2 *__restrict__ bmax) {
+                                                     int2 *__restrict__ bmax, uint32_t *__restrict__ moved) {
     __shared__ int red[2][kBeadBlock / 64];
     const int r = bblk[blockIdx.x];
     const RepInfo &R = ri[r];
@@ -125,9 +129,7 @@ __global__ __launch_bounds__(kBeadBlock) void k_keys(const RepInfo *__restrict__
             integrate1(R, p, v[s], f[s], q, vo, vt);
         }
         const int cx = cell_x(R, q.x), cy = cell_y(R, q.y);
-        const int c = R.coff + cx * R.ncy + cy;
-        key[s] = c;
-        atomicAdd(&count[c], 1);
+        key[s] = R.coff + cx * R.ncy + cy;
         dx = abs(cx - ox);
         dx = min(dx, R.ncx - dx);
         dy = abs(cy - oy);
@@ -147,89 +149,7 @@ __global__ __launch_bounds__(kBeadBlock) void k_keys(const RepInfo *__restrict__
             my = max(my, red[1][w]);
         }
         bmax[blockIdx.x] = make_int2(mx, my);
-    }
-}
-
-// exclusive scan of count[0..C) -> cstart[0..C]; count is cleared on the way
-__global__ __launch_bounds__(256) void k_scan_reduce(const int32_t *__restrict__ count, int64_t C,
-                                                     int32_t *__restrict__ bsum) {
-    __shared__ int red[4];
-    const int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * 8;
-    int s = 0;
-    for (int k = 0; k < 8; k++)
-        if (base + k < C) s += count[base + k];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) bsum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
-}
-
-// Scan of the block sums (one block), and the per-replica sort window radius
-// from the k_keys block maxima.
-__global__ __launch_bounds__(1024) void k_scan_top(int32_t *__restrict__ bsum, int nb, const RepInfo *__restrict__ ri,
-                                                   RepStep *__restrict__ rs, int nrep,
-                                                   const int2 *__restrict__ bmax) {
-    __shared__ int wsum[16];
-    const int per = (nb + 1023) / 1024;
-    const int b0 = threadIdx.x * per;
-    int loc = 0;
-    for (int k = 0; k < per; k++)
-        if (b0 + k < nb) loc += bsum[b0 + k];
-    const int inc = wave_incl_scan(loc);
-    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = inc;
-    __syncthreads();
-    int woff = 0;
-    for (int w = 0; w < (int)(threadIdx.x >> 6); w++) woff += wsum[w];
-    int run = woff + inc - loc;
-    for (int k = 0; k < per; k++)
-        if (b0 + k < nb) {
-            const int c = bsum[b0 + k];
-            bsum[b0 + k] = run;
-            run += c;
-        }
-    // one wavefront per replica (16 in flight)
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int r = wv; r < nrep; r += 16) {
-        const RepInfo &R = ri[r];
-        const int nblk = (R.N + kBeadBlock - 1) / kBeadBlock;
-        int mx = 0, my = 0;
-        for (int b = lane; b < nblk; b += 64) {
-            const int2 m = bmax[R.bblk0 + b];
-            mx = max(mx, m.x);
-            my = max(my, m.y);
-        }
-        mx = wave_max(mx);
-        my = wave_max(my);
-        if (lane == 0) {
-            rs[r].Rx = mx;
-            rs[r].Ry = my;
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_scan_down(int32_t *__restrict__ count, int64_t C,
-                                                   const int32_t *__restrict__ bsum, int32_t *__restrict__ cstart) {
-    __shared__ int wsum[4];
-    const int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * 8;
-    int c8[8];
-    int loc = 0;
-    for (int k = 0; k < 8; k++) {
-        c8[k] = base + k < C ? count[base + k] : 0;
-        loc += c8[k];
-    }
-    const int inc = wave_incl_scan(loc);
-    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = inc;
-    __syncthreads();
-    int woff = 0;
-    for (int w = 0; w < (int)(threadIdx.x >> 6); w++) woff += wsum[w];
-    int run = bsum[blockIdx.x] + woff + inc - loc;
-    for (int k = 0; k < 8; k++) {
-        if (base + k < C) {
-            cstart[base + k] = run;
-            count[base + k] = 0;
-            run += c8[k];
-            if (base + k == C - 1) cstart[C] = run;
-        }
+        if (mx > 1 || my > 1) atomicOr(moved, 1u);
     }
 }
 
@@ -242,42 +162,301 @@ __device__ __forceinline__ int win_pieces(int c, int R, int n, int *a, int *b) {
     return 1;
 }
 
-// One thread per new cell: scan the old cells within (Rx, Ry) in increasing
-// old-cell index (= increasing old slot) and collect the beads whose key is
-// this cell.  The result is the stable counting-sort order of
-// cel/engine/oercel.c:828-933.
-__global__ __launch_bounds__(kCellBlock) void k_gather(const RepInfo *__restrict__ ri, const RepStep *__restrict__ rs,
-                                                       const int32_t *__restrict__ cblk,
-                                                       const int32_t *__restrict__ key,
-                                                       const int32_t *__restrict__ cs_old,
-                                                       const int32_t *__restrict__ cs_new, int32_t *__restrict__ src,
-                                                       int32_t *__restrict__ dest, uint32_t *__restrict__ err) {
-    const int r = cblk[blockIdx.x];
-    const RepInfo &R = ri[r];
-    const int lc = (blockIdx.x - R.cblk0) * kCellBlock + threadIdx.x;
-    if (lc >= R.ncell) return;
-    const int c = R.coff + lc;
-    int out = cs_new[c];
-    const int end = cs_new[c + 1];
-    if (out == end) return;
-    const int cx = lc / R.ncy, cy = lc - cx * R.ncy;
-    int ca[2], cb[2], ra[2], rb[2];
-    const int nca = win_pieces(cx, rs[r].Rx, R.ncx, ca, cb);
-    const int nra = win_pieces(cy, rs[r].Ry, R.ncy, ra, rb);
-    for (int pc = 0; pc < nca; pc++)
-        for (int col = ca[pc]; col <= cb[pc]; col++) {
-            const int cbase = R.coff + col * R.ncy;
-            for (int pr = 0; pr < nra; pr++) {
-                const int s1 = cs_old[cbase + rb[pr] + 1];
-                for (int s = cs_old[cbase + ra[pr]]; s < s1; s++)
-                    if (key[s] == c) {
-                        src[out] = s;
-                        dest[s] = out;
-                        if (++out == end) return;
-                    }
+// Sort tiles: a column's rows [k0, k1) of new cells (one thread per row).
+// The stable counting sort of cel/engine/oercel.c:828-933 orders the beads
+// of a new cell by old slot; they come from the old cells within the
+// displacement radius.  Visiting those old cells in increasing cell index
+// (= increasing old slot) and keeping the beads whose key is the cell gives
+// that order.  k_tsort does it per tile with the candidates' keys staged in
+// LDS (read from global memory when they do not fit) and numbers the tile's
+// beads; k_tscan turns the tile counts into slot offsets; k_place writes the
+// new cell starts and gathers the state into the new order.  No atomics, so
+// no histogram to clear.  The radius is one cell unless k_keys saw a larger
+// displacement, then the replica's maximum (from the k_keys block maxima).
+constexpr int kSortRows = 256;                // rows (threads) per sort tile
+constexpr int kSortCand = 2048;               // staged candidates
+constexpr int kSortCols = 3;                  // staged columns
+constexpr int kSortMaxRy = 8;                 // staged row margin
+constexpr int kSortRowTab = kSortRows + 2 * kSortMaxRy + 1;
+constexpr int kSortCapLog = 10;
+constexpr int kSortCap = 1 << kSortCapLog;    // beads per sort tile
+
+__device__ __forceinline__ int block_excl_scan256(int v, int *tmp, int &total) {
+    // exclusive scan over a 256-thread block; tmp: 4 ints of LDS
+    const int inc = wave_incl_scan(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) tmp[w] = inc;
+    __syncthreads();
+    int off = 0;
+    for (int k = 0; k < w; k++) off += tmp[k];
+    total = tmp[0] + tmp[1] + tmp[2] + tmp[3];
+    __syncthreads();
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(kSortRows) void k_tsort(const RepInfo *__restrict__ ri, const Tile *__restrict__ tiles,
+                                                     const int32_t *__restrict__ cs_old,
+                                                     const int32_t *__restrict__ key,
+                                                     const int2 *__restrict__ bmax,
+                                                     const uint32_t *__restrict__ moved,
+                                                     int32_t *__restrict__ tcount, int32_t *__restrict__ locst,
+                                                     int32_t *__restrict__ tsrc, int32_t *__restrict__ tslot,
+                                                     uint32_t *__restrict__ err) {
+    __shared__ int32_t s_key[kSortCand];
+    __shared__ int32_t s_cs[kSortCols][kSortRowTab];  // cs_old of the staged rows (start; the last: end)
+    __shared__ int32_t s_c3[kSortCols][3];            // cs_old at ustart, column end, column start
+    __shared__ int32_t s_out[kSortCap];
+    __shared__ int s_tmp[4], s_rad[2][4];
+
+    const Tile T = tiles[blockIdx.x];
+    const RepInfo &R = ri[T.rep];
+    const int ncy = R.ncy, ncx = R.ncx;
+    const int t = threadIdx.x;
+    int Rx = 1, Ry = 1;
+    if (*moved) {
+        // the replica's largest displacement
+        const int nblk = (R.N + kBeadBlock - 1) / kBeadBlock;
+        int mx = 1, my = 1;
+        for (int b = t; b < nblk; b += kSortRows) {
+            const int2 m = bmax[R.bblk0 + b];
+            mx = max(mx, m.x);
+            my = max(my, m.y);
+        }
+        mx = wave_max(mx);
+        my = wave_max(my);
+        if ((t & 63) == 0) {
+            s_rad[0][t >> 6] = mx;
+            s_rad[1][t >> 6] = my;
+        }
+        __syncthreads();
+        Rx = max(max(s_rad[0][0], s_rad[0][1]), max(s_rad[0][2], s_rad[0][3]));
+        Ry = max(max(s_rad[1][0], s_rad[1][1]), max(s_rad[1][2], s_rad[1][3]));
+    }
+    // window geometry (every thread): staged columns in increasing index, rows
+    int ca[2], cb[2];
+    const int npc = win_pieces(T.cx, Rx, ncx, ca, cb);
+    int ncol = 0;
+    for (int q = 0; q < npc; q++) ncol += cb[q] - ca[q] + 1;
+    const int span = T.k1 - T.k0 + 2 * Ry;
+    int ustart, nr;
+    if (span >= ncy) { ustart = 0; nr = ncy; }
+    else { ustart = T.k0 - Ry < 0 ? T.k0 - Ry + ncy : T.k0 - Ry; nr = span; }
+    auto col_of = [&](int q) { return q < cb[0] - ca[0] + 1 ? ca[0] + q : ca[1] + q - (cb[0] - ca[0] + 1); };
+    bool lds = ncol <= kSortCols && nr + 1 <= kSortRowTab;
+    if (lds) {
+        // cs_old of the staged rows: row u is ustart + u (wrapped); entry nr is
+        // the end of the last staged row.  All loads first, then the stores.
+        constexpr int kPer = (kSortRowTab + kSortRows - 1) / kSortRows;
+        int v[kSortCols][kPer];
+#pragma unroll
+        for (int q = 0; q < kSortCols; q++)
+            if (q < ncol) {
+                const int cbq = R.coff + col_of(q) * ncy;
+#pragma unroll
+                for (int i = 0; i < kPer; i++) {
+                    const int u = t + i * kSortRows, k = ustart + u;
+                    if (u <= nr) v[q][i] = cs_old[cbq + (k <= ncy ? k : k - ncy)];
+                }
+            }
+        int w3 = 0;
+        if (t < 3 * ncol) {
+            const int q = t / 3, w = t - 3 * q;
+            w3 = cs_old[R.coff + col_of(q) * ncy + (w == 0 ? ustart : (w == 1 ? ncy : 0))];
+        }
+#pragma unroll
+        for (int q = 0; q < kSortCols; q++)
+            if (q < ncol) {
+#pragma unroll
+                for (int i = 0; i < kPer; i++) {
+                    const int u = t + i * kSortRows;
+                    if (u <= nr) s_cs[q][u] = v[q][i];
+                }
+            }
+        if (t < 3 * ncol) s_c3[t / 3][t % 3] = w3;
+    }
+    __syncthreads();
+    // staged pieces (every thread, registers): column q's rows from ustart to
+    // the column end are LDS [pl[2q], pl[2q+1]) = slots from c3[q][0]; the
+    // wrapped rows from row 0 follow, slots from c3[q][2].  rA/rB: LDS index
+    // of staged row u of column q = rA[q] + s_cs[q][u] (ustart + u <= ncy),
+    // rB[q] + s_cs[q][u] (wrapped).
+    int pf[2 * kSortCols], pl[2 * kSortCols + 1], rA[kSortCols], rB[kSortCols];
+    int nst = 0;
+    if (lds) {
+#pragma unroll
+        for (int q = 0; q < kSortCols; q++) {
+            if (q < ncol) {
+                const int c0 = s_c3[q][0], c1 = s_c3[q][1], c2 = s_c3[q][2], cn = s_cs[q][nr];
+                const bool wr = ustart + nr > ncy;
+                const int n0 = (wr ? c1 : cn) - c0, n1 = wr ? cn - c2 : 0;
+                pf[2 * q] = c0;
+                pl[2 * q] = nst;
+                pf[2 * q + 1] = c2;
+                pl[2 * q + 1] = nst + n0;
+                rA[q] = nst - c0;
+                rB[q] = nst + n0 - c2;
+                nst += n0 + n1;
+            } else {
+                pf[2 * q] = pf[2 * q + 1] = 0;
+                pl[2 * q] = pl[2 * q + 1] = nst;
+                rA[q] = rB[q] = 0;
             }
         }
-    atomicOr(err, ERR_SORT);
+        pl[2 * kSortCols] = nst;
+        lds = nst <= kSortCand;
+    }
+    auto row_lds = [&](int q, int u) {
+        const int b = ustart + u <= ncy ? (q == 0 ? rA[0] : (q == 1 ? rA[1] : rA[2]))
+                                        : (q == 0 ? rB[0] : (q == 1 ? rB[1] : rB[2]));
+        return b + s_cs[q][u];
+    };
+    // slot of staged candidate j: j + (pf - pl) of its piece, as a sum of the
+    // increments at the piece starts (no indexed lookup)
+    int dinc[2 * kSortCols];
+#pragma unroll
+    for (int p = 0; p < 2 * kSortCols; p++) dinc[p] = (pf[p] - pl[p]) - (p ? pf[p - 1] - pl[p - 1] : 0);
+    auto slot_of = [&](int j) {
+        int d = dinc[0];
+#pragma unroll
+        for (int p = 1; p < 2 * kSortCols; p++) d += j >= pl[p] ? dinc[p] : 0;
+        return j + d;
+    };
+    if (lds) {
+        constexpr int kPer = kSortCand / kSortRows;
+        const int nper = (nst + kSortRows - 1) / kSortRows;
+        int v[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; i++)
+            if (i < nper) {
+                const int j = t + i * kSortRows;
+                if (j < nst) v[i] = key[slot_of(j)];
+            }
+#pragma unroll
+        for (int i = 0; i < kPer; i++)
+            if (i < nper) {
+                const int j = t + i * kSortRows;
+                if (j < nst) s_key[j] = v[i];
+            }
+    }
+    __syncthreads();
+
+    // visit the candidates of this thread's cell in old-slot order; fn(id) with
+    // id an LDS index (staged) or a slot (global)
+    const int row = T.k0 + t;
+    const bool mine = row < T.k1;
+    const int c = R.coff + T.cx * ncy + row;
+    auto visit = [&](auto fn) {
+        int ra[2], rb[2];
+        const int npr = win_pieces(row, Ry, ncy, ra, rb);
+        if (lds) {
+#pragma unroll
+            for (int q = 0; q < kSortCols; q++) {
+                if (q >= ncol) break;
+#pragma unroll
+                for (int pr = 0; pr < 2; pr++) {
+                    if (pr >= npr) break;
+                    const int ua = ra[pr] - ustart < 0 ? ra[pr] - ustart + ncy : ra[pr] - ustart;
+                    const int ub = rb[pr] - ustart < 0 ? rb[pr] - ustart + ncy : rb[pr] - ustart;
+                    const int j1 = row_lds(q, ub + 1);
+                    for (int j = row_lds(q, ua); j < j1; j++)
+                        if (s_key[j] == c) fn(j);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int pc = 0; pc < 2; pc++) {
+                if (pc >= npc) break;
+                for (int col = ca[pc]; col <= cb[pc]; col++) {
+                    const int cbq = R.coff + col * ncy;
+#pragma unroll
+                    for (int pr = 0; pr < 2; pr++) {
+                        if (pr >= npr) break;
+                        const int j1 = cs_old[cbq + rb[pr] + 1];
+                        for (int j = cs_old[cbq + ra[pr]]; j < j1; j++)
+                            if (key[j] == c) fn(j);
+                    }
+                }
+            }
+        }
+    };
+    int cnt = 0;
+    if (mine) visit([&](int) { cnt++; });
+    int total;
+    const int start = block_excl_scan256(cnt, s_tmp, total);
+    if (total > kSortCap) {
+        // reported as an error; the tile places no bead, so every later
+        // index stays in bounds
+        if (mine) locst[c] = 0;
+        if (t == 0) {
+            tcount[blockIdx.x] = 0;
+            atomicOr(err, ERR_SORT);
+        }
+        return;
+    }
+    if (mine) {
+        int pos = start;
+        visit([&](int j) { s_out[pos++] = j; });
+        locst[c] = start;
+    }
+    __syncthreads();
+    const int tb = blockIdx.x << kSortCapLog;
+    for (int p = t; p < total; p += kSortRows) {
+        const int j = s_out[p];
+        const int sl = lds ? slot_of(j) : j;
+        tsrc[tb + p] = sl;
+        tslot[sl] = tb + p;
+    }
+    if (t == 0) tcount[blockIdx.x] = total;
+}
+
+// Exclusive scan of the tile counts -> first new slot of each tile (tiles are
+// in cell order, cells in slot order); clears the displacement flag; and,
+// when step_delta > 0, the next step's per-replica values (the step counter
+// advance of the step sequence).  One block.
+__global__ __launch_bounds__(1024) void k_tscan(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs, int nrep,
+                                                const int32_t *__restrict__ tcount, int nt,
+                                                int32_t *__restrict__ toff, uint32_t *__restrict__ moved,
+                                                int step_delta) {
+    __shared__ int wsum[16];
+    const int per = (nt + 1023) / 1024;
+    const int b0 = threadIdx.x * per;
+    int c8[8];
+    int loc = 0;
+    if (per <= 8) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            c8[k] = k < per && b0 + k < nt ? tcount[b0 + k] : 0;
+            loc += c8[k];
+        }
+    } else {
+        for (int k = 0; k < per; k++)
+            if (b0 + k < nt) loc += tcount[b0 + k];
+    }
+    const int inc = wave_incl_scan(loc);
+    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); w++) woff += wsum[w];
+    int run = woff + inc - loc;
+    if (per <= 8) {
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (k < per && b0 + k < nt) {
+                toff[b0 + k] = run;
+                run += c8[k];
+            }
+    } else {
+        for (int k = 0; k < per; k++)
+            if (b0 + k < nt) {
+                toff[b0 + k] = run;
+                run += tcount[b0 + k];
+            }
+    }
+    if (threadIdx.x == 0) *moved = 0u;
+    if (step_delta) {
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int r = wv; r < nrep; r += 16) step_params_wave(ri[r], rs[r], r, lane, step_delta, -1, nullptr);
+    }
 }
 
 struct StateP {
@@ -285,39 +464,87 @@ struct StateP {
     float2 *vt;
     uint32_t *gt;
     int8_t *mp;
-    int32_t *lnk;
+    int32_t *lnk;  // lipid bead: next bead of the chain; other bead: chemistry partner; -1: none
+    int32_t *prv;  // lipid bead: previous bead of the chain; -1 otherwise (derived from lnk)
 };
 
-// Gather into the new slot order; when INTEG the bead is integrated on the way
-// (same arithmetic as k_keys, so the new position is the one that was keyed).
+// New cell starts of the tile's rows, and the gather of its beads into the new
+// order (one block per sort tile; src[new slot] = old slot when src is given,
+// for the adjoint's tape); when INTEG each bead is integrated on the
+// way (same arithmetic as k_keys, so the new position is the one that was
+// keyed): the integrate of cel/engine/oercel.c:2303-2334 fused into the
+// scatter of sort_cells, :897-921.
 template <bool INTEG>
-__global__ __launch_bounds__(kBeadBlock) void k_permute(const RepInfo *__restrict__ ri, const int32_t *__restrict__ bblk,
-                                                        const int32_t *__restrict__ src,
-                                                        const int32_t *__restrict__ dest, StateP in, StateP out,
-                                                        const double2 *__restrict__ f) {
+__global__ __launch_bounds__(kSortRows) void k_place(const RepInfo *__restrict__ ri, const Tile *__restrict__ tiles,
+                                                     const int32_t *__restrict__ tcount,
+                                                     const int32_t *__restrict__ toff,
+                                                     const int32_t *__restrict__ locst,
+                                                     const int32_t *__restrict__ tsrc,
+                                                     const int32_t *__restrict__ tslot, int32_t *__restrict__ cs_new,
+                                                     int32_t *__restrict__ src, StateP in, StateP out,
+                                                     const double2 *__restrict__ f, uint32_t *__restrict__ err) {
+    const Tile T = tiles[blockIdx.x];
+    const RepInfo &R = ri[T.rep];
+    const int t = threadIdx.x;
+    const int base = toff[blockIdx.x], n = tcount[blockIdx.x];
+    const int row = T.k0 + t;
+    if (row < T.k1) {
+        const int c = R.coff + T.cx * R.ncy + row;
+        cs_new[c] = base + locst[c];
+        if (T.cx == R.ncx - 1 && row == R.ncy - 1) {
+            // the replica's last tile: its beads end at the replica's last slot
+            cs_new[c + 1] = R.off + R.N;
+            if (base + n != R.off + R.N) atomicOr(err, ERR_SORT);
+        }
+    }
+    const int tb = blockIdx.x << kSortCapLog;
+    for (int p = t; p < n; p += kSortRows) {
+        const int d = base + p;
+        const int s = tsrc[tb + p];
+        double2 q = in.xy[s], vv = in.v[s];
+        float2 vt;
+        if (INTEG) {
+            double2 po, vo;
+            integrate1(R, q, vv, f[s], po, vo, vt);
+            q = po;
+            vv = vo;
+        } else {
+            vt = in.vt[s];
+        }
+        out.xy[d] = q;
+        out.v[d] = vv;
+        out.vt[d] = vt;
+        out.gt[d] = in.gt[s];
+        if (src) src[d] = s;
+        const int8_t m = in.mp[s];
+        out.mp[d] = m;
+        const int32_t l = in.lnk[s];
+        int32_t nl = -1;
+        if (l >= 0) {
+            const int ts = tslot[l];
+            nl = toff[ts >> kSortCapLog] + (ts & (kSortCap - 1));
+        }
+        out.lnk[d] = nl;
+        // every bead's prv is written once: by itself (chain head, non-lipid) or
+        // by its chain predecessor
+        if (m <= 0) out.prv[d] = -1;
+        if (m >= 0 && nl >= 0) out.prv[nl] = d;
+    }
+}
+
+// prv from lnk (after a state is loaded or restored), the rule of k_place
+__global__ __launch_bounds__(kBeadBlock) void k_build_prv(const RepInfo *__restrict__ ri,
+                                                          const int32_t *__restrict__ bblk,
+                                                          const int8_t *__restrict__ mp,
+                                                          const int32_t *__restrict__ lnk, int32_t *__restrict__ prv) {
     const int r = bblk[blockIdx.x];
     const RepInfo &R = ri[r];
     const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
     if (loc >= R.N) return;
-    const int d = R.off + loc;
-    const int s = src[d];
-    double2 p = in.xy[s], vv = in.v[s];
-    float2 vt;
-    if (INTEG) {
-        double2 po, vo;
-        integrate1(R, p, vv, f[s], po, vo, vt);
-        p = po;
-        vv = vo;
-    } else {
-        vt = in.vt[s];
-    }
-    out.xy[d] = p;
-    out.v[d] = vv;
-    out.vt[d] = vt;
-    out.gt[d] = in.gt[s];
-    out.mp[d] = in.mp[s];
-    const int32_t l = in.lnk[s];
-    out.lnk[d] = l >= 0 ? dest[l] : -1;
+    const int s = R.off + loc;
+    const int m = mp[s], l = lnk[s];
+    if (m <= 0) prv[s] = -1;
+    if (m >= 0 && l >= 0) prv[l] = s;
 }
 
 // --------------------------------------------------------------------------- pair forces
@@ -350,9 +577,108 @@ __device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float 
     }
 }
 
+// harmonic bond i -> j (d = minimum image of p_j - p_i), the arithmetic of
+// cel/engine/oercel.c:1280-1289: force on j is +fb*d, on i -fb*d
+__device__ __forceinline__ void bond_vec(const RepInfo &R, double2 pi, double2 pj, double &dx, double &dy, double &r) {
+    dx = minimg(pj.x - pi.x, R.L, R.halfL);
+    dy = minimg(pj.y - pi.y, R.L, R.halfL);
+    r = sqrt(dx * dx + dy * dy);
+}
+
+// Bonded force on bead s in gather form: every bond and angle term that s
+// takes part in is evaluated whole, with the arithmetic of
+// chain_forces_linear (cel/engine/oercel.c:1267-1308), and s adds its share,
+// bonds along the chain first, then angles -- the order in which a per-chain
+// pass adds them to each bead, so the result is the same bit for bit and the
+// two beads of a bond receive exactly opposite forces.  A chemistry bond is
+// the bond from the smaller gid to the larger.  mb: largest bond length at s.
+__device__ __forceinline__ void bonded_gather(const RepInfo &R, int s, double2 ps, const double2 *__restrict__ xy,
+                                              const uint32_t *__restrict__ gt, const int8_t *__restrict__ mp,
+                                              const int32_t *__restrict__ lnk, const int32_t *__restrict__ prv,
+                                              double &fx, double &fy, double &mb) {
+    const int m = mp[s];
+    const int n1 = lnk[s];
+    if (m < 0) {
+        if (n1 < 0) return;
+        const double2 pn = xy[n1];
+        const bool lo = (gt[s] & kGidMask) < (gt[n1] & kGidMask);
+        double dx, dy, r;
+        bond_vec(R, lo ? ps : pn, lo ? pn : ps, dx, dy, r);
+        mb = fmax(mb, r);
+        const double fb = -R.kb * (r - R.r0) / r;
+        if (lo) {
+            fx -= fb * dx;
+            fy -= fb * dy;
+        } else {
+            fx += fb * dx;
+            fy += fb * dy;
+        }
+        return;
+    }
+    // chain beads at offsets -2..+2 (index 0..4, s at 2); bond b joins c[b] -> c[b+1]
+    int c[5];
+    c[2] = s;
+    c[1] = prv[s];
+    c[3] = n1;
+    const bool ang = R.ka != 0.0;
+    c[0] = (ang && c[1] >= 0) ? prv[c[1]] : -1;
+    c[4] = (ang && c[3] >= 0) ? lnk[c[3]] : -1;
+    double2 p[5];
+    p[2] = ps;
+    for (int k = 0; k < 5; k++)
+        if (k != 2 && c[k] >= 0) p[k] = xy[c[k]];
+    double bx[4], by[4], bl[4];
+    bool has[4];
+    for (int b = 0; b < 4; b++) {
+        has[b] = c[b] >= 0 && c[b + 1] >= 0;
+        if (has[b]) bond_vec(R, p[b], p[b + 1], bx[b], by[b], bl[b]);
+    }
+    if (has[1]) {  // s is the second bead of bond 1
+        mb = fmax(mb, bl[1]);
+        const double fb = -R.kb * (bl[1] - R.r0) / bl[1];
+        fx += fb * bx[1];
+        fy += fb * by[1];
+    }
+    if (has[2]) {  // s is the first bead of bond 2
+        mb = fmax(mb, bl[2]);
+        const double fb = -R.kb * (bl[2] - R.r0) / bl[2];
+        fx -= fb * bx[2];
+        fy -= fb * by[2];
+    }
+    if (!ang) return;
+    const double ka = R.ka;
+    // angle a uses bonds a, a+1 (beads c[a], c[a+1], c[a+2]); s is its third,
+    // second, first bead for a = 0, 1, 2
+    for (int a = 0; a < 3; a++) {
+        if (!has[a] || !has[a + 1]) continue;
+        const double ax = bx[a], ay = by[a], cxv = bx[a + 1], cyv = by[a + 1];
+        const double la = bl[a], lc = bl[a + 1];
+        const double ilac = 1.0 / (la * lc);
+        const double csn = (ax * cxv + ay * cyv) * ilac;
+        const double dUax = -ka * (cxv * ilac - csn * ax / (la * la));
+        const double dUay = -ka * (cyv * ilac - csn * ay / (la * la));
+        const double dUcx = -ka * (ax * ilac - csn * cxv / (lc * lc));
+        const double dUcy = -ka * (ay * ilac - csn * cyv / (lc * lc));
+        if (a == 0) {
+            fx -= dUcx;
+            fy -= dUcy;
+        } else if (a == 1) {
+            fx += -dUax + dUcx;
+            fy += -dUay + dUcy;
+        } else {
+            fx += dUax;
+            fy += dUay;
+        }
+    }
+}
+
 struct PairArgs {
     const RepInfo *ri;
     const RepStep *rs;
+    RepStep *rsw;        // running max bond length (maxbond_bits)
+    const int8_t *mp;    // bonded terms (gather form) when bonded != 0
+    const int32_t *lnk, *prv;
+    int bonded;
     const Tile *tiles;
     int ntiles, per_xcd;
     const int32_t *cs;
@@ -476,6 +802,19 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     // candidates overflow the LDS buffer (a dense clump) run the single pass on
     // global memory, with s_row offsets rebased per piece.  The next
     // candidate's position is loaded one iteration ahead.
+    double mb = 0.0;  // largest bond length seen by this lane
+    // per-bead epilogue: energy, or force (+ stirring, + bonded terms)
+    auto finish = [&](int slot, double2 pi, double ax, double ay, float e) {
+        if (ENERGY) {
+            A.epot[slot] = (double)e;
+        } else {
+            double bx = 0.0;
+            if (A.stir) bx = A.stir[slot] - S.stir_mean;
+            double fx = bx + ax, fy = ay;
+            if (A.bonded) bonded_gather(R, slot, pi, A.xy, A.gt, A.mp, A.lnk, A.prv, fx, fy, mb);
+            A.f[slot] = make_double2(fx, fy);
+        }
+    };
     auto run = [&](const double2 *XY, const float2 *VT, const uint32_t *GT, const bool stg) {
         const bool lists = CAPL > 0 && stg;
         for (int ii = ibeg + threadIdx.x; ii < iend; ii += BLOCK) {
@@ -496,6 +835,32 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             double ax = 0.0, ay = 0.0;
             float e = 0.0f;
             int cnt = 0;
+            // Flat single pass (staged, CAPL == 0, no periodic image anywhere in
+            // the wavefront's windows): a column's window rows are one
+            // contiguous LDS range (its staged pieces are concatenated in
+            // unwrapped row order), and the lane runs through its three column
+            // ranges as one sequence, so the wavefront iterates ~max over lanes
+            // of the whole window rather than the sum over columns of the
+            // per-column maxima.  The three ranges lie in increasing LDS order,
+            // so stepping past the end of one range lands on the next by two
+            // selects.
+            const bool img = sxc[0] != 0.0 || sxc[2] != 0.0 || klo < 0 || khi >= ncy;
+            if (stg && CAPL == 0 && !__any(img)) {
+                const int st0 = s_row[0][klo - kb0], en0 = s_row[0][khi + 1 - kb0];
+                const int st1 = s_row[1][klo - kb0], en1 = s_row[1][khi + 1 - kb0];
+                const int st2 = s_row[2][klo - kb0], en2 = s_row[2][khi + 1 - kb0];
+                int j = st0 == en0 ? st1 : st0;
+                j = j == en1 ? st2 : j;
+                while (j < en2) {
+                    pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], 0.0, 0.0, gam, sdt, sseed, ax, ay,
+                             e, ENERGY);
+                    ++j;
+                    j = j == en0 ? st1 : j;
+                    j = j == en1 ? st2 : j;
+                }
+                finish(slot, pi, ax, ay, e);
+                continue;
+            }
             for (int c = 0; c < 3; c++) {
                 // window split at the periodic seams: rows < 0 (image -L, staged piece 0),
                 // 0..ncy-1 (piece 1), >= ncy (image +L, piece 2)
@@ -557,17 +922,24 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                              e, ENERGY);
                 }
             }
-            if (ENERGY) {
-                A.epot[slot] = (double)e;
-            } else {
-                double bx = 0.0;
-                if (A.stir) bx = A.stir[slot] - S.stir_mean;
-                A.f[slot] = make_double2(bx + ax, ay);
-            }
+            finish(slot, pi, ax, ay, e);
         }
     };
     if (staged) run(s_xy, s_vt, s_gt, true);
     else run(A.xy, A.vt, A.gt, false);
+    if (ENERGY || !A.bonded) return;
+    // running max bond length of the replica: block max, then an atomic only
+    // when it exceeds the stored value (positive doubles order as their bits)
+    uint64_t bits = (uint64_t)__double_as_longlong(mb);
+    bits = wave_max_u64(bits);
+    __shared__ uint64_t s_mb[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) s_mb[threadIdx.x >> 6] = bits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BLOCK / 64; w++) bits = max(bits, s_mb[w]);
+        unsigned long long *mbw = (unsigned long long *)&A.rsw[T.rep].maxbond_bits;
+        if (bits > *(volatile unsigned long long *)mbw) atomicMax(mbw, (unsigned long long)bits);
+    }
 }
 
 // --------------------------------------------------------------------------- bonded forces
@@ -662,14 +1034,13 @@ __global__ __launch_bounds__(kBeadBlock) void k_bonded(const RepInfo *__restrict
 
 // --------------------------------------------------------------------------- precursor chemistry
 
-// Visit the beads within the cell window of bead i (cutoff <= 1) until fn
-// returns false.  Within one column the window's rows are contiguous cells,
-// hence one contiguous slot range per periodic piece (at most two per column);
-// the slot loop is unrolled so independent candidate loads overlap.
-// Distances: FP64 minimum image (the oracle's arithmetic).
+// Visit the candidate beads of the cell window of bead i (cutoff <= 1) until
+// fn(j) returns false; fn loads what it needs (type first, position last).
+// Within one column the window's rows are contiguous cells, hence one
+// contiguous slot range per periodic piece (at most two per column); the slot
+// loop is unrolled so independent candidate loads overlap.
 template <typename Fn>
-__device__ __forceinline__ void visit_nbrs(const RepInfo &R, const int32_t *cs, const double2 *xy, int i, double2 p,
-                                           Fn fn) {
+__device__ __forceinline__ void visit_cands(const RepInfo &R, const int32_t *cs, int i, double2 p, Fn fn) {
     const int cx = cell_x(R, p.x);
     const int khi = (int)((p.y + 1.0) * R.icsy);
     const int klo = (int)((p.y - 1.0) * R.icsy + R.ncy) - R.ncy;
@@ -688,63 +1059,168 @@ __device__ __forceinline__ void visit_nbrs(const RepInfo &R, const int32_t *cs, 
 #pragma unroll 4
             for (int j = j0; j < j1; j++) {
                 if (j == i) continue;
-                const double2 qq = xy[j];
-                const double dx = minimg(qq.x - p.x, R.L, R.halfL), dy = minimg(qq.y - p.y, R.L, R.halfL);
-                if (!fn(j, dx * dx + dy * dy)) return;
+                if (!fn(j)) return;
             }
         }
     }
 }
 
-enum : uint8_t { CF_BREAK = 1, CF_CONV = 2 };
+// FP64 minimum-image squared distance (the oracle's arithmetic)
+__device__ __forceinline__ double dist2(const RepInfo &R, double2 p, double2 q) {
+    const double dx = minimg(q.x - p.x, R.L, R.halfL), dy = minimg(q.y - p.y, R.L, R.halfL);
+    return dx * dx + dy * dy;
+}
 
-// Decisions from the state at the start of the check (own definition):
-// break of chemistry bonds, P -> T conversion next to a lipid tail, and the
-// bond-formation proposal of every free T.
-__global__ __launch_bounds__(kBeadBlock) void k_chem_decide(const RepInfo *__restrict__ ri,
-                                                            const RepStep *__restrict__ rs,
-                                                            const int32_t *__restrict__ bblk,
-                                                            const int32_t *__restrict__ cs,
-                                                            const double2 *__restrict__ xy,
-                                                            const uint32_t *__restrict__ gt,
-                                                            const int8_t *__restrict__ mp,
-                                                            const int32_t *__restrict__ lnk,
-                                                            uint8_t *__restrict__ flag, int32_t *__restrict__ best) {
-    const int r = bblk[blockIdx.x];
+// The cell window of a point (cutoff <= 1) as up to six contiguous slot
+// ranges: per column (x-1, x, x+1) the rows of [y-1, y+1], split at the
+// periodic seam.  slot(k) is the k-th candidate in that order.
+struct Win {
+    int first[6], cnt[6], total;
+    __device__ __forceinline__ int slot(int k) const {
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            if (k < cnt[q]) return first[q] + k;
+            k -= cnt[q];
+        }
+        return -1;
+    }
+};
+
+__device__ __forceinline__ Win window_of(const RepInfo &R, const int32_t *cs, double2 p) {
+    Win W;
+    const int cx = cell_x(R, p.x);
+    const int khi = (int)((p.y + 1.0) * R.icsy);
+    const int klo = (int)((p.y - 1.0) * R.icsy + R.ncy) - R.ncy;
+    W.total = 0;
+#pragma unroll
+    for (int dc = 0; dc < 3; dc++) {
+        int col = cx + dc - 1;
+        col = col < 0 ? col + R.ncx : (col >= R.ncx ? col - R.ncx : col);
+        const int cb = R.coff + col * R.ncy;
+        // the window spans fewer rows than the column (L >= 8): at most one seam
+        int a0, b0, a1 = 0, b1 = -1;
+        if (klo < 0) { a0 = klo + R.ncy; b0 = R.ncy - 1; a1 = 0; b1 = khi; }
+        else if (khi >= R.ncy) { a0 = klo; b0 = R.ncy - 1; a1 = 0; b1 = khi - R.ncy; }
+        else { a0 = klo; b0 = khi; }
+        W.first[2 * dc] = cs[cb + a0];
+        W.cnt[2 * dc] = cs[cb + b0 + 1] - W.first[2 * dc];
+        W.first[2 * dc + 1] = cs[cb + a1];
+        W.cnt[2 * dc + 1] = b1 >= a1 ? cs[cb + b1 + 1] - W.first[2 * dc + 1] : 0;
+        W.total += W.cnt[2 * dc] + W.cnt[2 * dc + 1];
+    }
+    return W;
+}
+
+enum : uint8_t { CF_BREAK = 1, CF_CONV = 2 };
+constexpr int kChemThreads = 1024;
+constexpr int kChemGroup = 16;
+
+// Decisions from the state at the start of the check (own definition): break
+// of chemistry bonds, P -> T conversion next to a lipid tail, and the
+// bond-formation proposal of every free T.  One block per reduction chunk
+// (kRedChunk beads of one replica): every bead is classified (break draw of a
+// chemistry bond, conversion draw of a P, free T), the beads that need a
+// neighbour search are compacted in LDS and searched by the block's threads
+// densely, and the chunk's beads with a decision are listed (act, actn) for
+// k_chem_apply.  flag/best are written for the searched and flagged beads
+// only; k_chem_apply reads nothing else.
+__global__ __launch_bounds__(kChemThreads) void k_chem_decide(const RepInfo *__restrict__ ri,
+                                                              const RepStep *__restrict__ rs,
+                                                              const int32_t *__restrict__ rblk,
+                                                              const int32_t *__restrict__ cs,
+                                                              const double2 *__restrict__ xy,
+                                                              const uint32_t *__restrict__ gt,
+                                                              const int8_t *__restrict__ mp,
+                                                              const int32_t *__restrict__ lnk,
+                                                              uint8_t *__restrict__ flag, int32_t *__restrict__ best,
+                                                              int32_t *__restrict__ act, int32_t *__restrict__ actn) {
+    __shared__ int32_t s_srch[kRedChunk];
+    __shared__ int32_t s_act[kRedChunk];
+    __shared__ int s_n, s_na;
+    const int r = rblk[blockIdx.x];
     const RepStep &S = rs[r];
     if (!S.chem_now) return;
     const RepInfo &R = ri[r];
-    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
-    if (loc >= R.N) return;
-    const int s = R.off + loc;
-    const uint32_t g = gt[s] & kGidMask, ty = gt[s] >> kGidBits;
-    const int m = mp[s], l = lnk[s];
-    uint8_t fl = 0;
-    int32_t b = -1;
-    if (m < 0 && l >= 0 && g < (gt[l] & kGidMask) && hash_u01(S.kbreak, g) < R.p_break) fl |= CF_BREAK;
-    if (ty == OC_N && hash_u01(S.kconv, g) < R.p_conv) {
-        bool found = false;
-        visit_nbrs(R, cs, xy, s, xy[s], [&](int j, double r2) {
-            if ((gt[j] >> kGidBits) == OC_T && mp[j] >= 0 && r2 < R.rc2) { found = true; return false; }
-            return true;
-        });
-        if (found) fl |= CF_CONV;
+    const int b0 = (blockIdx.x - R.rblk0) * kRedChunk;
+    if (threadIdx.x == 0) {
+        s_n = 0;
+        s_na = 0;
     }
-    if (ty == OC_T && m < 0 && l < 0) {
-        double bu = 2.0;
+    __syncthreads();
+    for (int k = 0; k < kRedChunk / kChemThreads; k++) {
+        const int loc = b0 + k * kChemThreads + threadIdx.x;
+        if (loc >= R.N) break;
+        const int s = R.off + loc;
+        const uint32_t w = gt[s], g = w & kGidMask, ty = w >> kGidBits;
+        const int m = mp[s], l = lnk[s];
+        const bool brk = m < 0 && l >= 0 && g < (gt[l] & kGidMask) && hash_u01(S.kbreak, g) < R.p_break;
+        const bool conv = ty == OC_N && hash_u01(S.kconv, g) < R.p_conv;
+        const bool form = ty == OC_T && m < 0 && l < 0;
+        if (conv || form) {
+            s_srch[atomicAdd(&s_n, 1)] = s | (brk ? (int32_t)0x80000000 : 0);
+        } else if (brk) {
+            flag[s] = CF_BREAK;
+            best[s] = -1;
+            s_act[atomicAdd(&s_na, 1)] = s;
+        }
+    }
+    __syncthreads();
+    const int ns = s_n;
+    // kChemGroup lanes per searcher: the window's candidates are dealt round
+    // robin to the lanes, then reduced over the group (any lipid tail in
+    // range; the smallest accepted draw, ties to the smaller gid)
+    const int gl = threadIdx.x & (kChemGroup - 1);
+    for (int e = threadIdx.x / kChemGroup; e < ns; e += kChemThreads / kChemGroup) {
+        const int s = s_srch[e] & 0x7fffffff;
+        const uint32_t g = gt[s] & kGidMask, ty = gt[s] >> kGidBits;
+        const double2 p = xy[s];
+        const Win W = window_of(R, cs, p);
+        bool found = false;
+        uint64_t bkey = ~0ull;  // (bits of u) of the best accepted draw
         uint32_t bg = 0xffffffffu;
-        visit_nbrs(R, cs, xy, s, xy[s], [&](int j, double r2) {
+        int32_t bsel = -1;
+        for (int k = gl; k < W.total; k += kChemGroup) {
+            const int j = W.slot(k);
+            if (j == s) continue;
             const uint32_t gj = gt[j];
-            if ((gj >> kGidBits) == OC_T && mp[j] < 0 && lnk[j] < 0 && r2 < R.rf2) {
+            if ((gj >> kGidBits) != OC_T) continue;
+            if (ty == OC_N) {
+                // a lipid tail bead within r_conv
+                if (mp[j] >= 0 && dist2(R, p, xy[j]) < R.rc2) found = true;
+            } else if (mp[j] < 0 && lnk[j] < 0 && dist2(R, p, xy[j]) < R.rf2) {
+                // a free T within r_form; draws in [0,1) order as their bits
                 const uint32_t gg = gj & kGidMask;
                 const double u = pair_u01(S.kform, g, gg);
-                if (u < R.p_form && (u < bu || (u == bu && gg < bg))) { bu = u; bg = gg; b = j; }
+                const uint64_t ku = (uint64_t)__double_as_longlong(u);
+                if (u < R.p_form && (ku < bkey || (ku == bkey && gg < bg))) {
+                    bkey = ku;
+                    bg = gg;
+                    bsel = j;
+                }
             }
-            return true;
-        });
+        }
+        for (int o = kChemGroup / 2; o > 0; o >>= 1) {
+            found |= __shfl_xor((int)found, o, kChemGroup) != 0;
+            const uint64_t ok = __shfl_xor(bkey, o, kChemGroup);
+            const uint32_t og = __shfl_xor(bg, o, kChemGroup);
+            const int32_t oj = __shfl_xor(bsel, o, kChemGroup);
+            if (ok < bkey || (ok == bkey && og < bg)) {
+                bkey = ok;
+                bg = og;
+                bsel = oj;
+            }
+        }
+        if (gl == 0) {
+            const uint8_t fl = (s_srch[e] < 0 ? CF_BREAK : 0) | (found ? CF_CONV : 0);
+            flag[s] = fl;
+            best[s] = bsel;
+            if (fl || bsel >= 0) s_act[atomicAdd(&s_na, 1)] = s;
+        }
     }
-    flag[s] = fl;
-    best[s] = b;
+    __syncthreads();
+    const int na = s_na;
+    for (int e = threadIdx.x; e < na; e += kChemThreads) act[(size_t)blockIdx.x * kRedChunk + e] = s_act[e];
+    if (threadIdx.x == 0) actn[blockIdx.x] = na;
 }
 
 __device__ __forceinline__ void push_event(Event *ev, unsigned long long *evn, int64_t cap, uint32_t *err, int64_t step,
@@ -759,40 +1235,43 @@ __device__ __forceinline__ void push_event(Event *ev, unsigned long long *evn, i
     }
 }
 
-__global__ __launch_bounds__(kBeadBlock) void k_chem_apply(const RepInfo *__restrict__ ri,
-                                                           const RepStep *__restrict__ rs,
-                                                           const int32_t *__restrict__ bblk, uint32_t *__restrict__ gt,
-                                                           int32_t *__restrict__ lnk, const uint8_t *__restrict__ flag,
-                                                           const int32_t *__restrict__ best, Event *__restrict__ ev,
-                                                           unsigned long long *__restrict__ evn, int64_t evcap,
-                                                           int record, uint32_t *__restrict__ err) {
-    const int r = bblk[blockIdx.x];
+// Applies the decisions of the listed beads of one chunk.  A proposal forms a
+// bond when it is mutual (best[best[s]] == s; best of a free T is always
+// written by k_chem_decide of the same check).
+__global__ __launch_bounds__(64) void k_chem_apply(const RepInfo *__restrict__ ri, const RepStep *__restrict__ rs,
+                                                   const int32_t *__restrict__ rblk, uint32_t *__restrict__ gt,
+                                                   int32_t *__restrict__ lnk, const uint8_t *__restrict__ flag,
+                                                   const int32_t *__restrict__ best, const int32_t *__restrict__ act,
+                                                   const int32_t *__restrict__ actn, Event *__restrict__ ev,
+                                                   unsigned long long *__restrict__ evn, int64_t evcap, int record,
+                                                   uint32_t *__restrict__ err) {
+    const int r = rblk[blockIdx.x];
     const RepStep &S = rs[r];
     if (!S.chem_now) return;
-    const RepInfo &R = ri[r];
-    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
-    if (loc >= R.N) return;
-    const int s = R.off + loc;
-    const uint32_t g = gt[s] & kGidMask;
-    const uint8_t fl = flag[s];
-    if (fl & CF_BREAK) {
-        const int p = lnk[s];
-        const uint32_t gp = gt[p] & kGidMask;
-        lnk[s] = -1;
-        lnk[p] = -1;
-        if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_BREAK, min(g, gp), max(g, gp), r);
-    }
-    if (fl & CF_CONV) {
-        gt[s] = g | ((uint32_t)OC_T << kGidBits);
-        if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_CONV, g, g, r);
-    }
-    const int b = best[s];
-    if (b >= 0 && best[b] == s) {
-        const uint32_t gb = gt[b] & kGidMask;
-        if (g < gb) {
-            lnk[s] = b;
-            lnk[b] = s;
-            if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_FORM, g, gb, r);
+    const int na = actn[blockIdx.x];
+    for (int e = threadIdx.x; e < na; e += 64) {
+        const int s = act[(size_t)blockIdx.x * kRedChunk + e];
+        const uint32_t g = gt[s] & kGidMask;
+        const uint8_t fl = flag[s];
+        if (fl & CF_BREAK) {
+            const int p = lnk[s];
+            const uint32_t gp = gt[p] & kGidMask;
+            lnk[s] = -1;
+            lnk[p] = -1;
+            if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_BREAK, min(g, gp), max(g, gp), r);
+        }
+        if (fl & CF_CONV) {
+            gt[s] = g | ((uint32_t)OC_T << kGidBits);
+            if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_CONV, g, g, r);
+        }
+        const int b = best[s];
+        if (b >= 0 && best[b] == s) {
+            const uint32_t gb = gt[b] & kGidMask;
+            if (g < gb) {
+                lnk[s] = b;
+                lnk[b] = s;
+                if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_FORM, g, gb, r);
+            }
         }
     }
 }
