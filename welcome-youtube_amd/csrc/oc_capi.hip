@@ -64,7 +64,7 @@ struct oc_engine {
     hipStream_t stream = nullptr;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
-    bool have_state = false, any_stir = false, any_chem = false;
+    bool have_state = false, any_stir = false, any_chem = false, any_angle = false;
     std::vector<std::vector<oc_event>> evq;
     std::vector<void *> allocs;
     bool events_dropped = false;
@@ -207,22 +207,23 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false) {
     A.err = h->d_err;
     const dim3 grid(8 * h->per_xcd);
     // CAP (staged candidates) is 3.5x the block; `lists` picks the two-phase
-    // variant with 16 listed pairs per lane, or the single pass
+    // variant with 24 listed pairs per lane, or the single pass
     const bool L2 = h->pair_lists;
 #define OC_PAIR(B, C, CL)                                                                        \
     do {                                                                                         \
-        if (energy) k_pair<B, C, CL, true><<<grid, B, 0, h->stream>>>(A);                         \
-        else k_pair<B, C, CL, false><<<grid, B, 0, h->stream>>>(A);                               \
+        if (energy) k_pair<B, C, CL, true, false><<<grid, B, 0, h->stream>>>(A);                  \
+        else if (h->any_angle) k_pair<B, C, CL, false, true><<<grid, B, 0, h->stream>>>(A);       \
+        else k_pair<B, C, CL, false, false><<<grid, B, 0, h->stream>>>(A);                        \
     } while (0)
     switch (h->pair_block) {
     case 64:
-        if (L2) OC_PAIR(64, 224, 16); else OC_PAIR(64, 224, 0);
+        if (L2) OC_PAIR(64, 224, 24); else OC_PAIR(64, 224, 0);
         break;
     case 128:
-        if (L2) OC_PAIR(128, 448, 16); else OC_PAIR(128, 448, 0);
+        if (L2) OC_PAIR(128, 448, 24); else OC_PAIR(128, 448, 0);
         break;
     default:
-        if (L2) OC_PAIR(256, 896, 16); else OC_PAIR(256, 896, 0);
+        if (L2) OC_PAIR(256, 896, 24); else OC_PAIR(256, 896, 0);
         break;
     }
 #undef OC_PAIR
@@ -482,6 +483,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
         for (int b = 0; b < grid_of(R.N, kRedChunk); b++) h->rblk.push_back(r);
         h->any_stir |= R.has_stir != 0;
         h->any_chem |= R.chem != 0;
+        h->any_angle |= R.ka != 0.0;
     }
     h->Ntot = off;
     h->Ctot = coff;

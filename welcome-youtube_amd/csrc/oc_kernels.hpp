@@ -551,26 +551,27 @@ __global__ __launch_bounds__(kBeadBlock) void k_build_prv(const RepInfo *__restr
 
 constexpr float kTheta32 = (float)(3.4641016151377545870548926830117 / 4294967296.0);  // sqrt(12) / 2^32
 
+// sdtk = sigma / sqrt(dt) * sqrt(12) / 2^32 (the draw scale folded in)
 __device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float vyi, uint32_t gi, const float *arow,
                                          double2 pj, float2 vj, uint32_t gtj, double sxs, double sys, float gam,
-                                         float sdt, uint64_t sseed, double &ax, double &ay, float &e, bool energy) {
-    // FP64 image difference, then FP32 force arithmetic.  Both orientations
-    // of a pair run the same operations on negated operands, so
-    // f_ji = -f_ij exactly.
+                                         float sdtk, uint64_t sseed, double &ax, double &ay, float &e, bool energy) {
+    // FP64 image difference, then FP32 force arithmetic (fused multiply-adds).
+    // Both orientations of a pair run the same operations on negated
+    // operands, so f_ji = -f_ij exactly.
     const float dx = (float)((xi - pj.x) - sxs), dy = (float)((yi - pj.y) - sys);
-    const float r2 = dx * dx + dy * dy;
+    const float r2 = __builtin_fmaf(dx, dx, dy * dy);
     if (r2 < 1.0f && r2 > 1e-24f) {
         const float ir = __builtin_amdgcn_rsqf(r2);
         const float r = r2 * ir;
         const float w = 1.0f - r;
         const float dvx = vxi - vj.x, dvy = vyi - vj.y;
-        const float rv = (dx * dvx + dy * dvy) * ir;
+        const float rv = __builtin_fmaf(dx, dvx, dy * dvy) * ir;
         const uint32_t gj = gtj & kGidMask;
         const uint64_t h = pair_hash(sseed, gi, gj);
         // theta on [-sqrt3, sqrt3) from the top 32 bits of the hash
-        const float th = (float)(int32_t)(uint32_t)(h >> 32) * kTheta32;
+        const float th = (float)(int32_t)(uint32_t)(h >> 32);
         const float a = arow[gtj >> kGidBits];
-        const float fm = w * (a - gam * w * rv + sdt * th) * ir;
+        const float fm = w * __builtin_fmaf(sdtk, th, __builtin_fmaf(-gam * w, rv, a)) * ir;
         ax += (double)(fm * dx);
         ay += (double)(fm * dy);
         if (energy) e += 0.25f * a * w * w;
@@ -585,6 +586,29 @@ __device__ __forceinline__ void bond_vec(const RepInfo &R, double2 pi, double2 p
     r = sqrt(dx * dx + dy * dy);
 }
 
+// Links and partner data of bead s for its bonded terms, loaded in two
+// rounds: bond_links at the start of the bead (the pair loop hides their
+// latency), bond_partners after the pair loop.
+struct BondPre {
+    int m, n1, p1;   // chain position (-1: none), next/partner slot, previous slot
+    double2 pn, pp;  // positions of n1 and p1
+    uint32_t gn;     // gid word of n1 (chemistry bond)
+};
+__device__ __forceinline__ void bond_links(int s, const int8_t *__restrict__ mp, const int32_t *__restrict__ lnk,
+                                           const int32_t *__restrict__ prv, BondPre &B) {
+    B.m = mp[s];
+    B.n1 = lnk[s];
+    B.p1 = prv[s];
+}
+__device__ __forceinline__ void bond_partners(const double2 *__restrict__ xy, const uint32_t *__restrict__ gt,
+                                              BondPre &B) {
+    if (B.n1 >= 0) {
+        B.pn = xy[B.n1];
+        B.gn = gt[B.n1];
+    }
+    if (B.m > 0 && B.p1 >= 0) B.pp = xy[B.p1];
+}
+
 // Bonded force on bead s in gather form: every bond and angle term that s
 // takes part in is evaluated whole, with the arithmetic of
 // chain_forces_linear (cel/engine/oercel.c:1267-1308), and s adds its share,
@@ -592,18 +616,18 @@ __device__ __forceinline__ void bond_vec(const RepInfo &R, double2 pi, double2 p
 // pass adds them to each bead, so the result is the same bit for bit and the
 // two beads of a bond receive exactly opposite forces.  A chemistry bond is
 // the bond from the smaller gid to the larger.  mb: largest bond length at s.
-__device__ __forceinline__ void bonded_gather(const RepInfo &R, int s, double2 ps, const double2 *__restrict__ xy,
-                                              const uint32_t *__restrict__ gt, const int8_t *__restrict__ mp,
-                                              const int32_t *__restrict__ lnk, const int32_t *__restrict__ prv,
-                                              double &fx, double &fy, double &mb) {
-    const int m = mp[s];
-    const int n1 = lnk[s];
+// The angle terms (ANG: some replica has k_angle != 0; each replica's own
+// k_angle applies) load the beads two links away here.
+template <bool ANG>
+__device__ __forceinline__ void bonded_gather(const RepInfo &R, double2 ps, uint32_t gs, const BondPre &B,
+                                              const double2 *__restrict__ xy, const int32_t *__restrict__ lnk,
+                                              const int32_t *__restrict__ prv, double &fx, double &fy, double &mb) {
+    const int m = B.m, n1 = B.n1;
     if (m < 0) {
         if (n1 < 0) return;
-        const double2 pn = xy[n1];
-        const bool lo = (gt[s] & kGidMask) < (gt[n1] & kGidMask);
+        const bool lo = (gs & kGidMask) < (B.gn & kGidMask);
         double dx, dy, r;
-        bond_vec(R, lo ? ps : pn, lo ? pn : ps, dx, dy, r);
+        bond_vec(R, lo ? ps : B.pn, lo ? B.pn : ps, dx, dy, r);
         mb = fmax(mb, r);
         const double fb = -R.kb * (r - R.r0) / r;
         if (lo) {
@@ -615,20 +639,43 @@ __device__ __forceinline__ void bonded_gather(const RepInfo &R, int s, double2 p
         }
         return;
     }
+    if (!ANG) {
+        // bonds to the previous and the next chain bead
+        if (m > 0 && B.p1 >= 0) {
+            double dx, dy, r;
+            bond_vec(R, B.pp, ps, dx, dy, r);
+            mb = fmax(mb, r);
+            const double fb = -R.kb * (r - R.r0) / r;
+            fx += fb * dx;
+            fy += fb * dy;
+        }
+        if (n1 >= 0) {
+            double dx, dy, r;
+            bond_vec(R, ps, B.pn, dx, dy, r);
+            mb = fmax(mb, r);
+            const double fb = -R.kb * (r - R.r0) / r;
+            fx -= fb * dx;
+            fy -= fb * dy;
+        }
+        return;
+    }
     // chain beads at offsets -2..+2 (index 0..4, s at 2); bond b joins c[b] -> c[b+1]
-    int c[5];
-    c[2] = s;
-    c[1] = prv[s];
-    c[3] = n1;
     const bool ang = R.ka != 0.0;
+    int c[5];
+    double2 p[5];
+    c[2] = 0;
+    p[2] = ps;
+    c[1] = m > 0 ? B.p1 : -1;
+    p[1] = B.pp;
+    c[3] = n1;
+    p[3] = B.pn;
     c[0] = (ang && c[1] >= 0) ? prv[c[1]] : -1;
     c[4] = (ang && c[3] >= 0) ? lnk[c[3]] : -1;
-    double2 p[5];
-    p[2] = ps;
-    for (int k = 0; k < 5; k++)
-        if (k != 2 && c[k] >= 0) p[k] = xy[c[k]];
+    if (c[0] >= 0) p[0] = xy[c[0]];
+    if (c[4] >= 0) p[4] = xy[c[4]];
     double bx[4], by[4], bl[4];
     bool has[4];
+#pragma unroll
     for (int b = 0; b < 4; b++) {
         has[b] = c[b] >= 0 && c[b + 1] >= 0;
         if (has[b]) bond_vec(R, p[b], p[b + 1], bx[b], by[b], bl[b]);
@@ -649,6 +696,7 @@ __device__ __forceinline__ void bonded_gather(const RepInfo &R, int s, double2 p
     const double ka = R.ka;
     // angle a uses bonds a, a+1 (beads c[a], c[a+1], c[a+2]); s is its third,
     // second, first bead for a = 0, 1, 2
+#pragma unroll
     for (int a = 0; a < 3; a++) {
         if (!has[a] || !has[a + 1]) continue;
         const double ax = bx[a], ay = by[a], cxv = bx[a + 1], cyv = by[a + 1];
@@ -699,10 +747,10 @@ struct PairArgs {
 // rows per staged column (tile rows + 2 * margin + 1) of a pair tile
 __host__ __device__ constexpr int pair_max_rows(int block) { return 2 * block; }
 
-template <int BLOCK, int CAP, int CAPL, bool ENERGY>
+template <int BLOCK, int CAP, int CAPL, bool ENERGY, bool ANG>
 __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     constexpr int kMaxRows = pair_max_rows(BLOCK);
-    __shared__ uint16_t s_lst[CAPL > 0 ? CAPL : 1][BLOCK];  // per-lane list of interacting candidates
+    __shared__ uint32_t s_lst[CAPL > 0 ? (CAPL + 3) / 4 : 1][BLOCK];  // per-lane lists, four entries a word
     __shared__ double2 s_xy[CAP];
     __shared__ float2 s_vt[CAP];
     __shared__ uint32_t s_gt[CAP];
@@ -786,7 +834,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
 
     const double L = R.L;
     const double sxc[3] = {T.cx == 0 ? -L : 0.0, 0.0, T.cx == ncx - 1 ? L : 0.0};
-    const float gam = S.gamma, sdt = S.sdt;
+    const float gam = S.gamma, sdtk = S.sdt * kTheta32;
     const uint64_t sseed = S.sseed;
     const int ibeg = s_row[1][T.k0 - kb0], iend = s_row[1][T.k1 - kb0];
     const int slot0 = cs[R.coff + T.cx * ncy + T.k0];
@@ -804,22 +852,24 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     // candidate's position is loaded one iteration ahead.
     double mb = 0.0;  // largest bond length seen by this lane
     // per-bead epilogue: energy, or force (+ stirring, + bonded terms)
-    auto finish = [&](int slot, double2 pi, double ax, double ay, float e) {
+    auto finish = [&](int slot, double2 pi, uint32_t gti, const BondPre &B, double ax, double ay, float e) {
         if (ENERGY) {
             A.epot[slot] = (double)e;
         } else {
             double bx = 0.0;
             if (A.stir) bx = A.stir[slot] - S.stir_mean;
             double fx = bx + ax, fy = ay;
-            if (A.bonded) bonded_gather(R, slot, pi, A.xy, A.gt, A.mp, A.lnk, A.prv, fx, fy, mb);
+            if (A.bonded) bonded_gather<ANG>(R, pi, gti, B, A.xy, A.lnk, A.prv, fx, fy, mb);
             A.f[slot] = make_double2(fx, fy);
         }
     };
     auto run = [&](const double2 *XY, const float2 *VT, const uint32_t *GT, const bool stg) {
-        const bool lists = CAPL > 0 && stg;
         for (int ii = ibeg + threadIdx.x; ii < iend; ii += BLOCK) {
             const int slot = slot0 + (ii - ibeg);
             const int is = stg ? ii : slot;
+            BondPre B;
+            const bool bond = !ENERGY && A.bonded;
+            if (bond) bond_links(slot, A.mp, A.lnk, A.prv, B);
             const double2 pi = XY[is];
             const float2 vi = VT[is];
             const uint32_t gti = GT[is];
@@ -834,33 +884,75 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             }
             double ax = 0.0, ay = 0.0;
             float e = 0.0f;
-            int cnt = 0;
-            // Flat single pass (staged, CAPL == 0, no periodic image anywhere in
-            // the wavefront's windows): a column's window rows are one
-            // contiguous LDS range (its staged pieces are concatenated in
-            // unwrapped row order), and the lane runs through its three column
-            // ranges as one sequence, so the wavefront iterates ~max over lanes
-            // of the whole window rather than the sum over columns of the
-            // per-column maxima.  The three ranges lie in increasing LDS order,
-            // so stepping past the end of one range lands on the next by two
-            // selects.
+            // Flat path (staged, no periodic image anywhere in the wavefront's
+            // windows): a column's window rows are one contiguous LDS range
+            // (its staged pieces are concatenated in unwrapped row order), and
+            // the lane runs through its three column ranges as one sequence
+            // (stepping past the end of one range lands on the next by two
+            // selects), so the wavefront iterates ~max over lanes of the whole
+            // window rather than the sum of per-column maxima.  With CAPL > 0
+            // it runs in two phases: the cutoff test over the sequence records
+            // the positions t < 256 of the interacting candidates (packed four
+            // per LDS word, CAPL per lane), then the force and noise arithmetic
+            // runs over that list only, so the wavefront executes it ~max over
+            // lanes of the pair count instead of ~max of the candidate count.
+            // A candidate that does not fit the list is evaluated in place.
             const bool img = sxc[0] != 0.0 || sxc[2] != 0.0 || klo < 0 || khi >= ncy;
-            if (stg && CAPL == 0 && !__any(img)) {
+            if (stg && !__any(img)) {
                 const int st0 = s_row[0][klo - kb0], en0 = s_row[0][khi + 1 - kb0];
                 const int st1 = s_row[1][klo - kb0], en1 = s_row[1][khi + 1 - kb0];
                 const int st2 = s_row[2][klo - kb0], en2 = s_row[2][khi + 1 - kb0];
                 int j = st0 == en0 ? st1 : st0;
                 j = j == en1 ? st2 : j;
-                while (j < en2) {
-                    pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], 0.0, 0.0, gam, sdt, sseed, ax, ay,
-                             e, ENERGY);
-                    ++j;
-                    j = j == en0 ? st1 : j;
-                    j = j == en1 ? st2 : j;
+                if (CAPL == 0) {
+                    while (j < en2) {
+                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], 0.0, 0.0, gam, sdtk, sseed,
+                                 ax, ay, e, ENERGY);
+                        ++j;
+                        j = j == en0 ? st1 : j;
+                        j = j == en1 ? st2 : j;
+                    }
+                } else {
+                    int cnt = 0, t = 0;
+                    uint32_t pack = 0;
+                    while (j < en2) {
+                        const double2 pj = XY[j];
+                        const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
+                        const float r2 = dx * dx + dy * dy;
+                        if (r2 < 1.0f && r2 > 1e-24f) {
+                            if (cnt < CAPL && t < 256) {
+                                pack |= (uint32_t)t << (8 * (cnt & 3));
+                                if ((cnt & 3) == 3) {
+                                    s_lst[cnt >> 2][threadIdx.x] = pack;
+                                    pack = 0;
+                                }
+                                cnt++;
+                            } else {
+                                pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, pj, VT[j], GT[j], 0.0, 0.0, gam, sdtk,
+                                         sseed, ax, ay, e, ENERGY);
+                            }
+                        }
+                        ++j;
+                        ++t;
+                        j = j == en0 ? st1 : j;
+                        j = j == en1 ? st2 : j;
+                    }
+                    if (cnt & 3) s_lst[cnt >> 2][threadIdx.x] = pack;
+                    const int n0 = en0 - st0, n01 = n0 + (en1 - st1);
+                    uint32_t w = 0;
+                    for (int k = 0; k < cnt; k++) {
+                        if ((k & 3) == 0) w = s_lst[k >> 2][threadIdx.x];
+                        const int tk = (w >> (8 * (k & 3))) & 255;
+                        const int jk = tk < n0 ? st0 + tk : (tk < n01 ? st1 + (tk - n0) : st2 + (tk - n01));
+                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jk], VT[jk], GT[jk], 0.0, 0.0, gam, sdtk, sseed,
+                                 ax, ay, e, ENERGY);
+                    }
                 }
-                finish(slot, pi, ax, ay, e);
+                if (bond) bond_partners(A.xy, A.gt, B);
+                finish(slot, pi, gti, B, ax, ay, e);
                 continue;
             }
+            // general path: per column and periodic piece (image shifts), single pass
             for (int c = 0; c < 3; c++) {
                 // window split at the periodic seams: rows < 0 (image -L, staged piece 0),
                 // 0..ncy-1 (piece 1), >= ncy (image +L, piece 2)
@@ -874,55 +966,13 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                     const double sx = sxc[c];
                     const int adj = stg ? 0 : s_piece[c][q][0] - s_piece[c][q][2];
                     const int ja = s_row[c][pa - kb0] + adj, jb = s_row[c][pb + 1 - kb0] + adj;
-                    if (ja >= jb) continue;
-                    if (!lists) {
-                        for (int j = ja; j < jb; j++)
-                            pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], sx, sy, gam, sdt, sseed,
-                                     ax, ay, e, ENERGY);
-                        continue;
-                    }
-                    const uint16_t tag = (uint16_t)((c << 10) | (q << 12));
-                    double2 pn = XY[ja];
-                    if (sx == 0.0 && sy == 0.0) {
-                        for (int j = ja; j < jb; j++) {
-                            const double2 pj = pn;
-                            if (j + 1 < jb) pn = XY[j + 1];
-                            const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
-                            const float r2 = dx * dx + dy * dy;
-                            if (r2 < 1.0f && r2 > 1e-24f) {
-                                if (cnt < CAPL) s_lst[cnt++ % (CAPL > 0 ? CAPL : 1)][threadIdx.x] = (uint16_t)j | tag;
-                                else pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, pj, VT[j], GT[j], sx, sy, gam, sdt,
-                                              sseed, ax, ay, e, ENERGY);
-                            }
-                        }
-                    } else {
-                        for (int j = ja; j < jb; j++) {
-                            const double2 pj = pn;
-                            if (j + 1 < jb) pn = XY[j + 1];
-                            const float dx = (float)((pi.x - pj.x) - sx), dy = (float)((pi.y - pj.y) - sy);
-                            const float r2 = dx * dx + dy * dy;
-                            if (r2 < 1.0f && r2 > 1e-24f) {
-                                if (cnt < CAPL) s_lst[cnt++ % (CAPL > 0 ? CAPL : 1)][threadIdx.x] = (uint16_t)j | tag;
-                                else pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, pj, VT[j], GT[j], sx, sy, gam, sdt,
-                                              sseed, ax, ay, e, ENERGY);
-                            }
-                        }
-                    }
+                    for (int j = ja; j < jb; j++)
+                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], sx, sy, gam, sdtk, sseed, ax,
+                                 ay, e, ENERGY);
                 }
             }
-            if (lists) {
-                int code = cnt > 0 ? s_lst[0][threadIdx.x] : 0;
-                for (int k = 0; k < cnt; k++) {
-                    const int cur = code;
-                    if (k + 1 < cnt) code = s_lst[(k + 1) % (CAPL > 0 ? CAPL : 1)][threadIdx.x];
-                    const int j = cur & 1023, c = (cur >> 10) & 3, q = cur >> 12;
-                    const double sy = q == 0 ? -L : (q == 2 ? L : 0.0);
-                    const double sx = c == 0 ? sxc[0] : (c == 1 ? 0.0 : sxc[2]);
-                    pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], sx, sy, gam, sdt, sseed, ax, ay,
-                             e, ENERGY);
-                }
-            }
-            finish(slot, pi, ax, ay, e);
+            if (bond) bond_partners(A.xy, A.gt, B);
+            finish(slot, pi, gti, B, ax, ay, e);
         }
     };
     if (staged) run(s_xy, s_vt, s_gt, true);
