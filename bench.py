@@ -292,7 +292,10 @@ def main():
     ev = e.events()
     pair_s = time_pair_kernel(e)
     ncell = e.cells()[0].size - 1
-    bytes_pair = N * (16 + 8 + 4 + 16) + 4 * (ncell + 1)
+    # algorithmic bytes of one force launch: staged xy (16), vt (8), gid/type
+    # (4), bonded links mp (1), lnk (4), prv (4), force write (16) per bead,
+    # and the cell starts
+    bytes_pair = N * (16 + 8 + 4 + 1 + 4 + 4 + 16) + 4 * (ncell + 1)
     achieved = bytes_pair / pair_s / 1e9
     e.close()
 

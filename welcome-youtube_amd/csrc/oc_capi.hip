@@ -272,19 +272,20 @@ static void launch_sort(oc_engine *h, bool integ, int step_delta) {
     else
         k_keys<false><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->st[c].xy, h->st[c].v, h->d_f,
                                                         h->d_key, h->d_bmax, h->d_moved);
-    k_tsort<<<nt, kSortRows, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_cs[c], h->d_key, h->d_bmax, h->d_moved,
-                                             h->d_tcount, h->d_locst, h->d_tsrc, h->d_tslot, h->d_err);
+    const int ng = std::min(nt, kSortGrid);
+    k_tsort<<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_cs[c], h->d_key, h->d_bmax, h->d_moved,
+                                             h->d_tcount, h->d_locst, h->d_tsrc, h->d_tslot, h->d_err, nt);
     k_tscan<<<1, 1024, 0, h->stream>>>(h->d_ri, h->d_rs, h->nrep, h->d_tcount, nt, h->d_toff, h->d_moved,
                                        step_delta);
     int32_t *src = h->record_src ? h->d_src : nullptr;
     if (integ)
-        k_place<true><<<nt, kSortRows, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,
+        k_place<true><<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,
                                                        h->d_tsrc, h->d_tslot, h->d_cs[n], src, h->st[c],
-                                                       h->st[n], h->d_f, h->d_err);
+                                                       h->st[n], h->d_f, h->d_err, nt);
     else
-        k_place<false><<<nt, kSortRows, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,
+        k_place<false><<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,
                                                         h->d_tsrc, h->d_tslot, h->d_cs[n], src, h->st[c],
-                                                        h->st[n], h->d_f, h->d_err);
+                                                        h->st[n], h->d_f, h->d_err, nt);
     h->cur = n;
 }
 

@@ -104,8 +104,9 @@ __global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ 
 
 // --------------------------------------------------------------------------- sort
 
-// New cell key of every bead (of its integrated position when INTEG); each
-// block stores its largest periodic cell displacement (x, y) in bmax[block]
+// New cell key of every bead (of its integrated position when INTEG; bit 31
+// set when the bead changed cells); each block stores its largest periodic
+// cell displacement (x, y) in bmax[block]
 // (no atomics), and a bead that moved by more than one cell sets *moved (a
 // rare event: the sort then widens its window to the replica's maximum).
 template <bool INTEG>
@@ -129,7 +130,8 @@ __global__ __launch_bounds__(kBeadBlock) void k_keys(const RepInfo *__restrict__
             integrate1(R, p, v[s], f[s], q, vo, vt);
         }
         const int cx = cell_x(R, q.x), cy = cell_y(R, q.y);
-        key[s] = R.coff + cx * R.ncy + cy;
+        // bit 31: the bead changed cells
+        key[s] = (R.coff + cx * R.ncy + cy) | (cx != ox || cy != oy ? (int32_t)0x80000000 : 0);
         dx = abs(cx - ox);
         dx = min(dx, R.ncx - dx);
         dy = abs(cy - oy);
@@ -162,251 +164,314 @@ __device__ __forceinline__ int win_pieces(int c, int R, int n, int *a, int *b) {
     return 1;
 }
 
-// Sort tiles: a column's rows [k0, k1) of new cells (one thread per row).
-// The stable counting sort of cel/engine/oercel.c:828-933 orders the beads
-// of a new cell by old slot; they come from the old cells within the
-// displacement radius.  Visiting those old cells in increasing cell index
-// (= increasing old slot) and keeping the beads whose key is the cell gives
-// that order.  k_tsort does it per tile with the candidates' keys staged in
-// LDS (read from global memory when they do not fit) and numbers the tile's
-// beads; k_tscan turns the tile counts into slot offsets; k_place writes the
-// new cell starts and gathers the state into the new order.  No atomics, so
-// no histogram to clear.  The radius is one cell unless k_keys saw a larger
-// displacement, then the replica's maximum (from the k_keys block maxima).
-constexpr int kSortRows = 256;                // rows (threads) per sort tile
-constexpr int kSortCand = 2048;               // staged candidates
-constexpr int kSortCols = 3;                  // staged columns
-constexpr int kSortMaxRy = 8;                 // staged row margin
-constexpr int kSortRowTab = kSortRows + 2 * kSortMaxRy + 1;
+// Sort tiles: a column's rows [k0, k1) of new cells (up to kSortRows rows,
+// kSortThr threads, two rows each).  The stable counting sort of
+// cel/engine/oercel.c:828-933 orders the beads of a new cell by old slot.
+// With a displacement radius of one cell (the normal case) the tile stages
+// the keys of the old rows k0-1 .. k1 of its three columns in LDS in old-slot
+// order (columns ascending; within a column the row pieces of the periodic
+// range ascending), so LDS order = old-slot order.  The beads of new cell c in
+// the stable order are the arrivals from lower cells, the stayers of c, the
+// arrivals from higher cells: a stayer's rank is (arrivals from lower cells)
+// + (its index in old cell c) - (departures from c before it); a bead that
+// changed cells counts the beads of c before it in c's window (one lane per
+// such bead).  Counts and arrival counts are LDS integer atomics (order
+// independent).  A larger radius, a box narrower than three columns or an
+// overflowing stage takes the general path: each row visits the old cells
+// within the radius in increasing cell index (= old slot) from global memory
+// and keeps the beads whose key is the cell.  k_tscan turns the tile counts
+// into slot offsets; k_place writes the new cell starts and gathers the state
+// into the new order.  No global atomics, no histogram to clear.
+constexpr int kSortRows = 256;                // rows per sort tile
+constexpr int kSortThr = 128;                 // threads per sort tile (two rows each)
+constexpr int kSortCand = 1536;               // staged candidates
+constexpr int kSortRowTab = kSortRows + 3;    // staged rows per column (+ end)
 constexpr int kSortCapLog = 10;
 constexpr int kSortCap = 1 << kSortCapLog;    // beads per sort tile
+constexpr int kSortGrid = 256 * 6;            // workgroups walking the sort tiles
 
-__device__ __forceinline__ int block_excl_scan256(int v, int *tmp, int &total) {
-    // exclusive scan over a 256-thread block; tmp: 4 ints of LDS
+__device__ __forceinline__ int block_excl_scan128(int v, int *tmp, int &total) {
+    // exclusive scan over a 128-thread block; tmp: 2 ints of LDS
     const int inc = wave_incl_scan(v);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 63) tmp[w] = inc;
     __syncthreads();
-    int off = 0;
-    for (int k = 0; k < w; k++) off += tmp[k];
-    total = tmp[0] + tmp[1] + tmp[2] + tmp[3];
+    const int off = w ? tmp[0] : 0;
+    total = tmp[0] + tmp[1];
     __syncthreads();
     return off + inc - v;
 }
 
-__global__ __launch_bounds__(kSortRows) void k_tsort(const RepInfo *__restrict__ ri, const Tile *__restrict__ tiles,
-                                                     const int32_t *__restrict__ cs_old,
-                                                     const int32_t *__restrict__ key,
-                                                     const int2 *__restrict__ bmax,
-                                                     const uint32_t *__restrict__ moved,
-                                                     int32_t *__restrict__ tcount, int32_t *__restrict__ locst,
-                                                     int32_t *__restrict__ tsrc, int32_t *__restrict__ tslot,
-                                                     uint32_t *__restrict__ err) {
+__global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ ri, const Tile *__restrict__ tiles,
+                                                    const int32_t *__restrict__ cs_old,
+                                                    const int32_t *__restrict__ key,
+                                                    const int2 *__restrict__ bmax,
+                                                    const uint32_t *__restrict__ moved,
+                                                    int32_t *__restrict__ tcount, int32_t *__restrict__ locst,
+                                                    int32_t *__restrict__ tsrc, int32_t *__restrict__ tslot,
+                                                    uint32_t *__restrict__ err, int nt) {
     __shared__ int32_t s_key[kSortCand];
-    __shared__ int32_t s_cs[kSortCols][kSortRowTab];  // cs_old of the staged rows (start; the last: end)
-    __shared__ int32_t s_c3[kSortCols][3];            // cs_old at ustart, column end, column start
+    __shared__ int32_t s_cs[3][kSortRowTab];  // cs_old of the staged rows (slot order); extras: piece ends
+    __shared__ int32_t s_e[3][2];
+    __shared__ int32_t s_pc[4][3];  // per staged column: LDS base, piece counts; [3][0]: total
     __shared__ int32_t s_out[kSortCap];
-    __shared__ int s_tmp[4], s_rad[2][4];
+    __shared__ int32_t s_cnt[kSortRows], s_alow[kSortRows], s_start[kSortRows];
+    __shared__ int32_t s_mv[kSortCap];  // staged ids of the tile's beads that changed cells
+    __shared__ int s_nmv;
+    __shared__ int s_tmp[2], s_rad[2][2];
 
-    const Tile T = tiles[blockIdx.x];
-    const RepInfo &R = ri[T.rep];
-    const int ncy = R.ncy, ncx = R.ncx;
-    const int t = threadIdx.x;
-    int Rx = 1, Ry = 1;
-    if (*moved) {
-        // the replica's largest displacement
-        const int nblk = (R.N + kBeadBlock - 1) / kBeadBlock;
-        int mx = 1, my = 1;
-        for (int b = t; b < nblk; b += kSortRows) {
-            const int2 m = bmax[R.bblk0 + b];
-            mx = max(mx, m.x);
-            my = max(my, m.y);
+    auto do_tile = [&](const int tix) {
+        const Tile T = tiles[tix];
+        const RepInfo &R = ri[T.rep];
+        const int ncy = R.ncy, ncx = R.ncx;
+        const int t = threadIdx.x;
+        const int nrows = T.k1 - T.k0;
+        const int c0 = R.coff + T.cx * ncy + T.k0;
+        int Rx = 1, Ry = 1;
+        if (*moved) {
+            // the replica's largest displacement
+            const int nblk = (R.N + kBeadBlock - 1) / kBeadBlock;
+            int mx = 1, my = 1;
+            for (int b = t; b < nblk; b += kSortThr) {
+                const int2 m = bmax[R.bblk0 + b];
+                mx = max(mx, m.x);
+                my = max(my, m.y);
+            }
+            mx = wave_max(mx);
+            my = wave_max(my);
+            if ((t & 63) == 0) {
+                s_rad[0][t >> 6] = mx;
+                s_rad[1][t >> 6] = my;
+            }
+            __syncthreads();
+            Rx = max(s_rad[0][0], s_rad[0][1]);
+            Ry = max(s_rad[1][0], s_rad[1][1]);
         }
-        mx = wave_max(mx);
-        my = wave_max(my);
-        if ((t & 63) == 0) {
-            s_rad[0][t >> 6] = mx;
-            s_rad[1][t >> 6] = my;
+        // staged rows k0-1 .. k1 (mod ncy) as row pieces P0 = [a0, b0], P1 =
+        // [a1, b1] in increasing row (= slot) order; the three columns ascending
+        int a0, b0, a1 = 0, b1 = -1;
+        if (nrows + 2 >= ncy) { a0 = 0; b0 = ncy - 1; }
+        else if (T.k0 == 0) { a0 = 0; b0 = T.k1; a1 = ncy - 1; b1 = ncy - 1; }
+        else if (T.k1 == ncy) { a0 = 0; b0 = 0; a1 = T.k0 - 1; b1 = ncy - 1; }
+        else { a0 = T.k0 - 1; b0 = T.k1; }
+        const int len0 = b0 - a0 + 1, len1 = b1 - a1 + 1, nr = len0 + len1;
+        int col[3];
+        {
+            const int cm = T.cx == 0 ? ncx - 1 : T.cx - 1, cp = T.cx == ncx - 1 ? 0 : T.cx + 1;
+            // ascending order of {cm, cx, cp}
+            col[0] = min(cm, min(T.cx, cp));
+            col[2] = max(cm, max(T.cx, cp));
+            col[1] = cm + T.cx + cp - col[0] - col[2];
+        }
+        const int qo = col[0] == T.cx ? 0 : (col[1] == T.cx ? 1 : 2);
+        bool fast = Rx == 1 && Ry == 1 && ncx >= 3 && nr + 1 <= kSortRowTab;
+        auto row_of = [&](int u) { return u < len0 ? a0 + u : a1 + (u - len0); };
+        if (fast) {
+            constexpr int kPer = (kSortRowTab + kSortThr - 1) / kSortThr;
+            int v[3][kPer];
+    #pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const int cbq = R.coff + col[q] * ncy;
+    #pragma unroll
+                for (int i = 0; i < kPer; i++) {
+                    const int u = t + i * kSortThr;
+                    if (u < nr) v[q][i] = cs_old[cbq + row_of(u)];
+                }
+            }
+            int ve = 0;
+            if (t < 6) {
+                const int q = t >> 1;
+                const int b = (t & 1) ? (len1 > 0 ? b1 : b0) : b0;
+                ve = cs_old[R.coff + col[q] * ncy + b + 1];
+            }
+    #pragma unroll
+            for (int q = 0; q < 3; q++)
+    #pragma unroll
+                for (int i = 0; i < kPer; i++) {
+                    const int u = t + i * kSortThr;
+                    if (u < nr) s_cs[q][u] = v[q][i];
+                }
+            if (t < 6) s_e[t >> 1][t & 1] = ve;
+        }
+        s_cnt[t] = 0;
+        s_cnt[t + kSortThr] = 0;
+        s_alow[t] = 0;
+        s_alow[t + kSortThr] = 0;
+        if (t == 0) s_nmv = 0;
+        __syncthreads();
+        // per column (LDS, indexed by run-time q): staged count of each piece, LDS base
+        if (t == 0) {
+            int b = 0;
+            for (int q = 0; q < 3; q++) {
+                const int m0 = fast ? s_e[q][0] - s_cs[q][0] : 0;
+                const int m1 = fast && len1 > 0 ? s_e[q][1] - s_cs[q][len0] : 0;
+                s_pc[q][0] = b;
+                s_pc[q][1] = m0;
+                s_pc[q][2] = m1;
+                b += m0 + m1;
+            }
+            s_pc[3][0] = b;
         }
         __syncthreads();
-        Rx = max(max(s_rad[0][0], s_rad[0][1]), max(s_rad[0][2], s_rad[0][3]));
-        Ry = max(max(s_rad[1][0], s_rad[1][1]), max(s_rad[1][2], s_rad[1][3]));
-    }
-    // window geometry (every thread): staged columns in increasing index, rows
-    int ca[2], cb[2];
-    const int npc = win_pieces(T.cx, Rx, ncx, ca, cb);
-    int ncol = 0;
-    for (int q = 0; q < npc; q++) ncol += cb[q] - ca[q] + 1;
-    const int span = T.k1 - T.k0 + 2 * Ry;
-    int ustart, nr;
-    if (span >= ncy) { ustart = 0; nr = ncy; }
-    else { ustart = T.k0 - Ry < 0 ? T.k0 - Ry + ncy : T.k0 - Ry; nr = span; }
-    auto col_of = [&](int q) { return q < cb[0] - ca[0] + 1 ? ca[0] + q : ca[1] + q - (cb[0] - ca[0] + 1); };
-    bool lds = ncol <= kSortCols && nr + 1 <= kSortRowTab;
-    if (lds) {
-        // cs_old of the staged rows: row u is ustart + u (wrapped); entry nr is
-        // the end of the last staged row.  All loads first, then the stores.
-        constexpr int kPer = (kSortRowTab + kSortRows - 1) / kSortRows;
-        int v[kSortCols][kPer];
-#pragma unroll
-        for (int q = 0; q < kSortCols; q++)
-            if (q < ncol) {
-                const int cbq = R.coff + col_of(q) * ncy;
-#pragma unroll
-                for (int i = 0; i < kPer; i++) {
-                    const int u = t + i * kSortRows, k = ustart + u;
-                    if (u <= nr) v[q][i] = cs_old[cbq + (k <= ncy ? k : k - ncy)];
+        const int nst = s_pc[3][0];
+        fast = fast && nst <= kSortCand;
+        auto lds_row = [&](int q, int u) {  // LDS index of the first bead of staged row u < nr
+            return s_pc[q][0] + (u < len0 ? s_cs[q][u] - s_cs[q][0] : s_pc[q][1] + s_cs[q][u] - s_cs[q][len0]);
+        };
+        auto lds_after = [&](int q, int u) {  // just past the beads of staged row u
+            return u + 1 == len0 ? s_pc[q][0] + s_pc[q][1]
+                                 : (u + 1 == nr ? s_pc[q][0] + s_pc[q][1] + s_pc[q][2] : lds_row(q, u + 1));
+        };
+        auto u_of = [&](int row) { return row >= a0 && row <= b0 ? row - a0 : len0 + (row - a1); };
+        auto slot_of = [&](int j) {  // old slot of staged candidate j
+            const int q = j < s_pc[1][0] ? 0 : (j < s_pc[2][0] ? 1 : 2);
+            const int o = j - s_pc[q][0];
+            return o < s_pc[q][1] ? s_cs[q][0] + o : s_cs[q][len0] + (o - s_pc[q][1]);
+        };
+        if (fast) {
+            // keys in LDS order; slot of LDS index j from the six pieces
+            constexpr int kPer = kSortCand / kSortThr;
+            const int nper = (nst + kSortThr - 1) / kSortThr;
+            int v[kPer];
+    #pragma unroll
+            for (int i = 0; i < kPer; i++)
+                if (i < nper) {
+                    const int j = t + i * kSortThr;
+                    if (j < nst) v[i] = key[slot_of(j)];
                 }
-            }
-        int w3 = 0;
-        if (t < 3 * ncol) {
-            const int q = t / 3, w = t - 3 * q;
-            w3 = cs_old[R.coff + col_of(q) * ncy + (w == 0 ? ustart : (w == 1 ? ncy : 0))];
-        }
-#pragma unroll
-        for (int q = 0; q < kSortCols; q++)
-            if (q < ncol) {
-#pragma unroll
-                for (int i = 0; i < kPer; i++) {
-                    const int u = t + i * kSortRows;
-                    if (u <= nr) s_cs[q][u] = v[q][i];
+    #pragma unroll
+            for (int i = 0; i < kPer; i++)
+                if (i < nper) {
+                    const int j = t + i * kSortThr;
+                    if (j < nst) s_key[j] = v[i];
                 }
-            }
-        if (t < 3 * ncol) s_c3[t / 3][t % 3] = w3;
-    }
-    __syncthreads();
-    // staged pieces (every thread, registers): column q's rows from ustart to
-    // the column end are LDS [pl[2q], pl[2q+1]) = slots from c3[q][0]; the
-    // wrapped rows from row 0 follow, slots from c3[q][2].  rA/rB: LDS index
-    // of staged row u of column q = rA[q] + s_cs[q][u] (ustart + u <= ncy),
-    // rB[q] + s_cs[q][u] (wrapped).
-    int pf[2 * kSortCols], pl[2 * kSortCols + 1], rA[kSortCols], rB[kSortCols];
-    int nst = 0;
-    if (lds) {
-#pragma unroll
-        for (int q = 0; q < kSortCols; q++) {
-            if (q < ncol) {
-                const int c0 = s_c3[q][0], c1 = s_c3[q][1], c2 = s_c3[q][2], cn = s_cs[q][nr];
-                const bool wr = ustart + nr > ncy;
-                const int n0 = (wr ? c1 : cn) - c0, n1 = wr ? cn - c2 : 0;
-                pf[2 * q] = c0;
-                pl[2 * q] = nst;
-                pf[2 * q + 1] = c2;
-                pl[2 * q + 1] = nst + n0;
-                rA[q] = nst - c0;
-                rB[q] = nst + n0 - c2;
-                nst += n0 + n1;
-            } else {
-                pf[2 * q] = pf[2 * q + 1] = 0;
-                pl[2 * q] = pl[2 * q + 1] = nst;
-                rA[q] = rB[q] = 0;
-            }
         }
-        pl[2 * kSortCols] = nst;
-        lds = nst <= kSortCand;
-    }
-    auto row_lds = [&](int q, int u) {
-        const int b = ustart + u <= ncy ? (q == 0 ? rA[0] : (q == 1 ? rA[1] : rA[2]))
-                                        : (q == 0 ? rB[0] : (q == 1 ? rB[1] : rB[2]));
-        return b + s_cs[q][u];
-    };
-    // slot of staged candidate j: j + (pf - pl) of its piece, as a sum of the
-    // increments at the piece starts (no indexed lookup)
-    int dinc[2 * kSortCols];
-#pragma unroll
-    for (int p = 0; p < 2 * kSortCols; p++) dinc[p] = (pf[p] - pl[p]) - (p ? pf[p - 1] - pl[p - 1] : 0);
-    auto slot_of = [&](int j) {
-        int d = dinc[0];
-#pragma unroll
-        for (int p = 1; p < 2 * kSortCols; p++) d += j >= pl[p] ? dinc[p] : 0;
-        return j + d;
-    };
-    if (lds) {
-        constexpr int kPer = kSortCand / kSortRows;
-        const int nper = (nst + kSortRows - 1) / kSortRows;
-        int v[kPer];
-#pragma unroll
-        for (int i = 0; i < kPer; i++)
-            if (i < nper) {
-                const int j = t + i * kSortRows;
-                if (j < nst) v[i] = key[slot_of(j)];
-            }
-#pragma unroll
-        for (int i = 0; i < kPer; i++)
-            if (i < nper) {
-                const int j = t + i * kSortRows;
-                if (j < nst) s_key[j] = v[i];
-            }
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // visit the candidates of this thread's cell in old-slot order; fn(id) with
-    // id an LDS index (staged) or a slot (global)
-    const int row = T.k0 + t;
-    const bool mine = row < T.k1;
-    const int c = R.coff + T.cx * ncy + row;
-    auto visit = [&](auto fn) {
-        int ra[2], rb[2];
-        const int npr = win_pieces(row, Ry, ncy, ra, rb);
-        if (lds) {
-#pragma unroll
-            for (int q = 0; q < kSortCols; q++) {
-                if (q >= ncol) break;
-#pragma unroll
-                for (int pr = 0; pr < 2; pr++) {
-                    if (pr >= npr) break;
-                    const int ua = ra[pr] - ustart < 0 ? ra[pr] - ustart + ncy : ra[pr] - ustart;
-                    const int ub = rb[pr] - ustart < 0 ? rb[pr] - ustart + ncy : rb[pr] - ustart;
-                    const int j1 = row_lds(q, ub + 1);
-                    for (int j = row_lds(q, ua); j < j1; j++)
-                        if (s_key[j] == c) fn(j);
+        int total;
+        const int ra = 2 * t, rb = 2 * t + 1;  // this thread's rows (local)
+        if (fast) {
+            auto cell_start = [&](int r) { return lds_row(qo, u_of(T.k0 + r)); };
+            for (int j = t; j < nst; j += kSortThr) {
+                const int k = s_key[j];
+                const int r = (k & 0x7fffffff) - c0;
+                if ((unsigned)r < (unsigned)nrows) {
+                    atomicAdd(&s_cnt[r], 1);
+                    if (k < 0 && j < cell_start(r)) atomicAdd(&s_alow[r], 1);
                 }
+            }
+            __syncthreads();
+            const int ca = ra < nrows ? s_cnt[ra] : 0, cb = rb < nrows ? s_cnt[rb] : 0;
+            const int start = block_excl_scan128(ca + cb, s_tmp, total);
+            if (total > kSortCap) {
+                // reported as an error; the tile places no bead, so every later
+                // index stays in bounds
+                if (ra < nrows) locst[c0 + ra] = 0;
+                if (rb < nrows) locst[c0 + rb] = 0;
+                if (t == 0) {
+                    tcount[tix] = 0;
+                    atomicOr(err, ERR_SORT);
+                }
+                return;
+            }
+            s_start[ra] = start;
+            s_start[rb] = start + ca;
+            if (ra < nrows) locst[c0 + ra] = start;
+            if (rb < nrows) locst[c0 + rb] = start + ca;
+            __syncthreads();
+            for (int j = t; j < nst; j += kSortThr) {
+                const int k = s_key[j];
+                const int r = (k & 0x7fffffff) - c0;
+                if ((unsigned)r >= (unsigned)nrows) continue;
+                if (k < 0) {  // changed cells: ranked below, one lane per bead
+                    s_mv[atomicAdd(&s_nmv, 1)] = j;
+                    continue;
+                }
+                const int o = cell_start(r);
+                int d = 0;
+                for (int jj = o; jj < j; jj++) d += s_key[jj] < 0 ? 1 : 0;
+                s_out[s_start[r] + s_alow[r] + (j - o) - d] = j;
+            }
+            __syncthreads();
+            for (int m = t; m < s_nmv; m += kSortThr) {
+                const int j = s_mv[m];
+                const int kc = s_key[j] & 0x7fffffff;
+                const int r = kc - c0;
+                int wa[2], wb[2];
+                const int np = win_pieces(T.k0 + r, 1, ncy, wa, wb);
+                int rank = 0;
+    #pragma unroll
+                for (int q = 0; q < 3; q++)
+    #pragma unroll
+                    for (int pc = 0; pc < 2; pc++) {
+                        if (pc >= np) break;
+                        const int j0 = lds_row(q, u_of(wa[pc]));
+                        const int j1 = min(lds_after(q, u_of(wb[pc])), j);
+                        for (int jj = j0; jj < j1; jj++) rank += (s_key[jj] & 0x7fffffff) == kc ? 1 : 0;
+                    }
+                s_out[s_start[r] + rank] = j;
             }
         } else {
-#pragma unroll
-            for (int pc = 0; pc < 2; pc++) {
-                if (pc >= npc) break;
-                for (int col = ca[pc]; col <= cb[pc]; col++) {
-                    const int cbq = R.coff + col * ncy;
-#pragma unroll
-                    for (int pr = 0; pr < 2; pr++) {
-                        if (pr >= npr) break;
-                        const int j1 = cs_old[cbq + rb[pr] + 1];
-                        for (int j = cs_old[cbq + ra[pr]]; j < j1; j++)
-                            if (key[j] == c) fn(j);
+            // general path: old cells within (Rx, Ry) of each row's cell, from global memory
+            auto visit = [&](int row, auto fn) {
+                const int c = c0 + (row - T.k0);
+                int ra2[2], rb2[2], ca2[2], cb2[2];
+                const int npr = win_pieces(row, Ry, ncy, ra2, rb2);
+                const int npc = win_pieces(T.cx, Rx, ncx, ca2, cb2);
+    #pragma unroll
+                for (int pc = 0; pc < 2; pc++) {
+                    if (pc >= npc) break;
+                    for (int cc = ca2[pc]; cc <= cb2[pc]; cc++) {
+                        const int cbq = R.coff + cc * ncy;
+    #pragma unroll
+                        for (int pr = 0; pr < 2; pr++) {
+                            if (pr >= npr) break;
+                            const int j1 = cs_old[cbq + rb2[pr] + 1];
+                            for (int j = cs_old[cbq + ra2[pr]]; j < j1; j++)
+                                if ((key[j] & 0x7fffffff) == c) fn(j);
+                        }
                     }
                 }
+            };
+            int ca = 0, cb = 0;
+            if (ra < nrows) visit(T.k0 + ra, [&](int) { ca++; });
+            if (rb < nrows) visit(T.k0 + rb, [&](int) { cb++; });
+            const int start = block_excl_scan128(ca + cb, s_tmp, total);
+            if (total > kSortCap) {
+                if (ra < nrows) locst[c0 + ra] = 0;
+                if (rb < nrows) locst[c0 + rb] = 0;
+                if (t == 0) {
+                    tcount[tix] = 0;
+                    atomicOr(err, ERR_SORT);
+                }
+                return;
+            }
+            int pos = start;
+            if (ra < nrows) {
+                locst[c0 + ra] = start;
+                visit(T.k0 + ra, [&](int j) { s_out[pos++] = j; });
+            }
+            if (rb < nrows) {
+                locst[c0 + rb] = pos;
+                visit(T.k0 + rb, [&](int j) { s_out[pos++] = j; });
             }
         }
-    };
-    int cnt = 0;
-    if (mine) visit([&](int) { cnt++; });
-    int total;
-    const int start = block_excl_scan256(cnt, s_tmp, total);
-    if (total > kSortCap) {
-        // reported as an error; the tile places no bead, so every later
-        // index stays in bounds
-        if (mine) locst[c] = 0;
-        if (t == 0) {
-            tcount[blockIdx.x] = 0;
-            atomicOr(err, ERR_SORT);
+        __syncthreads();
+        const int tb = tix << kSortCapLog;
+        for (int p = t; p < total; p += kSortThr) {
+            const int j = s_out[p];
+            const int sl = fast ? slot_of(j) : j;
+            tsrc[tb + p] = sl;
+            tslot[sl] = tb + p;
         }
-        return;
+        if (t == 0) tcount[tix] = total;
+    };
+    // a grid of at most a few blocks per CU walks the tiles (fewer, longer-lived
+    // workgroups)
+    for (int tix = blockIdx.x; tix < nt; tix += gridDim.x) {
+        do_tile(tix);
+        __syncthreads();
     }
-    if (mine) {
-        int pos = start;
-        visit([&](int j) { s_out[pos++] = j; });
-        locst[c] = start;
-    }
-    __syncthreads();
-    const int tb = blockIdx.x << kSortCapLog;
-    for (int p = t; p < total; p += kSortRows) {
-        const int j = s_out[p];
-        const int sl = lds ? slot_of(j) : j;
-        tsrc[tb + p] = sl;
-        tslot[sl] = tb + p;
-    }
-    if (t == 0) tcount[blockIdx.x] = total;
 }
 
 // Exclusive scan of the tile counts -> first new slot of each tile (tiles are
@@ -475,60 +540,87 @@ struct StateP {
 // keyed): the integrate of cel/engine/oercel.c:2303-2334 fused into the
 // scatter of sort_cells, :897-921.
 template <bool INTEG>
-__global__ __launch_bounds__(kSortRows) void k_place(const RepInfo *__restrict__ ri, const Tile *__restrict__ tiles,
+__global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ ri, const Tile *__restrict__ tiles,
                                                      const int32_t *__restrict__ tcount,
                                                      const int32_t *__restrict__ toff,
                                                      const int32_t *__restrict__ locst,
                                                      const int32_t *__restrict__ tsrc,
                                                      const int32_t *__restrict__ tslot, int32_t *__restrict__ cs_new,
                                                      int32_t *__restrict__ src, StateP in, StateP out,
-                                                     const double2 *__restrict__ f, uint32_t *__restrict__ err) {
-    const Tile T = tiles[blockIdx.x];
-    const RepInfo &R = ri[T.rep];
-    const int t = threadIdx.x;
-    const int base = toff[blockIdx.x], n = tcount[blockIdx.x];
-    const int row = T.k0 + t;
-    if (row < T.k1) {
-        const int c = R.coff + T.cx * R.ncy + row;
-        cs_new[c] = base + locst[c];
-        if (T.cx == R.ncx - 1 && row == R.ncy - 1) {
-            // the replica's last tile: its beads end at the replica's last slot
-            cs_new[c + 1] = R.off + R.N;
-            if (base + n != R.off + R.N) atomicOr(err, ERR_SORT);
+                                                     const double2 *__restrict__ f, uint32_t *__restrict__ err,
+                                                     int nt) {
+    for (int tix = blockIdx.x; tix < nt; tix += gridDim.x) {
+        const Tile T = tiles[tix];
+        const RepInfo &R = ri[T.rep];
+        const int t = threadIdx.x;
+        const int base = toff[tix], n = tcount[tix];
+        for (int row = T.k0 + t; row < T.k1; row += kSortThr) {
+            const int c = R.coff + T.cx * R.ncy + row;
+            cs_new[c] = base + locst[c];
+            if (T.cx == R.ncx - 1 && row == R.ncy - 1) {
+                // the replica's last tile: its beads end at the replica's last slot
+                cs_new[c + 1] = R.off + R.N;
+                if (base + n != R.off + R.N) atomicOr(err, ERR_SORT);
+            }
         }
-    }
-    const int tb = blockIdx.x << kSortCapLog;
-    for (int p = t; p < n; p += kSortRows) {
-        const int d = base + p;
-        const int s = tsrc[tb + p];
-        double2 q = in.xy[s], vv = in.v[s];
-        float2 vt;
-        if (INTEG) {
-            double2 po, vo;
-            integrate1(R, q, vv, f[s], po, vo, vt);
-            q = po;
-            vv = vo;
-        } else {
-            vt = in.vt[s];
+        // kPl beads per thread per round, each stage issued for all of them before
+        // the next (their loads overlap)
+        constexpr int kPl = 2;
+        const int tb = tix << kSortCapLog;
+        for (int p0 = t; p0 < n; p0 += kPl * kSortThr) {
+            bool ok[kPl];
+            int sv[kPl];
+    #pragma unroll
+            for (int k = 0; k < kPl; k++) {
+                ok[k] = p0 + k * kSortThr < n;
+                sv[k] = ok[k] ? tsrc[tb + p0 + k * kSortThr] : 0;
+            }
+            double2 xq[kPl], vq[kPl], fq[kPl];
+            float2 vt[kPl];
+            uint32_t gq[kPl];
+            int8_t mq[kPl];
+            int32_t lq[kPl];
+    #pragma unroll
+            for (int k = 0; k < kPl; k++)
+                if (ok[k]) {
+                    const int s = sv[k];
+                    xq[k] = in.xy[s];
+                    vq[k] = in.v[s];
+                    if (INTEG) fq[k] = f[s];
+                    else vt[k] = in.vt[s];
+                    gq[k] = in.gt[s];
+                    mq[k] = in.mp[s];
+                    lq[k] = in.lnk[s];
+                }
+            int32_t ts[kPl];
+    #pragma unroll
+            for (int k = 0; k < kPl; k++) ts[k] = ok[k] && lq[k] >= 0 ? tslot[lq[k]] : -1;
+    #pragma unroll
+            for (int k = 0; k < kPl; k++) {
+                if (!ok[k]) continue;
+                const int d = base + p0 + k * kSortThr;
+                double2 q = xq[k], vv = vq[k];
+                if (INTEG) {
+                    double2 po, vo;
+                    integrate1(R, q, vv, fq[k], po, vo, vt[k]);
+                    q = po;
+                    vv = vo;
+                }
+                out.xy[d] = q;
+                out.v[d] = vv;
+                out.vt[d] = vt[k];
+                out.gt[d] = gq[k];
+                if (src) src[d] = sv[k];
+                const int8_t m = mq[k];
+                out.mp[d] = m;
+                const int32_t nl = ts[k] >= 0 ? toff[ts[k] >> kSortCapLog] + (ts[k] & (kSortCap - 1)) : -1;
+                out.lnk[d] = nl;
+                // every bead's prv is written once: by itself (chain head, non-lipid) or
+                // by its chain predecessor
+                if (m <= 0) out.prv[d] = -1;
+                if (m >= 0 && nl >= 0) out.prv[nl] = d;
+            }
         }
-        out.xy[d] = q;
-        out.v[d] = vv;
-        out.vt[d] = vt;
-        out.gt[d] = in.gt[s];
-        if (src) src[d] = s;
-        const int8_t m = in.mp[s];
-        out.mp[d] = m;
-        const int32_t l = in.lnk[s];
-        int32_t nl = -1;
-        if (l >= 0) {
-            const int ts = tslot[l];
-            nl = toff[ts >> kSortCapLog] + (ts & (kSortCap - 1));
-        }
-        out.lnk[d] = nl;
-        // every bead's prv is written once: by itself (chain head, non-lipid) or
-        // by its chain predecessor
-        if (m <= 0) out.prv[d] = -1;
-        if (m >= 0 && nl >= 0) out.prv[nl] = d;
     }
 }
 
@@ -904,8 +996,12 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                 const int st2 = s_row[2][klo - kb0], en2 = s_row[2][khi + 1 - kb0];
                 int j = st0 == en0 ? st1 : st0;
                 j = j == en1 ? st2 : j;
+                // the iteration count bounds the walk even for inconsistent
+                // cell starts (flagged)
+                const int nj = max(en0 - st0, 0) + max(en1 - st1, 0) + max(en2 - st2, 0);
+                if (st1 < en0 || st2 < en1) atomicOr(A.err, ERR_TILE);
                 if (CAPL == 0) {
-                    while (j < en2) {
+                    for (int it = 0; it < nj; it++) {
                         pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], 0.0, 0.0, gam, sdtk, sseed,
                                  ax, ay, e, ENERGY);
                         ++j;
@@ -915,10 +1011,10 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                 } else {
                     int cnt = 0, t = 0;
                     uint32_t pack = 0;
-                    while (j < en2) {
+                    for (int it = 0; it < nj; it++) {
                         const double2 pj = XY[j];
                         const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
-                        const float r2 = dx * dx + dy * dy;
+                        const float r2 = __builtin_fmaf(dx, dx, dy * dy);  // the test of pair_one
                         if (r2 < 1.0f && r2 > 1e-24f) {
                             if (cnt < CAPL && t < 256) {
                                 pack |= (uint32_t)t << (8 * (cnt & 3));
