@@ -549,11 +549,29 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                                                      int32_t *__restrict__ src, StateP in, StateP out,
                                                      const double2 *__restrict__ f, uint32_t *__restrict__ err,
                                                      int nt) {
-    for (int tix = blockIdx.x; tix < nt; tix += gridDim.x) {
-        const Tile T = tiles[tix];
+    // Each workgroup walks tiles blockIdx.x, + gridDim.x, ...; the next tile's
+    // header and bead list are loaded while the current one is placed.
+    constexpr int kPl = 2;  // beads per thread per round
+    const int t = threadIdx.x;
+    int tix = blockIdx.x;
+    if (tix >= nt) return;
+    Tile T;
+    int base, n, sv[kPl];
+    auto load_head = [&](int x, Tile &T_, int &b_, int &n_, int *sv_) {
+        T_ = tiles[x];
+        b_ = toff[x];
+        n_ = tcount[x];
+        const int tb_ = x << kSortCapLog;
+#pragma unroll
+        for (int k = 0; k < kPl; k++) sv_[k] = tsrc[tb_ + t + k * kSortThr];  // in bounds; used when < n_
+    };
+    load_head(tix, T, base, n, sv);
+    for (;;) {
+        const int nx = tix + gridDim.x;
+        Tile Tn;
+        int basen = 0, nn = 0, svn[kPl];
+        if (nx < nt) load_head(nx, Tn, basen, nn, svn);
         const RepInfo &R = ri[T.rep];
-        const int t = threadIdx.x;
-        const int base = toff[tix], n = tcount[tix];
         for (int row = T.k0 + t; row < T.k1; row += kSortThr) {
             const int c = R.coff + T.cx * R.ncy + row;
             cs_new[c] = base + locst[c];
@@ -563,24 +581,20 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                 if (base + n != R.off + R.N) atomicOr(err, ERR_SORT);
             }
         }
-        // kPl beads per thread per round, each stage issued for all of them before
-        // the next (their loads overlap)
-        constexpr int kPl = 2;
         const int tb = tix << kSortCapLog;
         for (int p0 = t; p0 < n; p0 += kPl * kSortThr) {
             bool ok[kPl];
-            int sv[kPl];
-    #pragma unroll
+#pragma unroll
             for (int k = 0; k < kPl; k++) {
                 ok[k] = p0 + k * kSortThr < n;
-                sv[k] = ok[k] ? tsrc[tb + p0 + k * kSortThr] : 0;
+                if (p0 != t && ok[k]) sv[k] = tsrc[tb + p0 + k * kSortThr];  // rounds after the first
             }
             double2 xq[kPl], vq[kPl], fq[kPl];
             float2 vt[kPl];
             uint32_t gq[kPl];
             int8_t mq[kPl];
             int32_t lq[kPl];
-    #pragma unroll
+#pragma unroll
             for (int k = 0; k < kPl; k++)
                 if (ok[k]) {
                     const int s = sv[k];
@@ -593,9 +607,9 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                     lq[k] = in.lnk[s];
                 }
             int32_t ts[kPl];
-    #pragma unroll
+#pragma unroll
             for (int k = 0; k < kPl; k++) ts[k] = ok[k] && lq[k] >= 0 ? tslot[lq[k]] : -1;
-    #pragma unroll
+#pragma unroll
             for (int k = 0; k < kPl; k++) {
                 if (!ok[k]) continue;
                 const int d = base + p0 + k * kSortThr;
@@ -621,6 +635,13 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                 if (m >= 0 && nl >= 0) out.prv[nl] = d;
             }
         }
+        if (nx >= nt) break;
+        tix = nx;
+        T = Tn;
+        base = basen;
+        n = nn;
+#pragma unroll
+        for (int k = 0; k < kPl; k++) sv[k] = svn[k];
     }
 }
 
@@ -1258,8 +1279,9 @@ __device__ __forceinline__ Win window_of(const RepInfo &R, const int32_t *cs, do
 }
 
 enum : uint8_t { CF_BREAK = 1, CF_CONV = 2 };
-constexpr int kChemThreads = 1024;
-constexpr int kChemGroup = 16;
+constexpr int kChemThreads = 512;
+constexpr int kChemGroup = 8;  // lanes per search
+constexpr int kChemK = 3;      // candidates per lane per chunk
 
 // Decisions from the state at the start of the check (own definition): break
 // of chemistry bonds, P -> T conversion next to a lipid tail, and the
@@ -1325,23 +1347,48 @@ __global__ __launch_bounds__(kChemThreads) void k_chem_decide(const RepInfo *__r
         uint64_t bkey = ~0ull;  // (bits of u) of the best accepted draw
         uint32_t bg = 0xffffffffu;
         int32_t bsel = -1;
-        for (int k = gl; k < W.total; k += kChemGroup) {
-            const int j = W.slot(k);
-            if (j == s) continue;
-            const uint32_t gj = gt[j];
-            if ((gj >> kGidBits) != OC_T) continue;
-            if (ty == OC_N) {
-                // a lipid tail bead within r_conv
-                if (mp[j] >= 0 && dist2(R, p, xy[j]) < R.rc2) found = true;
-            } else if (mp[j] < 0 && lnk[j] < 0 && dist2(R, p, xy[j]) < R.rf2) {
-                // a free T within r_form; draws in [0,1) order as their bits
-                const uint32_t gg = gj & kGidMask;
-                const double u = pair_u01(S.kform, g, gg);
-                const uint64_t ku = (uint64_t)__double_as_longlong(u);
-                if (u < R.p_form && (ku < bkey || (ku == bkey && gg < bg))) {
-                    bkey = ku;
-                    bg = gg;
-                    bsel = j;
+        // kChemK candidates per lane per chunk, loaded in stages (slots and
+        // types, then the T candidates' links and positions) so that the loads
+        // of a stage overlap
+        for (int k0 = gl; k0 < W.total; k0 += kChemGroup * kChemK) {
+            int jc[kChemK];
+            uint32_t gc[kChemK];
+#pragma unroll
+            for (int i = 0; i < kChemK; i++) {
+                const int k = k0 + i * kChemGroup;
+                jc[i] = k < W.total ? W.slot(k) : -1;
+                if (jc[i] == s) jc[i] = -1;
+                gc[i] = jc[i] >= 0 ? gt[jc[i]] : 0u;
+            }
+            int8_t mc[kChemK];
+            int32_t lc[kChemK];
+            double2 pc[kChemK];
+#pragma unroll
+            for (int i = 0; i < kChemK; i++) {
+                const bool isT = jc[i] >= 0 && (gc[i] >> kGidBits) == OC_T;
+                if (!isT) jc[i] = -1;
+                if (isT) {
+                    mc[i] = mp[jc[i]];
+                    lc[i] = lnk[jc[i]];
+                    pc[i] = xy[jc[i]];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kChemK; i++) {
+                if (jc[i] < 0) continue;
+                if (ty == OC_N) {
+                    // a lipid tail bead within r_conv
+                    if (mc[i] >= 0 && dist2(R, p, pc[i]) < R.rc2) found = true;
+                } else if (mc[i] < 0 && lc[i] < 0 && dist2(R, p, pc[i]) < R.rf2) {
+                    // a free T within r_form; draws in [0,1) order as their bits
+                    const uint32_t gg = gc[i] & kGidMask;
+                    const double u = pair_u01(S.kform, g, gg);
+                    const uint64_t ku = (uint64_t)__double_as_longlong(u);
+                    if (u < R.p_form && (ku < bkey || (ku == bkey && gg < bg))) {
+                        bkey = ku;
+                        bg = gg;
+                        bsel = jc[i];
+                    }
                 }
             }
         }
