@@ -184,11 +184,12 @@ __device__ __forceinline__ int win_pieces(int c, int R, int n, int *a, int *b) {
 // into the new order.  No global atomics, no histogram to clear.
 constexpr int kSortRows = 256;                // rows per sort tile
 constexpr int kSortThr = 128;                 // threads per sort tile (two rows each)
-constexpr int kSortCand = 1536;               // staged candidates
+constexpr int kSortCand = 1024;               // staged candidates
+constexpr int kSortMv = 256;                  // listed beads that changed cells (more: ranked in place)
 constexpr int kSortRowTab = kSortRows + 3;    // staged rows per column (+ end)
 constexpr int kSortCapLog = 10;
 constexpr int kSortCap = 1 << kSortCapLog;    // beads per sort tile
-constexpr int kSortGrid = 256 * 6;            // workgroups walking the sort tiles
+constexpr int kSortGrid = 256 * 12;           // workgroups walking the sort tiles
 
 __device__ __forceinline__ int block_excl_scan128(int v, int *tmp, int &total) {
     // exclusive scan over a 128-thread block; tmp: 2 ints of LDS
@@ -214,9 +215,8 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
     __shared__ int32_t s_cs[3][kSortRowTab];  // cs_old of the staged rows (slot order); extras: piece ends
     __shared__ int32_t s_e[3][2];
     __shared__ int32_t s_pc[4][3];  // per staged column: LDS base, piece counts; [3][0]: total
-    __shared__ int32_t s_out[kSortCap];
     __shared__ int32_t s_cnt[kSortRows], s_alow[kSortRows], s_start[kSortRows];
-    __shared__ int32_t s_mv[kSortCap];  // staged ids of the tile's beads that changed cells
+    __shared__ int32_t s_mv[kSortMv];  // staged ids of the tile's beads that changed cells
     __shared__ int s_nmv;
     __shared__ int s_tmp[2], s_rad[2][2];
 
@@ -348,6 +348,29 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
         }
         __syncthreads();
 
+        const int tb = tix << kSortCapLog;
+        auto put = [&](int p, int sl) {  // the tile's p-th bead is old slot sl
+            tsrc[tb + p] = sl;
+            tslot[sl] = tb + p;
+        };
+        // rank of a bead that changed cells: the beads of its new cell before it
+        // in the cell's window (old-slot order)
+        auto mover_rank = [&](int j) {
+            const int kc = s_key[j] & 0x7fffffff;
+            int wa[2], wb[2];
+            const int np = win_pieces(T.k0 + (kc - c0), 1, ncy, wa, wb);
+            int rank = 0;
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+#pragma unroll
+                for (int pc = 0; pc < 2; pc++) {
+                    if (pc >= np) break;
+                    const int j0 = lds_row(q, u_of(wa[pc]));
+                    const int j1 = min(lds_after(q, u_of(wb[pc])), j);
+                    for (int jj = j0; jj < j1; jj++) rank += (s_key[jj] & 0x7fffffff) == kc ? 1 : 0;
+                }
+            return rank;
+        };
         int total;
         const int ra = 2 * t, rb = 2 * t + 1;  // this thread's rows (local)
         if (fast) {
@@ -384,32 +407,20 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
                 const int r = (k & 0x7fffffff) - c0;
                 if ((unsigned)r >= (unsigned)nrows) continue;
                 if (k < 0) {  // changed cells: ranked below, one lane per bead
-                    s_mv[atomicAdd(&s_nmv, 1)] = j;
+                    const int m = atomicAdd(&s_nmv, 1);
+                    if (m < kSortMv) s_mv[m] = j;
+                    else put(s_start[r] + mover_rank(j), slot_of(j));  // list full: rank in place
                     continue;
                 }
                 const int o = cell_start(r);
                 int d = 0;
                 for (int jj = o; jj < j; jj++) d += s_key[jj] < 0 ? 1 : 0;
-                s_out[s_start[r] + s_alow[r] + (j - o) - d] = j;
+                put(s_start[r] + s_alow[r] + (j - o) - d, slot_of(j));
             }
             __syncthreads();
-            for (int m = t; m < s_nmv; m += kSortThr) {
+            for (int m = t; m < min(s_nmv, kSortMv); m += kSortThr) {
                 const int j = s_mv[m];
-                const int kc = s_key[j] & 0x7fffffff;
-                const int r = kc - c0;
-                int wa[2], wb[2];
-                const int np = win_pieces(T.k0 + r, 1, ncy, wa, wb);
-                int rank = 0;
-    #pragma unroll
-                for (int q = 0; q < 3; q++)
-    #pragma unroll
-                    for (int pc = 0; pc < 2; pc++) {
-                        if (pc >= np) break;
-                        const int j0 = lds_row(q, u_of(wa[pc]));
-                        const int j1 = min(lds_after(q, u_of(wb[pc])), j);
-                        for (int jj = j0; jj < j1; jj++) rank += (s_key[jj] & 0x7fffffff) == kc ? 1 : 0;
-                    }
-                s_out[s_start[r] + rank] = j;
+                put(s_start[(s_key[j] & 0x7fffffff) - c0] + mover_rank(j), slot_of(j));
             }
         } else {
             // general path: old cells within (Rx, Ry) of each row's cell, from global memory
@@ -449,20 +460,12 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
             int pos = start;
             if (ra < nrows) {
                 locst[c0 + ra] = start;
-                visit(T.k0 + ra, [&](int j) { s_out[pos++] = j; });
+                visit(T.k0 + ra, [&](int j) { put(pos++, j); });
             }
             if (rb < nrows) {
                 locst[c0 + rb] = pos;
-                visit(T.k0 + rb, [&](int j) { s_out[pos++] = j; });
+                visit(T.k0 + rb, [&](int j) { put(pos++, j); });
             }
-        }
-        __syncthreads();
-        const int tb = tix << kSortCapLog;
-        for (int p = t; p < total; p += kSortThr) {
-            const int j = s_out[p];
-            const int sl = fast ? slot_of(j) : j;
-            tsrc[tb + p] = sl;
-            tslot[sl] = tb + p;
         }
         if (t == 0) tcount[tix] = total;
     };
