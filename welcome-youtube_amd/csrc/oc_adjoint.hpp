@@ -21,10 +21,16 @@ struct RdfArgs {
 
 constexpr double kFix = 4294967296.0;  // fixed-point scale of the RDF histogram (2^32)
 
-// Visit the (ta, tb) partners of bead i within `cut`, window of +-Rc columns.
-template <typename Fn>
+// The beads within `cut` of bead i (cut < L/2 - 1): columns cx-Rc..cx+Rc and
+// the rows of [y - cut, y + cut], each column as up to three contiguous slot
+// ranges (rows wrapped below, in the box, wrapped above).  The kRdfG lanes of a
+// group deal the candidates of each range among themselves; fn(j, dx, dy)
+// gets the FP32 image difference.  Types are tested (tfilt) before the
+// position is loaded.
+constexpr int kRdfG = 8;
+template <typename Pre, typename Fn>
 __device__ __forceinline__ void visit_wide(const RepInfo &R, const int32_t *cs, const double2 *xy, int i, double2 p,
-                                           float cut, int Rc, Fn fn) {
+                                           float cut, int Rc, int gl, Pre pre, Fn fn) {
     const int cx = (int)(p.x * R.icsx) >= R.ncx ? R.ncx - 1 : (int)(p.x * R.icsx);
     const int khi = (int)((p.y + cut) * R.icsy);
     const int klo = (int)((p.y - cut) * R.icsy + 4 * R.ncy) - 4 * R.ncy;
@@ -33,42 +39,63 @@ __device__ __forceinline__ void visit_wide(const RepInfo &R, const int32_t *cs, 
         double sx = 0.0;
         if (col < 0) { col += R.ncx; sx = -R.L; }
         else if (col >= R.ncx) { col -= R.ncx; sx = R.L; }
-        for (int k = klo; k <= khi; k++) {
-            const int kk = k < 0 ? k + R.ncy : (k >= R.ncy ? k - R.ncy : k);
-            const double sy = k < 0 ? -R.L : (k >= R.ncy ? R.L : 0.0);
-            const int c = R.coff + col * R.ncy + kk;
-            for (int j = cs[c]; j < cs[c + 1]; j++) {
-                if (j == i) continue;
-                const double2 q = xy[j];
-                fn(j, (float)((p.x - q.x) - sx), (float)((p.y - q.y) - sy));
+        const int cb = R.coff + col * R.ncy;
+        for (int q = 0; q < 3; q++) {
+            int pa, pb;
+            double sy;
+            if (q == 0) { pa = klo; pb = min(khi, -1); sy = -R.L; }
+            else if (q == 1) { pa = max(klo, 0); pb = min(khi, R.ncy - 1); sy = 0.0; }
+            else { pa = max(klo, R.ncy); pb = khi; sy = R.L; }
+            if (pa > pb) continue;
+            const int sh = q == 0 ? R.ncy : (q == 2 ? -R.ncy : 0);
+            const int j1 = cs[cb + pb + sh + 1];
+            for (int j = cs[cb + pa + sh] + gl; j < j1; j += kRdfG) {
+                if (j == i || !pre(j)) continue;
+                const double2 qq = xy[j];
+                fn(j, (float)((p.x - qq.x) - sx), (float)((p.y - qq.y) - sy));
             }
         }
     }
 }
 
-// Smooth RDF histogram of the current state: per block an LDS fixed-point
-// histogram (integer atomics: order independent), written as a partial.
+// Smooth RDF histogram of the current state.  Per block (kBeadBlock beads):
+// the (ta or tb)-type beads are listed in LDS and each is handled by a group
+// of kRdfG lanes; contributions go to per-wavefront LDS fixed-point
+// histograms (integer atomics: order independent), summed into the block's
+// partial.
 __global__ __launch_bounds__(kBeadBlock) void k_rdf(const RepInfo *__restrict__ ri, const int32_t *__restrict__ bblk,
                                                     const int32_t *__restrict__ cs, const double2 *__restrict__ xy,
                                                     const uint32_t *__restrict__ gt, RdfArgs A,
                                                     unsigned long long *__restrict__ part) {
-    __shared__ unsigned long long h[256];
-    for (int b = threadIdx.x; b < A.nbins; b += blockDim.x) h[b] = 0ull;
+    __shared__ unsigned long long h[kBeadBlock / 64][256];
+    __shared__ int s_list[kBeadBlock];
+    __shared__ int s_n;
+    for (int b = threadIdx.x; b < (kBeadBlock / 64) * 256; b += kBeadBlock) h[b >> 8][b & 255] = 0ull;
+    if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     const int r = bblk[blockIdx.x];
     const RepInfo &R = ri[r];
     const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
     if (loc < R.N) {
         const int i = R.off + loc;
+        const int ti = (int)(gt[i] >> kGidBits);
+        if (ti == A.ta || ti == A.tb) s_list[atomicAdd(&s_n, 1)] = i;
+    }
+    __syncthreads();
+    unsigned long long *hw = h[threadIdx.x >> 6];
+    const int gl = threadIdx.x & (kRdfG - 1);
+    const float pref = 35.0f / (32.0f * A.width), iw = 1.0f / A.width;
+    for (int e = threadIdx.x / kRdfG; e < s_n; e += kBeadBlock / kRdfG) {
+        const int i = s_list[e];
         const uint32_t gti = gt[i], ti = gti >> kGidBits, gi = gti & kGidMask;
-        if ((int)ti == A.ta || (int)ti == A.tb) {
-            const float pref = 35.0f / (32.0f * A.width), iw = 1.0f / A.width;
-            visit_wide(R, cs, xy, i, xy[i], A.cut, A.Rc, [&](int j, float dx, float dy) {
+        visit_wide(
+            R, cs, xy, i, xy[i], A.cut, A.Rc, gl,
+            [&](int j) {
                 const uint32_t gtj = gt[j], tj = gtj >> kGidBits;
-                bool own;
-                if (A.ta == A.tb) own = (int)tj == A.ta && gi < (gtj & kGidMask);
-                else own = (int)ti == A.ta && (int)tj == A.tb;
-                if (!own) return;
+                if (A.ta == A.tb) return (int)tj == A.ta && gi < (gtj & kGidMask);
+                return (int)ti == A.ta && (int)tj == A.tb;
+            },
+            [&](int j, float dx, float dy) {
                 const float r2 = dx * dx + dy * dy;
                 if (r2 >= A.cut * A.cut || r2 == 0.0f) return;
                 const float rr = sqrtf(r2);
@@ -78,67 +105,110 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdf(const RepInfo *__restrict__ 
                     const float t = (rr - (b + 0.5f) * A.dr) * iw;
                     if (fabsf(t) >= 1.0f) continue;
                     const float q = 1.0f - t * t;
-                    atomicAdd(&h[b], (unsigned long long)((double)(pref * q * q * q) * kFix + 0.5));
+                    atomicAdd(&hw[b], (unsigned long long)((double)(pref * q * q * q) * kFix + 0.5));
                 }
             });
-        }
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < A.nbins; b += blockDim.x) part[(size_t)blockIdx.x * A.nbins + b] = h[b];
+    for (int b = threadIdx.x; b < A.nbins; b += blockDim.x) {
+        unsigned long long v = 0;
+        for (int w = 0; w < kBeadBlock / 64; w++) v += h[w][b];
+        part[(size_t)blockIdx.x * A.nbins + b] = v;
+    }
 }
 
-// window accumulator += sum of the block partials of each replica
-__global__ void k_rdf_reduce(const RepInfo *__restrict__ ri, int nbins, const unsigned long long *__restrict__ part,
-                             unsigned long long *__restrict__ acc) {
+// window accumulator += sum of the block partials of each replica (one block
+// per replica; threads split the partials of a bin, then an LDS tree)
+__global__ __launch_bounds__(1024) void k_rdf_reduce(const RepInfo *__restrict__ ri, int nbins,
+                                                     const unsigned long long *__restrict__ part,
+                                                     unsigned long long *__restrict__ acc) {
+    __shared__ unsigned long long red[1024];
     const int r = blockIdx.x;
     const RepInfo &R = ri[r];
     const int nb = (R.N + kBeadBlock - 1) / kBeadBlock;
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
-        unsigned long long s = 0;
-        for (int k = 0; k < nb; k++) s += part[(size_t)(R.bblk0 + k) * nbins + b];
-        acc[(size_t)r * nbins + b] += s;
+    const int nsub = 1024 / nbins >= 1 ? 1024 / nbins : 1;  // threads per bin
+    for (int b0 = 0; b0 < nbins; b0 += 1024 / nsub) {
+        const int b = b0 + threadIdx.x / nsub, sub = threadIdx.x % nsub;
+        unsigned long long v = 0;
+        if (b < nbins && (int)threadIdx.x < (1024 / nsub) * nsub)
+            for (int k = sub; k < nb; k += nsub) v += part[(size_t)(R.bblk0 + k) * nbins + b];
+        red[threadIdx.x] = v;
+        __syncthreads();
+        if (sub == 0 && b < nbins && (int)threadIdx.x < (1024 / nsub) * nsub) {
+            unsigned long long s = 0;
+            for (int k = 0; k < nsub; k++) s += red[threadIdx.x + k];
+            acc[(size_t)r * nbins + b] += s;
+        }
+        __syncthreads();
     }
 }
 
-// X_i += d loss / d x_i of the RDF at the current state; gw[r][b] = dL/dg_b * C_b
+// X_i += d loss / d x_i of the RDF at the current state; gw[r][b] = dL/dg_b * C_b.
+// Beads of either type are listed per block and handled by kRdfG lanes each;
+// the group's partial sums are combined by a fixed shuffle tree.
 __global__ __launch_bounds__(kBeadBlock) void k_rdf_adj(const RepInfo *__restrict__ ri,
                                                         const int32_t *__restrict__ bblk,
                                                         const int32_t *__restrict__ cs, const double2 *__restrict__ xy,
                                                         const uint32_t *__restrict__ gt, RdfArgs A,
                                                         const double *__restrict__ gw, double2 *__restrict__ X) {
     __shared__ float w_s[256];
+    __shared__ int s_list[kBeadBlock];
+    __shared__ int s_n;
     const int r = bblk[blockIdx.x];
     for (int b = threadIdx.x; b < A.nbins; b += blockDim.x) w_s[b] = (float)gw[(size_t)r * A.nbins + b];
+    if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     const RepInfo &R = ri[r];
     const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
-    if (loc >= R.N) return;
-    const int i = R.off + loc;
-    const uint32_t ti = gt[i] >> kGidBits;
-    if ((int)ti != A.ta && (int)ti != A.tb) return;
+    if (loc < R.N) {
+        const int i = R.off + loc;
+        const int ti = (int)(gt[i] >> kGidBits);
+        if (ti == A.ta || ti == A.tb) s_list[atomicAdd(&s_n, 1)] = i;
+    }
+    __syncthreads();
+    const int gl = threadIdx.x & (kRdfG - 1);
     const float pref = 35.0f / (32.0f * A.width), iw = 1.0f / A.width;
-    double ax = 0.0, ay = 0.0;
-    visit_wide(R, cs, xy, i, xy[i], A.cut, A.Rc, [&](int j, float dx, float dy) {
-        const uint32_t tj = gt[j] >> kGidBits;
-        const bool pr = ((int)ti == A.ta && (int)tj == A.tb) || ((int)ti == A.tb && (int)tj == A.ta);
-        if (!pr) return;
-        const float r2 = dx * dx + dy * dy;
-        if (r2 >= A.cut * A.cut || r2 == 0.0f) return;
-        const float rr = sqrtf(r2);
-        const int b0 = max(0, (int)floorf((rr - A.width) / A.dr));
-        const int b1 = min(A.nbins - 1, (int)floorf((rr + A.width) / A.dr));
-        float ds = 0.0f;
-        for (int b = b0; b <= b1; b++) {
-            const float t = (rr - (b + 0.5f) * A.dr) * iw;
-            if (fabsf(t) >= 1.0f) continue;
-            const float q = 1.0f - t * t;
-            ds += w_s[b] * pref * 3.0f * q * q * (-2.0f * t * iw);
+    // rounds over the list; every lane of a group takes part in the shuffles
+    const int ng = kBeadBlock / kRdfG;
+    for (int e0 = 0; e0 < s_n; e0 += ng) {
+        const int e = e0 + (int)threadIdx.x / kRdfG;
+        double ax = 0.0, ay = 0.0;
+        int i = -1;
+        if (e < s_n) {
+            i = s_list[e];
+            const uint32_t ti = gt[i] >> kGidBits;
+            visit_wide(
+                R, cs, xy, i, xy[i], A.cut, A.Rc, gl,
+                [&](int j) {
+                    const uint32_t tj = gt[j] >> kGidBits;
+                    return ((int)ti == A.ta && (int)tj == A.tb) || ((int)ti == A.tb && (int)tj == A.ta);
+                },
+                [&](int j, float dx, float dy) {
+                    const float r2 = dx * dx + dy * dy;
+                    if (r2 >= A.cut * A.cut || r2 == 0.0f) return;
+                    const float rr = sqrtf(r2);
+                    const int b0 = max(0, (int)floorf((rr - A.width) / A.dr));
+                    const int b1 = min(A.nbins - 1, (int)floorf((rr + A.width) / A.dr));
+                    float ds = 0.0f;
+                    for (int b = b0; b <= b1; b++) {
+                        const float t = (rr - (b + 0.5f) * A.dr) * iw;
+                        if (fabsf(t) >= 1.0f) continue;
+                        const float q = 1.0f - t * t;
+                        ds += w_s[b] * pref * 3.0f * q * q * (-2.0f * t * iw);
+                    }
+                    ax += (double)(ds * dx / rr);
+                    ay += (double)(ds * dy / rr);
+                });
         }
-        ax += (double)(ds * dx / rr);
-        ay += (double)(ds * dy / rr);
-    });
-    X[i].x += ax;
-    X[i].y += ay;
+        for (int o = kRdfG / 2; o > 0; o >>= 1) {
+            ax += __shfl_xor(ax, o, kRdfG);
+            ay += __shfl_xor(ay, o, kRdfG);
+        }
+        if (gl == 0 && i >= 0) {
+            X[i].x += ax;
+            X[i].y += ay;
+        }
+    }
 }
 
 __device__ __forceinline__ int theta_index(uint32_t a, uint32_t b) {

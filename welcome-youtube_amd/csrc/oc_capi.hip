@@ -1138,7 +1138,7 @@ OC_API int oc_window_forward(oc_engine *h, int32_t nsteps, int32_t ckpt_every, c
         enqueue_step(h);
         const int c = h->cur;
         k_rdf<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->d_cs[c], h->st[c].xy, h->st[c].gt, RA, a->rpart);
-        k_rdf_reduce<<<h->nrep, 64, 0, h->stream>>>(h->d_ri, nb, a->rpart, a->racc);
+        k_rdf_reduce<<<h->nrep, 1024, 0, h->stream>>>(h->d_ri, nb, a->rpart, a->racc);
     }
     if (int rc = snap_copy(h, a->end, true, false)) return rc;
     HIPCHK(hipGetLastError());
