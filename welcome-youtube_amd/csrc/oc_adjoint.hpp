@@ -28,37 +28,67 @@ constexpr double kFix = 4294967296.0;  // fixed-point scale of the RDF histogram
 // gets the FP32 image difference.  Types are tested (tfilt) before the
 // position is loaded.
 constexpr int kRdfG = 8;
+constexpr int kRdfMaxCol = 9;  // columns of the window (Rc <= 4)
+constexpr int kRdfChunk = 64;  // beads per block of the RDF kernels (256 threads: 32 groups)
+constexpr int kRdfSub = kBeadBlock / kRdfChunk;
+constexpr int kAdjG = 4;  // lanes per bead of the pair adjoint (blocks of kRdfChunk beads)
 template <typename Pre, typename Fn>
 __device__ __forceinline__ void visit_wide(const RepInfo &R, const int32_t *cs, const double2 *xy, int i, double2 p,
                                            float cut, int Rc, int gl, Pre pre, Fn fn) {
     const int cx = (int)(p.x * R.icsx) >= R.ncx ? R.ncx - 1 : (int)(p.x * R.icsx);
     const int khi = (int)((p.y + cut) * R.icsy);
     const int klo = (int)((p.y - cut) * R.icsy + 4 * R.ncy) - 4 * R.ncy;
-    for (int dc = -Rc; dc <= Rc; dc++) {
+    // the in-box row range of every column first (all loads issued together)
+    const int pa1 = max(klo, 0), pb1 = min(khi, R.ncy - 1);
+    int j0[kRdfMaxCol], j1[kRdfMaxCol], cbs[kRdfMaxCol];
+#pragma unroll
+    for (int q = 0; q < kRdfMaxCol; q++) {
+        const int dc = q - Rc;
+        if (dc > Rc) break;
         int col = cx + dc;
-        double sx = 0.0;
-        if (col < 0) { col += R.ncx; sx = -R.L; }
-        else if (col >= R.ncx) { col -= R.ncx; sx = R.L; }
-        const int cb = R.coff + col * R.ncy;
-        for (int q = 0; q < 3; q++) {
-            int pa, pb;
+        col = col < 0 ? col + R.ncx : (col >= R.ncx ? col - R.ncx : col);
+        cbs[q] = R.coff + col * R.ncy;
+        j0[q] = cs[cbs[q] + pa1];
+        j1[q] = cs[cbs[q] + pb1 + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < kRdfMaxCol; q++) {
+        const int dc = q - Rc;
+        if (dc > Rc) break;
+        const int colu = cx + dc;
+        const double sx = colu < 0 ? -R.L : (colu >= R.ncx ? R.L : 0.0);
+        for (int piece = 0; piece < 3; piece++) {
+            int a, b;
             double sy;
-            if (q == 0) { pa = klo; pb = min(khi, -1); sy = -R.L; }
-            else if (q == 1) { pa = max(klo, 0); pb = min(khi, R.ncy - 1); sy = 0.0; }
-            else { pa = max(klo, R.ncy); pb = khi; sy = R.L; }
-            if (pa > pb) continue;
-            const int sh = q == 0 ? R.ncy : (q == 2 ? -R.ncy : 0);
-            const int j1 = cs[cb + pb + sh + 1];
-            for (int j = cs[cb + pa + sh] + gl; j < j1; j += kRdfG) {
-                if (j == i || !pre(j)) continue;
-                const double2 qq = xy[j];
-                fn(j, (float)((p.x - qq.x) - sx), (float)((p.y - qq.y) - sy));
+            if (piece == 1) { a = j0[q]; b = j1[q]; sy = 0.0; }
+            else if (piece == 0) {
+                if (klo >= 0) continue;
+                a = cs[cbs[q] + klo + R.ncy]; b = cs[cbs[q] + R.ncy]; sy = -R.L;  // rows wrapped from the top
+            } else {
+                if (khi < R.ncy) continue;
+                a = cs[cbs[q]]; b = cs[cbs[q] + khi - R.ncy + 1]; sy = R.L;       // rows wrapped from the bottom
+            }
+            // candidates dealt to the group's lanes, four at a time (their type loads overlap)
+            for (int j = a + gl; j < b; j += 4 * kRdfG) {
+                bool ok[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int jj = j + u * kRdfG;
+                    ok[u] = jj < b && jj != i && pre(jj);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (ok[u]) {
+                        const int jj = j + u * kRdfG;
+                        const double2 qq = xy[jj];
+                        fn(jj, (float)((p.x - qq.x) - sx), (float)((p.y - qq.y) - sy));
+                    }
             }
         }
     }
 }
 
-// Smooth RDF histogram of the current state.  Per block (kBeadBlock beads):
+// Smooth RDF histogram of the current state.  Per block (kRdfChunk beads):
 // the (ta or tb)-type beads are listed in LDS and each is handled by a group
 // of kRdfG lanes; contributions go to per-wavefront LDS fixed-point
 // histograms (integer atomics: order independent), summed into the block's
@@ -73,10 +103,12 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdf(const RepInfo *__restrict__ 
     for (int b = threadIdx.x; b < (kBeadBlock / 64) * 256; b += kBeadBlock) h[b >> 8][b & 255] = 0ull;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    const int r = bblk[blockIdx.x];
+    // block b: beads [(b % kRdfSub) * kRdfChunk, +kRdfChunk) of bead block b / kRdfSub
+    const int bb = blockIdx.x / kRdfSub;
+    const int r = bblk[bb];
     const RepInfo &R = ri[r];
-    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
-    if (loc < R.N) {
+    const int loc = (bb - R.bblk0) * kBeadBlock + (blockIdx.x % kRdfSub) * kRdfChunk + threadIdx.x;
+    if (threadIdx.x < kRdfChunk && loc < R.N) {
         const int i = R.off + loc;
         const int ti = (int)(gt[i] >> kGidBits);
         if (ti == A.ta || ti == A.tb) s_list[atomicAdd(&s_n, 1)] = i;
@@ -113,34 +145,28 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdf(const RepInfo *__restrict__ 
     for (int b = threadIdx.x; b < A.nbins; b += blockDim.x) {
         unsigned long long v = 0;
         for (int w = 0; w < kBeadBlock / 64; w++) v += h[w][b];
-        part[(size_t)blockIdx.x * A.nbins + b] = v;
+        part[(size_t)b * gridDim.x + blockIdx.x] = v;  // bin-major: the reduction reads coalesced
     }
 }
 
-// window accumulator += sum of the block partials of each replica (one block
-// per replica; threads split the partials of a bin, then an LDS tree)
-__global__ __launch_bounds__(1024) void k_rdf_reduce(const RepInfo *__restrict__ ri, int nbins,
-                                                     const unsigned long long *__restrict__ part,
-                                                     unsigned long long *__restrict__ acc) {
-    __shared__ unsigned long long red[1024];
-    const int r = blockIdx.x;
+// window accumulator += sum of the block partials of each replica: one block
+// per (bin, replica), a strided sum then an LDS tree (integer: exact)
+__global__ __launch_bounds__(256) void k_rdf_reduce(const RepInfo *__restrict__ ri, int nblk,
+                                                    const unsigned long long *__restrict__ part,
+                                                    unsigned long long *__restrict__ acc, int nbins) {
+    __shared__ unsigned long long red[256];
+    const int b = blockIdx.x, r = blockIdx.y;
     const RepInfo &R = ri[r];
-    const int nb = (R.N + kBeadBlock - 1) / kBeadBlock;
-    const int nsub = 1024 / nbins >= 1 ? 1024 / nbins : 1;  // threads per bin
-    for (int b0 = 0; b0 < nbins; b0 += 1024 / nsub) {
-        const int b = b0 + threadIdx.x / nsub, sub = threadIdx.x % nsub;
-        unsigned long long v = 0;
-        if (b < nbins && (int)threadIdx.x < (1024 / nsub) * nsub)
-            for (int k = sub; k < nb; k += nsub) v += part[(size_t)(R.bblk0 + k) * nbins + b];
-        red[threadIdx.x] = v;
-        __syncthreads();
-        if (sub == 0 && b < nbins && (int)threadIdx.x < (1024 / nsub) * nsub) {
-            unsigned long long s = 0;
-            for (int k = 0; k < nsub; k++) s += red[threadIdx.x + k];
-            acc[(size_t)r * nbins + b] += s;
-        }
+    const int k0 = R.bblk0 * kRdfSub, nb = (R.N + kBeadBlock - 1) / kBeadBlock * kRdfSub;
+    unsigned long long v = 0;
+    for (int k = threadIdx.x; k < nb; k += 256) v += part[(size_t)b * nblk + k0 + k];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
+    if (threadIdx.x == 0) acc[(size_t)r * nbins + b] += red[0];
 }
 
 // X_i += d loss / d x_i of the RDF at the current state; gw[r][b] = dL/dg_b * C_b.
@@ -154,13 +180,14 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdf_adj(const RepInfo *__restric
     __shared__ float w_s[256];
     __shared__ int s_list[kBeadBlock];
     __shared__ int s_n;
-    const int r = bblk[blockIdx.x];
+    const int bb = blockIdx.x / kRdfSub;
+    const int r = bblk[bb];
     for (int b = threadIdx.x; b < A.nbins; b += blockDim.x) w_s[b] = (float)gw[(size_t)r * A.nbins + b];
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     const RepInfo &R = ri[r];
-    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
-    if (loc < R.N) {
+    const int loc = (bb - R.bblk0) * kBeadBlock + (blockIdx.x % kRdfSub) * kRdfChunk + threadIdx.x;
+    if (threadIdx.x < kRdfChunk && loc < R.N) {
         const int i = R.off + loc;
         const int ti = (int)(gt[i] >> kGidBits);
         if (ti == A.ta || ti == A.tb) s_list[atomicAdd(&s_n, 1)] = i;
@@ -211,6 +238,29 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdf_adj(const RepInfo *__restric
     }
 }
 
+// Several device-to-device copies in one launch (the snapshots of the window
+// gradient): 16-byte words where source and destination are aligned, bytes
+// for the rest.
+constexpr int kCopyMax = 12;
+struct CopyJobs {
+    char *dst[kCopyMax];
+    const char *src[kCopyMax];
+    int64_t bytes[kCopyMax];
+    int n;
+};
+__global__ __launch_bounds__(256) void k_copy_multi(CopyJobs J) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, nth = (int64_t)gridDim.x * blockDim.x;
+    for (int k = 0; k < J.n; k++) {
+        char *d = J.dst[k];
+        const char *sp = J.src[k];
+        const int64_t nb = J.bytes[k];
+        const bool al = (((uintptr_t)d | (uintptr_t)sp) & 15) == 0;
+        const int64_t n16 = al ? nb >> 4 : 0;
+        for (int64_t w = tid; w < n16; w += nth) ((uint4 *)d)[w] = ((const uint4 *)sp)[w];
+        for (int64_t b = n16 * 16 + tid; b < nb; b += nth) d[b] = sp[b];
+    }
+}
+
 __device__ __forceinline__ int theta_index(uint32_t a, uint32_t b) {
     if (a > b) { const uint32_t t = a; a = b; b = t; }
     if (b > 2) return -1;
@@ -228,12 +278,16 @@ __global__ __launch_bounds__(kBeadBlock) void k_pair_adj(const RepInfo *__restri
                                                          const uint32_t *__restrict__ gt,
                                                          const double2 *__restrict__ Fa, double2 *__restrict__ X,
                                                          double2 *__restrict__ T, double *__restrict__ thb) {
-    const int r = bblk[blockIdx.x];
+    // block b: beads [(b % kRdfSub) * kRdfChunk, +kRdfChunk) of bead block
+    // b / kRdfSub, kAdjG lanes per bead
+    const int bb = blockIdx.x / kRdfSub;
+    const int r = bblk[bb];
     const RepInfo &R = ri[r];
     const RepStep &S = rs[r];
-    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
-    if (loc >= R.N) return;
-    const int i = R.off + loc;
+    const int loc = (bb - R.bblk0) * kBeadBlock + (blockIdx.x % kRdfSub) * kRdfChunk + (int)threadIdx.x / kAdjG;
+    const int gl = threadIdx.x & (kAdjG - 1);
+    const bool live = loc < R.N;
+    const int i = R.off + (live ? loc : 0);
     const double2 pi = xy[i];
     const float2 vi = vt[i];
     const uint32_t gti = gt[i], ti = gti >> kGidBits, gi = gti & kGidMask;
@@ -245,54 +299,81 @@ __global__ __launch_bounds__(kBeadBlock) void k_pair_adj(const RepInfo *__restri
     const int klo = (int)((pi.y - 1.0) * R.icsy + ncy) - ncy;
     double xa = 0.0, ya = 0.0, ta = 0.0, tb = 0.0;
     double th[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int dc = -1; dc <= 1; dc++) {
-        int col = cx + dc;
-        double sx = 0.0;
-        if (col < 0) { col += ncx; sx = -R.L; }
-        else if (col >= ncx) { col -= ncx; sx = R.L; }
-        for (int k = klo; k <= khi; k++) {
-            const int kk = k < 0 ? k + ncy : (k >= ncy ? k - ncy : k);
-            const double sy = k < 0 ? -R.L : (k >= ncy ? R.L : 0.0);
-            const int c = R.coff + col * ncy + kk;
-            for (int j = cs[c]; j < cs[c + 1]; j++) {
-                const double2 pj = xy[j];
-                const float dx = (float)((pi.x - pj.x) - sx), dy = (float)((pi.y - pj.y) - sy);
-                const float r2 = dx * dx + dy * dy;
-                if (!(r2 < 1.0f && r2 > 1e-24f)) continue;
-                const float ir = __builtin_amdgcn_rsqf(r2);
-                const float rr = r2 * ir, w = 1.0f - rr;
-                const float ex = dx * ir, ey = dy * ir;
-                const float2 vj = vt[j];
-                const float ux = vi.x - vj.x, uy = vi.y - vj.y;
-                const float eu = ex * ux + ey * uy;
-                const uint32_t gtj = gt[j], tj = gtj >> kGidBits;
-                const uint64_t hsh = pair_hash(S.sseed, gi, gtj & kGidMask);
-                const float thn = (float)(int32_t)(uint32_t)(hsh >> 32) * kTheta32;
-                const float a = S.a_eff[8 * ti + tj];
-                const float m = w * (a - gam * w * eu + sdt * thn);
-                const float dmr = -(a + sdt * thn) + 2.0f * gam * w * eu;
-                const float dmu = -gam * w * w;
-                const double2 Fj = Fa[j];
-                const float lx = (float)(Fi.x - Fj.x), ly = (float)(Fi.y - Fj.y);
-                const float el = ex * lx + ey * ly;
-                const float pux = ux - eu * ex, puy = uy - eu * ey;
-                const float plx = lx - el * ex, ply = ly - el * ey;
-                xa += (double)((dmr * ex + dmu * pux * ir) * el + m * plx * ir);
-                ya += (double)((dmr * ey + dmu * puy * ir) * el + m * ply * ir);
-                ta += (double)(dmu * el * ex);
-                tb += (double)(dmu * el * ey);
-                const double ha = (double)(0.5f * w * el);
-                const int q = theta_index(ti, tj);
-                if (q == 0) th[0] += ha;
-                else if (q == 1) th[1] += ha;
-                else if (q == 2) th[2] += ha;
-                else if (q == 3) th[3] += ha;
-                else if (q == 4) th[4] += ha;
-                else if (q == 5) th[5] += ha;
-                th[6] += (double)(0.5f * el * (-w * w * eu + sdg * w * thn));
+    auto one = [&](int j, double sx, double sy) {
+        const double2 pj = xy[j];
+        const float dx = (float)((pi.x - pj.x) - sx), dy = (float)((pi.y - pj.y) - sy);
+        const float r2 = dx * dx + dy * dy;
+        if (!(r2 < 1.0f && r2 > 1e-24f)) return;
+        const float ir = __builtin_amdgcn_rsqf(r2);
+        const float rr = r2 * ir, w = 1.0f - rr;
+        const float ex = dx * ir, ey = dy * ir;
+        const float2 vj = vt[j];
+        const float ux = vi.x - vj.x, uy = vi.y - vj.y;
+        const float eu = ex * ux + ey * uy;
+        const uint32_t gtj = gt[j], tj = gtj >> kGidBits;
+        const uint64_t hsh = pair_hash(S.sseed, gi, gtj & kGidMask);
+        const float thn = (float)(int32_t)(uint32_t)(hsh >> 32) * kTheta32;
+        const float a = S.a_eff[8 * ti + tj];
+        const float m = w * (a - gam * w * eu + sdt * thn);
+        const float dmr = -(a + sdt * thn) + 2.0f * gam * w * eu;
+        const float dmu = -gam * w * w;
+        const double2 Fj = Fa[j];
+        const float lx = (float)(Fi.x - Fj.x), ly = (float)(Fi.y - Fj.y);
+        const float el = ex * lx + ey * ly;
+        const float pux = ux - eu * ex, puy = uy - eu * ey;
+        const float plx = lx - el * ex, ply = ly - el * ey;
+        xa += (double)((dmr * ex + dmu * pux * ir) * el + m * plx * ir);
+        ya += (double)((dmr * ey + dmu * puy * ir) * el + m * ply * ir);
+        ta += (double)(dmu * el * ex);
+        tb += (double)(dmu * el * ey);
+        const double ha = (double)(0.5f * w * el);
+        const int q = theta_index(ti, tj);
+        if (q == 0) th[0] += ha;
+        else if (q == 1) th[1] += ha;
+        else if (q == 2) th[2] += ha;
+        else if (q == 3) th[3] += ha;
+        else if (q == 4) th[4] += ha;
+        else if (q == 5) th[5] += ha;
+        th[6] += (double)(0.5f * el * (-w * w * eu + sdg * w * thn));
+    };
+    if (live) {
+        // the in-box row range of each column first (loads issued together), then
+        // rows wrapped at the periodic seam; candidates dealt to the group's lanes
+        const int pa1 = max(klo, 0), pb1 = min(khi, ncy - 1);
+        int j0[3], j1[3], cb[3];
+        double sxs[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            int col = cx + c - 1;
+            sxs[c] = col < 0 ? -R.L : (col >= ncx ? R.L : 0.0);
+            col = col < 0 ? col + ncx : (col >= ncx ? col - ncx : col);
+            cb[c] = R.coff + col * ncy;
+            j0[c] = cs[cb[c] + pa1];
+            j1[c] = cs[cb[c] + pb1 + 1];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            if (klo < 0) {
+                const int e1 = cs[cb[c] + ncy];
+                for (int j = cs[cb[c] + klo + ncy] + gl; j < e1; j += kAdjG) one(j, sxs[c], -R.L);
+            }
+            for (int j = j0[c] + gl; j < j1[c]; j += kAdjG) one(j, sxs[c], 0.0);
+            if (khi >= ncy) {
+                const int e1 = cs[cb[c] + khi - ncy + 1];
+                for (int j = cs[cb[c]] + gl; j < e1; j += kAdjG) one(j, sxs[c], R.L);
             }
         }
     }
+    // sums over the group (fixed shuffle tree), written by its first lane
+    for (int o = kAdjG / 2; o > 0; o >>= 1) {
+        xa += __shfl_xor(xa, o, kAdjG);
+        ya += __shfl_xor(ya, o, kAdjG);
+        ta += __shfl_xor(ta, o, kAdjG);
+        tb += __shfl_xor(tb, o, kAdjG);
+#pragma unroll
+        for (int q = 0; q < 7; q++) th[q] += __shfl_xor(th[q], o, kAdjG);
+    }
+    if (!live || gl != 0) return;
     X[i].x += xa;
     X[i].y += ya;
     T[i] = make_double2(ta, tb);

@@ -1005,24 +1005,31 @@ static int snap_alloc(oc_engine *h, Adj *a, StateSnap &s, bool full, bool with_s
 
 // copy the engine's current state into (save) or out of (restore) a snapshot
 static int snap_copy(oc_engine *h, StateSnap &s, bool save, bool with_src) {
+    // all arrays in one launch (k_copy_multi)
     const size_t N = h->Ntot, C = h->Ctot + 1;
     const int c = h->cur;
-    auto cp = [&](void *eng, void *snp, size_t bytes) -> hipError_t {
-        return save ? hipMemcpyAsync(snp, eng, bytes, hipMemcpyDeviceToDevice, h->stream)
-                    : hipMemcpyAsync(eng, snp, bytes, hipMemcpyDeviceToDevice, h->stream);
+    CopyJobs J;
+    J.n = 0;
+    auto add = [&](void *eng, void *snp, size_t bytes) {
+        J.dst[J.n] = (char *)(save ? snp : eng);
+        J.src[J.n] = (const char *)(save ? eng : snp);
+        J.bytes[J.n] = (int64_t)bytes;
+        J.n++;
     };
-    HIPCHK(cp(h->st[c].xy, s.xy, N * sizeof(double2)));
-    HIPCHK(cp(h->st[c].vt, s.vt, N * sizeof(float2)));
-    HIPCHK(cp(h->st[c].gt, s.gt, N * sizeof(uint32_t)));
-    HIPCHK(cp(h->st[c].mp, s.mp, N));
-    HIPCHK(cp(h->st[c].lnk, s.lnk, N * sizeof(int32_t)));
-    HIPCHK(cp(h->d_cs[c], s.cs, C * sizeof(int32_t)));
-    if (with_src && save) HIPCHK(hipMemcpyAsync(s.src, h->d_src, N * sizeof(int32_t), hipMemcpyDeviceToDevice, h->stream));
+    add(h->st[c].xy, s.xy, N * sizeof(double2));
+    add(h->st[c].vt, s.vt, N * sizeof(float2));
+    add(h->st[c].gt, s.gt, N * sizeof(uint32_t));
+    add(h->st[c].mp, s.mp, N);
+    add(h->st[c].lnk, s.lnk, N * sizeof(int32_t));
+    add(h->d_cs[c], s.cs, C * sizeof(int32_t));
+    if (with_src && save) add(h->d_src, s.src, N * sizeof(int32_t));
     if (s.v) {
-        HIPCHK(cp(h->st[c].v, s.v, N * sizeof(double2)));
-        HIPCHK(cp(h->d_f, s.f, N * sizeof(double2)));
-        HIPCHK(cp(h->d_rs, s.rs, h->nrep * sizeof(RepStep)));
+        add(h->st[c].v, s.v, N * sizeof(double2));
+        add(h->d_f, s.f, N * sizeof(double2));
+        add(h->d_rs, s.rs, h->nrep * sizeof(RepStep));
     }
+    k_copy_multi<<<512, 256, 0, h->stream>>>(J);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
@@ -1050,7 +1057,7 @@ static int adj_setup(oc_engine *h, int K, int nseg, int nbins) {
     rc = rc ? rc : aalloc(a, &a->thb, N * 8);
     rc = rc ? rc : aalloc(a, &a->gw, (size_t)h->nrep * nbins);
     rc = rc ? rc : aalloc(a, &a->grad, (size_t)h->nrep * 8);
-    rc = rc ? rc : aalloc(a, &a->rpart, h->bblk.size() * (size_t)nbins);
+    rc = rc ? rc : aalloc(a, &a->rpart, h->bblk.size() * (size_t)kRdfSub * nbins);
     rc = rc ? rc : aalloc(a, &a->racc, (size_t)h->nrep * nbins);
     rc = rc ? rc : aalloc(a, &a->base, h->nrep);
     if (rc) adj_free(h);
@@ -1078,6 +1085,7 @@ OC_API int oc_window_forward(oc_engine *h, int32_t nsteps, int32_t ckpt_every, c
             return fail(OC_EINVAL, "window gradient needs k_angle = 0 and no stirring, kT cycle or chemistry");
         Rc = std::max(Rc, (int)ceil(cut * R.icsx));
     }
+    if (2 * Rc + 1 > kRdfMaxCol) return fail(OC_EINVAL, "rdf cutoff longer than four cell columns");
     for (int r = 0; r < h->nrep; r++)
         if (2 * Rc + 1 > h->ri[r].ncx || 2.0 * cut + 2.0 > h->ri[r].L)
             return fail(OC_EINVAL, "box too small for the RDF cutoff");
@@ -1137,8 +1145,8 @@ OC_API int oc_window_forward(oc_engine *h, int32_t nsteps, int32_t ckpt_every, c
             if (int rc = snap_copy(h, a->ckpt[n / K], true, false)) return rc;
         enqueue_step(h);
         const int c = h->cur;
-        k_rdf<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->d_cs[c], h->st[c].xy, h->st[c].gt, RA, a->rpart);
-        k_rdf_reduce<<<h->nrep, 1024, 0, h->stream>>>(h->d_ri, nb, a->rpart, a->racc);
+        k_rdf<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->d_cs[c], h->st[c].xy, h->st[c].gt, RA, a->rpart);
+        k_rdf_reduce<<<dim3(nb, h->nrep), 256, 0, h->stream>>>(h->d_ri, nbk * kRdfSub, a->rpart, a->racc, nb);
     }
     if (int rc = snap_copy(h, a->end, true, false)) return rc;
     HIPCHK(hipGetLastError());
@@ -1186,12 +1194,12 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
             zero_f64(h, (double *)a->V[ab], 2 * N);
             zero_f64(h, (double *)a->F[ab], 2 * N);
             StateSnap &t = a->tape[len];
-            k_rdf_adj<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, t.cs, t.xy, t.gt, RA, a->gw, a->X[ab]);
+            k_rdf_adj<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, t.cs, t.xy, t.gt, RA, a->gw, a->X[ab]);
         }
         for (int m = len; m >= 1; m--) {
             StateSnap &t = a->tape[m];
             k_step_params<<<h->nrep, 64, 0, h->stream>>>(h->d_ri, h->d_rs, n0 + m, -1, a->base);
-            k_pair_adj<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, t.cs, t.xy, t.vt, t.gt,
+            k_pair_adj<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, t.cs, t.xy, t.vt, t.gt,
                                                          a->F[ab], a->X[ab], a->T, a->thb);
             k_bond_adj<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, t.xy, t.gt, t.mp, t.lnk, a->F[ab],
                                                          a->X[ab]);
@@ -1200,7 +1208,7 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
             ab ^= 1;
             if (n0 + m - 1 >= 1) {
                 StateSnap &tp = a->tape[m - 1];
-                k_rdf_adj<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, tp.cs, tp.xy, tp.gt, RA, a->gw,
+                k_rdf_adj<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, tp.cs, tp.xy, tp.gt, RA, a->gw,
                                                             a->X[ab]);
             }
         }
@@ -1209,7 +1217,7 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
     {
         StateSnap &t = a->tape[0];
         k_step_params<<<h->nrep, 64, 0, h->stream>>>(h->d_ri, h->d_rs, 0, -1, a->base);
-        k_pair_adj<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, t.cs, t.xy, t.vt, t.gt, a->F[ab],
+        k_pair_adj<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, t.cs, t.xy, t.vt, t.gt, a->F[ab],
                                                      a->X[ab], a->T, a->thb);
     }
     k_theta_reduce<<<h->nrep, 256, 0, h->stream>>>(h->d_ri, a->thb, a->grad);
