@@ -27,9 +27,9 @@ constexpr double kFix = 4294967296.0;  // fixed-point scale of the RDF histogram
 // group deal the candidates of each range among themselves; fn(j, dx, dy)
 // gets the FP32 image difference.  Types are tested (tfilt) before the
 // position is loaded.
-constexpr int kRdfG = 8;
+constexpr int kRdfG = 16;
 constexpr int kRdfMaxCol = 9;  // columns of the window (Rc <= 4)
-constexpr int kRdfChunk = 64;  // beads per block of the RDF kernels (256 threads: 32 groups)
+constexpr int kRdfChunk = 64;  // beads per block of the RDF kernels (256 threads: 16 groups)
 constexpr int kRdfSub = kBeadBlock / kRdfChunk;
 constexpr int kAdjG = 4;  // lanes per bead of the pair adjoint (blocks of kRdfChunk beads)
 template <typename Pre, typename Fn>
@@ -51,39 +51,54 @@ __device__ __forceinline__ void visit_wide(const RepInfo &R, const int32_t *cs, 
         j0[q] = cs[cbs[q] + pa1];
         j1[q] = cs[cbs[q] + pb1 + 1];
     }
+    double sxq[kRdfMaxCol];
+    int lmax = 0;
 #pragma unroll
     for (int q = 0; q < kRdfMaxCol; q++) {
         const int dc = q - Rc;
         if (dc > Rc) break;
         const int colu = cx + dc;
-        const double sx = colu < 0 ? -R.L : (colu >= R.ncx ? R.L : 0.0);
-        for (int piece = 0; piece < 3; piece++) {
+        sxq[q] = colu < 0 ? -R.L : (colu >= R.ncx ? R.L : 0.0);
+        lmax = max(lmax, j1[q] - j0[q]);
+    }
+    // in-box rows: pass u takes offset gl + u*kRdfG of every column; a pass
+    // issues the type tests of all columns, then the positions of the matches
+    for (int off = gl; off < lmax; off += kRdfG) {
+        int jq[kRdfMaxCol];
+#pragma unroll
+        for (int q = 0; q < kRdfMaxCol; q++) {
+            jq[q] = -1;
+            if (q - Rc > Rc) continue;
+            const int j = j0[q] + off;
+            if (j < j1[q] && j != i && pre(j)) jq[q] = j;
+        }
+#pragma unroll
+        for (int q = 0; q < kRdfMaxCol; q++)
+            if (jq[q] >= 0) {
+                const double2 qq = xy[jq[q]];
+                fn(jq[q], (float)((p.x - qq.x) - sxq[q]), (float)((p.y - qq.y)));
+            }
+    }
+    // rows wrapped at the periodic seam (beads within cut of y = 0 or L)
+    if (klo >= 0 && khi < R.ncy) return;
+#pragma unroll
+    for (int q = 0; q < kRdfMaxCol; q++) {
+        if (q - Rc > Rc) break;
+        for (int piece = 0; piece < 2; piece++) {
             int a, b;
             double sy;
-            if (piece == 1) { a = j0[q]; b = j1[q]; sy = 0.0; }
-            else if (piece == 0) {
+            if (piece == 0) {
                 if (klo >= 0) continue;
                 a = cs[cbs[q] + klo + R.ncy]; b = cs[cbs[q] + R.ncy]; sy = -R.L;  // rows wrapped from the top
             } else {
                 if (khi < R.ncy) continue;
                 a = cs[cbs[q]]; b = cs[cbs[q] + khi - R.ncy + 1]; sy = R.L;       // rows wrapped from the bottom
             }
-            // candidates dealt to the group's lanes, four at a time (their type loads overlap)
-            for (int j = a + gl; j < b; j += 4 * kRdfG) {
-                bool ok[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int jj = j + u * kRdfG;
-                    ok[u] = jj < b && jj != i && pre(jj);
+            for (int j = a + gl; j < b; j += kRdfG)
+                if (j != i && pre(j)) {
+                    const double2 qq = xy[j];
+                    fn(j, (float)((p.x - qq.x) - sxq[q]), (float)((p.y - qq.y) - sy));
                 }
-#pragma unroll
-                for (int u = 0; u < 4; u++)
-                    if (ok[u]) {
-                        const int jj = j + u * kRdfG;
-                        const double2 qq = xy[jj];
-                        fn(jj, (float)((p.x - qq.x) - sx), (float)((p.y - qq.y) - sy));
-                    }
-            }
         }
     }
 }
