@@ -21,128 +21,205 @@ struct RdfArgs {
 
 constexpr double kFix = 4294967296.0;  // fixed-point scale of the RDF histogram (2^32)
 
-// The beads within `cut` of bead i (cut < L/2 - 1): columns cx-Rc..cx+Rc and
-// the rows of [y - cut, y + cut], each column as up to three contiguous slot
-// ranges (rows wrapped below, in the box, wrapped above).  The kRdfG lanes of a
-// group deal the candidates of each range among themselves; fn(j, dx, dy)
-// gets the FP32 image difference.  Types are tested (tfilt) before the
-// position is loaded.
-constexpr int kRdfG = 16;
 constexpr int kRdfMaxCol = 9;  // columns of the window (Rc <= 4)
-constexpr int kRdfChunk = 64;  // beads per block of the RDF kernels (256 threads: 16 groups)
+constexpr int kRdfChunk = 64;  // beads per block of the pair adjoint
 constexpr int kRdfSub = kBeadBlock / kRdfChunk;
 constexpr int kAdjG = 4;  // lanes per bead of the pair adjoint (blocks of kRdfChunk beads)
+constexpr int kTlG = 4;   // lanes per listed bead of the RDF kernels
+
+// One recorded state of the window gradient's tape, in that state's slot
+// order: cell starts, positions, gt words, the RDF gradient (output), and the
+// type list of the RDF: for every column, its beads of type ta or tb in slot
+// order, stored from the column's first slot (lxy, lgt, lidx = slot; lidx = -1
+// past the column's list), with lcs[col0 + col * (ncy + 1) + k] the list
+// position of row k's first listed bead (k = ncy: the column's list end).
+struct TapeRef {
+    const int32_t *cs;
+    const double2 *xy;
+    const uint32_t *gt;
+    double2 *rg;
+    double2 *lxy;
+    uint32_t *lgt;
+    int32_t *lidx, *lcs;
+};
+
+__device__ __forceinline__ int block_excl_scan256(int v, int *tmp, int &total) {
+    // exclusive scan over a 256-thread block; tmp: 4 ints of LDS
+    const int inc = wave_incl_scan(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) tmp[w] = inc;
+    __syncthreads();
+    int off = 0;
+    for (int k = 0; k < w; k++) off += tmp[k];
+    total = tmp[0] + tmp[1] + tmp[2] + tmp[3];
+    __syncthreads();
+    return off + inc - v;
+}
+
+// Type list of every tape state of a segment: block (column, state).  Rows
+// are taken 256 at a time, one thread each: count the row's listed beads, scan,
+// then write them in slot order.  With zero_rg the other beads' RDF gradient
+// is set to zero.
+__global__ __launch_bounds__(256) void k_tlist(const RepInfo *__restrict__ ri, const int2 *__restrict__ cols,
+                                               const TapeRef *__restrict__ tape, RdfArgs A, int zero_rg) {
+    __shared__ int s_tmp[4];
+    const TapeRef S = tape[blockIdx.y];
+    const int2 rc = cols[blockIdx.x];  // (replica, column)
+    const RepInfo &R = ri[rc.x];
+    const int cb = R.coff + rc.y * R.ncy, lb = R.coff + R.col0 + rc.y * (R.ncy + 1);
+    const int s1 = S.cs[cb + R.ncy];
+    int run = S.cs[cb];
+    for (int k0 = 0; k0 < R.ncy; k0 += 256) {
+        const int k = k0 + (int)threadIdx.x;
+        int a = 0, e = 0, cnt = 0;
+        if (k < R.ncy) {
+            a = S.cs[cb + k];
+            e = S.cs[cb + k + 1];
+            for (int d = a; d < e; d++) {
+                const int t = (int)(S.gt[d] >> kGidBits);
+                cnt += t == A.ta || t == A.tb;
+            }
+        }
+        int total;
+        const int base = run + block_excl_scan256(cnt, s_tmp, total);
+        if (k < R.ncy) {
+            S.lcs[lb + k] = base;
+            int q = base;
+            for (int d = a; d < e; d++) {
+                const uint32_t g = S.gt[d];
+                const int t = (int)(g >> kGidBits);
+                if (t == A.ta || t == A.tb) {
+                    S.lxy[q] = S.xy[d];
+                    S.lgt[q] = g;
+                    S.lidx[q] = d;
+                    q++;
+                } else if (zero_rg) {
+                    S.rg[d] = make_double2(0.0, 0.0);
+                }
+            }
+        }
+        run += total;
+    }
+    if (threadIdx.x == 0) S.lcs[lb + R.ncy] = run;
+    for (int q = run + (int)threadIdx.x; q < s1; q += 256) S.lidx[q] = -1;
+}
+
+// The listed beads within `cut` of listed bead i (cut < L/2 - 1): columns
+// cx-Rc..cx+Rc and the rows of [y - cut, y + cut], each column as up to three
+// contiguous list ranges (rows wrapped below, in the box, wrapped above).  The
+// kTlG lanes of a group deal the candidates among themselves: a pass takes
+// offset gl + u*kTlG of every column, loading the gt words of all columns,
+// then the positions (lanes without a candidate re-read bead i), so a pass
+// costs two memory latencies.  pre(j, gt) selects a candidate and fn(j, dx, dy)
+// gets the FP32 image difference.
 template <typename Pre, typename Fn>
-__device__ __forceinline__ void visit_wide(const RepInfo &R, const int32_t *cs, const double2 *xy, int i, double2 p,
-                                           float cut, int Rc, int gl, Pre pre, Fn fn) {
+__device__ __forceinline__ void visit_list(const RepInfo &R, const TapeRef &S, int i, double2 p, float cut, int Rc,
+                                           int gl, Pre pre, Fn fn) {
     const int cx = (int)(p.x * R.icsx) >= R.ncx ? R.ncx - 1 : (int)(p.x * R.icsx);
     const int khi = (int)((p.y + cut) * R.icsy);
     const int klo = (int)((p.y - cut) * R.icsy + 4 * R.ncy) - 4 * R.ncy;
-    // the in-box row range of every column first (all loads issued together)
     const int pa1 = max(klo, 0), pb1 = min(khi, R.ncy - 1);
     int j0[kRdfMaxCol], j1[kRdfMaxCol], cbs[kRdfMaxCol];
+    double sxq[kRdfMaxCol];
 #pragma unroll
     for (int q = 0; q < kRdfMaxCol; q++) {
-        const int dc = q - Rc;
-        if (dc > Rc) break;
-        int col = cx + dc;
-        col = col < 0 ? col + R.ncx : (col >= R.ncx ? col - R.ncx : col);
-        cbs[q] = R.coff + col * R.ncy;
-        j0[q] = cs[cbs[q] + pa1];
-        j1[q] = cs[cbs[q] + pb1 + 1];
+        const int colu = cx + q - Rc;
+        const int col = colu < 0 ? colu + R.ncx : (colu >= R.ncx ? colu - R.ncx : colu);
+        sxq[q] = colu < 0 ? -R.L : (colu >= R.ncx ? R.L : 0.0);
+        cbs[q] = R.coff + R.col0 + col * (R.ncy + 1);
+        const bool act = q <= 2 * Rc;  // columns past 2 Rc + 1 are empty
+        j0[q] = act ? S.lcs[cbs[q] + pa1] : 0;
+        j1[q] = act ? S.lcs[cbs[q] + pb1 + 1] : 0;
     }
-    double sxq[kRdfMaxCol];
     int lmax = 0;
 #pragma unroll
-    for (int q = 0; q < kRdfMaxCol; q++) {
-        const int dc = q - Rc;
-        if (dc > Rc) break;
-        const int colu = cx + dc;
-        sxq[q] = colu < 0 ? -R.L : (colu >= R.ncx ? R.L : 0.0);
-        lmax = max(lmax, j1[q] - j0[q]);
-    }
-    // in-box rows: pass u takes offset gl + u*kRdfG of every column; a pass
-    // issues the type tests of all columns, then the positions of the matches
-    for (int off = gl; off < lmax; off += kRdfG) {
+    for (int q = 0; q < kRdfMaxCol; q++) lmax = max(lmax, j1[q] - j0[q]);
+    for (int off = gl; off < lmax; off += kTlG) {
         int jq[kRdfMaxCol];
+        uint32_t gq[kRdfMaxCol];
 #pragma unroll
         for (int q = 0; q < kRdfMaxCol; q++) {
-            jq[q] = -1;
-            if (q - Rc > Rc) continue;
             const int j = j0[q] + off;
-            if (j < j1[q] && j != i && pre(j)) jq[q] = j;
+            jq[q] = j < j1[q] ? j : i;
+            gq[q] = S.lgt[jq[q]];
+        }
+        double2 pq[kRdfMaxCol];
+#pragma unroll
+        for (int q = 0; q < kRdfMaxCol; q++) {
+            if (jq[q] == i || !pre(gq[q])) jq[q] = -1;
+            pq[q] = S.lxy[jq[q] >= 0 ? jq[q] : i];
         }
 #pragma unroll
         for (int q = 0; q < kRdfMaxCol; q++)
-            if (jq[q] >= 0) {
-                const double2 qq = xy[jq[q]];
-                fn(jq[q], (float)((p.x - qq.x) - sxq[q]), (float)((p.y - qq.y)));
-            }
+            if (jq[q] >= 0) fn((float)((p.x - pq[q].x) - sxq[q]), (float)(p.y - pq[q].y));
     }
     // rows wrapped at the periodic seam (beads within cut of y = 0 or L)
     if (klo >= 0 && khi < R.ncy) return;
 #pragma unroll
     for (int q = 0; q < kRdfMaxCol; q++) {
-        if (q - Rc > Rc) break;
+        if (q > 2 * Rc) continue;
         for (int piece = 0; piece < 2; piece++) {
-            int a, b;
+            int a, e;
             double sy;
             if (piece == 0) {
                 if (klo >= 0) continue;
-                a = cs[cbs[q] + klo + R.ncy]; b = cs[cbs[q] + R.ncy]; sy = -R.L;  // rows wrapped from the top
+                a = S.lcs[cbs[q] + klo + R.ncy]; e = S.lcs[cbs[q] + R.ncy]; sy = -R.L;  // rows wrapped from the top
             } else {
                 if (khi < R.ncy) continue;
-                a = cs[cbs[q]]; b = cs[cbs[q] + khi - R.ncy + 1]; sy = R.L;       // rows wrapped from the bottom
+                a = S.lcs[cbs[q]]; e = S.lcs[cbs[q] + khi - R.ncy + 1]; sy = R.L;       // rows wrapped from the bottom
             }
-            for (int j = a + gl; j < b; j += kRdfG)
-                if (j != i && pre(j)) {
-                    const double2 qq = xy[j];
-                    fn(j, (float)((p.x - qq.x) - sxq[q]), (float)((p.y - qq.y) - sy));
+            for (int j = a + gl; j < e; j += kTlG)
+                if (j != i && pre(S.lgt[j])) {
+                    const double2 qq = S.lxy[j];
+                    fn((float)((p.x - qq.x) - sxq[q]), (float)((p.y - qq.y) - sy));
                 }
         }
     }
 }
 
-// Smooth RDF histogram of the current state.  Per block (kRdfChunk beads):
-// the (ta or tb)-type beads are listed in LDS and each is handled by a group
-// of kRdfG lanes; contributions go to per-wavefront LDS fixed-point
-// histograms (integer atomics: order independent), summed into the block's
-// partial.
+// list positions of block blockIdx.x (kBeadBlock slots of one replica) that
+// hold a listed bead, into s_list; returns the replica
+__device__ __forceinline__ int list_block(const RepInfo *ri, const int32_t *bblk, const TapeRef &S, int *s_list,
+                                          int *s_n) {
+    const int r = bblk[blockIdx.x];
+    const RepInfo &R = ri[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    if (loc < R.N && S.lidx[R.off + loc] >= 0) s_list[atomicAdd(s_n, 1)] = R.off + loc;
+    __syncthreads();
+    return r;
+}
+
+// Smooth RDF histogram of every tape state of a segment: block (bead block,
+// state).  Each listed bead is handled by a group of kTlG lanes;
+// contributions go to per-wavefront LDS fixed-point histograms (integer
+// atomics: order independent), summed into the block's partial
+// part[(bin * states + state) * blocks + block].
 __global__ __launch_bounds__(kBeadBlock) void k_rdf(const RepInfo *__restrict__ ri, const int32_t *__restrict__ bblk,
-                                                    const int32_t *__restrict__ cs, const double2 *__restrict__ xy,
-                                                    const uint32_t *__restrict__ gt, RdfArgs A,
+                                                    const TapeRef *__restrict__ tape, RdfArgs A,
                                                     unsigned long long *__restrict__ part) {
     __shared__ unsigned long long h[kBeadBlock / 64][256];
     __shared__ int s_list[kBeadBlock];
     __shared__ int s_n;
+    const TapeRef S = tape[blockIdx.y];
     for (int b = threadIdx.x; b < (kBeadBlock / 64) * 256; b += kBeadBlock) h[b >> 8][b & 255] = 0ull;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    // block b: beads [(b % kRdfSub) * kRdfChunk, +kRdfChunk) of bead block b / kRdfSub
-    const int bb = blockIdx.x / kRdfSub;
-    const int r = bblk[bb];
-    const RepInfo &R = ri[r];
-    const int loc = (bb - R.bblk0) * kBeadBlock + (blockIdx.x % kRdfSub) * kRdfChunk + threadIdx.x;
-    if (threadIdx.x < kRdfChunk && loc < R.N) {
-        const int i = R.off + loc;
-        const int ti = (int)(gt[i] >> kGidBits);
-        if (ti == A.ta || ti == A.tb) s_list[atomicAdd(&s_n, 1)] = i;
-    }
-    __syncthreads();
+    list_block(ri, bblk, S, s_list, &s_n);
+    const RepInfo &R = ri[bblk[blockIdx.x]];
     unsigned long long *hw = h[threadIdx.x >> 6];
-    const int gl = threadIdx.x & (kRdfG - 1);
+    const int gl = threadIdx.x & (kTlG - 1);
     const float pref = 35.0f / (32.0f * A.width), iw = 1.0f / A.width;
-    for (int e = threadIdx.x / kRdfG; e < s_n; e += kBeadBlock / kRdfG) {
+    for (int e = threadIdx.x / kTlG; e < s_n; e += kBeadBlock / kTlG) {
         const int i = s_list[e];
-        const uint32_t gti = gt[i], ti = gti >> kGidBits, gi = gti & kGidMask;
-        visit_wide(
-            R, cs, xy, i, xy[i], A.cut, A.Rc, gl,
-            [&](int j) {
-                const uint32_t gtj = gt[j], tj = gtj >> kGidBits;
-                if (A.ta == A.tb) return (int)tj == A.ta && gi < (gtj & kGidMask);
+        const uint32_t gti = S.lgt[i], ti = gti >> kGidBits, gi = gti & kGidMask;
+        visit_list(
+            R, S, i, S.lxy[i], A.cut, A.Rc, gl,
+            [&](uint32_t gtj) {
+                const uint32_t tj = gtj >> kGidBits;
+                if (A.ta == A.tb) return gi < (gtj & kGidMask);  // each pair once
                 return (int)ti == A.ta && (int)tj == A.tb;
             },
-            [&](int j, float dx, float dy) {
+            [&](float dx, float dy) {
                 const float r2 = dx * dx + dy * dy;
                 if (r2 >= A.cut * A.cut || r2 == 0.0f) return;
                 const float rr = sqrtf(r2);
@@ -160,21 +237,23 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdf(const RepInfo *__restrict__ 
     for (int b = threadIdx.x; b < A.nbins; b += blockDim.x) {
         unsigned long long v = 0;
         for (int w = 0; w < kBeadBlock / 64; w++) v += h[w][b];
-        part[(size_t)b * gridDim.x + blockIdx.x] = v;  // bin-major: the reduction reads coalesced
+        part[((size_t)b * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = v;  // the reduction reads coalesced
     }
 }
 
-// window accumulator += sum of the block partials of each replica: one block
-// per (bin, replica), a strided sum then an LDS tree (integer: exact)
-__global__ __launch_bounds__(256) void k_rdf_reduce(const RepInfo *__restrict__ ri, int nblk,
+// window accumulator += sum of the block partials of each replica over nst
+// states: one block per (bin, replica), a strided sum then an LDS tree
+// (integer: exact)
+__global__ __launch_bounds__(256) void k_rdf_reduce(const RepInfo *__restrict__ ri, int nblk, int nst,
                                                     const unsigned long long *__restrict__ part,
                                                     unsigned long long *__restrict__ acc, int nbins) {
     __shared__ unsigned long long red[256];
     const int b = blockIdx.x, r = blockIdx.y;
     const RepInfo &R = ri[r];
-    const int k0 = R.bblk0 * kRdfSub, nb = (R.N + kBeadBlock - 1) / kBeadBlock * kRdfSub;
+    const int k0 = R.bblk0, nb = (R.N + kBeadBlock - 1) / kBeadBlock;
     unsigned long long v = 0;
-    for (int k = threadIdx.x; k < nb; k += 256) v += part[(size_t)b * nblk + k0 + k];
+    for (int y = 0; y < nst; y++)
+        for (int k = threadIdx.x; k < nb; k += 256) v += part[((size_t)b * nst + y) * nblk + k0 + k];
     red[threadIdx.x] = v;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -184,48 +263,43 @@ __global__ __launch_bounds__(256) void k_rdf_reduce(const RepInfo *__restrict__ 
     if (threadIdx.x == 0) acc[(size_t)r * nbins + b] += red[0];
 }
 
-// X_i += d loss / d x_i of the RDF at the current state; gw[r][b] = dL/dg_b * C_b.
-// Beads of either type are listed per block and handled by kRdfG lanes each;
-// the group's partial sums are combined by a fixed shuffle tree.
+// rg_i = d loss / d x_i of the RDF at tape state blockIdx.y for the listed
+// beads (the others were zeroed with the list); gw[r][b] = dL/dg_b * C_b.  All
+// states of a recomputed segment go in one launch.  Each listed bead is
+// handled by kTlG lanes; the group's partial sums are combined by a fixed
+// shuffle tree.
 __global__ __launch_bounds__(kBeadBlock) void k_rdf_adj(const RepInfo *__restrict__ ri,
                                                         const int32_t *__restrict__ bblk,
-                                                        const int32_t *__restrict__ cs, const double2 *__restrict__ xy,
-                                                        const uint32_t *__restrict__ gt, RdfArgs A,
-                                                        const double *__restrict__ gw, double2 *__restrict__ X) {
+                                                        const TapeRef *__restrict__ tape, RdfArgs A,
+                                                        const double *__restrict__ gw) {
     __shared__ float w_s[256];
     __shared__ int s_list[kBeadBlock];
     __shared__ int s_n;
-    const int bb = blockIdx.x / kRdfSub;
-    const int r = bblk[bb];
+    const TapeRef S = tape[blockIdx.y];
+    const int r = bblk[blockIdx.x];
     for (int b = threadIdx.x; b < A.nbins; b += blockDim.x) w_s[b] = (float)gw[(size_t)r * A.nbins + b];
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
+    list_block(ri, bblk, S, s_list, &s_n);
     const RepInfo &R = ri[r];
-    const int loc = (bb - R.bblk0) * kBeadBlock + (blockIdx.x % kRdfSub) * kRdfChunk + threadIdx.x;
-    if (threadIdx.x < kRdfChunk && loc < R.N) {
-        const int i = R.off + loc;
-        const int ti = (int)(gt[i] >> kGidBits);
-        if (ti == A.ta || ti == A.tb) s_list[atomicAdd(&s_n, 1)] = i;
-    }
-    __syncthreads();
-    const int gl = threadIdx.x & (kRdfG - 1);
+    const int gl = threadIdx.x & (kTlG - 1);
     const float pref = 35.0f / (32.0f * A.width), iw = 1.0f / A.width;
     // rounds over the list; every lane of a group takes part in the shuffles
-    const int ng = kBeadBlock / kRdfG;
+    const int ng = kBeadBlock / kTlG;
     for (int e0 = 0; e0 < s_n; e0 += ng) {
-        const int e = e0 + (int)threadIdx.x / kRdfG;
+        const int e = e0 + (int)threadIdx.x / kTlG;
         double ax = 0.0, ay = 0.0;
         int i = -1;
         if (e < s_n) {
             i = s_list[e];
-            const uint32_t ti = gt[i] >> kGidBits;
-            visit_wide(
-                R, cs, xy, i, xy[i], A.cut, A.Rc, gl,
-                [&](int j) {
-                    const uint32_t tj = gt[j] >> kGidBits;
+            const uint32_t ti = S.lgt[i] >> kGidBits;
+            visit_list(
+                R, S, i, S.lxy[i], A.cut, A.Rc, gl,
+                [&](uint32_t gtj) {
+                    const uint32_t tj = gtj >> kGidBits;
                     return ((int)ti == A.ta && (int)tj == A.tb) || ((int)ti == A.tb && (int)tj == A.ta);
                 },
-                [&](int j, float dx, float dy) {
+                [&](float dx, float dy) {
                     const float r2 = dx * dx + dy * dy;
                     if (r2 >= A.cut * A.cut || r2 == 0.0f) return;
                     const float rr = sqrtf(r2);
@@ -242,14 +316,11 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdf_adj(const RepInfo *__restric
                     ay += (double)(ds * dy / rr);
                 });
         }
-        for (int o = kRdfG / 2; o > 0; o >>= 1) {
-            ax += __shfl_xor(ax, o, kRdfG);
-            ay += __shfl_xor(ay, o, kRdfG);
+        for (int o = kTlG / 2; o > 0; o >>= 1) {
+            ax += __shfl_xor(ax, o, kTlG);
+            ay += __shfl_xor(ay, o, kTlG);
         }
-        if (gl == 0 && i >= 0) {
-            X[i].x += ax;
-            X[i].y += ay;
-        }
+        if (gl == 0 && i >= 0) S.rg[S.lidx[i]] = make_double2(ax, ay);
     }
 }
 
@@ -282,23 +353,48 @@ __device__ __forceinline__ int theta_index(uint32_t a, uint32_t b) {
     return a == 0 ? (int)b : (a == 1 ? 2 + (int)b : 5);
 }
 
-// J^T products of the pair forces at state n for the force adjoint Fa:
-// X += Jx^T Fa (FP64 sums), T = Jv^T Fa, per-gid theta gradient += (half of
-// each pair: every pair is visited from both beads).
-__global__ __launch_bounds__(kBeadBlock) void k_pair_adj(const RepInfo *__restrict__ ri,
-                                                         const RepStep *__restrict__ rs,
-                                                         const int32_t *__restrict__ bblk,
-                                                         const int32_t *__restrict__ cs,
-                                                         const double2 *__restrict__ xy, const float2 *__restrict__ vt,
-                                                         const uint32_t *__restrict__ gt,
-                                                         const double2 *__restrict__ Fa, double2 *__restrict__ X,
-                                                         double2 *__restrict__ T, double *__restrict__ thb) {
+// Step parameters of steps base + first + y (y = blockIdx.y) into
+// rsa[y * nrep + r]: every reverse step of a segment in one launch.
+__global__ __launch_bounds__(64) void k_step_params_seg(const RepInfo *__restrict__ ri, RepStep *__restrict__ rsa,
+                                                        int64_t first, const int64_t *__restrict__ base) {
+    const int r = blockIdx.x, y = blockIdx.y;
+    step_params_wave(ri[r], rsa[(size_t)y * gridDim.x + r], r, threadIdx.x, first + y, -1, base);
+}
+
+// One reverse step at state n (the tape's arrays of that state): with the
+// force adjoint Fa,
+//   X_n = X + Jx^T Fa (pair and bond terms) + G_n, T_n = Jv^T Fa,
+//   theta_bar += Jtheta^T Fa (per gid, half of each pair: every pair is
+//   visited from both beads),
+// then X_{n-1}, V_{n-1}, F_{n-1} scattered to the order of step n-1 through
+// the sort map (new slot i <- old slot src[i]).  Every term of bead i is
+// summed by its own group (gather form, no atomics).  Xo == nullptr: theta
+// gradient only (the window's first force).
+struct AdjStep {
+    const RepStep *rs;  // step parameters of step n, per replica
+    const int32_t *cs, *src, *lnk, *prv;
+    const double2 *xy, *G;
+    const float2 *vt;
+    const uint32_t *gt;
+    const int8_t *mp;
+    const double2 *Fa, *X, *V;
+    double2 *Xo, *Vo, *Fo;
+    double *thb;
+};
+
+__global__ __launch_bounds__(kBeadBlock) void k_adj_rstep(const RepInfo *__restrict__ ri,
+                                                          const int32_t *__restrict__ bblk, AdjStep A) {
     // block b: beads [(b % kRdfSub) * kRdfChunk, +kRdfChunk) of bead block
     // b / kRdfSub, kAdjG lanes per bead
     const int bb = blockIdx.x / kRdfSub;
     const int r = bblk[bb];
     const RepInfo &R = ri[r];
-    const RepStep &S = rs[r];
+    const RepStep &S = A.rs[r];
+    const int32_t *__restrict__ cs = A.cs;
+    const double2 *__restrict__ xy = A.xy;
+    const float2 *__restrict__ vt = A.vt;
+    const uint32_t *__restrict__ gt = A.gt;
+    const double2 *__restrict__ Fa = A.Fa;
     const int loc = (bb - R.bblk0) * kBeadBlock + (blockIdx.x % kRdfSub) * kRdfChunk + (int)threadIdx.x / kAdjG;
     const int gl = threadIdx.x & (kAdjG - 1);
     const bool live = loc < R.N;
@@ -379,7 +475,7 @@ __global__ __launch_bounds__(kBeadBlock) void k_pair_adj(const RepInfo *__restri
             }
         }
     }
-    // sums over the group (fixed shuffle tree), written by its first lane
+    // sums over the group (fixed shuffle tree), finished by its first lane
     for (int o = kAdjG / 2; o > 0; o >>= 1) {
         xa += __shfl_xor(xa, o, kAdjG);
         ya += __shfl_xor(ya, o, kAdjG);
@@ -389,91 +485,72 @@ __global__ __launch_bounds__(kBeadBlock) void k_pair_adj(const RepInfo *__restri
         for (int q = 0; q < 7; q++) th[q] += __shfl_xor(th[q], o, kAdjG);
     }
     if (!live || gl != 0) return;
-    X[i].x += xa;
-    X[i].y += ya;
-    T[i] = make_double2(ta, tb);
-    double *o = thb + (size_t)(R.off + gi) * 8;
+    double *o = A.thb + (size_t)(R.off + gi) * 8;
     for (int q = 0; q < 7; q++) o[q] += th[q];
-}
-
-// bond terms of the force adjoint (FP64): lipid chains from their head, a
-// chemistry bond from its smaller-gid bead
-__global__ __launch_bounds__(kBeadBlock) void k_bond_adj(const RepInfo *__restrict__ ri,
-                                                         const int32_t *__restrict__ bblk,
-                                                         const double2 *__restrict__ xy,
-                                                         const uint32_t *__restrict__ gt,
-                                                         const int8_t *__restrict__ mp, const int32_t *__restrict__ lnk,
-                                                         const double2 *__restrict__ Fa, double2 *__restrict__ X) {
-    const int r = bblk[blockIdx.x];
-    const RepInfo &R = ri[r];
-    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
-    if (loc >= R.N) return;
-    const int s = R.off + loc;
-    auto adj = [&](int i, int j) {
-        const double2 p = xy[i], q = xy[j];
-        const double dx = minimg(q.x - p.x, R.L, R.halfL), dy = minimg(q.y - p.y, R.L, R.halfL);
+    if (!A.Xo) return;
+    // harmonic bonds of bead i (FP64): the chain neighbours of a lipid bead,
+    // the chemistry partner of another bead; with d = x_i - x_p and
+    // l = Fa_i - Fa_p, X_i += g l + g' r (e.l) e, g = -kb (r - r0) / r,
+    // g' = -kb r0 / r^2
+    auto bond = [&](int p) {
+        const double2 q = xy[p], Fp = Fa[p];
+        const double dx = minimg(pi.x - q.x, R.L, R.halfL), dy = minimg(pi.y - q.y, R.L, R.halfL);
         const double rr = sqrt(dx * dx + dy * dy);
         const double g = -R.kb * (rr - R.r0) / rr, gp = -R.kb * R.r0 / (rr * rr);
         const double ex = dx / rr, ey = dy / rr;
-        const double lx = Fa[j].x - Fa[i].x, ly = Fa[j].y - Fa[i].y;
+        const double lx = Fi.x - Fp.x, ly = Fi.y - Fp.y;
         const double el = ex * lx + ey * ly;
-        const double bx = g * lx + gp * rr * el * ex, by = g * ly + gp * rr * el * ey;
-        X[j].x += bx;
-        X[j].y += by;
-        X[i].x -= bx;
-        X[i].y -= by;
+        xa += g * lx + gp * rr * el * ex;
+        ya += g * ly + gp * rr * el * ey;
     };
-    const int m = mp[s];
-    if (m == 0) {
-        int prev = s, n = 1;
-        for (int d = lnk[s]; d >= 0 && n < R.mlen; d = lnk[d], n++) {
-            adj(prev, d);
-            prev = d;
-        }
-    } else if (m < 0) {
-        const int p = lnk[s];
-        if (p >= 0 && (gt[s] & kGidMask) < (gt[p] & kGidMask)) adj(s, p);
+    const int pn = A.lnk[i];
+    if (A.mp[i] >= 0) {
+        const int pp = A.prv[i];
+        if (pp >= 0) bond(pp);
     }
+    if (pn >= 0) bond(pn);
+    // + d loss / d x_n of the RDF, then the step back to n-1
+    const double2 xin = A.X[i], g = A.G[i], v = A.V[i];
+    const double xx = xin.x + xa + g.x, xy_ = xin.y + ya + g.y;
+    const int od = A.src[i];
+    A.Xo[od] = make_double2(xx, xy_);
+    A.Vo[od] = make_double2(R.dt * xx + ta + v.x, R.dt * xy_ + tb + v.y);
+    A.Fo[od] = make_double2(R.dt2 * xx + R.cvt * ta + R.dt * v.x, R.dt2 * xy_ + R.cvt * tb + R.dt * v.y);
 }
 
-// one reverse step: new adjoints in the order of step n, scattered to the
-// order of step n-1 through the sort map src (new slot d <- old slot src[d])
-__global__ __launch_bounds__(kBeadBlock) void k_adj_step(const RepInfo *__restrict__ ri,
-                                                         const int32_t *__restrict__ bblk,
-                                                         const int32_t *__restrict__ src,
-                                                         const double2 *__restrict__ X, const double2 *__restrict__ V,
-                                                         const double2 *__restrict__ F, const double2 *__restrict__ T,
-                                                         double2 *__restrict__ Xo, double2 *__restrict__ Vo,
-                                                         double2 *__restrict__ Fo) {
-    const int r = bblk[blockIdx.x];
-    const RepInfo &R = ri[r];
-    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
-    if (loc >= R.N) return;
-    const int d = R.off + loc;
-    const int o = src[d];
-    const double2 x = X[d], v = V[d], f = F[d], t = T[d];
-    Xo[o] = x;
-    Vo[o] = make_double2(R.dt * x.x + t.x + v.x, R.dt * x.y + t.y + v.y);
-    Fo[o] = make_double2(R.dt2 * x.x + R.cvt * t.x + R.dt * v.x, R.dt2 * x.y + R.cvt * t.y + R.dt * v.y);
-}
-
-// per-replica sum of the per-gid theta gradients (fixed partition, FP64)
-__global__ __launch_bounds__(256) void k_theta_reduce(const RepInfo *__restrict__ ri, const double *__restrict__ thb,
-                                                      double *__restrict__ out) {
-    __shared__ double red[256];
+// per-replica sum of the per-gid theta gradients in two fixed-order stages
+// (FP64): block (replica, part) sums a contiguous gid range into
+// tpart[(r * kThParts + part) * 8 + q]; k_theta_final adds the parts in order
+constexpr int kThParts = 64;
+__global__ __launch_bounds__(256) void k_theta_part(const RepInfo *__restrict__ ri, const double *__restrict__ thb,
+                                                    double *__restrict__ tpart) {
+    __shared__ double red[7][256];
     const RepInfo &R = ri[blockIdx.x];
-    for (int q = 0; q < 7; q++) {
-        double s = 0.0;
-        for (int g = threadIdx.x; g < R.N; g += 256) s += thb[(size_t)(R.off + g) * 8 + q];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[(size_t)blockIdx.x * 8 + q] = red[0];
+    const int per = (R.N + kThParts - 1) / kThParts;
+    const int g0 = blockIdx.y * per, g1 = min(R.N, g0 + per);
+    double sq[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int g = g0 + (int)threadIdx.x; g < g1; g += 256) {
+        const double *t = thb + (size_t)(R.off + g) * 8;
+#pragma unroll
+        for (int q = 0; q < 7; q++) sq[q] += t[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 7; q++) red[q][threadIdx.x] = sq[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int q = 0; q < 7; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
         __syncthreads();
     }
+    if (threadIdx.x < 7) tpart[((size_t)blockIdx.x * kThParts + blockIdx.y) * 8 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(64) void k_theta_final(const double *__restrict__ tpart, double *__restrict__ out) {
+    if (threadIdx.x >= 7) return;
+    double v = 0.0;
+    for (int k = 0; k < kThParts; k++) v += tpart[((size_t)blockIdx.x * kThParts + k) * 8 + threadIdx.x];
+    out[(size_t)blockIdx.x * 8 + threadIdx.x] = v;
 }
 
 __global__ void k_zero_f64(double *p, int64_t n) {

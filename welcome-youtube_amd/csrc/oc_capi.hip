@@ -32,6 +32,7 @@ struct oc_engine {
     std::vector<RepInfo> ri;
     int64_t Ntot = 0, Ctot = 0;
     std::vector<int32_t> bblk, rblk;
+    std::vector<int2> cols;  // (replica, column) of every cell column
     std::vector<Tile> tiles;   // pair-force tiles
     std::vector<Tile> stiles;  // sort tiles (rows of new cells, one thread each)
     int pair_block = 256, tile_beads = 0;
@@ -45,12 +46,12 @@ struct oc_engine {
     StateP st[2];
     int cur = 0;
     double2 *d_f = nullptr;
-    int32_t *d_key = nullptr, *d_src = nullptr, *d_cs[2] = {nullptr, nullptr};
+    int32_t *d_key = nullptr, *d_cs[2] = {nullptr, nullptr};
     // sort: per tile bead count, first new slot, per-tile bead list; per bead
     // its (tile, rank); per cell its rank offset within the tile
     int32_t *d_tcount = nullptr, *d_toff = nullptr, *d_tsrc = nullptr, *d_tslot = nullptr, *d_locst = nullptr;
     uint32_t *d_moved = nullptr;  // a bead moved by more than one cell this step
-    bool record_src = false;      // the sort writes src (new slot -> old slot) for the adjoint's tape
+    int32_t *rec_src = nullptr;   // when set, the sort writes src (new slot -> old slot) there
     int2 *d_bmax = nullptr;
     double *d_stir = nullptr, *d_epot = nullptr, *d_ebond = nullptr, *d_part = nullptr, *d_statout = nullptr;
     uint8_t *d_flag = nullptr;
@@ -277,7 +278,7 @@ static void launch_sort(oc_engine *h, bool integ, int step_delta) {
                                              h->d_tcount, h->d_locst, h->d_tsrc, h->d_tslot, h->d_err, nt);
     k_tscan<<<1, 1024, 0, h->stream>>>(h->d_ri, h->d_rs, h->nrep, h->d_tcount, nt, h->d_toff, h->d_moved,
                                        step_delta);
-    int32_t *src = h->record_src ? h->d_src : nullptr;
+    int32_t *src = h->rec_src;
     if (integ)
         k_place<true><<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,
                                                        h->d_tsrc, h->d_tslot, h->d_cs[n], src, h->st[c],
@@ -448,7 +449,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     h->nrep = nrep;
     h->prm.assign(p, p + nrep);
     h->ri.resize(nrep);
-    int64_t off = 0, coff = 0;
+    int64_t off = 0, coff = 0, col0 = 0;
     for (int r = 0; r < nrep; r++) {
         if (int rc = make_repinfo(p[r], N[r], h->ri[r])) {
             delete h;
@@ -457,9 +458,12 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
         RepInfo &R = h->ri[r];
         R.off = (int32_t)off;
         R.coff = (int32_t)coff;
+        R.col0 = (int32_t)col0;
+        for (int cx = 0; cx < R.ncx; cx++) h->cols.push_back(make_int2(r, cx));
         off += R.N;
         coff += R.ncell;
-        if (off >= INT32_MAX || coff >= INT32_MAX) {
+        col0 += R.ncx;
+        if (off >= INT32_MAX || coff + col0 >= INT32_MAX) {
             delete h;
             return fail(OC_EINVAL, "too many beads or cells for 32-bit slot indices");
         }
@@ -513,7 +517,6 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     }
     rc = rc ? rc : dalloc(h, &h->d_f, Nt);
     rc = rc ? rc : dalloc(h, &h->d_key, Nt);
-    rc = rc ? rc : dalloc(h, &h->d_src, Nt);
     rc = rc ? rc : dalloc(h, &h->d_tcount, h->stiles.size());
     rc = rc ? rc : dalloc(h, &h->d_toff, h->stiles.size());
     rc = rc ? rc : dalloc(h, &h->d_tsrc, h->stiles.size() << kSortCapLog);
@@ -950,14 +953,23 @@ struct StateSnap {
     int8_t *mp = nullptr;
     int32_t *lnk = nullptr, *cs = nullptr, *src = nullptr;
     RepStep *rs = nullptr;
+    // tape states: previous chain beads, the RDF gradient and the RDF's type
+    // list (TapeRef)
+    int32_t *prv = nullptr;
+    double2 *rg = nullptr, *lxy = nullptr;
+    uint32_t *lgt = nullptr;
+    int32_t *lidx = nullptr, *lcs = nullptr;
 };
 
 struct Adj {
     int K = 0, nseg = 0, nbins = 0;
     std::vector<StateSnap> tape, ckpt;
+    TapeRef *d_tape = nullptr;  // device copy of the tape's TapeRef
+    int2 *d_cols = nullptr;     // device copy of h->cols
     StateSnap end;
-    double2 *X[2] = {nullptr, nullptr}, *V[2] = {nullptr, nullptr}, *F[2] = {nullptr, nullptr}, *T = nullptr;
-    double *thb = nullptr, *gw = nullptr, *grad = nullptr;
+    double2 *X[2] = {nullptr, nullptr}, *V[2] = {nullptr, nullptr}, *F[2] = {nullptr, nullptr};
+    RepStep *rsa = nullptr;  // step parameters of a segment's reverse steps
+    double *thb = nullptr, *gw = nullptr, *grad = nullptr, *tpart = nullptr;
     unsigned long long *rpart = nullptr, *racc = nullptr;
     int64_t *base = nullptr;
     std::vector<void *> mem;
@@ -985,7 +997,9 @@ static int aalloc(Adj *a, T **p, size_t n) {
     return 0;
 }
 
-static int snap_alloc(oc_engine *h, Adj *a, StateSnap &s, bool full, bool with_src) {
+// full: also v, f and the step counters (checkpoints); tape: also the sort
+// map, prv, the RDF gradient and the type list
+static int snap_alloc(oc_engine *h, Adj *a, StateSnap &s, bool full, bool tape) {
     const size_t N = h->Ntot, C = h->Ctot + 1;
     int rc = 0;
     rc = rc ? rc : aalloc(a, &s.xy, N);
@@ -994,7 +1008,15 @@ static int snap_alloc(oc_engine *h, Adj *a, StateSnap &s, bool full, bool with_s
     rc = rc ? rc : aalloc(a, &s.mp, N);
     rc = rc ? rc : aalloc(a, &s.lnk, N);
     rc = rc ? rc : aalloc(a, &s.cs, C);
-    if (with_src) rc = rc ? rc : aalloc(a, &s.src, N);
+    if (tape) {
+        rc = rc ? rc : aalloc(a, &s.src, N);
+        rc = rc ? rc : aalloc(a, &s.prv, N);
+        rc = rc ? rc : aalloc(a, &s.rg, N);
+        rc = rc ? rc : aalloc(a, &s.lxy, N);
+        rc = rc ? rc : aalloc(a, &s.lgt, N);
+        rc = rc ? rc : aalloc(a, &s.lidx, N);
+        rc = rc ? rc : aalloc(a, &s.lcs, h->Ctot + h->cols.size());
+    }
     if (full) {
         rc = rc ? rc : aalloc(a, &s.v, N);
         rc = rc ? rc : aalloc(a, &s.f, N);
@@ -1004,7 +1026,7 @@ static int snap_alloc(oc_engine *h, Adj *a, StateSnap &s, bool full, bool with_s
 }
 
 // copy the engine's current state into (save) or out of (restore) a snapshot
-static int snap_copy(oc_engine *h, StateSnap &s, bool save, bool with_src) {
+static int snap_copy(oc_engine *h, StateSnap &s, bool save) {
     // all arrays in one launch (k_copy_multi)
     const size_t N = h->Ntot, C = h->Ctot + 1;
     const int c = h->cur;
@@ -1022,7 +1044,6 @@ static int snap_copy(oc_engine *h, StateSnap &s, bool save, bool with_src) {
     add(h->st[c].mp, s.mp, N);
     add(h->st[c].lnk, s.lnk, N * sizeof(int32_t));
     add(h->d_cs[c], s.cs, C * sizeof(int32_t));
-    if (with_src && save) add(h->d_src, s.src, N * sizeof(int32_t));
     if (s.v) {
         add(h->st[c].v, s.v, N * sizeof(double2));
         add(h->d_f, s.f, N * sizeof(double2));
@@ -1031,6 +1052,47 @@ static int snap_copy(oc_engine *h, StateSnap &s, bool save, bool with_src) {
     k_copy_multi<<<512, 256, 0, h->stream>>>(J);
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+// The window's steps write their states straight into tape buffers: before a
+// step, the next state's positions, gt words and cell starts (with full also
+// vt, mp, lnk, prv and the sort map) are pointed at tape state t.  The engine's
+// own buffers are put back with restore_bufs.
+struct EngBufs {
+    StateP st[2];
+    int32_t *cs[2];
+};
+
+static EngBufs save_bufs(const oc_engine *h) {
+    EngBufs b;
+    for (int k = 0; k < 2; k++) {
+        b.st[k] = h->st[k];
+        b.cs[k] = h->d_cs[k];
+    }
+    return b;
+}
+
+static void restore_bufs(oc_engine *h, const EngBufs &b) {
+    for (int k = 0; k < 2; k++) {
+        h->st[k] = b.st[k];
+        h->d_cs[k] = b.cs[k];
+    }
+    h->rec_src = nullptr;
+}
+
+static void step_into(oc_engine *h, const StateSnap &t, bool full) {
+    const int n = h->cur ^ 1;
+    h->st[n].xy = t.xy;
+    h->st[n].gt = t.gt;
+    h->d_cs[n] = t.cs;
+    if (full) {
+        h->st[n].vt = t.vt;
+        h->st[n].mp = t.mp;
+        h->st[n].lnk = t.lnk;
+        h->st[n].prv = t.prv;
+        h->rec_src = t.src;
+    }
+    enqueue_step(h);
 }
 
 static int adj_setup(oc_engine *h, int K, int nseg, int nbins) {
@@ -1045,6 +1107,18 @@ static int adj_setup(oc_engine *h, int K, int nseg, int nbins) {
     int rc = 0;
     a->tape.resize(K + 1);
     for (auto &t : a->tape) rc = rc ? rc : snap_alloc(h, a, t, false, true);
+    rc = rc ? rc : aalloc(a, &a->d_tape, K + 1);
+    rc = rc ? rc : aalloc(a, &a->d_cols, h->cols.size());
+    if (!rc) {
+        std::vector<TapeRef> tr(K + 1);
+        for (int m = 0; m <= K; m++) {
+            const StateSnap &t = a->tape[m];
+            tr[m] = TapeRef{t.cs, t.xy, t.gt, t.rg, t.lxy, t.lgt, t.lidx, t.lcs};
+        }
+        if (hipMemcpy(a->d_tape, tr.data(), tr.size() * sizeof(TapeRef), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(a->d_cols, h->cols.data(), h->cols.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(OC_EHIP, "tape table copy failed");
+    }
     a->ckpt.resize(nseg);
     for (auto &t : a->ckpt) rc = rc ? rc : snap_alloc(h, a, t, true, false);
     rc = rc ? rc : snap_alloc(h, a, a->end, true, false);
@@ -1053,11 +1127,12 @@ static int adj_setup(oc_engine *h, int K, int nseg, int nbins) {
         rc = rc ? rc : aalloc(a, &a->V[k], N);
         rc = rc ? rc : aalloc(a, &a->F[k], N);
     }
-    rc = rc ? rc : aalloc(a, &a->T, N);
+    rc = rc ? rc : aalloc(a, &a->rsa, (size_t)K * h->nrep);
     rc = rc ? rc : aalloc(a, &a->thb, N * 8);
     rc = rc ? rc : aalloc(a, &a->gw, (size_t)h->nrep * nbins);
     rc = rc ? rc : aalloc(a, &a->grad, (size_t)h->nrep * 8);
-    rc = rc ? rc : aalloc(a, &a->rpart, h->bblk.size() * (size_t)kRdfSub * nbins);
+    rc = rc ? rc : aalloc(a, &a->tpart, (size_t)h->nrep * kThParts * 8);
+    rc = rc ? rc : aalloc(a, &a->rpart, h->bblk.size() * (size_t)K * nbins);
     rc = rc ? rc : aalloc(a, &a->racc, (size_t)h->nrep * nbins);
     rc = rc ? rc : aalloc(a, &a->base, h->nrep);
     if (rc) adj_free(h);
@@ -1134,21 +1209,36 @@ OC_API int oc_window_forward(oc_engine *h, int32_t nsteps, int32_t ckpt_every, c
     RA.dr = (float)dr;
     RA.cut = cut;
     RA.Rc = Rc;
-    const int nbk = (int)h->bblk.size();
-    // f0 = F(x0, v~0; theta), then nsteps steps, checkpoints at the segment starts
+    const int nbk = (int)h->bblk.size(), ncol = (int)h->cols.size();
+    // f0 = F(x0, v~0; theta), then nsteps steps, checkpoints at the segment
+    // starts; each segment's states are stepped into the tape and their RDF
+    // histograms taken in one launch
     drop_graphs(h);
     launch_step_params(h, 0, -1);
     launch_forces(h, true);
     HIPCHK(hipMemsetAsync(a->racc, 0, (size_t)h->nrep * nb * sizeof(unsigned long long), h->stream));
-    for (int n = 0; n < nsteps; n++) {
-        if (n % K == 0)
-            if (int rc = snap_copy(h, a->ckpt[n / K], true, false)) return rc;
-        enqueue_step(h);
-        const int c = h->cur;
-        k_rdf<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->d_cs[c], h->st[c].xy, h->st[c].gt, RA, a->rpart);
-        k_rdf_reduce<<<dim3(nb, h->nrep), 256, 0, h->stream>>>(h->d_ri, nbk * kRdfSub, a->rpart, a->racc, nb);
+    const EngBufs eb = save_bufs(h);
+    for (int seg = 0; seg < nseg; seg++) {
+        const int len = std::min(K, nsteps - seg * K);
+        if (int rc = snap_copy(h, a->ckpt[seg], true)) return rc;
+        for (int m = 1; m <= len; m++) step_into(h, a->tape[m], false);
+        restore_bufs(h, eb);
+        // the engine's buffers take the segment's last state back
+        {
+            const StateSnap &t = a->tape[len];
+            const int c = h->cur;
+            CopyJobs J;
+            J.n = 3;
+            J.dst[0] = (char *)h->st[c].xy, J.src[0] = (const char *)t.xy, J.bytes[0] = (int64_t)(h->Ntot * sizeof(double2));
+            J.dst[1] = (char *)h->st[c].gt, J.src[1] = (const char *)t.gt, J.bytes[1] = (int64_t)(h->Ntot * sizeof(uint32_t));
+            J.dst[2] = (char *)h->d_cs[c], J.src[2] = (const char *)t.cs, J.bytes[2] = (int64_t)((h->Ctot + 1) * sizeof(int32_t));
+            k_copy_multi<<<512, 256, 0, h->stream>>>(J);
+        }
+        k_tlist<<<dim3(ncol, len), 256, 0, h->stream>>>(h->d_ri, a->d_cols, a->d_tape + 1, RA, 0);
+        k_rdf<<<dim3(nbk, len), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, a->d_tape + 1, RA, a->rpart);
+        k_rdf_reduce<<<dim3(nb, h->nrep), 256, 0, h->stream>>>(h->d_ri, nbk, len, a->rpart, a->racc, nb);
     }
-    if (int rc = snap_copy(h, a->end, true, false)) return rc;
+    if (int rc = snap_copy(h, a->end, true)) return rc;
     HIPCHK(hipGetLastError());
     if (int rc = check_err(h)) return rc;
     std::vector<unsigned long long> acc((size_t)h->nrep * nb);
@@ -1170,59 +1260,51 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipMemcpy(a->gw, gw.data(), gw.size() * sizeof(double), hipMemcpyHostToDevice));
     const RdfArgs RA = a->RA;
-    const int nbk = (int)h->bblk.size();
+    const int nbk = (int)h->bblk.size(), ncol = (int)h->cols.size();
     const size_t N = h->Ntot;
     zero_f64(h, a->thb, N * 8);
     int ab = 0;  // adjoint buffer parity
+    const EngBufs eb = save_bufs(h);
     for (int seg = nseg - 1; seg >= 0; seg--) {
         const int n0 = seg * K, n1 = std::min(n0 + K, nsteps), len = n1 - n0;
-        // recompute the segment from its checkpoint (bit-identical), recording each state
-        if (int rc = snap_copy(h, a->ckpt[seg], false, false)) return rc;
+        // recompute the segment from its checkpoint (bit-identical) into the tape
+        if (int rc = snap_copy(h, a->ckpt[seg], false)) return rc;
         launch_build_prv(h);
-        if (int rc = snap_copy(h, a->tape[0], true, false)) return rc;
-        h->record_src = true;
-        for (int m = 1; m <= len; m++) {
-            enqueue_step(h);
-            if (int rc = snap_copy(h, a->tape[m], true, true)) {
-                h->record_src = false;
-                return rc;
-            }
-        }
-        h->record_src = false;
+        if (seg == 0)
+            if (int rc = snap_copy(h, a->tape[0], true)) return rc;
+        for (int m = 1; m <= len; m++) step_into(h, a->tape[m], true);
+        restore_bufs(h, eb);
+        // d loss / d x of the RDF at the segment's states n0+1..n1, one launch
+        k_tlist<<<dim3(ncol, len), 256, 0, h->stream>>>(h->d_ri, a->d_cols, a->d_tape + 1, RA, 1);
+        k_rdf_adj<<<dim3(nbk, len), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, a->d_tape + 1, RA, a->gw);
         if (seg == nseg - 1) {
             zero_f64(h, (double *)a->X[ab], 2 * N);
             zero_f64(h, (double *)a->V[ab], 2 * N);
             zero_f64(h, (double *)a->F[ab], 2 * N);
-            StateSnap &t = a->tape[len];
-            k_rdf_adj<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, t.cs, t.xy, t.gt, RA, a->gw, a->X[ab]);
         }
+        // one launch per reverse step (pair and bond terms, the RDF gradient, the
+        // step back through the sort map)
+        k_step_params_seg<<<dim3(h->nrep, len), 64, 0, h->stream>>>(h->d_ri, a->rsa, n0 + 1, a->base);
         for (int m = len; m >= 1; m--) {
-            StateSnap &t = a->tape[m];
-            k_step_params<<<h->nrep, 64, 0, h->stream>>>(h->d_ri, h->d_rs, n0 + m, -1, a->base);
-            k_pair_adj<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, t.cs, t.xy, t.vt, t.gt,
-                                                         a->F[ab], a->X[ab], a->T, a->thb);
-            k_bond_adj<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, t.xy, t.gt, t.mp, t.lnk, a->F[ab],
-                                                         a->X[ab]);
-            k_adj_step<<<nbk, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, t.src, a->X[ab], a->V[ab], a->F[ab],
-                                                         a->T, a->X[ab ^ 1], a->V[ab ^ 1], a->F[ab ^ 1]);
+            const StateSnap &t = a->tape[m];
+            const AdjStep A = {a->rsa + (size_t)(m - 1) * h->nrep, t.cs, t.src, t.lnk, t.prv, t.xy, t.rg, t.vt, t.gt,
+                               t.mp, a->F[ab], a->X[ab], a->V[ab], a->X[ab ^ 1], a->V[ab ^ 1], a->F[ab ^ 1], a->thb};
+            k_adj_rstep<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, A);
             ab ^= 1;
-            if (n0 + m - 1 >= 1) {
-                StateSnap &tp = a->tape[m - 1];
-                k_rdf_adj<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, tp.cs, tp.xy, tp.gt, RA, a->gw,
-                                                            a->X[ab]);
-            }
         }
     }
     // the window's first force, f0 = F(x0, v~0; theta), carries parameter gradient too
     {
-        StateSnap &t = a->tape[0];
-        k_step_params<<<h->nrep, 64, 0, h->stream>>>(h->d_ri, h->d_rs, 0, -1, a->base);
-        k_pair_adj<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, t.cs, t.xy, t.vt, t.gt, a->F[ab],
-                                                     a->X[ab], a->T, a->thb);
+        const StateSnap &t = a->tape[0];
+        k_step_params_seg<<<dim3(h->nrep, 1), 64, 0, h->stream>>>(h->d_ri, a->rsa, 0, a->base);
+        const AdjStep A = {a->rsa, t.cs, nullptr, t.lnk, nullptr, t.xy, nullptr, t.vt, t.gt, t.mp, a->F[ab], a->X[ab],
+                           a->V[ab], nullptr, nullptr, nullptr, a->thb};
+        k_adj_rstep<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, A);
     }
-    k_theta_reduce<<<h->nrep, 256, 0, h->stream>>>(h->d_ri, a->thb, a->grad);
+    k_theta_part<<<dim3(h->nrep, kThParts), 256, 0, h->stream>>>(h->d_ri, a->thb, a->tpart);
+    k_theta_final<<<h->nrep, 64, 0, h->stream>>>(a->tpart, a->grad);
     // the engine continues from the window end
-    if (int rc = snap_copy(h, a->end, false, false)) return rc;
+    if (int rc = snap_copy(h, a->end, false)) return rc;
     launch_build_prv(h);
     HIPCHK(hipGetLastError());
     if (int rc = check_err(h)) return rc;

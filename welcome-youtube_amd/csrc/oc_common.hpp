@@ -25,7 +25,7 @@ struct RepInfo {
     int32_t ncx, ncy, ncell, coff;  // cells (column-major), first global cell
     int32_t mlen, nH;
     int32_t chem, n_chem;
-    int32_t has_stir, bblk0, rblk0, pad0;  // stirring on; first block of the bead / reduction tables
+    int32_t has_stir, bblk0, rblk0, col0;  // stirring on; first bead / reduction block; columns of earlier replicas
     double L, halfL, icsx, icsy, csx, csy;
     double dt, dt2, cvt, hdt;       // dt, dt^2, (1/2 + lambda) dt, dt/2
     double gamma, kT, kb, r0, ka;
