@@ -29,9 +29,9 @@ def soup_state(oracle, p, rng, vt_noise=0.3):
     return s
 
 
-def device_forces(eng, p, s, step, bonded):
+def device_forces(eng, p, s, step, bonded, lists=False):
     """forces of state s on the device: vh := predicted velocity, f := 0"""
-    e = eng.Engine(p)
+    e = eng.Engine(p, pair_lists=lists)
     st = {k: s[k] for k in ("x", "y", "type", "mpos", "nxt")}
     st["vx"], st["vy"] = s["vtx"], s["vty"]
     st["bond"] = s.get("bond", np.full(p.N, -1, np.int32))
@@ -51,15 +51,18 @@ def check_forces(fx, fy, ofx, ofy, sabs, what):
     assert rel < FORCE_TOL and glob < FORCE_TOL
 
 
-@pytest.mark.parametrize("L,nsub", [(9.0, 1), (12.5, 4), (23.7, 2), (31.3, 8)])
-def test_pair_forces_vs_oracle(oracle, eng, rng, L, nsub):
-    say(f"case: pair forces L={L} nsub={nsub}")
+@pytest.mark.parametrize("lists", [False, True])
+@pytest.mark.parametrize("L,nsub", [(9.0, 1), (12.5, 4), (23.7, 2), (31.3, 8), (47.1, 4)])
+def test_pair_forces_vs_oracle(oracle, eng, rng, L, nsub, lists):
+    """both pair kernel variants (single pass; cutoff test then the marked
+    pairs); L = 47.1 has interior tiles, which take the flat path"""
+    say(f"case: pair forces L={L} nsub={nsub} lists={lists}")
     p = mkparams(L, nsub=nsub, f_lip=0.15, c_N=0.1, seed=99)
     s = soup_state(oracle, p, rng)
     # the device keeps the predicted velocity in FP32: give the oracle the same values
     s["vtx"] = s["vtx"].astype(np.float32).astype(np.float64)
     s["vty"] = s["vty"].astype(np.float32).astype(np.float64)
-    out = device_forces(eng, p, s, 12345, bonded=False)
+    out = device_forces(eng, p, s, 12345, bonded=False, lists=lists)
     ref = oracle.pair_forces_gid(p.to_c(), s, 12345, theta32=1, sabs=True)
     check_forces(out["fx"], out["fy"], ref["fx"], ref["fy"], ref["sabs"], f"pair L={L} nsub={nsub}")
     # full shell, pairwise antisymmetric: total force vanishes to rounding

@@ -208,7 +208,7 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false) {
     A.err = h->d_err;
     const dim3 grid(8 * h->per_xcd);
     // CAP (staged candidates) is 3.5x the block; `lists` picks the two-phase
-    // variant with 24 listed pairs per lane, or the single pass
+    // variant (cutoff test, then the marked pairs), or the single pass
     const bool L2 = h->pair_lists;
 #define OC_PAIR(B, C, CL)                                                                        \
     do {                                                                                         \
@@ -218,13 +218,13 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false) {
     } while (0)
     switch (h->pair_block) {
     case 64:
-        if (L2) OC_PAIR(64, 224, 24); else OC_PAIR(64, 224, 0);
+        if (L2) OC_PAIR(64, 224, 1); else OC_PAIR(64, 224, 0);
         break;
     case 128:
-        if (L2) OC_PAIR(128, 448, 24); else OC_PAIR(128, 448, 0);
+        if (L2) OC_PAIR(128, 448, 1); else OC_PAIR(128, 448, 0);
         break;
     default:
-        if (L2) OC_PAIR(256, 896, 24); else OC_PAIR(256, 896, 0);
+        if (L2) OC_PAIR(256, 896, 1); else OC_PAIR(256, 896, 0);
         break;
     }
 #undef OC_PAIR

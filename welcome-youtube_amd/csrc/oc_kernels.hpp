@@ -866,7 +866,6 @@ __host__ __device__ constexpr int pair_max_rows(int block) { return 2 * block; }
 template <int BLOCK, int CAP, int CAPL, bool ENERGY, bool ANG>
 __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     constexpr int kMaxRows = pair_max_rows(BLOCK);
-    __shared__ uint32_t s_lst[CAPL > 0 ? (CAPL + 3) / 4 : 1][BLOCK];  // per-lane lists, four entries a word
     __shared__ double2 s_xy[CAP];
     __shared__ float2 s_vt[CAP];
     __shared__ uint32_t s_gt[CAP];
@@ -890,57 +889,123 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     const int nrow = T.k1 - T.k0 + 2 * M + 1;
     const int32_t *cs = A.cs;
 
-    if (threadIdx.x < 64) s_a[threadIdx.x] = S.a_eff[threadIdx.x];
-    if (threadIdx.x == 0) {
-        int base = 0;
-        for (int c = 0; c < 3; c++) {
+    // The prologue costs two rounds of memory latency: the cell starts of the
+    // nine staged pieces, then every row offset and staged bead.
+    // Pieces: lane q < 9 of wavefront 0 takes column q / 3 (T.cx - 1 + q / 3)
+    // and piece q % 3 of its staged rows lo..hi: A = rows lo..-1 (wrapped to
+    // the top of the column), B = rows max(lo,0)..min(hi,ncy-1), C = rows
+    // ncy..hi (wrapped to the bottom); LDS holds them in q order.
+    if (threadIdx.x < 64) {
+        const float av = S.a_eff[threadIdx.x];
+        const int q = threadIdx.x;
+        const int lo = kb0, hi = T.k1 - 1 + M;  // staged rows, inclusive
+        int f = 0, n = 0;
+        if (q < 9) {
+            const int c = q / 3, pc = q - 3 * c;
             int col = T.cx - 1 + c;
             col = col < 0 ? col + ncx : (col >= ncx ? col - ncx : col);
-            s_col[c] = col;
             const int cb = R.coff + col * ncy;
-            s_colbase[c] = base;
-            const int lo = kb0, hi = T.k1 - 1 + M;  // staged rows, inclusive
-            // piece A: rows lo..-1 (wrapped to the top of the column)
-            int fA = 0, nA = 0;
-            if (lo < 0) { fA = cs[cb + lo + ncy]; nA = cs[cb + ncy] - fA; }
-            // piece B: rows max(lo,0)..min(hi,ncy-1)
-            const int b0 = lo < 0 ? 0 : lo, b1 = hi >= ncy ? ncy - 1 : hi;
-            const int fB = cs[cb + b0], nB = cs[cb + b1 + 1] - fB;
-            // piece C: rows ncy..hi (wrapped to the bottom)
-            int fC = 0, nC = 0;
-            if (hi >= ncy) { fC = cs[cb]; nC = cs[cb + hi - ncy + 1] - fC; }
-            s_piece[c][0][0] = fA; s_piece[c][0][1] = nA; s_piece[c][0][2] = base;
-            s_piece[c][1][0] = fB; s_piece[c][1][1] = nB; s_piece[c][1][2] = base + nA;
-            s_piece[c][2][0] = fC; s_piece[c][2][1] = nC; s_piece[c][2][2] = base + nA + nB;
-            base += nA + nB + nC;
+            int ra = 0, rb = -1;  // rows of the piece within the column
+            if (pc == 0) {
+                if (lo < 0) { ra = lo + ncy; rb = ncy - 1; }
+            } else if (pc == 1) {
+                ra = max(lo, 0); rb = min(hi, ncy - 1);
+            } else if (hi >= ncy) {
+                ra = 0; rb = hi - ncy;
+            }
+            if (rb >= ra) {
+                f = cs[cb + ra];
+                n = cs[cb + rb + 1] - f;
+            }
+            if (pc == 0) s_col[c] = col;
         }
-        s_colbase[3] = base;
+        int incl = n;  // inclusive prefix of the lengths over q
+        for (int o = 1; o < 16; o <<= 1) {
+            const int v = __shfl_up(incl, o, 64);
+            if (q >= o) incl += v;
+        }
+        if (q < 9) {
+            const int c = q / 3;
+            s_piece[c][q - 3 * c][0] = f;
+            s_piece[c][q - 3 * c][1] = n;
+            s_piece[c][q - 3 * c][2] = incl - n;
+            if (q == 3 * c) s_colbase[c] = incl - n;
+            if (q == 8) s_colbase[3] = incl;
+        }
+        s_a[q] = av;
     }
     __syncthreads();
     const int total = s_colbase[3];
     const bool staged = total <= CAP && nrow <= kMaxRows;
 
-    // row offsets: s_row[c][k - kb0] = LDS (staged) or slot index of the first
-    // bead of unwrapped row k of column c
-    for (int e = threadIdx.x; e < 3 * nrow && nrow <= kMaxRows; e += BLOCK) {
-        const int c = e / nrow, k = kb0 + (e - c * nrow);
-        const int cb = R.coff + s_col[c] * ncy;
-        int off;
-        if (k < 0) off = s_piece[c][0][2] + (cs[cb + ncy + k] - s_piece[c][0][0]);
-        else if (k < ncy) off = s_piece[c][1][2] + (cs[cb + k] - s_piece[c][1][0]);
-        else off = s_piece[c][2][2] + (cs[cb + k - ncy] - s_piece[c][2][0]);
-        s_row[c][e - c * nrow] = off;
-    }
-    if (staged) {
-        for (int q = 0; q < 9; q++) {
-            const int cc = q / 3, pp = q - cc * 3;
-            const int first = s_piece[cc][pp][0], len = s_piece[cc][pp][1], dst = s_piece[cc][pp][2];
-            for (int e = threadIdx.x; e < len; e += BLOCK) {
-                s_xy[dst + e] = A.xy[first + e];
-                s_vt[dst + e] = A.vt[first + e];
-                s_gt[dst + e] = A.gt[first + e];
+    // row offsets: s_row[c][k - kb0] = LDS (staged) or piece-relative index of
+    // the first bead of unwrapped row k of column c; staging of the pieces.
+    // Every load of a thread is issued before its first LDS store.
+    {
+        constexpr int kRowPer = (3 * (kMaxRows + 1) + BLOCK - 1) / BLOCK;
+        constexpr int kStgPer = (CAP + BLOCK - 1) / BLOCK;
+        const int nrow3 = nrow <= kMaxRows ? 3 * nrow : 0;
+        int pb[9], pf[9];
+#pragma unroll
+        for (int w = 0; w < 9; w++) {
+            pb[w] = s_piece[w / 3][w % 3][2];
+            pf[w] = s_piece[w / 3][w % 3][0];
+        }
+        int rcs[kRowPer];
+#pragma unroll
+        for (int u = 0; u < kRowPer; u++) {
+            const int e = (int)threadIdx.x + u * BLOCK;
+            int idx = R.coff;  // a valid index for threads without an entry
+            if (e < nrow3) {
+                const int c = e >= 2 * nrow ? 2 : (e >= nrow ? 1 : 0);
+                const int k = kb0 + (e - c * nrow);
+                idx = R.coff + s_col[c] * ncy + (k < 0 ? k + ncy : (k < ncy ? k : k - ncy));
+            }
+            rcs[u] = cs[idx];
+        }
+        double2 sx[kStgPer];
+        float2 sv[kStgPer];
+        uint32_t sg[kStgPer];
+        int sd[kStgPer];
+#pragma unroll
+        for (int u = 0; u < kStgPer; u++) {
+            const int e = (int)threadIdx.x + u * BLOCK;
+            int src = R.off;  // a valid slot for threads without a bead
+            sd[u] = -1;
+            if (staged && e < total) {
+                int q = 0;  // the last piece starting at or before e (empty pieces skipped)
+#pragma unroll
+                for (int w = 1; w < 9; w++) q = e >= pb[w] ? w : q;
+                int fq = pf[0], bq = pb[0];
+#pragma unroll
+                for (int w = 1; w < 9; w++) {
+                    fq = q == w ? pf[w] : fq;
+                    bq = q == w ? pb[w] : bq;
+                }
+                src = fq + (e - bq);
+                sd[u] = e;
+            }
+            sx[u] = A.xy[src];
+            sv[u] = A.vt[src];
+            sg[u] = A.gt[src];
+        }
+#pragma unroll
+        for (int u = 0; u < kRowPer; u++) {
+            const int e = (int)threadIdx.x + u * BLOCK;
+            if (e < nrow3) {
+                const int c = e >= 2 * nrow ? 2 : (e >= nrow ? 1 : 0);
+                const int k = kb0 + (e - c * nrow);
+                const int pc = k < 0 ? 0 : (k < ncy ? 1 : 2);
+                s_row[c][e - c * nrow] = pb[3 * c + pc] + (rcs[u] - pf[3 * c + pc]);
             }
         }
+#pragma unroll
+        for (int u = 0; u < kStgPer; u++)
+            if (sd[u] >= 0) {
+                s_xy[sd[u]] = sx[u];
+                s_vt[sd[u]] = sv[u];
+                s_gt[sd[u]] = sg[u];
+            }
     }
     __syncthreads();
     if (nrow > kMaxRows) {  // the host sizes tiles so that this cannot happen
@@ -953,19 +1018,10 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     const float gam = S.gamma, sdtk = S.sdt * kTheta32;
     const uint64_t sseed = S.sseed;
     const int ibeg = s_row[1][T.k0 - kb0], iend = s_row[1][T.k1 - kb0];
-    const int slot0 = cs[R.coff + T.cx * ncy + T.k0];
+    const int slot0 = s_piece[1][1][0] + (ibeg - s_piece[1][1][2]);  // T.k0 lies in piece B of column 1
     // A tile whose candidates overflow the LDS buffer (a dense clump) runs the
     // same loop on global memory; s_row offsets are then rebased per piece.
-    // Each lane owns one i-bead at a time.  With CAPL > 0, staged tiles run two
-    // phases: the cutoff test over all candidates appends the interacting ones
-    // to the lane's LDS list (u16 code: LDS index | column << 10 | piece << 12),
-    // then the force and noise arithmetic runs over that list only, so a
-    // wavefront executes it ~max(pairs per lane) times instead of
-    // ~max(candidates per lane) times; a full list spills to in-place
-    // evaluation.  With CAPL = 0 a single pass evaluates in place.  Tiles whose
-    // candidates overflow the LDS buffer (a dense clump) run the single pass on
-    // global memory, with s_row offsets rebased per piece.  The next
-    // candidate's position is loaded one iteration ahead.
+    // Each lane owns one i-bead at a time.
     double mb = 0.0;  // largest bond length seen by this lane
     // per-bead epilogue: energy, or force (+ stirring, + bonded terms)
     auto finish = [&](int slot, double2 pi, uint32_t gti, const BondPre &B, double ax, double ay, float e) {
@@ -1007,12 +1063,11 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             // (stepping past the end of one range lands on the next by two
             // selects), so the wavefront iterates ~max over lanes of the whole
             // window rather than the sum of per-column maxima.  With CAPL > 0
-            // it runs in two phases: the cutoff test over the sequence records
-            // the positions t < 256 of the interacting candidates (packed four
-            // per LDS word, CAPL per lane), then the force and noise arithmetic
-            // runs over that list only, so the wavefront executes it ~max over
+            // it runs in two phases: the cutoff test over the sequence marks
+            // the interacting candidates among the first 64 in a 64-bit
+            // register mask, then the force and noise arithmetic runs over
+            // the marked ones only, so the wavefront executes it ~max over
             // lanes of the pair count instead of ~max of the candidate count.
-            // A candidate that does not fit the list is evaluated in place.
             const bool img = sxc[0] != 0.0 || sxc[2] != 0.0 || klo < 0 || khi >= ncy;
             if (stg && !__any(img)) {
                 const int st0 = s_row[0][klo - kb0], en0 = s_row[0][khi + 1 - kb0];
@@ -1033,37 +1088,30 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                         j = j == en1 ? st2 : j;
                     }
                 } else {
-                    int cnt = 0, t = 0;
-                    uint32_t pack = 0;
+                    // pass 1: bit t of mk marks an interacting candidate t < 64
+                    // of the sequence (later ones are evaluated in place)
+                    uint64_t mk = 0;
                     for (int it = 0; it < nj; it++) {
                         const double2 pj = XY[j];
                         const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
                         const float r2 = __builtin_fmaf(dx, dx, dy * dy);  // the test of pair_one
                         if (r2 < 1.0f && r2 > 1e-24f) {
-                            if (cnt < CAPL && t < 256) {
-                                pack |= (uint32_t)t << (8 * (cnt & 3));
-                                if ((cnt & 3) == 3) {
-                                    s_lst[cnt >> 2][threadIdx.x] = pack;
-                                    pack = 0;
-                                }
-                                cnt++;
-                            } else {
+                            if (it < 64)
+                                mk |= 1ull << it;
+                            else
                                 pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, pj, VT[j], GT[j], 0.0, 0.0, gam, sdtk,
                                          sseed, ax, ay, e, ENERGY);
-                            }
                         }
                         ++j;
-                        ++t;
                         j = j == en0 ? st1 : j;
                         j = j == en1 ? st2 : j;
                     }
-                    if (cnt & 3) s_lst[cnt >> 2][threadIdx.x] = pack;
-                    const int n0 = en0 - st0, n01 = n0 + (en1 - st1);
-                    uint32_t w = 0;
-                    for (int k = 0; k < cnt; k++) {
-                        if ((k & 3) == 0) w = s_lst[k >> 2][threadIdx.x];
-                        const int tk = (w >> (8 * (k & 3))) & 255;
-                        const int jk = tk < n0 ? st0 + tk : (tk < n01 ? st1 + (tk - n0) : st2 + (tk - n01));
+                    // pass 2: the marked candidates in sequence order
+                    const int c0 = max(en0 - st0, 0), c01 = c0 + max(en1 - st1, 0);
+                    while (mk) {
+                        const int tk = __builtin_ctzll(mk);
+                        mk &= mk - 1;
+                        const int jk = tk < c0 ? st0 + tk : (tk < c01 ? st1 + (tk - c0) : st2 + (tk - c01));
                         pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jk], VT[jk], GT[jk], 0.0, 0.0, gam, sdtk, sseed,
                                  ax, ay, e, ENERGY);
                     }
