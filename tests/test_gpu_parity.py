@@ -130,6 +130,35 @@ def test_step_sort_bit_exact(oracle, eng, rng, nsub, boost):
     e.close()
 
 
+@pytest.mark.parametrize("nsub,boost", [(4, None), (4, (40, 60.0))])
+def test_force_kernel_keys_equal_separate_keys(oracle, eng, rng, nsub, boost):
+    """The second step's sort takes its cell keys from the force kernel of the
+    first step; a run that reloads the state between the two steps (keys from
+    the separate key kernel) reaches the same positions, velocities and cells
+    (the order inside a cell follows the load order, so cells are compared as
+    sets)."""
+    say(f"case: fused keys nsub={nsub} boost={boost}")
+    p = mkparams(20.3, nsub=nsub, f_lip=0.2, seed=3)
+    st = oracle_after_steps(oracle, p, 5, rng, boost)
+    e1 = eng.Engine(p)
+    e1.set_state(st, step=5)
+    e1.step(2, graph=False)
+    e2 = eng.Engine(p)
+    e2.set_state(st, step=5)
+    e2.step(1, graph=False)
+    e2.set_state(e2.get_state(), step=6)
+    e2.step(1, graph=False)
+    a, b = e1.get_state(), e2.get_state()
+    for k in ("x", "y", "vx", "vy"):
+        assert np.array_equal(a[k], b[k]), f"{k} differs"
+    (cs1, g1), (cs2, g2) = e1.cells(), e2.cells()
+    assert np.array_equal(cs1, cs2), "cstart differs"
+    cell = np.repeat(np.arange(cs1.size - 1, dtype=np.int64), np.diff(cs1))
+    assert np.array_equal(np.sort((cell << 32) | g1.astype(np.int64)), np.sort((cell << 32) | g2.astype(np.int64)))
+    e1.close()
+    e2.close()
+
+
 def test_sort_matches_reference_order(oracle, eng, rng):
     """The device's cell list of a loaded state equals the stable counting sort
     (oracle restatement of sort_cells) of the same positions."""

@@ -52,7 +52,8 @@ struct oc_engine {
     int32_t *d_tcount = nullptr, *d_toff = nullptr, *d_tsrc = nullptr, *d_tslot = nullptr, *d_locst = nullptr;
     uint32_t *d_moved = nullptr;  // a bead moved by more than one cell this step
     int32_t *rec_src = nullptr;   // when set, the sort writes src (new slot -> old slot) there
-    int2 *d_bmax = nullptr;
+    int2 *d_rmax = nullptr;      // per replica: largest cell displacement of a bead that moved > 1 cell
+    bool keys_fresh = false;     // d_key holds the keys of the current state (written by the force kernel)
     double *d_stir = nullptr, *d_epot = nullptr, *d_ebond = nullptr, *d_part = nullptr, *d_statout = nullptr;
     uint8_t *d_flag = nullptr;
     int32_t *d_best = nullptr;
@@ -206,6 +207,12 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false) {
     A.f = h->d_f;
     A.epot = h->d_epot;
     A.err = h->d_err;
+    // forces also leave the next step's cell keys
+    A.v = h->st[c].v;
+    A.key = energy ? nullptr : h->d_key;
+    A.rmax = h->d_rmax;
+    A.moved = h->d_moved;
+    if (!energy) h->keys_fresh = true;
     const dim3 grid(8 * h->per_xcd);
     // CAP (staged candidates) is 3.5x the block; `lists` picks the two-phase
     // variant (cutoff test, then the marked pairs), or the single pass
@@ -264,20 +271,27 @@ static void launch_chem(oc_engine *h) {
 // sort (with integration when integ): keys, tile sort, tile offsets,
 // placement; flips cur.  step_delta advances every replica's step counter and
 // per-step values (in k_tscan; no later kernel of the sort reads them).
-static void launch_sort(oc_engine *h, bool integ, int step_delta) {
-    const int c = h->cur, n = c ^ 1;
-    const int nb = (int)h->bblk.size(), nt = (int)h->stiles.size();
+static void launch_keys(oc_engine *h, bool integ) {
+    const int c = h->cur, nb = (int)h->bblk.size();
     if (integ)
         k_keys<true><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->st[c].xy, h->st[c].v, h->d_f, h->d_key,
-                                                       h->d_bmax, h->d_moved);
+                                                       h->d_rmax, h->d_moved);
     else
         k_keys<false><<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->st[c].xy, h->st[c].v, h->d_f,
-                                                        h->d_key, h->d_bmax, h->d_moved);
+                                                        h->d_key, h->d_rmax, h->d_moved);
+}
+
+static void launch_sort(oc_engine *h, bool integ, int step_delta) {
+    const int c = h->cur, n = c ^ 1;
+    const int nt = (int)h->stiles.size();
+    // the force kernel left the keys of this state's integrate unless the
+    // state changed since
+    if (!(integ && h->keys_fresh)) launch_keys(h, integ);
     const int ng = std::min(nt, kSortGrid);
-    k_tsort<<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_cs[c], h->d_key, h->d_bmax, h->d_moved,
+    k_tsort<<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_cs[c], h->d_key, h->d_rmax, h->d_moved,
                                              h->d_tcount, h->d_locst, h->d_tsrc, h->d_tslot, h->d_err, nt);
     k_tscan<<<1, 1024, 0, h->stream>>>(h->d_ri, h->d_rs, h->nrep, h->d_tcount, nt, h->d_toff, h->d_moved,
-                                       step_delta);
+                                       h->d_rmax, step_delta);
     int32_t *src = h->rec_src;
     if (integ)
         k_place<true><<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,
@@ -288,6 +302,7 @@ static void launch_sort(oc_engine *h, bool integ, int step_delta) {
                                                         h->d_tsrc, h->d_tslot, h->d_cs[n], src, h->st[c],
                                                         h->st[n], h->d_f, h->d_err, nt);
     h->cur = n;
+    h->keys_fresh = false;
 }
 
 static void launch_step_params(oc_engine *h, int64_t delta, int64_t set_step) {
@@ -523,7 +538,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     rc = rc ? rc : dalloc(h, &h->d_tslot, Nt);
     rc = rc ? rc : dalloc(h, &h->d_locst, Ct);
     rc = rc ? rc : dalloc(h, &h->d_moved, 1);
-    rc = rc ? rc : dalloc(h, &h->d_bmax, h->bblk.size());
+    rc = rc ? rc : dalloc(h, &h->d_rmax, h->nrep);
     rc = rc ? rc : dalloc(h, &h->d_epot, Nt);
     rc = rc ? rc : dalloc(h, &h->d_ebond, Nt);
     rc = rc ? rc : dalloc(h, &h->d_part, h->rblk.size() * 16);
@@ -556,6 +571,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     e = e ? e : hipMemset(h->d_evn, 0, sizeof(unsigned long long));
     e = e ? e : hipMemset(h->d_err, 0, 4);
     e = e ? e : hipMemset(h->d_moved, 0, 4);
+    e = e ? e : hipMemset(h->d_rmax, 0, h->nrep * sizeof(int2));
     e = e ? e : hipMemset(h->d_ebond, 0, Nt * sizeof(double));
     e = e ? e : hipMemset(h->d_epot, 0, Nt * sizeof(double));
     e = e ? e : hipMemset(h->d_f, 0, Nt * sizeof(double2));
@@ -697,6 +713,7 @@ OC_API int oc_start(oc_engine *h, const oc_state *s) {
     launch_step_params(h, 0, -1);
     launch_forces(h, true);
     k_half_kick<<<(int)h->bblk.size(), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, h->st[h->cur].v, h->d_f);
+    h->keys_fresh = false;
     HIPCHK(hipGetLastError());
     return check_err(h);
 }
@@ -706,6 +723,7 @@ OC_API int oc_set_state(oc_engine *h, int32_t r, const oc_state *s, int64_t step
     HIPCHK(hipStreamSynchronize(h->stream));
     if (int rc = load_replica(h, r, s, 0, false)) return rc;
     if (int rc = set_step_fields(h, r, step)) return rc;
+    h->keys_fresh = false;
     launch_build_prv(h);
     launch_step_params(h, 0, -1);
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -749,15 +767,20 @@ OC_API int oc_get_state(oc_engine *h, int32_t r, oc_state *s) {
 
 // --------------------------------------------------------------------------- stepping
 
+// Two steps, captured with the keys of the first taken from the force
+// kernel before it (oc_step makes them fresh before a replay).
 static int capture_graph(oc_engine *h, int parity) {
     const int saved = h->cur;
+    const bool fresh = h->keys_fresh;
     h->cur = parity;
+    h->keys_fresh = true;
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     enqueue_step(h);
     enqueue_step(h);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     h->cur = saved;
+    h->keys_fresh = fresh;
     if (e != hipSuccess) return fail(OC_EHIP, std::string("graph capture failed: ") + hipGetErrorString(e));
     h->graph[parity] = g;
     HIPCHK(hipGraphInstantiate(&h->gexec[parity], g, nullptr, nullptr, 0));
@@ -773,7 +796,9 @@ OC_API int oc_step(oc_engine *h, int64_t n, int32_t use_graph) {
         const int p = h->cur;
         if (!h->gexec[p])
             if (int rc = capture_graph(h, p)) return rc;
+        if (!h->keys_fresh) launch_keys(h, true);
         for (; k + 2 <= n; k += 2) HIPCHK(hipGraphLaunch(h->gexec[p], h->stream));
+        h->keys_fresh = true;
     }
     for (; k < n; k++) enqueue_step(h);
     HIPCHK(hipGetLastError());
@@ -1050,6 +1075,7 @@ static int snap_copy(oc_engine *h, StateSnap &s, bool save) {
         add(h->d_rs, s.rs, h->nrep * sizeof(RepStep));
     }
     k_copy_multi<<<512, 256, 0, h->stream>>>(J);
+    if (!save) h->keys_fresh = false;
     HIPCHK(hipGetLastError());
     return 0;
 }

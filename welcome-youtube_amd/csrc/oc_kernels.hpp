@@ -105,54 +105,45 @@ __global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ 
 // --------------------------------------------------------------------------- sort
 
 // New cell key of every bead (of its integrated position when INTEG; bit 31
-// set when the bead changed cells); each block stores its largest periodic
-// cell displacement (x, y) in bmax[block]
-// (no atomics), and a bead that moved by more than one cell sets *moved (a
-// rare event: the sort then widens its window to the replica's maximum).
+// set when the bead changed cells).  A bead that moved by more than one cell
+// (a rare event) sets *moved and raises its replica's largest periodic cell
+// displacement rmax[r] (x, y); the sort then widens its window to it.  The
+// force kernel computes the same keys in its epilogue (new_key), so a step
+// whose forces came from it skips this launch.
+__device__ __forceinline__ void new_key(const RepInfo &R, int r, int s, double2 p, double2 q, int32_t *key,
+                                        int2 *rmax, uint32_t *moved) {
+    const int ox = cell_x(R, p.x), oy = cell_y(R, p.y);
+    const int cx = cell_x(R, q.x), cy = cell_y(R, q.y);
+    // bit 31: the bead changed cells
+    key[s] = (R.coff + cx * R.ncy + cy) | (cx != ox || cy != oy ? (int32_t)0x80000000 : 0);
+    int dx = abs(cx - ox), dy = abs(cy - oy);
+    dx = min(dx, R.ncx - dx);
+    dy = min(dy, R.ncy - dy);
+    if (dx > 1 || dy > 1) {
+        atomicMax(&rmax[r].x, dx);
+        atomicMax(&rmax[r].y, dy);
+        atomicOr(moved, 1u);
+    }
+}
+
 template <bool INTEG>
 __global__ __launch_bounds__(kBeadBlock) void k_keys(const RepInfo *__restrict__ ri, const int32_t *__restrict__ bblk,
                                                      const double2 *__restrict__ xy, const double2 *__restrict__ v,
                                                      const double2 *__restrict__ f, int32_t *__restrict__ key,
-                                                     int2 *__restrict__ bmax, uint32_t *__restrict__ moved) {
-    __shared__ int red[2][kBeadBlock / 64];
+                                                     int2 *__restrict__ rmax, uint32_t *__restrict__ moved) {
     const int r = bblk[blockIdx.x];
     const RepInfo &R = ri[r];
     const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
-    int dx = 0, dy = 0;
-    if (loc < R.N) {
-        const int s = R.off + loc;
-        const double2 p = xy[s];
-        const int ox = cell_x(R, p.x), oy = cell_y(R, p.y);
-        double2 q = p;
-        if (INTEG) {
-            double2 vo;
-            float2 vt;
-            integrate1(R, p, v[s], f[s], q, vo, vt);
-        }
-        const int cx = cell_x(R, q.x), cy = cell_y(R, q.y);
-        // bit 31: the bead changed cells
-        key[s] = (R.coff + cx * R.ncy + cy) | (cx != ox || cy != oy ? (int32_t)0x80000000 : 0);
-        dx = abs(cx - ox);
-        dx = min(dx, R.ncx - dx);
-        dy = abs(cy - oy);
-        dy = min(dy, R.ncy - dy);
+    if (loc >= R.N) return;
+    const int s = R.off + loc;
+    const double2 p = xy[s];
+    double2 q = p;
+    if (INTEG) {
+        double2 vo;
+        float2 vt;
+        integrate1(R, p, v[s], f[s], q, vo, vt);
     }
-    dx = wave_max(dx);
-    dy = wave_max(dy);
-    if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x >> 6] = dx;
-        red[1][threadIdx.x >> 6] = dy;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int mx = 0, my = 0;
-        for (int w = 0; w < kBeadBlock / 64; w++) {
-            mx = max(mx, red[0][w]);
-            my = max(my, red[1][w]);
-        }
-        bmax[blockIdx.x] = make_int2(mx, my);
-        if (mx > 1 || my > 1) atomicOr(moved, 1u);
-    }
+    new_key(R, r, s, p, q, key, rmax, moved);
 }
 
 // Piece list of the periodic window [c - R, c + R] of 0..n-1 in ascending order.
@@ -206,7 +197,7 @@ __device__ __forceinline__ int block_excl_scan128(int v, int *tmp, int &total) {
 __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ ri, const Tile *__restrict__ tiles,
                                                     const int32_t *__restrict__ cs_old,
                                                     const int32_t *__restrict__ key,
-                                                    const int2 *__restrict__ bmax,
+                                                    const int2 *__restrict__ rmax,
                                                     const uint32_t *__restrict__ moved,
                                                     int32_t *__restrict__ tcount, int32_t *__restrict__ locst,
                                                     int32_t *__restrict__ tsrc, int32_t *__restrict__ tslot,
@@ -218,7 +209,7 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
     __shared__ int32_t s_cnt[kSortRows], s_alow[kSortRows], s_start[kSortRows];
     __shared__ int32_t s_mv[kSortMv];  // staged ids of the tile's beads that changed cells
     __shared__ int s_nmv;
-    __shared__ int s_tmp[2], s_rad[2][2];
+    __shared__ int s_tmp[2];
 
     auto do_tile = [&](const int tix) {
         const Tile T = tiles[tix];
@@ -230,22 +221,9 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
         int Rx = 1, Ry = 1;
         if (*moved) {
             // the replica's largest displacement
-            const int nblk = (R.N + kBeadBlock - 1) / kBeadBlock;
-            int mx = 1, my = 1;
-            for (int b = t; b < nblk; b += kSortThr) {
-                const int2 m = bmax[R.bblk0 + b];
-                mx = max(mx, m.x);
-                my = max(my, m.y);
-            }
-            mx = wave_max(mx);
-            my = wave_max(my);
-            if ((t & 63) == 0) {
-                s_rad[0][t >> 6] = mx;
-                s_rad[1][t >> 6] = my;
-            }
-            __syncthreads();
-            Rx = max(s_rad[0][0], s_rad[0][1]);
-            Ry = max(s_rad[1][0], s_rad[1][1]);
+            const int2 m = rmax[T.rep];
+            Rx = max(1, m.x);
+            Ry = max(1, m.y);
         }
         // staged rows k0-1 .. k1 (mod ncy) as row pieces P0 = [a0, b0], P1 =
         // [a1, b1] in increasing row (= slot) order; the three columns ascending
@@ -478,13 +456,14 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
 }
 
 // Exclusive scan of the tile counts -> first new slot of each tile (tiles are
-// in cell order, cells in slot order); clears the displacement flag; and,
+// in cell order, cells in slot order); clears the displacement flag and
+// maxima; and,
 // when step_delta > 0, the next step's per-replica values (the step counter
 // advance of the step sequence).  One block.
 __global__ __launch_bounds__(1024) void k_tscan(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs, int nrep,
                                                 const int32_t *__restrict__ tcount, int nt,
                                                 int32_t *__restrict__ toff, uint32_t *__restrict__ moved,
-                                                int step_delta) {
+                                                int2 *__restrict__ rmax, int step_delta) {
     __shared__ int wsum[16];
     const int per = (nt + 1023) / 1024;
     const int b0 = threadIdx.x * per;
@@ -521,6 +500,7 @@ __global__ __launch_bounds__(1024) void k_tscan(const RepInfo *__restrict__ ri, 
             }
     }
     if (threadIdx.x == 0) *moved = 0u;
+    for (int r = threadIdx.x; r < nrep; r += blockDim.x) rmax[r] = make_int2(0, 0);
     if (step_delta) {
         const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
         for (int r = wv; r < nrep; r += 16) step_params_wave(ri[r], rs[r], r, lane, step_delta, -1, nullptr);
@@ -853,6 +833,11 @@ struct PairArgs {
     double2 *f;
     double *epot;
     uint32_t *err;
+    // next step's cell keys (new_key of the integrated position) when key != 0
+    const double2 *v;
+    int32_t *key;
+    int2 *rmax;
+    uint32_t *moved;
 };
 
 // Full-shell pair forces of the i-beads of one tile (a column, rows [k0,k1)).
@@ -1024,7 +1009,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     // Each lane owns one i-bead at a time.
     double mb = 0.0;  // largest bond length seen by this lane
     // per-bead epilogue: energy, or force (+ stirring, + bonded terms)
-    auto finish = [&](int slot, double2 pi, uint32_t gti, const BondPre &B, double ax, double ay, float e) {
+    auto finish = [&](int slot, double2 pi, uint32_t gti, const BondPre &B, double ax, double ay, float e,
+                      double2 vh) {
         if (ENERGY) {
             A.epot[slot] = (double)e;
         } else {
@@ -1033,6 +1019,13 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             double fx = bx + ax, fy = ay;
             if (A.bonded) bonded_gather<ANG>(R, pi, gti, B, A.xy, A.lnk, A.prv, fx, fy, mb);
             A.f[slot] = make_double2(fx, fy);
+            if (A.key) {
+                // the next step's integrate, as k_keys<true> would run it
+                double2 q, vo;
+                float2 vtn;
+                integrate1(R, pi, vh, make_double2(fx, fy), q, vo, vtn);
+                new_key(R, T.rep, slot, pi, q, A.key, A.rmax, A.moved);
+            }
         }
     };
     auto run = [&](const double2 *XY, const float2 *VT, const uint32_t *GT, const bool stg) {
@@ -1042,6 +1035,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             BondPre B;
             const bool bond = !ENERGY && A.bonded;
             if (bond) bond_links(slot, A.mp, A.lnk, A.prv, B);
+            const double2 vh = !ENERGY && A.key ? A.v[slot] : make_double2(0.0, 0.0);
             const double2 pi = XY[is];
             const float2 vi = VT[is];
             const uint32_t gti = GT[is];
@@ -1117,7 +1111,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                     }
                 }
                 if (bond) bond_partners(A.xy, A.gt, B);
-                finish(slot, pi, gti, B, ax, ay, e);
+                finish(slot, pi, gti, B, ax, ay, e, vh);
                 continue;
             }
             // general path: per column and periodic piece (image shifts), single pass
@@ -1140,7 +1134,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                 }
             }
             if (bond) bond_partners(A.xy, A.gt, B);
-            finish(slot, pi, gti, B, ax, ay, e);
+            finish(slot, pi, gti, B, ax, ay, e, vh);
         }
     };
     if (staged) run(s_xy, s_vt, s_gt, true);
