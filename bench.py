@@ -344,7 +344,7 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": "plain 2D DPD step at scale: N=1,048,576, rho=3, L=591.2067, 50% W / 20% P / 30% H-T-T "
+            "workload": f"plain 2D DPD step at scale: N={N:,}, rho=3, L={p.L:.4f}, 50% W / 20% P / 30% H-T-T "
                         "lipid beads, precursor chemistry every step, dt=0.01, k_angle=0, warmup=0",
             "N": N, "L": p.L, "nsub": a.nsub, "graph": not a.no_graph,
             "state": "fp64 positions/velocities/forces, fp32 pair arithmetic",

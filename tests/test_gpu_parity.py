@@ -159,6 +159,29 @@ def test_force_kernel_keys_equal_separate_keys(oracle, eng, rng, nsub, boost):
     e2.close()
 
 
+def test_sort_invariants_many_tiles(eng):
+    """N = 2,000,000 (L = 816.5, more than 8,192 sort tiles, so the tile-offset
+    scan runs over several blocks): after graph-replayed steps the cell list is
+    a permutation and every bead lies in the cell its slot belongs to."""
+    say("case: sort invariants N=2M")
+    p = mkparams(816.5, f_lip=0.3)
+    e = eng.Engine(p)
+    e.start()
+    e.step(4)
+    cs, g = e.cells()
+    st = e.get_state()
+    ncx = int(np.floor(p.L))
+    ncy = ncx * p.nsub
+    x, y = st["x"][g], st["y"][g]
+    cx = np.minimum((x * (ncx / p.L)).astype(np.int64), ncx - 1)
+    cy = np.minimum((y * (ncy / p.L)).astype(np.int64), ncy - 1)
+    assert cs[0] == 0 and cs[-1] == p.N and np.all(np.diff(cs) >= 0)
+    assert np.unique(g).size == p.N
+    cell = np.repeat(np.arange(cs.size - 1, dtype=np.int64), np.diff(cs))
+    assert np.array_equal(cell, cx * ncy + cy)
+    e.close()
+
+
 def test_sort_matches_reference_order(oracle, eng, rng):
     """The device's cell list of a loaded state equals the stable counting sort
     (oracle restatement of sort_cells) of the same positions."""

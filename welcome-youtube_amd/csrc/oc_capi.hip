@@ -52,6 +52,7 @@ struct oc_engine {
     int32_t *d_tcount = nullptr, *d_toff = nullptr, *d_tsrc = nullptr, *d_tslot = nullptr, *d_locst = nullptr;
     uint32_t *d_moved = nullptr;  // a bead moved by more than one cell this step
     int32_t *rec_src = nullptr;   // when set, the sort writes src (new slot -> old slot) there
+    int32_t *d_tpart = nullptr;  // chunk totals of the tile-count scan
     int2 *d_rmax = nullptr;      // per replica: largest cell displacement of a bead that moved > 1 cell
     bool keys_fresh = false;     // d_key holds the keys of the current state (written by the force kernel)
     double *d_stir = nullptr, *d_epot = nullptr, *d_ebond = nullptr, *d_part = nullptr, *d_statout = nullptr;
@@ -290,8 +291,10 @@ static void launch_sort(oc_engine *h, bool integ, int step_delta) {
     const int ng = std::min(nt, kSortGrid);
     k_tsort<<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_cs[c], h->d_key, h->d_rmax, h->d_moved,
                                              h->d_tcount, h->d_locst, h->d_tsrc, h->d_tslot, h->d_err, nt);
-    k_tscan<<<1, 1024, 0, h->stream>>>(h->d_ri, h->d_rs, h->nrep, h->d_tcount, nt, h->d_toff, h->d_moved,
-                                       h->d_rmax, step_delta);
+    const int nch = (nt + kTScanChunk - 1) / kTScanChunk;
+    if (nch > 1) k_tsum<<<nch, 1024, 0, h->stream>>>(h->d_tcount, nt, h->d_tpart);
+    k_tscan<<<nch, 1024, 0, h->stream>>>(h->d_ri, h->d_rs, h->nrep, h->d_tcount, nt, nch > 1 ? h->d_tpart : nullptr,
+                                         h->d_toff, h->d_moved, h->d_rmax, step_delta);
     int32_t *src = h->rec_src;
     if (integ)
         k_place<true><<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,
@@ -539,6 +542,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     rc = rc ? rc : dalloc(h, &h->d_locst, Ct);
     rc = rc ? rc : dalloc(h, &h->d_moved, 1);
     rc = rc ? rc : dalloc(h, &h->d_rmax, h->nrep);
+    rc = rc ? rc : dalloc(h, &h->d_tpart, (h->stiles.size() + kTScanChunk - 1) / kTScanChunk);
     rc = rc ? rc : dalloc(h, &h->d_epot, Nt);
     rc = rc ? rc : dalloc(h, &h->d_ebond, Nt);
     rc = rc ? rc : dalloc(h, &h->d_part, h->rblk.size() * 16);

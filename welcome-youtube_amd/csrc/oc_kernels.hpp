@@ -456,49 +456,60 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
 }
 
 // Exclusive scan of the tile counts -> first new slot of each tile (tiles are
-// in cell order, cells in slot order); clears the displacement flag and
-// maxima; and,
-// when step_delta > 0, the next step's per-replica values (the step counter
-// advance of the step sequence).  One block.
+// in cell order, cells in slot order), kTScanChunk counts per block (eight
+// per thread, loaded together).  With more than one chunk, part holds the
+// chunk totals (k_tsum) and each block adds those of the chunks before it.
+// Block 0 also clears the displacement flag and maxima and, when
+// step_delta > 0, sets the next step's per-replica values (the step counter
+// advance of the step sequence).
+constexpr int kTScanChunk = 1024 * 8;
+__global__ __launch_bounds__(1024) void k_tsum(const int32_t *__restrict__ tcount, int nt,
+                                               int32_t *__restrict__ part) {
+    __shared__ int wsum[16];
+    const int b0 = blockIdx.x * kTScanChunk + threadIdx.x * 8;
+    int loc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) loc += b0 + k < nt ? tcount[b0 + k] : 0;
+    const int inc = wave_incl_scan(loc);
+    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < 16; w++) t += wsum[w];
+        part[blockIdx.x] = t;
+    }
+}
+
 __global__ __launch_bounds__(1024) void k_tscan(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs, int nrep,
                                                 const int32_t *__restrict__ tcount, int nt,
-                                                int32_t *__restrict__ toff, uint32_t *__restrict__ moved,
-                                                int2 *__restrict__ rmax, int step_delta) {
+                                                const int32_t *__restrict__ part, int32_t *__restrict__ toff,
+                                                uint32_t *__restrict__ moved, int2 *__restrict__ rmax,
+                                                int step_delta) {
     __shared__ int wsum[16];
-    const int per = (nt + 1023) / 1024;
-    const int b0 = threadIdx.x * per;
+    const int b0 = blockIdx.x * kTScanChunk + threadIdx.x * 8;
     int c8[8];
     int loc = 0;
-    if (per <= 8) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            c8[k] = k < per && b0 + k < nt ? tcount[b0 + k] : 0;
-            loc += c8[k];
-        }
-    } else {
-        for (int k = 0; k < per; k++)
-            if (b0 + k < nt) loc += tcount[b0 + k];
+    for (int k = 0; k < 8; k++) {
+        c8[k] = b0 + k < nt ? tcount[b0 + k] : 0;
+        loc += c8[k];
     }
+    int base = 0;  // counts of the chunks before this one
+    if (part)
+        for (int q = 0; q < (int)blockIdx.x; q++) base += part[q];
     const int inc = wave_incl_scan(loc);
     if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = inc;
     __syncthreads();
     int woff = 0;
     for (int w = 0; w < (int)(threadIdx.x >> 6); w++) woff += wsum[w];
-    int run = woff + inc - loc;
-    if (per <= 8) {
+    int run = base + woff + inc - loc;
 #pragma unroll
-        for (int k = 0; k < 8; k++)
-            if (k < per && b0 + k < nt) {
-                toff[b0 + k] = run;
-                run += c8[k];
-            }
-    } else {
-        for (int k = 0; k < per; k++)
-            if (b0 + k < nt) {
-                toff[b0 + k] = run;
-                run += tcount[b0 + k];
-            }
-    }
+    for (int k = 0; k < 8; k++)
+        if (b0 + k < nt) {
+            toff[b0 + k] = run;
+            run += c8[k];
+        }
+    if (blockIdx.x != 0) return;
     if (threadIdx.x == 0) *moved = 0u;
     for (int r = threadIdx.x; r < nrep; r += blockDim.x) rmax[r] = make_int2(0, 0);
     if (step_delta) {
