@@ -291,10 +291,10 @@ static void launch_sort(oc_engine *h, bool integ, int step_delta) {
     const int ng = std::min(nt, kSortGrid);
     k_tsort<<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_cs[c], h->d_key, h->d_rmax, h->d_moved,
                                              h->d_tcount, h->d_locst, h->d_tsrc, h->d_tslot, h->d_err, nt);
-    const int nch = (nt + kTScanChunk - 1) / kTScanChunk;
-    if (nch > 1) k_tsum<<<nch, 1024, 0, h->stream>>>(h->d_tcount, nt, h->d_tpart);
-    k_tscan<<<nch, 1024, 0, h->stream>>>(h->d_ri, h->d_rs, h->nrep, h->d_tcount, nt, nch > 1 ? h->d_tpart : nullptr,
-                                         h->d_toff, h->d_moved, h->d_rmax, step_delta);
+    const int nch = (nt + kTScanChunk - 1) / kTScanChunk, per = nch > 1 ? 8 : (nt + 1023) / 1024;
+    if (nch > 1) k_tsum<<<nch, 1024, 0, h->stream>>>(h->d_tcount, nt, per, h->d_tpart);
+    k_tscan<<<nch, 1024, 0, h->stream>>>(h->d_ri, h->d_rs, h->nrep, h->d_tcount, nt, per,
+                                         nch > 1 ? h->d_tpart : nullptr, h->d_toff, h->d_moved, h->d_rmax, step_delta);
     int32_t *src = h->rec_src;
     if (integ)
         k_place<true><<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,

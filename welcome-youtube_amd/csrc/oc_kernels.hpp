@@ -456,20 +456,21 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
 }
 
 // Exclusive scan of the tile counts -> first new slot of each tile (tiles are
-// in cell order, cells in slot order), kTScanChunk counts per block (eight
-// per thread, loaded together).  With more than one chunk, part holds the
-// chunk totals (k_tsum) and each block adds those of the chunks before it.
+// in cell order, cells in slot order), 1024 * per counts per block (per <= 8
+// consecutive counts a thread, loaded together).  With more than one chunk,
+// part holds the chunk totals (k_tsum) and each block adds those of the
+// chunks before it.
 // Block 0 also clears the displacement flag and maxima and, when
 // step_delta > 0, sets the next step's per-replica values (the step counter
 // advance of the step sequence).
-constexpr int kTScanChunk = 1024 * 8;
-__global__ __launch_bounds__(1024) void k_tsum(const int32_t *__restrict__ tcount, int nt,
+constexpr int kTScanChunk = 1024 * 8;  // largest chunk
+__global__ __launch_bounds__(1024) void k_tsum(const int32_t *__restrict__ tcount, int nt, int per,
                                                int32_t *__restrict__ part) {
     __shared__ int wsum[16];
-    const int b0 = blockIdx.x * kTScanChunk + threadIdx.x * 8;
+    const int b0 = (blockIdx.x * 1024 + threadIdx.x) * per;
     int loc = 0;
 #pragma unroll
-    for (int k = 0; k < 8; k++) loc += b0 + k < nt ? tcount[b0 + k] : 0;
+    for (int k = 0; k < 8; k++) loc += k < per && b0 + k < nt ? tcount[b0 + k] : 0;
     const int inc = wave_incl_scan(loc);
     if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = inc;
     __syncthreads();
@@ -481,17 +482,17 @@ __global__ __launch_bounds__(1024) void k_tsum(const int32_t *__restrict__ tcoun
 }
 
 __global__ __launch_bounds__(1024) void k_tscan(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs, int nrep,
-                                                const int32_t *__restrict__ tcount, int nt,
+                                                const int32_t *__restrict__ tcount, int nt, int per,
                                                 const int32_t *__restrict__ part, int32_t *__restrict__ toff,
                                                 uint32_t *__restrict__ moved, int2 *__restrict__ rmax,
                                                 int step_delta) {
     __shared__ int wsum[16];
-    const int b0 = blockIdx.x * kTScanChunk + threadIdx.x * 8;
+    const int b0 = (blockIdx.x * 1024 + threadIdx.x) * per;
     int c8[8];
     int loc = 0;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        c8[k] = b0 + k < nt ? tcount[b0 + k] : 0;
+        c8[k] = k < per && b0 + k < nt ? tcount[b0 + k] : 0;
         loc += c8[k];
     }
     int base = 0;  // counts of the chunks before this one
@@ -505,7 +506,7 @@ __global__ __launch_bounds__(1024) void k_tscan(const RepInfo *__restrict__ ri, 
     int run = base + woff + inc - loc;
 #pragma unroll
     for (int k = 0; k < 8; k++)
-        if (b0 + k < nt) {
+        if (k < per && b0 + k < nt) {
             toff[b0 + k] = run;
             run += c8[k];
         }
@@ -587,7 +588,7 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
             float2 vt[kPl];
             uint32_t gq[kPl];
             int8_t mq[kPl];
-            int32_t lq[kPl];
+            int32_t lq[kPl], pq[kPl];
 #pragma unroll
             for (int k = 0; k < kPl; k++)
                 if (ok[k]) {
@@ -599,10 +600,15 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                     gq[k] = in.gt[s];
                     mq[k] = in.mp[s];
                     lq[k] = in.lnk[s];
+                    pq[k] = in.prv[s];
                 }
-            int32_t ts[kPl];
+            // (tile, rank) of the partners' old slots
+            int32_t ts[kPl], tp[kPl];
 #pragma unroll
-            for (int k = 0; k < kPl; k++) ts[k] = ok[k] && lq[k] >= 0 ? tslot[lq[k]] : -1;
+            for (int k = 0; k < kPl; k++) {
+                ts[k] = ok[k] && lq[k] >= 0 ? tslot[lq[k]] : -1;
+                tp[k] = ok[k] && mq[k] > 0 && pq[k] >= 0 ? tslot[pq[k]] : -1;
+            }
 #pragma unroll
             for (int k = 0; k < kPl; k++) {
                 if (!ok[k]) continue;
@@ -621,12 +627,10 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                 if (src) src[d] = sv[k];
                 const int8_t m = mq[k];
                 out.mp[d] = m;
-                const int32_t nl = ts[k] >= 0 ? toff[ts[k] >> kSortCapLog] + (ts[k] & (kSortCap - 1)) : -1;
-                out.lnk[d] = nl;
-                // every bead's prv is written once: by itself (chain head, non-lipid) or
-                // by its chain predecessor
-                if (m <= 0) out.prv[d] = -1;
-                if (m >= 0 && nl >= 0) out.prv[nl] = d;
+                // the partners' new slots: next / chemistry partner, and the
+                // chain predecessor of a lipid bead past the head
+                out.lnk[d] = ts[k] >= 0 ? toff[ts[k] >> kSortCapLog] + (ts[k] & (kSortCap - 1)) : -1;
+                out.prv[d] = tp[k] >= 0 ? toff[tp[k] >> kSortCapLog] + (tp[k] & (kSortCap - 1)) : -1;
             }
         }
         if (nx >= nt) break;
