@@ -21,7 +21,9 @@ import time
 from . import engine as _engine
 from .params import Params
 
-CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
+# the cache file: OC_AUTOTUNE_CACHE, or a JSON file next to this module
+CACHE = os.environ.get("OC_AUTOTUNE_CACHE") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                             "autotune_cache.json")
 NSUB = (1, 2, 3, 4, 6)
 BLOCKS = (64, 128, 256)
 TILE_FACTORS = (0.6, 0.8, 1.2)
