@@ -1251,20 +1251,33 @@ OC_API int oc_window_forward(oc_engine *h, int32_t nsteps, int32_t ckpt_every, c
     for (int seg = 0; seg < nseg; seg++) {
         const int len = std::min(K, nsteps - seg * K);
         if (int rc = snap_copy(h, a->ckpt[seg], true)) return rc;
-        for (int m = 1; m <= len; m++) step_into(h, a->tape[m], false);
+        // whole states (with the sort maps) into the tape: the last segment's
+        // tape is then the one the backward sweep starts from
+        for (int m = 1; m <= len; m++) step_into(h, a->tape[m], true);
         restore_bufs(h, eb);
         // the engine's buffers take the segment's last state back
         {
             const StateSnap &t = a->tape[len];
             const int c = h->cur;
+            const size_t N = h->Ntot;
             CopyJobs J;
-            J.n = 3;
-            J.dst[0] = (char *)h->st[c].xy, J.src[0] = (const char *)t.xy, J.bytes[0] = (int64_t)(h->Ntot * sizeof(double2));
-            J.dst[1] = (char *)h->st[c].gt, J.src[1] = (const char *)t.gt, J.bytes[1] = (int64_t)(h->Ntot * sizeof(uint32_t));
-            J.dst[2] = (char *)h->d_cs[c], J.src[2] = (const char *)t.cs, J.bytes[2] = (int64_t)((h->Ctot + 1) * sizeof(int32_t));
+            J.n = 0;
+            auto add = [&](void *d, const void *sp, size_t bytes) {
+                J.dst[J.n] = (char *)d;
+                J.src[J.n] = (const char *)sp;
+                J.bytes[J.n] = (int64_t)bytes;
+                J.n++;
+            };
+            add(h->st[c].xy, t.xy, N * sizeof(double2));
+            add(h->st[c].gt, t.gt, N * sizeof(uint32_t));
+            add(h->d_cs[c], t.cs, (h->Ctot + 1) * sizeof(int32_t));
+            add(h->st[c].vt, t.vt, N * sizeof(float2));
+            add(h->st[c].mp, t.mp, N);
+            add(h->st[c].lnk, t.lnk, N * sizeof(int32_t));
+            add(h->st[c].prv, t.prv, N * sizeof(int32_t));
             k_copy_multi<<<512, 256, 0, h->stream>>>(J);
         }
-        k_tlist<<<dim3(ncol, len), 256, 0, h->stream>>>(h->d_ri, a->d_cols, a->d_tape + 1, RA, 0);
+        k_tlist<<<dim3(ncol, len), 256, 0, h->stream>>>(h->d_ri, a->d_cols, a->d_tape + 1, RA, seg == nseg - 1);
         k_rdf<<<dim3(nbk, len), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, a->d_tape + 1, RA, a->rpart);
         k_rdf_reduce<<<dim3(nb, h->nrep), 256, 0, h->stream>>>(h->d_ri, nbk, len, a->rpart, a->racc, nb);
     }
@@ -1297,15 +1310,19 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
     const EngBufs eb = save_bufs(h);
     for (int seg = nseg - 1; seg >= 0; seg--) {
         const int n0 = seg * K, n1 = std::min(n0 + K, nsteps), len = n1 - n0;
-        // recompute the segment from its checkpoint (bit-identical) into the tape
-        if (int rc = snap_copy(h, a->ckpt[seg], false)) return rc;
-        launch_build_prv(h);
-        if (seg == 0)
-            if (int rc = snap_copy(h, a->tape[0], true)) return rc;
-        for (int m = 1; m <= len; m++) step_into(h, a->tape[m], true);
-        restore_bufs(h, eb);
+        // recompute the segment from its checkpoint (bit-identical) into the
+        // tape; the forward window left the last segment's tape (with its type
+        // lists) in place, so only segment 0 of a one-segment window is redone
+        if (seg != nseg - 1 || seg == 0) {
+            if (int rc = snap_copy(h, a->ckpt[seg], false)) return rc;
+            launch_build_prv(h);
+            if (seg == 0)
+                if (int rc = snap_copy(h, a->tape[0], true)) return rc;
+            for (int m = 1; m <= len; m++) step_into(h, a->tape[m], true);
+            restore_bufs(h, eb);
+            k_tlist<<<dim3(ncol, len), 256, 0, h->stream>>>(h->d_ri, a->d_cols, a->d_tape + 1, RA, 1);
+        }
         // d loss / d x of the RDF at the segment's states n0+1..n1, one launch
-        k_tlist<<<dim3(ncol, len), 256, 0, h->stream>>>(h->d_ri, a->d_cols, a->d_tape + 1, RA, 1);
         k_rdf_adj<<<dim3(nbk, len), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, a->d_tape + 1, RA, a->gw);
         if (seg == nseg - 1) {
             zero_f64(h, (double *)a->X[ab], 2 * N);
