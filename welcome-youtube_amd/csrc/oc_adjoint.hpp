@@ -16,6 +16,7 @@ namespace oc {
 struct RdfArgs {
     int32_t ta, tb, nbins;
     float rmax, width, dr, cut;
+    float idr;  // 1 / dr: the bin range of a pair; a bin at its ends has |t| >= 1 and adds nothing
     int32_t Rc;  // columns searched on each side
 };
 
@@ -223,8 +224,8 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdf(const RepInfo *__restrict__ 
                 const float r2 = dx * dx + dy * dy;
                 if (r2 >= A.cut * A.cut || r2 == 0.0f) return;
                 const float rr = sqrtf(r2);
-                const int b0 = max(0, (int)floorf((rr - A.width) / A.dr));
-                const int b1 = min(A.nbins - 1, (int)floorf((rr + A.width) / A.dr));
+                const int b0 = max(0, (int)floorf((rr - A.width) * A.idr));
+                const int b1 = min(A.nbins - 1, (int)floorf((rr + A.width) * A.idr));
                 for (int b = b0; b <= b1; b++) {
                     const float t = (rr - (b + 0.5f) * A.dr) * iw;
                     if (fabsf(t) >= 1.0f) continue;
@@ -302,9 +303,9 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdf_adj(const RepInfo *__restric
                 [&](float dx, float dy) {
                     const float r2 = dx * dx + dy * dy;
                     if (r2 >= A.cut * A.cut || r2 == 0.0f) return;
-                    const float rr = sqrtf(r2);
-                    const int b0 = max(0, (int)floorf((rr - A.width) / A.dr));
-                    const int b1 = min(A.nbins - 1, (int)floorf((rr + A.width) / A.dr));
+                    const float ir = __builtin_amdgcn_rsqf(r2), rr = r2 * ir;
+                    const int b0 = max(0, (int)floorf((rr - A.width) * A.idr));
+                    const int b1 = min(A.nbins - 1, (int)floorf((rr + A.width) * A.idr));
                     float ds = 0.0f;
                     for (int b = b0; b <= b1; b++) {
                         const float t = (rr - (b + 0.5f) * A.dr) * iw;
@@ -312,8 +313,8 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdf_adj(const RepInfo *__restric
                         const float q = 1.0f - t * t;
                         ds += w_s[b] * pref * 3.0f * q * q * (-2.0f * t * iw);
                     }
-                    ax += (double)(ds * dx / rr);
-                    ay += (double)(ds * dy / rr);
+                    ax += (double)(ds * dx * ir);
+                    ay += (double)(ds * dy * ir);
                 });
         }
         for (int o = kTlG / 2; o > 0; o >>= 1) {

@@ -1237,6 +1237,7 @@ OC_API int oc_window_forward(oc_engine *h, int32_t nsteps, int32_t ckpt_every, c
     RA.rmax = (float)spec->rmax;
     RA.width = (float)spec->width;
     RA.dr = (float)dr;
+    RA.idr = (float)(1.0 / dr);
     RA.cut = cut;
     RA.Rc = Rc;
     const int nbk = (int)h->bblk.size(), ncol = (int)h->cols.size();
