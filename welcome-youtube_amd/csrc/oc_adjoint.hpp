@@ -365,8 +365,8 @@ __global__ __launch_bounds__(64) void k_step_params_seg(const RepInfo *__restric
 // One reverse step at state n (the tape's arrays of that state): with the
 // force adjoint Fa,
 //   X_n = X + Jx^T Fa (pair and bond terms) + G_n, T_n = Jv^T Fa,
-//   theta_bar += Jtheta^T Fa (per gid, half of each pair: every pair is
-//   visited from both beads),
+//   theta_bar += Jtheta^T Fa (half of each pair: every pair is visited from
+//   both beads; summed per block in a fixed order),
 // then X_{n-1}, V_{n-1}, F_{n-1} scattered to the order of step n-1 through
 // the sort map (new slot i <- old slot src[i]).  Every term of bead i is
 // summed by its own group (gather form, no atomics).  Xo == nullptr: theta
@@ -380,7 +380,7 @@ struct AdjStep {
     const int8_t *mp;
     const double2 *Fa, *X, *V;
     double2 *Xo, *Vo, *Fo;
-    double *thb;
+    double *thp;  // per block of this kernel: running theta-gradient sums (8 doubles)
 };
 
 __global__ __launch_bounds__(kBeadBlock) void k_adj_rstep(const RepInfo *__restrict__ ri,
@@ -404,6 +404,19 @@ __global__ __launch_bounds__(kBeadBlock) void k_adj_rstep(const RepInfo *__restr
     const float2 vi = vt[i];
     const uint32_t gti = gt[i], ti = gti >> kGidBits, gi = gti & kGidMask;
     const double2 Fi = Fa[i];
+    // the step-back inputs of bead i, loaded now so that they arrive during
+    // the pair loop
+    const bool step_back = A.Xo != nullptr;
+    int pn = -1, pp = -1, od = 0;
+    double2 xin = make_double2(0.0, 0.0), gin = xin, vin = xin;
+    if (step_back) {
+        pn = A.lnk[i];
+        pp = A.mp[i] > 0 ? A.prv[i] : -1;
+        od = A.src[i];
+        xin = A.X[i];
+        gin = A.G[i];
+        vin = A.V[i];
+    }
     const float gam = S.gamma, sdt = S.sdt, sdg = S.sdt / (2.0f * S.gamma);
     const int ncy = R.ncy, ncx = R.ncx;
     const int cx = (int)(pi.x * R.icsx) >= ncx ? ncx - 1 : (int)(pi.x * R.icsx);
@@ -485,16 +498,37 @@ __global__ __launch_bounds__(kBeadBlock) void k_adj_rstep(const RepInfo *__restr
 #pragma unroll
         for (int q = 0; q < 7; q++) th[q] += __shfl_xor(th[q], o, kAdjG);
     }
-    if (!live || gl != 0) return;
-    double *o = A.thb + (size_t)(R.off + gi) * 8;
-    for (int q = 0; q < 7; q++) o[q] += th[q];
-    if (!A.Xo) return;
+    // theta gradient: the block's beads in a fixed tree, added to the block's
+    // running sum thp[block] (steps in the order of the sweep)
+    __shared__ double s_th[7][kRdfChunk];
+    const int bead = (int)threadIdx.x / kAdjG;
+    if (gl == 0)
+#pragma unroll
+        for (int q = 0; q < 7; q++) s_th[q][bead] = live ? th[q] : 0.0;
+    __syncthreads();
+    for (int w = kRdfChunk / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < 7 * w) {
+            const int q = threadIdx.x / w, k = threadIdx.x - q * w;
+            s_th[q][k] += s_th[q][k + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) A.thp[(size_t)blockIdx.x * 8 + threadIdx.x] += s_th[threadIdx.x][0];
+    if (!live || gl != 0 || !step_back) return;
     // harmonic bonds of bead i (FP64): the chain neighbours of a lipid bead,
     // the chemistry partner of another bead; with d = x_i - x_p and
     // l = Fa_i - Fa_p, X_i += g l + g' r (e.l) e, g = -kb (r - r0) / r,
     // g' = -kb r0 / r^2
-    auto bond = [&](int p) {
-        const double2 q = xy[p], Fp = Fa[p];
+    double2 qn = pi, fn = Fi, qp = pi, fp = Fi;
+    if (pn >= 0) {
+        qn = xy[pn];
+        fn = Fa[pn];
+    }
+    if (pp >= 0) {
+        qp = xy[pp];
+        fp = Fa[pp];
+    }
+    auto bond = [&](double2 q, double2 Fp) {
         const double dx = minimg(pi.x - q.x, R.L, R.halfL), dy = minimg(pi.y - q.y, R.L, R.halfL);
         const double rr = sqrt(dx * dx + dy * dy);
         const double g = -R.kb * (rr - R.r0) / rr, gp = -R.kb * R.r0 / (rr * rr);
@@ -504,34 +538,25 @@ __global__ __launch_bounds__(kBeadBlock) void k_adj_rstep(const RepInfo *__restr
         xa += g * lx + gp * rr * el * ex;
         ya += g * ly + gp * rr * el * ey;
     };
-    const int pn = A.lnk[i];
-    if (A.mp[i] >= 0) {
-        const int pp = A.prv[i];
-        if (pp >= 0) bond(pp);
-    }
-    if (pn >= 0) bond(pn);
+    if (pp >= 0) bond(qp, fp);
+    if (pn >= 0) bond(qn, fn);
     // + d loss / d x_n of the RDF, then the step back to n-1
-    const double2 xin = A.X[i], g = A.G[i], v = A.V[i];
-    const double xx = xin.x + xa + g.x, xy_ = xin.y + ya + g.y;
-    const int od = A.src[i];
+    const double xx = xin.x + xa + gin.x, xy_ = xin.y + ya + gin.y;
     A.Xo[od] = make_double2(xx, xy_);
-    A.Vo[od] = make_double2(R.dt * xx + ta + v.x, R.dt * xy_ + tb + v.y);
-    A.Fo[od] = make_double2(R.dt2 * xx + R.cvt * ta + R.dt * v.x, R.dt2 * xy_ + R.cvt * tb + R.dt * v.y);
+    A.Vo[od] = make_double2(R.dt * xx + ta + vin.x, R.dt * xy_ + tb + vin.y);
+    A.Fo[od] = make_double2(R.dt2 * xx + R.cvt * ta + R.dt * vin.x, R.dt2 * xy_ + R.cvt * tb + R.dt * vin.y);
 }
 
-// per-replica sum of the per-gid theta gradients in two fixed-order stages
-// (FP64): block (replica, part) sums a contiguous gid range into
-// tpart[(r * kThParts + part) * 8 + q]; k_theta_final adds the parts in order
-constexpr int kThParts = 64;
-__global__ __launch_bounds__(256) void k_theta_part(const RepInfo *__restrict__ ri, const double *__restrict__ thb,
-                                                    double *__restrict__ tpart) {
+// per-replica theta gradient: the sum of the block partials of the replica's
+// blocks, strided then an LDS tree (fixed order, FP64)
+__global__ __launch_bounds__(256) void k_theta_reduce(const RepInfo *__restrict__ ri, const double *__restrict__ thp,
+                                                      double *__restrict__ out) {
     __shared__ double red[7][256];
     const RepInfo &R = ri[blockIdx.x];
-    const int per = (R.N + kThParts - 1) / kThParts;
-    const int g0 = blockIdx.y * per, g1 = min(R.N, g0 + per);
+    const int k0 = R.bblk0 * kRdfSub, nb = (R.N + kBeadBlock - 1) / kBeadBlock * kRdfSub;
     double sq[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int g = g0 + (int)threadIdx.x; g < g1; g += 256) {
-        const double *t = thb + (size_t)(R.off + g) * 8;
+    for (int k = threadIdx.x; k < nb; k += 256) {
+        const double *t = thp + (size_t)(k0 + k) * 8;
 #pragma unroll
         for (int q = 0; q < 7; q++) sq[q] += t[q];
     }
@@ -544,14 +569,7 @@ __global__ __launch_bounds__(256) void k_theta_part(const RepInfo *__restrict__ 
             for (int q = 0; q < 7; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x < 7) tpart[((size_t)blockIdx.x * kThParts + blockIdx.y) * 8 + threadIdx.x] = red[threadIdx.x][0];
-}
-
-__global__ __launch_bounds__(64) void k_theta_final(const double *__restrict__ tpart, double *__restrict__ out) {
-    if (threadIdx.x >= 7) return;
-    double v = 0.0;
-    for (int k = 0; k < kThParts; k++) v += tpart[((size_t)blockIdx.x * kThParts + k) * 8 + threadIdx.x];
-    out[(size_t)blockIdx.x * 8 + threadIdx.x] = v;
+    if (threadIdx.x < 7) out[(size_t)blockIdx.x * 8 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 __global__ void k_zero_f64(double *p, int64_t n) {

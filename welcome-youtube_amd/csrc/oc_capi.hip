@@ -998,7 +998,7 @@ struct Adj {
     StateSnap end;
     double2 *X[2] = {nullptr, nullptr}, *V[2] = {nullptr, nullptr}, *F[2] = {nullptr, nullptr};
     RepStep *rsa = nullptr;  // step parameters of a segment's reverse steps
-    double *thb = nullptr, *gw = nullptr, *grad = nullptr, *tpart = nullptr;
+    double *thp = nullptr, *gw = nullptr, *grad = nullptr;
     unsigned long long *rpart = nullptr, *racc = nullptr;
     int64_t *base = nullptr;
     std::vector<void *> mem;
@@ -1158,10 +1158,9 @@ static int adj_setup(oc_engine *h, int K, int nseg, int nbins) {
         rc = rc ? rc : aalloc(a, &a->F[k], N);
     }
     rc = rc ? rc : aalloc(a, &a->rsa, (size_t)K * h->nrep);
-    rc = rc ? rc : aalloc(a, &a->thb, N * 8);
+    rc = rc ? rc : aalloc(a, &a->thp, h->bblk.size() * (size_t)kRdfSub * 8);
     rc = rc ? rc : aalloc(a, &a->gw, (size_t)h->nrep * nbins);
     rc = rc ? rc : aalloc(a, &a->grad, (size_t)h->nrep * 8);
-    rc = rc ? rc : aalloc(a, &a->tpart, (size_t)h->nrep * kThParts * 8);
     rc = rc ? rc : aalloc(a, &a->rpart, h->bblk.size() * (size_t)K * nbins);
     rc = rc ? rc : aalloc(a, &a->racc, (size_t)h->nrep * nbins);
     rc = rc ? rc : aalloc(a, &a->base, h->nrep);
@@ -1306,7 +1305,7 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
     const RdfArgs RA = a->RA;
     const int nbk = (int)h->bblk.size(), ncol = (int)h->cols.size();
     const size_t N = h->Ntot;
-    zero_f64(h, a->thb, N * 8);
+    zero_f64(h, a->thp, h->bblk.size() * (size_t)kRdfSub * 8);
     int ab = 0;  // adjoint buffer parity
     const EngBufs eb = save_bufs(h);
     for (int seg = nseg - 1; seg >= 0; seg--) {
@@ -1336,7 +1335,7 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
         for (int m = len; m >= 1; m--) {
             const StateSnap &t = a->tape[m];
             const AdjStep A = {a->rsa + (size_t)(m - 1) * h->nrep, t.cs, t.src, t.lnk, t.prv, t.xy, t.rg, t.vt, t.gt,
-                               t.mp, a->F[ab], a->X[ab], a->V[ab], a->X[ab ^ 1], a->V[ab ^ 1], a->F[ab ^ 1], a->thb};
+                               t.mp, a->F[ab], a->X[ab], a->V[ab], a->X[ab ^ 1], a->V[ab ^ 1], a->F[ab ^ 1], a->thp};
             k_adj_rstep<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, A);
             ab ^= 1;
         }
@@ -1346,11 +1345,10 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
         const StateSnap &t = a->tape[0];
         k_step_params_seg<<<dim3(h->nrep, 1), 64, 0, h->stream>>>(h->d_ri, a->rsa, 0, a->base);
         const AdjStep A = {a->rsa, t.cs, nullptr, t.lnk, nullptr, t.xy, nullptr, t.vt, t.gt, t.mp, a->F[ab], a->X[ab],
-                           a->V[ab], nullptr, nullptr, nullptr, a->thb};
+                           a->V[ab], nullptr, nullptr, nullptr, a->thp};
         k_adj_rstep<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, A);
     }
-    k_theta_part<<<dim3(h->nrep, kThParts), 256, 0, h->stream>>>(h->d_ri, a->thb, a->tpart);
-    k_theta_final<<<h->nrep, 64, 0, h->stream>>>(a->tpart, a->grad);
+    k_theta_reduce<<<h->nrep, 256, 0, h->stream>>>(h->d_ri, a->thp, a->grad);
     // the engine continues from the window end
     if (int rc = snap_copy(h, a->end, false)) return rc;
     launch_build_prv(h);
