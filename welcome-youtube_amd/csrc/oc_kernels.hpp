@@ -1340,8 +1340,8 @@ __device__ __forceinline__ Win window_of(const RepInfo &R, const int32_t *cs, do
 
 enum : uint8_t { CF_BREAK = 1, CF_CONV = 2 };
 constexpr int kChemThreads = 512;
-constexpr int kChemGroup = 8;  // lanes per search
-constexpr int kChemK = 3;      // candidates per lane per chunk
+constexpr int kChemGroup = 4;  // lanes per search
+constexpr int kChemK = 2;      // candidates per lane per chunk
 
 // Decisions from the state at the start of the check (own definition): break
 // of chemistry bonds, P -> T conversion next to a lipid tail, and the
