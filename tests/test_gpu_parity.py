@@ -70,6 +70,24 @@ def test_pair_forces_vs_oracle(oracle, eng, rng, L, nsub, lists):
     assert abs(out["fy"].sum()) < 1e-9 * np.abs(out["fy"]).sum() + 1e-9
 
 
+@pytest.mark.parametrize("lists", [False, True])
+def test_pair_forces_dense_clump(oracle, eng, rng, lists):
+    """A clump of 900 beads in a 3 x 2 area: the pair tiles around it stage
+    more candidates than their LDS buffer holds and run from global memory."""
+    say(f"case: dense clump lists={lists}")
+    p = mkparams(30.0, f_lip=0.1, seed=12)
+    s = soup_state(oracle, p, rng)
+    w = s["mpos"] < 0
+    idx = rng.choice(np.flatnonzero(w), size=900, replace=False)
+    s["x"][idx] = 10.0 + 3.0 * rng.random(900)
+    s["y"][idx] = 10.0 + 2.0 * rng.random(900)
+    s["vtx"] = s["vtx"].astype(np.float32).astype(np.float64)
+    s["vty"] = s["vty"].astype(np.float32).astype(np.float64)
+    out = device_forces(eng, p, s, 4321, bonded=False, lists=lists)
+    ref = oracle.pair_forces_gid(p.to_c(), s, 4321, theta32=1, sabs=True)
+    check_forces(out["fx"], out["fy"], ref["fx"], ref["fy"], ref["sabs"], "dense clump")
+
+
 def test_bonded_forces_vs_oracle(oracle, eng, rng):
     say("case: bonded forces L=16")
     p = mkparams(16.0, f_lip=0.3, k_angle=10.0, seed=5)
