@@ -277,24 +277,92 @@ def bench_adjoint(a, pm, eng):
     print(json.dumps(out), flush=True)
 
 
+def bench_fit(a, pm, eng):
+    """Batched parameter fit: R replicas of N beads (config-4 physics, distinct
+    seeds) run adjoint windows together; the replica-mean gradient of the RDF
+    loss drives Adam on the six a_ij toward the RDF of a seed run at a_TT = 20,
+    a_WT = 90.  value = replica particle-steps per second through the window
+    gradient (forward window + adjoint sweep) of one Adam step."""
+    import numpy as np
+    abi = importlib.import_module("welcome-youtube_amd.abi")
+    fit = importlib.import_module("welcome-youtube_amd.fit")
+    spec = abi.rdf_spec(2, 2, 64, 3.0)
+    R, N, nwin = a.replicas, a.N, a.window
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    pt = adjoint_params(pm, N=N, seed=2, a_T_T=20, a_W_T=90)
+    with eng.Engine(pt) as et:
+        et.start()
+        et.step(a.equil)
+        target = et.window_forward(nwin, spec)[0]
+    log(f"target RDF ready (N={N}, a_TT=20, a_WT=90)")
+    ps = [adjoint_params(pm, N=N, seed=1000 + r) for r in range(R)]
+    e = eng.Engine(ps)
+    e.start()
+    e.step(a.equil)
+    e.step_count()
+    t0 = time.perf_counter()
+    e.step(nwin)
+    e.step_count()
+    t_plain = time.perf_counter() - t0
+    log(f"{R} x {N} replicas equilibrated; {nwin} plain steps {1e3 * t_plain:.1f} ms")
+
+    def cb(rec):
+        th = rec["theta"]
+        log(f"adam {rec['step']:3d}  loss {rec['loss']:.6f}  a_WT {th[2]:.3f}  a_TT {th[5]:.3f}  "
+            f"window {1e3 * rec['window_s']:.1f} ms")
+
+    t0 = time.perf_counter()
+    hist = fit.fit_rdf(e, spec, target, steps=a.steps, lr=a.lr, window=nwin, callback=cb)
+    total = time.perf_counter() - t0
+    e.close()
+    tw = float(np.median([h["window_s"] for h in hist]))
+    names = eng.Engine.THETA_NAMES
+    out = {
+        "metric": "batched parameter fit: replica particle-steps/s through the window gradient",
+        "value": R * N * nwin / tw, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.equil,
+        "ms_per_step": 1e3 * total / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"batched parameter fit: {R} replicas x {N:,} beads (rho=3, 30% H-T-T lipids, "
+                               f"k_angle=0, chemistry off, distinct seeds), {nwin}-step adjoint windows, checkpoint "
+                               f"every 20, tail-tail RDF 64 bins on [0,3), replica-mean gradient, {a.steps} Adam "
+                               f"steps (lr {a.lr}) on the six a_ij toward the RDF of a seed run at a_TT=20, a_WT=90",
+                   "replicas": R, "N": N},
+        "gradient_cost_vs_plain": tw / t_plain,
+        "loss_first": hist[0]["loss"], "loss_last": hist[-1]["loss"],
+        "loss_min": min(h["loss"] for h in hist),
+        "theta_first": {n: float(v) for n, v in zip(names, hist[0]["theta"])},
+        "theta_last": {n: float(v) for n, v in zip(names, hist[-1]["theta"])},
+        "loss_trace": [round(h["loss"], 6) for h in hist],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 500; fit: 50 Adam steps)")
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--N", type=int, default=1048576)
+    ap.add_argument("--N", type=int, default=None, help="beads (default 1,048,576; fit: 262,144 per replica)")
     ap.add_argument("--nsub", type=int, default=4)
     ap.add_argument("--pair-block", type=int, default=0)
     ap.add_argument("--tile-beads", type=int, default=0)
     ap.add_argument("--pair-lists", action="store_true", help="two-phase pair kernel variant")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--workload", choices=("plain", "ensemble", "adjoint", "copyprobe"), default="plain")
+    ap.add_argument("--workload", choices=("plain", "ensemble", "adjoint", "fit", "copyprobe"), default="plain")
     ap.add_argument("--window", type=int, default=200)
     ap.add_argument("--equil", type=int, default=1000)
-    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--replicas", type=int, default=None, help="default 256 (ensemble), 32 (fit)")
+    ap.add_argument("--lr", type=float, default=1.0, help="Adam step size of the fit workload")
     ap.add_argument("--tune", action="store_true", help="autotune nsub / pair launch shape first")
     a = ap.parse_args()
+    fitw = a.workload == "fit"
+    a.steps = a.steps if a.steps is not None else (50 if fitw else 500)
+    a.N = a.N if a.N is not None else (262144 if fitw else 1048576)
+    a.replicas = a.replicas if a.replicas is not None else (32 if fitw else 256)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the step on a single MI355X "
               "(multi-GPU domain decomposition is a design note, not implemented).", file=sys.stderr)
@@ -308,6 +376,8 @@ def main():
         return bench_ensemble(a, pm, eng)
     if a.workload == "adjoint":
         return bench_adjoint(a, pm, eng)
+    if a.workload == "fit":
+        return bench_fit(a, pm, eng)
     p = config2_params(pm, a.N, a.nsub)
     N = p.N
     e = eng.Engine(p, pair_block=a.pair_block, tile_beads=a.tile_beads, pair_lists=a.pair_lists)

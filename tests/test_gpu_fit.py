@@ -1,0 +1,82 @@
+"""Batched parameter fit on the GPU: the gradient that drives Adam is the
+replica mean of the per-replica window gradients, the first update is Adam's
+(lr * g / (|g| + eps)), and the second window continues from the first
+window's last state at the updated parameters."""
+import importlib
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import mkparams
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def states(oracle, specs):
+    out = []
+    for L, seed in specs:
+        p = mkparams(L, nsub=4, f_lip=0.3, seed=seed)
+        cp = p.to_c()
+        o = oracle.System(cp, p.N)
+        o.start(oracle.soup(cp, p.N))
+        o.step(60)
+        out.append((p, o.get_state(), 60))
+    return out
+
+
+def test_fit_gradient_is_replica_mean_and_adam_update(oracle, eng):
+    abi = importlib.import_module("welcome-youtube_amd.abi")
+    fit = importlib.import_module("welcome-youtube_amd.fit")
+    spec = abi.rdf_spec(2, 2, 16, 3.0)
+    sys_ = states(oracle, ((13.0, 1), (17.5, 2), (15.2, 3)))
+    ps = [s[0] for s in sys_]
+    target = np.full(16, 0.9)
+    lr, win, ck = 0.5, 8, 4
+    say("fit: 3 replicas, 2 Adam steps, window", win)
+    ea = eng.Engine(ps)
+    for r, (p, st, s0) in enumerate(sys_):
+        ea.set_state(st, step=s0, r=r)
+    hist = fit.fit_rdf(ea, spec, target, steps=2, lr=lr, window=win, ckpt_every=ck)
+    ea.close()
+
+    say("per-replica window gradients at the same states")
+    eb = eng.Engine(ps)
+    for r, (p, st, s0) in enumerate(sys_):
+        eb.set_state(st, step=s0, r=r)
+    free = list(fit.A_IJ)
+    for k in range(2):
+        th = hist[k]["theta"]
+        for r in range(3):
+            eb.set_theta(th, r)
+        lb, gb, _ = eb.window_grad(win, spec, target, ckpt_every=ck)
+        gm = gb.mean(0)
+        say(f"step {k}: loss fit {hist[k]['loss']:.10f} mean {lb.mean():.10f}")
+        assert abs(hist[k]["loss"] - lb.mean()) <= 1e-12 * lb.mean()
+        d = np.abs(hist[k]["grad"][free] - gm[free]).max() / np.abs(gm[free]).max()
+        say(f"step {k}: max |g_fit - mean g_r| / max|g| = {d:.2e}")
+        assert d < 1e-6
+        assert hist[k]["grad"][6] == 0.0
+    eb.close()
+
+    g0 = hist[0]["grad"][free]
+    want = hist[0]["theta"][free] - lr * g0 / (np.abs(g0) + 1e-8)
+    assert np.allclose(hist[1]["theta"][free], want, rtol=0, atol=1e-12)
+    assert hist[1]["theta"][6] == hist[0]["theta"][6]
+
+
+def test_fit_rejects_bad_arguments(eng):
+    abi = importlib.import_module("welcome-youtube_amd.abi")
+    fit = importlib.import_module("welcome-youtube_amd.fit")
+    spec = abi.rdf_spec(2, 2, 16, 3.0)
+    e = eng.Engine(mkparams(12.0, f_lip=0.3))
+    with pytest.raises(ValueError):
+        fit.fit_rdf(e, spec, np.ones(15), steps=1)
+    with pytest.raises(ValueError):
+        fit.fit_rdf(e, spec, np.ones(16), steps=1, free=(0, 7))
+    e.close()

@@ -5,6 +5,7 @@ inputs; pair forces within 1e-5 of the oracle relative to the per-bead force
 scale sum_j |f_ij| (and to max |F|), with the same per-pair noise (the oracle
 draws theta from the same 32-bit cut of the same hash).
 """
+import importlib
 import sys
 
 import numpy as np
@@ -365,3 +366,44 @@ def test_momentum_and_temperature_config1(eng, f_lip):
     assert abs(T - 1.0) < 0.01
     assert pmax < 1e-6
     e.close()
+
+
+def test_tail_tail_rdf_vs_oracle(oracle, eng):
+    """Stochastic run: the tail-tail RDF (64 bins on [0, 3), triweight width
+    one bin) averaged over 400 steps after 1,500 steps from the same soup,
+    device against the FP64 oracle.  The two trajectories have decorrelated by
+    then (FP32 forces), so the difference is sampling noise; its scale is set
+    by an oracle run from another seed.  Stated bound: relative L2 error
+    ||g_dev - g_oracle|| / ||g_oracle|| < 0.03, and below twice the distance
+    between the two oracle runs."""
+    abi = importlib.import_module("welcome-youtube_amd.abi")
+    p = mkparams(40.0, f_lip=0.3, seed=11)
+    cp = p.to_c()
+    spec, ospec = abi.rdf_spec(2, 2, 64, 3.0), oracle.rdf_spec(2, 2, 64, 3.0)
+    nwin, eq = 400, 1500
+
+    def oracle_run(pp):
+        o = oracle.System(pp.to_c(), pp.N)
+        o.start(oracle.soup(pp.to_c(), pp.N))
+        o.step(eq)
+        g = np.zeros(64)
+        for _ in range(nwin):
+            o.step(1)
+            g += o.rdf(ospec)
+        return g / nwin
+
+    s0 = oracle.soup(cp, p.N)
+    e = eng.Engine(p)
+    e.start([s0])
+    e.step(eq)
+    gd = e.window_forward(nwin, spec)[0]
+    e.close()
+    say("device done; oracle runs")
+    go = oracle_run(p)
+    go2 = oracle_run(mkparams(40.0, f_lip=0.3, seed=12))
+    err = np.linalg.norm(gd - go) / np.linalg.norm(go)
+    noise = np.linalg.norm(go2 - go) / np.linalg.norm(go)
+    say(f"tail-tail RDF: rel L2 device-oracle {err:.4f}, oracle seed 11 vs 12 {noise:.4f}, "
+        f"peak g {go.max():.3f} at r = {(np.argmax(go) + 0.5) * 3 / 64:.3f}")
+    assert err < 0.03
+    assert err < 2.0 * noise

@@ -227,3 +227,44 @@ def test_window_adjoint_vs_central_differences(oracle, nsteps):
         e = np.eye(7)[q] * eps
         fd = (run(th0 + e, tgt)[0] - run(th0 - e, tgt)[0]) / (2 * eps)
         assert abs(fd - g[q]) < 2e-5 * np.abs(g).max(), (q, fd, g[q])
+
+
+def test_rdf_of_ideal_gas_is_one(oracle):
+    """The smooth RDF's normalisation, pinned by a closed form: for uniformly
+    random positions E[g_b] = 1 - 1/n (n beads of the type, pairs counted once)
+    in every bin whose kernel support stays at r > 0, since a symmetric kernel
+    maps the linear pair density 2 pi r rho onto itself.  A factor 2 in the
+    pair count, a wrong shell area or kernel constant fails this."""
+    p = mkparams(30.0, f_lip=0.0, seed=1)
+    N = p.N
+    rng = np.random.default_rng(7)
+    spec = oracle.rdf_spec(2, 2, 24, 3.0)
+    acc = np.zeros(24)
+    n_t = N // 2
+    samples = 40
+    for k in range(samples):
+        s = {"x": rng.uniform(0, p.L, N), "y": rng.uniform(0, p.L, N), "vx": np.zeros(N), "vy": np.zeros(N),
+             "type": np.where(np.arange(N) < n_t, 2, 0).astype(np.uint8), "mpos": np.full(N, -1, np.int8),
+             "nxt": np.full(N, -1, np.int32), "bond": np.full(N, -1, np.int32)}
+        o = oracle.System(p.to_c(), N)
+        o.set_state(s, 0)
+        acc += o.rdf(spec)
+    g = acc / samples
+    want = 1.0 - 1.0 / n_t
+    # bins 1..23: kernel support [(b - 0.5) dr, (b + 1.5) dr] inside r > 0;
+    # shot noise of bin b ~ 1/sqrt(pairs in its shell over all samples)
+    dr = 3.0 / 24
+    for b in range(1, 24):
+        pairs = samples * 0.5 * n_t * (n_t / p.L ** 2) * np.pi * (((b + 1) * dr) ** 2 - (b * dr) ** 2)
+        assert abs(g[b] - want) < 5.0 / np.sqrt(pairs), (b, g[b])
+    # bin 0: the part of the kernel below r = 0 (where the linear density would
+    # be negative) is cut off, so E[g_0] = want * int_0 K(r - c) r dr / c
+    c, h = 0.5 * dr, dr
+    r = np.linspace(0.0, c + h, 200001)
+    t = (r - c) / h
+    K = 35.0 / (32.0 * h) * (1 - t * t) ** 3
+    f = K * r
+    want0 = want * float(np.sum((f[1:] + f[:-1]) * 0.5 * np.diff(r))) / c
+    pairs0 = samples * 0.5 * n_t * (n_t / p.L ** 2) * np.pi * dr ** 2
+    assert want0 > want
+    assert abs(g[0] - want0) < 5.0 / np.sqrt(pairs0), (g[0], want0)
