@@ -349,7 +349,9 @@ def main():
     ap.add_argument("--nsub", type=int, default=4)
     ap.add_argument("--pair-block", type=int, default=0)
     ap.add_argument("--tile-beads", type=int, default=0)
-    ap.add_argument("--pair-lists", action="store_true", help="two-phase pair kernel variant")
+    ap.add_argument("--pair-variant", type=int, default=None,
+                    help="pair kernel: 0 single pass, 1 per-lane two phases, 2 cross-lane pair queue "
+                         "(default: the library's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--workload", choices=("plain", "ensemble", "adjoint", "fit", "copyprobe"), default="plain")
@@ -380,7 +382,7 @@ def main():
         return bench_fit(a, pm, eng)
     p = config2_params(pm, a.N, a.nsub)
     N = p.N
-    e = eng.Engine(p, pair_block=a.pair_block, tile_beads=a.tile_beads, pair_lists=a.pair_lists)
+    e = eng.Engine(p, pair_block=a.pair_block, tile_beads=a.tile_beads, pair_variant=a.pair_variant)
     e.start()
     e.step(a.warmup, graph=not a.no_graph)
     e.step_count()  # device synchronised

@@ -30,9 +30,9 @@ def soup_state(oracle, p, rng, vt_noise=0.3):
     return s
 
 
-def device_forces(eng, p, s, step, bonded, lists=False):
+def device_forces(eng, p, s, step, bonded, variant=0):
     """forces of state s on the device: vh := predicted velocity, f := 0"""
-    e = eng.Engine(p, pair_lists=lists)
+    e = eng.Engine(p, pair_variant=variant)
     st = {k: s[k] for k in ("x", "y", "type", "mpos", "nxt")}
     st["vx"], st["vy"] = s["vtx"], s["vty"]
     st["bond"] = s.get("bond", np.full(p.N, -1, np.int32))
@@ -52,18 +52,19 @@ def check_forces(fx, fy, ofx, ofy, sabs, what):
     assert rel < FORCE_TOL and glob < FORCE_TOL
 
 
-@pytest.mark.parametrize("lists", [False, True])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("L,nsub", [(9.0, 1), (12.5, 4), (23.7, 2), (31.3, 8), (47.1, 4)])
-def test_pair_forces_vs_oracle(oracle, eng, rng, L, nsub, lists):
-    """both pair kernel variants (single pass; cutoff test then the marked
-    pairs); L = 47.1 has interior tiles, which take the flat path"""
-    say(f"case: pair forces L={L} nsub={nsub} lists={lists}")
+def test_pair_forces_vs_oracle(oracle, eng, rng, L, nsub, variant):
+    """every pair kernel variant (single pass; per-lane cutoff test then the
+    marked pairs; cross-lane pair queue); L = 47.1 has interior tiles, which
+    take the flat path"""
+    say(f"case: pair forces L={L} nsub={nsub} variant={variant}")
     p = mkparams(L, nsub=nsub, f_lip=0.15, c_N=0.1, seed=99)
     s = soup_state(oracle, p, rng)
     # the device keeps the predicted velocity in FP32: give the oracle the same values
     s["vtx"] = s["vtx"].astype(np.float32).astype(np.float64)
     s["vty"] = s["vty"].astype(np.float32).astype(np.float64)
-    out = device_forces(eng, p, s, 12345, bonded=False, lists=lists)
+    out = device_forces(eng, p, s, 12345, bonded=False, variant=variant)
     ref = oracle.pair_forces_gid(p.to_c(), s, 12345, theta32=1, sabs=True)
     check_forces(out["fx"], out["fy"], ref["fx"], ref["fy"], ref["sabs"], f"pair L={L} nsub={nsub}")
     # full shell, pairwise antisymmetric: total force vanishes to rounding
@@ -71,11 +72,11 @@ def test_pair_forces_vs_oracle(oracle, eng, rng, L, nsub, lists):
     assert abs(out["fy"].sum()) < 1e-9 * np.abs(out["fy"]).sum() + 1e-9
 
 
-@pytest.mark.parametrize("lists", [False, True])
-def test_pair_forces_dense_clump(oracle, eng, rng, lists):
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_pair_forces_dense_clump(oracle, eng, rng, variant):
     """A clump of 900 beads in a 3 x 2 area: the pair tiles around it stage
     more candidates than their LDS buffer holds and run from global memory."""
-    say(f"case: dense clump lists={lists}")
+    say(f"case: dense clump variant={variant}")
     p = mkparams(30.0, f_lip=0.1, seed=12)
     s = soup_state(oracle, p, rng)
     w = s["mpos"] < 0
@@ -84,9 +85,38 @@ def test_pair_forces_dense_clump(oracle, eng, rng, lists):
     s["y"][idx] = 10.0 + 2.0 * rng.random(900)
     s["vtx"] = s["vtx"].astype(np.float32).astype(np.float64)
     s["vty"] = s["vty"].astype(np.float32).astype(np.float64)
-    out = device_forces(eng, p, s, 4321, bonded=False, lists=lists)
+    out = device_forces(eng, p, s, 4321, bonded=False, variant=variant)
     ref = oracle.pair_forces_gid(p.to_c(), s, 4321, theta32=1, sabs=True)
     check_forces(out["fx"], out["fy"], ref["fx"], ref["fy"], ref["sabs"], "dense clump")
+
+
+def test_pair_queue_bit_identical_to_single_pass(oracle, eng, rng):
+    """The cross-lane pair queue adds the same FP32 pair terms in the same
+    order as the single pass: forces (with bonded terms) and a 30-step
+    trajectory with chemistry are bitwise equal.  The second system has a
+    clump of 300 beads, whose wavefronts exceed 64 candidates per lane and fall
+    back to the single pass."""
+    for L, clump in ((61.0, 0), (40.0, 300)):
+        say(f"case: queue vs single pass L={L} clump={clump}")
+        p = mkparams(L, nsub=4, f_lip=0.3, c_N=0.2, seed=31, pchem=1, p_conv=0.2, p_form=0.2, p_break=0.05)
+        s = soup_state(oracle, p, rng)
+        if clump:
+            idx = rng.choice(np.flatnonzero(s["mpos"] < 0), size=clump, replace=False)
+            s["x"][idx] = 10.0 + 1.5 * rng.random(clump)
+            s["y"][idx] = 10.0 + 1.5 * rng.random(clump)
+        a = device_forces(eng, p, s, 777, bonded=True, variant=0)
+        b = device_forces(eng, p, s, 777, bonded=True, variant=2)
+        assert np.array_equal(a["fx"], b["fx"]) and np.array_equal(a["fy"], b["fy"])
+        st = {k: s[k] for k in ("x", "y", "vx", "vy", "type", "mpos", "nxt")}
+        outs = []
+        for v in (0, 2):
+            e = eng.Engine(p, pair_variant=v)
+            e.start([st])
+            e.step(30)
+            outs.append(e.get_state())
+            e.close()
+        for k in outs[0]:
+            assert np.array_equal(outs[0][k], outs[1][k]), k
 
 
 def test_bonded_forces_vs_oracle(oracle, eng, rng):

@@ -36,7 +36,7 @@ struct oc_engine {
     std::vector<Tile> tiles;   // pair-force tiles
     std::vector<Tile> stiles;  // sort tiles (rows of new cells, one thread each)
     int pair_block = 256, tile_beads = 0;
-    bool pair_lists = false;
+    int pair_variant = 0;  // k_pair: 0 single pass, 1 per-lane two phases, 2 cross-lane pair queue
     int per_xcd = 0;
     // device
     RepInfo *d_ri = nullptr;
@@ -215,9 +215,9 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false) {
     A.moved = h->d_moved;
     if (!energy) h->keys_fresh = true;
     const dim3 grid(8 * h->per_xcd);
-    // CAP (staged candidates) is 3.5x the block; `lists` picks the two-phase
-    // variant (cutoff test, then the marked pairs), or the single pass
-    const bool L2 = h->pair_lists;
+    // CAP (staged candidates) is 3.5x the block; the variant picks the single
+    // pass, the per-lane two phases or the cross-lane pair queue
+    const int V = h->pair_variant;
 #define OC_PAIR(B, C, CL)                                                                        \
     do {                                                                                         \
         if (energy) k_pair<B, C, CL, true, false><<<grid, B, 0, h->stream>>>(A);                  \
@@ -226,13 +226,13 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false) {
     } while (0)
     switch (h->pair_block) {
     case 64:
-        if (L2) OC_PAIR(64, 224, 1); else OC_PAIR(64, 224, 0);
+        if (V == 2) OC_PAIR(64, 224, 2); else if (V == 1) OC_PAIR(64, 224, 1); else OC_PAIR(64, 224, 0);
         break;
     case 128:
-        if (L2) OC_PAIR(128, 448, 1); else OC_PAIR(128, 448, 0);
+        if (V == 2) OC_PAIR(128, 448, 2); else if (V == 1) OC_PAIR(128, 448, 1); else OC_PAIR(128, 448, 0);
         break;
     default:
-        if (L2) OC_PAIR(256, 896, 1); else OC_PAIR(256, 896, 0);
+        if (V == 2) OC_PAIR(256, 896, 2); else if (V == 1) OC_PAIR(256, 896, 1); else OC_PAIR(256, 896, 0);
         break;
     }
 #undef OC_PAIR
@@ -607,7 +607,8 @@ OC_API void *oc_stream(oc_engine *h) { return h ? (void *)h->stream : nullptr; }
 
 OC_API int oc_set_pair_variant(oc_engine *h, int32_t lists) {
     if (!h) return fail(OC_EINVAL, "null engine");
-    h->pair_lists = lists != 0;
+    if (lists < 0 || lists > 2) return fail(OC_EINVAL, "pair variant must be 0, 1 or 2");
+    h->pair_variant = lists;
     drop_graphs(h);
     return 0;
 }

@@ -660,9 +660,30 @@ __global__ __launch_bounds__(kBeadBlock) void k_build_prv(const RepInfo *__restr
 
 // --------------------------------------------------------------------------- pair forces
 
+// entries of a wavefront's pair queue per round (8 bytes of LDS each)
+constexpr int kPairQueue = 256;
+
 constexpr float kTheta32 = (float)(3.4641016151377545870548926830117 / 4294967296.0);  // sqrt(12) / 2^32
 
-// sdtk = sigma / sqrt(dt) * sqrt(12) / 2^32 (the draw scale folded in)
+// Force factor fm of an interacting pair (0 < r2 < 1): F_ij = fm * (dx, dy),
+// fm = w (a_ij - gamma w r.v + sigma/sqrt(dt) theta_ij) / r, w = 1 - r.
+// sdtk = sigma / sqrt(dt) * sqrt(12) / 2^32 (the draw scale folded in).
+__device__ __forceinline__ float pair_core(float dx, float dy, float r2, float vxi, float vyi, uint32_t gi,
+                                           const float *arow, float2 vj, uint32_t gtj, float gam, float sdtk,
+                                           uint64_t sseed, float &w) {
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * ir;
+    w = 1.0f - r;
+    const float dvx = vxi - vj.x, dvy = vyi - vj.y;
+    const float rv = __builtin_fmaf(dx, dvx, dy * dvy) * ir;
+    const uint32_t gj = gtj & kGidMask;
+    const uint64_t h = pair_hash(sseed, gi, gj);
+    // theta on [-sqrt3, sqrt3) from the top 32 bits of the hash
+    const float th = (float)(int32_t)(uint32_t)(h >> 32);
+    const float a = arow[gtj >> kGidBits];
+    return w * __builtin_fmaf(sdtk, th, __builtin_fmaf(-gam * w, rv, a)) * ir;
+}
+
 __device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float vyi, uint32_t gi, const float *arow,
                                          double2 pj, float2 vj, uint32_t gtj, double sxs, double sys, float gam,
                                          float sdtk, uint64_t sseed, double &ax, double &ay, float &e, bool energy) {
@@ -672,20 +693,11 @@ __device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float 
     const float dx = (float)((xi - pj.x) - sxs), dy = (float)((yi - pj.y) - sys);
     const float r2 = __builtin_fmaf(dx, dx, dy * dy);
     if (r2 < 1.0f && r2 > 1e-24f) {
-        const float ir = __builtin_amdgcn_rsqf(r2);
-        const float r = r2 * ir;
-        const float w = 1.0f - r;
-        const float dvx = vxi - vj.x, dvy = vyi - vj.y;
-        const float rv = __builtin_fmaf(dx, dvx, dy * dvy) * ir;
-        const uint32_t gj = gtj & kGidMask;
-        const uint64_t h = pair_hash(sseed, gi, gj);
-        // theta on [-sqrt3, sqrt3) from the top 32 bits of the hash
-        const float th = (float)(int32_t)(uint32_t)(h >> 32);
-        const float a = arow[gtj >> kGidBits];
-        const float fm = w * __builtin_fmaf(sdtk, th, __builtin_fmaf(-gam * w, rv, a)) * ir;
+        float w;
+        const float fm = pair_core(dx, dy, r2, vxi, vyi, gi, arow, vj, gtj, gam, sdtk, sseed, w);
         ax += (double)(fm * dx);
         ay += (double)(fm * dy);
-        if (energy) e += 0.25f * a * w * w;
+        if (energy) e += 0.25f * arow[gtj >> kGidBits] * w * w;
     }
 }
 
@@ -865,6 +877,7 @@ __host__ __device__ constexpr int pair_max_rows(int block) { return 2 * block; }
 
 template <int BLOCK, int CAP, int CAPL, bool ENERGY, bool ANG>
 __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
+    static_assert(CAP <= 65536, "queue entries pack two staged indices in 16 bits each");
     constexpr int kMaxRows = pair_max_rows(BLOCK);
     __shared__ double2 s_xy[CAP];
     __shared__ float2 s_vt[CAP];
@@ -1043,14 +1056,23 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             }
         }
     };
+    // Cross-lane pair queue (CAPL == 2): per wavefront, kPairQueue entries of
+    // LDS, each first an (i, j) pair of staged indices, then its force.
+    __shared__ uint2 s_q[CAPL == 2 ? (BLOCK / 64) * kPairQueue : 1];
     auto run = [&](const double2 *XY, const float2 *VT, const uint32_t *GT, const bool stg) {
-        for (int ii = ibeg + threadIdx.x; ii < iend; ii += BLOCK) {
-            const int slot = slot0 + (ii - ibeg);
-            const int is = stg ? ii : slot;
+        const int lane = (int)threadIdx.x & 63;
+        // block-uniform bead loop: every lane of a wavefront takes part in the
+        // wavefront-wide steps, lanes past the tile's last bead (act false)
+        // with an empty window
+        for (int ib = ibeg; ib < iend; ib += BLOCK) {
+            const int ii = ib + (int)threadIdx.x;
+            const bool act = ii < iend;
+            const int slot = slot0 + ((act ? ii : ibeg) - ibeg);
+            const int is = stg ? (act ? ii : ibeg) : slot;
             BondPre B;
-            const bool bond = !ENERGY && A.bonded;
+            const bool bond = !ENERGY && A.bonded && act;
             if (bond) bond_links(slot, A.mp, A.lnk, A.prv, B);
-            const double2 vh = !ENERGY && A.key ? A.v[slot] : make_double2(0.0, 0.0);
+            const double2 vh = !ENERGY && A.key && act ? A.v[slot] : make_double2(0.0, 0.0);
             const double2 pi = XY[is];
             const float2 vi = VT[is];
             const uint32_t gti = GT[is];
@@ -1071,13 +1093,19 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             // the lane runs through its three column ranges as one sequence
             // (stepping past the end of one range lands on the next by two
             // selects), so the wavefront iterates ~max over lanes of the whole
-            // window rather than the sum of per-column maxima.  With CAPL > 0
-            // it runs in two phases: the cutoff test over the sequence marks
-            // the interacting candidates among the first 64 in a 64-bit
-            // register mask, then the force and noise arithmetic runs over
-            // the marked ones only, so the wavefront executes it ~max over
-            // lanes of the pair count instead of ~max of the candidate count.
-            const bool img = sxc[0] != 0.0 || sxc[2] != 0.0 || klo < 0 || khi >= ncy;
+            // window rather than the sum of per-column maxima.
+            //   CAPL == 1: two phases per lane: the cutoff test over the
+            // sequence marks the interacting candidates among the first 64 in
+            // a 64-bit register mask, then the force and noise arithmetic runs
+            // over the marked ones only (the wavefront still executes it ~max
+            // over lanes of the pair count).
+            //   CAPL == 2: the marked pairs of all 64 lanes go into one queue
+            // in lane order, the whole wavefront evaluates it 64 pairs at a
+            // time (~mean rather than ~max of the pair count), and each lane
+            // sums its own pairs' forces back in sequence order -- the same
+            // FP32 terms added in the same order as the single pass, so the
+            // forces are bit-identical to it.
+            const bool img = act && (sxc[0] != 0.0 || sxc[2] != 0.0 || klo < 0 || khi >= ncy);
             if (stg && !__any(img)) {
                 const int st0 = s_row[0][klo - kb0], en0 = s_row[0][khi + 1 - kb0];
                 const int st1 = s_row[1][klo - kb0], en1 = s_row[1][khi + 1 - kb0];
@@ -1086,9 +1114,69 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                 j = j == en1 ? st2 : j;
                 // the iteration count bounds the walk even for inconsistent
                 // cell starts (flagged)
-                const int nj = max(en0 - st0, 0) + max(en1 - st1, 0) + max(en2 - st2, 0);
-                if (st1 < en0 || st2 < en1) atomicOr(A.err, ERR_TILE);
-                if (CAPL == 0) {
+                const int nj = act ? max(en0 - st0, 0) + max(en1 - st1, 0) + max(en2 - st2, 0) : 0;
+                if (act && (st1 < en0 || st2 < en1)) atomicOr(A.err, ERR_TILE);
+                const int c0 = max(en0 - st0, 0), c01 = c0 + max(en1 - st1, 0);
+                if (CAPL == 2 && !ENERGY && !__any(nj > 64)) {
+                    // phase 1: bit t of mk marks an interacting candidate t
+                    uint64_t mk = 0;
+                    for (int it = 0; it < nj; it++) {
+                        const double2 pj = XY[j];
+                        const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
+                        const float r2 = __builtin_fmaf(dx, dx, dy * dy);  // the test of pair_one
+                        if (r2 < 1.0f && r2 > 1e-24f) mk |= 1ull << it;
+                        ++j;
+                        j = j == en0 ? st1 : j;
+                        j = j == en1 ? st2 : j;
+                    }
+                    // this lane's first entry in the wavefront's queue
+                    const int cnt = __popcll(mk);
+                    int incl = cnt;
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const int v = __shfl_up(incl, o, 64);
+                        if (lane >= o) incl += v;
+                    }
+                    const int off = incl - cnt, total = __shfl(incl, 63, 64);
+                    uint2 *qb = s_q + ((int)threadIdx.x >> 6) * kPairQueue;
+                    uint64_t rem = mk;
+                    int pos = off;
+                    // rounds of kPairQueue entries (wavefront-uniform)
+                    for (int w0 = 0; w0 < total; w0 += kPairQueue) {
+                        const int w1 = min(w0 + kPairQueue, total);
+                        // phase 2: this lane's pairs that fall in the round
+                        while (rem != 0 && pos < w1) {
+                            const int tk = __builtin_ctzll(rem);
+                            rem &= rem - 1;
+                            const int jk = tk < c0 ? st0 + tk : (tk < c01 ? st1 + (tk - c0) : st2 + (tk - c01));
+                            qb[pos - w0] = make_uint2((uint32_t)ii | ((uint32_t)jk << 16), 0u);
+                            ++pos;
+                        }
+                        __syncwarp();
+                        // phase 3: the round's pairs, one per lane
+                        for (int q = lane; q < w1 - w0; q += 64) {
+                            const uint32_t u = qb[q].x;
+                            const int i1 = (int)(u & 0xffffu), j1 = (int)(u >> 16);
+                            const double2 p1 = XY[i1], p2 = XY[j1];
+                            const float2 v1 = VT[i1], v2 = VT[j1];
+                            const uint32_t g1 = GT[i1], g2 = GT[j1];
+                            const float dx = (float)(p1.x - p2.x), dy = (float)(p1.y - p2.y);
+                            const float r2 = __builtin_fmaf(dx, dx, dy * dy);
+                            float w;
+                            const float fm = pair_core(dx, dy, r2, v1.x, v1.y, g1 & kGidMask,
+                                                       s_a + 8 * (g1 >> kGidBits), v2, g2, gam, sdtk, sseed, w);
+                            qb[q] = make_uint2(__float_as_uint(fm * dx), __float_as_uint(fm * dy));
+                        }
+                        __syncwarp();
+                        // phase 4: this lane's forces of the round, in sequence order
+                        const int a1 = min(off + cnt, w1);
+                        for (int p = max(off, w0); p < a1; p++) {
+                            const uint2 fq = qb[p - w0];
+                            ax += (double)__uint_as_float(fq.x);
+                            ay += (double)__uint_as_float(fq.y);
+                        }
+                        __syncwarp();
+                    }
+                } else if (CAPL != 1) {
                     for (int it = 0; it < nj; it++) {
                         pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], 0.0, 0.0, gam, sdtk, sseed,
                                  ax, ay, e, ENERGY);
@@ -1116,7 +1204,6 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                         j = j == en1 ? st2 : j;
                     }
                     // pass 2: the marked candidates in sequence order
-                    const int c0 = max(en0 - st0, 0), c01 = c0 + max(en1 - st1, 0);
                     while (mk) {
                         const int tk = __builtin_ctzll(mk);
                         mk &= mk - 1;
@@ -1125,31 +1212,31 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                                  ax, ay, e, ENERGY);
                     }
                 }
-                if (bond) bond_partners(A.xy, A.gt, B);
-                finish(slot, pi, gti, B, ax, ay, e, vh);
-                continue;
-            }
-            // general path: per column and periodic piece (image shifts), single pass
-            for (int c = 0; c < 3; c++) {
-                // window split at the periodic seams: rows < 0 (image -L, staged piece 0),
-                // 0..ncy-1 (piece 1), >= ncy (image +L, piece 2)
-                for (int q = 0; q < 3; q++) {
-                    int pa, pb;
-                    double sy;
-                    if (q == 0) { pa = klo; pb = min(khi, -1); sy = -L; }
-                    else if (q == 1) { pa = max(klo, 0); pb = min(khi, ncy - 1); sy = 0.0; }
-                    else { pa = max(klo, ncy); pb = khi; sy = L; }
-                    if (pa > pb) continue;
-                    const double sx = sxc[c];
-                    const int adj = stg ? 0 : s_piece[c][q][0] - s_piece[c][q][2];
-                    const int ja = s_row[c][pa - kb0] + adj, jb = s_row[c][pb + 1 - kb0] + adj;
-                    for (int j = ja; j < jb; j++)
-                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], sx, sy, gam, sdtk, sseed, ax,
-                                 ay, e, ENERGY);
+            } else if (act) {
+                // general path: per column and periodic piece (image shifts), single pass
+                for (int c = 0; c < 3; c++) {
+                    // window split at the periodic seams: rows < 0 (image -L, staged piece 0),
+                    // 0..ncy-1 (piece 1), >= ncy (image +L, piece 2)
+                    for (int q = 0; q < 3; q++) {
+                        int pa, pb;
+                        double sy;
+                        if (q == 0) { pa = klo; pb = min(khi, -1); sy = -L; }
+                        else if (q == 1) { pa = max(klo, 0); pb = min(khi, ncy - 1); sy = 0.0; }
+                        else { pa = max(klo, ncy); pb = khi; sy = L; }
+                        if (pa > pb) continue;
+                        const double sx = sxc[c];
+                        const int adj = stg ? 0 : s_piece[c][q][0] - s_piece[c][q][2];
+                        const int ja = s_row[c][pa - kb0] + adj, jb = s_row[c][pb + 1 - kb0] + adj;
+                        for (int jj = ja; jj < jb; jj++)
+                            pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jj], VT[jj], GT[jj], sx, sy, gam, sdtk,
+                                     sseed, ax, ay, e, ENERGY);
+                    }
                 }
             }
-            if (bond) bond_partners(A.xy, A.gt, B);
-            finish(slot, pi, gti, B, ax, ay, e, vh);
+            if (act) {
+                if (bond) bond_partners(A.xy, A.gt, B);
+                finish(slot, pi, gti, B, ax, ay, e, vh);
+            }
         }
     };
     if (staged) run(s_xy, s_vt, s_gt, true);

@@ -55,7 +55,7 @@ class Engine:
     """One engine handle over one or more replicas (a single system is an
     ensemble of one)."""
 
-    def __init__(self, params, N=None, pair_block=0, tile_beads=0, pair_lists=False):
+    def __init__(self, params, N=None, pair_block=0, tile_beads=0, pair_lists=False, pair_variant=None):
         plist = params if isinstance(params, (list, tuple)) else [params]
         self.params = [p if isinstance(p, Params) else p for p in plist]
         for p in self.params:
@@ -75,8 +75,10 @@ class Engine:
         self.h = h
         if pair_block or tile_beads:
             self.set_tuning(pair_block, tile_beads)
-        if pair_lists:
-            self._chk(self.lib.oc_set_pair_variant(self.h, 1))
+        if pair_variant is None:
+            pair_variant = 1 if pair_lists else None
+        if pair_variant is not None:
+            self._chk(self.lib.oc_set_pair_variant(self.h, int(pair_variant)))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
