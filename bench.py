@@ -315,7 +315,7 @@ def bench_fit(a, pm, eng):
             f"window {1e3 * rec['window_s']:.1f} ms")
 
     t0 = time.perf_counter()
-    hist = fit.fit_rdf(e, spec, target, steps=a.steps, lr=a.lr, window=nwin, callback=cb)
+    hist = fit.fit_rdf(e, spec, target, steps=a.steps, lr=a.lr, window=nwin, rewind=not a.no_rewind, callback=cb)
     total = time.perf_counter() - t0
     e.close()
     tw = float(np.median([h["window_s"] for h in hist]))
@@ -328,7 +328,9 @@ def bench_fit(a, pm, eng):
         "config": {"workload": f"batched parameter fit: {R} replicas x {N:,} beads (rho=3, 30% H-T-T lipids, "
                                f"k_angle=0, chemistry off, distinct seeds), {nwin}-step adjoint windows, checkpoint "
                                f"every 20, tail-tail RDF 64 bins on [0,3), replica-mean gradient, {a.steps} Adam "
-                               f"steps (lr {a.lr}) on the six a_ij toward the RDF of a seed run at a_TT=20, a_WT=90",
+                               f"steps (lr {a.lr}) on the six a_ij toward the RDF of a seed run at a_TT=20, a_WT=90, "
+                               + ("trajectories continued" if a.no_rewind else
+                                  "every window from the equilibrated state (same age as the target)"),
                    "replicas": R, "N": N},
         "gradient_cost_vs_plain": tw / t_plain,
         "loss_first": hist[0]["loss"], "loss_last": hist[-1]["loss"],
@@ -358,7 +360,10 @@ def main():
     ap.add_argument("--window", type=int, default=200)
     ap.add_argument("--equil", type=int, default=1000)
     ap.add_argument("--replicas", type=int, default=None, help="default 256 (ensemble), 32 (fit)")
-    ap.add_argument("--lr", type=float, default=1.0, help="Adam step size of the fit workload")
+    ap.add_argument("--lr", type=float, default=0.5, help="Adam step size of the fit workload")
+    ap.add_argument("--no-rewind", action="store_true",
+                    help="fit workload: continue the trajectories from window to window instead of restarting every "
+                         "window from the equilibrated state")
     ap.add_argument("--tune", action="store_true", help="autotune nsub / pair launch shape first")
     a = ap.parse_args()
     fitw = a.workload == "fit"

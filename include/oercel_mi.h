@@ -160,6 +160,13 @@ int oc_start(oc_engine *h, const oc_state *s);
 /* Load / read replica r's state (gid order).  oc_set_state mirrors a restart
  * from a checkpoint (cel/engine/oercel.c:1942-2010): vh and f as stored. */
 int oc_set_state(oc_engine *h, int32_t r, const oc_state *s, int64_t step);
+
+/* oc_set_state with the slot order of a checkpoint: slot_gid[q] = gid of the
+ * bead in slot q (a permutation of 0..N-1).  The reference keeps its
+ * checkpoint in slot order and continues from it (read_checkpoint,
+ * cel/engine/oercel.c:1942-2010), so the next stable sort, and with it every
+ * force sum, is the one the uninterrupted run makes: continuation is bitwise. */
+int oc_set_state_slots(oc_engine *h, int32_t r, const oc_state *s, int64_t step, const uint32_t *slot_gid);
 int oc_get_state(oc_engine *h, int32_t r, oc_state *s);
 
 /* n full steps of every replica (integrate, sort, chemistry, pair forces,
@@ -183,6 +190,13 @@ int oc_get_cells(oc_engine *h, int32_t r, int32_t *cstart, uint32_t *gid_of_slot
 int oc_get_events(oc_engine *h, int32_t r, oc_event *ev, int64_t cap, int64_t *n_out);
 
 int oc_get_stats(oc_engine *h, int32_t r, oc_stats *out);
+
+/* Running max bond length of replica r (the value oc_get_stats reports and
+ * resets): *get receives it, then *set (>= 0) replaces it; either may be NULL.
+ * A checkpoint carries it (the reference's write_ckpt / read_ckpt,
+ * cel/engine/oercel.c:1891-2010, store s->maxbond), so the stats row after a
+ * restart equals the uninterrupted run's. */
+int oc_max_bond(oc_engine *h, int32_t r, double *get, const double *set);
 
 /* Current step counter of replica r. */
 int oc_get_step(oc_engine *h, int32_t r, int64_t *step);
@@ -212,6 +226,12 @@ int oc_window_forward(oc_engine *h, int32_t nsteps, int32_t ckpt_every, const oc
  * a_WT, a_HH, a_HT, a_TT, gamma), the noise stream held fixed.  Recomputes the
  * window from its checkpoints; the engine stays at the window's last step. */
 int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad);
+
+/* Put every replica back at the start of the last forward window (state, step
+ * counters), so the next window runs from the same state: a parameter fit
+ * whose windows all start there minimises one deterministic loss of theta
+ * (the noise stream is fixed).  No counterpart in the reference. */
+int oc_window_rewind(oc_engine *h);
 
 /* Window gradient of every replica (no counterpart in the reference): with the
  * noise stream fixed and chemistry off, recompute f at the current state, run

@@ -1,7 +1,8 @@
 """Batched parameter fit on the GPU: the gradient that drives Adam is the
 replica mean of the per-replica window gradients, the first update is Adam's
-(lr * g / (|g| + eps)), and the second window continues from the first
-window's last state at the updated parameters."""
+(lr * g / (|g| + eps)), and the second window starts from the first window's
+start (rewind) at the updated parameters; a rewound window repeats the first
+bit for bit."""
 import importlib
 import sys
 
@@ -55,6 +56,7 @@ def test_fit_gradient_is_replica_mean_and_adam_update(oracle, eng):
         for r in range(3):
             eb.set_theta(th, r)
         lb, gb, _ = eb.window_grad(win, spec, target, ckpt_every=ck)
+        eb.window_rewind()
         gm = gb.mean(0)
         say(f"step {k}: loss fit {hist[k]['loss']:.10f} mean {lb.mean():.10f}")
         assert abs(hist[k]["loss"] - lb.mean()) <= 1e-12 * lb.mean()
@@ -68,6 +70,26 @@ def test_fit_gradient_is_replica_mean_and_adam_update(oracle, eng):
     want = hist[0]["theta"][free] - lr * g0 / (np.abs(g0) + 1e-8)
     assert np.allclose(hist[1]["theta"][free], want, rtol=0, atol=1e-12)
     assert hist[1]["theta"][6] == hist[0]["theta"][6]
+
+
+def test_rewind_repeats_the_window(oracle, eng):
+    abi = importlib.import_module("welcome-youtube_amd.abi")
+    spec = abi.rdf_spec(2, 2, 16, 3.0)
+    (p, st, s0), = states(oracle, ((14.0, 4),))
+    e = eng.Engine(p)
+    e.set_state(st, step=s0)
+    g1 = e.window_forward(10, spec, ckpt_every=4)
+    assert e.step_count() == s0 + 10
+    e.window_rewind()
+    assert e.step_count() == s0
+    a = e.get_state()
+    assert np.array_equal(a["x"], st["x"]) and np.array_equal(a["vx"], st["vx"])
+    g2 = e.window_forward(10, spec, ckpt_every=4)
+    assert np.array_equal(g1, g2)
+    e.window_backward(np.ones((1, 16)))
+    e.window_rewind()
+    assert e.step_count() == s0
+    e.close()
 
 
 def test_fit_rejects_bad_arguments(eng):

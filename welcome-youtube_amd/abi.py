@@ -113,6 +113,7 @@ def _declare(lib):
         "oc_soup_host": (C.c_int, [P(oc_params), C.c_int32, P(oc_state)]),
         "oc_start": (C.c_int, [vp, P(oc_state)]),
         "oc_set_state": (C.c_int, [vp, C.c_int32, P(oc_state), C.c_int64]),
+        "oc_set_state_slots": (C.c_int, [vp, C.c_int32, P(oc_state), C.c_int64, P(C.c_uint32)]),
         "oc_get_state": (C.c_int, [vp, C.c_int32, P(oc_state)]),
         "oc_step": (C.c_int, [vp, C.c_int64, C.c_int32]),
         "oc_sort": (C.c_int, [vp]),
@@ -122,12 +123,14 @@ def _declare(lib):
         "oc_get_events": (C.c_int, [vp, C.c_int32, P(oc_event), C.c_int64, P(C.c_int64)]),
         "oc_get_stats": (C.c_int, [vp, C.c_int32, P(oc_stats)]),
         "oc_get_step": (C.c_int, [vp, C.c_int32, P(C.c_int64)]),
+        "oc_max_bond": (C.c_int, [vp, C.c_int32, P(C.c_double), P(C.c_double)]),
         "oc_stream": (vp, [vp]),
         "oc_launch_pair": (C.c_int, [vp, C.c_int32]),
         "oc_abi_sizes": (C.c_int, [P(C.c_int32), C.c_int32]),
         "oc_set_pair_params": (C.c_int, [vp, C.c_int32, P(C.c_double), C.c_double]),
         "oc_window_forward": (C.c_int, [vp, C.c_int32, C.c_int32, P(oc_rdf_spec), P(C.c_double)]),
         "oc_window_backward": (C.c_int, [vp, P(C.c_double), P(C.c_double)]),
+        "oc_window_rewind": (C.c_int, [vp]),
         "oc_window_grad": (C.c_int, [vp, C.c_int32, C.c_int32, P(oc_rdf_spec), P(C.c_double), C.c_int32,
                                      P(C.c_double), P(C.c_double), P(C.c_double)]),
     }

@@ -625,11 +625,22 @@ OC_API int oc_abi_sizes(int32_t *out, int32_t n) {
 
 // Load replica r from gid-order host arrays: stable counting sort by cell on
 // the host (the order sort_cells gives a gid-ordered input), upload in slot order.
-static int load_replica(oc_engine *h, int r, const oc_state *s, size_t base, bool vt_from_v) {
+// order (optional): gids in the order the beads held their slots (a
+// checkpoint's slot order); the stable sort by cell then keeps that order
+// within each cell, as the reference's state does after a restart.
+static int load_replica(oc_engine *h, int r, const oc_state *s, size_t base, bool vt_from_v,
+                        const uint32_t *order = nullptr) {
     const RepInfo &R = h->ri[r];
     const int N = R.N;
     if (!s->x || !s->y || !s->vx || !s->vy || !s->type || !s->mpos) return fail(OC_EINVAL, "state: missing array");
     std::vector<int32_t> key(N), cnt(R.ncell + 1, 0), src(N), dest(N);
+    if (order) {
+        std::vector<uint8_t> seen(N, 0);
+        for (int q = 0; q < N; q++) {
+            if (order[q] >= (uint32_t)N || seen[order[q]]) return fail(OC_EINVAL, "slot order is not a permutation");
+            seen[order[q]] = 1;
+        }
+    }
     for (int g = 0; g < N; g++) {
         const double x = s->x[base + g], y = s->y[base + g];
         if (!(x >= 0.0 && x < R.L && y >= 0.0 && y < R.L)) return fail(OC_EINVAL, "state: position outside [0, L)");
@@ -647,7 +658,8 @@ static int load_replica(oc_engine *h, int r, const oc_state *s, size_t base, boo
     }
     cs[R.ncell] = run;
     std::vector<int32_t> fill(cs.begin(), cs.end() - 1);
-    for (int g = 0; g < N; g++) {
+    for (int q = 0; q < N; q++) {
+        const int g = order ? (int)order[q] : q;
         const int d = fill[key[g]]++;
         src[d] = g;
         dest[g] = d;
@@ -724,9 +736,13 @@ OC_API int oc_start(oc_engine *h, const oc_state *s) {
 }
 
 OC_API int oc_set_state(oc_engine *h, int32_t r, const oc_state *s, int64_t step) {
+    return oc_set_state_slots(h, r, s, step, nullptr);
+}
+
+OC_API int oc_set_state_slots(oc_engine *h, int32_t r, const oc_state *s, int64_t step, const uint32_t *slot_gid) {
     if (!h || !s || r < 0 || r >= h->nrep) return fail(OC_EINVAL, "bad arguments");
     HIPCHK(hipStreamSynchronize(h->stream));
-    if (int rc = load_replica(h, r, s, 0, false)) return rc;
+    if (int rc = load_replica(h, r, s, 0, false, slot_gid)) return rc;
     if (int rc = set_step_fields(h, r, step)) return rc;
     h->keys_fresh = false;
     launch_build_prv(h);
@@ -950,6 +966,20 @@ OC_API int oc_get_stats(oc_engine *h, int32_t r, oc_stats *out) {
     out->n_bonds = (int64_t)v[13];
     rs.maxbond_bits = 0;
     HIPCHK(hipMemcpy(h->d_rs + r, &rs, sizeof rs, hipMemcpyHostToDevice));
+    return 0;
+}
+
+OC_API int oc_max_bond(oc_engine *h, int32_t r, double *get, const double *set) {
+    if (!h || r < 0 || r >= h->nrep) return fail(OC_EINVAL, "bad arguments");
+    if (set && !(*set >= 0.0)) return fail(OC_EINVAL, "max bond must be >= 0");
+    RepStep rs;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(&rs, h->d_rs + r, sizeof rs, hipMemcpyDeviceToHost));
+    if (get) memcpy(get, &rs.maxbond_bits, sizeof *get);
+    if (set) {
+        memcpy(&rs.maxbond_bits, set, sizeof *set);
+        HIPCHK(hipMemcpy(h->d_rs + r, &rs, sizeof rs, hipMemcpyHostToDevice));
+    }
     return 0;
 }
 
@@ -1361,6 +1391,18 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
         for (int r = 0; r < h->nrep; r++)
             for (int q = 0; q < 7; q++) grad[(size_t)r * 7 + q] = g8[(size_t)r * 8 + q];
     return 0;
+}
+
+OC_API int oc_window_rewind(oc_engine *h) {
+    if (!h) return fail(OC_EINVAL, "null engine");
+    Adj *a = h->adj;
+    if (!a || !a->ready || a->ckpt.empty()) return fail(OC_ESTATE, "no forward window to rewind to");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // the window's first checkpoint: the state (and step counters) it started from
+    if (int rc = snap_copy(h, a->ckpt[0], false)) return rc;
+    launch_build_prv(h);
+    HIPCHK(hipGetLastError());
+    return check_err(h);
 }
 
 OC_API int oc_window_grad(oc_engine *h, int32_t nsteps, int32_t ckpt_every, const oc_rdf_spec *spec,

@@ -123,12 +123,21 @@ class Engine:
         self._chk(self.lib.oc_start(self.h, C.byref(st)))
         return self
 
-    def set_state(self, s, step=0, r=0):
+    def set_state(self, s, step=0, r=0, slot_gid=None):
         """Restart-style load of replica r: x, y, vh, f as given (f optional);
-        the predicted velocity is set to vh."""
+        the predicted velocity is set to vh.  slot_gid (gids in slot order, as
+        a checkpoint or cells() holds them) restores the slot order too, so the
+        continuation is bitwise the uninterrupted run's."""
         c = _contig(s)
         st = abi.make_state(c)
-        self._chk(self.lib.oc_set_state(self.h, int(r), C.byref(st), int(step)))
+        if slot_gid is None:
+            self._chk(self.lib.oc_set_state(self.h, int(r), C.byref(st), int(step)))
+        else:
+            o = np.ascontiguousarray(slot_gid, dtype=np.uint32)
+            if o.shape != (self.N[r],):
+                raise ValueError(f"slot_gid must hold {self.N[r]} gids")
+            self._chk(self.lib.oc_set_state_slots(self.h, int(r), C.byref(st), int(step),
+                                                  o.ctypes.data_as(C.POINTER(C.c_uint32))))
 
     def get_state(self, r=0):
         s = empty_state(self.N[r])
@@ -222,6 +231,10 @@ class Engine:
                                               g.ctypes.data_as(C.POINTER(C.c_double))))
         return g
 
+    def window_rewind(self):
+        """Back to the start of the last forward window (state and steps)."""
+        self._chk(self.lib.oc_window_rewind(self.h))
+
     def window_grad(self, nsteps, spec, target, ckpt_every=20):
         """L2 RDF loss per replica, its gradient (nrep, 7) and the mean RDF."""
         t = np.ascontiguousarray(target, dtype=np.float64)
@@ -233,6 +246,14 @@ class Engine:
         self._chk(self.lib.oc_window_grad(self.h, int(nsteps), int(ckpt_every), C.byref(spec), t.ctypes.data_as(P),
                                           stride, loss.ctypes.data_as(P), g.ctypes.data_as(P), gb.ctypes.data_as(P)))
         return loss, g, gb
+
+    def max_bond(self, r=0, set_to=None):
+        """Running max bond length of replica r (since the last stats() call);
+        set_to replaces it afterwards (a checkpoint's value on restart)."""
+        g = C.c_double()
+        sv = None if set_to is None else C.byref(C.c_double(float(set_to)))
+        self._chk(self.lib.oc_max_bond(self.h, int(r), C.byref(g), sv))
+        return g.value
 
     def step_count(self, r=0):
         s = C.c_int64()

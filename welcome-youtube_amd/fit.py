@@ -7,9 +7,13 @@ Every optimiser step runs one adjoint window on all replicas of one engine
 tail-tail RDF of each replica averaged over the window, the loss
 L = (1/R) sum_r sum_b (gbar_rb - target_b)^2 and its gradient with respect to
 one theta shared by the R replicas, which is the mean of the per-replica
-adjoint gradients.  torch.optim.Adam then updates the free entries of theta,
-and the replicas continue from the window's last state, so successive windows
-sample successive stretches of each trajectory.
+adjoint gradients.  torch.optim.Adam then updates the free entries of theta.
+With rewind=True every window starts from the replicas' state at the first
+step (oc_window_rewind): the loss is then one deterministic function of theta
+(fixed start, fixed noise), at the same age of the trajectories as a target
+taken at that age.  With rewind=False the replicas continue from each window's
+last state, so successive windows sample successive stretches of each
+trajectory.
 
 theta = (a_WW, a_WH, a_WT, a_HH, a_HT, a_TT, gamma); `free` selects the
 entries Adam moves (default: the six a_ij, gamma fixed).  The noise stream is
@@ -28,7 +32,7 @@ A_IJ = (0, 1, 2, 3, 4, 5)
 
 
 def fit_rdf(engine, spec, target, steps=50, lr=0.5, window=200, ckpt_every=20, free=A_IJ, theta0=None,
-            betas=(0.9, 0.999), callback=None):
+            betas=(0.9, 0.999), rewind=True, callback=None):
     """Adam steps on the shared theta of every replica of `engine`.
 
     Returns a list with one dict per step: the loss and gradient of the window
@@ -63,6 +67,8 @@ def fit_rdf(engine, spec, target, steps=50, lr=0.5, window=200, ckpt_every=20, f
         rec = {"step": k, "loss": float(loss.item()), "theta": theta.detach().numpy().copy(), "grad": g,
                "window_s": t1 - t0}
         opt.step()
+        if rewind:
+            engine.window_rewind()
         rec["update_s"] = time.perf_counter() - t1
         hist.append(rec)
         if callback is not None:

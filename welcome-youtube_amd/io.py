@@ -246,7 +246,8 @@ def save_checkpoint(engine, path, r=0):
     step = engine.step_count(r)
     hdr = {"step": step, "time": p.t_anchor + (step - p.step_anchor) * p.dt, "L": p.L, "seed": p.seed,
            "mlen": p.mlen, "nH": p.lip_nH, "warm_end": max(p.warm_end, p.step_anchor + p.warmup),
-           "t_anchor": p.t_anchor, "step_anchor": p.step_anchor, "dt_anchor": p.dt, "soup_seed": p.seed}
+           "t_anchor": p.t_anchor, "step_anchor": p.step_anchor, "dt_anchor": p.dt, "soup_seed": p.seed,
+           "maxbond": engine.max_bond(r)}
     write_checkpoint(path, s, slot_gid, hdr)
     return hdr
 
@@ -269,10 +270,13 @@ def restart_params(params, hdr):
 
 def load_checkpoint(engine, path, r=0):
     """Load a checkpoint into replica r of an Engine created with
-    restart_params(params, header) (same L)."""
-    s, hdr, _ = read_checkpoint(path)
+    restart_params(params, header) (same L), in the checkpoint's slot order:
+    the continuation is bitwise the uninterrupted run's, as the reference's
+    restart is (cel/engine/tests/validate.sh:80-105)."""
+    s, hdr, slot_gid = read_checkpoint(path)
     p = engine.params[r]
     if abs(hdr["L"] - p.L) > 1e-12:
         raise FormatError(f"checkpoint L={hdr['L']} differs from parameter L={p.L}")
-    engine.set_state(s, step=hdr["step"], r=r)
+    engine.set_state(s, step=hdr["step"], r=r, slot_gid=slot_gid)
+    engine.max_bond(r, set_to=hdr["maxbond"])
     return hdr
