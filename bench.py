@@ -400,9 +400,10 @@ def main():
     pair_s = time_pair_kernel(e)
     ncell = e.cells()[0].size - 1
     # algorithmic bytes of one force launch: staged xy (16), vt (8), gid/type
-    # (4), bonded links mp (1), lnk (4), prv (4), force write (16) per bead,
-    # and the cell starts
-    bytes_pair = N * (16 + 8 + 4 + 1 + 4 + 4 + 16) + 4 * (ncell + 1)
+    # (4), bonded links mp (1), lnk (4), prv (4), the half-step velocity the
+    # next-key epilogue integrates (16), force (16) and next cell key (4)
+    # writes per bead, and the cell starts
+    bytes_pair = N * (16 + 8 + 4 + 1 + 4 + 4 + 16 + 16 + 4) + 4 * (ncell + 1)
     achieved = bytes_pair / pair_s / 1e9
     e.close()
 
