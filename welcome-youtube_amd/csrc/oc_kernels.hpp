@@ -1462,13 +1462,33 @@ __global__ __launch_bounds__(kChemThreads) void k_chem_decide(const RepInfo *__r
         s_na = 0;
     }
     __syncthreads();
-    for (int k = 0; k < kRedChunk / kChemThreads; k++) {
-        const int loc = b0 + k * kChemThreads + threadIdx.x;
+    // classification: every load of the thread's beads first (own words, then
+    // the bond partners' gid words), so the chunk costs two load latencies
+    constexpr int kPer = kRedChunk / kChemThreads;
+    uint32_t wq[kPer], wp[kPer];
+    int8_t mq[kPer];
+    int32_t lq[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const int loc = b0 + k * kChemThreads + (int)threadIdx.x;
+        const int s = R.off + (loc < R.N ? loc : 0);
+        wq[k] = gt[s];
+        mq[k] = mp[s];
+        lq[k] = lnk[s];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const int loc = b0 + k * kChemThreads + (int)threadIdx.x;
+        wp[k] = loc < R.N && mq[k] < 0 && lq[k] >= 0 ? gt[lq[k]] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const int loc = b0 + k * kChemThreads + (int)threadIdx.x;
         if (loc >= R.N) break;
         const int s = R.off + loc;
-        const uint32_t w = gt[s], g = w & kGidMask, ty = w >> kGidBits;
-        const int m = mp[s], l = lnk[s];
-        const bool brk = m < 0 && l >= 0 && g < (gt[l] & kGidMask) && hash_u01(S.kbreak, g) < R.p_break;
+        const uint32_t w = wq[k], g = w & kGidMask, ty = w >> kGidBits;
+        const int m = mq[k], l = lq[k];
+        const bool brk = m < 0 && l >= 0 && g < (wp[k] & kGidMask) && hash_u01(S.kbreak, g) < R.p_break;
         const bool conv = ty == OC_N && hash_u01(S.kconv, g) < R.p_conv;
         const bool form = ty == OC_T && m < 0 && l < 0;
         if (conv || form) {
