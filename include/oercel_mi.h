@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define OC_ABI_VERSION 1
+#define OC_ABI_VERSION 2
 #define OC_NTYPES 7
 #define OC_MAXMLEN 12
 
@@ -52,8 +52,9 @@ enum { OC_EV_CONV = 0, OC_EV_FORM = 1, OC_EV_BREAK = 2 };
  * (cel/engine/oercel.c:178-221) that the DPD step reads, after the ini/key=value
  * rules of set_param/finalize_params (cel/engine/oercel.c:338-395, 578-619) have
  * been applied by the host (the Python package mirrors those rules).
- * Lipids are linear chains of lip_nH heads then lip_nT tails
- * (cel/engine/oercel.c:485-499).
+ * Lipids follow the template of lip_arch (linear, pi, y or bola); the beads
+ * of a lipid are chained through nxt in template order (link_lipid,
+ * cel/engine/oercel.c:801-806).
  */
 typedef struct oc_params {
     /* physics */
@@ -79,7 +80,17 @@ typedef struct oc_params {
     /* time axis: t(step) = t_anchor + (step - step_anchor) * dt */
     double t_anchor;
     int64_t step_anchor;
+    /* lipid template, mol_build (cel/engine/oercel.c:479-576): OC_ARCH_*.
+     * linear: lip_nH heads then lip_nT tails; pi: two bonded heads, each with a
+     * tail of lip_nT beads; y: one head with two tails; bola: a head pair and a
+     * single head joined by two tails (a macrocycle). */
+    int32_t lip_arch;
+    int32_t lip_tie;           /* pi / y / bola: ladder rungs between the two tails */
+    double r0_head;            /* pi / bola: rest length of the head-head bond */
+    double k_head, cos_head;   /* junction angle terms U = k_head (cos - cos_head)^2 (0 = none) */
 } oc_params;
+
+enum { OC_ARCH_LINEAR = 0, OC_ARCH_PI = 1, OC_ARCH_Y = 2, OC_ARCH_BOLA = 3 };
 
 /*
  * Per-bead state in gid order.  Any pointer may be NULL in oc_get_state (that

@@ -355,6 +355,99 @@ OR_API void or_pair_brute(const oc_params *p, int32_t N, const double *x, const 
     }
 }
 
+/* ------------------------------------------------------------------ lipid templates */
+
+/* mol_build, cel/engine/oercel.c:479-576: bead count, heads, bonds (bi, bj,
+ * rest) and angle terms (i, j = middle, k; kind 0: K (1 - cos), kind 1:
+ * K (cos - c0)^2), in the reference's order, and each bead's soup-init offset
+ * from bead 0 (template centred on its centroid first). */
+struct Tmpl {
+    int arch = 0, mlen = 0, nH = 0;
+    std::vector<int> bi, bj, ai, aj, ak, akind;
+    std::vector<double> br0, aK, ac0, ix, iy;
+    void bond(int i, int j, double r0) { bi.push_back(i); bj.push_back(j); br0.push_back(r0); }
+    void angle(int i, int j, int k, int kind, double K, double c0) {
+        if (K == 0.0) return;
+        ai.push_back(i); aj.push_back(j); ak.push_back(k); akind.push_back(kind); aK.push_back(K); ac0.push_back(c0);
+    }
+    void chain(const std::vector<int> &c, double r0, double ka) {
+        for (size_t b = 0; b + 1 < c.size(); b++) bond(c[b], c[b + 1], r0);
+        for (size_t b = 0; b + 2 < c.size(); b++) angle(c[b], c[b + 1], c[b + 2], 0, ka, 0.0);
+    }
+};
+
+static Tmpl make_tmpl(const oc_params *p) {
+    Tmpl M;
+    const double r0 = p->r0, h = p->r0_head;
+    const int nT = p->lip_nT;
+    M.arch = p->lip_arch;
+    std::vector<double> px, py;
+    if (p->lip_arch == OC_ARCH_LINEAR) {
+        M.nH = p->lip_nH;
+        M.mlen = p->lip_nH + nT;
+        for (int b = 0; b + 1 < M.mlen; b++) M.bond(b, b + 1, r0);
+        for (int b = 0; b + 2 < M.mlen; b++) M.angle(b, b + 1, b + 2, 0, p->k_angle, 0.0);
+        for (int b = 0; b < M.mlen; b++) { M.ix.push_back(b * r0); M.iy.push_back(0.0); }
+        return M;
+    }
+    std::vector<int> ca, cb;
+    if (p->lip_arch == OC_ARCH_PI) {
+        M.nH = 2;
+        M.mlen = 2 + 2 * nT;
+        M.bond(0, 1, h);
+        ca.push_back(0); cb.push_back(1);
+        for (int k = 1; k <= nT; k++) { ca.push_back(1 + k); cb.push_back(1 + nT + k); }
+        M.chain(ca, r0, p->k_angle);
+        M.chain(cb, r0, p->k_angle);
+        for (int m = 1; m <= p->lip_tie && m <= nT; m++) M.bond(ca[m], cb[m], h);
+        M.angle(1, 0, ca[1], 1, p->k_head, p->cos_head);
+        M.angle(0, 1, cb[1], 1, p->k_head, p->cos_head);
+        px.assign(M.mlen, 0.0); py.assign(M.mlen, 0.0);
+        py[0] = -0.5 * h; py[1] = 0.5 * h;
+        for (int k = 1; k <= nT; k++) { px[ca[k]] = k * r0; py[ca[k]] = -0.5 * h; px[cb[k]] = k * r0; py[cb[k]] = 0.5 * h; }
+    } else if (p->lip_arch == OC_ARCH_Y) {
+        M.nH = 1;
+        M.mlen = 1 + 2 * nT;
+        ca.push_back(0); cb.push_back(0);
+        for (int k = 1; k <= nT; k++) { ca.push_back(k); cb.push_back(nT + k); }
+        M.chain(ca, r0, p->k_angle);
+        M.chain(cb, r0, p->k_angle);
+        for (int m = 1; m <= p->lip_tie && m <= nT; m++) M.bond(ca[m], cb[m], 1.2 * r0);
+        M.angle(ca[1], 0, cb[1], 1, p->k_head, p->cos_head);
+        px.assign(M.mlen, 0.0); py.assign(M.mlen, 0.0);
+        for (int k = 1; k <= nT; k++) {
+            px[ca[k]] = 0.8 * r0 + (k - 1) * r0; py[ca[k]] = -0.6 * r0;
+            px[cb[k]] = 0.8 * r0 + (k - 1) * r0; py[cb[k]] = 0.6 * r0;
+        }
+    } else {
+        M.nH = 3;
+        M.mlen = 3 + 2 * nT;
+        M.bond(0, 1, h);
+        ca.push_back(0); cb.push_back(1);
+        for (int k = 1; k <= nT; k++) { ca.push_back(2 + k); cb.push_back(2 + nT + k); }
+        ca.push_back(2); cb.push_back(2);
+        M.chain(ca, r0, p->k_angle);
+        M.chain(cb, r0, p->k_angle);
+        for (int m = 1; m <= p->lip_tie && m <= nT; m++) M.bond(ca[m], cb[m], h * (1.0 - (double)m / (nT + 1)));
+        M.angle(1, 0, ca[1], 1, p->k_head, p->cos_head);
+        M.angle(0, 1, cb[1], 1, p->k_head, p->cos_head);
+        const double X = sqrt((nT + 1) * r0 * (nT + 1) * r0 - 0.25 * h * h);
+        px.assign(M.mlen, 0.0); py.assign(M.mlen, 0.0);
+        py[0] = -0.5 * h; py[1] = 0.5 * h; px[2] = X;
+        for (int k = 1; k <= nT; k++) {
+            const double f = (double)k / (nT + 1);
+            px[ca[k]] = f * X; py[ca[k]] = -0.5 * h * (1.0 - f);
+            px[cb[k]] = f * X; py[cb[k]] = 0.5 * h * (1.0 - f);
+        }
+    }
+    double mx = 0, my = 0;
+    for (int b = 0; b < M.mlen; b++) { mx += px[b]; my += py[b]; }
+    mx /= M.mlen; my /= M.mlen;
+    for (int b = 0; b < M.mlen; b++) { px[b] -= mx; py[b] -= my; }
+    for (int b = 0; b < M.mlen; b++) { M.ix.push_back(px[b] - px[0]); M.iy.push_back(py[b] - py[0]); }
+    return M;
+}
+
 /* ------------------------------------------------------------------ bonded forces */
 
 /* one harmonic bond i -> j (d = x_j - x_i, minimum image):
@@ -384,11 +477,41 @@ static inline void bond_term(double L, double kb, double r0, int i, int j, const
  */
 OR_API void or_bonded(const oc_params *p, int32_t N, const double *x, const double *y, const int8_t *mpos,
                       const int32_t *lnk, const uint32_t *gid, double *fx, double *fy, double *maxbond, double *ebond) {
-    const int ml = p->lip_nH + p->lip_nT;
+    const Tmpl M = make_tmpl(p);
+    const int ml = M.mlen;
     const double L = p->L, kb = p->k_bond, r0 = p->r0, ka = p->k_angle;
     double mb = *maxbond, eb = 0.0;
     for (int h = 0; h < N; h++) {
-        if (mpos[h] == 0) {
+        if (mpos[h] == 0 && M.arch != OC_ARCH_LINEAR) {
+            /* template bonds, then angle terms: mol_forces, cel/engine/oercel.c:1218-1257 */
+            int sl[OC_MAXMLEN];
+            int n = 0;
+            for (int d = h; d >= 0 && n < ml; d = lnk[d]) sl[n++] = d;
+            if (n != ml) continue;
+            for (size_t q = 0; q < M.bi.size(); q++)
+                bond_term(L, kb, M.br0[q], sl[M.bi[q]], sl[M.bj[q]], x, y, fx, fy, &mb, ebond ? &eb : nullptr, nullptr,
+                          nullptr, nullptr);
+            for (size_t q = 0; q < M.ai.size(); q++) {
+                const int i = sl[M.ai[q]], j = sl[M.aj[q]], k = sl[M.ak[q]];
+                const double ax = minimg(x[j] - x[i], L), ay = minimg(y[j] - y[i], L);
+                const double cxv = minimg(x[k] - x[j], L), cyv = minimg(y[k] - y[j], L);
+                const double la = sqrt(ax * ax + ay * ay), lc = sqrt(cxv * cxv + cyv * cyv);
+                const double ilac = 1.0 / (la * lc);
+                const double cs = (ax * cxv + ay * cyv) * ilac;
+                const double g = M.akind[q] == 0 ? -M.aK[q] : 2.0 * M.aK[q] * (cs - M.ac0[q]);
+                const double dUax = g * (cxv * ilac - cs * ax / (la * la));
+                const double dUay = g * (cyv * ilac - cs * ay / (la * la));
+                const double dUcx = g * (ax * ilac - cs * cxv / (lc * lc));
+                const double dUcy = g * (ay * ilac - cs * cyv / (lc * lc));
+                fx[i] += dUax;
+                fy[i] += dUay;
+                fx[j] += -dUax + dUcx;
+                fy[j] += -dUay + dUcy;
+                fx[k] -= dUcx;
+                fy[k] -= dUcy;
+                eb += M.akind[q] == 0 ? M.aK[q] * (1.0 - cs) : M.aK[q] * (cs - M.ac0[q]) * (cs - M.ac0[q]);
+            }
+        } else if (mpos[h] == 0) {
             int sl[OC_MAXMLEN];
             double bx[OC_MAXMLEN], by[OC_MAXMLEN], bl[OC_MAXMLEN];
             int n = 0;
@@ -456,7 +579,8 @@ OR_API int or_soup(const oc_params *p, int32_t N, double *x, double *y, double *
                    int8_t *mpos, int32_t *nxt) {
     Rng rng;
     rng_seed(&rng, p->seed);
-    const int ml = p->lip_nH + p->lip_nT;
+    const Tmpl M = make_tmpl(p);
+    const int ml = M.mlen;
     const double L = p->L;
     const long nlip = lround(p->f_lip * N / ml);
     const long nKa = lround(p->f_Ka * N), nKb = lround(p->f_Kb * N), nN = lround(p->c_N * N);
@@ -465,12 +589,18 @@ OR_API int or_soup(const oc_params *p, int32_t N, double *x, double *y, double *
     for (long m = 0; m < nlip; m++) {
         const double hx = L * rng_unif(&rng), hy = L * rng_unif(&rng);
         const double phi = 2.0 * M_PI * rng_unif(&rng);
+        const double cph = cos(phi), sph = sin(phi);
         for (int b = 0; b < ml; b++) {
-            x[gi] = hx + b * p->r0 * cos(phi);
-            y[gi] = hy + b * p->r0 * sin(phi);
+            if (M.arch == OC_ARCH_LINEAR) {
+                x[gi] = hx + b * p->r0 * cos(phi);
+                y[gi] = hy + b * p->r0 * sin(phi);
+            } else {
+                x[gi] = hx + (M.ix[b] * cph - M.iy[b] * sph);
+                y[gi] = hy + (M.ix[b] * sph + M.iy[b] * cph);
+            }
             wrap1(&x[gi], L);
             wrap1(&y[gi], L);
-            type[gi] = b < p->lip_nH ? OC_H : OC_T;
+            type[gi] = b < M.nH ? OC_H : OC_T;
             mpos[gi] = (int8_t)b;
             nxt[gi] = b + 1 < ml ? gi + 1 : -1;
             gi++;
@@ -1006,7 +1136,9 @@ static void tape_load(OrSys *s, const TapeStep &t) {
 OR_API int or_window_grad(OrSys *s, int32_t nsteps, const RdfSpec *spec, const double *target, int32_t theta32,
                           double *loss, double *grad, double *gmean) {
     const oc_params *p = &s->p;
-    if (p->k_angle != 0.0 || p->stir_A != 0.0 || p->kT_amp != 0.0 || p->chem || s->step < p->warm_end) return -1;
+    if (p->k_angle != 0.0 || p->stir_A != 0.0 || p->kT_amp != 0.0 || p->chem || s->step < p->warm_end ||
+        p->lip_arch != OC_ARCH_LINEAR)
+        return -1;
     const RdfSpec sp = *spec;
     const int N = s->N, nb = sp.nbins;
     int na = 0, nbb = 0;

@@ -33,6 +33,11 @@ static void ref_params(const oc_params *p, int threads) {
     P->warmup = p->warmup; P->seed = p->seed;
     P->f_lip = p->f_lip; P->f_Ka = p->f_Ka; P->f_Kb = p->f_Kb; P->c_N = p->c_N;
     P->threads = threads; P->chem = 0;
+    {
+        static const char *const arch[4] = {"linear", "pi", "y", "bola"};
+        snprintf(P->lip_arch, sizeof P->lip_arch, "%s", arch[p->lip_arch >= 0 && p->lip_arch < 4 ? p->lip_arch : 0]);
+        P->lip_tie = p->lip_tie; P->r0_head = p->r0_head; P->k_head = p->k_head; P->cos_head = p->cos_head;
+    }
     finalize_params(P);
     g_warm_end = p->warm_end;
     omp_set_num_threads(threads);

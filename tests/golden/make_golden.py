@@ -18,6 +18,10 @@ Fixtures (numpy .npz / JSON, small systems):
   forces.npz        pair_forces (plain; stirring + kT cycle) and bonded_forces
   traj.npz          the CLI's checkpoint after 20 steps from a fresh soup
   params.json       the reference's --params table: keys, defaults, a_ij
+  templates.npz     branched lipids (pi, y, bola): soups, bonded forces of
+                    distorted lipids, and a 20-step CLI trajectory of pi lipids
+
+`python tests/golden/make_golden.py templates` writes templates.npz only.
 """
 import ctypes as C
 import json
@@ -75,6 +79,50 @@ def hashes(R):
     keys = out["hash_key"]
     out["hash_u01"] = np.array([R.ref_hash_u01(int(keys[k % 16]), int(gi[k])) for k in range(64)])
     return out
+
+
+# branched lipid templates: (name, L, overrides)
+TEMPLATES = [
+    ("pi", 14.0, dict(lip_arch="pi", lip_nT=2, lip_tie=1, r0_head=0.7, k_head=5, cos_head=0.3, k_angle=10)),
+    ("pi_membranes", 14.0, dict(lip_arch="pi", lip_nT=1, lip_tie=1, r0_head=0.7, k_bond=300)),
+    ("y", 14.0, dict(lip_arch="y", lip_nT=2, lip_tie=1, k_head=4, cos_head=0.2, k_angle=8)),
+    ("bola", 14.0, dict(lip_arch="bola", lip_nT=3, lip_tie=2, r0_head=0.6, k_head=3, cos_head=-0.1, k_angle=6)),
+]
+
+
+def templates():
+    out = {}
+    rng = np.random.default_rng(11)
+    for name, L, kw in TEMPLATES:
+        p = mk(L, 4, f_lip=0.3, c_N=0.1, f_Ka=0.0, seed=3, **kw)
+        cp = p.to_c()
+        s = oracle.ref_soup(cp, p.N)
+        for k, v in s.items():
+            out[f"{name}_{k}"] = v
+        st = dict(s, vtx=s["vx"], vty=s["vy"])
+        xb, yb = s["x"].copy(), s["y"].copy()
+        lip = s["mpos"] >= 0
+        xb[lip] = (xb[lip] + 0.15 * rng.standard_normal(lip.sum())) % L
+        yb[lip] = (yb[lip] + 0.15 * rng.standard_normal(lip.sum())) % L
+        fx, fy, mb = oracle.ref_forces_gid(cp, dict(st, x=xb, y=yb), 0, 0.0, with_pair=False, with_bonded=True)
+        out[f"{name}_bx"], out[f"{name}_by"] = xb, yb
+        out[f"{name}_bfx"], out[f"{name}_bfy"], out[f"{name}_maxbond"] = fx, fy, np.array(mb)
+    # a 20-step CLI run of the membranes lipid (tied pi H2T2) with its a_ij
+    with tempfile.TemporaryDirectory() as tmp:
+        ov = dict(L="16", dt="0.02", f_lip="0.2", c_N="0.1", f_Ka="0", lip_arch="pi", lip_nT="1", lip_tie="1",
+                  r0_head="0.7", k_bond="300", a_W_T="200", a_H_N="22", a_W_N="45", a_T_N="120", warmup="10",
+                  seed="4", nsub="4", chem="0", steps="20", snap_every="0", stats_every="0", ckpt_every="0",
+                  log_every="0", events="0", threads="2")
+        oracle.run_ref_cli(os.path.join(tmp, "pi"), ov)
+        ck = oracle.read_ckpt(os.path.join(tmp, "pi", "ckpt.bin"))
+        g = oracle.ckpt_gid_order(ck)
+        for k in ("x", "y", "vx", "vy", "fx", "fy", "type", "mpos", "nxt"):
+            out[f"traj_{k}"] = g[k]
+        out["traj_slot_gid"] = ck["gid"]
+        out["traj_step"] = np.array(ck["step"])
+        out["traj_maxbond"] = np.array(ck["maxbond"])
+        out["traj_overrides"] = np.array(json.dumps(ov))
+    np.savez_compressed(os.path.join(HERE, "templates.npz"), **out)
 
 
 def main():
@@ -154,8 +202,14 @@ def main():
            "amatrix": {n: [float(x) for x in row.split("|") if x.strip()] for n, row in amat}}
     with open(os.path.join(HERE, "params.json"), "w") as f:
         json.dump(tab, f, indent=1)
+    templates()
     print("golden fixtures written to", HERE)
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["templates"]:
+        if not oracle.ref_available():
+            sys.exit("oracle/_ref is not built: make -C oracle ref")
+        templates()
+    else:
+        main()

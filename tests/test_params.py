@@ -56,7 +56,7 @@ def test_typed_values():
 @pytest.mark.parametrize("kv,msg", [
     ({"L": "7.9"}, "L must be"), ({"nsub": "9"}, "nsub"), ({"dt": "0"}, "dt"), ({"kT_amp": "1"}, "kT_amp"),
     ({"init": "seeded"}, "seeded"), ({"feed": "1"}, "flood"), ({"feed_cap": "1"}, "feed_cap"),
-    ({"lip_arch": "pi"}, "not implemented"), ({"lip_nT": "0"}, "lipid length"), ({"init": "restart"}, "restart"),
+    ({"lip_arch": "star"}, "lip_arch must be"), ({"lip_nT": "0"}, "lipid length"), ({"init": "restart"}, "restart"),
     ({"p_conv": "2"}, "p_conv"), ({"r_form": "1.5"}, "r_conv and r_form"),
 ])
 def test_finalize_errors(kv, msg):

@@ -34,6 +34,8 @@ class oc_params(C.Structure):
         ("p_conv", C.c_double), ("r_conv", C.c_double), ("p_form", C.c_double), ("r_form", C.c_double),
         ("p_break", C.c_double),
         ("t_anchor", C.c_double), ("step_anchor", C.c_int64),
+        ("lip_arch", C.c_int32), ("lip_tie", C.c_int32), ("r0_head", C.c_double), ("k_head", C.c_double),
+        ("cos_head", C.c_double),
     ]
 
 

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "oc_adjoint.hpp"
+#include "oc_mol.hpp"
 
 using namespace oc;
 
@@ -68,6 +69,7 @@ struct oc_engine {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
     bool have_state = false, any_stir = false, any_chem = false, any_angle = false;
+    bool any_tmpl = false;  // some replica has branched lipids (k_mol; the force kernel leaves no keys)
     std::vector<std::vector<oc_event>> evq;
     std::vector<void *> allocs;
     bool events_dropped = false;
@@ -94,15 +96,32 @@ static int make_repinfo(const oc_params &p, int32_t N, RepInfo &R) {
     if (p.nsub < 1 || p.nsub > 8) return fail(OC_EINVAL, "nsub must be 1..8");
     if (p.dt <= 0) return fail(OC_EINVAL, "dt must be > 0");
     if (N < 2 || N >= (1 << kGidBits)) return fail(OC_EINVAL, "N must be in [2, 2^29)");
-    if (p.lip_nH < 1 || p.lip_nT < 1 || p.lip_nH + p.lip_nT > OC_MAXMLEN)
-        return fail(OC_EINVAL, "lipid length must be 2..12 with >= 1 head and tail");
+    MolTmpl M;
+    const char *why = "";
+    if (!build_template(p, M, &why)) return fail(OC_EINVAL, why);
     if (p.chem && (p.r_conv > 1.0 || p.r_form > 1.0)) return fail(OC_EINVAL, "r_conv and r_form must be <= 1");
     R.N = N;
     R.ncx = (int)floor(p.L);
     R.ncy = (int)floor(p.L) * p.nsub;
     R.ncell = R.ncx * R.ncy;
-    R.mlen = p.lip_nH + p.lip_nT;
-    R.nH = p.lip_nH;
+    R.mlen = M.mlen;
+    R.nH = M.nH;
+    R.arch = M.arch;
+    R.tnb = M.nb;
+    R.tna = M.na;
+    for (int q = 0; q < M.nb; q++) {
+        R.bi[q] = M.bi[q];
+        R.bj[q] = M.bj[q];
+        R.br0[q] = M.br0[q];
+    }
+    for (int q = 0; q < M.na; q++) {
+        R.ai[q] = M.ai[q];
+        R.aj[q] = M.aj[q];
+        R.ak[q] = M.ak[q];
+        R.akind[q] = M.akind[q];
+        R.aK[q] = M.aK[q];
+        R.ac0[q] = M.ac0[q];
+    }
     R.chem = p.chem;
     R.n_chem = p.n_chem < 1 ? 1 : p.n_chem;
     R.has_stir = p.stir_A != 0.0;
@@ -210,10 +229,10 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false) {
     A.err = h->d_err;
     // forces also leave the next step's cell keys
     A.v = h->st[c].v;
-    A.key = energy ? nullptr : h->d_key;
+    A.key = energy || h->any_tmpl ? nullptr : h->d_key;
     A.rmax = h->d_rmax;
     A.moved = h->d_moved;
-    if (!energy) h->keys_fresh = true;
+    if (!energy) h->keys_fresh = !h->any_tmpl;
     const dim3 grid(8 * h->per_xcd);
     // CAP (staged candidates) is 3.5x the block; the variant picks the single
     // pass, the per-lane two phases or the cross-lane pair queue
@@ -249,6 +268,12 @@ static void launch_stir(oc_engine *h) {
 static void launch_forces(oc_engine *h, bool with_bonded) {
     launch_stir(h);
     launch_pair(h, false, with_bonded);
+    if (with_bonded && h->any_tmpl) {
+        const int c = h->cur;
+        k_mol<false><<<(int)h->bblk.size(), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy,
+                                                                       h->st[c].mp, h->st[c].lnk, h->d_f, nullptr,
+                                                                       h->d_err);
+    }
 }
 
 static void launch_build_prv(oc_engine *h) {
@@ -384,7 +409,10 @@ OC_API int oc_soup_host(const oc_params *p, int32_t N, oc_state *o) {
     if (!p || !o || !o->x || !o->y || !o->vx || !o->vy || !o->type || !o->mpos || !o->nxt)
         return fail(OC_EINVAL, "oc_soup_host: missing output array");
     if (N < 2) return fail(OC_EINVAL, "N must be >= 2");
-    const int ml = p->lip_nH + p->lip_nT;
+    MolTmpl M;
+    const char *why = "";
+    if (!build_template(*p, M, &why)) return fail(OC_EINVAL, why);
+    const int ml = M.mlen;
     const double L = p->L;
     // composition, cel/engine/oercel.c:1775-1780
     const long nlip = lround(p->f_lip * N / ml);
@@ -396,10 +424,17 @@ OC_API int oc_soup_host(const oc_params *p, int32_t N, oc_state *o) {
         // straight template chain from a random head position and angle
         const double hx = L * rng.unif(), hy = L * rng.unif();
         const double phi = 2.0 * M_PI * rng.unif();
+        const double cph = cos(phi), sph = sin(phi);
         for (int b = 0; b < ml; b++, g++) {
-            o->x[g] = wrap_box(hx + b * p->r0 * cos(phi), L);
-            o->y[g] = wrap_box(hy + b * p->r0 * sin(phi), L);
-            o->type[g] = b < p->lip_nH ? OC_H : OC_T;
+            if (M.arch == OC_ARCH_LINEAR) {
+                o->x[g] = wrap_box(hx + b * p->r0 * cos(phi), L);
+                o->y[g] = wrap_box(hy + b * p->r0 * sin(phi), L);
+            } else {
+                // the template rotated by phi about bead 0
+                o->x[g] = wrap_box(hx + (M.ix[b] * cph - M.iy[b] * sph), L);
+                o->y[g] = wrap_box(hy + (M.ix[b] * sph + M.iy[b] * cph), L);
+            }
+            o->type[g] = b < M.nH ? OC_H : OC_T;
             o->mpos[g] = (int8_t)b;
             o->nxt[g] = b + 1 < ml ? g + 1 : -1;
             if (o->bond) o->bond[g] = -1;
@@ -506,7 +541,8 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
         for (int b = 0; b < grid_of(R.N, kRedChunk); b++) h->rblk.push_back(r);
         h->any_stir |= R.has_stir != 0;
         h->any_chem |= R.chem != 0;
-        h->any_angle |= R.ka != 0.0;
+        h->any_angle |= R.ka != 0.0 && R.arch == 0;
+        h->any_tmpl |= R.arch != 0;
     }
     h->Ntot = off;
     h->Ctot = coff;
@@ -794,7 +830,7 @@ static int capture_graph(oc_engine *h, int parity) {
     const int saved = h->cur;
     const bool fresh = h->keys_fresh;
     h->cur = parity;
-    h->keys_fresh = true;
+    h->keys_fresh = !h->any_tmpl;  // with branched lipids the graph's first step computes its keys
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     enqueue_step(h);
     enqueue_step(h);
@@ -817,9 +853,9 @@ OC_API int oc_step(oc_engine *h, int64_t n, int32_t use_graph) {
         const int p = h->cur;
         if (!h->gexec[p])
             if (int rc = capture_graph(h, p)) return rc;
-        if (!h->keys_fresh) launch_keys(h, true);
+        if (!h->keys_fresh && !h->any_tmpl) launch_keys(h, true);
         for (; k + 2 <= n; k += 2) HIPCHK(hipGraphLaunch(h->gexec[p], h->stream));
-        h->keys_fresh = true;
+        h->keys_fresh = !h->any_tmpl;
     }
     for (; k < n; k++) enqueue_step(h);
     HIPCHK(hipGetLastError());
@@ -938,6 +974,10 @@ OC_API int oc_get_stats(oc_engine *h, int32_t r, oc_stats *out) {
     k_bonded<true><<<(int)h->bblk.size(), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy,
                                                                       h->st[c].gt, h->st[c].mp, h->st[c].lnk, h->d_f,
                                                                       h->d_ebond, h->d_err);
+    if (h->any_tmpl)
+        k_mol<true><<<(int)h->bblk.size(), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy,
+                                                                      h->st[c].mp, h->st[c].lnk, h->d_f, h->d_ebond,
+                                                                      h->d_err);
     k_stats_partial<<<(int)h->rblk.size(), 256, 0, h->stream>>>(h->d_ri, h->d_rblk, h->st[c].v, h->d_f, h->st[c].gt,
                                                                h->st[c].mp, h->st[c].lnk, h->d_epot, h->d_ebond,
                                                                h->d_part);
@@ -1218,6 +1258,7 @@ OC_API int oc_window_forward(oc_engine *h, int32_t nsteps, int32_t ckpt_every, c
         const oc_params &P = h->prm[r];
         if (P.k_angle != 0.0 || P.stir_A != 0.0 || P.kT_amp != 0.0 || P.chem)
             return fail(OC_EINVAL, "window gradient needs k_angle = 0 and no stirring, kT cycle or chemistry");
+        if (h->ri[r].arch != OC_ARCH_LINEAR) return fail(OC_EINVAL, "window gradient needs linear lipids");
         Rc = std::max(Rc, (int)ceil(cut * R.icsx));
     }
     if (2 * Rc + 1 > kRdfMaxCol) return fail(OC_EINVAL, "rdf cutoff longer than four cell columns");

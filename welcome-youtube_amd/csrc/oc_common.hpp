@@ -35,6 +35,13 @@ struct RepInfo {
     uint64_t seed;
     double p_conv, rc2, p_form, rf2, p_break;
     double a[64];
+    // lipid template of a branched architecture (arch != 0): bonds (bi, bj,
+    // rest br0) and angle terms (ai, aj = middle, ak; kind 0: K (1 - cos),
+    // kind 1: K (cos - c0)^2), in template order
+    int32_t arch, tnb, tna, tpad;
+    int32_t bi[2 * OC_MAXMLEN], bj[2 * OC_MAXMLEN];
+    int32_t ai[2 * OC_MAXMLEN], aj[2 * OC_MAXMLEN], ak[2 * OC_MAXMLEN], akind[2 * OC_MAXMLEN];
+    double br0[2 * OC_MAXMLEN], aK[2 * OC_MAXMLEN], ac0[2 * OC_MAXMLEN];
 };
 
 // Per-step values of one replica, written on the device by k_step_params.

@@ -1045,7 +1045,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             double bx = 0.0;
             if (A.stir) bx = A.stir[slot] - S.stir_mean;
             double fx = bx + ax, fy = ay;
-            if (A.bonded) bonded_gather<ANG>(R, pi, gti, B, A.xy, A.lnk, A.prv, fx, fy, mb);
+            // a branched lipid's terms come from k_mol (chemistry bonds stay here)
+            if (A.bonded && (R.arch == 0 || B.m < 0)) bonded_gather<ANG>(R, pi, gti, B, A.xy, A.lnk, A.prv, fx, fy, mb);
             A.f[slot] = make_double2(fx, fy);
             if (A.key) {
                 // the next step's integrate, as k_keys<true> would run it
@@ -1296,7 +1297,7 @@ __global__ __launch_bounds__(kBeadBlock) void k_bonded(const RepInfo *__restrict
     if (loc < R.N) {
         const int s = R.off + loc;
         const int m = mp[s];
-        if (m == 0) {
+        if (m == 0 && R.arch == 0) {
             owner = true;
             int sl[OC_MAXMLEN];
             double bx[OC_MAXMLEN], by[OC_MAXMLEN], bl[OC_MAXMLEN];
@@ -1336,6 +1337,93 @@ __global__ __launch_bounds__(kBeadBlock) void k_bonded(const RepInfo *__restrict
             }
         }
         if (ENERGY) ebond[s] = owner ? eb : 0.0;
+    }
+    if (!ENERGY) {
+        uint64_t bits = (uint64_t)__double_as_longlong(mb);
+        bits = wave_max_u64(bits);
+        unsigned long long *mbw = (unsigned long long *)&rs[r].maxbond_bits;
+        if ((threadIdx.x & 63) == 0 && bits > __hip_atomic_load(mbw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(mbw, (unsigned long long)bits);
+    }
+}
+
+// Bonded terms of branched lipids (pi, y, bola): one thread per bead; the head
+// (mpos 0) of a lipid of a replica with a template collects its molecule's
+// slots along the chain (template order) and evaluates the template's bonds,
+// then its angle terms, each added to the beads it acts on in template order
+// -- the arithmetic of mol_forces (cel/engine/oercel.c:1218-1257).  The
+// molecule's forces are read once, updated in that order and written back:
+// every bead belongs to one molecule, so the writes never collide, and each
+// bead's sum is its pair force plus its terms in the reference's order.
+// ENERGY: the molecule's bond and angle energy at its head (ebond), no forces.
+template <bool ENERGY>
+__global__ __launch_bounds__(kBeadBlock) void k_mol(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs,
+                                                    const int32_t *__restrict__ bblk,
+                                                    const double2 *__restrict__ xy, const int8_t *__restrict__ mp,
+                                                    const int32_t *__restrict__ lnk, double2 *__restrict__ f,
+                                                    double *__restrict__ ebond, uint32_t *__restrict__ err) {
+    const int r = bblk[blockIdx.x];
+    const RepInfo &R = ri[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    double mb = 0.0;
+    if (R.arch != 0 && loc < R.N) {
+        const int s = R.off + loc;
+        if (mp[s] == 0) {
+            int sl[OC_MAXMLEN];
+            int n = 0;
+            for (int d = s; d >= 0 && n < R.mlen; d = lnk[d]) sl[n++] = d;
+            double eb = 0.0;
+            if (n != R.mlen) {
+                atomicOr(err, ERR_CHAIN);
+            } else {
+                double2 q[OC_MAXMLEN], fl[OC_MAXMLEN];
+                for (int b = 0; b < n; b++) {
+                    q[b] = xy[sl[b]];
+                    if (!ENERGY) fl[b] = f[sl[b]];
+                }
+                const double L = R.L, hL = R.halfL, kb = R.kb;
+                for (int t = 0; t < R.tnb; t++) {
+                    const int i = R.bi[t], j = R.bj[t];
+                    const double dx = minimg(q[j].x - q[i].x, L, hL), dy = minimg(q[j].y - q[i].y, L, hL);
+                    const double rr = sqrt(dx * dx + dy * dy);
+                    if (rr > mb) mb = rr;
+                    const double fb = -kb * (rr - R.br0[t]) / rr;  // force on j along (dx, dy)
+                    if (!ENERGY) {
+                        fl[j].x += fb * dx;
+                        fl[j].y += fb * dy;
+                        fl[i].x -= fb * dx;
+                        fl[i].y -= fb * dy;
+                    }
+                    eb += 0.5 * kb * (rr - R.br0[t]) * (rr - R.br0[t]);
+                }
+                for (int t = 0; t < R.tna; t++) {
+                    const int i = R.ai[t], j = R.aj[t], k = R.ak[t];
+                    const double ax = minimg(q[j].x - q[i].x, L, hL), ay = minimg(q[j].y - q[i].y, L, hL);
+                    const double cxv = minimg(q[k].x - q[j].x, L, hL), cyv = minimg(q[k].y - q[j].y, L, hL);
+                    const double la = sqrt(ax * ax + ay * ay), lc = sqrt(cxv * cxv + cyv * cyv);
+                    const double ilac = 1.0 / (la * lc);
+                    const double cs = (ax * cxv + ay * cyv) * ilac;
+                    // g = dU/dcos: kind 0, U = K (1 - cos); kind 1, U = K (cos - c0)^2
+                    const double g = R.akind[t] == 0 ? -R.aK[t] : 2.0 * R.aK[t] * (cs - R.ac0[t]);
+                    if (!ENERGY) {
+                        const double dUax = g * (cxv * ilac - cs * ax / (la * la));
+                        const double dUay = g * (cyv * ilac - cs * ay / (la * la));
+                        const double dUcx = g * (ax * ilac - cs * cxv / (lc * lc));
+                        const double dUcy = g * (ay * ilac - cs * cyv / (lc * lc));
+                        fl[i].x += dUax;
+                        fl[i].y += dUay;
+                        fl[j].x += -dUax + dUcx;
+                        fl[j].y += -dUay + dUcy;
+                        fl[k].x -= dUcx;
+                        fl[k].y -= dUcy;
+                    }
+                    eb += R.akind[t] == 0 ? R.aK[t] * (1.0 - cs) : R.aK[t] * (cs - R.ac0[t]) * (cs - R.ac0[t]);
+                }
+                if (!ENERGY)
+                    for (int b = 0; b < n; b++) f[sl[b]] = fl[b];
+            }
+            if (ENERGY) ebond[s] = eb;
+        }
     }
     if (!ENERGY) {
         uint64_t bits = (uint64_t)__double_as_longlong(mb);

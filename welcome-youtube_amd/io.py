@@ -49,9 +49,12 @@ def rng_seed_state(seed):
     return out
 
 
-def lipid_bonds(s, mlen):
-    """Template bonds of linear lipids (b, b+1), ordered by the gid of the
-    bond's first bead then template order, each as (min gid, max gid)."""
+def lipid_bonds(s, mlen, tbonds=None):
+    """Template bonds of the lipids, ordered by the gid of the bond's first
+    template bead then template order, each as (min gid, max gid).  tbonds:
+    the template's (bi, bj) pairs (default: a linear chain, (b, b+1))."""
+    if tbonds is None:
+        tbonds = [(b, b + 1) for b in range(mlen - 1)]
     mpos, nxt = s["mpos"], s["nxt"]
     heads = np.flatnonzero(mpos == 0)
     out = []
@@ -59,9 +62,11 @@ def lipid_bonds(s, mlen):
         chain = [int(h)]
         while len(chain) < mlen and nxt[chain[-1]] >= 0:
             chain.append(int(nxt[chain[-1]]))
-        for b in range(len(chain) - 1):
-            i, j = chain[b], chain[b + 1]
-            out.append((i, b, min(i, j), max(i, j)))
+        if len(chain) != mlen:
+            raise FormatError(f"broken lipid chain at gid {int(h)}")
+        for q, (bi, bj) in enumerate(tbonds):
+            i, j = chain[bi], chain[bj]
+            out.append((i, q, min(i, j), max(i, j)))
     out.sort(key=lambda t: (t[0], t[1]))
     return [(a, b) for _, _, a, b in out]
 
@@ -74,10 +79,11 @@ def chem_bonds(s):
     return [(int(i), int(bd[i])) for i in g]
 
 
-def write_snapshot(path, s, step, time, L, mlen):
-    """OCEL v1 (cel/engine/oercel.c:1830-1884)."""
+def write_snapshot(path, s, step, time, L, mlen, tbonds=None):
+    """OCEL v1 (cel/engine/oercel.c:1830-1884); tbonds: the lipid template's
+    (bi, bj) bond pairs (default: a linear chain)."""
     Nb = len(s["x"])
-    bonds = lipid_bonds(s, mlen) + chem_bonds(s)
+    bonds = lipid_bonds(s, mlen, tbonds) + chem_bonds(s)
     with open(path, "wb") as f:
         f.write(b"OCEL")
         f.write(struct.pack("<iii", 1, Nb, len(bonds)))
@@ -245,7 +251,7 @@ def save_checkpoint(engine, path, r=0):
     _, slot_gid = engine.cells(r)
     step = engine.step_count(r)
     hdr = {"step": step, "time": p.t_anchor + (step - p.step_anchor) * p.dt, "L": p.L, "seed": p.seed,
-           "mlen": p.mlen, "nH": p.lip_nH, "warm_end": max(p.warm_end, p.step_anchor + p.warmup),
+           "mlen": p.mlen, "nH": p.nH, "warm_end": max(p.warm_end, p.step_anchor + p.warmup),
            "t_anchor": p.t_anchor, "step_anchor": p.step_anchor, "dt_anchor": p.dt, "soup_seed": p.seed,
            "maxbond": engine.max_bond(r)}
     write_checkpoint(path, s, slot_gid, hdr)

@@ -10,8 +10,9 @@ Same key names, types, defaults and error behaviour: an unknown key, a bad
 value or a removed key raises ParamError (the reference prints the message and
 exits).  Keys of the reference's catalyst chemistry, IO cadence and lipid
 architectures are accepted and kept so a reference ini loads unchanged; the
-engine refuses values that ask for features it does not implement (chem = 1,
-lip_arch other than linear) with an explicit error.
+engine refuses values that ask for features it does not implement (chem = 1)
+with an explicit error.  Lipid templates (lip_arch linear, pi, y, bola and
+their keys) are the reference's mol_build.
 
 Precursor-chemistry keys (no counterpart in the reference): pchem, p_conv,
 r_conv, p_form, r_form, p_break.
@@ -251,13 +252,19 @@ class Params:
         the linear lipid architecture this engine implements."""
         v = self.__dict__["_v"]
         self.__dict__["sigma"] = math.sqrt(2.0 * v["gamma"] * v["kT"])
-        if v["lip_arch"] in ("pi", "y", "bola"):
-            raise ParamError(f"lip_arch = {v['lip_arch']} is not implemented by this engine (linear only)")
-        if v["lip_arch"] != "linear":
+        if v["lip_arch"] not in ARCHS:
             raise ParamError("lip_arch must be linear, pi, y or bola")
-        mlen = v["lip_nH"] + v["lip_nT"]
-        if v["lip_nH"] < 1 or v["lip_nT"] < 1 or mlen > abi.MAXMLEN:
-            raise ParamError(f"lipid length must be 2..{abi.MAXMLEN} with >=1 head and tail")
+        if v["lip_arch"] == "linear":
+            mlen = v["lip_nH"] + v["lip_nT"]
+            if v["lip_nH"] < 1 or v["lip_nT"] < 1 or mlen > abi.MAXMLEN:
+                raise ParamError(f"lipid length must be 2..{abi.MAXMLEN} with >=1 head and tail")
+        else:
+            if v["lip_nT"] < 1:
+                raise ParamError("lip_nT must be >= 1")
+            mlen = mol_template(self).mlen
+            if mlen > abi.MAXMLEN:
+                raise ParamError(f"{v['lip_arch']} lipid with lip_nT={v['lip_nT']} has {mlen} beads "
+                                 f"(max {abi.MAXMLEN})")
         if v["syn_nN"] < 0 or v["syn_nN"] > mlen:
             raise ParamError(f"syn_nN must be 0..{mlen}")
         if v["r0_head"] <= 0:
@@ -334,7 +341,12 @@ class Params:
 
     @property
     def mlen(self):
-        return self.lip_nH + self.lip_nT
+        return mol_template(self).mlen
+
+    @property
+    def nH(self):
+        """heads per lipid (lip_nH for linear lipids, 2 / 1 / 3 for pi / y / bola)"""
+        return mol_template(self).nH
 
     def to_c(self):
         """oc_params for the C ABI (call finalize() first)."""
@@ -354,6 +366,8 @@ class Params:
         c.p_conv, c.r_conv, c.p_form, c.r_form, c.p_break = v["p_conv"], v["r_conv"], v["p_form"], v["r_form"], v[
             "p_break"]
         c.t_anchor, c.step_anchor = self.t_anchor, self.step_anchor
+        c.lip_arch = ARCHS.index(v["lip_arch"])
+        c.lip_tie, c.r0_head, c.k_head, c.cos_head = v["lip_tie"], v["r0_head"], v["k_head"], v["cos_head"]
         return c
 
     def items(self):
@@ -371,6 +385,102 @@ class Params:
             for j in range(i, 7):
                 out.append(f"a_{TYPE_NAMES[i]}_{TYPE_NAMES[j]} = {self.a[i][j]!r}")
         return "\n".join(out) + "\n"
+
+
+ARCHS = ("linear", "pi", "y", "bola")
+
+
+class MolTemplate:
+    """The lipid template of mol_build (cel/engine/oercel.c:479-576): bead
+    count, heads, bonds (i, j, rest length) and angle terms (i, j, k with j the
+    middle bead, kind 0: U = K (1 - cos), kind 1: U = K (cos - c0)^2), in the
+    reference's order, and the soup-init offsets (ix, iy) of each bead from
+    bead 0."""
+
+    def __init__(self, arch):
+        self.arch, self.mlen, self.nH = arch, 0, 0
+        self.bonds, self.angles = [], []
+        self.px, self.py = [], []
+
+    def bond(self, i, j, r0):
+        self.bonds.append((i, j, r0))
+
+    def angle(self, i, j, k, kind, K, c0):
+        if K != 0.0:  # zero terms are skipped
+            self.angles.append((i, j, k, kind, K, c0))
+
+    def chain(self, c, r0, ka):
+        for b in range(len(c) - 1):
+            self.bond(c[b], c[b + 1], r0)
+        for b in range(len(c) - 2):
+            self.angle(c[b], c[b + 1], c[b + 2], 0, ka, 0.0)
+
+
+def mol_template(p):
+    """mol_build, cel/engine/oercel.c:479-576, for a Params p."""
+    v = p.__dict__["_v"]
+    arch, nT, r0 = v["lip_arch"], v["lip_nT"], v["r0"]
+    M = MolTemplate(arch)
+    if arch == "linear":
+        M.nH, M.mlen = v["lip_nH"], v["lip_nH"] + nT
+        for b in range(M.mlen - 1):
+            M.bond(b, b + 1, r0)
+        for b in range(M.mlen - 2):
+            M.angle(b, b + 1, b + 2, 0, v["k_angle"], 0.0)
+        M.px = [(b - 0.5 * (M.mlen - 1)) * r0 for b in range(M.mlen)]
+        M.py = [0.0] * M.mlen
+    elif arch == "pi":
+        h = v["r0_head"]
+        M.nH, M.mlen = 2, 2 + 2 * nT
+        M.bond(0, 1, h)
+        ca = [0] + [1 + k for k in range(1, nT + 1)]
+        cb = [1] + [1 + nT + k for k in range(1, nT + 1)]
+        M.chain(ca, r0, v["k_angle"])
+        M.chain(cb, r0, v["k_angle"])
+        for m in range(1, min(v["lip_tie"], nT) + 1):
+            M.bond(ca[m], cb[m], h)
+        M.angle(1, 0, ca[1], 1, v["k_head"], v["cos_head"])
+        M.angle(0, 1, cb[1], 1, v["k_head"], v["cos_head"])
+        M.px, M.py = [0.0] * M.mlen, [0.0] * M.mlen
+        M.py[0], M.py[1] = -0.5 * h, 0.5 * h
+        for k in range(1, nT + 1):
+            M.px[ca[k]], M.py[ca[k]] = k * r0, -0.5 * h
+            M.px[cb[k]], M.py[cb[k]] = k * r0, 0.5 * h
+    elif arch == "y":
+        M.nH, M.mlen = 1, 1 + 2 * nT
+        ca = [0] + list(range(1, nT + 1))
+        cb = [0] + [nT + k for k in range(1, nT + 1)]
+        M.chain(ca, r0, v["k_angle"])
+        M.chain(cb, r0, v["k_angle"])
+        for m in range(1, min(v["lip_tie"], nT) + 1):
+            M.bond(ca[m], cb[m], 1.2 * r0)
+        M.angle(ca[1], 0, cb[1], 1, v["k_head"], v["cos_head"])
+        M.px, M.py = [0.0] * M.mlen, [0.0] * M.mlen
+        for k in range(1, nT + 1):
+            M.px[ca[k]], M.py[ca[k]] = 0.8 * r0 + (k - 1) * r0, -0.6 * r0
+            M.px[cb[k]], M.py[cb[k]] = 0.8 * r0 + (k - 1) * r0, 0.6 * r0
+    elif arch == "bola":
+        h = v["r0_head"]
+        M.nH, M.mlen = 3, 3 + 2 * nT
+        M.bond(0, 1, h)
+        ca = [0] + [2 + k for k in range(1, nT + 1)] + [2]
+        cb = [1] + [2 + nT + k for k in range(1, nT + 1)] + [2]
+        M.chain(ca, r0, v["k_angle"])
+        M.chain(cb, r0, v["k_angle"])
+        for m in range(1, min(v["lip_tie"], nT) + 1):
+            M.bond(ca[m], cb[m], h * (1.0 - m / (nT + 1)))
+        M.angle(1, 0, ca[1], 1, v["k_head"], v["cos_head"])
+        M.angle(0, 1, cb[1], 1, v["k_head"], v["cos_head"])
+        X = math.sqrt((nT + 1) * r0 * (nT + 1) * r0 - 0.25 * h * h)
+        M.px, M.py = [0.0] * M.mlen, [0.0] * M.mlen
+        M.py[0], M.py[1], M.px[2] = -0.5 * h, 0.5 * h, X
+        for k in range(1, nT + 1):
+            f = k / (nT + 1)
+            M.px[ca[k]], M.py[ca[k]] = f * X, -0.5 * h * (1.0 - f)
+            M.px[cb[k]], M.py[cb[k]] = f * X, 0.5 * h * (1.0 - f)
+    else:
+        raise ParamError("lip_arch must be linear, pi, y or bola")
+    return M
 
 
 def read_ini(path):

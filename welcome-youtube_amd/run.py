@@ -32,7 +32,7 @@ import time
 
 from . import io
 from .engine import Engine
-from .params import ParamError, load
+from .params import ParamError, load, mol_template
 
 USAGE = "usage: python -m welcome-youtube_amd.run <params.ini> <outdir> [key=value ...]"
 CHUNK = 100
@@ -139,10 +139,12 @@ def _run(p, outdir, logmsg):
             ef = open(epath, "w")
             ef.write(io.EVENTS_HEADER + "\n")
 
+    tbonds = [(b[0], b[1]) for b in mol_template(p).bonds]
+
     def snapshot(st):
         s = eng.get_state()
         io.write_snapshot(os.path.join(outdir, f"snap_{st:010d}.bin"), s, st, p.t_anchor + (st - p.step_anchor) * p.dt,
-                          p.L, p.mlen)
+                          p.L, p.mlen, tbonds)
 
     def flush_events():
         if ef is None:
