@@ -171,3 +171,25 @@ def test_batched_windows_equal_single(oracle, eng):
         assert l1[0] == lb[r] and np.array_equal(g1[0], gb[r]), r
         e.close()
     ens.close()
+
+
+def test_tiled_reverse_step_equals_bead_block_reverse_step(oracle, eng):
+    """The reverse step over LDS-staged pair tiles (used when there are at
+    least 1,024 tiles: here forced with 4-bead tiles) and over bead blocks
+    (global memory, four lanes per bead) give the same gradient up to the FP64
+    summation order, and the same loss (the forward window is the same)."""
+    abi = importlib.import_module("welcome-youtube_amd.abi")
+    p, st, s0 = system(oracle, L=40.0, seed=8, nsteps=40)
+    spec = abi.rdf_spec(2, 2, 32, 3.0)
+    tgt = np.full(32, 0.8)
+    out = []
+    for tb in (0, 4):
+        e = eng.Engine(p, tile_beads=tb)
+        e.set_state(st, step=s0)
+        out.append(e.window_grad(12, spec, tgt, ckpt_every=4))
+        e.close()
+    (l0, g0, m0), (l1, g1, m1) = out
+    say("bead blocks", g0[0])
+    say("tiles      ", g1[0])
+    assert np.array_equal(m0, m1) and l0[0] == l1[0]
+    assert np.abs(g0[0] - g1[0]).max() < 1e-9 * np.abs(g0[0]).max()

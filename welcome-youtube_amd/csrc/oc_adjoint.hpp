@@ -22,10 +22,10 @@ struct RdfArgs {
 
 constexpr double kFix = 4294967296.0;  // fixed-point scale of the RDF histogram (2^32)
 
-constexpr int kRdfMaxCol = 9;  // columns of the window (Rc <= 4)
-constexpr int kRdfChunk = 64;  // beads per block of the pair adjoint
+constexpr int kRdfChunk = 64;  // beads per block of the bead-block reverse step
 constexpr int kRdfSub = kBeadBlock / kRdfChunk;
-constexpr int kAdjG = 4;  // lanes per bead of the pair adjoint (blocks of kRdfChunk beads)
+constexpr int kAdjG = 4;  // lanes per bead of the bead-block reverse step
+constexpr int kRdfMaxCol = 9;  // columns of the window (Rc <= 4)
 constexpr int kTlG = 4;   // lanes per listed bead of the RDF kernels
 
 // One recorded state of the window gradient's tape, in that state's slot
@@ -366,10 +366,10 @@ __global__ __launch_bounds__(64) void k_step_params_seg(const RepInfo *__restric
 // force adjoint Fa,
 //   X_n = X + Jx^T Fa (pair and bond terms) + G_n, T_n = Jv^T Fa,
 //   theta_bar += Jtheta^T Fa (half of each pair: every pair is visited from
-//   both beads; summed per block in a fixed order),
+//   both beads; summed per tile in a fixed order),
 // then X_{n-1}, V_{n-1}, F_{n-1} scattered to the order of step n-1 through
 // the sort map (new slot i <- old slot src[i]).  Every term of bead i is
-// summed by its own group (gather form, no atomics).  Xo == nullptr: theta
+// summed by its own lane (gather form, no atomics).  Xo == nullptr: theta
 // gradient only (the window's first force).
 struct AdjStep {
     const RepStep *rs;  // step parameters of step n, per replica
@@ -380,9 +380,12 @@ struct AdjStep {
     const int8_t *mp;
     const double2 *Fa, *X, *V;
     double2 *Xo, *Vo, *Fo;
-    double *thp;  // per block of this kernel: running theta-gradient sums (8 doubles)
+    double *thp;  // per pair tile: running theta-gradient sums (8 doubles)
 };
 
+// One reverse step with kAdjG lanes per bead, bead blocks of kRdfChunk beads
+// (the launch for small systems, whose pair tiles are too few to fill the
+// chip): candidates read from global memory; theta summed per block.
 __global__ __launch_bounds__(kBeadBlock) void k_adj_rstep(const RepInfo *__restrict__ ri,
                                                           const int32_t *__restrict__ bblk, AdjStep A) {
     // block b: beads [(b % kRdfSub) * kRdfChunk, +kRdfChunk) of bead block
@@ -547,13 +550,250 @@ __global__ __launch_bounds__(kBeadBlock) void k_adj_rstep(const RepInfo *__restr
     A.Fo[od] = make_double2(R.dt2 * xx + R.cvt * ta + R.dt * vin.x, R.dt2 * xy_ + R.cvt * tb + R.dt * vin.y);
 }
 
-// per-replica theta gradient: the sum of the block partials of the replica's
-// blocks, strided then an LDS tree (fixed order, FP64)
-__global__ __launch_bounds__(256) void k_theta_reduce(const RepInfo *__restrict__ ri, const double *__restrict__ thp,
+// One reverse step per pair tile (the force kernel's tiles: a column x a
+// range of rows): the candidates of the tile's three columns -- position, v~,
+// gid|type and the force adjoint Fa of state n -- are staged in LDS once, each
+// lane takes one bead of the tile and sums the pair terms above over its
+// window (per column and periodic piece, with the image shifts) from LDS; a
+// tile whose candidates do not fit runs from global memory.  The bond terms,
+// the RDF gradient and the step back through the sort map follow per bead.
+// theta: each lane's sum over its beads, a fixed shuffle tree per wavefront,
+// the wavefronts in order, added to the tile's running sum thp[tile].
+template <int BLOCK, int CAP>
+__global__ __launch_bounds__(BLOCK) void k_adj_tile(const RepInfo *__restrict__ ri, const Tile *__restrict__ tiles,
+                                                    int ntiles, int per_xcd, AdjStep A, uint32_t *__restrict__ err) {
+    constexpr int kMaxRows = pair_max_rows(BLOCK);
+    __shared__ double2 s_xy[CAP];
+    __shared__ double2 s_fa[CAP];
+    __shared__ float2 s_vt[CAP];
+    __shared__ uint32_t s_gt[CAP];
+    __shared__ int32_t s_row[3][kMaxRows + 1];
+    __shared__ int32_t s_piece[3][3][3];  // [col][piece] = {first slot, length, staged start}
+    __shared__ int32_t s_col[3];
+    __shared__ int32_t s_tot;
+    __shared__ double s_th[BLOCK / 64][8];
+    const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (t >= ntiles) return;
+    const Tile T = tiles[t];
+    const RepInfo &R = ri[T.rep];
+    const RepStep &S = A.rs[T.rep];
+    const int ncy = R.ncy, ncx = R.ncx;
+    const int M = (int)ceil(R.icsy) + 1;  // staging margin in rows
+    const int kb0 = T.k0 - M;
+    const int nrow = T.k1 - T.k0 + 2 * M + 1;
+    const int32_t *__restrict__ cs = A.cs;
+    if (nrow > kMaxRows) {  // the host sizes tiles so that this cannot happen
+        if (threadIdx.x == 0) atomicOr(err, ERR_TILE);
+        return;
+    }
+    // pieces of the staged rows lo..hi of each column: A = rows < 0 (wrapped
+    // from the top), B = rows in the box, C = rows >= ncy (wrapped from the
+    // bottom); staged in (column, piece) order
+    if (threadIdx.x < 64) {
+        const int q = threadIdx.x;
+        const int lo = kb0, hi = T.k1 - 1 + M;
+        int f = 0, n = 0;
+        if (q < 9) {
+            const int c = q / 3, pc = q - 3 * c;
+            int col = T.cx - 1 + c;
+            col = col < 0 ? col + ncx : (col >= ncx ? col - ncx : col);
+            const int cb = R.coff + col * ncy;
+            int ra = 0, rb = -1;
+            if (pc == 0) {
+                if (lo < 0) { ra = lo + ncy; rb = ncy - 1; }
+            } else if (pc == 1) {
+                ra = max(lo, 0); rb = min(hi, ncy - 1);
+            } else if (hi >= ncy) {
+                ra = 0; rb = hi - ncy;
+            }
+            if (rb >= ra) {
+                f = cs[cb + ra];
+                n = cs[cb + rb + 1] - f;
+            }
+            if (pc == 0) s_col[c] = col;
+        }
+        int incl = n;
+        for (int o = 1; o < 16; o <<= 1) {
+            const int v = __shfl_up(incl, o, 64);
+            if (q >= o) incl += v;
+        }
+        if (q < 9) {
+            const int c = q / 3;
+            s_piece[c][q - 3 * c][0] = f;
+            s_piece[c][q - 3 * c][1] = n;
+            s_piece[c][q - 3 * c][2] = incl - n;
+        }
+        if (q == 8) s_tot = incl;
+    }
+    __syncthreads();
+    const int total = s_tot;
+    const bool stg = total <= CAP;
+    // s_row[c][k - kb0]: staged (or piece-relative) index of the first bead of
+    // unwrapped row k of column c
+    for (int e = threadIdx.x; e < 3 * nrow; e += BLOCK) {
+        const int c = e / nrow, k = kb0 + (e - c * nrow);
+        const int pc = k < 0 ? 0 : (k < ncy ? 1 : 2);
+        const int kk = k < 0 ? k + ncy : (k < ncy ? k : k - ncy);
+        s_row[c][e - c * nrow] = s_piece[c][pc][2] + (cs[R.coff + s_col[c] * ncy + kk] - s_piece[c][pc][0]);
+    }
+    if (stg)
+        for (int e = threadIdx.x; e < total; e += BLOCK) {
+            int w = 0;  // the last piece starting at or before e
+#pragma unroll
+            for (int u = 1; u < 9; u++) w = e >= s_piece[u / 3][u % 3][2] ? u : w;
+            const int src = s_piece[w / 3][w % 3][0] + (e - s_piece[w / 3][w % 3][2]);
+            s_xy[e] = A.xy[src];
+            s_vt[e] = A.vt[src];
+            s_gt[e] = A.gt[src];
+            s_fa[e] = A.Fa[src];
+        }
+    __syncthreads();
+
+    const double L = R.L;
+    const double sxc[3] = {T.cx == 0 ? -L : 0.0, 0.0, T.cx == ncx - 1 ? L : 0.0};
+    const float gam = S.gamma, sdt = S.sdt, sdg = S.sdt / (2.0f * S.gamma);
+    const int ibeg = s_row[1][T.k0 - kb0], iend = s_row[1][T.k1 - kb0];
+    const int slot0 = s_piece[1][1][0] + (ibeg - s_piece[1][1][2]);  // T.k0 lies in piece B of column 1
+    const bool step_back = A.Xo != nullptr;
+    double th[7] = {0, 0, 0, 0, 0, 0, 0};
+    auto run = [&](const double2 *XY, const float2 *VT, const uint32_t *GT, const double2 *FA) {
+        for (int ii = ibeg + (int)threadIdx.x; ii < iend; ii += BLOCK) {
+            const int i = slot0 + (ii - ibeg);
+            const int is = stg ? ii : i;
+            // the step-back inputs of bead i, loaded now so that they arrive
+            // during the pair loop
+            int pn = -1, pp = -1, od = 0;
+            double2 xin = make_double2(0.0, 0.0), gin = xin, vin = xin;
+            if (step_back) {
+                pn = A.lnk[i];
+                pp = A.mp[i] > 0 ? A.prv[i] : -1;
+                od = A.src[i];
+                xin = A.X[i];
+                gin = A.G[i];
+                vin = A.V[i];
+            }
+            const double2 pi = XY[is];
+            const float2 vi = VT[is];
+            const uint32_t gti = GT[is], ti = gti >> kGidBits, gi = gti & kGidMask;
+            const double2 Fi = FA[is];
+            int khi = (int)((pi.y + 1.0) * R.icsy);
+            int klo = (int)((pi.y - 1.0) * R.icsy + ncy) - ncy;
+            if (klo < kb0 || khi > T.k1 - 1 + M) {  // cannot happen for rows k0..k1-1 (margin M); guard
+                atomicOr(err, ERR_TILE);
+                klo = max(klo, kb0);
+                khi = min(khi, T.k1 - 1 + M);
+            }
+            double xa = 0.0, ya = 0.0, ta = 0.0, tb = 0.0;
+            auto one = [&](int j, double sx, double sy) {
+                const double2 pj = XY[j];
+                const float dx = (float)((pi.x - pj.x) - sx), dy = (float)((pi.y - pj.y) - sy);
+                const float r2 = dx * dx + dy * dy;
+                if (!(r2 < 1.0f && r2 > 1e-24f)) return;
+                const float ir = __builtin_amdgcn_rsqf(r2);
+                const float rr = r2 * ir, w = 1.0f - rr;
+                const float ex = dx * ir, ey = dy * ir;
+                const float2 vj = VT[j];
+                const float ux = vi.x - vj.x, uy = vi.y - vj.y;
+                const float eu = ex * ux + ey * uy;
+                const uint32_t gtj = GT[j], tj = gtj >> kGidBits;
+                const uint64_t hsh = pair_hash(S.sseed, gi, gtj & kGidMask);
+                const float thn = (float)(int32_t)(uint32_t)(hsh >> 32) * kTheta32;
+                const float a = S.a_eff[8 * ti + tj];
+                const float m = w * (a - gam * w * eu + sdt * thn);
+                const float dmr = -(a + sdt * thn) + 2.0f * gam * w * eu;
+                const float dmu = -gam * w * w;
+                const double2 Fj = FA[j];
+                const float lx = (float)(Fi.x - Fj.x), ly = (float)(Fi.y - Fj.y);
+                const float el = ex * lx + ey * ly;
+                const float pux = ux - eu * ex, puy = uy - eu * ey;
+                const float plx = lx - el * ex, ply = ly - el * ey;
+                xa += (double)((dmr * ex + dmu * pux * ir) * el + m * plx * ir);
+                ya += (double)((dmr * ey + dmu * puy * ir) * el + m * ply * ir);
+                ta += (double)(dmu * el * ex);
+                tb += (double)(dmu * el * ey);
+                const double ha = (double)(0.5f * w * el);
+                const int q = theta_index(ti, tj);
+                if (q == 0) th[0] += ha;
+                else if (q == 1) th[1] += ha;
+                else if (q == 2) th[2] += ha;
+                else if (q == 3) th[3] += ha;
+                else if (q == 4) th[4] += ha;
+                else if (q == 5) th[5] += ha;
+                th[6] += (double)(0.5f * el * (-w * w * eu + sdg * w * thn));
+            };
+            for (int c = 0; c < 3; c++) {
+                for (int q = 0; q < 3; q++) {
+                    int pa, pb;
+                    double sy;
+                    if (q == 0) { pa = klo; pb = min(khi, -1); sy = -L; }
+                    else if (q == 1) { pa = max(klo, 0); pb = min(khi, ncy - 1); sy = 0.0; }
+                    else { pa = max(klo, ncy); pb = khi; sy = L; }
+                    if (pa > pb) continue;
+                    const int adj = stg ? 0 : s_piece[c][q][0] - s_piece[c][q][2];
+                    const int ja = s_row[c][pa - kb0] + adj, jb = s_row[c][pb + 1 - kb0] + adj;
+                    for (int j = ja; j < jb; j++) one(j, sxc[c], sy);
+                }
+            }
+            if (!step_back) continue;
+            // harmonic bonds of bead i (FP64): the chain neighbours of a lipid
+            // bead, the chemistry partner of another bead; with d = x_i - x_p
+            // and l = Fa_i - Fa_p, X_i += g l + g' r (e.l) e,
+            // g = -kb (r - r0) / r, g' = -kb r0 / r^2
+            double2 qn = pi, fn = Fi, qp = pi, fp = Fi;
+            if (pn >= 0) {
+                qn = A.xy[pn];
+                fn = A.Fa[pn];
+            }
+            if (pp >= 0) {
+                qp = A.xy[pp];
+                fp = A.Fa[pp];
+            }
+            auto bond = [&](double2 qq, double2 Fp) {
+                const double dx = minimg(pi.x - qq.x, R.L, R.halfL), dy = minimg(pi.y - qq.y, R.L, R.halfL);
+                const double rr = sqrt(dx * dx + dy * dy);
+                const double g = -R.kb * (rr - R.r0) / rr, gp = -R.kb * R.r0 / (rr * rr);
+                const double ex = dx / rr, ey = dy / rr;
+                const double lx = Fi.x - Fp.x, ly = Fi.y - Fp.y;
+                const double el = ex * lx + ey * ly;
+                xa += g * lx + gp * rr * el * ex;
+                ya += g * ly + gp * rr * el * ey;
+            };
+            if (pp >= 0) bond(qp, fp);
+            if (pn >= 0) bond(qn, fn);
+            // + d loss / d x_n of the RDF, then the step back to n-1
+            const double xx = xin.x + xa + gin.x, xy_ = xin.y + ya + gin.y;
+            A.Xo[od] = make_double2(xx, xy_);
+            A.Vo[od] = make_double2(R.dt * xx + ta + vin.x, R.dt * xy_ + tb + vin.y);
+            A.Fo[od] = make_double2(R.dt2 * xx + R.cvt * ta + R.dt * vin.x, R.dt2 * xy_ + R.cvt * tb + R.dt * vin.y);
+        }
+    };
+    if (stg) run(s_xy, s_vt, s_gt, s_fa);
+    else run(A.xy, A.vt, A.gt, A.Fa);
+    // theta: fixed shuffle tree per wavefront, then the wavefronts in order
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        double v = th[q];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        th[q] = v;
+    }
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < 7; q++) s_th[threadIdx.x >> 6][q] = th[q];
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        double v = 0.0;
+        for (int w = 0; w < BLOCK / 64; w++) v += s_th[w][threadIdx.x];
+        A.thp[(size_t)t * 8 + threadIdx.x] += v;
+    }
+}
+
+// per-replica theta gradient: the sum of the tile partials of the replica's
+// tiles (trng[r] = first tile, count), strided then an LDS tree (fixed order,
+// FP64)
+__global__ __launch_bounds__(256) void k_theta_reduce(const int2 *__restrict__ trng, const double *__restrict__ thp,
                                                       double *__restrict__ out) {
     __shared__ double red[7][256];
-    const RepInfo &R = ri[blockIdx.x];
-    const int k0 = R.bblk0 * kRdfSub, nb = (R.N + kBeadBlock - 1) / kBeadBlock * kRdfSub;
+    const int k0 = trng[blockIdx.x].x, nb = trng[blockIdx.x].y;  // the replica's tiles
     double sq[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int k = threadIdx.x; k < nb; k += 256) {
         const double *t = thp + (size_t)(k0 + k) * 8;

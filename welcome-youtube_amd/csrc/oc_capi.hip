@@ -43,6 +43,9 @@ struct oc_engine {
     RepInfo *d_ri = nullptr;
     RepStep *d_rs = nullptr;
     int32_t *d_bblk = nullptr, *d_rblk = nullptr;
+    std::vector<int2> trng;     // per replica: first pair tile, tile count
+    int2 *d_trng = nullptr;
+    int2 *d_brng = nullptr;     // per replica: first bead-block partial of the reverse step, count
     Tile *d_tiles = nullptr, *d_stiles = nullptr;
     StateP st[2];
     int cur = 0;
@@ -163,8 +166,10 @@ static int make_repinfo(const oc_params &p, int32_t N, RepInfo &R) {
 static int build_tiles(oc_engine *h) {
     h->tiles.clear();
     const int target = h->tile_beads > 0 ? h->tile_beads : (h->pair_block * 9) / 10;
+    h->trng.assign(h->nrep, make_int2(0, 0));
     for (int r = 0; r < h->nrep; r++) {
         const RepInfo &R = h->ri[r];
+        h->trng[r].x = (int)h->tiles.size();
         const double per_row = (double)R.N / ((double)R.ncx * R.ncy);
         const int M = (int)ceil(R.icsy) + 1;
         int rows = (int)lround(target / std::max(per_row, 1e-9));
@@ -181,6 +186,7 @@ static int build_tiles(oc_engine *h) {
                 T.k1 = (int)(((int64_t)(t + 1) * R.ncy) / nt);
                 h->tiles.push_back(T);
             }
+        h->trng[r].y = (int)h->tiles.size() - h->trng[r].x;
     }
     h->per_xcd = (int)((h->tiles.size() + 7) / 8);
     if (h->d_tiles) {
@@ -190,6 +196,9 @@ static int build_tiles(oc_engine *h) {
     }
     if (int rc = dalloc(h, &h->d_tiles, h->tiles.size())) return rc;
     HIPCHK(hipMemcpy(h->d_tiles, h->tiles.data(), h->tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    if (!h->d_trng)
+        if (int rc = dalloc(h, &h->d_trng, h->nrep)) return rc;
+    HIPCHK(hipMemcpy(h->d_trng, h->trng.data(), h->nrep * sizeof(int2), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -557,6 +566,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     rc = rc ? rc : dalloc(h, &h->d_ri, nrep);
     rc = rc ? rc : dalloc(h, &h->d_rs, nrep);
     rc = rc ? rc : dalloc(h, &h->d_bblk, h->bblk.size());
+    rc = rc ? rc : dalloc(h, &h->d_brng, nrep);
     rc = rc ? rc : dalloc(h, &h->d_stiles, h->stiles.size());
     rc = rc ? rc : dalloc(h, &h->d_rblk, h->rblk.size());
     for (int k = 0; k < 2; k++) {
@@ -606,6 +616,12 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     e = e ? e : hipMemcpy(h->d_ri, h->ri.data(), nrep * sizeof(RepInfo), hipMemcpyHostToDevice);
     e = e ? e : hipMemcpy(h->d_rs, rs.data(), nrep * sizeof(RepStep), hipMemcpyHostToDevice);
     e = e ? e : hipMemcpy(h->d_bblk, h->bblk.data(), h->bblk.size() * 4, hipMemcpyHostToDevice);
+    {
+        std::vector<int2> br(nrep);
+        for (int r = 0; r < nrep; r++)
+            br[r] = make_int2(h->ri[r].bblk0 * kRdfSub, grid_of(h->ri[r].N, kBeadBlock) * kRdfSub);
+        e = e ? e : hipMemcpy(h->d_brng, br.data(), nrep * sizeof(int2), hipMemcpyHostToDevice);
+    }
     e = e ? e : hipMemcpy(h->d_stiles, h->stiles.data(), h->stiles.size() * sizeof(Tile), hipMemcpyHostToDevice);
     e = e ? e : hipMemcpy(h->d_rblk, h->rblk.data(), h->rblk.size() * 4, hipMemcpyHostToDevice);
     e = e ? e : hipMemset(h->d_evn, 0, sizeof(unsigned long long));
@@ -636,6 +652,7 @@ OC_API int oc_set_tuning(oc_engine *h, int32_t pair_block, int32_t tile_beads) {
     h->pair_block = pair_block ? pair_block : 256;
     h->tile_beads = tile_beads > 0 ? tile_beads : 0;
     drop_graphs(h);
+    adj_free(h);  // the window gradient's per-tile sums follow the tiles
     return build_tiles(h);
 }
 
@@ -1196,6 +1213,32 @@ static void step_into(oc_engine *h, const StateSnap &t, bool full) {
     enqueue_step(h);
 }
 
+// The reverse step runs over the pair tiles when they fill the chip (at
+// least kAdjTileMin of them), else over bead blocks with four lanes per bead.
+constexpr int kAdjTileMin = 1024;
+static bool adj_tiled(const oc_engine *h) { return (int)h->tiles.size() >= kAdjTileMin; }
+
+// one reverse step over the pair tiles (their shape follows the pair block)
+static void launch_adj_tile(oc_engine *h, const AdjStep &A) {
+    if (!adj_tiled(h)) {
+        k_adj_rstep<<<(int)h->bblk.size() * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, A);
+        return;
+    }
+    const dim3 grid(8 * h->per_xcd);
+    const int nt = (int)h->tiles.size();
+    switch (h->pair_block) {
+    case 64:
+        k_adj_tile<64, 224><<<grid, 64, 0, h->stream>>>(h->d_ri, h->d_tiles, nt, h->per_xcd, A, h->d_err);
+        break;
+    case 128:
+        k_adj_tile<128, 448><<<grid, 128, 0, h->stream>>>(h->d_ri, h->d_tiles, nt, h->per_xcd, A, h->d_err);
+        break;
+    default:
+        k_adj_tile<256, 896><<<grid, 256, 0, h->stream>>>(h->d_ri, h->d_tiles, nt, h->per_xcd, A, h->d_err);
+        break;
+    }
+}
+
 static int adj_setup(oc_engine *h, int K, int nseg, int nbins) {
     if (h->adj && h->adj->K == K && h->adj->nseg == nseg && h->adj->nbins == nbins) return 0;
     adj_free(h);
@@ -1229,7 +1272,7 @@ static int adj_setup(oc_engine *h, int K, int nseg, int nbins) {
         rc = rc ? rc : aalloc(a, &a->F[k], N);
     }
     rc = rc ? rc : aalloc(a, &a->rsa, (size_t)K * h->nrep);
-    rc = rc ? rc : aalloc(a, &a->thp, h->bblk.size() * (size_t)kRdfSub * 8);
+    rc = rc ? rc : aalloc(a, &a->thp, std::max(h->tiles.size(), h->bblk.size() * (size_t)kRdfSub) * 8);
     rc = rc ? rc : aalloc(a, &a->gw, (size_t)h->nrep * nbins);
     rc = rc ? rc : aalloc(a, &a->grad, (size_t)h->nrep * 8);
     rc = rc ? rc : aalloc(a, &a->rpart, h->bblk.size() * (size_t)K * nbins);
@@ -1377,7 +1420,7 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
     const RdfArgs RA = a->RA;
     const int nbk = (int)h->bblk.size(), ncol = (int)h->cols.size();
     const size_t N = h->Ntot;
-    zero_f64(h, a->thp, h->bblk.size() * (size_t)kRdfSub * 8);
+    zero_f64(h, a->thp, std::max(h->tiles.size(), h->bblk.size() * (size_t)kRdfSub) * 8);
     int ab = 0;  // adjoint buffer parity
     const EngBufs eb = save_bufs(h);
     for (int seg = nseg - 1; seg >= 0; seg--) {
@@ -1408,7 +1451,7 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
             const StateSnap &t = a->tape[m];
             const AdjStep A = {a->rsa + (size_t)(m - 1) * h->nrep, t.cs, t.src, t.lnk, t.prv, t.xy, t.rg, t.vt, t.gt,
                                t.mp, a->F[ab], a->X[ab], a->V[ab], a->X[ab ^ 1], a->V[ab ^ 1], a->F[ab ^ 1], a->thp};
-            k_adj_rstep<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, A);
+            launch_adj_tile(h, A);
             ab ^= 1;
         }
     }
@@ -1418,9 +1461,9 @@ OC_API int oc_window_backward(oc_engine *h, const double *dL_dgbar, double *grad
         k_step_params_seg<<<dim3(h->nrep, 1), 64, 0, h->stream>>>(h->d_ri, a->rsa, 0, a->base);
         const AdjStep A = {a->rsa, t.cs, nullptr, t.lnk, nullptr, t.xy, nullptr, t.vt, t.gt, t.mp, a->F[ab], a->X[ab],
                            a->V[ab], nullptr, nullptr, nullptr, a->thp};
-        k_adj_rstep<<<nbk * kRdfSub, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_bblk, A);
+        launch_adj_tile(h, A);
     }
-    k_theta_reduce<<<h->nrep, 256, 0, h->stream>>>(h->d_ri, a->thp, a->grad);
+    k_theta_reduce<<<h->nrep, 256, 0, h->stream>>>(adj_tiled(h) ? h->d_trng : h->d_brng, a->thp, a->grad);
     // the engine continues from the window end
     if (int rc = snap_copy(h, a->end, false)) return rc;
     launch_build_prv(h);
