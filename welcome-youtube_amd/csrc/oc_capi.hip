@@ -156,6 +156,10 @@ static int make_repinfo(const oc_params &p, int32_t N, RepInfo &R) {
     R.rc2 = p.r_conv * p.r_conv;
     R.p_form = p.p_form;
     R.rf2 = p.r_form * p.r_form;
+    // the rows a search visits: within r + 1e-6 of the bead (the margin covers
+    // the FP64 rounding of the row bounds; beads further off fail the test)
+    R.rwc = p.r_conv + 1e-6;
+    R.rwf = p.r_form + 1e-6;
     R.p_break = p.p_break;
     for (int i = 0; i < 8; i++)
         for (int j = 0; j < 8; j++) R.a[8 * i + j] = (i < OC_NTYPES && j < OC_NTYPES) ? p.a[i][j] : 0.0;

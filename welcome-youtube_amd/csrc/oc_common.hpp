@@ -34,6 +34,7 @@ struct RepInfo {
     int64_t step_anchor, warmup, warm_end;
     uint64_t seed;
     double p_conv, rc2, p_form, rf2, p_break;
+    double rwc, rwf;  // half-heights of the chemistry search windows (r_conv, r_form + 1e-6)
     double a[64];
     // lipid template of a branched architecture (arch != 0): bonds (bi, bj,
     // rest br0) and angle terms (ai, aj = middle, ak; kind 0: K (1 - cos),

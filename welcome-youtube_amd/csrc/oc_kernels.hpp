@@ -1474,8 +1474,8 @@ __device__ __forceinline__ double dist2(const RepInfo &R, double2 p, double2 q) 
 }
 
 // The cell window of a point (cutoff <= 1) as up to six contiguous slot
-// ranges: per column (x-1, x, x+1) the rows of [y-1, y+1], split at the
-// periodic seam.  slot(k) is the k-th candidate in that order.
+// ranges: per column (x-1, x, x+1) the rows of [y-rw, y+rw] (rw <= 1 + 1e-6),
+// split at the periodic seam.  slot(k) is the k-th candidate in that order.
 struct Win {
     int first[6], cnt[6], total;
     __device__ __forceinline__ int slot(int k) const {
@@ -1488,11 +1488,11 @@ struct Win {
     }
 };
 
-__device__ __forceinline__ Win window_of(const RepInfo &R, const int32_t *cs, double2 p) {
+__device__ __forceinline__ Win window_of(const RepInfo &R, const int32_t *cs, double2 p, double rw) {
     Win W;
     const int cx = cell_x(R, p.x);
-    const int khi = (int)((p.y + 1.0) * R.icsy);
-    const int klo = (int)((p.y - 1.0) * R.icsy + R.ncy) - R.ncy;
+    const int khi = (int)((p.y + rw) * R.icsy);
+    const int klo = (int)((p.y - rw) * R.icsy + R.ncy) - R.ncy;
     W.total = 0;
 #pragma unroll
     for (int dc = 0; dc < 3; dc++) {
@@ -1597,7 +1597,7 @@ __global__ __launch_bounds__(kChemThreads) void k_chem_decide(const RepInfo *__r
         const int s = s_srch[e] & 0x7fffffff;
         const uint32_t g = gt[s] & kGidMask, ty = gt[s] >> kGidBits;
         const double2 p = xy[s];
-        const Win W = window_of(R, cs, p);
+        const Win W = window_of(R, cs, p, ty == OC_N ? R.rwc : R.rwf);
         bool found = false;
         uint64_t bkey = ~0ull;  // (bits of u) of the best accepted draw
         uint32_t bg = 0xffffffffu;
