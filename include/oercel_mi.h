@@ -44,8 +44,10 @@ enum {
     OC_ESTATE = 5    /* call made in the wrong state (e.g. step before any state) */
 };
 
-/* event kinds of the precursor chemistry (no counterpart in the reference) */
-enum { OC_EV_CONV = 0, OC_EV_FORM = 1, OC_EV_BREAK = 2 };
+/* event kinds: the precursor chemistry (no counterpart in the reference), then
+ * the reference chemistry's decay and food supply (its dlip, dN and feed
+ * events, cel/engine/oercel.c:658-659; feed is reported per converted bead) */
+enum { OC_EV_CONV = 0, OC_EV_FORM = 1, OC_EV_BREAK = 2, OC_EV_DLIP = 3, OC_EV_DN = 4, OC_EV_FEED = 5 };
 
 /*
  * Parameters of one replica: the numeric subset of the reference's Params
@@ -88,7 +90,21 @@ typedef struct oc_params {
     int32_t lip_tie;           /* pi / y / bola: ladder rungs between the two tails */
     double r0_head;            /* pi / bola: rest length of the head-head bond */
     double k_head, cos_head;   /* junction angle terms U = k_head (cos - cos_head)^2 (0 = none) */
+    /* the reference's chemistry without catalysts (its chem = 1 when no Ka / Kb
+     * bead exists, cel/engine/oercel.c:1656-1753): at every check, lipid decay
+     * (a lipid becomes water, p_ld per molecule), nutrient decay (N -> W, p_nd),
+     * then the food supply (every eligible W -> N, p_feed) -- per-bead draws of
+     * hash_u01 with salts 11, 13, 14 */
+    int32_t rchem;             /* 1 = on */
+    int32_t feed_mode;         /* OC_FEED_NONE / UNIFORM / VENT */
+    double p_ld, p_nd, p_feed;
+    int32_t vent_shape;        /* OC_VENT_DISK / VSTRIP / HSTRIP */
+    int32_t pad2;
+    double vent_x, vent_y, vent_r, vent_w;  /* fixed vent region (feed = vent) */
 } oc_params;
+
+enum { OC_FEED_NONE = 0, OC_FEED_UNIFORM = 1, OC_FEED_VENT = 2 };
+enum { OC_VENT_DISK = 0, OC_VENT_VSTRIP = 1, OC_VENT_HSTRIP = 2 };
 
 enum { OC_ARCH_LINEAR = 0, OC_ARCH_PI = 1, OC_ARCH_Y = 2, OC_ARCH_BOLA = 3 };
 
@@ -108,12 +124,13 @@ typedef struct oc_state {
     int32_t *bond;
 } oc_state;
 
-/* one chemistry event; kind OC_EV_CONV has gid_b = gid_a */
+/* one chemistry event; kinds OC_EV_CONV, DN, FEED have gid_b = gid_a */
 typedef struct oc_event {
     int64_t step;
     int32_t kind;
-    uint32_t gid_a, gid_b;
+    uint32_t gid_a, gid_b;     /* the beads (decay, feed: gid_a = gid_b = the bead; dlip: its first bead) */
     int32_t pad;
+    double x, y;               /* position of gid_a at the decision (decay and feed events; 0 otherwise) */
 } oc_event;
 
 /* stats of one replica (cel/engine/oercel.c:2053-2071 kinetic_T/count_types,

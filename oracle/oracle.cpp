@@ -739,6 +739,76 @@ OR_API int64_t or_chem(const oc_params *p, int32_t N, int64_t step, const double
     return n;
 }
 
+/* ------------------------------------------------------------------ reference chemistry without catalysts */
+
+/*
+ * The reference's chemistry check (cel/engine/oercel.c:1563-1753) when no
+ * catalyst exists: the catalyst loop is empty, so a check is the decay pass
+ * (:1656-1694; a lipid whose first bead draws hash_u01(kl, gid) < p_ld becomes
+ * water along its chain, an N with hash_u01(kn, gid) < p_nd becomes W;
+ * decisions collected first, then applied) followed by the food supply
+ * (:1711-1753; every W, or every W inside the vent region, with
+ * hash_u01(kf, gid) < p_feed becomes N; no cap).  Keys hash_key(seed, step,
+ * 11 / 13 / 14).  Events (dlip: first bead; dN; feed: one per bead) with the
+ * bead's position, sorted by (step, kind, gid_a, gid_b).
+ */
+static bool in_vent(const oc_params *p, double x, double y) {
+    const double L = p->L;
+    if (p->vent_shape == OC_VENT_DISK) {
+        const double dx = minimg(x - p->vent_x, L), dy = minimg(y - p->vent_y, L);
+        return dx * dx + dy * dy < p->vent_r * p->vent_r;
+    }
+    if (p->vent_shape == OC_VENT_VSTRIP) return fabs(minimg(x - p->vent_x, L)) < 0.5 * p->vent_w;
+    return fabs(minimg(y - p->vent_y, L)) < 0.5 * p->vent_w;
+}
+
+OR_API int64_t or_rchem(const oc_params *p, int32_t N, int64_t step, const double *x, const double *y, uint8_t *type,
+                        int8_t *mpos, int32_t *lnk, const uint32_t *gid, oc_event *ev, int64_t cap) {
+    if (!p->rchem || p->n_chem < 1 || step % p->n_chem != 0) return 0;
+    const Tmpl M = make_tmpl(p);
+    const uint64_t kl = hash_key(p->seed, step, 11), kn = hash_key(p->seed, step, 13),
+                   kf = hash_key(p->seed, step, 14);
+    std::vector<oc_event> out;
+    std::vector<int> dl;
+    for (int d = 0; d < N; d++)
+        if ((mpos[d] == 0 && p->p_ld > 0 && hash_u01(kl, gid[d]) < p->p_ld) ||
+            (type[d] == OC_N && p->p_nd > 0 && hash_u01(kn, gid[d]) < p->p_nd))
+            dl.push_back(d);
+    for (int d : dl) {
+        if (mpos[d] == 0) {
+            oc_event e{step, OC_EV_DLIP, gid[d], gid[d], 0, x[d], y[d]};
+            int c = d, n = 0;
+            while (c >= 0 && n < M.mlen) {
+                const int nx = lnk[c];
+                type[c] = OC_W;
+                mpos[c] = -1;
+                lnk[c] = -1;
+                c = nx;
+                n++;
+            }
+            out.push_back(e);
+        } else {
+            type[d] = OC_W;
+            out.push_back(oc_event{step, OC_EV_DN, gid[d], gid[d], 0, x[d], y[d]});
+        }
+    }
+    if (p->feed_mode != OC_FEED_NONE && p->p_feed > 0)
+        for (int d = 0; d < N; d++)
+            if (type[d] == OC_W && (p->feed_mode == OC_FEED_UNIFORM || in_vent(p, x[d], y[d])) &&
+                hash_u01(kf, gid[d]) < p->p_feed) {
+                type[d] = OC_N;
+                out.push_back(oc_event{step, OC_EV_FEED, gid[d], gid[d], 0, x[d], y[d]});
+            }
+    std::sort(out.begin(), out.end(), [](const oc_event &a, const oc_event &b) {
+        if (a.kind != b.kind) return a.kind < b.kind;
+        if (a.gid_a != b.gid_a) return a.gid_a < b.gid_a;
+        return a.gid_b < b.gid_b;
+    });
+    const int64_t n = (int64_t)out.size();
+    for (int64_t k = 0; k < n && k < cap; k++) ev[k] = out[k];
+    return n;
+}
+
 /* ------------------------------------------------------------------ system (composition of the above) */
 
 struct OrSys {
@@ -868,6 +938,12 @@ OR_API void or_step(OrSys *s, int64_t n, int32_t theta32) {
         or_integrate(&s->p, s->N, s->x.data(), s->y.data(), s->vx.data(), s->vy.data(), s->fx.data(), s->fy.data(),
                      s->vtx.data(), s->vty.data());
         sys_sort(s);
+        if (s->p.rchem) {
+            std::vector<oc_event> tmp(s->N + 16);
+            const int64_t ne = or_rchem(&s->p, s->N, ns, s->x.data(), s->y.data(), s->type.data(), s->mpos.data(),
+                                        s->lnk.data(), s->gid.data(), tmp.data(), (int64_t)tmp.size());
+            s->ev.insert(s->ev.end(), tmp.begin(), tmp.begin() + std::min<int64_t>(ne, (int64_t)tmp.size()));
+        }
         if (s->p.chem) {
             std::vector<oc_event> tmp(s->N * 2 + 16);
             const int64_t ne = or_chem(&s->p, s->N, ns, s->x.data(), s->y.data(), s->type.data(), s->mpos.data(),
@@ -879,8 +955,15 @@ OR_API void or_step(OrSys *s, int64_t n, int32_t theta32) {
     }
 }
 
-/* one chemistry check on the loaded (sorted) state, events appended */
+/* one chemistry check on the loaded (sorted) state (the reference chemistry's
+ * decay and feed, then the precursor chemistry), events appended */
 OR_API int64_t or_sys_chem(OrSys *s, int64_t step) {
+    if (s->p.rchem) {
+        std::vector<oc_event> tmp(s->N + 16);
+        const int64_t ne = or_rchem(&s->p, s->N, step, s->x.data(), s->y.data(), s->type.data(), s->mpos.data(),
+                                    s->lnk.data(), s->gid.data(), tmp.data(), (int64_t)tmp.size());
+        s->ev.insert(s->ev.end(), tmp.begin(), tmp.begin() + std::min<int64_t>(ne, (int64_t)tmp.size()));
+    }
     std::vector<oc_event> tmp(s->N * 2 + 16);
     const int64_t ne = or_chem(&s->p, s->N, step, s->x.data(), s->y.data(), s->type.data(), s->mpos.data(),
                                s->lnk.data(), s->gid.data(), s->cstart.data(), tmp.data(), (int64_t)tmp.size());
@@ -898,7 +981,14 @@ OR_API void or_sys_forces(OrSys *s, int64_t step, int32_t theta32, int32_t with_
                   s->fy.data(), &s->maxbond, nullptr);
 }
 
+/* pending events, sorted by (step, kind, gid_a, gid_b) as the device reports them */
 OR_API int64_t or_get_events(OrSys *s, oc_event *ev, int64_t cap) {
+    std::stable_sort(s->ev.begin(), s->ev.end(), [](const oc_event &a, const oc_event &b) {
+        if (a.step != b.step) return a.step < b.step;
+        if (a.kind != b.kind) return a.kind < b.kind;
+        if (a.gid_a != b.gid_a) return a.gid_a < b.gid_a;
+        return a.gid_b < b.gid_b;
+    });
     const int64_t n = (int64_t)s->ev.size();
     for (int64_t k = 0; k < n && k < cap; k++) ev[k] = s->ev[k];
     s->ev.clear();

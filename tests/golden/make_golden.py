@@ -20,8 +20,10 @@ Fixtures (numpy .npz / JSON, small systems):
   params.json       the reference's --params table: keys, defaults, a_ij
   templates.npz     branched lipids (pi, y, bola): soups, bonded forces of
                     distorted lipids, and a 20-step CLI trajectory of pi lipids
+  rchem.npz         a 20-step CLI run with the reference chemistry on and no
+                    catalysts (decay, vent feed): checkpoint and events.csv
 
-`python tests/golden/make_golden.py templates` writes templates.npz only.
+`python tests/golden/make_golden.py templates` (or `rchem`) writes that file only.
 """
 import ctypes as C
 import json
@@ -125,6 +127,29 @@ def templates():
     np.savez_compressed(os.path.join(HERE, "templates.npz"), **out)
 
 
+def rchem():
+    """The reference CLI with its chemistry on and no catalysts: lipid and N
+    decay, vent feed (no cap), checks every 5 steps, 20 steps; its checkpoint
+    and events.csv."""
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        ov = dict(L="16", dt="0.01", f_lip="0.3", lip_nT="2", c_N="0.2", f_Ka="0", f_Kb="0", k_angle="5",
+                  warmup="0", seed="6", nsub="4", chem="1", n_chem="5", p_ld="0.05", p_nd="0.05", feed="vent",
+                  vent_shape="vstrip", vent_x="4", vent_w="6", p_feed="0.2", feed_cap="0", steps="20",
+                  snap_every="0", stats_every="0", ckpt_every="0", log_every="0", events="1", threads="2")
+        oracle.run_ref_cli(os.path.join(tmp, "rc"), ov)
+        ck = oracle.read_ckpt(os.path.join(tmp, "rc", "ckpt.bin"))
+        g = oracle.ckpt_gid_order(ck)
+        for k in ("x", "y", "vx", "vy", "fx", "fy", "type", "mpos", "nxt"):
+            out[f"rc_{k}"] = g[k]
+        out["rc_slot_gid"] = ck["gid"]
+        out["rc_step"] = np.array(ck["step"])
+        out["rc_overrides"] = np.array(json.dumps(ov))
+        with open(os.path.join(tmp, "rc", "events.csv")) as f:
+            out["rc_events"] = np.array(f.read())
+    np.savez_compressed(os.path.join(HERE, "rchem.npz"), **out)
+
+
 def main():
     if not oracle.ref_available():
         sys.exit("oracle/_ref is not built: make -C oracle ref")
@@ -203,13 +228,14 @@ def main():
     with open(os.path.join(HERE, "params.json"), "w") as f:
         json.dump(tab, f, indent=1)
     templates()
+    rchem()
     print("golden fixtures written to", HERE)
 
 
 if __name__ == "__main__":
-    if sys.argv[1:] == ["templates"]:
+    if sys.argv[1:] in (["templates"], ["rchem"]):
         if not oracle.ref_available():
             sys.exit("oracle/_ref is not built: make -C oracle ref")
-        templates()
+        templates() if sys.argv[1] == "templates" else rchem()
     else:
         main()

@@ -36,6 +36,9 @@ class oc_params(C.Structure):
         ("t_anchor", C.c_double), ("step_anchor", C.c_int64),
         ("lip_arch", C.c_int32), ("lip_tie", C.c_int32), ("r0_head", C.c_double), ("k_head", C.c_double),
         ("cos_head", C.c_double),
+        ("rchem", C.c_int32), ("feed_mode", C.c_int32), ("p_ld", C.c_double), ("p_nd", C.c_double),
+        ("p_feed", C.c_double), ("vent_shape", C.c_int32), ("pad2", C.c_int32), ("vent_x", C.c_double),
+        ("vent_y", C.c_double), ("vent_r", C.c_double), ("vent_w", C.c_double),
     ]
 
 
@@ -51,7 +54,7 @@ class oc_state(C.Structure):
 
 class oc_event(C.Structure):
     _fields_ = [("step", C.c_int64), ("kind", C.c_int32), ("gid_a", C.c_uint32), ("gid_b", C.c_uint32),
-                ("pad", C.c_int32)]
+                ("pad", C.c_int32), ("x", C.c_double), ("y", C.c_double)]
 
 
 class oc_stats(C.Structure):
@@ -80,7 +83,8 @@ def event_dtype():
     import numpy as np
     global EVENT_DTYPE
     if EVENT_DTYPE is None:
-        EVENT_DTYPE = np.dtype([("step", "<i8"), ("kind", "<i4"), ("gid_a", "<u4"), ("gid_b", "<u4"), ("pad", "<i4")])
+        EVENT_DTYPE = np.dtype([("step", "<i8"), ("kind", "<i4"), ("gid_a", "<u4"), ("gid_b", "<u4"), ("pad", "<i4"),
+                                ("x", "<f8"), ("y", "<f8")])
     return EVENT_DTYPE
 
 

@@ -73,6 +73,7 @@ struct oc_engine {
     hipGraph_t graph[2] = {nullptr, nullptr};
     bool have_state = false, any_stir = false, any_chem = false, any_angle = false;
     bool any_tmpl = false;  // some replica has branched lipids (k_mol; the force kernel leaves no keys)
+    bool any_rchem = false; // some replica runs the reference chemistry (decay, feed)
     std::vector<std::vector<oc_event>> evq;
     std::vector<void *> allocs;
     bool events_dropped = false;
@@ -160,6 +161,18 @@ static int make_repinfo(const oc_params &p, int32_t N, RepInfo &R) {
     // the FP64 rounding of the row bounds; beads further off fail the test)
     R.rwc = p.r_conv + 1e-6;
     R.rwf = p.r_form + 1e-6;
+    R.rchem = p.rchem;
+    R.feed_mode = p.feed_mode;
+    R.vent_shape = p.vent_shape;
+    R.p_ld = p.p_ld;
+    R.p_nd = p.p_nd;
+    R.p_feed = p.p_feed;
+    R.vent_x = p.vent_x;
+    R.vent_y = p.vent_y;
+    R.vent_r2 = p.vent_r * p.vent_r;
+    R.vent_hw = 0.5 * p.vent_w;
+    if (p.rchem && (p.feed_mode < 0 || p.feed_mode > 2 || p.vent_shape < 0 || p.vent_shape > 2))
+        return fail(OC_EINVAL, "bad feed mode or vent shape");
     R.p_break = p.p_break;
     for (int i = 0; i < 8; i++)
         for (int j = 0; j < 8; j++) R.a[8 * i + j] = (i < OC_NTYPES && j < OC_NTYPES) ? p.a[i][j] : 0.0;
@@ -296,8 +309,18 @@ static void launch_build_prv(oc_engine *h) {
 }
 
 static void launch_chem(oc_engine *h) {
-    if (!h->any_chem) return;
     const int c = h->cur;
+    if (h->any_rchem) {
+        // the reference chemistry's decay, then its food supply (before the
+        // precursor chemistry of the same check)
+        const int nb = (int)h->bblk.size();
+        k_rdecay<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy, h->st[c].gt,
+                                                   h->st[c].mp, h->st[c].lnk, h->d_ev, h->d_evn, h->evcap, h->record,
+                                                   h->d_err);
+        k_rfeed<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy, h->st[c].gt, h->d_ev,
+                                                  h->d_evn, h->evcap, h->record, h->d_err);
+    }
+    if (!h->any_chem) return;
     const int nk = (int)h->rblk.size();
     k_chem_decide<<<nk, kChemThreads, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_rblk, h->d_cs[c], h->st[c].xy,
                                                       h->st[c].gt, h->st[c].mp, h->st[c].lnk, h->d_flag, h->d_best,
@@ -556,6 +579,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
         h->any_chem |= R.chem != 0;
         h->any_angle |= R.ka != 0.0 && R.arch == 0;
         h->any_tmpl |= R.arch != 0;
+        h->any_rchem |= R.rchem != 0;
     }
     h->Ntot = off;
     h->Ctot = coff;
@@ -731,6 +755,9 @@ static int load_replica(oc_engine *h, int r, const oc_state *s, size_t base, boo
         const size_t k = base + g;
         const int ty = s->type[k];
         if (ty < 0 || ty >= OC_NTYPES) return fail(OC_EINVAL, "state: bad type");
+        if (R.rchem && (ty == OC_KA || ty == OC_KB))
+            return fail(OC_EINVAL, "chem = 1 with catalyst beads (Ka / Kb) needs the catalyst chemistry, "
+                                   "which this engine does not implement");
         xy[d] = make_double2(s->x[k], s->y[k]);
         v[d] = make_double2(s->vx[k], s->vy[k]);
         f[d] = make_double2(s->fx ? s->fx[k] : 0.0, s->fy ? s->fy[k] : 0.0);
@@ -957,6 +984,8 @@ static int drain_events(oc_engine *h) {
             o.gid_a = e.gid_a;
             o.gid_b = e.gid_b;
             o.pad = 0;
+            o.x = e.x;
+            o.y = e.y;
             h->evq[e.rep].push_back(o);
         }
     }

@@ -35,6 +35,9 @@ struct RepInfo {
     uint64_t seed;
     double p_conv, rc2, p_form, rf2, p_break;
     double rwc, rwf;  // half-heights of the chemistry search windows (r_conv, r_form + 1e-6)
+    // the reference chemistry without catalysts: decay and food supply
+    int32_t rchem, feed_mode, vent_shape, rpad;
+    double p_ld, p_nd, p_feed, vent_x, vent_y, vent_r2, vent_hw;
     double a[64];
     // lipid template of a branched architecture (arch != 0): bonds (bi, bj,
     // rest br0) and angle terms (ai, aj = middle, ak; kind 0: K (1 - cos),
@@ -51,11 +54,12 @@ struct RepStep {
     double t;
     uint64_t sseed;           // pair-noise key of this step
     uint64_t kconv, kform, kbreak;  // chemistry keys
+    uint64_t kld, knd, kfeed;       // reference chemistry keys (salts 11, 13, 14)
     float a_eff[64];          // repulsion at this step's warm-up ramp
     float gamma, sdt;         // gamma, sigma(t)/sqrt(dt)
     int32_t Rx, Ry;           // sort window radii (cells) of the last integrate
-    int32_t chem_now;
-    int32_t pad;
+    int32_t chem_now;         // precursor chemistry check at this step
+    int32_t rchem_now;        // reference chemistry (decay, feed) check at this step
     double stir_mean;
     uint64_t maxbond_bits;    // running max bond length (bits of a positive double)
 };
@@ -65,6 +69,7 @@ struct Event {
     int32_t kind;
     uint32_t gid_a, gid_b;
     int32_t rep;
+    double x, y;
 };
 
 // Pair tile: replica, column, sub-cell rows [k0, k1).

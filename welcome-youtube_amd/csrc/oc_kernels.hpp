@@ -93,6 +93,11 @@ __device__ __forceinline__ void step_params_wave(const RepInfo &R, RepStep &S, i
     S.sdt = (float)(sqrt(2.0 * R.gamma * kTt) / sqrt(R.dt));
     S.gamma = (float)R.gamma;
     S.chem_now = (R.chem && R.n_chem > 0 && (st % R.n_chem) == 0) ? 1 : 0;
+    S.rchem_now = (R.rchem && R.n_chem > 0 && (st % R.n_chem) == 0) ? 1 : 0;
+    // decay and feed keys, cel/engine/oercel.c:1664-1665, 1724
+    S.kld = hash_key(R.seed, st, 11);
+    S.knd = hash_key(R.seed, st, 13);
+    S.kfeed = hash_key(R.seed, st, 14);
 }
 
 __global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs,
@@ -1672,15 +1677,94 @@ __global__ __launch_bounds__(kChemThreads) void k_chem_decide(const RepInfo *__r
 }
 
 __device__ __forceinline__ void push_event(Event *ev, unsigned long long *evn, int64_t cap, uint32_t *err, int64_t step,
-                                           int kind, uint32_t a, uint32_t b, int rep) {
+                                           int kind, uint32_t a, uint32_t b, int rep, double x = 0.0,
+                                           double y = 0.0) {
     const unsigned long long k = atomicAdd(evn, 1ull);
     if ((int64_t)k < cap) {
         Event e;
-        e.step = step; e.kind = kind; e.gid_a = a; e.gid_b = b; e.rep = rep;
+        e.step = step; e.kind = kind; e.gid_a = a; e.gid_b = b; e.rep = rep; e.x = x; e.y = y;
         ev[k] = e;
     } else {
         atomicOr(err, ERR_EVOVF);
     }
+}
+
+// The reference chemistry without catalysts, cel/engine/oercel.c:1656-1753,
+// as two per-bead passes over every bead of the replicas whose check it is.
+// Decay: a lipid whose first bead (mpos 0) draws hash_u01(kld, gid) < p_ld
+// becomes water -- that bead's thread walks the chain and clears type, mpos
+// and link of each bead (every bead belongs to one lipid: no collisions) --
+// and a nutrient N with hash_u01(knd, gid) < p_nd becomes W.  All decisions
+// read the state before the pass.
+__global__ __launch_bounds__(kBeadBlock) void k_rdecay(const RepInfo *__restrict__ ri, const RepStep *__restrict__ rs,
+                                                       const int32_t *__restrict__ bblk, const double2 *__restrict__ xy,
+                                                       uint32_t *__restrict__ gt, int8_t *__restrict__ mp,
+                                                       int32_t *__restrict__ lnk, Event *__restrict__ ev,
+                                                       unsigned long long *__restrict__ evn, int64_t evcap, int record,
+                                                       uint32_t *__restrict__ err) {
+    const int r = bblk[blockIdx.x];
+    const RepStep &S = rs[r];
+    if (!S.rchem_now) return;
+    const RepInfo &R = ri[r];
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    if (loc >= R.N) return;
+    const int s = R.off + loc;
+    const uint32_t w = gt[s], g = w & kGidMask, ty = w >> kGidBits;
+    const int m = mp[s];
+    if (m == 0 && R.p_ld > 0 && hash_u01(S.kld, g) < R.p_ld) {
+        const double2 p = xy[s];
+        int c = s, n = 0;
+        while (c >= 0 && n < R.mlen) {
+            const int nx = lnk[c];
+            gt[c] = (gt[c] & kGidMask) | ((uint32_t)OC_W << kGidBits);
+            mp[c] = -1;
+            lnk[c] = -1;
+            c = nx;
+            n++;
+        }
+        if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_DLIP, g, g, r, p.x, p.y);
+    } else if (ty == OC_N && R.p_nd > 0 && hash_u01(S.knd, g) < R.p_nd) {
+        gt[s] = g | ((uint32_t)OC_W << kGidBits);
+        if (record) {
+            const double2 p = xy[s];
+            push_event(ev, evn, evcap, err, S.step, OC_EV_DN, g, g, r, p.x, p.y);
+        }
+    }
+}
+
+// inside the fixed vent region?  cel/engine/oercel.c:1702-1709
+__device__ __forceinline__ bool in_vent(const RepInfo &R, double x, double y) {
+    if (R.vent_shape == OC_VENT_DISK) {
+        const double dx = minimg(x - R.vent_x, R.L, R.halfL), dy = minimg(y - R.vent_y, R.L, R.halfL);
+        return dx * dx + dy * dy < R.vent_r2;
+    }
+    if (R.vent_shape == OC_VENT_VSTRIP) return fabs(minimg(x - R.vent_x, R.L, R.halfL)) < R.vent_hw;
+    return fabs(minimg(y - R.vent_y, R.L, R.halfL)) < R.vent_hw;
+}
+
+// Food supply after the decay pass (so a bead that decayed to W may be fed in
+// the same check): every W (feed = uniform) or every W inside the vent region
+// (feed = vent) with hash_u01(kfeed, gid) < p_feed becomes N.  No cap on the
+// N fraction (feed_cap = 0; the cap's random subset is not implemented).
+__global__ __launch_bounds__(kBeadBlock) void k_rfeed(const RepInfo *__restrict__ ri, const RepStep *__restrict__ rs,
+                                                      const int32_t *__restrict__ bblk, const double2 *__restrict__ xy,
+                                                      uint32_t *__restrict__ gt, Event *__restrict__ ev,
+                                                      unsigned long long *__restrict__ evn, int64_t evcap, int record,
+                                                      uint32_t *__restrict__ err) {
+    const int r = bblk[blockIdx.x];
+    const RepStep &S = rs[r];
+    if (!S.rchem_now) return;
+    const RepInfo &R = ri[r];
+    if (R.feed_mode == OC_FEED_NONE || !(R.p_feed > 0)) return;
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    if (loc >= R.N) return;
+    const int s = R.off + loc;
+    const uint32_t w = gt[s], g = w & kGidMask;
+    if ((w >> kGidBits) != OC_W || !(hash_u01(S.kfeed, g) < R.p_feed)) return;
+    const double2 p = xy[s];
+    if (R.feed_mode == OC_FEED_VENT && !in_vent(R, p.x, p.y)) return;
+    gt[s] = g | ((uint32_t)OC_N << kGidBits);
+    if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_FEED, g, g, r, p.x, p.y);
 }
 
 // Applies the decisions of the listed beads of one chunk.  A proposal forms a

@@ -238,10 +238,26 @@ def stats_row(st):
 
 
 def events_rows(ev):
-    """events.csv rows of the precursor chemistry (kinds conv / form / break;
-    i = first gid, variant = second gid)."""
-    names = ("conv", "form", "break")
-    return [f"{int(e['step'])},{names[int(e['kind'])]},{int(e['gid_a'])},nan,nan,{int(e['gid_b'])}" for e in ev]
+    """events.csv rows (step,kind,i,x,y,variant), in the order of `ev`:
+      - the precursor chemistry: conv / form / break, i = first gid,
+        variant = second gid, no position;
+      - the reference chemistry (cel/engine/oercel.c:2036-2047): dlip (i = the
+        lipid's first bead) and dN with the bead's position (%.4f), variant "-";
+        feed as one row per check with i = the number of conversions."""
+    names = ("conv", "form", "break", "dlip", "dN")
+    rows, feed = [], {}
+    for e in ev:
+        k, st = int(e["kind"]), int(e["step"])
+        if k <= 2:
+            rows.append(f"{st},{names[k]},{int(e['gid_a'])},nan,nan,{int(e['gid_b'])}")
+        elif k <= 4:
+            rows.append(f"{st},{names[k]},{int(e['gid_a'])},{float(e['x']):.4f},{float(e['y']):.4f},-")
+        else:
+            if st not in feed:
+                feed[st] = 0
+                rows.append(st)  # placeholder, filled below
+            feed[st] += 1
+    return [f"{r},feed,{feed[r]},nan,nan,-" if isinstance(r, int) else r for r in rows]
 
 
 def save_checkpoint(engine, path, r=0):

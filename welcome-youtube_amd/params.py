@@ -328,10 +328,18 @@ class Params:
         return self
 
     def check_engine(self):
-        """Features the HIP engine implements (raises for the others)."""
-        if self.chem:
-            raise ParamError("chem = 1 (the catalyst/decay/feed chemistry) is not implemented by this engine; "
-                             "set chem = 0 (and pchem = 1 for the precursor chemistry)")
+        """Features the HIP engine implements (raises for the others).  Of the
+        reference chemistry (chem = 1) it implements the part that runs without
+        catalysts: lipid and nutrient decay and the food supply without a cap."""
+        v = self.__dict__["_v"]
+        if v["chem"]:
+            if v["f_Ka"] > 0 or v["f_Kb"] > 0:
+                raise ParamError("chem = 1 with catalysts (f_Ka or f_Kb > 0) needs the catalyst chemistry "
+                                 "(synthesis, replication), which this engine does not implement; set f_Ka = "
+                                 "f_Kb = 0, or chem = 0 (and pchem = 1 for the precursor chemistry)")
+            if v["feed"] != "none" and v["feed_cap"] > 0:
+                raise ParamError("feed_cap > 0 (a random subset of the feed conversions) is not implemented by "
+                                 "this engine; set feed_cap = 0 or feed = none")
         return self
 
     @property
@@ -368,6 +376,11 @@ class Params:
         c.t_anchor, c.step_anchor = self.t_anchor, self.step_anchor
         c.lip_arch = ARCHS.index(v["lip_arch"])
         c.lip_tie, c.r0_head, c.k_head, c.cos_head = v["lip_tie"], v["r0_head"], v["k_head"], v["cos_head"]
+        c.rchem = 1 if v["chem"] else 0
+        c.feed_mode = ("none", "uniform", "vent").index(v["feed"])
+        c.p_ld, c.p_nd, c.p_feed = v["p_ld"], v["p_nd"], v["p_feed"]
+        c.vent_shape = ("disk", "vstrip", "hstrip").index(v["vent_shape"])
+        c.vent_x, c.vent_y, c.vent_r, c.vent_w = v["vent_x"], v["vent_y"], v["vent_r"], v["vent_w"]
         return c
 
     def items(self):
