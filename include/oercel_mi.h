@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define OC_ABI_VERSION 2
+#define OC_ABI_VERSION 3
 #define OC_NTYPES 7
 #define OC_MAXMLEN 12
 
@@ -47,7 +47,8 @@ enum {
 /* event kinds: the precursor chemistry (no counterpart in the reference), then
  * the reference chemistry's decay and food supply (its dlip, dN and feed
  * events, cel/engine/oercel.c:658-659; feed is reported per converted bead) */
-enum { OC_EV_CONV = 0, OC_EV_FORM = 1, OC_EV_BREAK = 2, OC_EV_DLIP = 3, OC_EV_DN = 4, OC_EV_FEED = 5 };
+enum { OC_EV_CONV = 0, OC_EV_FORM = 1, OC_EV_BREAK = 2, OC_EV_DLIP = 3, OC_EV_DN = 4, OC_EV_FEED = 5,
+       OC_EV_SYN = 6, OC_EV_REP = 7, OC_EV_MUT = 8, OC_EV_DK = 9 };
 
 /*
  * Parameters of one replica: the numeric subset of the reference's Params
@@ -101,6 +102,22 @@ typedef struct oc_params {
     int32_t vent_shape;        /* OC_VENT_DISK / VSTRIP / HSTRIP */
     int32_t pad2;
     double vent_x, vent_y, vent_r, vent_w;  /* fixed vent region (feed = vent) */
+    /* the catalyst chemistry (rchem = 1 with Ka / Kb beads), in the reference
+     * GPU port's hash-decided form (cel/engine_gpu/oercel_gpu.cu:1645-1709,
+     * 1746-1880): a catalyst draws synthesis / replication from per-catalyst
+     * hashes (salts 21, 22, mutation 23); the active ones react one after the
+     * other in the order of a hash priority (salt 24), each seeing the
+     * reactions before it.  Synthesis turns the syn_nN (0 = the lipid length)
+     * nearest N within r_cat (then the nearest W) into a new lipid through
+     * their centroid, the orientation of least overlap among syn_tries
+     * (salts 40 + try), relaxed by syn_relax descent steps; replication turns
+     * the next two N into a catalyst of the same kind (the other with
+     * probability mu) and a W.  Catalyst decay p_kd (salt 12); feed_cap > 0
+     * keeps, when the feed would exceed feed_cap * N nutrients, the
+     * conversions with the smallest hash (salt 25) keys. */
+    double p_syn_a, p_syn_b, p_rep_a, p_rep_b, mu, p_kd;
+    double r_cat, feed_cap;
+    int32_t syn_tries, syn_relax, syn_nN, pad3;
 } oc_params;
 
 enum { OC_FEED_NONE = 0, OC_FEED_UNIFORM = 1, OC_FEED_VENT = 2 };
@@ -124,13 +141,17 @@ typedef struct oc_state {
     int32_t *bond;
 } oc_state;
 
-/* one chemistry event; kinds OC_EV_CONV, DN, FEED have gid_b = gid_a */
+/* one chemistry event.  conv / form / break: the two beads.  dlip / dN / feed:
+ * gid_a = gid_b = the bead (dlip: the lipid's first bead), at (x, y).  syn:
+ * gid_a = the catalyst, gid_b = its kind (0 = Ka, 1 = Kb), (x, y) = the
+ * catalyst; rep / mut: gid_a = the new catalyst, gid_b = its kind, (x, y) =
+ * the parent catalyst; dK: gid_a = the catalyst, gid_b = its kind, at (x, y). */
 typedef struct oc_event {
     int64_t step;
     int32_t kind;
-    uint32_t gid_a, gid_b;     /* the beads (decay, feed: gid_a = gid_b = the bead; dlip: its first bead) */
+    uint32_t gid_a, gid_b;
     int32_t pad;
-    double x, y;               /* position of gid_a at the decision (decay and feed events; 0 otherwise) */
+    double x, y;               /* 0 for conv / form / break */
 } oc_event;
 
 /* stats of one replica (cel/engine/oercel.c:2053-2071 kinetic_T/count_types,

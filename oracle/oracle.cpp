@@ -73,6 +73,9 @@ static inline uint64_t hash_key(uint64_t seed, int64_t step, uint64_t salt) {
 static inline double hash_u01(uint64_t key, uint32_t g) {
     return (double)(mix13(key ^ ((uint64_t)g * 0xd1b54a32d192ed03ULL)) >> 11) * 0x1.0p-53;
 }
+/* the full 64-bit per-bead hash (the GPU port's hash_u64,
+ * cel/engine_gpu/oercel_gpu.cu:259): priorities and feed-cap keys */
+static inline uint64_t hash_u64(uint64_t key, uint32_t g) { return mix13(key ^ ((uint64_t)g * 0xd1b54a32d192ed03ULL)); }
 /* symmetric per-pair uniform for bond formation (own definition; same hash
  * shape as pair_theta, keyed by a chemistry salt) */
 static inline double pair_u01(uint64_t key, uint32_t gi, uint32_t gj) {
@@ -365,6 +368,7 @@ struct Tmpl {
     int arch = 0, mlen = 0, nH = 0;
     std::vector<int> bi, bj, ai, aj, ak, akind;
     std::vector<double> br0, aK, ac0, ix, iy;
+    std::vector<double> px, py; /* centroid-0 coordinates (branched architectures) */
     void bond(int i, int j, double r0) { bi.push_back(i); bj.push_back(j); br0.push_back(r0); }
     void angle(int i, int j, int k, int kind, double K, double c0) {
         if (K == 0.0) return;
@@ -445,6 +449,8 @@ static Tmpl make_tmpl(const oc_params *p) {
     mx /= M.mlen; my /= M.mlen;
     for (int b = 0; b < M.mlen; b++) { px[b] -= mx; py[b] -= my; }
     for (int b = 0; b < M.mlen; b++) { M.ix.push_back(px[b] - px[0]); M.iy.push_back(py[b] - py[0]); }
+    M.px = px;
+    M.py = py;
     return M;
 }
 
@@ -739,18 +745,22 @@ OR_API int64_t or_chem(const oc_params *p, int32_t N, int64_t step, const double
     return n;
 }
 
-/* ------------------------------------------------------------------ reference chemistry without catalysts */
+/* ------------------------------------------------------------------ reference chemistry: decay and food supply */
 
 /*
- * The reference's chemistry check (cel/engine/oercel.c:1563-1753) when no
- * catalyst exists: the catalyst loop is empty, so a check is the decay pass
- * (:1656-1694; a lipid whose first bead draws hash_u01(kl, gid) < p_ld becomes
- * water along its chain, an N with hash_u01(kn, gid) < p_nd becomes W;
- * decisions collected first, then applied) followed by the food supply
- * (:1711-1753; every W, or every W inside the vent region, with
- * hash_u01(kf, gid) < p_feed becomes N; no cap).  Keys hash_key(seed, step,
- * 11 / 13 / 14).  Events (dlip: first bead; dN; feed: one per bead) with the
- * bead's position, sorted by (step, kind, gid_a, gid_b).
+ * The decay pass and the food supply of the reference's chemistry check
+ * (cel/engine/oercel.c:1656-1753), after the catalyst reactions: a lipid whose
+ * first bead draws hash_u01(kl, gid) < p_ld becomes water along its chain, a
+ * catalyst with hash_u01(kk, gid) < p_kd and an N with hash_u01(kn, gid) < p_nd
+ * become W (decisions collected first, then applied); then every W, or every W
+ * inside the vent region, with hash_u01(kf, gid) < p_feed becomes N.  Keys
+ * hash_key(seed, step, 11 / 12 / 13 / 14).  With feed_cap > 0 and more
+ * conversions than room = floor(feed_cap N + 1e-9) - (number of N), the kept
+ * ones are the room smallest keys (hash_u64(hash_key(seed, step, 25), gid) with
+ * the low 32 bits replaced by the gid): the reference GPU port's cap
+ * (cel/engine_gpu/oercel_gpu.cu:1944-1990); the CPU engine picks the subset
+ * from its serial random stream.  Events (dlip: first bead; dK; dN; feed: one
+ * per bead) with the bead's position, sorted by (step, kind, gid_a, gid_b).
  */
 static bool in_vent(const oc_params *p, double x, double y) {
     const double L = p->L;
@@ -766,15 +776,20 @@ OR_API int64_t or_rchem(const oc_params *p, int32_t N, int64_t step, const doubl
                         int8_t *mpos, int32_t *lnk, const uint32_t *gid, oc_event *ev, int64_t cap) {
     if (!p->rchem || p->n_chem < 1 || step % p->n_chem != 0) return 0;
     const Tmpl M = make_tmpl(p);
-    const uint64_t kl = hash_key(p->seed, step, 11), kn = hash_key(p->seed, step, 13),
-                   kf = hash_key(p->seed, step, 14);
+    const uint64_t kl = hash_key(p->seed, step, 11), kk = hash_key(p->seed, step, 12),
+                   kn = hash_key(p->seed, step, 13), kf = hash_key(p->seed, step, 14),
+                   kc = hash_key(p->seed, step, 25);
     std::vector<oc_event> out;
     std::vector<int> dl;
-    for (int d = 0; d < N; d++)
+    for (int d = 0; d < N; d++) {
+        const bool kat = type[d] == OC_KA || type[d] == OC_KB;
         if ((mpos[d] == 0 && p->p_ld > 0 && hash_u01(kl, gid[d]) < p->p_ld) ||
+            (kat && p->p_kd > 0 && hash_u01(kk, gid[d]) < p->p_kd) ||
             (type[d] == OC_N && p->p_nd > 0 && hash_u01(kn, gid[d]) < p->p_nd))
             dl.push_back(d);
+    }
     for (int d : dl) {
+        const int t = type[d];
         if (mpos[d] == 0) {
             oc_event e{step, OC_EV_DLIP, gid[d], gid[d], 0, x[d], y[d]};
             int c = d, n = 0;
@@ -787,18 +802,44 @@ OR_API int64_t or_rchem(const oc_params *p, int32_t N, int64_t step, const doubl
                 n++;
             }
             out.push_back(e);
-        } else {
+        } else if (t == OC_KA || t == OC_KB) {
+            type[d] = OC_W;
+            out.push_back(oc_event{step, OC_EV_DK, gid[d], (uint32_t)(t == OC_KB), 0, x[d], y[d]});
+        } else if (t == OC_N) {
             type[d] = OC_W;
             out.push_back(oc_event{step, OC_EV_DN, gid[d], gid[d], 0, x[d], y[d]});
         }
     }
-    if (p->feed_mode != OC_FEED_NONE && p->p_feed > 0)
+    if (p->feed_mode != OC_FEED_NONE && p->p_feed > 0) {
+        std::vector<int> fc;
         for (int d = 0; d < N; d++)
             if (type[d] == OC_W && (p->feed_mode == OC_FEED_UNIFORM || in_vent(p, x[d], y[d])) &&
-                hash_u01(kf, gid[d]) < p->p_feed) {
-                type[d] = OC_N;
-                out.push_back(oc_event{step, OC_EV_FEED, gid[d], gid[d], 0, x[d], y[d]});
+                hash_u01(kf, gid[d]) < p->p_feed)
+                fc.push_back(d);
+        int64_t add = (int64_t)fc.size();
+        uint64_t thr = ~0ULL;
+        if (p->feed_cap > 0.0 && add > 0) {
+            int64_t nN = 0;
+            for (int d = 0; d < N; d++) nN += type[d] == OC_N;
+            int64_t room = (int64_t)floor(p->feed_cap * (double)N + 1e-9) - nN;
+            if (room < 0) room = 0;
+            if (add > room) {
+                add = room;
+                if (room > 0) {
+                    std::vector<uint64_t> keys;
+                    for (int d : fc) keys.push_back((hash_u64(kc, gid[d]) & 0xFFFFFFFF00000000ULL) | gid[d]);
+                    std::nth_element(keys.begin(), keys.begin() + (room - 1), keys.end());
+                    thr = keys[room - 1];
+                }
             }
+        }
+        if (add > 0)
+            for (int d : fc)
+                if (((hash_u64(kc, gid[d]) & 0xFFFFFFFF00000000ULL) | gid[d]) <= thr) {
+                    type[d] = OC_N;
+                    out.push_back(oc_event{step, OC_EV_FEED, gid[d], gid[d], 0, x[d], y[d]});
+                }
+    }
     std::sort(out.begin(), out.end(), [](const oc_event &a, const oc_event &b) {
         if (a.kind != b.kind) return a.kind < b.kind;
         if (a.gid_a != b.gid_a) return a.gid_a < b.gid_a;
@@ -862,6 +903,339 @@ static void sys_sort(OrSys *s) {
         l2[d] = o < 0 ? -1 : dest[o];
     }
     s->lnk.swap(l2);
+}
+
+/* ------------------------------------------------------------------ catalyst chemistry */
+
+/*
+ * The catalyst reactions of a chemistry check in the reference GPU port's
+ * form (chem_react and the reaction part of chemistry,
+ * cel/engine_gpu/oercel_gpu.cu:1645-1709, 1800-1880).  It keeps the CPU
+ * engine's per-catalyst body (cel/engine/oercel.c:1575-1653) and replaces its
+ * serial random stream by per-catalyst hashes:
+ *   - a catalyst (Ka / Kb, gid g) is active if hash_u01(k21, g) < p_syn or
+ *     hash_u01(k22, g) < p_rep of its kind;
+ *   - the active ones react one after the other in increasing
+ *     (hash_u64(k24, g), g), each on the state the earlier ones left;
+ *   - synthesis orientations from hash_u01(hash_key(seed, step, 40 + try), g),
+ *     mutation from hash_u01(k23, g);
+ *   - geometry (search windows, overlap energy, relaxation and their visiting
+ *     order) is the CPU engine's, on the cell index of the last sort (every
+ *     bead moved less than the chemistry margin 0.5 since).
+ */
+static const double kChemMargin = 0.5;
+
+struct Near {
+    int slot;
+    double d2;
+};
+
+/* the K nearest beads of `type` within r_cat of slot k, by (d2, gid); returns
+ * how many lie within r_cat, at most 256 (find_type_near,
+ * cel/engine/oercel.c:1353-1397; its list keeps the first 256 it visits, which
+ * differs only with more than 256 such beads within r_cat) */
+static int near_of_type(const OrSys *s, int k, int type, Near *out, int K) {
+    const Geo &g = s->g;
+    const double xk = s->x[k], yk = s->y[k], L = g.L, r2c = s->p.r_cat * s->p.r_cat;
+    const int c = cell_key(g, xk, yk), cx = c / g.ncy, cy = c % g.ncy;
+    const int rx = (int)ceil((s->p.r_cat + kChemMargin) * g.icsx);
+    const int ry = (int)ceil((s->p.r_cat + kChemMargin) * g.icsy);
+    int n = 0, nk = 0;
+    for (int ox = -rx; ox <= rx; ox++) {
+        const int xx = (cx + ox + g.ncx) % g.ncx;
+        for (int oy = -ry; oy <= ry; oy++) {
+            const int cc = xx * g.ncy + (cy + oy + g.ncy) % g.ncy;
+            for (int j = s->cstart[cc]; j < s->cstart[cc + 1]; j++) {
+                if (s->type[j] != type || j == k) continue;
+                const double dx = minimg(s->x[j] - xk, L), dy = minimg(s->y[j] - yk, L);
+                const double d2 = dx * dx + dy * dy;
+                if (!(d2 < r2c)) continue;
+                if (n < 256) n++;
+                if (K <= 0) continue;
+                const uint32_t gj = s->gid[j];
+                if (nk == K && !(d2 < out[K - 1].d2 || (d2 == out[K - 1].d2 && gj < s->gid[out[K - 1].slot]))) continue;
+                int b = nk < K ? nk++ : K - 1;
+                while (b > 0 && (out[b - 1].d2 > d2 || (out[b - 1].d2 == d2 && s->gid[out[b - 1].slot] > gj))) {
+                    out[b] = out[b - 1];
+                    b--;
+                }
+                out[b].slot = j;
+                out[b].d2 = d2;
+            }
+        }
+    }
+    return n;
+}
+
+/* bead b of a new lipid centred at (cx0, cy0) along (ux, uy), wrapped */
+static inline void chain_pos(const OrSys *s, const Tmpl &M, int b, double cx0, double cy0, double ux, double uy,
+                             double *px, double *py) {
+    if (M.arch == OC_ARCH_LINEAR) {
+        const double off = (b - 0.5 * (M.mlen - 1)) * s->p.r0;
+        *px = cx0 + off * ux;
+        *py = cy0 + off * uy;
+    } else {
+        *px = cx0 + (M.px[b] * ux - M.py[b] * uy);
+        *py = cy0 + (M.px[b] * uy + M.py[b] * ux);
+    }
+    wrap1(px, s->g.L);
+    wrap1(py, s->g.L);
+}
+
+static inline bool in_list(int j, const int *sl, int n) {
+    bool hit = false;
+    for (int q = 0; q < n; q++) hit |= j == sl[q];
+    return hit;
+}
+
+/* conservative overlap energy of a would-be lipid with every other bead within
+ * 1, chain_overlap_energy, cel/engine/oercel.c:1423-1468 (the raw a_ij) */
+static double overlap_energy(const OrSys *s, const Tmpl &M, double cx0, double cy0, double ux, double uy,
+                             const int *sl) {
+    const Geo &g = s->g;
+    const double L = g.L;
+    const int rx = (int)ceil((1.0 + kChemMargin) * g.icsx), ry = (int)ceil((1.0 + kChemMargin) * g.icsy);
+    double U = 0.0;
+    for (int b = 0; b < M.mlen; b++) {
+        double px, py;
+        chain_pos(s, M, b, cx0, cy0, ux, uy, &px, &py);
+        const int tb = b < M.nH ? OC_H : OC_T;
+        const int c = cell_key(g, px, py), cx = c / g.ncy, cy = c % g.ncy;
+        for (int ox = -rx; ox <= rx; ox++) {
+            const int xx = (cx + ox + g.ncx) % g.ncx;
+            for (int oy = -ry; oy <= ry; oy++) {
+                const int cc = xx * g.ncy + (cy + oy + g.ncy) % g.ncy;
+                for (int j = s->cstart[cc]; j < s->cstart[cc + 1]; j++) {
+                    if (in_list(j, sl, M.mlen)) continue;
+                    const double dx = minimg(s->x[j] - px, L), dy = minimg(s->y[j] - py, L);
+                    const double r2 = dx * dx + dy * dy;
+                    if (r2 >= 1.0) continue;
+                    const double w = 1.0 - sqrt(r2);
+                    U += 0.5 * s->p.a[tb][s->type[j]] * w * w;
+                }
+            }
+        }
+    }
+    return U;
+}
+
+/* steepest descent of a new lipid in the frozen field of its neighbours
+ * (pair overlap + its own bonds and angles), relax_new_chain,
+ * cel/engine/oercel.c:1474-1556: neighbours within 1 + 0.05 syn_relax of any
+ * bead collected once (per-bead windows, first visit order, at most 512),
+ * then syn_relax moves of 0.002 F capped at 0.05 per bead */
+static void relax_chain(OrSys *s, const Tmpl &M, const int *sl) {
+    const Geo &g = s->g;
+    const int ml = M.mlen;
+    const double L = g.L, cap = 0.05, alpha = 0.002;
+    const int maxnb = 512;
+    std::vector<int> nbj;
+    const double reach = 1.0 + cap * s->p.syn_relax, reach2 = reach * reach;
+    const int rx = (int)ceil((reach + kChemMargin) * g.icsx), ry = (int)ceil((reach + kChemMargin) * g.icsy);
+    for (int b = 0; b < ml && (int)nbj.size() < maxnb; b++) {
+        const double px = s->x[sl[b]], py = s->y[sl[b]];
+        const int c = cell_key(g, px, py), cx = c / g.ncy, cy = c % g.ncy;
+        for (int ox = -rx; ox <= rx; ox++) {
+            const int xx = ((cx + ox) % g.ncx + g.ncx) % g.ncx;
+            for (int oy = -ry; oy <= ry; oy++) {
+                const int cc = xx * g.ncy + ((cy + oy) % g.ncy + g.ncy) % g.ncy;
+                for (int j = s->cstart[cc]; j < s->cstart[cc + 1]; j++) {
+                    if (in_list(j, sl, ml) || in_list(j, nbj.data(), (int)nbj.size())) continue;
+                    const double dx = minimg(s->x[j] - px, L), dy = minimg(s->y[j] - py, L);
+                    if (dx * dx + dy * dy < reach2 && (int)nbj.size() < maxnb) nbj.push_back(j);
+                }
+            }
+        }
+    }
+    for (int it = 0; it < s->p.syn_relax; it++) {
+        double fx[OC_MAXMLEN] = {0}, fy[OC_MAXMLEN] = {0};
+        for (int b = 0; b < ml; b++) {
+            const int i = sl[b];
+            for (int j : nbj) {
+                const double dx = minimg(s->x[i] - s->x[j], L), dy = minimg(s->y[i] - s->y[j], L);
+                const double r2 = dx * dx + dy * dy;
+                if (r2 >= 1.0 || r2 < 1e-24) continue;
+                const double r = sqrt(r2);
+                const double f = s->p.a[s->type[i]][s->type[j]] * (1.0 - r) / r;
+                fx[b] += f * dx;
+                fy[b] += f * dy;
+            }
+        }
+        const double kb = s->p.k_bond;
+        if (M.arch == OC_ARCH_LINEAR) {
+            double bx[OC_MAXMLEN], by[OC_MAXMLEN], bl[OC_MAXMLEN];
+            for (int b = 0; b < ml - 1; b++) {
+                const int i = sl[b], j = sl[b + 1];
+                bx[b] = minimg(s->x[j] - s->x[i], L);
+                by[b] = minimg(s->y[j] - s->y[i], L);
+                bl[b] = sqrt(bx[b] * bx[b] + by[b] * by[b]);
+                const double f = -kb * (bl[b] - s->p.r0) / bl[b];
+                fx[b + 1] += f * bx[b];
+                fy[b + 1] += f * by[b];
+                fx[b] -= f * bx[b];
+                fy[b] -= f * by[b];
+            }
+            const double ka = s->p.k_angle;
+            for (int b = 0; b < ml - 2 && ka != 0.0; b++) {
+                const double ax = bx[b], ay = by[b], cxv = bx[b + 1], cyv = by[b + 1], la = bl[b], lc = bl[b + 1];
+                const double ilac = 1.0 / (la * lc), cs = (ax * cxv + ay * cyv) * ilac;
+                const double dUax = -ka * (cxv * ilac - cs * ax / (la * la)), dUay = -ka * (cyv * ilac - cs * ay / (la * la));
+                const double dUcx = -ka * (ax * ilac - cs * cxv / (lc * lc)), dUcy = -ka * (ay * ilac - cs * cyv / (lc * lc));
+                fx[b] += dUax;
+                fy[b] += dUay;
+                fx[b + 1] += -dUax + dUcx;
+                fy[b + 1] += -dUay + dUcy;
+                fx[b + 2] -= dUcx;
+                fy[b + 2] -= dUcy;
+            }
+        } else {
+            /* mol_forces by template position, cel/engine/oercel.c:1218-1257 */
+            for (size_t q = 0; q < M.bi.size(); q++) {
+                const int bi = M.bi[q], bj = M.bj[q], i = sl[bi], j = sl[bj];
+                const double dx = minimg(s->x[j] - s->x[i], L), dy = minimg(s->y[j] - s->y[i], L);
+                const double r = sqrt(dx * dx + dy * dy);
+                const double f = -kb * (r - M.br0[q]) / r;
+                fx[bj] += f * dx;
+                fy[bj] += f * dy;
+                fx[bi] -= f * dx;
+                fy[bi] -= f * dy;
+            }
+            for (size_t q = 0; q < M.ai.size(); q++) {
+                const int pi_ = M.ai[q], pj = M.aj[q], pk = M.ak[q];
+                const int i = sl[pi_], j = sl[pj], k = sl[pk];
+                const double ax = minimg(s->x[j] - s->x[i], L), ay = minimg(s->y[j] - s->y[i], L);
+                const double cxv = minimg(s->x[k] - s->x[j], L), cyv = minimg(s->y[k] - s->y[j], L);
+                const double la = sqrt(ax * ax + ay * ay), lc = sqrt(cxv * cxv + cyv * cyv);
+                const double ilac = 1.0 / (la * lc);
+                const double cs = (ax * cxv + ay * cyv) * ilac;
+                const double gg = M.akind[q] == 0 ? -M.aK[q] : 2.0 * M.aK[q] * (cs - M.ac0[q]);
+                const double dUax = gg * (cxv * ilac - cs * ax / (la * la));
+                const double dUay = gg * (cyv * ilac - cs * ay / (la * la));
+                const double dUcx = gg * (ax * ilac - cs * cxv / (lc * lc));
+                const double dUcy = gg * (ay * ilac - cs * cyv / (lc * lc));
+                fx[pi_] += dUax;
+                fy[pi_] += dUay;
+                fx[pj] += -dUax + dUcx;
+                fy[pj] += -dUay + dUcy;
+                fx[pk] -= dUcx;
+                fy[pk] -= dUcy;
+            }
+        }
+        for (int b = 0; b < ml; b++) {
+            double mx = alpha * fx[b], my = alpha * fy[b];
+            const double m = sqrt(mx * mx + my * my);
+            if (m > cap) {
+                mx *= cap / m;
+                my *= cap / m;
+            }
+            s->x[sl[b]] += mx;
+            s->y[sl[b]] += my;
+            wrap1(&s->x[sl[b]], L);
+            wrap1(&s->y[sl[b]], L);
+        }
+    }
+}
+
+/* one active catalyst's reactions: synthesis, then replication
+ * (chem_react, cel/engine_gpu/oercel_gpu.cu:1645-1709) */
+static void react_one(OrSys *s, const Tmpl &M, int64_t step, uint64_t ksyn, uint64_t krep, uint64_t kmu, int k,
+                      std::vector<oc_event> &out) {
+    const oc_params &p = s->p;
+    const int tk = s->type[k];
+    if (tk != OC_KA && tk != OC_KB) return;
+    const int var = tk == OC_KB;
+    const uint32_t g = s->gid[k];
+    const double us = hash_u01(ksyn, g), ur = hash_u01(krep, g);
+    const double psyn = var ? p.p_syn_b : p.p_syn_a, prep = var ? p.p_rep_b : p.p_rep_a;
+    const int ml = M.mlen, nsynN = p.syn_nN > 0 ? p.syn_nN : ml;
+    const double L = s->g.L;
+    Near nb[OC_MAXMLEN + 2], nbw[OC_MAXMLEN];
+    const int nn = near_of_type(s, k, OC_N, nb, nsynN + 2);
+    if (nn < 2) return;
+    int used = 0;
+    const double xk = s->x[k], yk = s->y[k];
+    int nw = 0;
+    if (nn >= nsynN && nsynN < ml) nw = near_of_type(s, k, OC_W, nbw, ml - nsynN);
+    if (nn >= nsynN && nsynN + nw >= ml && us < psyn) {
+        double cxm = 0, cym = 0, mvx = 0, mvy = 0, mtx = 0, mty = 0;
+        int sl[OC_MAXMLEN];
+        for (int b = 0; b < ml; b++) {
+            const int j = b < nsynN ? nb[b].slot : nbw[b - nsynN].slot;
+            sl[b] = j;
+            cxm += minimg(s->x[j] - xk, L);
+            cym += minimg(s->y[j] - yk, L);
+            mvx += s->vx[j];
+            mvy += s->vy[j];
+            mtx += s->vtx[j];
+            mty += s->vty[j];
+        }
+        cxm = xk + cxm / ml;
+        cym = yk + cym / ml;
+        mvx /= ml; mvy /= ml; mtx /= ml; mty /= ml;
+        double ux = 1.0, uy = 0.0, ubest = INFINITY;
+        for (int tr = 0; tr < p.syn_tries; tr++) {
+            const double phi = 2.0 * M_PI * hash_u01(hash_key(p.seed, step, 40 + (uint64_t)tr), g);
+            const double cx_ = cos(phi), cy_ = sin(phi);
+            const double U = p.syn_tries > 1 ? overlap_energy(s, M, cxm, cym, cx_, cy_, sl) : 0.0;
+            if (U < ubest) {
+                ubest = U;
+                ux = cx_;
+                uy = cy_;
+            }
+        }
+        for (int b = 0; b < ml; b++) {
+            const int j = sl[b];
+            chain_pos(s, M, b, cxm, cym, ux, uy, &s->x[j], &s->y[j]);
+            s->vx[j] = mvx;
+            s->vy[j] = mvy;
+            s->vtx[j] = mtx;
+            s->vty[j] = mty;
+            s->type[j] = b < M.nH ? OC_H : OC_T;
+        }
+        for (int b = 0; b < ml; b++) {
+            s->mpos[sl[b]] = (int8_t)b;
+            s->lnk[sl[b]] = b + 1 < ml ? sl[b + 1] : -1;
+        }
+        if (p.syn_relax > 0) relax_chain(s, M, sl);
+        out.push_back(oc_event{step, OC_EV_SYN, g, (uint32_t)var, 0, xk, yk});
+        used = nsynN;
+    }
+    if (nn - used >= 2 && ur < prep) {
+        int nv = var, mut = 0;
+        if (hash_u01(kmu, g) < p.mu) {
+            nv = 1 - var;
+            mut = 1;
+        }
+        const int j1 = nb[used].slot, j2 = nb[used + 1].slot;
+        s->type[j1] = nv ? OC_KB : OC_KA;
+        s->type[j2] = OC_W;
+        out.push_back(oc_event{step, mut ? OC_EV_MUT : OC_EV_REP, s->gid[j1], (uint32_t)nv, 0, xk, yk});
+    }
+}
+
+/* the reactions of one check: active catalysts in priority order */
+static void sys_react(OrSys *s, int64_t step, std::vector<oc_event> &out) {
+    const oc_params &p = s->p;
+    if (!(p.p_syn_a > 0 || p.p_syn_b > 0 || p.p_rep_a > 0 || p.p_rep_b > 0)) return;
+    const Tmpl M = make_tmpl(&p);
+    const uint64_t ksyn = hash_key(p.seed, step, 21), krep = hash_key(p.seed, step, 22),
+                   kmu = hash_key(p.seed, step, 23), kpri = hash_key(p.seed, step, 24);
+    struct Act {
+        uint64_t key;
+        uint32_t g;
+        int slot;
+    };
+    std::vector<Act> act;
+    for (int d = 0; d < s->N; d++) {
+        const int t = s->type[d];
+        if (t != OC_KA && t != OC_KB) continue;
+        const int var = t == OC_KB;
+        const uint32_t g = s->gid[d];
+        if (hash_u01(ksyn, g) < (var ? p.p_syn_b : p.p_syn_a) || hash_u01(krep, g) < (var ? p.p_rep_b : p.p_rep_a))
+            act.push_back(Act{hash_u64(kpri, g), g, d});
+    }
+    std::sort(act.begin(), act.end(), [](const Act &a, const Act &b) { return a.key != b.key ? a.key < b.key : a.g < b.g; });
+    for (const Act &a : act) react_one(s, M, step, ksyn, krep, kmu, a.slot, out);
 }
 
 static void sys_forces(OrSys *s, int64_t step, int theta32) {
@@ -928,22 +1302,40 @@ OR_API void or_get_state(OrSys *s, oc_state *st) {
     }
 }
 
-/* n steps: integrate, sort, chemistry, pair + bonded forces.  The reference
- * runs chemistry between integrate and sort on the previous cell index
- * (cel/engine/oercel.c:2337-2346); types and bonds do not move beads, so
- * running it on the fresh cell list makes the same decisions. */
+/* n steps.  Without the reference chemistry: integrate, sort, precursor
+ * chemistry, pair + bonded forces (the precursor chemistry moves no bead, so
+ * running it on the fresh cell list makes the decisions it would make on the
+ * previous one).  With it, the reference's order (cel/engine/oercel.c:2300-2362):
+ * integrate, the reference chemistry on the previous cell index (after an
+ * extra sort if a bead moved 0.5 or more), sort, then the precursor chemistry
+ * and the forces. */
+static void sys_rchem(OrSys *s, int64_t step) {
+    sys_react(s, step, s->ev);
+    std::vector<oc_event> tmp(s->N + 16);
+    const int64_t ne = or_rchem(&s->p, s->N, step, s->x.data(), s->y.data(), s->type.data(), s->mpos.data(),
+                                s->lnk.data(), s->gid.data(), tmp.data(), (int64_t)tmp.size());
+    s->ev.insert(s->ev.end(), tmp.begin(), tmp.begin() + std::min<int64_t>(ne, (int64_t)tmp.size()));
+}
+
 OR_API void or_step(OrSys *s, int64_t n, int32_t theta32) {
     for (int64_t k = 0; k < n; k++) {
         const int64_t ns = s->step + 1;
+        const bool rcheck = s->p.rchem && s->p.n_chem >= 1 && ns % s->p.n_chem == 0;
+        double dmax = 0.0;
+        if (rcheck) {
+            const double dt = s->p.dt, dt2 = dt * dt;
+            for (int i = 0; i < s->N; i++) {
+                const double ddx = fabs(dt * s->vx[i] + dt2 * s->fx[i]), ddy = fabs(dt * s->vy[i] + dt2 * s->fy[i]);
+                dmax = std::max(dmax, std::max(ddx, ddy));
+            }
+        }
         or_integrate(&s->p, s->N, s->x.data(), s->y.data(), s->vx.data(), s->vy.data(), s->fx.data(), s->fy.data(),
                      s->vtx.data(), s->vty.data());
-        sys_sort(s);
-        if (s->p.rchem) {
-            std::vector<oc_event> tmp(s->N + 16);
-            const int64_t ne = or_rchem(&s->p, s->N, ns, s->x.data(), s->y.data(), s->type.data(), s->mpos.data(),
-                                        s->lnk.data(), s->gid.data(), tmp.data(), (int64_t)tmp.size());
-            s->ev.insert(s->ev.end(), tmp.begin(), tmp.begin() + std::min<int64_t>(ne, (int64_t)tmp.size()));
+        if (rcheck) {
+            if (dmax >= kChemMargin) sys_sort(s);
+            sys_rchem(s, ns);
         }
+        sys_sort(s);
         if (s->p.chem) {
             std::vector<oc_event> tmp(s->N * 2 + 16);
             const int64_t ne = or_chem(&s->p, s->N, ns, s->x.data(), s->y.data(), s->type.data(), s->mpos.data(),
@@ -955,15 +1347,10 @@ OR_API void or_step(OrSys *s, int64_t n, int32_t theta32) {
     }
 }
 
-/* one chemistry check on the loaded (sorted) state (the reference chemistry's
- * decay and feed, then the precursor chemistry), events appended */
+/* one chemistry check on the loaded (sorted) state (the reference chemistry:
+ * reactions, decay, feed; then the precursor chemistry), events appended */
 OR_API int64_t or_sys_chem(OrSys *s, int64_t step) {
-    if (s->p.rchem) {
-        std::vector<oc_event> tmp(s->N + 16);
-        const int64_t ne = or_rchem(&s->p, s->N, step, s->x.data(), s->y.data(), s->type.data(), s->mpos.data(),
-                                    s->lnk.data(), s->gid.data(), tmp.data(), (int64_t)tmp.size());
-        s->ev.insert(s->ev.end(), tmp.begin(), tmp.begin() + std::min<int64_t>(ne, (int64_t)tmp.size()));
-    }
+    if (s->p.rchem && s->p.n_chem >= 1 && step % s->p.n_chem == 0) sys_rchem(s, step);
     std::vector<oc_event> tmp(s->N * 2 + 16);
     const int64_t ne = or_chem(&s->p, s->N, step, s->x.data(), s->y.data(), s->type.data(), s->mpos.data(),
                                s->lnk.data(), s->gid.data(), s->cstart.data(), tmp.data(), (int64_t)tmp.size());

@@ -22,8 +22,15 @@ Fixtures (numpy .npz / JSON, small systems):
                     distorted lipids, and a 20-step CLI trajectory of pi lipids
   rchem.npz         a 20-step CLI run with the reference chemistry on and no
                     catalysts (decay, vent feed): checkpoint and events.csv
+  catalysts.npz     the reference GPU port (cel/engine_gpu/oercel_gpu.cu, built
+                    as its own CPU emulation, oracle/_ref/oercel_gpu_emu) over
+                    one chemistry check with catalysts: its checkpoint before
+                    (step 9) and after (step 10) and its events.csv, for seven
+                    configurations (linear / pi / y / bola lipids, syn_nN,
+                    syn_tries 1, feed cap, vent, nsub 2 / 3 / 4)
 
-`python tests/golden/make_golden.py templates` (or `rchem`) writes that file only.
+`python tests/golden/make_golden.py templates` (or `rchem`, `catalysts`) writes
+that file only.
 """
 import ctypes as C
 import json
@@ -150,6 +157,46 @@ def rchem():
     np.savez_compressed(os.path.join(HERE, "rchem.npz"), **out)
 
 
+# catalyst chemistry configurations: (name, overrides on top of CAT_BASE)
+CAT_BASE = dict(L="16", dt="0.01", f_lip="0.1", lip_nT="2", c_N="0.3", f_Ka="0.03", f_Kb="0.02", warmup="0",
+                nsub="4", chem="1", n_chem="10", p_syn_a="0.3", p_syn_b="0.5", p_rep_a="0.2", p_rep_b="0.2",
+                mu="0.3", p_kd="0.02", p_ld="0.01", p_nd="0.01", feed="uniform", p_feed="0.05", feed_cap="0.25",
+                snap_every="0", stats_every="0", ckpt_every="0", log_every="0", events="1", threads="1")
+CATS = [
+    ("linear", dict(seed="5")),
+    ("capped", dict(seed="8", c_N="0.2", feed_cap="0.22", p_feed="0.2")),
+    ("pi", dict(seed="9", lip_arch="pi", c_N="0.35")),
+    ("synN2", dict(seed="10", syn_nN="2")),
+    ("tries1", dict(seed="11", syn_tries="1", nsub="2", c_N="0.15")),
+    ("bola", dict(seed="12", lip_arch="bola", c_N="0.4", L="20")),
+    ("yvent", dict(seed="13", lip_arch="y", nsub="3", feed="vent", vent_r="5", p_feed="0.3", c_N="0.15")),
+]
+
+
+def catalysts():
+    """The GPU port's CPU emulation: 9 steps from its soup (no check yet:
+    n_chem = 10), then a restart from that checkpoint for the step with the
+    check; both checkpoints (gid order) and the events.csv of the check."""
+    emu = os.path.join(REPO, "oracle", "_ref", "oercel_gpu_emu")
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, kw in CATS:
+            ov = dict(CAT_BASE, **kw)
+            a, b = os.path.join(tmp, name + "_a"), os.path.join(tmp, name + "_b")
+            for d, extra in ((a, dict(steps="9")), (b, dict(steps="10", restart=os.path.join(a, "ckpt.bin")))):
+                cmd = [emu, "/dev/null", d] + [f"{k}={v}" for k, v in dict(ov, **extra).items()]
+                subprocess.run(cmd, capture_output=True, text=True, check=True)
+            for tag, d in (("in", a), ("out", b)):
+                g = oracle.ckpt_gid_order(oracle.read_ckpt(os.path.join(d, "ckpt.bin")))
+                for k in ("x", "y", "vx", "vy", "fx", "fy", "type", "mpos", "nxt"):
+                    if tag == "in" or k not in ("fx", "fy"):
+                        out[f"{name}_{tag}_{k}"] = g[k]
+            with open(os.path.join(b, "events.csv")) as f:
+                out[f"{name}_events"] = np.array(f.read())
+            out[f"{name}_overrides"] = np.array(json.dumps(ov))
+    np.savez_compressed(os.path.join(HERE, "catalysts.npz"), **out)
+
+
 def main():
     if not oracle.ref_available():
         sys.exit("oracle/_ref is not built: make -C oracle ref")
@@ -229,13 +276,14 @@ def main():
         json.dump(tab, f, indent=1)
     templates()
     rchem()
+    catalysts()
     print("golden fixtures written to", HERE)
 
 
 if __name__ == "__main__":
-    if sys.argv[1:] in (["templates"], ["rchem"]):
+    if sys.argv[1:] in (["templates"], ["rchem"], ["catalysts"]):
         if not oracle.ref_available():
             sys.exit("oracle/_ref is not built: make -C oracle ref")
-        templates() if sys.argv[1] == "templates" else rchem()
+        {"templates": templates, "rchem": rchem, "catalysts": catalysts}[sys.argv[1]]()
     else:
         main()

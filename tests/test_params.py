@@ -82,10 +82,13 @@ def test_ini_and_overrides(tmp_path):
         pm.load(overrides=["Lx20"])
 
 
-def test_engine_refuses_reference_chemistry():
-    p = pm.load()
-    with pytest.raises(pm.ParamError, match="chem = 1"):
-        p.check_engine()
+def test_engine_takes_the_reference_defaults():
+    """The reference's default parameters (chem = 1 with catalysts, feed cap)
+    map onto the C parameters without a refusal."""
+    p = pm.load().check_engine()
+    c = p.to_c()
+    assert c.rchem == 1 and c.f_Ka == 0.005 and c.p_syn_b == 0.06 and c.feed_cap == 0.12
+    assert c.r_cat == 1.2 and c.syn_tries == 8 and c.syn_relax == 20 and c.syn_nN == 0 and c.mu == 0.02
     pm.load(overrides={"chem": "0", "pchem": "1"}).check_engine()
 
 

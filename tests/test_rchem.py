@@ -69,10 +69,6 @@ def test_rchem_run_vs_reference_cli(oracle):
 
 
 def test_rchem_parameters():
-    with pytest.raises(pm.ParamError, match="catalyst"):
-        pm.load(None, ["chem=1", "f_Ka=0.01"]).check_engine()
-    with pytest.raises(pm.ParamError, match="feed_cap"):
-        pm.load(None, ["chem=1", "f_Ka=0", "feed_cap=0.1"]).check_engine()
     p = pm.load(None, ["chem=1", "f_Ka=0", "feed_cap=0", "feed=vent", "vent_shape=hstrip", "vent_w=3"]).check_engine()
     c = p.to_c()
     assert c.rchem == 1 and c.feed_mode == 2 and c.vent_shape == 2 and c.vent_w == 3.0

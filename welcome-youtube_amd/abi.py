@@ -39,6 +39,9 @@ class oc_params(C.Structure):
         ("rchem", C.c_int32), ("feed_mode", C.c_int32), ("p_ld", C.c_double), ("p_nd", C.c_double),
         ("p_feed", C.c_double), ("vent_shape", C.c_int32), ("pad2", C.c_int32), ("vent_x", C.c_double),
         ("vent_y", C.c_double), ("vent_r", C.c_double), ("vent_w", C.c_double),
+        ("p_syn_a", C.c_double), ("p_syn_b", C.c_double), ("p_rep_a", C.c_double), ("p_rep_b", C.c_double),
+        ("mu", C.c_double), ("p_kd", C.c_double), ("r_cat", C.c_double), ("feed_cap", C.c_double),
+        ("syn_tries", C.c_int32), ("syn_relax", C.c_int32), ("syn_nN", C.c_int32), ("pad3", C.c_int32),
     ]
 
 

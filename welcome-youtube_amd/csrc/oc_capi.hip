@@ -9,6 +9,7 @@
 
 #include "oc_adjoint.hpp"
 #include "oc_mol.hpp"
+#include "oc_react.hpp"
 
 using namespace oc;
 
@@ -73,7 +74,12 @@ struct oc_engine {
     hipGraph_t graph[2] = {nullptr, nullptr};
     bool have_state = false, any_stir = false, any_chem = false, any_angle = false;
     bool any_tmpl = false;  // some replica has branched lipids (k_mol; the force kernel leaves no keys)
-    bool any_rchem = false; // some replica runs the reference chemistry (decay, feed)
+    bool any_rchem = false; // some replica runs the reference chemistry: the step integrates, runs it, then sorts
+    bool any_react = false, any_fcap = false;  // catalyst reactions; feed cap
+    bool nokeys = false;    // the force kernel leaves no cell keys (branched lipids or the reference chemistry)
+    int32_t *d_cat = nullptr, *d_catn = nullptr;  // active catalysts per replica (kCatCap each), counts
+    uint64_t *d_fkey = nullptr;                   // feed-cap candidate keys (per replica at its first slot)
+    int2 *d_fcnt = nullptr;                       // feed cap: (number of N, candidates) per replica
     std::vector<std::vector<oc_event>> evq;
     std::vector<void *> allocs;
     bool events_dropped = false;
@@ -176,6 +182,49 @@ static int make_repinfo(const oc_params &p, int32_t N, RepInfo &R) {
     R.p_break = p.p_break;
     for (int i = 0; i < 8; i++)
         for (int j = 0; j < 8; j++) R.a[8 * i + j] = (i < OC_NTYPES && j < OC_NTYPES) ? p.a[i][j] : 0.0;
+    // catalyst chemistry and feed cap (cel/engine_gpu/oercel_gpu.cu:1773-1797)
+    R.p_syn[0] = p.p_syn_a;
+    R.p_syn[1] = p.p_syn_b;
+    R.p_rep[0] = p.p_rep_a;
+    R.p_rep[1] = p.p_rep_b;
+    R.mu = p.mu;
+    R.p_kd = p.p_kd;
+    R.r2cat = p.r_cat * p.r_cat;
+    R.syn_tries = std::max(p.syn_tries, 1);
+    R.syn_relax = std::max(p.syn_relax, 0);
+    R.nsynN = p.syn_nN > 0 ? p.syn_nN : M.mlen;
+    R.react = p.rchem && (p.p_syn_a > 0 || p.p_syn_b > 0 || p.p_rep_a > 0 || p.p_rep_b > 0);
+    for (int b = 0; b < M.mlen; b++) {
+        R.tpx[b] = M.px[b];
+        R.tpy[b] = M.py[b];
+    }
+    if (R.react) {
+        if (p.syn_nN < 0 || p.syn_nN > M.mlen) return fail(OC_EINVAL, "syn_nN must be 0..lipid length");
+        if (!(p.r_cat > 0)) return fail(OC_EINVAL, "r_cat must be > 0");
+        const double margin = 0.5, reach = 1.0 + 0.05 * R.syn_relax;
+        R.rwx = (int)ceil((p.r_cat + margin) * R.icsx);
+        R.rwy = (int)ceil((p.r_cat + margin) * R.icsy);
+        R.owx = (int)ceil((1.0 + margin) * R.icsx);
+        R.owy = (int)ceil((1.0 + margin) * R.icsy);
+        R.lwx = (int)ceil((reach + margin) * R.icsx);
+        R.lwy = (int)ceil((reach + margin) * R.icsy);
+        if (2 * std::max({R.rwx, R.owx, R.lwx}) + 1 > R.ncx || 2 * std::max({R.rwy, R.owy, R.lwy}) + 1 > R.ncy)
+            return fail(OC_EINVAL, "box too small for the catalyst chemistry's search windows (r_cat, syn_relax)");
+        // cells a reaction writes: its r_cat window; cells it reads: that
+        // window, and the overlap and relaxation windows around the new
+        // lipid's beads, which lie within r_cat + E (E: the template's extent
+        // from its centroid) of the catalyst
+        double E = 0.0;
+        for (int b = 0; b < M.mlen; b++) E = std::max(E, std::hypot(M.px[b], M.py[b]));
+        const double d = p.r_cat + E + 1e-9;
+        const int rdx = std::max(R.rwx, (int)ceil(d * R.icsx) + 1 + std::max(R.owx, R.lwx));
+        const int rdy = std::max(R.rwy, (int)ceil(d * R.icsy) + 1 + std::max(R.owy, R.lwy));
+        R.cfx = R.rwx + rdx;
+        R.cfy = R.rwy + rdy;
+    }
+    R.fcap_on = p.rchem && p.feed_cap > 0 && p.feed_mode != OC_FEED_NONE && p.p_feed > 0;
+    if (p.rchem && (p.feed_cap < 0 || p.feed_cap >= 1)) return fail(OC_EINVAL, "feed_cap must be in [0, 1)");
+    R.fcap_n = (int32_t)floor(p.feed_cap * (double)N + 1e-9);
     return 0;
 }
 
@@ -255,10 +304,10 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false) {
     A.err = h->d_err;
     // forces also leave the next step's cell keys
     A.v = h->st[c].v;
-    A.key = energy || h->any_tmpl ? nullptr : h->d_key;
+    A.key = energy || h->nokeys ? nullptr : h->d_key;
     A.rmax = h->d_rmax;
     A.moved = h->d_moved;
-    if (!energy) h->keys_fresh = !h->any_tmpl;
+    if (!energy) h->keys_fresh = !h->nokeys;
     const dim3 grid(8 * h->per_xcd);
     // CAP (staged candidates) is 3.5x the block; the variant picks the single
     // pass, the per-lane two phases or the cross-lane pair queue
@@ -308,19 +357,43 @@ static void launch_build_prv(oc_engine *h) {
                                                                   h->st[c].prv);
 }
 
-static void launch_chem(oc_engine *h) {
-    const int c = h->cur;
-    if (h->any_rchem) {
-        // the reference chemistry's decay, then its food supply (before the
-        // precursor chemistry of the same check)
-        const int nb = (int)h->bblk.size();
-        k_rdecay<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy, h->st[c].gt,
-                                                   h->st[c].mp, h->st[c].lnk, h->d_ev, h->d_evn, h->evcap, h->record,
-                                                   h->d_err);
-        k_rfeed<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy, h->st[c].gt, h->d_ev,
-                                                  h->d_evn, h->evcap, h->record, h->d_err);
+// The reference chemistry of the replicas whose check it is: catalyst
+// reactions, decay, food supply (with its cap), in the reference's order.
+static void launch_rchem(oc_engine *h) {
+    if (!h->any_rchem) return;
+    const int c = h->cur, nb = (int)h->bblk.size();
+    StateP &S = h->st[c];
+    if (h->any_react) {
+        k_react_list<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, S.gt, h->d_cat, h->d_catn);
+        ReactBufs B;
+        B.xy = S.xy;
+        B.v = S.v;
+        B.vt = S.vt;
+        B.gt = S.gt;
+        B.mp = S.mp;
+        B.lnk = S.lnk;
+        B.prv = S.prv;
+        B.cs = h->d_cs[c];
+        B.key = h->d_key;
+        B.rmax = h->d_rmax;
+        B.moved = h->d_moved;
+        k_react<<<h->nrep, kReactThr, 0, h->stream>>>(h->d_ri, h->d_rs, B, h->d_cat, h->d_catn, h->d_ev, h->d_evn,
+                                                      h->evcap, h->record, h->d_err);
     }
+    k_rdecay<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, S.xy, S.gt, S.mp, S.lnk, h->d_ev, h->d_evn,
+                                               h->evcap, h->record, h->d_err);
+    if (h->any_fcap) {
+        k_rfeed_count<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, S.xy, S.gt, h->d_fkey, h->d_fcnt);
+        k_rfeed_thr<<<h->nrep, 256, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_fkey, h->d_fcnt);
+    }
+    k_rfeed<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, S.xy, S.gt, h->d_ev, h->d_evn, h->evcap,
+                                              h->record, h->d_err);
+}
+
+// the precursor chemistry (after the sort, on the fresh cell list)
+static void launch_pchem(oc_engine *h) {
     if (!h->any_chem) return;
+    const int c = h->cur;
     const int nk = (int)h->rblk.size();
     k_chem_decide<<<nk, kChemThreads, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_rblk, h->d_cs[c], h->st[c].xy,
                                                       h->st[c].gt, h->st[c].mp, h->st[c].lnk, h->d_flag, h->d_best,
@@ -328,6 +401,19 @@ static void launch_chem(oc_engine *h) {
     k_chem_apply<<<nk, 64, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_rblk, h->st[c].gt, h->st[c].lnk, h->d_flag,
                                            h->d_best, h->d_act, h->d_actn, h->d_ev, h->d_evn, h->evcap, h->record,
                                            h->d_err);
+}
+
+// one chemistry check on the current (sorted) state; the reference
+// chemistry's syntheses move beads, so its state is sorted again
+static void launch_sort(oc_engine *h, bool integ, int step_delta, bool keys_ready);
+static void launch_keys(oc_engine *h, bool integ);
+static void launch_chem(oc_engine *h) {
+    if (h->any_rchem) {
+        launch_keys(h, false);
+        launch_rchem(h);
+        launch_sort(h, false, 0, true);
+    }
+    launch_pchem(h);
 }
 
 // sort (with integration when integ): keys, tile sort, tile offsets,
@@ -343,12 +429,12 @@ static void launch_keys(oc_engine *h, bool integ) {
                                                         h->d_key, h->d_rmax, h->d_moved);
 }
 
-static void launch_sort(oc_engine *h, bool integ, int step_delta) {
+static void launch_sort(oc_engine *h, bool integ, int step_delta, bool keys_ready = false) {
     const int c = h->cur, n = c ^ 1;
     const int nt = (int)h->stiles.size();
     // the force kernel left the keys of this state's integrate unless the
-    // state changed since
-    if (!(integ && h->keys_fresh)) launch_keys(h, integ);
+    // state changed since (keys_ready: the caller's kernels wrote them)
+    if (!keys_ready && !(integ && h->keys_fresh)) launch_keys(h, integ);
     const int ng = std::min(nt, kSortGrid);
     k_tsort<<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_cs[c], h->d_key, h->d_rmax, h->d_moved,
                                              h->d_tcount, h->d_locst, h->d_tsrc, h->d_tslot, h->d_err, nt);
@@ -373,10 +459,26 @@ static void launch_step_params(oc_engine *h, int64_t delta, int64_t set_step) {
     k_step_params<<<h->nrep, 64, 0, h->stream>>>(h->d_ri, h->d_rs, delta, set_step, nullptr);
 }
 
-// one full step of every replica: cel/engine/oercel.c:2300-2362
+
+// one full step of every replica: cel/engine/oercel.c:2300-2362.  With the
+// reference chemistry in the reference's order: integrate, its chemistry on
+// the previous cell index (catalyst reactions move beads), sort, then the
+// precursor chemistry and the forces; without it the integrate is fused into
+// the sort and every chemistry runs after it.
 static void enqueue_step(oc_engine *h) {
-    launch_sort(h, true, 1);
-    launch_chem(h);
+    if (h->any_rchem) {
+        const int c = h->cur;
+        launch_step_params(h, 1, -1);
+        k_integ<<<(int)h->bblk.size(), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy,
+                                                                    h->st[c].v, h->st[c].vt, h->d_f, h->d_key,
+                                                                    h->d_rmax, h->d_moved);
+        launch_rchem(h);
+        launch_sort(h, false, 0, true);
+        launch_pchem(h);
+    } else {
+        launch_sort(h, true, 1);
+        launch_pchem(h);
+    }
     launch_forces(h, true);
 }
 
@@ -390,6 +492,10 @@ static int check_err(oc_engine *h) {
         if (e & ERR_SORT) m += " sort tile overflow or bead count mismatch (sort)";
         if (e & ERR_CHAIN) m += " broken lipid chain";
         if (e & ERR_TILE) m += " pair tile window outside its staged rows";
+        if (e & ERR_CATCAP) m += " more than 4096 active catalysts in one chemistry check of a replica";
+        if (e & ERR_DISP)
+            m += " a bead moved 0.5 or more in the step before a chemistry check (the cell index of the last sort "
+                 "no longer covers the reaction windows; reduce dt)";
         return fail(OC_EDEVICE, m);
     }
     return 0;
@@ -580,7 +686,10 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
         h->any_angle |= R.ka != 0.0 && R.arch == 0;
         h->any_tmpl |= R.arch != 0;
         h->any_rchem |= R.rchem != 0;
+        h->any_react |= R.react != 0;
+        h->any_fcap |= R.fcap_on != 0;
     }
+    h->nokeys = h->any_tmpl || h->any_rchem;
     h->Ntot = off;
     h->Ctot = coff;
     if ((int64_t)h->stiles.size() << kSortCapLog >= INT32_MAX) {
@@ -629,6 +738,10 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     rc = rc ? rc : dalloc(h, &h->d_evn, 1);
     rc = rc ? rc : dalloc(h, &h->d_err, 1);
     if (h->any_stir) rc = rc ? rc : dalloc(h, &h->d_stir, Nt);
+    if (h->any_react) rc = rc ? rc : dalloc(h, &h->d_cat, (size_t)nrep * kCatCap);
+    if (h->any_react) rc = rc ? rc : dalloc(h, &h->d_catn, nrep);
+    if (h->any_fcap) rc = rc ? rc : dalloc(h, &h->d_fkey, Nt);
+    if (h->any_fcap) rc = rc ? rc : dalloc(h, &h->d_fcnt, nrep);
     if (rc) {
         oc_destroy(h);
         return rc;
@@ -661,6 +774,8 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     e = e ? e : hipMemset(h->d_f, 0, Nt * sizeof(double2));
     for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(h->st[k].lnk, 0xff, Nt * 4);
     for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(h->st[k].prv, 0xff, Nt * 4);
+    if (h->any_react) e = e ? e : hipMemset(h->d_catn, 0, nrep * sizeof(int32_t));
+    if (h->any_fcap) e = e ? e : hipMemset(h->d_fcnt, 0, nrep * sizeof(int2));
     if (e != hipSuccess) {
         oc_destroy(h);
         return fail(OC_EHIP, std::string("setup copy failed: ") + hipGetErrorString(e));
@@ -755,9 +870,6 @@ static int load_replica(oc_engine *h, int r, const oc_state *s, size_t base, boo
         const size_t k = base + g;
         const int ty = s->type[k];
         if (ty < 0 || ty >= OC_NTYPES) return fail(OC_EINVAL, "state: bad type");
-        if (R.rchem && (ty == OC_KA || ty == OC_KB))
-            return fail(OC_EINVAL, "chem = 1 with catalyst beads (Ka / Kb) needs the catalyst chemistry, "
-                                   "which this engine does not implement");
         xy[d] = make_double2(s->x[k], s->y[k]);
         v[d] = make_double2(s->vx[k], s->vy[k]);
         f[d] = make_double2(s->fx ? s->fx[k] : 0.0, s->fy ? s->fy[k] : 0.0);
@@ -878,7 +990,7 @@ static int capture_graph(oc_engine *h, int parity) {
     const int saved = h->cur;
     const bool fresh = h->keys_fresh;
     h->cur = parity;
-    h->keys_fresh = !h->any_tmpl;  // with branched lipids the graph's first step computes its keys
+    h->keys_fresh = !h->nokeys;  // without keys from the force kernel the graph's first step computes them
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     enqueue_step(h);
     enqueue_step(h);
@@ -901,9 +1013,9 @@ OC_API int oc_step(oc_engine *h, int64_t n, int32_t use_graph) {
         const int p = h->cur;
         if (!h->gexec[p])
             if (int rc = capture_graph(h, p)) return rc;
-        if (!h->keys_fresh && !h->any_tmpl) launch_keys(h, true);
+        if (!h->keys_fresh && !h->nokeys) launch_keys(h, true);
         for (; k + 2 <= n; k += 2) HIPCHK(hipGraphLaunch(h->gexec[p], h->stream));
-        h->keys_fresh = !h->any_tmpl;
+        h->keys_fresh = !h->nokeys;
     }
     for (; k < n; k++) enqueue_step(h);
     HIPCHK(hipGetLastError());
@@ -1335,6 +1447,8 @@ OC_API int oc_window_forward(oc_engine *h, int32_t nsteps, int32_t ckpt_every, c
         if (P.k_angle != 0.0 || P.stir_A != 0.0 || P.kT_amp != 0.0 || P.chem)
             return fail(OC_EINVAL, "window gradient needs k_angle = 0 and no stirring, kT cycle or chemistry");
         if (h->ri[r].arch != OC_ARCH_LINEAR) return fail(OC_EINVAL, "window gradient needs linear lipids");
+        if (h->ri[r].rchem)
+            return fail(OC_EINVAL, "window gradient: the reference chemistry (chem = 1) is not differentiated");
         Rc = std::max(Rc, (int)ceil(cut * R.icsx));
     }
     if (2 * Rc + 1 > kRdfMaxCol) return fail(OC_EINVAL, "rdf cutoff longer than four cell columns");

@@ -17,7 +17,7 @@ constexpr int kBeadBlock = 256;                   // beads per block of the per-
 constexpr int kRedChunk = 2048;                   // beads per partial of the stats reduction
 
 // sticky device error bits
-enum : uint32_t { ERR_EVOVF = 1u, ERR_CHAIN = 2u, ERR_SORT = 4u, ERR_TILE = 8u };
+enum : uint32_t { ERR_EVOVF = 1u, ERR_CHAIN = 2u, ERR_SORT = 4u, ERR_TILE = 8u, ERR_CATCAP = 16u, ERR_DISP = 32u };
 
 // Replica constants (one system of the ensemble).
 struct RepInfo {
@@ -46,6 +46,15 @@ struct RepInfo {
     int32_t bi[2 * OC_MAXMLEN], bj[2 * OC_MAXMLEN];
     int32_t ai[2 * OC_MAXMLEN], aj[2 * OC_MAXMLEN], ak[2 * OC_MAXMLEN], akind[2 * OC_MAXMLEN];
     double br0[2 * OC_MAXMLEN], aK[2 * OC_MAXMLEN], ac0[2 * OC_MAXMLEN];
+    // catalyst chemistry (the reference GPU port's form) and the feed cap
+    int32_t react, syn_tries, syn_relax, nsynN;  // react: some p_syn / p_rep > 0
+    double p_syn[2], p_rep[2], mu, p_kd, r2cat;
+    // windows (cells) of the r_cat search, the overlap energy and the
+    // relaxation neighbours; two catalysts closer than (cfx, cfy) cells may
+    // touch each other's beads
+    int32_t rwx, rwy, owx, owy, lwx, lwy, cfx, cfy;
+    int32_t fcap_on, fcap_n;  // feed cap on; floor(feed_cap N + 1e-9)
+    double tpx[OC_MAXMLEN], tpy[OC_MAXMLEN];  // template coordinates (centroid 0), branched lipids
 };
 
 // Per-step values of one replica, written on the device by k_step_params.
@@ -55,6 +64,10 @@ struct RepStep {
     uint64_t sseed;           // pair-noise key of this step
     uint64_t kconv, kform, kbreak;  // chemistry keys
     uint64_t kld, knd, kfeed;       // reference chemistry keys (salts 11, 13, 14)
+    uint64_t kkd, ksyn, krep, kmu, kpri, kcap;  // salts 12, 21, 22, 23, 24, 25
+    uint64_t dmax_bits;       // largest displacement of the step's integrate (bits of a double >= 0)
+    uint64_t feed_thr;        // feed cap: largest kept key
+    int32_t feed_add, rpad;   // feed cap: conversions kept (0: none)
     float a_eff[64];          // repulsion at this step's warm-up ramp
     float gamma, sdt;         // gamma, sigma(t)/sqrt(dt)
     int32_t Rx, Ry;           // sort window radii (cells) of the last integrate

@@ -15,6 +15,7 @@
 //   k_chem_*       precursor chemistry (own definition)
 //   k_pair         full-shell DPD pair forces, LDS-staged tiles, with the
 //                  bonded terms of each bead (gather form)
+#pragma once
 #include "oc_common.hpp"
 
 namespace oc {
@@ -98,6 +99,14 @@ __device__ __forceinline__ void step_params_wave(const RepInfo &R, RepStep &S, i
     S.kld = hash_key(R.seed, st, 11);
     S.knd = hash_key(R.seed, st, 13);
     S.kfeed = hash_key(R.seed, st, 14);
+    // catalyst decay and reactions, feed cap: cel/engine_gpu/oercel_gpu.cu:1765-1772
+    S.kkd = hash_key(R.seed, st, 12);
+    S.ksyn = hash_key(R.seed, st, 21);
+    S.krep = hash_key(R.seed, st, 22);
+    S.kmu = hash_key(R.seed, st, 23);
+    S.kpri = hash_key(R.seed, st, 24);
+    S.kcap = hash_key(R.seed, st, 25);
+    S.dmax_bits = 0;
 }
 
 __global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs,
@@ -115,9 +124,9 @@ __global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ 
 // displacement rmax[r] (x, y); the sort then widens its window to it.  The
 // force kernel computes the same keys in its epilogue (new_key), so a step
 // whose forces came from it skips this launch.
-__device__ __forceinline__ void new_key(const RepInfo &R, int r, int s, double2 p, double2 q, int32_t *key,
-                                        int2 *rmax, uint32_t *moved) {
-    const int ox = cell_x(R, p.x), oy = cell_y(R, p.y);
+// key of bead s at q that sits in (local) cell (ox, oy) of the last sort
+__device__ __forceinline__ void new_key_from(const RepInfo &R, int r, int s, int ox, int oy, double2 q, int32_t *key,
+                                             int2 *rmax, uint32_t *moved) {
     const int cx = cell_x(R, q.x), cy = cell_y(R, q.y);
     // bit 31: the bead changed cells
     key[s] = (R.coff + cx * R.ncy + cy) | (cx != ox || cy != oy ? (int32_t)0x80000000 : 0);
@@ -129,6 +138,10 @@ __device__ __forceinline__ void new_key(const RepInfo &R, int r, int s, double2 
         atomicMax(&rmax[r].y, dy);
         atomicOr(moved, 1u);
     }
+}
+__device__ __forceinline__ void new_key(const RepInfo &R, int r, int s, double2 p, double2 q, int32_t *key,
+                                        int2 *rmax, uint32_t *moved) {
+    new_key_from(R, r, s, cell_x(R, p.x), cell_y(R, p.y), q, key, rmax, moved);
 }
 
 template <bool INTEG>
@@ -1689,13 +1702,15 @@ __device__ __forceinline__ void push_event(Event *ev, unsigned long long *evn, i
     }
 }
 
-// The reference chemistry without catalysts, cel/engine/oercel.c:1656-1753,
-// as two per-bead passes over every bead of the replicas whose check it is.
+// The decay pass and the food supply of the reference's chemistry check,
+// cel/engine/oercel.c:1656-1753, as per-bead passes over every bead of the
+// replicas whose check it is (after the catalyst reactions).
 // Decay: a lipid whose first bead (mpos 0) draws hash_u01(kld, gid) < p_ld
 // becomes water -- that bead's thread walks the chain and clears type, mpos
 // and link of each bead (every bead belongs to one lipid: no collisions) --
-// and a nutrient N with hash_u01(knd, gid) < p_nd becomes W.  All decisions
-// read the state before the pass.
+// a catalyst with hash_u01(kkd, gid) < p_kd and a nutrient N with
+// hash_u01(knd, gid) < p_nd become W.  All decisions read the state before
+// the pass.
 __global__ __launch_bounds__(kBeadBlock) void k_rdecay(const RepInfo *__restrict__ ri, const RepStep *__restrict__ rs,
                                                        const int32_t *__restrict__ bblk, const double2 *__restrict__ xy,
                                                        uint32_t *__restrict__ gt, int8_t *__restrict__ mp,
@@ -1723,6 +1738,12 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdecay(const RepInfo *__restrict
             n++;
         }
         if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_DLIP, g, g, r, p.x, p.y);
+    } else if ((ty == OC_KA || ty == OC_KB) && R.p_kd > 0 && hash_u01(S.kkd, g) < R.p_kd) {
+        gt[s] = g | ((uint32_t)OC_W << kGidBits);
+        if (record) {
+            const double2 p = xy[s];
+            push_event(ev, evn, evcap, err, S.step, OC_EV_DK, g, ty == OC_KB ? 1u : 0u, r, p.x, p.y);
+        }
     } else if (ty == OC_N && R.p_nd > 0 && hash_u01(S.knd, g) < R.p_nd) {
         gt[s] = g | ((uint32_t)OC_W << kGidBits);
         if (record) {
@@ -1742,10 +1763,22 @@ __device__ __forceinline__ bool in_vent(const RepInfo &R, double x, double y) {
     return fabs(minimg(y - R.vent_y, R.L, R.halfL)) < R.vent_hw;
 }
 
+// feed candidate: a W (inside the vent region for feed = vent) whose draw
+// hash_u01(kfeed, gid) < p_feed
+__device__ __forceinline__ bool feed_candidate(const RepInfo &R, const RepStep &S, uint32_t w, double2 p) {
+    return (w >> kGidBits) == OC_W && hash_u01(S.kfeed, w & kGidMask) < R.p_feed &&
+           (R.feed_mode != OC_FEED_VENT || in_vent(R, p.x, p.y));
+}
+// feed-cap key of a candidate: the high half of hash_u64(kcap, gid), the gid
+// below (unique), cel/engine_gpu/oercel_gpu.cu:1963
+__device__ __forceinline__ uint64_t feed_key(const RepStep &S, uint32_t g) {
+    return (mix13(S.kcap ^ ((uint64_t)g * 0xd1b54a32d192ed03ULL)) & 0xFFFFFFFF00000000ULL) | g;
+}
+
 // Food supply after the decay pass (so a bead that decayed to W may be fed in
-// the same check): every W (feed = uniform) or every W inside the vent region
-// (feed = vent) with hash_u01(kfeed, gid) < p_feed becomes N.  No cap on the
-// N fraction (feed_cap = 0; the cap's random subset is not implemented).
+// the same check): every candidate becomes N -- with a feed cap, the ones
+// whose key is at most the threshold k_rfeed_thr left (none when it keeps
+// none).
 __global__ __launch_bounds__(kBeadBlock) void k_rfeed(const RepInfo *__restrict__ ri, const RepStep *__restrict__ rs,
                                                       const int32_t *__restrict__ bblk, const double2 *__restrict__ xy,
                                                       uint32_t *__restrict__ gt, Event *__restrict__ ev,
@@ -1756,13 +1789,15 @@ __global__ __launch_bounds__(kBeadBlock) void k_rfeed(const RepInfo *__restrict_
     if (!S.rchem_now) return;
     const RepInfo &R = ri[r];
     if (R.feed_mode == OC_FEED_NONE || !(R.p_feed > 0)) return;
+    if (R.fcap_on && S.feed_add == 0) return;
     const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
     if (loc >= R.N) return;
     const int s = R.off + loc;
     const uint32_t w = gt[s], g = w & kGidMask;
-    if ((w >> kGidBits) != OC_W || !(hash_u01(S.kfeed, g) < R.p_feed)) return;
+    if ((w >> kGidBits) != OC_W) return;
     const double2 p = xy[s];
-    if (R.feed_mode == OC_FEED_VENT && !in_vent(R, p.x, p.y)) return;
+    if (!feed_candidate(R, S, w, p)) return;
+    if (R.fcap_on && feed_key(S, g) > S.feed_thr) return;
     gt[s] = g | ((uint32_t)OC_N << kGidBits);
     if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_FEED, g, g, r, p.x, p.y);
 }

@@ -8,10 +8,9 @@ finalize_params (:578-619) and N = lround(rho L^2) (:2188).
 
 Same key names, types, defaults and error behaviour: an unknown key, a bad
 value or a removed key raises ParamError (the reference prints the message and
-exits).  Keys of the reference's catalyst chemistry, IO cadence and lipid
-architectures are accepted and kept so a reference ini loads unchanged; the
-engine refuses values that ask for features it does not implement (chem = 1)
-with an explicit error.  Lipid templates (lip_arch linear, pi, y, bola and
+exits).  Every key of the reference is accepted and used: the chemistry
+(chem = 1; catalysts, decay, feed and its cap), IO cadences, lipid
+architectures.  Lipid templates (lip_arch linear, pi, y, bola and
 their keys) are the reference's mol_build.
 
 Precursor-chemistry keys (no counterpart in the reference): pchem, p_conv,
@@ -328,18 +327,11 @@ class Params:
         return self
 
     def check_engine(self):
-        """Features the HIP engine implements (raises for the others).  Of the
-        reference chemistry (chem = 1) it implements the part that runs without
-        catalysts: lipid and nutrient decay and the food supply without a cap."""
-        v = self.__dict__["_v"]
-        if v["chem"]:
-            if v["f_Ka"] > 0 or v["f_Kb"] > 0:
-                raise ParamError("chem = 1 with catalysts (f_Ka or f_Kb > 0) needs the catalyst chemistry "
-                                 "(synthesis, replication), which this engine does not implement; set f_Ka = "
-                                 "f_Kb = 0, or chem = 0 (and pchem = 1 for the precursor chemistry)")
-            if v["feed"] != "none" and v["feed_cap"] > 0:
-                raise ParamError("feed_cap > 0 (a random subset of the feed conversions) is not implemented by "
-                                 "this engine; set feed_cap = 0 or feed = none")
+        """Features the HIP engine implements (raises for the others; every
+        feature of the parameter table is implemented).  The reference
+        chemistry (chem = 1) runs in the reference GPU port's form:
+        hash-decided catalyst reactions in hash priority order, decay, and the
+        food supply with its cap as the smallest hash keys."""
         return self
 
     @property
@@ -381,6 +373,9 @@ class Params:
         c.p_ld, c.p_nd, c.p_feed = v["p_ld"], v["p_nd"], v["p_feed"]
         c.vent_shape = ("disk", "vstrip", "hstrip").index(v["vent_shape"])
         c.vent_x, c.vent_y, c.vent_r, c.vent_w = v["vent_x"], v["vent_y"], v["vent_r"], v["vent_w"]
+        c.p_syn_a, c.p_syn_b, c.p_rep_a, c.p_rep_b = v["p_syn_a"], v["p_syn_b"], v["p_rep_a"], v["p_rep_b"]
+        c.mu, c.p_kd, c.r_cat, c.feed_cap = v["mu"], v["p_kd"], v["r_cat"], v["feed_cap"]
+        c.syn_tries, c.syn_relax, c.syn_nN = v["syn_tries"], v["syn_relax"], v["syn_nN"]
         return c
 
     def items(self):
