@@ -194,6 +194,20 @@ def catalysts():
             with open(os.path.join(b, "events.csv")) as f:
                 out[f"{name}_events"] = np.array(f.read())
             out[f"{name}_overrides"] = np.array(json.dumps(ov))
+        # a 30-step CLI run from the port's soup with the reference's default
+        # physics (warm-up ramp, dt, a_ij, feed cap) and busier catalysts
+        ov = dict(L="24", f_Ka="0.03", f_Kb="0.02", c_N="0.2", p_syn_a="0.2", p_syn_b="0.3", p_rep_a="0.05",
+                  p_rep_b="0.05", mu="0.3", p_kd="0.01", p_ld="0.01", p_feed="0.05", seed="17", steps="30",
+                  snap_every="0", stats_every="0", ckpt_every="0", log_every="0", events="1", threads="1")
+        d = os.path.join(tmp, "cli")
+        cmd = [emu, "/dev/null", d] + [f"{k}={v}" for k, v in ov.items()]
+        subprocess.run(cmd, capture_output=True, text=True, check=True)
+        g = oracle.ckpt_gid_order(oracle.read_ckpt(os.path.join(d, "ckpt.bin")))
+        for k in ("x", "y", "vx", "vy", "type", "mpos", "nxt"):
+            out[f"cli_{k}"] = g[k]
+        with open(os.path.join(d, "events.csv")) as f:
+            out["cli_events"] = np.array(f.read())
+        out["cli_overrides"] = np.array(json.dumps(ov))
     np.savez_compressed(os.path.join(HERE, "catalysts.npz"), **out)
 
 

@@ -95,3 +95,45 @@ def test_sigterm_stop_and_restart_same_files(tmp_path):
     run.run(str(ini), d, args(restart=os.path.join(d, "ckpt.bin"), **cad))
     for f in ("stats.csv", "events.csv", "snap_0000030000.bin", "ckpt.bin", "snap_0000010000.bin"):
         assert same_bytes(os.path.join(c, f), os.path.join(d, f)), f
+
+
+def test_cli_with_catalysts_vs_the_gpu_port(tmp_path):
+    """The reference's default physics with busier catalysts, 30 steps from a
+    fresh soup, three chemistry checks: the reference GPU port's own CPU build
+    (tests/golden/catalysts.npz) and this CLI write the same events (as
+    multisets per step; the port lists reactions in priority order) and end
+    with the same types and lipid chains; positions agree to 1e-6 typically
+    and 1e-4 at worst (the two FP32 pair kernels round differently and 30
+    steps at dt 0.03 amplify that)."""
+    import json
+
+    import numpy as np
+
+    from conftest import GOLDEN
+
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+
+    run = importlib.import_module("welcome-youtube_amd.run")
+    G = np.load(os.path.join(GOLDEN, "catalysts.npz"))
+    ov = json.loads(str(G["cli_overrides"]))
+    ini = tmp_path / "empty.ini"
+    ini.write_text("# all parameters on the command line\n")
+    d = str(tmp_path / "cat")
+    say("CLI run with catalysts, 30 steps")
+    run.run(str(ini), d, [f"{k}={v}" for k, v in ov.items()])
+    with open(os.path.join(d, "events.csv")) as f:
+        ours = sorted(ln.strip().split(",") for ln in f.readlines()[1:])
+    ref = sorted(ln.split(",") for ln in str(G["cli_events"]).strip().splitlines()[1:])
+    say(f"events: {len(ours)} (port {len(ref)})")
+    assert [(r[0], r[1], r[2], r[5]) for r in ours] == [(r[0], r[1], r[2], r[5]) for r in ref]
+    assert all(abs(float(a[3]) - float(b[3])) < 2e-4 and abs(float(a[4]) - float(b[4])) < 2e-4
+               for a, b in zip(ours, ref) if a[3] != "nan")
+    g = oracle.ckpt_gid_order(oracle.read_ckpt(os.path.join(d, "ckpt.bin")))
+    for k in ("type", "mpos", "nxt"):
+        assert np.array_equal(g[k], G[f"cli_{k}"]), k
+    L = float(ov["L"])
+    for k in ("x", "y"):
+        dd = np.abs((g[k] - G[f"cli_{k}"] + L / 2) % L - L / 2)
+        say(f"{k}: median {np.median(dd):.2e}, max {dd.max():.2e}")
+        assert np.median(dd) < 1e-6 and dd.max() < 1e-4, (k, np.median(dd), dd.max())

@@ -241,22 +241,26 @@ def events_rows(ev):
     """events.csv rows (step,kind,i,x,y,variant), in the order of `ev`:
       - the precursor chemistry: conv / form / break, i = first gid,
         variant = second gid, no position;
-      - the reference chemistry (cel/engine/oercel.c:2036-2047): dlip (i = the
-        lipid's first bead) and dN with the bead's position (%.4f), variant "-";
-        feed as one row per check with i = the number of conversions."""
-    names = ("conv", "form", "break", "dlip", "dN")
+      - the reference chemistry (flush_events, cel/engine/oercel.c:2036-2047):
+        syn (i = the catalyst), rep / mut (i = the new catalyst) at the
+        catalyst's position and dK at the bead's, variant a / b (the kind of
+        catalyst); dlip (i = the lipid's first bead) and dN at the bead's
+        position, variant "-"; feed as one row per check with i = the number
+        of conversions."""
+    names = {0: "conv", 1: "form", 2: "break", 3: "dlip", 4: "dN", 6: "syn", 7: "rep", 8: "mut", 9: "dK"}
     rows, feed = [], {}
     for e in ev:
         k, st = int(e["kind"]), int(e["step"])
         if k <= 2:
             rows.append(f"{st},{names[k]},{int(e['gid_a'])},nan,nan,{int(e['gid_b'])}")
-        elif k <= 4:
-            rows.append(f"{st},{names[k]},{int(e['gid_a'])},{float(e['x']):.4f},{float(e['y']):.4f},-")
-        else:
+        elif k == 5:
             if st not in feed:
                 feed[st] = 0
                 rows.append(st)  # placeholder, filled below
             feed[st] += 1
+        else:
+            v = "ab"[int(e["gid_b"])] if k >= 6 else "-"
+            rows.append(f"{st},{names[k]},{int(e['gid_a'])},{float(e['x']):.4f},{float(e['y']):.4f},{v}")
     return [f"{r},feed,{feed[r]},nan,nan,-" if isinstance(r, int) else r for r in rows]
 
 
