@@ -91,11 +91,12 @@ typedef struct oc_params {
     int32_t lip_tie;           /* pi / y / bola: ladder rungs between the two tails */
     double r0_head;            /* pi / bola: rest length of the head-head bond */
     double k_head, cos_head;   /* junction angle terms U = k_head (cos - cos_head)^2 (0 = none) */
-    /* the reference's chemistry without catalysts (its chem = 1 when no Ka / Kb
-     * bead exists, cel/engine/oercel.c:1656-1753): at every check, lipid decay
-     * (a lipid becomes water, p_ld per molecule), nutrient decay (N -> W, p_nd),
-     * then the food supply (every eligible W -> N, p_feed) -- per-bead draws of
-     * hash_u01 with salts 11, 13, 14 */
+    /* the reference's chemistry (its chem = 1, cel/engine/oercel.c:1565-1756):
+     * at every check the catalyst reactions (below), then lipid decay (a lipid
+     * becomes water, p_ld per molecule), catalyst decay (p_kd) and nutrient
+     * decay (N -> W, p_nd), then the food supply (every eligible W -> N,
+     * p_feed, capped by feed_cap) -- per-bead draws of hash_u01 with salts
+     * 11, 12, 13, 14 */
     int32_t rchem;             /* 1 = on */
     int32_t feed_mode;         /* OC_FEED_NONE / UNIFORM / VENT */
     double p_ld, p_nd, p_feed;
