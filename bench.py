@@ -342,6 +342,70 @@ def bench_fit(a, pm, eng):
     print(json.dumps(out), flush=True)
 
 
+def refchem_params(pm, N, nsub, seed=1):
+    """The reference's default parameters -- chem = 1 (catalyst synthesis and
+    replication, decay, food supply with its 0.12 cap, a check every 10
+    steps), 10% H-T-T lipids, 0.5% Ka, 12% N, dt 0.03, the warm-up ramp -- at
+    N beads (L from N at rho = 3)."""
+    return pm.load(overrides={"L": repr(pm.L_for_N(N)), "nsub": str(nsub), "seed": str(seed)}).check_engine()
+
+
+def bench_refchem(a, pm, eng):
+    """The reference engine's own default workload (physics and chemistry) on
+    the GPU, and the reference CPU engine on the same parameters."""
+    p = refchem_params(pm, a.N, a.nsub)
+    N = p.N
+    e = eng.Engine(p, pair_block=a.pair_block, tile_beads=a.tile_beads, pair_variant=a.pair_variant)
+    e.start()
+    e.step(a.warmup, graph=not a.no_graph)
+    e.events()
+    e.step_count()
+    t0 = time.perf_counter()
+    e.step(a.steps, graph=not a.no_graph)
+    sec = time.perf_counter() - t0
+    ev = e.events()
+    st = e.stats()
+    e.close()
+    names = {3: "dlip", 4: "dN", 5: "feed", 6: "syn", 7: "rep", 8: "mut", 9: "dK"}
+    kinds = {names[k]: int((ev["kind"] == k).sum()) for k in names}
+    cpu = None
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    binp = oracle.ref_binary(fast=True)
+    if not a.no_cpu and os.path.exists(binp):
+        threads = max(1, min(16, os.cpu_count() or 1))
+        ov = {"L": repr(p.L), "nsub": str(a.nsub), "seed": "1", "snap_every": "0", "stats_every": "0",
+              "ckpt_every": "0", "log_every": "0", "events": "0", "threads": str(threads)}
+        tmp = tempfile.mkdtemp(prefix="ocref_")
+        try:
+            def run(steps):
+                out = oracle.run_ref_cli(os.path.join(tmp, f"r{steps}"), dict(ov, steps=str(steps)), fast=True)
+                return float(re.search(r"timing: (\d+) steps, ([0-9.]+) ms/step", out).group(2))
+            ms = run(10)
+            steps = int(max(20, min(3000, 12.0 / (ms * 1e-3))))
+            steps = (steps + 9) // 10 * 10
+            ms = run(steps)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        cpu = {"value": N / (ms * 1e-3), "unit": "particle-steps/s", "cores": threads, "kind": "reference",
+               "sample": f"reference CLI {os.path.basename(binp)} (cel/engine/oercel.c, -O3), N={N}, its default "
+                         f"parameters (chemistry on), {steps} steps from a fresh soup, {threads} OpenMP threads, "
+                         f"{ms:.3f} ms/step"}
+    out = {
+        "metric": "2D DPD particle-steps/s with the reference's default chemistry, N=1M",
+        "value": N * a.steps / sec, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * sec / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"the reference's default parameters at N={N:,} (L={p.L:.4f}): chem=1 with "
+                               "catalysts (synthesis, replication), decay and the capped food supply every 10 "
+                               "steps, 10% H-T-T lipids, dt=0.03, warm-up ramp",
+                   "N": N, "nsub": a.nsub, "graph": not a.no_graph, "events_in_timed_region": kinds,
+                   "T_kin_end": st["T_kin"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -356,7 +420,8 @@ def main():
                          "(default: the library's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--workload", choices=("plain", "ensemble", "adjoint", "fit", "copyprobe"), default="plain")
+    ap.add_argument("--workload", choices=("plain", "ensemble", "adjoint", "fit", "refchem", "copyprobe"),
+                    default="plain")
     ap.add_argument("--window", type=int, default=200)
     ap.add_argument("--equil", type=int, default=1000)
     ap.add_argument("--replicas", type=int, default=None, help="default 256 (ensemble), 32 (fit)")
@@ -385,6 +450,8 @@ def main():
         return bench_adjoint(a, pm, eng)
     if a.workload == "fit":
         return bench_fit(a, pm, eng)
+    if a.workload == "refchem":
+        return bench_refchem(a, pm, eng)
     p = config2_params(pm, a.N, a.nsub)
     N = p.N
     e = eng.Engine(p, pair_block=a.pair_block, tile_beads=a.tile_beads, pair_variant=a.pair_variant)

@@ -78,6 +78,11 @@ struct oc_engine {
     bool any_react = false, any_fcap = false;  // catalyst reactions; feed cap
     bool nokeys = false;    // the force kernel leaves no cell keys (branched lipids or the reference chemistry)
     int32_t *d_cat = nullptr, *d_catn = nullptr;  // active catalysts per replica (kCatCap each), counts
+    int2 *d_ccell = nullptr;                      // their cells
+    uint8_t *d_cdone = nullptr;                   // reacted in k_react_free
+    GBead *d_gs = nullptr;                        // per-wavefront gather lists of the reactions
+    int gcap = 0;                                 // beads per gather list
+    int32_t *d_npart = nullptr;                   // feed cap: N count per bead block
     uint64_t *d_fkey = nullptr;                   // feed-cap candidate keys (per replica at its first slot)
     int2 *d_fcnt = nullptr;                       // feed cap: (number of N, candidates) per replica
     std::vector<std::vector<oc_event>> evq;
@@ -221,6 +226,11 @@ static int make_repinfo(const oc_params &p, int32_t N, RepInfo &R) {
         const int rdy = std::max(R.rwy, (int)ceil(d * R.icsy) + 1 + std::max(R.owy, R.lwy));
         R.cfx = R.rwx + rdx;
         R.cfy = R.rwy + rdy;
+        R.rdx = rdx;
+        R.rdy = rdy;
+        const double rg = d + std::max(1.0, reach) + 1e-6, ro = d + 1.0 + 1e-6;
+        R.rg2 = rg * rg;
+        R.ro2 = ro * ro;
     }
     R.fcap_on = p.rchem && p.feed_cap > 0 && p.feed_mode != OC_FEED_NONE && p.p_feed > 0;
     if (p.rchem && (p.feed_cap < 0 || p.feed_cap >= 1)) return fail(OC_EINVAL, "feed_cap must be in [0, 1)");
@@ -377,14 +387,27 @@ static void launch_rchem(oc_engine *h) {
         B.key = h->d_key;
         B.rmax = h->d_rmax;
         B.moved = h->d_moved;
-        k_react<<<h->nrep, kReactThr, 0, h->stream>>>(h->d_ri, h->d_rs, B, h->d_cat, h->d_catn, h->d_ev, h->d_evn,
-                                                      h->evcap, h->record, h->d_err);
+        B.cat = h->d_cat;
+        B.catn = h->d_catn;
+        B.ccell = h->d_ccell;
+        B.cdone = h->d_cdone;
+        B.gs = h->d_gs;
+        B.gcap = h->gcap;
+        B.ev = h->d_ev;
+        B.evn = h->d_evn;
+        B.evcap = h->evcap;
+        B.record = h->record;
+        B.err = h->d_err;
+        k_react_prep<<<h->nrep, kPrepThr, 0, h->stream>>>(h->d_ri, h->d_rs, B);
+        k_react_free<<<kFreeBlocks, 256, 0, h->stream>>>(h->d_ri, h->d_rs, B, h->nrep);
+        k_react_rest<<<h->nrep, kRestThr, 0, h->stream>>>(h->d_ri, h->d_rs, B, kFreeBlocks * 4);
     }
     k_rdecay<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, S.xy, S.gt, S.mp, S.lnk, h->d_ev, h->d_evn,
                                                h->evcap, h->record, h->d_err);
     if (h->any_fcap) {
-        k_rfeed_count<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, S.xy, S.gt, h->d_fkey, h->d_fcnt);
-        k_rfeed_thr<<<h->nrep, 256, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_fkey, h->d_fcnt);
+        k_rfeed_count<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, S.xy, S.gt, h->d_fkey, h->d_fcnt,
+                                                        h->d_npart);
+        k_rfeed_thr<<<h->nrep, 256, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_fkey, h->d_fcnt, h->d_npart);
     }
     k_rfeed<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, S.xy, S.gt, h->d_ev, h->d_evn, h->evcap,
                                               h->record, h->d_err);
@@ -492,7 +515,9 @@ static int check_err(oc_engine *h) {
         if (e & ERR_SORT) m += " sort tile overflow or bead count mismatch (sort)";
         if (e & ERR_CHAIN) m += " broken lipid chain";
         if (e & ERR_TILE) m += " pair tile window outside its staged rows";
-        if (e & ERR_CATCAP) m += " more than 4096 active catalysts in one chemistry check of a replica";
+        if (e & ERR_CATCAP)
+            m += " catalyst chemistry over capacity (more than 4096 active catalysts in one check of a replica, or "
+                 "more beads within a reaction's reach than its gather list holds)";
         if (e & ERR_DISP)
             m += " a bead moved 0.5 or more in the step before a chemistry check (the cell index of the last sort "
                  "no longer covers the reaction windows; reduce dt)";
@@ -738,10 +763,21 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     rc = rc ? rc : dalloc(h, &h->d_evn, 1);
     rc = rc ? rc : dalloc(h, &h->d_err, 1);
     if (h->any_stir) rc = rc ? rc : dalloc(h, &h->d_stir, Nt);
-    if (h->any_react) rc = rc ? rc : dalloc(h, &h->d_cat, (size_t)nrep * kCatCap);
-    if (h->any_react) rc = rc ? rc : dalloc(h, &h->d_catn, nrep);
+    if (h->any_react) {
+        // gather lists: three times the expected beads within the reach, plus a margin
+        double want = 0.0;
+        for (const RepInfo &R : h->ri)
+            if (R.react) want = std::max(want, 3.0 * (R.N / (R.L * R.L)) * M_PI * R.rg2 + 128.0);
+        h->gcap = (int)std::min(65536.0, std::ceil(want / 64.0) * 64.0);
+        rc = rc ? rc : dalloc(h, &h->d_cat, (size_t)nrep * kCatCap);
+        rc = rc ? rc : dalloc(h, &h->d_catn, nrep);
+        rc = rc ? rc : dalloc(h, &h->d_ccell, (size_t)nrep * kCatCap);
+        rc = rc ? rc : dalloc(h, &h->d_cdone, (size_t)nrep * kCatCap);
+        rc = rc ? rc : dalloc(h, &h->d_gs, (size_t)(kFreeBlocks * 4 + (kRestThr / 64) * nrep) * h->gcap);
+    }
     if (h->any_fcap) rc = rc ? rc : dalloc(h, &h->d_fkey, Nt);
     if (h->any_fcap) rc = rc ? rc : dalloc(h, &h->d_fcnt, nrep);
+    if (h->any_fcap) rc = rc ? rc : dalloc(h, &h->d_npart, h->bblk.size());
     if (rc) {
         oc_destroy(h);
         return rc;

@@ -54,6 +54,9 @@ struct RepInfo {
     // touch each other's beads
     int32_t rwx, rwy, owx, owy, lwx, lwy, cfx, cfy;
     int32_t fcap_on, fcap_n;  // feed cap on; floor(feed_cap N + 1e-9)
+    int32_t rdx, rdy;         // cells (from the catalyst's) that hold every bead a reaction reads
+    double rg2;               // squared radius around the catalyst of every bead a reaction reads
+    double ro2;               // squared radius around the catalyst of every overlap-energy partner
     double tpx[OC_MAXMLEN], tpy[OC_MAXMLEN];  // template coordinates (centroid 0), branched lipids
 };
 
@@ -65,7 +68,7 @@ struct RepStep {
     uint64_t kconv, kform, kbreak;  // chemistry keys
     uint64_t kld, knd, kfeed;       // reference chemistry keys (salts 11, 13, 14)
     uint64_t kkd, ksyn, krep, kmu, kpri, kcap;  // salts 12, 21, 22, 23, 24, 25
-    uint64_t dmax_bits;       // largest displacement of the step's integrate (bits of a double >= 0)
+    uint64_t disp;            // the step's integrate moved some bead by the chemistry margin 0.5 or more
     uint64_t feed_thr;        // feed cap: largest kept key
     int32_t feed_add, rpad;   // feed cap: conversions kept (0: none)
     float a_eff[64];          // repulsion at this step's warm-up ramp

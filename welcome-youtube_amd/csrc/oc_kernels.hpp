@@ -106,7 +106,7 @@ __device__ __forceinline__ void step_params_wave(const RepInfo &R, RepStep &S, i
     S.kmu = hash_key(R.seed, st, 23);
     S.kpri = hash_key(R.seed, st, 24);
     S.kcap = hash_key(R.seed, st, 25);
-    S.dmax_bits = 0;
+    S.disp = 0;
 }
 
 __global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs,
