@@ -179,3 +179,38 @@ def test_long_run_invariants(eng):
     for k in ("syn", "rep", "mut", "dK", "dN", "dlip", "feed"):
         assert int((ev["kind"] == EVK[k]).sum()) > 0, k
     e.close()
+
+
+def test_one_check_on_a_loaded_state_vs_oracle(oracle, eng):
+    """oc_chem: one chemistry check (reactions, decay, capped feed) on a loaded
+    state, without a step, against the oracle's; the device sorts again
+    after the syntheses moved beads."""
+    p = _catalyst_params(seed=5, feed_cap=0.26, p_feed=0.2)
+    say(f"case: catalyst check on a loaded state, N={p.N}")
+    cp = p.to_c()
+    o = oracle.System(cp, p.N)
+    o.start(oracle.soup(cp, p.N))
+    o.step(3)
+    o.events()
+    st = o.get_state()
+    e = eng.Engine(p)
+    e.set_state(st, step=3)
+    o = oracle.System(cp, p.N)
+    o.set_state(st, 3)
+    e.chem(5)
+    o.chem(5)
+    a, b = e.get_state(), o.get_state()
+    for k in ("type", "mpos", "nxt"):
+        assert np.array_equal(a[k], b[k]), k
+    L = p.L
+    for k in ("x", "y"):
+        d = np.abs((a[k] - b[k] + L / 2) % L - L / 2)
+        assert d.max() < 1e-13, (k, d.max())
+    ed, eo = e.events(), o.events()
+    assert int((eo["kind"] == EVK["syn"]).sum()) > 0
+    for k in ("step", "kind", "gid_a", "gid_b", "x", "y"):
+        assert np.array_equal(ed[k], eo[k]), k
+    # the reloaded slot order is a valid sort: the next steps run
+    e.step(4)
+    assert np.isfinite(e.stats()["T_kin"])
+    e.close()

@@ -37,3 +37,27 @@ def test_error_paths_leave_the_engine_usable(eng):
     st = e.stats()
     assert np.isfinite(st["T_kin"]) and e.step_count() == 5
     e.close()
+
+
+def test_reference_chemistry_limits(eng):
+    """The window gradient refuses the reference chemistry (not
+    differentiated); a step that moves a bead by the chemistry margin 0.5
+    before a check is reported (the reactions' windows rely on the last
+    sort's cell index), and the engine keeps working after a state reload."""
+    abi = importlib.import_module("welcome-youtube_amd.abi")
+    oc = importlib.import_module("welcome-youtube_amd")
+    p = mkparams(16.0, f_lip=0.1, c_N=0.3, f_Ka=0.03, chem=1, n_chem=1, p_syn_a=0.5, seed=4)
+    e = eng.Engine(p)
+    e.start()
+    with pytest.raises(abi.OcError, match="reference chemistry"):
+        e.window_forward(10, oc.abi.rdf_spec(2, 2, 16, 3.0))
+    s = e.get_state()
+    fast = dict(s, vx=s["vx"].copy())
+    fast["vx"][5] = 60.0  # dt 0.01: a 0.6 move in one step
+    e.set_state(fast, step=0)
+    with pytest.raises(abi.OcError, match="moved 0.5 or more"):
+        e.step(1, graph=False)
+    e.set_state(s, step=0)
+    e.step(3)
+    assert np.isfinite(e.stats()["T_kin"])
+    e.close()

@@ -933,12 +933,16 @@ static int load_replica(oc_engine *h, int r, const oc_state *s, size_t base, boo
     return 0;
 }
 
+// step counter of a loaded state; a state load also clears the device error
+// word (an error a step reported leaves the engine usable after a reload)
 static int set_step_fields(oc_engine *h, int r, int64_t step) {
     RepStep rs;
     HIPCHK(hipMemcpy(&rs, h->d_rs + r, sizeof rs, hipMemcpyDeviceToHost));
     rs.step = step;
     rs.maxbond_bits = 0;
+    rs.disp = 0;
     HIPCHK(hipMemcpy(h->d_rs + r, &rs, sizeof rs, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(h->d_err, 0, sizeof(uint32_t)));
     return 0;
 }
 
