@@ -369,12 +369,13 @@ static void launch_build_prv(oc_engine *h) {
 
 // The reference chemistry of the replicas whose check it is: catalyst
 // reactions, decay, food supply (with its cap), in the reference's order.
-static void launch_rchem(oc_engine *h) {
+// list: list the active catalysts here (a step's k_integ lists them).
+static void launch_rchem(oc_engine *h, bool list) {
     if (!h->any_rchem) return;
     const int c = h->cur, nb = (int)h->bblk.size();
     StateP &S = h->st[c];
     if (h->any_react) {
-        k_react_list<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, S.gt, h->d_cat, h->d_catn);
+        if (list) k_react_list<<<nb, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, S.gt, h->d_cat, h->d_catn);
         ReactBufs B;
         B.xy = S.xy;
         B.v = S.v;
@@ -433,7 +434,7 @@ static void launch_keys(oc_engine *h, bool integ);
 static void launch_chem(oc_engine *h) {
     if (h->any_rchem) {
         launch_keys(h, false);
-        launch_rchem(h);
+        launch_rchem(h, true);
         launch_sort(h, false, 0, true);
     }
     launch_pchem(h);
@@ -494,8 +495,9 @@ static void enqueue_step(oc_engine *h) {
         launch_step_params(h, 1, -1);
         k_integ<<<(int)h->bblk.size(), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy,
                                                                     h->st[c].v, h->st[c].vt, h->d_f, h->d_key,
-                                                                    h->d_rmax, h->d_moved);
-        launch_rchem(h);
+                                                                    h->d_rmax, h->d_moved, h->st[c].gt, h->d_cat,
+                                                                    h->d_catn);
+        launch_rchem(h, false);
         launch_sort(h, false, 0, true);
         launch_pchem(h);
     } else {

@@ -4,8 +4,9 @@
 //
 //   k_integ        the step's integrate in place (cel/engine/oercel.c:2303-2334),
 //                  for engines that run the reference chemistry between the
-//                  integrate and the sort, with the largest displacement
-//   k_react_list   catalysts that drew a synthesis or a replication
+//                  integrate and the sort; at a check it also lists the
+//                  catalysts that drew a synthesis or a replication
+//   k_react_list   the same list for a check on a loaded state
 //   k_react_prep   one block per replica: the active catalysts sorted by their
 //                  hash priority (LDS bitonic sort), with their cells
 //   k_react_free   grid-wide, one wavefront per catalyst: every catalyst that
@@ -58,32 +59,43 @@ __device__ __forceinline__ double wrap_box(double v, double L) {
 // Integrate of every bead, in slot order and in place, with its new cell key
 // (against the cell of the last sort); flags a replica in which a bead moved
 // by the chemistry margin or more (cel/engine/oercel.c:2313-2317, 2337-2343).
+// At a check, also lists the active catalysts: a Ka / Kb whose draws
+// hash_u01(ksyn, gid) < p_syn or hash_u01(krep, gid) < p_rep (of its kind),
+// appended to the replica's list (any order: k_react_prep sorts them).
 __global__ __launch_bounds__(kBeadBlock) void k_integ(const RepInfo *__restrict__ ri, RepStep *__restrict__ rs,
                                                       const int32_t *__restrict__ bblk, double2 *__restrict__ xy,
                                                       double2 *__restrict__ v, float2 *__restrict__ vt,
                                                       const double2 *__restrict__ f, int32_t *__restrict__ key,
-                                                      int2 *__restrict__ rmax, uint32_t *__restrict__ moved) {
+                                                      int2 *__restrict__ rmax, uint32_t *__restrict__ moved,
+                                                      const uint32_t *__restrict__ gt, int32_t *__restrict__ cat,
+                                                      int32_t *__restrict__ catn) {
     const int r = bblk[blockIdx.x];
     const RepInfo &R = ri[r];
     const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
-    if (loc < R.N) {
-        const int s = R.off + loc;
-        const double2 p = xy[s], vv = v[s], ff = f[s];
-        const double ddx = R.dt * vv.x + R.dt2 * ff.x, ddy = R.dt * vv.y + R.dt2 * ff.y;
-        if (!(fmax(fabs(ddx), fabs(ddy)) < kChemMargin)) atomicOr((unsigned long long *)&rs[r].disp, 1ull);
-        double2 po, vo;
-        float2 t;
-        integrate1(R, p, vv, ff, po, vo, t);
-        xy[s] = po;
-        v[s] = vo;
-        vt[s] = t;
-        new_key(R, r, s, p, po, key, rmax, moved);
-    }
+    if (loc >= R.N) return;
+    const int s = R.off + loc;
+    const double2 p = xy[s], vv = v[s], ff = f[s];
+    const double ddx = R.dt * vv.x + R.dt2 * ff.x, ddy = R.dt * vv.y + R.dt2 * ff.y;
+    if (!(fmax(fabs(ddx), fabs(ddy)) < kChemMargin)) atomicOr((unsigned long long *)&rs[r].disp, 1ull);
+    double2 po, vo;
+    float2 t;
+    integrate1(R, p, vv, ff, po, vo, t);
+    xy[s] = po;
+    v[s] = vo;
+    vt[s] = t;
+    new_key(R, r, s, p, po, key, rmax, moved);
+    const RepStep &S = rs[r];
+    if (!cat || !S.rchem_now || !R.react) return;
+    const uint32_t w = gt[s], ty = w >> kGidBits, g = w & kGidMask;
+    if (ty != OC_KA && ty != OC_KB) return;
+    const int var = ty == OC_KB;
+    if (!(hash_u01(S.ksyn, g) < R.p_syn[var] || hash_u01(S.krep, g) < R.p_rep[var])) return;
+    const int k = atomicAdd(&catn[r], 1);
+    if (k < kCatCap) cat[(size_t)r * kCatCap + k] = s;
 }
 
-// Active catalysts of the replicas whose check it is: a Ka / Kb whose draws
-// hash_u01(ksyn, gid) < p_syn or hash_u01(krep, gid) < p_rep (of its kind),
-// appended to the replica's list (any order: k_react sorts them).
+// The active catalysts of a check on a loaded state (oc_chem), as k_integ
+// lists them in a step.
 __global__ __launch_bounds__(kBeadBlock) void k_react_list(const RepInfo *__restrict__ ri,
                                                            const RepStep *__restrict__ rs,
                                                            const int32_t *__restrict__ bblk,
