@@ -71,6 +71,10 @@ struct oc_engine {
     uint32_t *d_err = nullptr;
     hipStream_t stream = nullptr;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    // with the reference chemistry: one graph of n_chem steps from a check
+    // (chemistry kernels only in its last step), per starting buffer
+    hipGraphExec_t pexec[2] = {nullptr, nullptr};
+    hipGraph_t pgraph[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
     bool have_state = false, any_stir = false, any_chem = false, any_angle = false;
     bool any_tmpl = false;  // some replica has branched lipids (k_mol; the force kernel leaves no keys)
@@ -284,6 +288,10 @@ static void drop_graphs(oc_engine *h) {
         if (h->graph[k]) (void)hipGraphDestroy(h->graph[k]);
         h->gexec[k] = nullptr;
         h->graph[k] = nullptr;
+        if (h->pexec[k]) (void)hipGraphExecDestroy(h->pexec[k]);
+        if (h->pgraph[k]) (void)hipGraphDestroy(h->pgraph[k]);
+        h->pexec[k] = nullptr;
+        h->pgraph[k] = nullptr;
     }
 }
 
@@ -489,7 +497,9 @@ static void launch_step_params(oc_engine *h, int64_t delta, int64_t set_step) {
 // the previous cell index (catalyst reactions move beads), sort, then the
 // precursor chemistry and the forces; without it the integrate is fused into
 // the sort and every chemistry runs after it.
-static void enqueue_step(oc_engine *h) {
+// chem = false leaves out the chemistry kernels (a step that is no check of
+// any replica: they would find nothing to do).
+static void enqueue_step(oc_engine *h, bool chem = true) {
     if (h->any_rchem) {
         const int c = h->cur;
         launch_step_params(h, 1, -1);
@@ -497,9 +507,9 @@ static void enqueue_step(oc_engine *h) {
                                                                     h->st[c].v, h->st[c].vt, h->d_f, h->d_key,
                                                                     h->d_rmax, h->d_moved, h->st[c].gt, h->d_cat,
                                                                     h->d_catn);
-        launch_rchem(h, false);
+        if (chem) launch_rchem(h, false);
         launch_sort(h, false, 0, true);
-        launch_pchem(h);
+        if (chem) launch_pchem(h);
     } else {
         launch_sort(h, true, 1);
         launch_pchem(h);
@@ -1046,12 +1056,52 @@ static int capture_graph(oc_engine *h, int parity) {
     return 0;
 }
 
+// The n_chem steps from a check to the next one as one graph: the chemistry
+// kernels run only in its last step (the check).
+static int capture_period(oc_engine *h, int parity, int nc) {
+    const int saved = h->cur;
+    h->cur = parity;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 1; s <= nc; s++) enqueue_step(h, s == nc);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    h->cur = saved;
+    if (e != hipSuccess) return fail(OC_EHIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+    h->pgraph[parity] = g;
+    HIPCHK(hipGraphInstantiate(&h->pexec[parity], g, nullptr, nullptr, 0));
+    return 0;
+}
+
+// n_chem of every replica when they share it and their step counters agree
+// modulo it (then the check steps are known here), else 0; step: replica 0's
+static int common_period(oc_engine *h, int64_t *step) {
+    const int nc = h->ri[0].n_chem;
+    if (nc < 2 || nc > 1000) return 0;
+    std::vector<RepStep> rs(h->nrep);
+    if (hipMemcpy(rs.data(), h->d_rs, h->nrep * sizeof(RepStep), hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    for (int r = 0; r < h->nrep; r++)
+        if (h->ri[r].n_chem != nc || ((rs[r].step - rs[0].step) % nc) != 0) return 0;
+    *step = rs[0].step;
+    return nc;
+}
+
 OC_API int oc_step(oc_engine *h, int64_t n, int32_t use_graph) {
     if (!h) return fail(OC_EINVAL, "null engine");
     if (!h->have_state) return fail(OC_ESTATE, "no state: call oc_start or oc_set_state first");
     if (n <= 0) return 0;
-    int64_t k = 0;
-    if (use_graph && n >= 2) {
+    int64_t k = 0, step = 0;
+    const int nc = use_graph && h->any_rchem && n >= 2 ? common_period(h, &step) : 0;
+    if (nc) {
+        // full steps up to the next check, then whole periods from the graph
+        for (; k < n && step % nc != 0; k++, step++) enqueue_step(h);
+        for (; k + nc <= n; k += nc) {
+            const int p = h->cur;
+            if (!h->pexec[p])
+                if (int rc = capture_period(h, p, nc)) return rc;
+            HIPCHK(hipGraphLaunch(h->pexec[p], h->stream));
+            h->cur = p ^ (nc & 1);
+        }
+    } else if (use_graph && n >= 2) {
         const int p = h->cur;
         if (!h->gexec[p])
             if (int rc = capture_graph(h, p)) return rc;
