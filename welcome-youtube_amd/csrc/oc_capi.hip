@@ -706,6 +706,8 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
             const double per_row = (double)R.N / ((double)R.ncx * R.ncy);
             const int rows = std::max(1, std::min(kSortRows, (int)lround(192.0 / std::max(per_row, 1e-9))));
             const int nt = (R.ncy + rows - 1) / rows;
+            R.stile0 = (int32_t)h->stiles.size();
+            R.ntc = nt;
             for (int cx = 0; cx < R.ncx; cx++)
                 for (int k = 0; k < nt; k++) {
                     Tile T;
@@ -760,7 +762,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     rc = rc ? rc : dalloc(h, &h->d_tsrc, h->stiles.size() << kSortCapLog);
     rc = rc ? rc : dalloc(h, &h->d_tslot, Nt);
     rc = rc ? rc : dalloc(h, &h->d_locst, Ct);
-    rc = rc ? rc : dalloc(h, &h->d_moved, 1);
+    rc = rc ? rc : dalloc(h, &h->d_moved, 1 + h->stiles.size());
     rc = rc ? rc : dalloc(h, &h->d_rmax, h->nrep);
     rc = rc ? rc : dalloc(h, &h->d_tpart, (h->stiles.size() + kTScanChunk - 1) / kTScanChunk);
     rc = rc ? rc : dalloc(h, &h->d_epot, Nt);
@@ -815,7 +817,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     e = e ? e : hipMemcpy(h->d_rblk, h->rblk.data(), h->rblk.size() * 4, hipMemcpyHostToDevice);
     e = e ? e : hipMemset(h->d_evn, 0, sizeof(unsigned long long));
     e = e ? e : hipMemset(h->d_err, 0, 4);
-    e = e ? e : hipMemset(h->d_moved, 0, 4);
+    e = e ? e : hipMemset(h->d_moved, 0, 4 * (1 + h->stiles.size()));
     e = e ? e : hipMemset(h->d_rmax, 0, h->nrep * sizeof(int2));
     e = e ? e : hipMemset(h->d_ebond, 0, Nt * sizeof(double));
     e = e ? e : hipMemset(h->d_epot, 0, Nt * sizeof(double));

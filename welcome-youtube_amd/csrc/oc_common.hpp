@@ -55,6 +55,7 @@ struct RepInfo {
     int32_t rwx, rwy, owx, owy, lwx, lwy, cfx, cfy;
     int32_t fcap_on, fcap_n;  // feed cap on; floor(feed_cap N + 1e-9)
     int32_t rdx, rdy;         // cells (from the catalyst's) that hold every bead a reaction reads
+    int32_t stile0, ntc;      // first sort tile of the replica, sort tiles per column
     double rg2;               // squared radius around the catalyst of every bead a reaction reads
     double ro2;               // squared radius around the catalyst of every overlap-energy partner
     double tpx[OC_MAXMLEN], tpy[OC_MAXMLEN];  // template coordinates (centroid 0), branched lipids

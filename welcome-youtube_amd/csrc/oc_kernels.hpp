@@ -124,7 +124,10 @@ __global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ 
 // displacement rmax[r] (x, y); the sort then widens its window to it.  The
 // force kernel computes the same keys in its epilogue (new_key), so a step
 // whose forces came from it skips this launch.
-// key of bead s at q that sits in (local) cell (ox, oy) of the last sort
+// key of bead s at q that sits in (local) cell (ox, oy) of the last sort.
+// moved[0]: some bead moved by more than one cell; moved[1 + tile]: such a
+// bead arrives in that sort tile (which then collects its beads over the
+// replica's largest displacement)
 __device__ __forceinline__ void new_key_from(const RepInfo &R, int r, int s, int ox, int oy, double2 q, int32_t *key,
                                              int2 *rmax, uint32_t *moved) {
     const int cx = cell_x(R, q.x), cy = cell_y(R, q.y);
@@ -137,6 +140,11 @@ __device__ __forceinline__ void new_key_from(const RepInfo &R, int r, int s, int
         atomicMax(&rmax[r].x, dx);
         atomicMax(&rmax[r].y, dy);
         atomicOr(moved, 1u);
+        // the tile of column cx whose rows [k ncy / ntc, (k + 1) ncy / ntc) hold cy
+        int k = (int)(((int64_t)cy * R.ntc) / R.ncy);
+        while (k + 1 < R.ntc && ((int64_t)(k + 1) * R.ncy) / R.ntc <= cy) k++;
+        while (k > 0 && ((int64_t)k * R.ncy) / R.ntc > cy) k--;
+        atomicOr(&moved[1 + R.stile0 + cx * R.ntc + k], 1u);
     }
 }
 __device__ __forceinline__ void new_key(const RepInfo &R, int r, int s, double2 p, double2 q, int32_t *key,
@@ -216,7 +224,7 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
                                                     const int32_t *__restrict__ cs_old,
                                                     const int32_t *__restrict__ key,
                                                     const int2 *__restrict__ rmax,
-                                                    const uint32_t *__restrict__ moved,
+                                                    uint32_t *__restrict__ moved,
                                                     int32_t *__restrict__ tcount, int32_t *__restrict__ locst,
                                                     int32_t *__restrict__ tsrc, int32_t *__restrict__ tslot,
                                                     uint32_t *__restrict__ err, int nt) {
@@ -237,8 +245,9 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
         const int nrows = T.k1 - T.k0;
         const int c0 = R.coff + T.cx * ncy + T.k0;
         int Rx = 1, Ry = 1;
-        if (*moved) {
-            // the replica's largest displacement
+        if (*moved && moved[1 + tix]) {
+            // a bead arrives from further than one cell: the replica's
+            // largest displacement
             const int2 m = rmax[T.rep];
             Rx = max(1, m.x);
             Ry = max(1, m.y);
@@ -527,6 +536,7 @@ __global__ __launch_bounds__(1024) void k_tscan(const RepInfo *__restrict__ ri, 
         if (k < per && b0 + k < nt) {
             toff[b0 + k] = run;
             run += c8[k];
+            moved[1 + b0 + k] = 0u;  // the tile's far-arrival flag, read by k_tsort
         }
     if (blockIdx.x != 0) return;
     if (threadIdx.x == 0) *moved = 0u;
