@@ -13,8 +13,8 @@ arguments and files:
     restart continues the uninterrupted run bit for bit;
   - output cadences: stats.csv every stats_every steps (and step 0 of a fresh
     run), snap_<step>.bin every snap_every, events.csv (events = 1) with the
-    chemistry's events (precursor conv / form / break, reference dlip / dN /
-    feed), ckpt.bin every ckpt_every and at the end
+    chemistry's events (reference syn / rep / mut / dK / dlip / dN / feed,
+    precursor conv / form / break), ckpt.bin every ckpt_every and at the end
     (ckpt_<step>.bin copies with ckpt_keep = 1), a progress line in log.txt
     every log_every steps;
   - a restart into an existing directory truncates stats.csv and events.csv
