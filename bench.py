@@ -13,6 +13,12 @@ with a device synchronisation on both sides of the timed region.  The pair
 kernel is then timed alone with HIP events on the library's stream for the
 roofline; the reference engine (compiled from its source) is timed on the host
 cores on a bounded sample of the same physics as the CPU baseline.
+
+Other workloads (--workload): ensemble (256 replicas in one engine against a
+sequential loop), adjoint (window gradient against plain steps), fit (batched
+parameter fit), refchem (the reference's own default parameters with its
+chemistry -- catalysts, decay, capped feed -- at N = 1M, against the
+reference CPU engine on the same parameters).
 """
 import argparse
 import ctypes as C
