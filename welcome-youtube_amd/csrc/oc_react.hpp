@@ -859,6 +859,7 @@ __global__ __launch_bounds__(256) void k_rfeed_thr(const RepInfo *__restrict__ r
     __shared__ unsigned int hist[256];
     __shared__ uint64_t s_pre;
     __shared__ int s_rank, s_nN;
+    __shared__ bool s_one;
     if (t == 0) s_nN = 0;
     __syncthreads();
     {
@@ -896,6 +897,7 @@ __global__ __launch_bounds__(256) void k_rfeed_thr(const RepInfo *__restrict__ r
                 if (cum + (int)hist[b] > rank) {
                     s_pre = pre | ((uint64_t)b << sh);
                     s_rank = rank - cum;
+                    s_one = hist[b] == 1u;
                     break;
                 }
                 cum += (int)hist[b];
@@ -905,7 +907,18 @@ __global__ __launch_bounds__(256) void k_rfeed_thr(const RepInfo *__restrict__ r
         pre = s_pre;
         rank = s_rank;
         mask |= 0xffull << sh;
+        const bool one = s_one;
         __syncthreads();
+        if (one) {
+            // a single key has these leading digits: it is the threshold
+            for (int q = t; q < nc; q += 256) {
+                const uint64_t k = K[q];
+                if ((k & mask) == pre) s_pre = k;
+            }
+            __syncthreads();
+            pre = s_pre;
+            break;
+        }
     }
     if (t == 0) {
         S.feed_add = room;
