@@ -37,6 +37,8 @@ sys.path.insert(0, REPO)
 
 METRIC = "2D DPD particle-steps/s, N=1M ρ=3; batched-ensemble & adjoint vs plain"
 HBM_PEAK_GBS = 8000.0
+# the pair arithmetic is FP32; positions, velocities, force sums and reductions are FP64
+DTYPE = "fp32 pair arithmetic, fp64 state"
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "k_pair_N1M_traffic.json")
 
 
@@ -123,6 +125,38 @@ def time_pair_kernel(e, reps=50):
     return ms * 1e-3 / reps
 
 
+def host_info():
+    """Host CPU of this run: logical CPUs, the ones this process may use, the
+    model name and the OpenMP thread limit set in the environment."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return {"nproc": os.cpu_count() or 1, "usable_cores": usable, "cpu_model": model,
+            "omp_num_threads_env": int(omp) if omp.isdigit() else None}
+
+
+def cpu_threads(L, host):
+    """OpenMP threads of the reference CPU run: the reference's own rule
+    T = min(nproc, floor(L)/2) (at most floor(L)/2 column strips run at once,
+    cel/engine/oercel.c:750-756), with nproc the cores this process may use,
+    further capped by OMP_NUM_THREADS when the environment sets it."""
+    t = min(host["usable_cores"], int(math.floor(L)) // 2)
+    if host["omp_num_threads_env"]:
+        t = min(t, host["omp_num_threads_env"])
+    return max(1, t)
+
+
 def cpu_baseline(pm, N, nsub, target_s=12.0):
     """The reference engine (oracle/_ref, built from cel/engine/oercel.c with its
     own -O3 flags) on the host cores, same physics with P as its type N and
@@ -131,7 +165,8 @@ def cpu_baseline(pm, N, nsub, target_s=12.0):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
     p = config2_params(pm, N, nsub, chem=False)
-    threads = max(1, min(16, os.cpu_count() or 1, int(math.floor(p.L)) // 2))
+    host = host_info()
+    threads = cpu_threads(p.L, host)
     binp = oracle.ref_binary(fast=True)
     if os.path.exists(binp):
         ov = {"L": repr(p.L), "dt": "0.01", "chem": "0", "f_lip": "0.3", "lip_nH": "1", "lip_nT": "2",
@@ -152,8 +187,9 @@ def cpu_baseline(pm, N, nsub, target_s=12.0):
             shutil.rmtree(tmp, ignore_errors=True)
         return {"value": N / (ms * 1e-3), "unit": "particle-steps/s", "cores": threads, "kind": "reference",
                 "sample": f"reference CLI {os.path.basename(binp)} (cel/engine/oercel.c, -O3), N={N}, same physics "
-                          f"with P as type N, chemistry off, {steps} steps after setup, {threads} OpenMP threads, "
-                          f"{ms:.3f} ms/step"}
+                          f"with P as type N, chemistry off, {steps} steps after setup, {threads} OpenMP threads "
+                          f"(min of usable cores, floor(L)/2 strips and OMP_NUM_THREADS), {ms:.3f} ms/step",
+                "host": host}
     # restatement fallback
     cp = p.to_c()
     sysm = oracle.System(cp, N)
@@ -161,8 +197,8 @@ def cpu_baseline(pm, N, nsub, target_s=12.0):
     t0 = time.perf_counter()
     sysm.step(5)
     dt = (time.perf_counter() - t0) / 5
-    return {"value": N / dt, "unit": "particle-steps/s", "cores": os.cpu_count() or 1, "kind": "port",
-            "sample": f"oracle restatement, N={N}, chemistry off, 5 steps"}
+    return {"value": N / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle restatement, N={N}, chemistry off, 5 steps", "host": host}
 
 
 def ensemble_params(pm, nrep, seed=0):
@@ -181,46 +217,72 @@ def ensemble_params(pm, nrep, seed=0):
     return out
 
 
-def bench_ensemble(a, pm, eng):
+def ensemble_measure(pm, eng, replicas, steps, equil, tune=True):
     """Throughput of R replicas stepped together by one engine (one launch per
-    phase for all replicas, hipGraph replay) vs. the same replicas stepped one
-    after another by single-replica engines (also graph replay)."""
-    ps = ensemble_params(pm, a.replicas)
-    if a.tune:
+    phase for all replicas, hipGraph replay) against the same replicas stepped
+    one after another by single-replica engines (also graph replay), both
+    timed over the same number of steps from the same equilibrated states:
+    the batched engine runs `equil` untimed steps from the soups, and every
+    single engine is loaded with its replica's state in slot order."""
+    ps = ensemble_params(pm, replicas)
+    t_tune = 0.0
+    if tune:
         at = importlib.import_module("welcome-youtube_amd.autotune")
-        ps, blk, tb = at.tune_ensemble(ps)
+        t0 = time.perf_counter()
+        ps, blk, tb = at.tune_ensemble(ps, verbose=True)
+        t_tune = time.perf_counter() - t0
     else:
-        blk, tb = a.pair_block, a.tile_beads
+        blk, tb = 0, 0
     Ntot = sum(p.N for p in ps)
     e = eng.Engine(ps, pair_block=blk, tile_beads=tb)
     e.start()
-    e.step(a.warmup)
+    e.step(equil)
+    states = [(e.get_state(r), e.cells(r)[1], e.step_count(r)) for r in range(len(ps))]
+    e.step(2)  # graph warm
     e.step_count()
     t0 = time.perf_counter()
-    e.step(a.steps)
+    e.step(steps)
+    e.step_count()
     batched = time.perf_counter() - t0
+    T_end = [e.stats(r)["T_kin"] for r in range(len(ps))]
     e.close()
-    singles = [eng.Engine(p) for p in ps]
-    for s in singles:
-        s.start()
-        s.step(a.warmup)
+    singles = []
+    for p, (s, order, st) in zip(ps, states):
+        q = eng.Engine(p)
+        q.set_state(s, step=st, slot_gid=order)
+        q.step(2)  # graph warm
+        singles.append(q)
+    for q in singles:
+        q.step_count()
     t0 = time.perf_counter()
-    for s in singles:
-        s.step(a.steps)
+    for q in singles:
+        q.step(steps)
+    for q in singles:
+        q.step_count()
     seq = time.perf_counter() - t0
-    for s in singles:
-        s.close()
+    for q in singles:
+        q.close()
+    return {
+        "value": Ntot * steps / batched, "unit": "particle-steps/s", "steps": steps, "equil": equil,
+        "ms_per_step": 1e3 * batched / steps,
+        "sequential_value": Ntot * steps / seq, "sequential_ms_per_step": 1e3 * seq / steps,
+        "speedup_vs_sequential": seq / batched,
+        "workload": f"batched ensemble, {replicas} replicas, N log-uniform [2048, 65536], rho=3, f_lip U[0.05,0.4], "
+                    "a_WT U[60,100], distinct seeds, precursor chemistry on, hipGraph replay",
+        "N_total": Ntot, "N_min": min(p.N for p in ps), "N_max": max(p.N for p in ps),
+        "autotuned": bool(tune), "tune_s": t_tune, "pair_block": blk, "tile_beads": tb,
+        "nsub": sorted({p.nsub for p in ps}),
+        "T_kin_end_mean": sum(T_end) / len(T_end),
+    }
+
+
+def bench_ensemble(a, pm, eng):
+    d = ensemble_measure(pm, eng, a.replicas, a.steps, a.equil, tune=not a.no_tune)
     out = {
         "metric": "batched-ensemble particle-steps/s (vs a sequential loop of single-replica engines)",
-        "value": Ntot * a.steps / batched, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": 1e3 * batched / a.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": f"batched ensemble, {a.replicas} replicas, N log-uniform [2048, 65536], rho=3, "
-                               f"f_lip U[0.05,0.4], a_WT U[60,100], precursor chemistry on",
-                   "N_total": Ntot, "pair_block": blk, "tile_beads": tb,
-                   "nsub": sorted({p.nsub for p in ps})},
-        "sequential_value": Ntot * a.steps / seq,
-        "speedup_vs_sequential": seq / batched,
+        "value": d["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.equil,
+        "ms_per_step": d["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": DTYPE, "data": "synthetic", "config": {"workload": d["workload"]}, "ensemble": d,
     }
     print(json.dumps(out), flush=True)
 
@@ -234,33 +296,35 @@ def adjoint_params(pm, N=65536, seed=1, **over):
     return p.finalize()
 
 
-def bench_adjoint(a, pm, eng):
-    """Cost of a 200-step window gradient (forward with checkpoints every 20
-    steps + the adjoint sweep with recomputation) against 200 plain steps."""
+def adjoint_measure(pm, eng, nwin, equil):
+    """Cost of a window gradient (forward with checkpoints every 20 steps +
+    the adjoint sweep with recomputation) against `nwin` plain graph-replayed
+    steps of the same system from the same state, N = 65,536."""
     abi = importlib.import_module("welcome-youtube_amd.abi")
     spec = abi.rdf_spec(2, 2, 64, 3.0)
-    nwin = a.window
     # target: the tail-tail RDF of a seed run at a_TT = 20, a_WT = 90
     pt = adjoint_params(pm, seed=2, a_T_T=20, a_W_T=90)
     with eng.Engine(pt) as et:
         et.start()
-        et.step(a.equil)
+        et.step(equil)
         target = et.window_forward(nwin, spec)[0]
     p = adjoint_params(pm)
     e = eng.Engine(p)
     e.start()
-    e.step(a.equil)
+    e.step(equil)
     st0 = e.get_state()
     s0 = e.step_count()
+    order = e.cells()[1]
     e.step(nwin)  # warm the graph
-    e.set_state(st0, step=s0)
+    e.set_state(st0, step=s0, slot_gid=order)
     e.step_count()
     t0 = time.perf_counter()
     e.step(nwin)
+    e.step_count()
     t_fwd = time.perf_counter() - t0
-    e.set_state(st0, step=s0)
+    e.set_state(st0, step=s0, slot_gid=order)
     e.window_forward(nwin, spec)  # warm buffers
-    e.set_state(st0, step=s0)
+    e.set_state(st0, step=s0, slot_gid=order)
     t0 = time.perf_counter()
     gbar = e.window_forward(nwin, spec)
     t1 = time.perf_counter()
@@ -269,16 +333,25 @@ def bench_adjoint(a, pm, eng):
     t2 = time.perf_counter()
     loss = float(((gbar[0] - target) ** 2).sum())
     e.close()
+    return {
+        "value": (t2 - t0) / t_fwd, "unit": "x forward", "higher_is_better": False, "window": nwin, "equil": equil,
+        "workload": "adjoint window: N=65,536, rho=3, 30% H-T-T lipids, k_angle=0, chemistry off, "
+                    f"{nwin}-step window, checkpoint every 20, tail-tail RDF 64 bins on [0,3), target from a seed "
+                    "run at a_TT=20, a_WT=90",
+        "N": p.N, "forward_ms": 1e3 * t_fwd, "window_forward_ms": 1e3 * (t1 - t0),
+        "window_backward_ms": 1e3 * (t2 - t1), "loss": loss,
+        "grad": {n: float(g) for n, g in zip(eng.Engine.THETA_NAMES, grad)},
+    }
+
+
+def bench_adjoint(a, pm, eng):
+    d = adjoint_measure(pm, eng, a.window, a.equil)
     out = {
         "metric": "adjoint window: gradient cost vs plain forward steps",
-        "value": (t2 - t0) / t_fwd, "unit": "x forward", "n_gpus": 1, "steps": nwin, "warmup": a.equil,
-        "ms_per_step": 1e3 * (t2 - t0) / nwin, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "adjoint window: N=65,536, rho=3, 30% H-T-T lipids, k_angle=0, chemistry off, "
-                               f"{nwin}-step window, checkpoint every 20, tail-tail RDF 64 bins on [0,3), target "
-                               "from a seed run at a_TT=20, a_WT=90", "N": p.N},
-        "forward_ms": 1e3 * t_fwd, "window_forward_ms": 1e3 * (t1 - t0), "window_backward_ms": 1e3 * (t2 - t1),
-        "loss": loss, "grad": {n: float(g) for n, g in zip(eng.Engine.THETA_NAMES, grad)},
+        "value": d["value"], "unit": "x forward", "n_gpus": 1, "steps": a.window, "warmup": a.equil,
+        "ms_per_step": (d["window_forward_ms"] + d["window_backward_ms"]) / a.window, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPE, "data": "synthetic",
+        "config": {"workload": d["workload"], "N": d["N"]}, "adjoint": d,
     }
     print(json.dumps(out), flush=True)
 
@@ -330,7 +403,7 @@ def bench_fit(a, pm, eng):
         "metric": "batched parameter fit: replica particle-steps/s through the window gradient",
         "value": R * N * nwin / tw, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.equil,
         "ms_per_step": 1e3 * total / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic",
+        "dtype": DTYPE, "data": "synthetic",
         "config": {"workload": f"batched parameter fit: {R} replicas x {N:,} beads (rho=3, 30% H-T-T lipids, "
                                f"k_angle=0, chemistry off, distinct seeds), {nwin}-step adjoint windows, checkpoint "
                                f"every 20, tail-tail RDF 64 bins on [0,3), replica-mean gradient, {a.steps} Adam "
@@ -379,7 +452,8 @@ def bench_refchem(a, pm, eng):
     import oracle
     binp = oracle.ref_binary(fast=True)
     if not a.no_cpu and os.path.exists(binp):
-        threads = max(1, min(16, os.cpu_count() or 1))
+        host = host_info()
+        threads = cpu_threads(p.L, host)
         ov = {"L": repr(p.L), "nsub": str(a.nsub), "seed": "1", "snap_every": "0", "stats_every": "0",
               "ckpt_every": "0", "log_every": "0", "events": "0", "threads": str(threads)}
         tmp = tempfile.mkdtemp(prefix="ocref_")
@@ -396,12 +470,12 @@ def bench_refchem(a, pm, eng):
         cpu = {"value": N / (ms * 1e-3), "unit": "particle-steps/s", "cores": threads, "kind": "reference",
                "sample": f"reference CLI {os.path.basename(binp)} (cel/engine/oercel.c, -O3), N={N}, its default "
                          f"parameters (chemistry on), {steps} steps from a fresh soup, {threads} OpenMP threads, "
-                         f"{ms:.3f} ms/step"}
+                         f"{ms:.3f} ms/step", "host": host}
     out = {
         "metric": "2D DPD particle-steps/s with the reference's default chemistry, N=1M",
         "value": N * a.steps / sec, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * sec / a.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "vs_baseline": None, "dtype": DTYPE, "data": "synthetic",
         "config": {"workload": f"the reference's default parameters at N={N:,} (L={p.L:.4f}): chem=1 with "
                                "catalysts (synthesis, replication), decay and the capped food supply every 10 "
                                "steps, 10% H-T-T lipids, dt=0.03, warm-up ramp",
@@ -435,10 +509,14 @@ def main():
     ap.add_argument("--no-rewind", action="store_true",
                     help="fit workload: continue the trajectories from window to window instead of restarting every "
                          "window from the equilibrated state")
-    ap.add_argument("--tune", action="store_true", help="autotune nsub / pair launch shape first")
+    ap.add_argument("--no-tune", action="store_true", help="ensemble: library launch shape, no autotuner")
+    ap.add_argument("--no-sub", action="store_true",
+                    help="plain workload: skip the ensemble and adjoint sub-measurements")
+    ap.add_argument("--ens-steps", type=int, default=200, help="timed steps of the ensemble sub-measurement")
+    ap.add_argument("--ens-equil", type=int, default=500, help="untimed steps of the ensemble sub-measurement")
     a = ap.parse_args()
     fitw = a.workload == "fit"
-    a.steps = a.steps if a.steps is not None else (50 if fitw else 500)
+    a.steps = a.steps if a.steps is not None else (50 if fitw else (200 if a.workload == "ensemble" else 500))
     a.N = a.N if a.N is not None else (262144 if fitw else 1048576)
     a.replicas = a.replicas if a.replicas is not None else (32 if fitw else 256)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -460,26 +538,50 @@ def main():
         return bench_refchem(a, pm, eng)
     p = config2_params(pm, a.N, a.nsub)
     N = p.N
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
     e = eng.Engine(p, pair_block=a.pair_block, tile_beads=a.tile_beads, pair_variant=a.pair_variant)
     e.start()
+    # untimed equilibration from the random soup (no warm-up ramp), so the
+    # timed steps run at the steady state (kT ~ 1), then the user's warm-up
+    e.step(a.equil, graph=not a.no_graph)
+    e.events()  # drain
+    st0 = e.stats()  # T_kin at the start of the warm-up steps
     e.step(a.warmup, graph=not a.no_graph)
-    e.step_count()  # device synchronised
+    s0 = e.step_count()  # device synchronised; events of steps > s0 are the timed region's
+    log(f"plain N={N}: {a.equil} + {a.warmup} untimed steps, T_kin {st0['T_kin']:.4f}")
     t0 = time.perf_counter()
     e.step(a.steps, graph=not a.no_graph)  # returns after a stream synchronisation
     t1 = time.perf_counter()
     sec = t1 - t0
     st = e.stats()
     ev = e.events()
+    ev = ev[ev["step"] > s0]
+    log(f"plain: {1e3 * sec / a.steps:.4f} ms/step, T_kin {st['T_kin']:.4f}")
     pair_s = time_pair_kernel(e)
     ncell = e.cells()[0].size - 1
-    # algorithmic bytes of one force launch: staged xy (16), vt (8), gid/type
-    # (4), bonded links mp (1), lnk (4), prv (4), the half-step velocity the
-    # next-key epilogue integrates (16), force (16) and next cell key (4)
-    # writes per bead, and the cell starts
-    bytes_pair = N * (16 + 8 + 4 + 1 + 4 + 4 + 16 + 16 + 4) + 4 * (ncell + 1)
-    achieved = bytes_pair / pair_s / 1e9
+    # algorithmic bytes of one force launch, per bead at the compulsory
+    # layout: the pair force (FP32 cell-relative position + predicted velocity
+    # 16, gid|type 4, force 8, cell starts ~1.3), the bonded terms of the 30%
+    # lipid beads (~11) and the next step's cell key (4)
+    bytes_alg = int(N * (16 + 4 + 8 + 11 + 4)) + 4 * (ncell + 1)
+    # bytes the kernel must move in this build's layout: staged xy (FP64 16),
+    # vt (8), gid|type (4), links mp (1), lnk (4), prv (4), the half-step
+    # velocity the next-key epilogue integrates (16), force (16) and key (4)
+    bytes_layout = N * (16 + 8 + 4 + 1 + 4 + 4 + 16 + 16 + 4) + 4 * (ncell + 1)
+    achieved = bytes_alg / pair_s / 1e9
     e.close()
-
+    sub = {}
+    if not a.no_sub:
+        log("ensemble sub-workload ...")
+        sub["ensemble"] = ensemble_measure(pm, eng, 256, a.ens_steps, a.ens_equil, tune=not a.no_tune)
+        log(f"ensemble: {sub['ensemble']['value']:.3e} particle-steps/s, "
+            f"{sub['ensemble']['speedup_vs_sequential']:.2f}x sequential")
+        log("adjoint sub-workload ...")
+        sub["adjoint"] = adjoint_measure(pm, eng, a.window, 1000)
+        log(f"adjoint: {sub['adjoint']['value']:.3f}x forward")
     cpu = None if a.no_cpu else cpu_baseline(pm, N, a.nsub)
     out = {
         "metric": METRIC,
@@ -492,24 +594,28 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": DTYPE,
         "data": "synthetic",
         "config": {
             "workload": f"plain 2D DPD step at scale: N={N:,}, rho=3, L={p.L:.4f}, 50% W / 20% P / 30% H-T-T "
-                        "lipid beads, precursor chemistry every step, dt=0.01, k_angle=0, warmup=0",
-            "N": N, "L": p.L, "nsub": a.nsub, "graph": not a.no_graph,
+                        "lipid beads, precursor chemistry every step, dt=0.01, k_angle=0, no warm-up ramp, "
+                        f"{a.equil} untimed equilibration steps from the random soup before the warm-up",
+            "N": N, "L": p.L, "nsub": a.nsub, "graph": not a.no_graph, "equil": a.equil,
             "state": "fp64 positions/velocities/forces, fp32 pair arithmetic",
-            "events_in_timed_region": int(len(ev)), "T_kin_end": st["T_kin"],
+            "T_kin_start": st0["T_kin"], "T_kin_end": st["T_kin"],  # start: before the warm-up steps
+            "events_in_timed_region": int(len(ev)),
             "momentum_end": [st["px"], st["py"]],
         },
         "roofline": {
             "bound": "hbm", "kernel": "k_pair", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(N),
-            "avg_launch_us": pair_s * 1e6, "algorithmic_bytes_per_launch": bytes_pair,
+            "avg_launch_us": pair_s * 1e6, "algorithmic_bytes_per_launch": bytes_alg,
+            "layout_bytes_per_launch": bytes_layout,
             "limiter": "VALU issue (hash and force arithmetic per candidate pair), not HBM",
         },
         "cpu_baseline": cpu,
     }
+    out.update(sub)
     print(json.dumps(out), flush=True)
 
 

@@ -16,6 +16,7 @@ engine of the same class reuses them without timing.
 import json
 import math
 import os
+import sys
 import time
 
 from . import engine as _engine
@@ -73,7 +74,7 @@ def tune(params, steps=20, use_cache=True, verbose=False):
         p.nsub = ns
         t = _time_steps([p], 256, 0, steps)
         if verbose:
-            print(f"  nsub={ns}: {t * 1e3:.3f} ms/step")
+            print(f"  {key} nsub={ns}: {t * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
         if best is None or t < best[0]:
             best = (t, ns, 256, 0)
     # 2) pair workgroup and tile size at that cell height
@@ -84,7 +85,7 @@ def tune(params, steps=20, use_cache=True, verbose=False):
             tb = int(b * fct)
             t = _time_steps([p], b, tb, steps)
             if verbose:
-                print(f"  block={b} tile={tb}: {t * 1e3:.3f} ms/step")
+                print(f"  {key} block={b} tile={tb}: {t * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
             if t < best[0]:
                 best = (t, best[1], b, tb)
     t, ns, b, tb = best
@@ -112,7 +113,7 @@ def tune_ensemble(params_list, steps=10, verbose=False):
             tb = int(b * fct)
             t = _time_steps(out, b, tb, steps)
             if verbose:
-                print(f"  ensemble block={b} tile={tb}: {t * 1e3:.3f} ms/step")
+                print(f"  ensemble block={b} tile={tb}: {t * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
             if best is None or t < best[0]:
                 best = (t, b, tb)
     return out, best[1], best[2]
