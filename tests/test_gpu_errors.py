@@ -29,7 +29,7 @@ def test_error_paths_leave_the_engine_usable(eng):
     with pytest.raises(abi.OcError, match="max bond"):
         e.max_bond(0, set_to=-1.0)
     with pytest.raises(abi.OcError, match="pair variant"):
-        e._chk(e.lib.oc_set_pair_variant(e.h, 3))
+        e._chk(e.lib.oc_set_pair_variant(e.h, 4))
     with pytest.raises(abi.OcError):
         e.set_tuning(pair_block=96)
     e.set_state(s, step=0, slot_gid=e.cells()[1])

@@ -2,8 +2,14 @@
 
 Bars: cell lists, integration and chemistry events bit-exact on identical
 inputs; pair forces within 1e-5 of the oracle relative to the per-bead force
-scale sum_j |f_ij| (and to max |F|), with the same per-pair noise (the oracle
-draws theta from the same 32-bit cut of the same hash).
+scale sum_j |f_ij| (plus an absolute 1e-5 in force units) and to max |F|, with
+the same per-pair noise (the oracle draws theta from the same 32-bit cut of
+the same hash).  The absolute term is the resolution of FP32 pair arithmetic
+at the cutoff: r and w = 1 - r carry ~6e-8 absolute, so a pair near r = 1
+carries ~6e-8 * (a_ij + sigma theta / sqrt(dt)) ~ 4e-6 whatever its force; a
+bead whose only pairs sit there (sum_j |f_ij| ~ 0.04 in a sparse region) has
+no relative bar an FP32 kernel can meet, while typical beads have
+sum_j |f_ij| ~ 1e2, where the term is 1e-7 relative.
 """
 import importlib
 import sys
@@ -16,6 +22,7 @@ from conftest import mkparams
 pytestmark = pytest.mark.gpu
 
 FORCE_TOL = 1e-5
+FORCE_ABS = 1e-5  # force units: the FP32 resolution of pairs at the cutoff (module docstring)
 
 
 def say(*a):
@@ -45,14 +52,14 @@ def device_forces(eng, p, s, step, bonded, variant=0):
 
 def check_forces(fx, fy, ofx, ofy, sabs, what):
     d = np.hypot(fx - ofx, fy - ofy)
-    scale = np.maximum(sabs, 1e-12)
+    scale = np.maximum(sabs, 1e-12) + FORCE_ABS / FORCE_TOL
     rel = (d / scale).max()
     glob = d.max() / np.hypot(ofx, ofy).max()
     say(f"{what}: max |dF|/sum|f_ij| = {rel:.3e}, max|dF|/max|F| = {glob:.3e}, max|dF| = {d.max():.3e}")
     assert rel < FORCE_TOL and glob < FORCE_TOL
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("L,nsub", [(9.0, 1), (12.5, 4), (23.7, 2), (31.3, 8), (47.1, 4)])
 def test_pair_forces_vs_oracle(oracle, eng, rng, L, nsub, variant):
     """every pair kernel variant (single pass; per-lane cutoff test then the
@@ -72,7 +79,7 @@ def test_pair_forces_vs_oracle(oracle, eng, rng, L, nsub, variant):
     assert abs(out["fy"].sum()) < 1e-9 * np.abs(out["fy"]).sum() + 1e-9
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_pair_forces_dense_clump(oracle, eng, rng, variant):
     """A clump of 900 beads in a 3 x 2 area: the pair tiles around it stage
     more candidates than their LDS buffer holds and run from global memory."""

@@ -9,6 +9,7 @@
 
 #include "oc_adjoint.hpp"
 #include "oc_mol.hpp"
+#include "oc_pairfx.hpp"
 #include "oc_react.hpp"
 
 using namespace oc;
@@ -63,7 +64,11 @@ struct oc_engine {
     double *d_stir = nullptr, *d_epot = nullptr, *d_ebond = nullptr, *d_part = nullptr, *d_statout = nullptr;
     uint8_t *d_flag = nullptr;
     int32_t *d_best = nullptr;
-    int32_t *d_act = nullptr, *d_actn = nullptr;  // per-chunk lists of beads with a chemistry decision
+    // precursor chemistry: the check's list of beads with a role, in segments
+    // (one per placement tile, or per bead block of k_chem_list)
+    int32_t *d_clist = nullptr;
+    ChemSeg *d_cseg = nullptr;
+    int cseg_n = 0;  // segments of the last list
     Event *d_ev = nullptr;
     unsigned long long *d_evn = nullptr;
     int64_t evcap = 0;
@@ -336,6 +341,20 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false) {
         else if (h->any_angle) k_pair<B, C, CL, false, true><<<grid, B, 0, h->stream>>>(A);       \
         else k_pair<B, C, CL, false, false><<<grid, B, 0, h->stream>>>(A);                        \
     } while (0)
+#define OC_PAIRFX(B, C)                                                                          \
+    do {                                                                                         \
+        if (h->any_angle) k_pair_fx<B, C, kPairList, true><<<grid, B, 0, h->stream>>>(A);         \
+        else k_pair_fx<B, C, kPairList, false><<<grid, B, 0, h->stream>>>(A);                     \
+    } while (0)
+    if (V == 3 && !energy) {
+        switch (h->pair_block) {
+        case 64: OC_PAIRFX(64, 224); break;
+        case 128: OC_PAIRFX(128, 448); break;
+        default: OC_PAIRFX(256, 896); break;
+        }
+        return;
+    }
+#undef OC_PAIRFX
     switch (h->pair_block) {
     case 64:
         if (V == 2) OC_PAIR(64, 224, 2); else if (V == 1) OC_PAIR(64, 224, 1); else OC_PAIR(64, 224, 0);
@@ -423,16 +442,21 @@ static void launch_rchem(oc_engine *h, bool list) {
 }
 
 // the precursor chemistry (after the sort, on the fresh cell list)
-static void launch_pchem(oc_engine *h) {
+static void launch_pchem(oc_engine *h, bool listed) {
     if (!h->any_chem) return;
     const int c = h->cur;
-    const int nk = (int)h->rblk.size();
-    k_chem_decide<<<nk, kChemThreads, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_rblk, h->d_cs[c], h->st[c].xy,
-                                                      h->st[c].gt, h->st[c].mp, h->st[c].lnk, h->d_flag, h->d_best,
-                                                      h->d_act, h->d_actn);
-    k_chem_apply<<<nk, 64, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_rblk, h->st[c].gt, h->st[c].lnk, h->d_flag,
-                                           h->d_best, h->d_act, h->d_actn, h->d_ev, h->d_evn, h->evcap, h->record,
-                                           h->d_err);
+    if (!listed) {
+        // a state that was not just placed: list the roles of every bead
+        h->cseg_n = (int)h->bblk.size();
+        k_chem_list<<<h->cseg_n, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].gt, h->st[c].mp,
+                                                             h->st[c].lnk, h->d_clist, h->d_cseg);
+    }
+    // one wavefront per segment (XCD bands: a multiple of 8 workgroups)
+    const int ng = h->cseg_n >= 8 ? (std::min(h->cseg_n, kChemGrid) + 7) & ~7 : h->cseg_n;
+    k_chem_decide<<<ng, 64, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_cs[c], h->st[c].xy, h->st[c].gt, h->st[c].mp,
+                                            h->st[c].lnk, h->d_clist, h->d_cseg, h->cseg_n, h->d_flag, h->d_best);
+    k_chem_apply<<<ng, 64, 0, h->stream>>>(h->d_rs, h->st[c].gt, h->st[c].lnk, h->d_flag, h->d_best, h->d_clist,
+                                           h->d_cseg, h->cseg_n, h->d_ev, h->d_evn, h->evcap, h->record, h->d_err);
 }
 
 // one chemistry check on the current (sorted) state; the reference
@@ -445,7 +469,7 @@ static void launch_chem(oc_engine *h) {
         launch_rchem(h, true);
         launch_sort(h, false, 0, true);
     }
-    launch_pchem(h);
+    launch_pchem(h, h->any_rchem);  // the placement lists the roles
 }
 
 // sort (with integration when integ): keys, tile sort, tile offsets,
@@ -467,22 +491,26 @@ static void launch_sort(oc_engine *h, bool integ, int step_delta, bool keys_read
     // the force kernel left the keys of this state's integrate unless the
     // state changed since (keys_ready: the caller's kernels wrote them)
     if (!keys_ready && !(integ && h->keys_fresh)) launch_keys(h, integ);
-    const int ng = std::min(nt, kSortGrid);
+    // a multiple of 8 workgroups (one band of tiles per XCD) when there are 8 tiles or more
+    const int ng = nt >= 8 ? std::min(nt, kSortGrid) & ~7 : nt;
     k_tsort<<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_cs[c], h->d_key, h->d_rmax, h->d_moved,
                                              h->d_tcount, h->d_locst, h->d_tsrc, h->d_tslot, h->d_err, nt);
     const int nch = (nt + kTScanChunk - 1) / kTScanChunk, per = nch > 1 ? 8 : (nt + 1023) / 1024;
     if (nch > 1) k_tsum<<<nch, 1024, 0, h->stream>>>(h->d_tcount, nt, per, h->d_tpart);
     k_tscan<<<nch, 1024, 0, h->stream>>>(h->d_ri, h->d_rs, h->nrep, h->d_tcount, nt, per,
                                          nch > 1 ? h->d_tpart : nullptr, h->d_toff, h->d_moved, h->d_rmax, step_delta);
+    // the placement lists the precursor chemistry's roles of the replicas whose check it is
+    int32_t *cl = h->any_chem ? h->d_clist : nullptr;
+    if (cl) h->cseg_n = nt;
     int32_t *src = h->rec_src;
     if (integ)
         k_place<true><<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,
                                                        h->d_tsrc, h->d_tslot, h->d_cs[n], src, h->st[c],
-                                                       h->st[n], h->d_f, h->d_err, nt);
+                                                       h->st[n], h->d_f, h->d_err, nt, h->d_rs, cl, h->d_cseg);
     else
         k_place<false><<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,
                                                         h->d_tsrc, h->d_tslot, h->d_cs[n], src, h->st[c],
-                                                        h->st[n], h->d_f, h->d_err, nt);
+                                                        h->st[n], h->d_f, h->d_err, nt, h->d_rs, cl, h->d_cseg);
     h->cur = n;
     h->keys_fresh = false;
 }
@@ -509,10 +537,10 @@ static void enqueue_step(oc_engine *h, bool chem = true) {
                                                                     h->d_catn);
         if (chem) launch_rchem(h, false);
         launch_sort(h, false, 0, true);
-        if (chem) launch_pchem(h);
+        if (chem) launch_pchem(h, true);
     } else {
         launch_sort(h, true, 1);
-        launch_pchem(h);
+        launch_pchem(h, true);
     }
     launch_forces(h, true);
 }
@@ -771,8 +799,8 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     rc = rc ? rc : dalloc(h, &h->d_statout, (size_t)nrep * 16);
     rc = rc ? rc : dalloc(h, &h->d_flag, Nt);
     rc = rc ? rc : dalloc(h, &h->d_best, Nt);
-    rc = rc ? rc : dalloc(h, &h->d_act, h->rblk.size() * (size_t)kRedChunk);
-    rc = rc ? rc : dalloc(h, &h->d_actn, h->rblk.size());
+    rc = rc ? rc : dalloc(h, &h->d_clist, h->any_chem ? Nt : 1);
+    rc = rc ? rc : dalloc(h, &h->d_cseg, h->any_chem ? std::max(h->stiles.size(), h->bblk.size()) : 1);
     rc = rc ? rc : dalloc(h, &h->d_ev, (size_t)h->evcap);
     rc = rc ? rc : dalloc(h, &h->d_evn, 1);
     rc = rc ? rc : dalloc(h, &h->d_err, 1);
@@ -853,7 +881,7 @@ OC_API void *oc_stream(oc_engine *h) { return h ? (void *)h->stream : nullptr; }
 
 OC_API int oc_set_pair_variant(oc_engine *h, int32_t lists) {
     if (!h) return fail(OC_EINVAL, "null engine");
-    if (lists < 0 || lists > 2) return fail(OC_EINVAL, "pair variant must be 0, 1 or 2");
+    if (lists < 0 || lists > 3) return fail(OC_EINVAL, "pair variant must be 0, 1, 2 or 3");
     h->pair_variant = lists;
     drop_graphs(h);
     return 0;

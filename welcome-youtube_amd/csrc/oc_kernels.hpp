@@ -172,6 +172,30 @@ __global__ __launch_bounds__(kBeadBlock) void k_keys(const RepInfo *__restrict__
     new_key(R, r, s, p, q, key, rmax, moved);
 }
 
+// Chemistry role of a bead at a check (own definition): bit 0 -- the bead
+// breaks its chemistry bond (the bond's smaller gid draws hash_u01(kbreak) <
+// p_break); bit 1 -- it searches its neighbourhood (a P whose draw
+// hash_u01(kconv) < p_conv looks for a lipid tail, a free T -- type T, in no
+// lipid, unbonded -- for a partner).  wp: the gid word of the chemistry
+// partner (read only for a bonded bead outside lipids).
+constexpr int kRoleShift = 30;
+__device__ __forceinline__ int chem_role(const RepInfo &R, const RepStep &S, uint32_t w, int m, int l, uint32_t wp) {
+    const uint32_t g = w & kGidMask, ty = w >> kGidBits;
+    const bool brk = m < 0 && l >= 0 && g < (wp & kGidMask) && hash_u01(S.kbreak, g) < R.p_break;
+    const bool conv = ty == OC_N && hash_u01(S.kconv, g) < R.p_conv;
+    const bool form = ty == OC_T && m < 0 && l < 0;
+    return (brk ? 1 : 0) | (conv || form ? 2 : 0);
+}
+
+// The check's list is kept in segments: a producer block (a placement tile, or
+// a bead block of k_chem_list) writes the entries of its beads with a role
+// (slot | role << 30, in any order: every decision is per bead) to
+// list[base .. base + cnt), base = its first slot, and seg = (base, cnt,
+// replica); LDS counters only, no global atomics.
+struct ChemSeg {
+    int32_t base, cnt, rep, pad;
+};
+
 // Piece list of the periodic window [c - R, c + R] of 0..n-1 in ascending order.
 __device__ __forceinline__ int win_pieces(int c, int R, int n, int *a, int *b) {
     if (2 * R + 1 >= n) { a[0] = 0; b[0] = n - 1; return 1; }
@@ -207,6 +231,31 @@ constexpr int kSortRowTab = kSortRows + 3;    // staged rows per column (+ end)
 constexpr int kSortCapLog = 10;
 constexpr int kSortCap = 1 << kSortCapLog;    // beads per sort tile
 constexpr int kSortGrid = 256 * 12;           // workgroups walking the sort tiles
+
+// Tile walk of a grid over n tiles that keeps neighbouring tiles on one XCD.
+// Workgroups are dispatched to the eight XCDs round robin (workgroup b runs
+// on XCD b % 8), and each XCD has its own L2: XCD x takes the contiguous band
+// [x per, (x + 1) per) of tiles, its gridDim / 8 workgroups walking the band
+// with that stride, so the columns that share staged keys, state lines and
+// the (tile, rank) entries of lipid partners are read through one L2.  A grid
+// that is not a multiple of 8 walks the tiles with stride gridDim.
+struct TileWalk {
+    int first, end, stride;
+};
+__device__ __forceinline__ TileWalk tile_walk(int n) {
+    TileWalk w;
+    if ((gridDim.x & 7) == 0) {
+        const int per = (n + 7) >> 3, x = (int)(blockIdx.x & 7);
+        w.first = x * per + (int)(blockIdx.x >> 3);
+        w.end = min((x + 1) * per, n);
+        w.stride = (int)(gridDim.x >> 3);
+    } else {
+        w.first = (int)blockIdx.x;
+        w.end = n;
+        w.stride = (int)gridDim.x;
+    }
+    return w;
+}
 
 __device__ __forceinline__ int block_excl_scan128(int v, int *tmp, int &total) {
     // exclusive scan over a 128-thread block; tmp: 2 ints of LDS
@@ -476,7 +525,8 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
     };
     // a grid of at most a few blocks per CU walks the tiles (fewer, longer-lived
     // workgroups)
-    for (int tix = blockIdx.x; tix < nt; tix += gridDim.x) {
+    const TileWalk W = tile_walk(nt);
+    for (int tix = W.first; tix < W.end; tix += W.stride) {
         do_tile(tix);
         __syncthreads();
     }
@@ -571,13 +621,18 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                                                      const int32_t *__restrict__ tslot, int32_t *__restrict__ cs_new,
                                                      int32_t *__restrict__ src, StateP in, StateP out,
                                                      const double2 *__restrict__ f, uint32_t *__restrict__ err,
-                                                     int nt) {
+                                                     int nt, const RepStep *__restrict__ rs,
+                                                     int32_t *__restrict__ clist, ChemSeg *__restrict__ cseg) {
     // Each workgroup walks tiles blockIdx.x, + gridDim.x, ...; the next tile's
     // header and bead list are loaded while the current one is placed.
     constexpr int kPl = 2;  // beads per thread per round
     const int t = threadIdx.x;
-    int tix = blockIdx.x;
-    if (tix >= nt) return;
+    const TileWalk W = tile_walk(nt);
+    int tix = W.first;
+    if (tix >= W.end) return;
+    __shared__ int s_cn;  // the tile's precursor-chemistry list entries
+    if (t == 0) s_cn = 0;
+    __syncthreads();
     Tile T;
     int base, n, sv[kPl];
     auto load_head = [&](int x, Tile &T_, int &b_, int &n_, int *sv_) {
@@ -590,10 +645,10 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
     };
     load_head(tix, T, base, n, sv);
     for (;;) {
-        const int nx = tix + gridDim.x;
+        const int nx = tix + W.stride;
         Tile Tn;
         int basen = 0, nn = 0, svn[kPl];
-        if (nx < nt) load_head(nx, Tn, basen, nn, svn);
+        if (nx < W.end) load_head(nx, Tn, basen, nn, svn);
         const RepInfo &R = ri[T.rep];
         for (int row = T.k0 + t; row < T.k1; row += kSortThr) {
             const int c = R.coff + T.cx * R.ncy + row;
@@ -630,12 +685,16 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                     lq[k] = in.lnk[s];
                     pq[k] = in.prv[s];
                 }
-            // (tile, rank) of the partners' old slots
+            // (tile, rank) of the partners' old slots; the chemistry partner's
+            // gid word at a precursor-chemistry check
+            const bool chk = clist && rs[T.rep].chem_now;
             int32_t ts[kPl], tp[kPl];
+            uint32_t wp[kPl];
 #pragma unroll
             for (int k = 0; k < kPl; k++) {
                 ts[k] = ok[k] && lq[k] >= 0 ? tslot[lq[k]] : -1;
                 tp[k] = ok[k] && mq[k] > 0 && pq[k] >= 0 ? tslot[pq[k]] : -1;
+                wp[k] = chk && ok[k] && mq[k] < 0 && lq[k] >= 0 ? in.gt[lq[k]] : 0u;
             }
 #pragma unroll
             for (int k = 0; k < kPl; k++) {
@@ -657,11 +716,24 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                 out.mp[d] = m;
                 // the partners' new slots: next / chemistry partner, and the
                 // chain predecessor of a lipid bead past the head
-                out.lnk[d] = ts[k] >= 0 ? toff[ts[k] >> kSortCapLog] + (ts[k] & (kSortCap - 1)) : -1;
+                const int nl = ts[k] >= 0 ? toff[ts[k] >> kSortCapLog] + (ts[k] & (kSortCap - 1)) : -1;
+                out.lnk[d] = nl;
                 out.prv[d] = tp[k] >= 0 ? toff[tp[k] >> kSortCapLog] + (tp[k] & (kSortCap - 1)) : -1;
+                if (chk) {
+                    const int role = chem_role(R, rs[T.rep], gq[k], m, nl, wp[k]);
+                    if (role) clist[base + atomicAdd(&s_cn, 1)] = d | (role << kRoleShift);
+                }
             }
         }
-        if (nx >= nt) break;
+        if (clist) {
+            __syncthreads();
+            if (t == 0) {
+                cseg[tix] = ChemSeg{base, s_cn, T.rep, 0};
+                s_cn = 0;
+            }
+            __syncthreads();
+        }
+        if (nx >= W.end) break;
         tix = nx;
         T = Tn;
         base = basen;
@@ -1542,120 +1614,99 @@ __device__ __forceinline__ Win window_of(const RepInfo &R, const int32_t *cs, do
 }
 
 enum : uint8_t { CF_BREAK = 1, CF_CONV = 2 };
-constexpr int kChemThreads = 512;
-constexpr int kChemGroup = 4;  // lanes per search
+constexpr int kChemGroup = 8;  // lanes per search
 constexpr int kChemK = 2;      // candidates per lane per chunk
+constexpr int kChemGrid = 1 << 16;  // wavefronts walking the segments of the check's list
 
-// Decisions from the state at the start of the check (own definition): break
-// of chemistry bonds, P -> T conversion next to a lipid tail, and the
-// bond-formation proposal of every free T.  One block per reduction chunk
-// (kRedChunk beads of one replica): every bead is classified (break draw of a
-// chemistry bond, conversion draw of a P, free T), the beads that need a
-// neighbour search are compacted in LDS and searched by the block's threads
-// densely, and the chunk's beads with a decision are listed (act, actn) for
-// k_chem_apply.  flag/best are written for the searched and flagged beads
-// only; k_chem_apply reads nothing else.
-__global__ __launch_bounds__(kChemThreads) void k_chem_decide(const RepInfo *__restrict__ ri,
-                                                              const RepStep *__restrict__ rs,
-                                                              const int32_t *__restrict__ rblk,
-                                                              const int32_t *__restrict__ cs,
-                                                              const double2 *__restrict__ xy,
-                                                              const uint32_t *__restrict__ gt,
-                                                              const int8_t *__restrict__ mp,
-                                                              const int32_t *__restrict__ lnk,
-                                                              uint8_t *__restrict__ flag, int32_t *__restrict__ best,
-                                                              int32_t *__restrict__ act, int32_t *__restrict__ actn) {
-    __shared__ int32_t s_srch[kRedChunk];
-    __shared__ int32_t s_act[kRedChunk];
-    __shared__ int s_n, s_na;
-    const int r = rblk[blockIdx.x];
-    const RepStep &S = rs[r];
-    if (!S.chem_now) return;
+// The check's list for a state that was not just placed (a chemistry check
+// on a loaded state): one segment per bead block.
+__global__ __launch_bounds__(kBeadBlock) void k_chem_list(const RepInfo *__restrict__ ri,
+                                                          const RepStep *__restrict__ rs,
+                                                          const int32_t *__restrict__ bblk,
+                                                          const uint32_t *__restrict__ gt,
+                                                          const int8_t *__restrict__ mp,
+                                                          const int32_t *__restrict__ lnk, int32_t *__restrict__ list,
+                                                          ChemSeg *__restrict__ seg) {
+    __shared__ int s_n;
+    const int r = bblk[blockIdx.x];
     const RepInfo &R = ri[r];
-    const int b0 = (blockIdx.x - R.rblk0) * kRedChunk;
-    if (threadIdx.x == 0) {
-        s_n = 0;
-        s_na = 0;
-    }
+    const RepStep &S = rs[r];
+    const int base = R.off + (blockIdx.x - R.bblk0) * kBeadBlock;
+    if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    // classification: every load of the thread's beads first (own words, then
-    // the bond partners' gid words), so the chunk costs two load latencies
-    constexpr int kPer = kRedChunk / kChemThreads;
-    uint32_t wq[kPer], wp[kPer];
-    int8_t mq[kPer];
-    int32_t lq[kPer];
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        const int loc = b0 + k * kChemThreads + (int)threadIdx.x;
-        const int s = R.off + (loc < R.N ? loc : 0);
-        wq[k] = gt[s];
-        mq[k] = mp[s];
-        lq[k] = lnk[s];
-    }
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        const int loc = b0 + k * kChemThreads + (int)threadIdx.x;
-        wp[k] = loc < R.N && mq[k] < 0 && lq[k] >= 0 ? gt[lq[k]] : 0u;
-    }
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        const int loc = b0 + k * kChemThreads + (int)threadIdx.x;
-        if (loc >= R.N) break;
+    const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
+    if (S.chem_now && loc < R.N) {
         const int s = R.off + loc;
-        const uint32_t w = wq[k], g = w & kGidMask, ty = w >> kGidBits;
-        const int m = mq[k], l = lq[k];
-        const bool brk = m < 0 && l >= 0 && g < (wp[k] & kGidMask) && hash_u01(S.kbreak, g) < R.p_break;
-        const bool conv = ty == OC_N && hash_u01(S.kconv, g) < R.p_conv;
-        const bool form = ty == OC_T && m < 0 && l < 0;
-        if (conv || form) {
-            s_srch[atomicAdd(&s_n, 1)] = s | (brk ? (int32_t)0x80000000 : 0);
-        } else if (brk) {
-            flag[s] = CF_BREAK;
-            best[s] = -1;
-            s_act[atomicAdd(&s_na, 1)] = s;
-        }
+        const uint32_t w = gt[s];
+        const int m = mp[s], l = lnk[s];
+        const int role = chem_role(R, S, w, m, l, m < 0 && l >= 0 ? gt[l] : 0u);
+        if (role) list[base + atomicAdd(&s_n, 1)] = s | (role << kRoleShift);
     }
     __syncthreads();
-    const int ns = s_n;
-    // kChemGroup lanes per searcher: the window's candidates are dealt round
-    // robin to the lanes, then reduced over the group (any lipid tail in
-    // range; the smallest accepted draw, ties to the smaller gid)
+    if (threadIdx.x == 0) seg[blockIdx.x] = ChemSeg{base, s_n, r, 0};
+}
+
+// Decisions from the state at the start of the check for the listed beads:
+// a searching P: any lipid tail within r_conv; a searching free T: the free T
+// within r_form with the smallest symmetric draw pair_u01(kform) below p_form
+// (ties: the smaller gid).  kChemGroup lanes per entry (the window's
+// candidates dealt round robin, then reduced over the group).  flag / best
+// are written for the listed beads only; k_chem_apply reads nothing else.
+__global__ __launch_bounds__(64) void k_chem_decide(const RepInfo *__restrict__ ri, const RepStep *__restrict__ rs,
+                                                    const int32_t *__restrict__ cs, const double2 *__restrict__ xy,
+                                                    const uint32_t *__restrict__ gt, const int8_t *__restrict__ mp,
+                                                    const int32_t *__restrict__ lnk, const int32_t *__restrict__ list,
+                                                    const ChemSeg *__restrict__ seg, int nseg,
+                                                    uint8_t *__restrict__ flag, int32_t *__restrict__ best) {
     const int gl = threadIdx.x & (kChemGroup - 1);
-    for (int e = threadIdx.x / kChemGroup; e < ns; e += kChemThreads / kChemGroup) {
-        const int s = s_srch[e] & 0x7fffffff;
-        const uint32_t g = gt[s] & kGidMask, ty = gt[s] >> kGidBits;
+    constexpr int per = 64 / kChemGroup;
+    // one wavefront per segment, kChemGroup lanes per entry; segments walked
+    // in the placement's XCD bands (their beads are in this XCD's L2)
+    const TileWalk Wk = tile_walk(nseg);
+    for (int k = Wk.first; k < Wk.end; k += Wk.stride) {
+        const ChemSeg G = seg[k];
+        const RepStep &S = rs[G.rep];
+        if (!S.chem_now) continue;
+        const RepInfo &R = ri[G.rep];
+        for (int e = (int)threadIdx.x / kChemGroup; e < G.cnt; e += per) {
+        const int32_t ent = list[G.base + e];
+        const int s = ent & ((1 << kRoleShift) - 1), role = (int)((uint32_t)ent >> kRoleShift);
+        if (!(role & 2)) {  // a break only
+            if (gl == 0) {
+                flag[s] = CF_BREAK;
+                best[s] = -1;
+            }
+            continue;
+        }
+        const uint32_t w = gt[s], g = w & kGidMask, ty = w >> kGidBits;
         const double2 p = xy[s];
         const Win W = window_of(R, cs, p, ty == OC_N ? R.rwc : R.rwf);
         bool found = false;
         uint64_t bkey = ~0ull;  // (bits of u) of the best accepted draw
         uint32_t bg = 0xffffffffu;
         int32_t bsel = -1;
-        // kChemK candidates per lane per chunk, loaded in stages (slots and
-        // types, then the T candidates' links and positions) so that the loads
-        // of a stage overlap
+        // kChemK candidates per lane per chunk (one chunk covers most
+        // windows), every field of a candidate loaded in one round
         for (int k0 = gl; k0 < W.total; k0 += kChemGroup * kChemK) {
             int jc[kChemK];
             uint32_t gc[kChemK];
-#pragma unroll
-            for (int i = 0; i < kChemK; i++) {
-                const int k = k0 + i * kChemGroup;
-                jc[i] = k < W.total ? W.slot(k) : -1;
-                if (jc[i] == s) jc[i] = -1;
-                gc[i] = jc[i] >= 0 ? gt[jc[i]] : 0u;
-            }
             int8_t mc[kChemK];
             int32_t lc[kChemK];
             double2 pc[kChemK];
 #pragma unroll
             for (int i = 0; i < kChemK; i++) {
-                const bool isT = jc[i] >= 0 && (gc[i] >> kGidBits) == OC_T;
-                if (!isT) jc[i] = -1;
-                if (isT) {
-                    mc[i] = mp[jc[i]];
-                    lc[i] = lnk[jc[i]];
-                    pc[i] = xy[jc[i]];
-                }
+                const int k = k0 + i * kChemGroup;
+                jc[i] = k < W.total ? W.slot(k) : -1;
+                if (jc[i] == s) jc[i] = -1;
+                const int jj = jc[i] >= 0 ? jc[i] : s;  // a valid slot for the loads
+                gc[i] = gt[jj];
+                mc[i] = mp[jj];
+                lc[i] = lnk[jj];
+                pc[i] = xy[jj];
             }
+#pragma unroll
+            for (int i = 0; i < kChemK; i++)
+                if (jc[i] >= 0 && (gc[i] >> kGidBits) != OC_T) jc[i] = -1;
 #pragma unroll
             for (int i = 0; i < kChemK; i++) {
                 if (jc[i] < 0) continue;
@@ -1687,16 +1738,11 @@ __global__ __launch_bounds__(kChemThreads) void k_chem_decide(const RepInfo *__r
             }
         }
         if (gl == 0) {
-            const uint8_t fl = (s_srch[e] < 0 ? CF_BREAK : 0) | (found ? CF_CONV : 0);
-            flag[s] = fl;
+            flag[s] = (role & 1 ? CF_BREAK : 0) | (found ? CF_CONV : 0);
             best[s] = bsel;
-            if (fl || bsel >= 0) s_act[atomicAdd(&s_na, 1)] = s;
+        }
         }
     }
-    __syncthreads();
-    const int na = s_na;
-    for (int e = threadIdx.x; e < na; e += kChemThreads) act[(size_t)blockIdx.x * kRedChunk + e] = s_act[e];
-    if (threadIdx.x == 0) actn[blockIdx.x] = na;
 }
 
 __device__ __forceinline__ void push_event(Event *ev, unsigned long long *evn, int64_t cap, uint32_t *err, int64_t step,
@@ -1815,19 +1861,24 @@ __global__ __launch_bounds__(kBeadBlock) void k_rfeed(const RepInfo *__restrict_
 // Applies the decisions of the listed beads of one chunk.  A proposal forms a
 // bond when it is mutual (best[best[s]] == s; best of a free T is always
 // written by k_chem_decide of the same check).
-__global__ __launch_bounds__(64) void k_chem_apply(const RepInfo *__restrict__ ri, const RepStep *__restrict__ rs,
-                                                   const int32_t *__restrict__ rblk, uint32_t *__restrict__ gt,
+// Apply the check's decisions to the listed beads: breaks (the bond's smaller
+// gid), conversions, and bonds of mutual best proposals (written by the
+// smaller gid).  Every write has one writer: the order is irrelevant, and
+// the events are sorted when read.
+__global__ __launch_bounds__(64) void k_chem_apply(const RepStep *__restrict__ rs, uint32_t *__restrict__ gt,
                                                    int32_t *__restrict__ lnk, const uint8_t *__restrict__ flag,
-                                                   const int32_t *__restrict__ best, const int32_t *__restrict__ act,
-                                                   const int32_t *__restrict__ actn, Event *__restrict__ ev,
+                                                   const int32_t *__restrict__ best, const int32_t *__restrict__ list,
+                                                   const ChemSeg *__restrict__ seg, int nseg, Event *__restrict__ ev,
                                                    unsigned long long *__restrict__ evn, int64_t evcap, int record,
                                                    uint32_t *__restrict__ err) {
-    const int r = rblk[blockIdx.x];
-    const RepStep &S = rs[r];
-    if (!S.chem_now) return;
-    const int na = actn[blockIdx.x];
-    for (int e = threadIdx.x; e < na; e += 64) {
-        const int s = act[(size_t)blockIdx.x * kRedChunk + e];
+    const TileWalk Wk = tile_walk(nseg);
+    for (int k = Wk.first; k < Wk.end; k += Wk.stride) {
+        const ChemSeg G = seg[k];
+        const RepStep &S = rs[G.rep];
+        if (!S.chem_now) continue;
+        const int r = G.rep;
+        for (int e = threadIdx.x; e < G.cnt; e += 64) {
+        const int s = list[G.base + e] & ((1 << kRoleShift) - 1);
         const uint32_t g = gt[s] & kGidMask;
         const uint8_t fl = flag[s];
         if (fl & CF_BREAK) {
@@ -1849,6 +1900,7 @@ __global__ __launch_bounds__(64) void k_chem_apply(const RepInfo *__restrict__ r
                 lnk[b] = s;
                 if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_FORM, g, gb, r);
             }
+        }
         }
     }
 }
