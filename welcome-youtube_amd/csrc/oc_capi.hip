@@ -1127,14 +1127,22 @@ OC_API int oc_step(oc_engine *h, int64_t n, int32_t use_graph) {
         for (; k + nc <= n; k += nc) {
             const int p = h->cur;
             if (!h->pexec[p])
-                if (int rc = capture_period(h, p, nc)) return rc;
+                for (int q = 0; q < 2; q++)  // both buffer parities, uploaded, before the first replay
+                    if (!h->pexec[q]) {
+                        if (int rc = capture_period(h, q, nc)) return rc;
+                        HIPCHK(hipGraphUpload(h->pexec[q], h->stream));
+                    }
             HIPCHK(hipGraphLaunch(h->pexec[p], h->stream));
             h->cur = p ^ (nc & 1);
         }
     } else if (use_graph && n >= 2) {
         const int p = h->cur;
         if (!h->gexec[p])
-            if (int rc = capture_graph(h, p)) return rc;
+            for (int q = 0; q < 2; q++)  // both buffer parities, uploaded, before the first replay
+                if (!h->gexec[q]) {
+                    if (int rc = capture_graph(h, q)) return rc;
+                    HIPCHK(hipGraphUpload(h->gexec[q], h->stream));
+                }
         if (!h->keys_fresh && !h->nokeys) launch_keys(h, true);
         for (; k + 2 <= n; k += 2) HIPCHK(hipGraphLaunch(h->gexec[p], h->stream));
         h->keys_fresh = !h->nokeys;
