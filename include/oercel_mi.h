@@ -308,6 +308,18 @@ int oc_abi_sizes(int32_t *out, int32_t n);
  * their own work or time kernels with events on it. */
 void *oc_stream(oc_engine *h);
 
+/* Kinetic temperature and total momentum of n >= 2 beads whose half-step
+ * velocities and forces the caller holds in device memory, in any order:
+ * kinetic_T (cel/engine/oercel.c:2058-2071) with v = vh + dt/2 f, written to
+ * the device array out[3] = (sum |v|^2 / (2n - 2), sum v_x, sum v_y).  FP64
+ * over a fixed partition: the same inputs give the same bits.  work: device
+ * memory of at least oc_kinetic_dev_work(n) bytes for the block partials.
+ * Asynchronous on stream (a hipStream_t; NULL = the null stream); needs no
+ * engine handle.  Errors: OC_EINVAL (null array, n < 2, small workspace). */
+size_t oc_kinetic_dev_work(int64_t n);
+int oc_kinetic_dev(const double *vx, const double *vy, const double *fx, const double *fy, int64_t n, double dt,
+                   double *out, void *work, size_t work_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

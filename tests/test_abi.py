@@ -93,3 +93,19 @@ def test_create_without_device_or_bad_params_fails_cleanly():
     assert rc in (abi.OC_EHIP, abi.OC_EINVAL) and not h.value
     assert lib.oc_last_error()
     assert lib.oc_step(None, 1, 0) == abi.OC_EINVAL
+
+
+def test_kinetic_dev_host_checks():
+    """The device-array kinetic_T entry validates its arguments on the host:
+    workspace size from n, null arrays, n < 2, a short workspace (no GPU call
+    is reached)."""
+    abi = pkg().abi
+    lib = abi.load_library()
+    assert lib.oc_kinetic_dev_work(1) == 0
+    assert lib.oc_kinetic_dev_work(4096) == 3 * 8
+    assert lib.oc_kinetic_dev_work(4097) == 2 * 3 * 8
+    assert lib.oc_kinetic_dev(None, None, None, None, 10, 0.01, None, None, 0, None) == abi.OC_EINVAL
+    b = C.c_void_p(8)  # never dereferenced: the checks fail first
+    assert lib.oc_kinetic_dev(b, b, b, b, 1, 0.01, b, b, 1 << 20, None) == abi.OC_EINVAL
+    assert lib.oc_kinetic_dev(b, b, b, b, 5000, 0.01, b, b, 24, None) == abi.OC_EINVAL
+    assert b"workspace" in lib.oc_last_error()

@@ -142,6 +142,8 @@ def _declare(lib):
         "oc_window_rewind": (C.c_int, [vp]),
         "oc_window_grad": (C.c_int, [vp, C.c_int32, C.c_int32, P(oc_rdf_spec), P(C.c_double), C.c_int32,
                                      P(C.c_double), P(C.c_double), P(C.c_double)]),
+        "oc_kinetic_dev_work": (C.c_size_t, [C.c_int64]),
+        "oc_kinetic_dev": (C.c_int, [vp, vp, vp, vp, C.c_int64, C.c_double, vp, vp, C.c_size_t, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

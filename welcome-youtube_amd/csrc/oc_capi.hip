@@ -1786,3 +1786,84 @@ OC_API int oc_window_grad(oc_engine *h, int32_t nsteps, int32_t ckpt_every, cons
     if (gbar) std::copy(g.begin(), g.end(), gbar);
     return oc_window_backward(h, dl.data(), grad);
 }
+
+// --------------------------------------------------------------------------- kinetic_T on device arrays
+
+// Kinetic temperature and momentum of n beads held by the caller in device
+// memory (kinetic_T, cel/engine/oercel.c:2058-2071): v = vh + dt/2 f,
+// out = (sum |v|^2 / (2n - 2), sum v_x, sum v_y).  FP64 over a fixed
+// partition (kKinChunk beads per block, a fixed tree in the block and over the
+// blocks), so the same inputs give the same bits.
+constexpr int kKinChunk = 4096;
+
+__global__ __launch_bounds__(256) void k_kin_partial(const double *__restrict__ vx, const double *__restrict__ vy,
+                                                     const double *__restrict__ fx, const double *__restrict__ fy,
+                                                     int64_t n, double hdt, double *__restrict__ part) {
+    __shared__ double red[3][256];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const int64_t b0 = (int64_t)blockIdx.x * kKinChunk;
+    for (int q = 0; q < kKinChunk / 256; q++) {
+        const int64_t i = b0 + q * 256 + threadIdx.x;
+        if (i >= n) break;
+        const double ux = vx[i] + hdt * fx[i], uy = vy[i] + hdt * fy[i];
+        a0 += ux * ux + uy * uy;
+        a1 += ux;
+        a2 += uy;
+    }
+    red[0][threadIdx.x] = a0;
+    red[1][threadIdx.x] = a1;
+    red[2][threadIdx.x] = a2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[(size_t)blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_kin_final(const double *__restrict__ part, int nb, int64_t n,
+                                                   double *__restrict__ out) {
+    __shared__ double red[3][256];
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += 256)
+        for (int k = 0; k < 3; k++) a[k] += part[(size_t)b * 3 + k];
+    for (int k = 0; k < 3; k++) red[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0][0] / (2.0 * (double)n - 2.0);
+        out[1] = red[1][0];
+        out[2] = red[2][0];
+    }
+}
+
+static int64_t kin_blocks(int64_t n) { return (n + kKinChunk - 1) / kKinChunk; }
+
+static void launch_kin(const double *vx, const double *vy, const double *fx, const double *fy, int64_t n, double dt,
+                       double *out, double *part, hipStream_t st) {
+    const int nb = (int)kin_blocks(n);
+    k_kin_partial<<<nb, 256, 0, st>>>(vx, vy, fx, fy, n, 0.5 * dt, part);
+    k_kin_final<<<1, 256, 0, st>>>(part, nb, n, out);
+}
+
+OC_API size_t oc_kinetic_dev_work(int64_t n) {
+    return n < 2 ? 0 : (size_t)kin_blocks(n) * 3 * sizeof(double);
+}
+
+OC_API int oc_kinetic_dev(const double *vx, const double *vy, const double *fx, const double *fy, int64_t n,
+                          double dt, double *out, void *work, size_t work_bytes, void *stream) {
+    if (!vx || !vy || !fx || !fy || !out) return fail(OC_EINVAL, "null array");
+    if (n < 2) return fail(OC_EINVAL, "kinetic temperature needs n >= 2 beads");
+    if (kin_blocks(n) > INT32_MAX / 3) return fail(OC_EINVAL, "too many beads");
+    if (!work || work_bytes < oc_kinetic_dev_work(n))
+        return fail(OC_EINVAL, "workspace smaller than oc_kinetic_dev_work(n) bytes");
+    launch_kin(vx, vy, fx, fy, n, dt, out, (double *)work, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
