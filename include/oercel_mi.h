@@ -320,6 +320,16 @@ size_t oc_kinetic_dev_work(int64_t n);
 int oc_kinetic_dev(const double *vx, const double *vy, const double *fx, const double *fy, int64_t n, double dt,
                    double *out, void *work, size_t work_bytes, void *stream);
 
+/* oc_kinetic_dev without a workspace and without any allocation (no
+ * hipMalloc, hipFree or allocating library call), for callers that cannot
+ * allocate: code under HIP graph capture, tight serving loops.  No
+ * counterpart in the reference.  The block partials live in the library's
+ * fixed device scratch of 384 KiB, one copy per GPU held until the process
+ * ends, so n <= 67,108,864 (OC_EINVAL beyond); calls that overlap on one GPU
+ * race on that scratch.  Same kernels and bits as oc_kinetic_dev. */
+int oc_kinetic_dev_noalloc(const double *vx, const double *vy, const double *fx, const double *fy, int64_t n,
+                           double dt, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

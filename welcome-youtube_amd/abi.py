@@ -144,6 +144,7 @@ def _declare(lib):
                                      P(C.c_double), P(C.c_double), P(C.c_double)]),
         "oc_kinetic_dev_work": (C.c_size_t, [C.c_int64]),
         "oc_kinetic_dev": (C.c_int, [vp, vp, vp, vp, C.c_int64, C.c_double, vp, vp, C.c_size_t, vp]),
+        "oc_kinetic_dev_noalloc": (C.c_int, [vp, vp, vp, vp, C.c_int64, C.c_double, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

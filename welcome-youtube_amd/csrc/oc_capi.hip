@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1867,3 +1868,49 @@ OC_API int oc_kinetic_dev(const double *vx, const double *vy, const double *fx, 
     return 0;
 }
 
+// --------------------------------------------------------------------------- allocation-free kinetic_T
+
+// Block partials of oc_kinetic_dev_noalloc: up to kKinScratchBlocks blocks,
+// i.e. n <= kKinScratchBlocks * kKinChunk = 67,108,864 beads, 384 KiB.  Each
+// GPU the library runs on holds its own copy until the process ends; every
+// call on a GPU uses that GPU's copy (calls that overlap on one GPU race).
+// Every partial a call reads is written by the same call first: nothing
+// needs clearing between calls.
+constexpr int kKinScratchBlocks = 16384;
+static __device__ __align__(256) double scratch[3 * kKinScratchBlocks];
+
+// Device address of scratch on the current device, looked up once per device
+// (host code uses this address, never the array's name).
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices];
+
+static int scratch_ptr(double **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(OC_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        double *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(OC_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+OC_API int oc_kinetic_dev_noalloc(const double *vx, const double *vy, const double *fx, const double *fy, int64_t n,
+                                  double dt, double *out, void *stream) {
+    if (!vx || !vy || !fx || !fy || !out) return fail(OC_EINVAL, "null array");
+    if (n < 2) return fail(OC_EINVAL, "kinetic temperature needs n >= 2 beads");
+    if (kin_blocks(n) > kKinScratchBlocks)
+        return fail(OC_EINVAL, "oc_kinetic_dev_noalloc: n exceeds 67,108,864 beads (its fixed scratch); use "
+                               "oc_kinetic_dev with a workspace");
+    double *part = nullptr;
+    if (int rc = scratch_ptr(&part)) return rc;
+    launch_kin(vx, vy, fx, fy, n, dt, out, part, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
