@@ -177,7 +177,10 @@ const char *oc_last_error(void);
 int oc_abi_version(void);
 
 /* Engine over nrep replicas; p[r] and N[r] describe replica r.  Allocates all
- * device memory up front (nothing is allocated by oc_step).
+ * device memory up front (nothing is allocated by oc_step).  The chemistry
+ * event buffer holds max(65536, 4 N) events (at most 8M; the environment
+ * variable OC_EVENT_CAP lowers it); oc_get_events drains it, and events past
+ * a full buffer are dropped and reported once by the next oc_get_events.
  * Replaces sys_alloc (cel/engine/oercel.c:714-759) for one or many systems. */
 int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engine **out);
 void oc_destroy(oc_engine *h);

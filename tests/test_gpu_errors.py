@@ -46,7 +46,8 @@ def test_reference_chemistry_limits(eng):
     sort's cell index), and the engine keeps working after a state reload."""
     abi = importlib.import_module("welcome-youtube_amd.abi")
     oc = importlib.import_module("welcome-youtube_amd")
-    p = mkparams(16.0, f_lip=0.1, c_N=0.3, f_Ka=0.03, chem=1, n_chem=1, p_syn_a=0.5, seed=4)
+    p = mkparams(16.0, f_lip=0.1, c_N=0.3, f_Ka=0.03, chem=1, n_chem=1, p_syn_a=0.5, p_feed=0.5, p_nd=0.5, p_ld=0.5,
+                 seed=4)
     e = eng.Engine(p)
     e.start()
     with pytest.raises(abi.OcError, match="reference chemistry"):
@@ -57,6 +58,10 @@ def test_reference_chemistry_limits(eng):
     e.set_state(fast, step=0)
     with pytest.raises(abi.OcError, match="moved 0.5 or more"):
         e.step(1, graph=False)
+    # the refused check changed no type, chain or bond (no reaction, decay or feed)
+    after = e.get_state()
+    for k in ("type", "mpos", "nxt", "bond"):
+        assert np.array_equal(after[k], fast[k]), k
     e.set_state(s, step=0)
     e.step(3)
     assert np.isfinite(e.stats()["T_kin"])

@@ -137,3 +137,24 @@ def test_cli_with_catalysts_vs_the_gpu_port(tmp_path):
         dd = np.abs((g[k] - G[f"cli_{k}"] + L / 2) % L - L / 2)
         say(f"{k}: median {np.median(dd):.2e}, max {dd.max():.2e}")
         assert np.median(dd) < 1e-6 and dd.max() < 1e-4, (k, np.median(dd), dd.max())
+
+
+def test_event_overflow_does_not_stop_the_run(tmp_path, monkeypatch):
+    """With an event buffer far smaller than a chunk's events (OC_EVENT_CAP),
+    events are drained at every output point, an overflow is logged as a
+    warning, and the run still ends with its checkpoint (ckpt_every = 0)."""
+    monkeypatch.setenv("OC_EVENT_CAP", "8")
+    run = importlib.import_module("welcome-youtube_amd.run")
+    io = importlib.import_module("welcome-youtube_amd.io")
+    ini = tmp_path / "empty.ini"
+    ini.write_text("# all parameters on the command line\n")
+    d = str(tmp_path / "ovf")
+    say("run: 300 steps, 8-event buffer, no periodic checkpoint")
+    assert run.run(str(ini), d, args(steps=300, stats_every=50, ckpt_every=0, p_conv="0.3", p_form="0.3")) == 300
+    log = open(os.path.join(d, "log.txt")).read()
+    assert "event buffer overflowed" in log and "done at step 300" in log
+    s, hdr, _ = io.read_checkpoint(os.path.join(d, "ckpt.bin"))
+    assert hdr["step"] == 300
+    assert not os.path.exists(os.path.join(d, "ckpt.bin.tmp"))
+    rows = open(os.path.join(d, "events.csv")).read().splitlines()[1:]
+    assert len(rows) > 0 and all(0 < int(r.split(",", 1)[0]) <= 300 for r in rows)

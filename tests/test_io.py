@@ -97,3 +97,72 @@ def test_stats_row_format():
     assert io.stats_row(st) == "100,1.000000,0.998765,10,2,4,0,0,0,0,2,0.812346"
     assert io.STATS_HEADER.count(",") == io.stats_row(st).count(",")
     json.dumps(st)
+
+
+PI_OV = dict(OV, lip_arch="pi", lip_nT="2", lip_tie="1", k_angle="0", f_lip="0.15")
+
+
+def test_branched_checkpoint_v3_bytes_and_reference_restart(oracle, tmp_path):
+    """A pi-lipid checkpoint is version 3 with the architecture code: the
+    reference's own v3 file is re-written byte for byte, and the reference CLI
+    restarts from one written here (its continuation matches the oracle's)."""
+    ref_cli(oracle)
+    d = str(tmp_path / "run")
+    oracle.run_ref_cli(d, dict(PI_OV, steps="10"))
+    s, hdr, slot_gid = io.read_checkpoint(os.path.join(d, "ckpt.bin"))
+    assert hdr["version"] == 3 and hdr["arch"] == 1
+    out = str(tmp_path / "ck.bin")
+    io.write_checkpoint(out, s, slot_gid, hdr)
+    assert open(out, "rb").read() == open(os.path.join(d, "ckpt.bin"), "rb").read()
+    p = pm.load(overrides={k: v for k, v in PI_OV.items()})
+    io.check_architecture(p, hdr)
+    cp = p.to_c()
+    sysm = oracle.System(cp, p.N)
+    sysm.start(oracle.soup(cp, p.N), theta32=0)
+    sysm.step(15, theta32=0)
+    st = sysm.get_state()
+    _, order = sysm.cells()
+    ck = str(tmp_path / "ours.bin")
+    io.write_checkpoint(ck, st, order, {"step": 15, "time": 15 * p.dt, "L": p.L, "seed": p.seed, "mlen": p.mlen,
+                                        "nH": p.nH, "arch": 1, "warm_end": p.warmup, "dt_anchor": p.dt,
+                                        "maxbond": 0.0})
+    assert io.read_checkpoint(ck)[1]["version"] == 3
+    d2 = str(tmp_path / "cont")
+    oracle.run_ref_cli(d2, dict(PI_OV, steps="25", restart=ck, init="restart"))
+    ref, rh, _ = io.read_checkpoint(os.path.join(d2, "ckpt.bin"))
+    assert rh["step"] == 25 and rh["arch"] == 1
+    sysm.step(10, theta32=0)
+    o = sysm.get_state()
+    L = p.L
+    assert np.abs((o["x"] - ref["x"] + L / 2) % L - L / 2).max() < 1e-10
+    assert np.array_equal(o["type"], ref["type"]) and np.array_equal(o["nxt"], ref["nxt"])
+
+
+def test_restart_refuses_another_architecture(tmp_path):
+    """The reference's restart check (cel/engine/oercel.c:1976): mlen, nH and
+    the architecture code of the checkpoint must be the parameters'."""
+    lin = pm.load(overrides={"L": "10", "lip_nT": "2"})
+    pi = pm.load(overrides={"L": "10", "lip_arch": "pi", "lip_nT": "2"})
+    io.check_architecture(lin, {"mlen": lin.mlen, "nH": lin.nH})
+    io.check_architecture(pi, {"mlen": pi.mlen, "nH": pi.nH, "arch": 1})
+    with pytest.raises(io.FormatError, match="architecture"):
+        io.check_architecture(lin, {"mlen": pi.mlen, "nH": pi.nH, "arch": 1})
+    with pytest.raises(io.FormatError, match="architecture"):
+        io.check_architecture(pi, {"mlen": pi.mlen, "nH": pi.nH, "arch": 0})
+    with pytest.raises(io.FormatError, match="architecture"):
+        io.check_architecture(lin, {"mlen": lin.mlen + 1, "nH": lin.nH})
+
+
+def test_checkpoint_write_is_atomic_and_keeps_a_link(tmp_path):
+    N = 6
+    s = {"x": np.linspace(0, 5, N), "y": np.ones(N), "vx": np.zeros(N), "vy": np.zeros(N), "fx": np.zeros(N),
+         "fy": np.zeros(N), "type": np.zeros(N, np.uint8), "mpos": np.full(N, -1, np.int8),
+         "nxt": np.full(N, -1, np.int32)}
+    path, keep = str(tmp_path / "ckpt.bin"), str(tmp_path / "ckpt_0000000007.bin")
+    open(path, "wb").write(b"previous")
+    io.write_checkpoint(path, s, np.arange(N), {"step": 7, "time": 0.07, "L": 6.0, "seed": 1, "mlen": 3, "nH": 1},
+                        keep_path=keep)
+    assert not os.path.exists(path + ".tmp")
+    assert open(path, "rb").read() == open(keep, "rb").read()
+    assert os.stat(path).st_ino == os.stat(keep).st_ino
+    assert io.read_checkpoint(path)[1]["step"] == 7

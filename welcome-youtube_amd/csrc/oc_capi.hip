@@ -3,6 +3,7 @@
 // gid order, the step launch sequence, hipGraph capture, stats and events.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -550,7 +551,13 @@ static int check_err(oc_engine *h) {
     uint32_t e = 0;
     HIPCHK(hipMemcpyAsync(&e, h->d_err, sizeof e, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    if (e & ERR_EVOVF) h->events_dropped = true;
+    if (e & ERR_EVOVF) {
+        // reported once, by the next oc_get_events; the bit is cleared so that
+        // later drains report only new overflows
+        h->events_dropped = true;
+        const uint32_t cleared = e & ~ERR_EVOVF;
+        HIPCHK(hipMemcpy(h->d_err, &cleared, sizeof cleared, hipMemcpyHostToDevice));
+    }
     if (e & ~ERR_EVOVF) {
         std::string m = "device error:";
         if (e & ERR_SORT) m += " sort tile overflow or bead count mismatch (sort)";
@@ -766,6 +773,10 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     }
     h->evq.resize(nrep);
     h->evcap = std::min<int64_t>(std::max<int64_t>(1 << 16, 4 * h->Ntot), 8 << 20);
+    if (const char *ec = getenv("OC_EVENT_CAP")) {  // a smaller event buffer (tests of the overflow path)
+        const long long v = atoll(ec);
+        if (v > 0) h->evcap = std::min<int64_t>(h->evcap, v);
+    }
     int rc = 0;
     const size_t Nt = (size_t)h->Ntot, Ct = (size_t)h->Ctot;
     rc = rc ? rc : dalloc(h, &h->d_ri, nrep);

@@ -1745,6 +1745,8 @@ __global__ __launch_bounds__(64) void k_chem_decide(const RepInfo *__restrict__ 
     }
 }
 
+__device__ __forceinline__ bool R_react(const RepInfo &R) { return R.react != 0; }
+
 __device__ __forceinline__ void push_event(Event *ev, unsigned long long *evn, int64_t cap, uint32_t *err, int64_t step,
                                            int kind, uint32_t a, uint32_t b, int rep, double x = 0.0,
                                            double y = 0.0) {
@@ -1775,7 +1777,10 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdecay(const RepInfo *__restrict
                                                        uint32_t *__restrict__ err) {
     const int r = bblk[blockIdx.x];
     const RepStep &S = rs[r];
-    if (!S.rchem_now) return;
+    // with the catalyst reactions, a check after a step that moved a bead by
+    // the chemistry margin is reported (ERR_DISP, k_react_prep) and changes
+    // nothing: no reaction, decay or feed
+    if (!S.rchem_now || (S.disp && R_react(ri[r]))) return;
     const RepInfo &R = ri[r];
     const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
     if (loc >= R.N) return;
@@ -1842,7 +1847,7 @@ __global__ __launch_bounds__(kBeadBlock) void k_rfeed(const RepInfo *__restrict_
                                                       uint32_t *__restrict__ err) {
     const int r = bblk[blockIdx.x];
     const RepStep &S = rs[r];
-    if (!S.rchem_now) return;
+    if (!S.rchem_now || (S.disp && R_react(ri[r]))) return;  // see k_rdecay
     const RepInfo &R = ri[r];
     if (R.feed_mode == OC_FEED_NONE || !(R.p_feed > 0)) return;
     if (R.fcap_on && S.feed_add == 0) return;

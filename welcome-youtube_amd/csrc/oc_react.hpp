@@ -664,8 +664,8 @@ __global__ __launch_bounds__(kPrepThr) void k_react_prep(const RepInfo *__restri
     __shared__ uint64_t s_key[kCatCap];
     __shared__ int32_t s_slot[kCatCap];
     const int n = B.catn[r];
-    if (n == 0) return;
     const bool bad_n = n > kCatCap, bad_d = S.disp != 0;
+    if (n == 0 && !bad_d) return;
     __syncthreads();  // every thread has read catn
     if (bad_n || bad_d) {
         if (t == 0) {

@@ -3,10 +3,11 @@ CPU engine:
 
   snapshot    snap_<step>.bin, "OCEL" v1: positions (float32) and types in gid
               order plus the bond list (cel/engine/oercel.c:1830-1884)
-  checkpoint  ckpt.bin, "OCKP" v2: header (step, time, L, seed, rng state,
-              max bond, warm-up end, time anchors, soup seed) and the slot-ordered
-              state gid/type/mpos/nxt/x/y/vh/f (cel/engine/oercel.c:1886-1936),
-              read back with the reference's checks (:1942-2010)
+  checkpoint  ckpt.bin, "OCKP" v2 (v3 with the architecture code for branched
+              lipids): header (step, time, L, seed, rng state, max bond, warm-up
+              end, time anchors, soup seed) and the slot-ordered state
+              gid/type/mpos/nxt/x/y/vh/f (cel/engine/oercel.c:1886-1936), read
+              back with the reference's checks (:1942-2010)
   stats.csv   step,time,T_kin,nW..nZ,n_lipids,max_bond (:2264-2267, :2349)
   events.csv  step,kind,i,x,y,variant (:2273)
 
@@ -18,6 +19,7 @@ bead), which the reference's reader never reaches.  The reference's reader
 accepts T beads only inside lipids, so checkpoints with free T are for this
 engine only.
 """
+import os
 import struct
 
 import numpy as np
@@ -112,12 +114,17 @@ def read_snapshot(path):
     return {"step": step, "time": time, "L": L, "x": x.copy(), "y": y.copy(), "type": t.copy(), "bonds": bonds.copy()}
 
 
-def write_checkpoint(path, s, slot_gid, hdr):
-    """OCKP v2 (cel/engine/oercel.c:1886-1936).  s: gid-order state (x, y,
-    vx, vy = vh, fx, fy, type, mpos, nxt[, bond]); slot_gid: gid of each slot
-    (the cell order, e.g. Engine.cells()[1]); hdr: step, time, L, seed, mlen,
-    nH, and optionally rng (4 ints), maxbond, warm_end, t_anchor, step_anchor,
-    dt_anchor, soup_seed."""
+def write_checkpoint(path, s, slot_gid, hdr, keep_path=None):
+    """OCKP checkpoint (cel/engine/oercel.c:1886-1936): version 2, or version 3
+    (version 2 + the lipid architecture code) when hdr["arch"] is not linear,
+    as the reference writes them.  s: gid-order state (x, y, vx, vy = vh, fx,
+    fy, type, mpos, nxt[, bond]); slot_gid: gid of each slot (the cell order,
+    e.g. Engine.cells()[1]); hdr: step, time, L, seed, mlen, nH, and
+    optionally arch (0 linear, 1 pi, 2 y, 3 bola), rng (4 ints), maxbond,
+    warm_end, t_anchor, step_anchor, dt_anchor, soup_seed.  Written to
+    path + ".tmp", flushed to disk and renamed onto path, so a kill during the
+    write leaves the previous checkpoint; keep_path (the reference's ckpt_keep
+    copy) is a hard link to the same file."""
     slot_gid = np.asarray(slot_gid, dtype=np.int64)
     Nb = len(slot_gid)
     slot = np.empty(Nb, np.int64)
@@ -125,9 +132,13 @@ def write_checkpoint(path, s, slot_gid, hdr):
     nxt_g = np.asarray(s["nxt"])
     nxt_slot = np.where(nxt_g >= 0, slot[np.maximum(nxt_g, 0)], -1).astype(np.int32)[slot_gid]
     rng = hdr.get("rng") or rng_seed_state(int(hdr["seed"]))
-    with open(path, "wb") as f:
+    arch = int(hdr.get("arch", 0))
+    tmp = path + ".tmp"
+    with open(tmp, "wb") as f:
         f.write(b"OCKP")
-        f.write(struct.pack("<iiii", 2, Nb, int(hdr["mlen"]), int(hdr["nH"])))
+        f.write(struct.pack("<iiii", 2 if arch == 0 else 3, Nb, int(hdr["mlen"]), int(hdr["nH"])))
+        if arch != 0:
+            f.write(struct.pack("<i", arch))
         f.write(struct.pack("<qddQ", int(hdr["step"]), float(hdr["time"]), float(hdr["L"]), int(hdr["seed"])))
         f.write(struct.pack("<4Q", *[int(v) & ((1 << 64) - 1) for v in rng]))
         f.write(struct.pack("<dqdqdQ", float(hdr.get("maxbond", 0.0)), int(hdr.get("warm_end", 0)),
@@ -144,6 +155,14 @@ def write_checkpoint(path, s, slot_gid, hdr):
         if bd is not None and np.any(bd >= 0):
             f.write(b"OCMB")
             f.write(np.asarray(bd, np.int32).tobytes())
+        f.flush()
+        os.fsync(f.fileno())
+    if keep_path:
+        try:
+            os.link(tmp, keep_path)
+        except FileExistsError:
+            pass
+    os.replace(tmp, path)
 
 
 def read_checkpoint(path):
@@ -171,8 +190,8 @@ def _read_checkpoint(b, path):
     if ver == 3:
         (arch,) = struct.unpack_from("<i", b, o)
         o += 4
-    if arch != 0:
-        raise FormatError("checkpoint lipid architecture is not linear")
+        if arch not in (1, 2, 3):
+            raise FormatError(f"checkpoint: unknown lipid architecture code {arch}")
     step, time, L, seed = struct.unpack_from("<qddQ", b, o)
     o += 32
     rng = list(struct.unpack_from("<4Q", b, o))
@@ -224,7 +243,7 @@ def _read_checkpoint(b, path):
     nx[gid[has]] = gid[nxt[has]]
     s["nxt"] = nx
     s["bond"] = bond if bond is not None else np.full(Nb, -1, np.int32)
-    hdr = {"version": ver, "N": Nb, "mlen": ml, "nH": nH, "step": step, "time": time, "L": L, "seed": seed,
+    hdr = {"version": ver, "N": Nb, "mlen": ml, "nH": nH, "arch": arch, "step": step, "time": time, "L": L, "seed": seed,
            "rng": rng, "maxbond": maxbond, "warm_end": warm_end, "t_anchor": t_anchor, "step_anchor": step_anchor,
            "dt_anchor": dt_anchor, "soup_seed": soup_seed}
     return s, hdr, gid
@@ -264,18 +283,32 @@ def events_rows(ev):
     return [f"{r},feed,{feed[r]},nan,nan,-" if isinstance(r, int) else r for r in rows]
 
 
-def save_checkpoint(engine, path, r=0):
-    """Checkpoint of replica r of an Engine in the reference's format."""
+def save_checkpoint(engine, path, r=0, keep_path=None):
+    """Checkpoint of replica r of an Engine in the reference's format (version
+    3 with the architecture code for branched lipids); keep_path: an extra
+    name for the same file (ckpt_keep)."""
+    from .params import ARCHS
     p = engine.params[r]
     s = engine.get_state(r)
     _, slot_gid = engine.cells(r)
     step = engine.step_count(r)
     hdr = {"step": step, "time": p.t_anchor + (step - p.step_anchor) * p.dt, "L": p.L, "seed": p.seed,
-           "mlen": p.mlen, "nH": p.nH, "warm_end": max(p.warm_end, p.step_anchor + p.warmup),
+           "mlen": p.mlen, "nH": p.nH, "arch": ARCHS.index(p.lip_arch),
+           "warm_end": max(p.warm_end, p.step_anchor + p.warmup),
            "t_anchor": p.t_anchor, "step_anchor": p.step_anchor, "dt_anchor": p.dt, "soup_seed": p.seed,
            "maxbond": engine.max_bond(r)}
-    write_checkpoint(path, s, slot_gid, hdr)
+    write_checkpoint(path, s, slot_gid, hdr, keep_path=keep_path)
     return hdr
+
+
+def check_architecture(params, hdr):
+    """The reference's restart check (cel/engine/oercel.c:1976): the lipid
+    length, heads and architecture of the checkpoint are the parameters'."""
+    from .params import ARCHS
+    want = (params.mlen, params.nH, ARCHS.index(params.lip_arch))
+    got = (int(hdr["mlen"]), int(hdr["nH"]), int(hdr.get("arch", 0)))
+    if got != want:
+        raise FormatError(f"checkpoint lipid architecture (mlen, nH, arch) = {got} differs from the parameters' {want}")
 
 
 def restart_params(params, hdr):
@@ -301,6 +334,7 @@ def load_checkpoint(engine, path, r=0):
     restart is (cel/engine/tests/validate.sh:80-105)."""
     s, hdr, slot_gid = read_checkpoint(path)
     p = engine.params[r]
+    check_architecture(p, hdr)
     if abs(hdr["L"] - p.L) > 1e-12:
         raise FormatError(f"checkpoint L={hdr['L']} differs from parameter L={p.L}")
     engine.set_state(s, step=hdr["step"], r=r, slot_gid=slot_gid)

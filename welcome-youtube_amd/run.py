@@ -32,6 +32,7 @@ import sys
 import time
 
 from . import io
+from .abi import OcError
 from .engine import Engine
 from .params import ParamError, load, mol_template
 
@@ -148,10 +149,19 @@ def _run(p, outdir, logmsg):
                           p.L, p.mlen, tbonds)
 
     def flush_events():
+        # drained at every output point (at most CHUNK steps apart), so the
+        # device buffer holds one chunk's events; an overflow is reported in
+        # the log and the run goes on
+        try:
+            ev = eng.events()
+        except OcError as err:
+            if "overflowed" not in str(err):
+                raise
+            logmsg(f"warning: {err} (before step {step})\n")
+            ev = eng.events()
         if ef is None:
-            eng.events()  # drain
             return
-        for row in io.events_rows(eng.events()):
+        for row in io.events_rows(ev):
             ef.write(row + "\n")
         ef.flush()
 
@@ -188,13 +198,11 @@ def _run(p, outdir, logmsg):
             stopped = "signal"
         elif p.walltime > 0 and now - t_start > p.walltime:
             stopped = "walltime"
+        flush_events()
         if (p.ckpt_every > 0 and step % p.ckpt_every == 0) or stopped or step >= p.steps:
-            flush_events()
             sf.flush()
-            hdr = io.save_checkpoint(eng, os.path.join(outdir, "ckpt.bin"))
-            if p.ckpt_keep:
-                io.write_checkpoint(os.path.join(outdir, f"ckpt_{step:010d}.bin"), eng.get_state(), eng.cells()[1],
-                                    hdr)
+            io.save_checkpoint(eng, os.path.join(outdir, "ckpt.bin"),
+                               keep_path=os.path.join(outdir, f"ckpt_{step:010d}.bin") if p.ckpt_keep else None)
         if p.log_every > 0 and step % p.log_every == 0:
             logmsg(f"step {step} t={p.t_anchor + (step - p.step_anchor) * p.dt:.2f} "
                    f"{1e3 * (now - t_last) / max(step - s_last, 1):.3f} ms/step\n")
