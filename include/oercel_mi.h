@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define OC_ABI_VERSION 3
+#define OC_ABI_VERSION 4
 #define OC_NTYPES 7
 #define OC_MAXMLEN 12
 
@@ -169,6 +169,9 @@ typedef struct oc_stats {
     int64_t count[8];   /* beads per type */
     int64_t n_lipids;   /* beads with mpos == 0 */
     int64_t n_bonds;    /* chemistry bonds */
+    double virial;      /* sum_{i<j} r_ij . F^C_ij of the conservative pair forces (current ramp) */
+    double pressure;    /* N T_kin / L^2 + virial / (2 L^2): the 2D virial pressure of the
+                         * reference's water check (cel/engine_gpu/tests/compare.py:82-127) */
 } oc_stats;
 
 typedef struct oc_engine oc_engine;

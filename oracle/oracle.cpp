@@ -1387,7 +1387,41 @@ OR_API void or_get_cells(OrSys *s, int32_t *cstart, uint32_t *gid_of_slot) {
     if (gid_of_slot) memcpy(gid_of_slot, s->gid.data(), sizeof(uint32_t) * s->N);
 }
 
-/* kinetic_T and count_types, cel/engine/oercel.c:2053-2071, plus energies */
+/* Conservative virial sum_{i<j} r_ij . F^C_ij = sum_{i<j} a_ij (1 - r) r over
+ * pairs within r < 1 (FP64, minimum image; every pair visited from both
+ * sides and halved), the sum of the reference's water check
+ * (cel/engine_gpu/tests/compare.py:82-110, there for a_WW only). */
+static double pair_virial(const oc_params *p, int N, const double *x, const double *y, const uint8_t *type,
+                          const int32_t *cstart, int64_t step) {
+    const Geo g = make_geo(p, N);
+    const PairPar pp = pair_par(p, step, time_of(p, step));
+    const int ncx = g.ncx, ncy = g.ncy;
+    const double L = g.L;
+    double vir = 0.0;
+    for (int cx = 0; cx < ncx; cx++)
+        for (int i = cstart[cx * ncy]; i < cstart[(cx + 1) * ncy]; i++) {
+            const int khi = (int)((y[i] + 1.0) * g.icsy);
+            const int klo = (int)((y[i] - 1.0) * g.icsy + ncy) - ncy;
+            for (int dc = -1; dc <= 1; dc++) {
+                const int col = (cx + dc + ncx) % ncx;
+                for (int k = klo; k <= khi; k++) {
+                    const int row = (k + ncy) % ncy;
+                    for (int j = cstart[col * ncy + row]; j < cstart[col * ncy + row + 1]; j++) {
+                        if (j == i) continue;
+                        double dx = x[i] - x[j], dy = y[i] - y[j];
+                        dx -= L * std::round(dx / L);
+                        dy -= L * std::round(dy / L);
+                        const double r = sqrt(dx * dx + dy * dy);
+                        if (r < 1.0) vir += 0.5 * pp.a[8 * type[i] + type[j]] * (1.0 - r) * r;
+                    }
+                }
+            }
+        }
+    return vir;
+}
+
+/* kinetic_T and count_types, cel/engine/oercel.c:2053-2071, plus energies,
+ * the conservative virial and the 2D virial pressure */
 OR_API void or_stats(OrSys *s, oc_stats *o) {
     const int N = s->N;
     const double h = 0.5 * s->p.dt;
@@ -1420,6 +1454,8 @@ OR_API void or_stats(OrSys *s, oc_stats *o) {
     o->E_bond = eb;
     o->max_bond = s->maxbond;
     s->maxbond = 0;
+    o->virial = pair_virial(&s->p, N, s->x.data(), s->y.data(), s->type.data(), s->cstart.data(), s->step);
+    o->pressure = (double)N / (s->p.L * s->p.L) * o->T_kin + o->virial / (2.0 * s->p.L * s->p.L);
 }
 
 OR_API int64_t or_get_step(OrSys *s) { return s->step; }

@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 20
     for n in names:
         assert hasattr(lib, n), n
-    assert lib.oc_abi_version() == 3
+    assert lib.oc_abi_version() == 4
 
 
 def test_struct_sizes_match():

@@ -370,6 +370,9 @@ def test_stats_vs_oracle(oracle, eng):
     assert abs(a["px"] - b.px) < 1e-10 and abs(a["py"] - b.py) < 1e-10
     assert abs(a["E_pair"] - b.E_pair) < 1e-5 * abs(b.E_pair)
     assert abs(a["E_bond"] - b.E_bond) < 1e-9 * abs(b.E_bond)
+    say(f"virial {a['virial']:.6f} vs {b.virial:.6f}; pressure {a['pressure']:.6f} vs {b.pressure:.6f}")
+    assert abs(a["virial"] - b.virial) < 1e-5 * abs(b.virial)
+    assert abs(a["pressure"] - b.pressure) < 1e-5 * abs(b.pressure)
     assert a["count"] == list(b.count)[:7]
     assert a["n_lipids"] == b.n_lipids
     e.close()

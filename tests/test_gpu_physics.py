@@ -1,0 +1,205 @@
+"""The HIP step against the reference's own published physics (statistical
+checks; the trajectories differ from the reference's in the last FP32 bits,
+so the comparison is with the reference's numbers and their errors):
+
+  water, L = 64, dt = 0.03, 900 t.u. (cel/engine/README.md:503-505,
+  cel/engine/tests/validate.sh:49-77): <T_kin> = 0.9958 +- 0.0003, g(r) with
+  its first peak 1.079 at r = 0.85 and g(0.05) = 0.018 (30 bins on [0, 3)),
+  virial pressure 29.500 +- 0.006 (cel/engine_gpu/README.md:257, the formula
+  of cel/engine_gpu/tests/compare.py:82-127)                         ~40 s
+  Kolmogorov stirring, water L = 48, A = 0.0015, 600 t.u.
+  (cel/engine/README.md:253, validate.sh:143-170): <v_x>(y) = u0 sin(2 pi
+  y / L) with u0 = 0.266 +- 0.005, no cos part, T_kin - u0^2/4 ~ 1    ~15 s
+  temperature cycle kT = 1 + 0.2 sin(2 pi t / 30), water L = 32, 300 t.u.
+  (cel/engine/README.md:278, validate.sh:172-190): T_kin = 0.9953 + 0.1961
+  sin(2 pi t / 30 - 0.102)                                            ~10 s
+and the forces with stirring and the temperature cycle against the oracle
+(itself pinned to the reference's forces with both on).
+"""
+import importlib
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import mkparams
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-5
+FORCE_ABS = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def water(**kw):
+    """the reference's params/water.ini (pure water soup, seed 7) at dt 0.03"""
+    pm = importlib.import_module("welcome-youtube_amd.params")
+    ov = dict(L="32", dt="0.03", f_lip="0", f_Ka="0", f_Kb="0", c_N="0", chem="0", seed="7")
+    ov.update({k: str(v) for k, v in kw.items()})
+    return pm.load(overrides=ov).check_engine()
+
+
+def block_err(x, nblk=20):
+    b = np.array([c.mean() for c in np.array_split(np.asarray(x), nblk)])
+    return b.std(ddof=1) / np.sqrt(nblk)
+
+
+def rdf_virial(s, L, a_ww=25.0, rmax=3.0, nbin=30):
+    """g(r) and the conservative virial sum of one snapshot, as the reference's
+    scripts compute them (periodic k-d tree, pairs counted once)"""
+    cKDTree = pytest.importorskip("scipy.spatial").cKDTree
+    pos = np.c_[s["x"], s["y"]] % L
+    n = len(pos)
+    pr = cKDTree(pos, boxsize=L).query_pairs(rmax, output_type="ndarray")
+    d = pos[pr[:, 0]] - pos[pr[:, 1]]
+    d -= L * np.round(d / L)
+    r = np.hypot(d[:, 0], d[:, 1])
+    h, e = np.histogram(r, bins=nbin, range=(0, rmax))
+    g = 2 * h / (n * (n / L**2) * np.pi * (e[1:]**2 - e[:-1]**2))
+    rr = r[r < 1.0]
+    return 0.5 * (e[1:] + e[:-1]), g, (a_ww * (1 - rr) * rr).sum()
+
+
+def test_water_temperature_rdf_pressure(eng):
+    p = water(L=64)
+    say(f"case: water L=64 dt=0.03 N={p.N}, 900 t.u.")
+    e = eng.Engine(p)
+    e.start()
+    T, P, gs = [], [], []
+    steps = int(round(900 / 0.03))
+    teq = int(round(30 / 0.03))
+    for s0 in range(0, steps, 10):
+        e.step(10)
+        st = e.stats()
+        assert abs(st["px"]) < 1e-8 and abs(st["py"]) < 1e-8
+        if s0 + 10 > teq:
+            T.append(st["T_kin"])
+            P.append(st["pressure"])
+        if s0 + 10 > teq and (s0 + 10) % 500 == 0:
+            snap = e.get_state()
+            r, g, vir = rdf_virial(snap, p.L)
+            gs.append(g)
+            if len(gs) == 1:
+                say(f"  virial: device {st['virial']:.6f}, k-d tree {vir:.6f}")
+                assert abs(st["virial"] - vir) < 1e-5 * vir
+    e.close()
+    Tm, eT = np.mean(T), block_err(T)
+    Pm, eP = np.mean(P), block_err(P)
+    g = np.mean(gs, axis=0)
+    pk = int(g.argmax())
+    say(f"  <T> = {Tm:.5f} +- {eT:.5f} (reference 0.9958 +- 0.0003)")
+    say(f"  g(r): first peak {g[pk]:.4f} at r = {r[pk]:.2f} (reference 1.079 at 0.85), g(0.05) = {g[0]:.4f} "
+        f"(reference 0.018), <g(r > 2)> = {g[r > 2].mean():.4f}")
+    say(f"  pressure {Pm:.4f} +- {eP:.4f} (reference 29.500 +- 0.006)")
+    assert abs(Tm - 0.9958) < 4 * np.hypot(eT, 0.0003)
+    assert abs(r[pk] - 0.85) < 1e-9 and abs(g[pk] - 1.079) < 0.01
+    # g(0.05) counts the few pairs closer than 0.1: the reference's value
+    # comes from 5 snapshots (validate.sh:69), ~52 pairs, so each side's error
+    # is Poisson in its own pair count
+    per_snap = (p.N / 2) * (p.N / p.L**2) * np.pi * 0.1**2  # pairs in bin 0 per snapshot at g = 1
+    e_ref = 0.018 / np.sqrt(0.018 * per_snap * 5)
+    e_dev = max(g[0], 1e-3) / np.sqrt(max(g[0], 1e-3) * per_snap * len(gs))
+    say(f"  g(0.05): {g[0]:.4f} +- {e_dev:.4f} vs 0.018 +- {e_ref:.4f}")
+    assert abs(g[0] - 0.018) < 4 * np.hypot(e_dev, e_ref)
+    assert abs(g[r > 2].mean() - 1) < 0.02
+    assert abs(Pm - 29.5) < 0.01 * 29.5
+
+
+def test_stirring_profile(eng):
+    A = 0.0015
+    p = water(L=48, stir_A=A)
+    say(f"case: stirring water L=48 A={A} N={p.N}, 600 t.u.")
+    e = eng.Engine(p)
+    e.start()
+    L = p.L
+    k = 2 * np.pi / L
+    e.step(10000)  # 300 t.u.: the flow develops on 1/(nu k^2) ~ 180 t.u.
+    fits, T, ys, vxs = [], [], [], []
+    for _ in range(100):
+        e.step(100)
+        s = e.get_state()
+        st = e.stats()
+        assert abs(st["px"]) < 1e-8 and abs(st["py"]) < 1e-8
+        T.append(st["T_kin"])
+        vx = s["vx"] + 0.5 * p.dt * s["fx"]  # full velocity
+        M = np.c_[np.sin(k * s["y"]), np.cos(k * s["y"]), np.ones(p.N)]
+        fits.append(np.linalg.lstsq(M, vx, rcond=None)[0])
+    e.close()
+    fits = np.array(fits)
+    u0, c1 = fits[:, 0].mean(), fits[:, 1].mean()
+    eu, ec = block_err(fits[:, 0], 10), block_err(fits[:, 1], 10)
+    Tm = np.mean(T)
+    say(f"  u0 = {u0:.4f} +- {eu:.4f} (reference 0.266 +- 0.005), cos part {c1:+.4f} +- {ec:.4f}, "
+        f"nu = {A / (u0 * k * k):.3f} (reference 0.33), T_kin - u0^2/4 = {Tm - u0 ** 2 / 4:.4f} (reference 0.9963)")
+    assert abs(u0 - 0.266) < 4 * np.hypot(eu, 0.005)
+    assert abs(c1) < 4 * ec + 1e-3
+    assert abs(Tm - u0 ** 2 / 4 - 1) < 0.03
+
+
+def test_temperature_cycle(eng):
+    p = water(L=32, kT_amp=0.2, kT_period=30)
+    say(f"case: temperature cycle, water L=32 N={p.N}, kT = 1 + 0.2 sin(2 pi t / 30), 300 t.u.")
+    e = eng.Engine(p)
+    e.start()
+    t, T = [], []
+    for _ in range(1000):
+        e.step(10)
+        st = e.stats()
+        t.append(st["time"])
+        T.append(st["T_kin"])
+    e.close()
+    t, T = np.array(t), np.array(T)
+    m = t >= 30
+    w = 2 * np.pi / 30
+
+    def fit(tt, TT):
+        M = np.c_[np.ones_like(tt), np.sin(w * tt), np.cos(w * tt)]
+        c = np.linalg.lstsq(M, TT, rcond=None)[0]
+        return c[0], np.hypot(c[1], c[2]), np.arctan2(-c[2], c[1])
+
+    c0, amp, lag = fit(t[m], T[m])
+    # errors from the scatter of the nine whole periods' own fits
+    per = np.array([fit(t[(t >= 30 * q) & (t < 30 * (q + 1))], T[(t >= 30 * q) & (t < 30 * (q + 1))])
+                    for q in range(1, 10)])
+    e0, ea, el = per.std(0, ddof=1) / 3.0
+    say(f"  T_kin = {c0:.4f} + {amp:.4f} sin(2 pi t/30 - {lag:.3f})  (+- {e0:.4f}, {ea:.4f}, {el:.3f}); "
+        f"reference 0.9953 + 0.1961 sin(2 pi t/30 - 0.102)")
+    # the reference's own bars (validate.sh:186), then its measured values
+    # (their error taken as ours: the same run length)
+    assert abs(amp - 0.2) < 0.01 and abs(c0 - 1) < 0.03 and 0 <= lag < 0.2
+    assert abs(c0 - 0.9953) < 4 * np.sqrt(2) * e0
+    assert abs(amp - 0.1961) < 4 * np.sqrt(2) * ea
+    assert abs(lag - 0.102) < 4 * np.sqrt(2) * el
+
+
+@pytest.mark.parametrize("stir_A,stir_n,kT_amp", [(0.7, 1, 0.0), (0.05, 2, 0.3), (0.0, 1, 0.3)])
+def test_forces_with_stirring_and_kT_cycle_vs_oracle(oracle, eng, rng, stir_A, stir_n, kT_amp):
+    """Pair forces plus the mean-free stirring force, with the noise amplitude
+    of kT(t), per bead within 1e-5 of the oracle (pinned to the reference's
+    forces with both on)."""
+    say(f"case: forces stir_A={stir_A} stir_n={stir_n} kT_amp={kT_amp}")
+    p = mkparams(23.7, nsub=4, f_lip=0.15, c_N=0.1, seed=21, stir_A=stir_A, stir_n=stir_n, kT_amp=kT_amp,
+                 kT_period=7.0)
+    s = oracle.soup(p.to_c(), p.N)
+    s["vtx"] = (s["vx"] + 0.3 * rng.standard_normal(p.N)).astype(np.float32).astype(np.float64)
+    s["vty"] = s["vy"].astype(np.float32).astype(np.float64)
+    step = 1234  # t = 12.34: kT(t) away from kT
+    e = eng.Engine(p)
+    ld = {k: s[k] for k in ("x", "y", "type", "mpos", "nxt")}
+    ld["vx"], ld["vy"] = s["vtx"], s["vty"]
+    ld["bond"] = np.full(p.N, -1, np.int32)
+    e.set_state(ld, step=0)
+    e.forces(step, bonded=False)
+    out = e.get_state()
+    e.close()
+    ref = oracle.pair_forces_gid(p.to_c(), s, step, theta32=1, sabs=True)
+    d = np.hypot(out["fx"] - ref["fx"], out["fy"] - ref["fy"])
+    rel = (d / (ref["sabs"] + FORCE_ABS / FORCE_TOL + stir_A)).max()
+    say(f"  max |dF| / (sum|f_ij| + 1 + A) = {rel:.3e}, max|dF| = {d.max():.3e}")
+    assert rel < FORCE_TOL
+    # the stirring force is mean-free: the total force still vanishes
+    assert abs(out["fx"].sum()) < 1e-9 * np.abs(out["fx"]).sum() + 1e-9

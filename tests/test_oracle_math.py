@@ -268,3 +268,30 @@ def test_rdf_of_ideal_gas_is_one(oracle):
     pairs0 = samples * 0.5 * n_t * (n_t / p.L ** 2) * np.pi * dr ** 2
     assert want0 > want
     assert abs(g[0] - want0) < 5.0 / np.sqrt(pairs0), (g[0], want0)
+
+
+def test_virial_pressure_vs_the_reference_formula(oracle):
+    """The oracle's conservative virial equals the reference water check's own
+    sum (1/(2V)) sum_{i<j} a (1 - r) r over pairs found by a periodic k-d tree
+    (cel/engine_gpu/tests/compare.py:82-110), and the pressure is
+    rho T_kin + that sum; for a mixture the pair's a_ij enters."""
+    cKDTree = pytest.importorskip("scipy.spatial").cKDTree
+    for kw in (dict(f_lip=0.0, c_N=0.0), dict(f_lip=0.2, c_N=0.1)):
+        p = mkparams(17.3, **kw)
+        cp = p.to_c()
+        o = oracle.System(cp, p.N)
+        o.start(oracle.soup(cp, p.N))
+        o.step(20)
+        st = o.stats()
+        s = o.get_state()
+        L = p.L
+        pos = np.c_[s["x"], s["y"]] % L
+        pr = cKDTree(pos, boxsize=L).query_pairs(1.0, output_type="ndarray")
+        d = pos[pr[:, 0]] - pos[pr[:, 1]]
+        d -= L * np.round(d / L)
+        r = np.hypot(d[:, 0], d[:, 1])
+        a = np.array(p.a)[s["type"][pr[:, 0]], s["type"][pr[:, 1]]]
+        m = r < 1.0
+        vir = (a[m] * (1 - r[m]) * r[m]).sum()
+        assert abs(st.virial - vir) < 1e-10 * vir
+        assert abs(st.pressure - (p.N / L**2 * st.T_kin + vir / (2 * L * L))) < 1e-10 * st.pressure

@@ -65,6 +65,7 @@ class oc_stats(C.Structure):
         ("step", C.c_int64), ("time", C.c_double), ("T_kin", C.c_double), ("px", C.c_double), ("py", C.c_double),
         ("E_kin", C.c_double), ("E_pair", C.c_double), ("E_bond", C.c_double), ("max_bond", C.c_double),
         ("count", C.c_int64 * 8), ("n_lipids", C.c_int64), ("n_bonds", C.c_int64),
+        ("virial", C.c_double), ("pressure", C.c_double),
     ]
 
 

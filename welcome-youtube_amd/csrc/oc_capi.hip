@@ -63,7 +63,8 @@ struct oc_engine {
     int32_t *d_tpart = nullptr;  // chunk totals of the tile-count scan
     int2 *d_rmax = nullptr;      // per replica: largest cell displacement of a bead that moved > 1 cell
     bool keys_fresh = false;     // d_key holds the keys of the current state (written by the force kernel)
-    double *d_stir = nullptr, *d_epot = nullptr, *d_ebond = nullptr, *d_part = nullptr, *d_statout = nullptr;
+    double *d_stir = nullptr, *d_epot = nullptr, *d_evir = nullptr, *d_ebond = nullptr, *d_part = nullptr,
+           *d_statout = nullptr;
     uint8_t *d_flag = nullptr;
     int32_t *d_best = nullptr;
     // precursor chemistry: the check's list of beads with a role, in segments
@@ -326,6 +327,7 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false) {
     A.stir = h->any_stir ? h->d_stir : nullptr;
     A.f = h->d_f;
     A.epot = h->d_epot;
+    A.evir = h->d_evir;
     A.err = h->d_err;
     // forces also leave the next step's cell keys
     A.v = h->st[c].v;
@@ -806,6 +808,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     rc = rc ? rc : dalloc(h, &h->d_rmax, h->nrep);
     rc = rc ? rc : dalloc(h, &h->d_tpart, (h->stiles.size() + kTScanChunk - 1) / kTScanChunk);
     rc = rc ? rc : dalloc(h, &h->d_epot, Nt);
+    rc = rc ? rc : dalloc(h, &h->d_evir, Nt);
     rc = rc ? rc : dalloc(h, &h->d_ebond, Nt);
     rc = rc ? rc : dalloc(h, &h->d_part, h->rblk.size() * 16);
     rc = rc ? rc : dalloc(h, &h->d_statout, (size_t)nrep * 16);
@@ -861,6 +864,7 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     e = e ? e : hipMemset(h->d_rmax, 0, h->nrep * sizeof(int2));
     e = e ? e : hipMemset(h->d_ebond, 0, Nt * sizeof(double));
     e = e ? e : hipMemset(h->d_epot, 0, Nt * sizeof(double));
+    e = e ? e : hipMemset(h->d_evir, 0, Nt * sizeof(double));
     e = e ? e : hipMemset(h->d_f, 0, Nt * sizeof(double2));
     for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(h->st[k].lnk, 0xff, Nt * 4);
     for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(h->st[k].prv, 0xff, Nt * 4);
@@ -1284,7 +1288,7 @@ OC_API int oc_get_stats(oc_engine *h, int32_t r, oc_stats *out) {
                                                                       h->d_err);
     k_stats_partial<<<(int)h->rblk.size(), 256, 0, h->stream>>>(h->d_ri, h->d_rblk, h->st[c].v, h->d_f, h->st[c].gt,
                                                                h->st[c].mp, h->st[c].lnk, h->d_epot, h->d_ebond,
-                                                               h->d_part);
+                                                               h->d_evir, h->d_part);
     k_stats_final<<<h->nrep, 256, 0, h->stream>>>(h->d_ri, h->d_part, h->d_statout);
     HIPCHK(hipGetLastError());
     if (int rc = check_err(h)) return rc;
@@ -1308,6 +1312,8 @@ OC_API int oc_get_stats(oc_engine *h, int32_t r, oc_stats *out) {
     for (int k = 0; k < OC_NTYPES; k++) out->count[k] = (int64_t)v[5 + k];
     out->n_lipids = (int64_t)v[12];
     out->n_bonds = (int64_t)v[13];
+    out->virial = v[15];
+    out->pressure = (double)R.N / (R.L * R.L) * out->T_kin + v[15] / (2.0 * R.L * R.L);
     rs.maxbond_bits = 0;
     HIPCHK(hipMemcpy(h->d_rs + r, &rs, sizeof rs, hipMemcpyHostToDevice));
     return 0;

@@ -786,7 +786,7 @@ __device__ __forceinline__ float pair_core(float dx, float dy, float r2, float v
 
 __device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float vyi, uint32_t gi, const float *arow,
                                          double2 pj, float2 vj, uint32_t gtj, double sxs, double sys, float gam,
-                                         float sdtk, uint64_t sseed, double &ax, double &ay, float &e, bool energy) {
+                                         float sdtk, uint64_t sseed, double &ax, double &ay, float2 &e, bool energy) {
     // FP64 image difference, then FP32 force arithmetic (fused multiply-adds).
     // Both orientations of a pair run the same operations on negated
     // operands, so f_ji = -f_ij exactly.
@@ -797,7 +797,13 @@ __device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float 
         const float fm = pair_core(dx, dy, r2, vxi, vyi, gi, arow, vj, gtj, gam, sdtk, sseed, w);
         ax += (double)(fm * dx);
         ay += (double)(fm * dy);
-        if (energy) e += 0.25f * arow[gtj >> kGidBits] * w * w;
+        if (energy) {
+            // e.x: half the pair's energy a/2 (1-r)^2; e.y: half its
+            // conservative virial r . F^C = a (1-r) r (both orientations add)
+            const float a = arow[gtj >> kGidBits];
+            e.x += 0.25f * a * w * w;
+            e.y += 0.5f * a * w * (1.0f - w);
+        }
     }
 }
 
@@ -958,7 +964,7 @@ struct PairArgs {
     const uint32_t *gt;
     const double *stir;
     double2 *f;
-    double *epot;
+    double *epot, *evir;  // per-bead pair energy and conservative virial (ENERGY)
     uint32_t *err;
     // next step's cell keys (new_key of the integrated position) when key != 0
     const double2 *v;
@@ -1137,10 +1143,11 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     // Each lane owns one i-bead at a time.
     double mb = 0.0;  // largest bond length seen by this lane
     // per-bead epilogue: energy, or force (+ stirring, + bonded terms)
-    auto finish = [&](int slot, double2 pi, uint32_t gti, const BondPre &B, double ax, double ay, float e,
+    auto finish = [&](int slot, double2 pi, uint32_t gti, const BondPre &B, double ax, double ay, float2 e,
                       double2 vh) {
         if (ENERGY) {
-            A.epot[slot] = (double)e;
+            A.epot[slot] = (double)e.x;
+            A.evir[slot] = (double)e.y;
         } else {
             double bx = 0.0;
             if (A.stir) bx = A.stir[slot] - S.stir_mean;
@@ -1187,7 +1194,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                 khi = min(khi, T.k1 - 1 + M);
             }
             double ax = 0.0, ay = 0.0;
-            float e = 0.0f;
+            float2 e = make_float2(0.0f, 0.0f);
             // Flat path (staged, no periodic image anywhere in the wavefront's
             // windows): a column's window rows are one contiguous LDS range
             // (its staged pieces are concatenated in unwrapped row order), and
@@ -1912,7 +1919,7 @@ __global__ __launch_bounds__(64) void k_chem_apply(const RepStep *__restrict__ r
 
 // --------------------------------------------------------------------------- reductions
 
-constexpr int kNStat = 16;  // k2, px, py, epair, ebond, count[7], lipids, bonds, stir, spare
+constexpr int kNStat = 16;  // k2, px, py, epair, ebond, count[7], lipids, bonds, stir, virial
 
 // Fixed-partition FP64 sums: chunk b of a replica covers beads
 // [b*kRedChunk, (b+1)*kRedChunk); a thread sums a strided subset in order,
@@ -1922,7 +1929,8 @@ __global__ __launch_bounds__(256) void k_stats_partial(const RepInfo *__restrict
                                                        const uint32_t *__restrict__ gt, const int8_t *__restrict__ mp,
                                                        const int32_t *__restrict__ lnk,
                                                        const double *__restrict__ epot,
-                                                       const double *__restrict__ ebond, double *__restrict__ part) {
+                                                       const double *__restrict__ ebond,
+                                                       const double *__restrict__ evir, double *__restrict__ part) {
     __shared__ double red[kNStat][256];
     const int r = rblk[blockIdx.x];
     const RepInfo &R = ri[r];
@@ -1945,6 +1953,7 @@ __global__ __launch_bounds__(256) void k_stats_partial(const RepInfo *__restrict
         acc[12] += mp[s] == 0 ? 1.0 : 0.0;
         const int l = lnk[s];
         acc[13] += (mp[s] < 0 && l >= 0 && (gt[s] & kGidMask) < (gt[l] & kGidMask)) ? 1.0 : 0.0;
+        acc[15] += evir[s];
     }
     for (int k = 0; k < kNStat; k++) red[k][threadIdx.x] = acc[k];
     __syncthreads();
