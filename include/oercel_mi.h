@@ -235,6 +235,26 @@ int oc_sort(oc_engine *h);                       /* sort_cells, :828-933 */
 int oc_forces(oc_engine *h, int64_t step, int32_t with_bonded); /* pair_forces (+ bonded_forces), :1198-1216, :1317-1343 */
 int oc_chem(oc_engine *h, int64_t step);         /* precursor chemistry at this step (no counterpart) */
 
+/* oc_forces with the caller's pair noise in place of the counter hash: the
+ * reference's force test compares forces on identical positions, velocities
+ * and noise (cel/engine/tests/forcetest.c:91-103).  theta[q] is the
+ * unit-variance deviate of the pair (pair_lo[q], pair_hi[q]) of gids, lo < hi,
+ * entries in strictly increasing (lo, hi) order; an interacting pair missing
+ * from the table gets theta = 0 and the call returns OC_EDEVICE.  The
+ * single-pass pair kernel; one replica. */
+int oc_forces_theta(oc_engine *h, int64_t step, int32_t with_bonded, const uint32_t *pair_lo,
+                    const uint32_t *pair_hi, const float *theta, int64_t npair);
+
+/* oc_chem with the caller's uniforms in place of the counter hashes (no
+ * counterpart in the reference): u_conv[gid] and u_break[gid] are the P
+ * conversion and bond break draws of every bead, u_form[q] the symmetric
+ * formation draw of the pair (form_lo[q], form_hi[q]) (lo < hi, strictly
+ * increasing); a free-T pair within r_form missing from the table does not
+ * bond and the call returns OC_EDEVICE.  Precursor chemistry only; one
+ * replica. */
+int oc_chem_uniforms(oc_engine *h, int64_t step, const double *u_conv, const double *u_break,
+                     const uint32_t *form_lo, const uint32_t *form_hi, const double *u_form, int64_t nform);
+
 /* Cell list of replica r: cstart[ncell+1] (slot indices relative to the
  * replica) and gid_of_slot[N]; ncell = floor(L) * floor(L) * nsub, column-major
  * (cel/engine/oercel.c:739-745, 779-786). */

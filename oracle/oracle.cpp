@@ -244,6 +244,52 @@ static PairPar pair_par(const oc_params *p, int64_t step, double t) {
 
 /* one pair term on bead i from j (dx = x_i - x_j image), the arithmetic of
  * pair_seg, cel/engine/oercel.c:968-979; returns the force on i */
+/* Caller-supplied draws, the oracle side of oc_forces_theta / oc_chem_uniforms:
+ * per-pair theta and formation uniforms keyed by (min gid << 32 | max gid),
+ * per-gid conversion and break uniforms.  While set they replace the hashes
+ * in pair_term and or_chem. */
+static struct {
+    std::vector<uint64_t> tk, fk;
+    std::vector<double> tv, fv, uconv, ubreak;
+    bool theta = false, chem = false;
+    int64_t missing = 0;
+} g_inj;
+
+static double inj_lookup(const std::vector<uint64_t> &k, const std::vector<double> &v, uint32_t gi, uint32_t gj,
+                         double absent) {
+    const uint64_t key = gi < gj ? ((uint64_t)gi << 32) | gj : ((uint64_t)gj << 32) | gi;
+    const auto it = std::lower_bound(k.begin(), k.end(), key);
+    if (it == k.end() || *it != key) {
+#pragma omp atomic
+        g_inj.missing++;
+        return absent;
+    }
+    return v[it - k.begin()];
+}
+
+static std::vector<uint64_t> inj_keys(const uint32_t *lo, const uint32_t *hi, int64_t n) {
+    std::vector<uint64_t> k((size_t)n);
+    for (int64_t q = 0; q < n; q++) k[q] = ((uint64_t)lo[q] << 32) | hi[q];
+    return k;
+}
+
+/* theta table (theta != NULL), chemistry uniforms (uconv != NULL); NULL clears */
+OR_API void or_inject(const uint32_t *tlo, const uint32_t *thi, const double *theta, int64_t nt, int32_t N,
+                      const double *uconv, const double *ubreak, const uint32_t *flo, const uint32_t *fhi,
+                      const double *uform, int64_t nf) {
+    g_inj.theta = theta != nullptr;
+    g_inj.tk = theta ? inj_keys(tlo, thi, nt) : std::vector<uint64_t>();
+    g_inj.tv = theta ? std::vector<double>(theta, theta + nt) : std::vector<double>();
+    g_inj.chem = uconv != nullptr;
+    g_inj.uconv = uconv ? std::vector<double>(uconv, uconv + N) : std::vector<double>();
+    g_inj.ubreak = uconv ? std::vector<double>(ubreak, ubreak + N) : std::vector<double>();
+    g_inj.fk = uconv ? inj_keys(flo, fhi, nf) : std::vector<uint64_t>();
+    g_inj.fv = uconv ? std::vector<double>(uform, uform + nf) : std::vector<double>();
+    g_inj.missing = 0;
+}
+
+OR_API int64_t or_inject_missing(void) { return g_inj.missing; }
+
 static inline void pair_term(const PairPar &pp, int theta32, double dx, double dy, double dvx, double dvy, int ti, int tj,
                              uint32_t gi, uint32_t gj, double *fx, double *fy, double *e, double *sabs) {
     const double r2 = dx * dx + dy * dy;
@@ -252,7 +298,8 @@ static inline void pair_term(const PairPar &pp, int theta32, double dx, double d
     const double ir = 1.0 / r;
     const double w = 1.0 - r;
     const double rv = (dx * dvx + dy * dvy) * ir;
-    const double th = theta32 ? pair_theta32(pp.sseed, gi, gj) : pair_theta(pp.sseed, gi, gj);
+    const double th = g_inj.theta ? inj_lookup(g_inj.tk, g_inj.tv, gi, gj, 0.0)
+                                  : (theta32 ? pair_theta32(pp.sseed, gi, gj) : pair_theta(pp.sseed, gi, gj));
     const double a = pp.a[8 * ti + tj];
     const double fm = w * (a - pp.gamma * w * rv + pp.sdt * th) * ir;
     *fx += fm * dx;
@@ -693,8 +740,10 @@ OR_API int64_t or_chem(const oc_params *p, int32_t N, int64_t step, const double
         }
     };
     for (int i = 0; i < N; i++) {
-        if (mpos[i] < 0 && lnk[i] >= 0 && gid[i] < gid[lnk[i]] && hash_u01(kb, gid[i]) < p->p_break) brk[i] = 1;
-        if (type[i] == OC_N && hash_u01(kc, gid[i]) < p->p_conv) {
+        const double ub = g_inj.chem ? g_inj.ubreak[gid[i]] : hash_u01(kb, gid[i]);
+        const double uc = g_inj.chem ? g_inj.uconv[gid[i]] : hash_u01(kc, gid[i]);
+        if (mpos[i] < 0 && lnk[i] >= 0 && gid[i] < gid[lnk[i]] && ub < p->p_break) brk[i] = 1;
+        if (type[i] == OC_N && uc < p->p_conv) {
             bool found = false;
             visit(i, [&](int j, double r2) {
                 if (type[j] == OC_T && mpos[j] >= 0 && r2 < rc2) { found = true; return false; }
@@ -708,7 +757,8 @@ OR_API int64_t or_chem(const oc_params *p, int32_t N, int64_t step, const double
             int bj = -1;
             visit(i, [&](int j, double r2) {
                 if (type[j] == OC_T && mpos[j] < 0 && lnk[j] < 0 && r2 < rf2) {
-                    const double u = pair_u01(kf, gid[i], gid[j]);
+                    const double u = g_inj.chem ? inj_lookup(g_inj.fk, g_inj.fv, gid[i], gid[j], 2.0)
+                                                : pair_u01(kf, gid[i], gid[j]);
                     if (u < p->p_form && (u < bu || (u == bu && gid[j] < bg))) { bu = u; bg = gid[j]; bj = j; }
                 }
                 return true;

@@ -95,6 +95,9 @@ def lib():
             "or_window_grad": (C.c_int, [vp, C.c_int32, P(RdfSpec), dp, C.c_int32, dp, dp, dp]),
             "or_rdf": (None, [vp, P(RdfSpec), dp]),
             "or_set_pair_params": (None, [vp, dp, C.c_double]),
+            "or_inject": (None, [P(C.c_uint32), P(C.c_uint32), dp, C.c_int64, C.c_int32, dp, dp, P(C.c_uint32),
+                                 P(C.c_uint32), dp, C.c_int64]),
+            "or_inject_missing": (C.c_int64, []),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -414,3 +417,36 @@ def run_ref_cli(outdir, overrides, fast=False, timeout=600):
     if r.returncode:
         raise RuntimeError(f"reference run failed ({r.returncode}): {r.stderr[-2000:]}")
     return r.stdout
+
+
+def inject(theta=None, chem=None, N=0):
+    """Caller-supplied draws in place of the hashes (the oracle side of
+    oc_forces_theta / oc_chem_uniforms) until inject() clears them.
+    theta: (lo, hi, values) pair table; chem: (u_conv, u_break, (lo, hi, u_form)),
+    per-gid arrays of N beads.  Returns nothing; or_inject_missing() counts
+    lookups of pairs absent from a table."""
+    keep = []
+
+    def arr(a, t):
+        a = np.ascontiguousarray(a, dtype=t)
+        keep.append(a)
+        return a
+
+    tl = th = tv = None
+    nt = 0
+    if theta is not None:
+        tl, th, tv = arr(theta[0], np.uint32), arr(theta[1], np.uint32), arr(theta[2], np.float64)
+        nt = len(tv)
+    uc = ub = fl = fh = fv = None
+    nf = 0
+    if chem is not None:
+        uc, ub = arr(chem[0], np.float64), arr(chem[1], np.float64)
+        fl, fh, fv = arr(chem[2][0], np.uint32), arr(chem[2][1], np.uint32), arr(chem[2][2], np.float64)
+        nf = len(fv)
+    u32 = lambda a: None if a is None else _p(a, C.c_uint32)
+    dd = lambda a: None if a is None else d(a)
+    lib().or_inject(u32(tl), u32(th), dd(tv), nt, int(N), dd(uc), dd(ub), u32(fl), u32(fh), dd(fv), nf)
+
+
+def inject_missing():
+    return lib().or_inject_missing()

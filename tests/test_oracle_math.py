@@ -295,3 +295,56 @@ def test_virial_pressure_vs_the_reference_formula(oracle):
         vir = (a[m] * (1 - r[m]) * r[m]).sum()
         assert abs(st.virial - vir) < 1e-10 * vir
         assert abs(st.pressure - (p.N / L**2 * st.T_kin + vir / (2 * L * L))) < 1e-10 * st.pressure
+
+
+def pair_table(s, L, rmax, rng):
+    """(lo, hi, value) of every gid pair closer than rmax (periodic), sorted by
+    (lo, hi), with uniform values on [-sqrt3, sqrt3)"""
+    cKDTree = pytest.importorskip("scipy.spatial").cKDTree
+    pos = np.c_[s["x"], s["y"]] % L
+    pr = cKDTree(pos, boxsize=L).query_pairs(rmax, output_type="ndarray")
+    lo, hi = np.minimum(pr[:, 0], pr[:, 1]).astype(np.uint32), np.maximum(pr[:, 0], pr[:, 1]).astype(np.uint32)
+    o = np.lexsort((hi, lo))
+    return lo[o], hi[o], rng.uniform(-np.sqrt(3), np.sqrt(3), len(o))
+
+
+def test_injected_noise_forces_vs_numpy_brute_force(oracle):
+    """The oracle's pair forces with a caller's noise table equal an
+    independent FP64 numpy statement of the DPD pair force with that noise
+    (cel/engine/tests/forcetest.c:91-103: conservative + dissipative + random,
+    F_ij = w (a_ij - gamma w r.v + sigma/sqrt(dt) theta_ij) r_hat)."""
+    rng = np.random.default_rng(4)
+    p = mkparams(12.5, nsub=4, f_lip=0.2, c_N=0.1, seed=8)
+    cp = p.to_c()
+    s = oracle.soup(cp, p.N)
+    s["vtx"] = s["vx"] + 0.2 * rng.standard_normal(p.N)
+    s["vty"] = s["vy"].copy()
+    lo, hi, th = pair_table(s, p.L, 1.001, rng)
+    oracle.inject(theta=(lo, hi, th))
+    try:
+        f = oracle.pair_forces_gid(cp, s, 77, theta32=1)
+        assert oracle.inject_missing() == 0
+    finally:
+        oracle.inject()
+    L = p.L
+    d = np.c_[s["x"][lo] - s["x"][hi], s["y"][lo] - s["y"][hi]]
+    d -= L * np.round(d / L)
+    r = np.hypot(d[:, 0], d[:, 1])
+    m = r < 1.0
+    r, d, th_m, a = r[m], d[m], th[m], np.array(p.a)[s["type"][lo[m]], s["type"][hi[m]]]
+    e = d / r[:, None]
+    w = 1 - r
+    rv = e[:, 0] * (s["vtx"][lo[m]] - s["vtx"][hi[m]]) + e[:, 1] * (s["vty"][lo[m]] - s["vty"][hi[m]])
+    sdt = np.sqrt(2 * p.gamma * p.kT) / np.sqrt(p.dt)
+    fm = a * w - p.gamma * w * w * rv + sdt * w * th_m
+    bfx = np.zeros(p.N)
+    bfy = np.zeros(p.N)
+    np.add.at(bfx, lo[m], fm * e[:, 0])
+    np.add.at(bfy, lo[m], fm * e[:, 1])
+    np.add.at(bfx, hi[m], -fm * e[:, 0])
+    np.add.at(bfy, hi[m], -fm * e[:, 1])
+    assert np.abs(f["fx"] - bfx).max() < 1e-9 * max(1, np.abs(bfx).max())
+    assert np.abs(f["fy"] - bfy).max() < 1e-9 * max(1, np.abs(bfy).max())
+    # the noise enters: forces with the table differ from the hashed ones
+    g = oracle.pair_forces_gid(cp, s, 77, theta32=1)
+    assert np.abs(g["fx"] - bfx).max() > 1.0

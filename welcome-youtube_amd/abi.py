@@ -143,6 +143,10 @@ def _declare(lib):
         "oc_window_rewind": (C.c_int, [vp]),
         "oc_window_grad": (C.c_int, [vp, C.c_int32, C.c_int32, P(oc_rdf_spec), P(C.c_double), C.c_int32,
                                      P(C.c_double), P(C.c_double), P(C.c_double)]),
+        "oc_forces_theta": (C.c_int, [vp, C.c_int64, C.c_int32, P(C.c_uint32), P(C.c_uint32), P(C.c_float),
+                                      C.c_int64]),
+        "oc_chem_uniforms": (C.c_int, [vp, C.c_int64, P(C.c_double), P(C.c_double), P(C.c_uint32), P(C.c_uint32),
+                                       P(C.c_double), C.c_int64]),
         "oc_kinetic_dev_work": (C.c_size_t, [C.c_int64]),
         "oc_kinetic_dev": (C.c_int, [vp, vp, vp, vp, C.c_int64, C.c_double, vp, vp, C.c_size_t, vp]),
         "oc_kinetic_dev_noalloc": (C.c_int, [vp, vp, vp, vp, C.c_int64, C.c_double, vp, vp]),

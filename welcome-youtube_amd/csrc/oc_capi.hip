@@ -307,7 +307,7 @@ static void drop_graphs(oc_engine *h) {
 
 // pair forces (+ bonded terms when bonded, in the same pass), or the pair
 // energies when energy
-static void launch_pair(oc_engine *h, bool energy, bool bonded = false) {
+static void launch_pair(oc_engine *h, bool energy, bool bonded = false, const InjTheta *inj = nullptr) {
     const int c = h->cur;
     PairArgs A;
     A.ri = h->d_ri;
@@ -350,6 +350,21 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false) {
         if (h->any_angle) k_pair_fx<B, C, kPairList, true><<<grid, B, 0, h->stream>>>(A);         \
         else k_pair_fx<B, C, kPairList, false><<<grid, B, 0, h->stream>>>(A);                     \
     } while (0)
+    if (inj) {  // caller-supplied pair noise: the single pass
+        A.inj = *inj;
+#define OC_PAIRINJ(B, C)                                                                         \
+    do {                                                                                         \
+        if (h->any_angle) k_pair<B, C, 0, false, true, true><<<grid, B, 0, h->stream>>>(A);       \
+        else k_pair<B, C, 0, false, false, true><<<grid, B, 0, h->stream>>>(A);                   \
+    } while (0)
+        switch (h->pair_block) {
+        case 64: OC_PAIRINJ(64, 224); break;
+        case 128: OC_PAIRINJ(128, 448); break;
+        default: OC_PAIRINJ(256, 896); break;
+        }
+#undef OC_PAIRINJ
+        return;
+    }
     if (V == 3 && !energy) {
         switch (h->pair_block) {
         case 64: OC_PAIRFX(64, 224); break;
@@ -381,9 +396,9 @@ static void launch_stir(oc_engine *h) {
     k_stir_final<<<h->nrep, 256, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_part);
 }
 
-static void launch_forces(oc_engine *h, bool with_bonded) {
+static void launch_forces(oc_engine *h, bool with_bonded, const InjTheta *inj = nullptr) {
     launch_stir(h);
-    launch_pair(h, false, with_bonded);
+    launch_pair(h, false, with_bonded, inj);
     if (with_bonded && h->any_tmpl) {
         const int c = h->cur;
         k_mol<false><<<(int)h->bblk.size(), kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy,
@@ -446,19 +461,20 @@ static void launch_rchem(oc_engine *h, bool list) {
 }
 
 // the precursor chemistry (after the sort, on the fresh cell list)
-static void launch_pchem(oc_engine *h, bool listed) {
+static void launch_pchem(oc_engine *h, bool listed, const ChemInj *inj = nullptr) {
     if (!h->any_chem) return;
     const int c = h->cur;
+    const ChemInj ci = inj ? *inj : ChemInj{nullptr, nullptr, nullptr, nullptr, 0, nullptr};
     if (!listed) {
         // a state that was not just placed: list the roles of every bead
         h->cseg_n = (int)h->bblk.size();
         k_chem_list<<<h->cseg_n, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].gt, h->st[c].mp,
-                                                             h->st[c].lnk, h->d_clist, h->d_cseg);
+                                                             h->st[c].lnk, h->d_clist, h->d_cseg, ci);
     }
     // one wavefront per segment (XCD bands: a multiple of 8 workgroups)
     const int ng = h->cseg_n >= 8 ? (std::min(h->cseg_n, kChemGrid) + 7) & ~7 : h->cseg_n;
     k_chem_decide<<<ng, 64, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_cs[c], h->st[c].xy, h->st[c].gt, h->st[c].mp,
-                                            h->st[c].lnk, h->d_clist, h->d_cseg, h->cseg_n, h->d_flag, h->d_best);
+                                            h->st[c].lnk, h->d_clist, h->d_cseg, h->cseg_n, h->d_flag, h->d_best, ci);
     k_chem_apply<<<ng, 64, 0, h->stream>>>(h->d_rs, h->st[c].gt, h->st[c].lnk, h->d_flag, h->d_best, h->d_clist,
                                            h->d_cseg, h->cseg_n, h->d_ev, h->d_evn, h->evcap, h->record, h->d_err);
 }
@@ -568,6 +584,8 @@ static int check_err(oc_engine *h) {
         if (e & ERR_CATCAP)
             m += " catalyst chemistry over capacity (more than 4096 active catalysts in one check of a replica, or "
                  "more beads within a reaction's reach than its gather list holds)";
+        if (e & ERR_INJ)
+            m += " a pair within the cutoff (or a formation candidate) is missing from the caller's table";
         if (e & ERR_DISP)
             m += " a bead moved 0.5 or more in the step before a chemistry check (the cell index of the last sort "
                  "no longer covers the reaction windows; reduce dt)";
@@ -1195,6 +1213,74 @@ OC_API int oc_chem(oc_engine *h, int64_t step) {
     if (!h || !h->have_state) return fail(OC_ESTATE, "no state");
     launch_step_params(h, 0, step);
     launch_chem(h);
+    HIPCHK(hipGetLastError());
+    return check_err(h);
+}
+
+// Sorted (lo << 32 | hi) keys of a caller's pair table, with its checks.
+static int pair_keys(const uint32_t *lo, const uint32_t *hi, int64_t n, std::vector<uint64_t> &k) {
+    if (n < 0 || (n > 0 && (!lo || !hi))) return fail(OC_EINVAL, "bad pair table");
+    k.resize((size_t)n);
+    for (int64_t q = 0; q < n; q++) {
+        if (lo[q] >= hi[q]) return fail(OC_EINVAL, "pair table: every entry needs lo < hi");
+        k[q] = ((uint64_t)lo[q] << 32) | hi[q];
+        if (q > 0 && k[q] <= k[q - 1]) return fail(OC_EINVAL, "pair table: keys (lo, hi) must strictly increase");
+    }
+    return 0;
+}
+
+// Device copies of a caller's tables, freed when the call ends.
+struct CallMem {
+    std::vector<void *> p;
+    ~CallMem() {
+        for (void *q : p) (void)hipFree(q);
+    }
+    template <class T>
+    int put(const T *src, size_t n, T **out) {
+        void *q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return fail(OC_ENOMEM, "hipMalloc failed");
+        p.push_back(q);
+        if (n && hipMemcpy(q, src, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(OC_EHIP, "copy of a caller table failed");
+        *out = (T *)q;
+        return 0;
+    }
+};
+
+OC_API int oc_forces_theta(oc_engine *h, int64_t step, int32_t with_bonded, const uint32_t *pair_lo,
+                           const uint32_t *pair_hi, const float *theta, int64_t npair) {
+    if (!h || !h->have_state) return fail(OC_ESTATE, "no state");
+    if (h->nrep != 1) return fail(OC_EINVAL, "oc_forces_theta needs an engine of one replica");
+    if (npair > 0 && !theta) return fail(OC_EINVAL, "null theta");
+    std::vector<uint64_t> k;
+    if (int rc = pair_keys(pair_lo, pair_hi, npair, k)) return rc;
+    CallMem m;
+    InjTheta inj{nullptr, nullptr, npair, h->d_err};
+    if (int rc = m.put(k.data(), k.size(), (uint64_t **)&inj.k)) return rc;
+    if (int rc = m.put(theta, (size_t)npair, (float **)&inj.v)) return rc;
+    launch_step_params(h, 0, step);
+    launch_forces(h, with_bonded != 0, &inj);
+    HIPCHK(hipGetLastError());
+    return check_err(h);
+}
+
+OC_API int oc_chem_uniforms(oc_engine *h, int64_t step, const double *u_conv, const double *u_break,
+                            const uint32_t *form_lo, const uint32_t *form_hi, const double *u_form, int64_t nform) {
+    if (!h || !h->have_state) return fail(OC_ESTATE, "no state");
+    if (h->nrep != 1) return fail(OC_EINVAL, "oc_chem_uniforms needs an engine of one replica");
+    if (!h->any_chem || h->any_rchem) return fail(OC_EINVAL, "oc_chem_uniforms: precursor chemistry only (chem on, rchem off)");
+    if (!u_conv || !u_break || (nform > 0 && !u_form)) return fail(OC_EINVAL, "null uniforms");
+    std::vector<uint64_t> k;
+    if (int rc = pair_keys(form_lo, form_hi, nform, k)) return rc;
+    const size_t N = (size_t)h->ri[0].N;
+    CallMem m;
+    ChemInj inj{nullptr, nullptr, nullptr, nullptr, nform, h->d_err};
+    if (int rc = m.put(u_conv, N, (double **)&inj.uconv)) return rc;
+    if (int rc = m.put(u_break, N, (double **)&inj.ubreak)) return rc;
+    if (int rc = m.put(k.data(), k.size(), (uint64_t **)&inj.fk)) return rc;
+    if (int rc = m.put(u_form, (size_t)nform, (double **)&inj.fv)) return rc;
+    launch_step_params(h, 0, step);
+    launch_pchem(h, false, &inj);
     HIPCHK(hipGetLastError());
     return check_err(h);
 }

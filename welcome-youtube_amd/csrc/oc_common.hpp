@@ -17,7 +17,8 @@ constexpr int kBeadBlock = 256;                   // beads per block of the per-
 constexpr int kRedChunk = 2048;                   // beads per partial of the stats reduction
 
 // sticky device error bits
-enum : uint32_t { ERR_EVOVF = 1u, ERR_CHAIN = 2u, ERR_SORT = 4u, ERR_TILE = 8u, ERR_CATCAP = 16u, ERR_DISP = 32u };
+enum : uint32_t { ERR_EVOVF = 1u, ERR_CHAIN = 2u, ERR_SORT = 4u, ERR_TILE = 8u, ERR_CATCAP = 16u, ERR_DISP = 32u,
+                  ERR_INJ = 64u };
 
 // Replica constants (one system of the ensemble).
 struct RepInfo {

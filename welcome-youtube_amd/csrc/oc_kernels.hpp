@@ -179,10 +179,23 @@ __global__ __launch_bounds__(kBeadBlock) void k_keys(const RepInfo *__restrict__
 // lipid, unbonded -- for a partner).  wp: the gid word of the chemistry
 // partner (read only for a bonded bead outside lipids).
 constexpr int kRoleShift = 30;
-__device__ __forceinline__ int chem_role(const RepInfo &R, const RepStep &S, uint32_t w, int m, int l, uint32_t wp) {
+// Caller-supplied draws of a check (oc_chem_uniforms, one replica): the
+// conversion and break uniforms per gid, the formation uniform per pair (keys
+// lo << 32 | hi, strictly increasing; a pair missing from it raises ERR_INJ
+// and does not bond).
+struct ChemInj {
+    const double *uconv, *ubreak;
+    const uint64_t *fk;
+    const double *fv;
+    int64_t nf;
+    uint32_t *err;
+};
+__device__ __forceinline__ int chem_role(const RepInfo &R, const RepStep &S, uint32_t w, int m, int l, uint32_t wp,
+                                         const ChemInj *inj = nullptr) {
     const uint32_t g = w & kGidMask, ty = w >> kGidBits;
-    const bool brk = m < 0 && l >= 0 && g < (wp & kGidMask) && hash_u01(S.kbreak, g) < R.p_break;
-    const bool conv = ty == OC_N && hash_u01(S.kconv, g) < R.p_conv;
+    const bool brk = m < 0 && l >= 0 && g < (wp & kGidMask) &&
+                     (inj ? inj->ubreak[g] : hash_u01(S.kbreak, g)) < R.p_break;
+    const bool conv = ty == OC_N && (inj ? inj->uconv[g] : hash_u01(S.kconv, g)) < R.p_conv;
     const bool form = ty == OC_T && m < 0 && l < 0;
     return (brk ? 1 : 0) | (conv || form ? 2 : 0);
 }
@@ -765,28 +778,58 @@ constexpr int kPairQueue = 256;
 
 constexpr float kTheta32 = (float)(3.4641016151377545870548926830117 / 4294967296.0);  // sqrt(12) / 2^32
 
+// Caller-supplied pair noise (oc_forces_theta): theta of the pair (min gid,
+// max gid) from a table of keys (lo << 32 | hi), strictly increasing; a pair
+// missing from the table gets theta = 0 and raises ERR_INJ.
+struct InjTheta {
+    const uint64_t *k;
+    const float *v;
+    int64_t n;
+    uint32_t *err;
+};
+constexpr float kInvTheta32 = (float)(4294967296.0 / 3.4641016151377545870548926830117);  // 2^32 / sqrt(12)
+
+__device__ __forceinline__ int64_t inj_find(const uint64_t *k, int64_t n, uint32_t gi, uint32_t gj) {
+    const uint64_t key = gi < gj ? (((uint64_t)gi << 32) | gj) : (((uint64_t)gj << 32) | gi);
+    int64_t lo = 0, hi = n;  // first entry >= key
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (k[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && k[lo] == key ? lo : -1;
+}
+
 // Force factor fm of an interacting pair (0 < r2 < 1): F_ij = fm * (dx, dy),
 // fm = w (a_ij - gamma w r.v + sigma/sqrt(dt) theta_ij) / r, w = 1 - r.
 // sdtk = sigma / sqrt(dt) * sqrt(12) / 2^32 (the draw scale folded in).
 __device__ __forceinline__ float pair_core(float dx, float dy, float r2, float vxi, float vyi, uint32_t gi,
                                            const float *arow, float2 vj, uint32_t gtj, float gam, float sdtk,
-                                           uint64_t sseed, float &w) {
+                                           uint64_t sseed, float &w, const InjTheta *inj = nullptr) {
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * ir;
     w = 1.0f - r;
     const float dvx = vxi - vj.x, dvy = vyi - vj.y;
     const float rv = __builtin_fmaf(dx, dvx, dy * dvy) * ir;
     const uint32_t gj = gtj & kGidMask;
-    const uint64_t h = pair_hash(sseed, gi, gj);
-    // theta on [-sqrt3, sqrt3) from the top 32 bits of the hash
-    const float th = (float)(int32_t)(uint32_t)(h >> 32);
+    float th;
+    if (inj) {  // the caller's theta, in the units of the hash word below
+        const int64_t q = inj_find(inj->k, inj->n, gi, gj);
+        if (q < 0) atomicOr(inj->err, ERR_INJ);
+        th = q < 0 ? 0.0f : inj->v[q] * kInvTheta32;
+    } else {
+        // theta on [-sqrt3, sqrt3) from the top 32 bits of the hash
+        const uint64_t h = pair_hash(sseed, gi, gj);
+        th = (float)(int32_t)(uint32_t)(h >> 32);
+    }
     const float a = arow[gtj >> kGidBits];
     return w * __builtin_fmaf(sdtk, th, __builtin_fmaf(-gam * w, rv, a)) * ir;
 }
 
 __device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float vyi, uint32_t gi, const float *arow,
                                          double2 pj, float2 vj, uint32_t gtj, double sxs, double sys, float gam,
-                                         float sdtk, uint64_t sseed, double &ax, double &ay, float2 &e, bool energy) {
+                                         float sdtk, uint64_t sseed, double &ax, double &ay, float2 &e, bool energy,
+                                         const InjTheta *inj = nullptr) {
     // FP64 image difference, then FP32 force arithmetic (fused multiply-adds).
     // Both orientations of a pair run the same operations on negated
     // operands, so f_ji = -f_ij exactly.
@@ -794,7 +837,7 @@ __device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float 
     const float r2 = __builtin_fmaf(dx, dx, dy * dy);
     if (r2 < 1.0f && r2 > 1e-24f) {
         float w;
-        const float fm = pair_core(dx, dy, r2, vxi, vyi, gi, arow, vj, gtj, gam, sdtk, sseed, w);
+        const float fm = pair_core(dx, dy, r2, vxi, vyi, gi, arow, vj, gtj, gam, sdtk, sseed, w, inj);
         ax += (double)(fm * dx);
         ay += (double)(fm * dy);
         if (energy) {
@@ -965,6 +1008,7 @@ struct PairArgs {
     const double *stir;
     double2 *f;
     double *epot, *evir;  // per-bead pair energy and conservative virial (ENERGY)
+    InjTheta inj;          // caller-supplied pair noise (INJ)
     uint32_t *err;
     // next step's cell keys (new_key of the integrated position) when key != 0
     const double2 *v;
@@ -981,8 +1025,9 @@ struct PairArgs {
 // rows per staged column (tile rows + 2 * margin + 1) of a pair tile
 __host__ __device__ constexpr int pair_max_rows(int block) { return 2 * block; }
 
-template <int BLOCK, int CAP, int CAPL, bool ENERGY, bool ANG>
+template <int BLOCK, int CAP, int CAPL, bool ENERGY, bool ANG, bool INJ = false>
 __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
+    const InjTheta *inj = INJ ? &A.inj : nullptr;  // caller-supplied pair noise (single pass only)
     static_assert(CAP <= 65536, "queue entries pack two staged indices in 16 bits each");
     constexpr int kMaxRows = pair_max_rows(BLOCK);
     __shared__ double2 s_xy[CAP];
@@ -1287,7 +1332,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                 } else if (CAPL != 1) {
                     for (int it = 0; it < nj; it++) {
                         pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], 0.0, 0.0, gam, sdtk, sseed,
-                                 ax, ay, e, ENERGY);
+                                 ax, ay, e, ENERGY, inj);
                         ++j;
                         j = j == en0 ? st1 : j;
                         j = j == en1 ? st2 : j;
@@ -1305,7 +1350,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                                 mk |= 1ull << it;
                             else
                                 pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, pj, VT[j], GT[j], 0.0, 0.0, gam, sdtk,
-                                         sseed, ax, ay, e, ENERGY);
+                                         sseed, ax, ay, e, ENERGY, inj);
                         }
                         ++j;
                         j = j == en0 ? st1 : j;
@@ -1317,7 +1362,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                         mk &= mk - 1;
                         const int jk = tk < c0 ? st0 + tk : (tk < c01 ? st1 + (tk - c0) : st2 + (tk - c01));
                         pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jk], VT[jk], GT[jk], 0.0, 0.0, gam, sdtk, sseed,
-                                 ax, ay, e, ENERGY);
+                                 ax, ay, e, ENERGY, inj);
                     }
                 }
             } else if (act) {
@@ -1337,7 +1382,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                         const int ja = s_row[c][pa - kb0] + adj, jb = s_row[c][pb + 1 - kb0] + adj;
                         for (int jj = ja; jj < jb; jj++)
                             pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jj], VT[jj], GT[jj], sx, sy, gam, sdtk,
-                                     sseed, ax, ay, e, ENERGY);
+                                     sseed, ax, ay, e, ENERGY, inj);
                     }
                 }
             }
@@ -1633,7 +1678,7 @@ __global__ __launch_bounds__(kBeadBlock) void k_chem_list(const RepInfo *__restr
                                                           const uint32_t *__restrict__ gt,
                                                           const int8_t *__restrict__ mp,
                                                           const int32_t *__restrict__ lnk, int32_t *__restrict__ list,
-                                                          ChemSeg *__restrict__ seg) {
+                                                          ChemSeg *__restrict__ seg, ChemInj inj) {
     __shared__ int s_n;
     const int r = bblk[blockIdx.x];
     const RepInfo &R = ri[r];
@@ -1646,7 +1691,7 @@ __global__ __launch_bounds__(kBeadBlock) void k_chem_list(const RepInfo *__restr
         const int s = R.off + loc;
         const uint32_t w = gt[s];
         const int m = mp[s], l = lnk[s];
-        const int role = chem_role(R, S, w, m, l, m < 0 && l >= 0 ? gt[l] : 0u);
+        const int role = chem_role(R, S, w, m, l, m < 0 && l >= 0 ? gt[l] : 0u, inj.uconv ? &inj : nullptr);
         if (role) list[base + atomicAdd(&s_n, 1)] = s | (role << kRoleShift);
     }
     __syncthreads();
@@ -1664,7 +1709,8 @@ __global__ __launch_bounds__(64) void k_chem_decide(const RepInfo *__restrict__ 
                                                     const uint32_t *__restrict__ gt, const int8_t *__restrict__ mp,
                                                     const int32_t *__restrict__ lnk, const int32_t *__restrict__ list,
                                                     const ChemSeg *__restrict__ seg, int nseg,
-                                                    uint8_t *__restrict__ flag, int32_t *__restrict__ best) {
+                                                    uint8_t *__restrict__ flag, int32_t *__restrict__ best,
+                                                    ChemInj inj) {
     const int gl = threadIdx.x & (kChemGroup - 1);
     constexpr int per = 64 / kChemGroup;
     // one wavefront per segment, kChemGroup lanes per entry; segments walked
@@ -1723,7 +1769,14 @@ __global__ __launch_bounds__(64) void k_chem_decide(const RepInfo *__restrict__ 
                 } else if (mc[i] < 0 && lc[i] < 0 && dist2(R, p, pc[i]) < R.rf2) {
                     // a free T within r_form; draws in [0,1) order as their bits
                     const uint32_t gg = gc[i] & kGidMask;
-                    const double u = pair_u01(S.kform, g, gg);
+                    double u;
+                    if (inj.fk) {  // the caller's formation uniform of the pair
+                        const int64_t q = inj_find(inj.fk, inj.nf, g, gg);
+                        if (q < 0) atomicOr(inj.err, ERR_INJ);
+                        u = q < 0 ? 2.0 : inj.fv[q];
+                    } else {
+                        u = pair_u01(S.kform, g, gg);
+                    }
                     const uint64_t ku = (uint64_t)__double_as_longlong(u);
                     if (u < R.p_form && (ku < bkey || (ku == bkey && gg < bg))) {
                         bkey = ku;

@@ -158,6 +158,31 @@ class Engine:
     def chem(self, step):
         self._chk(self.lib.oc_chem(self.h, int(step)))
 
+    def forces_theta(self, step, lo, hi, theta, bonded=True):
+        """Forces with the caller's pair noise: theta[q] of the gid pair
+        (lo[q] < hi[q]), entries in increasing (lo, hi) order (oc_forces_theta)."""
+        lo = np.ascontiguousarray(lo, dtype=np.uint32)
+        hi = np.ascontiguousarray(hi, dtype=np.uint32)
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        P = C.POINTER
+        self._chk(self.lib.oc_forces_theta(self.h, int(step), 1 if bonded else 0, lo.ctypes.data_as(P(C.c_uint32)),
+                                           hi.ctypes.data_as(P(C.c_uint32)), th.ctypes.data_as(P(C.c_float)),
+                                           len(th)))
+
+    def chem_uniforms(self, step, u_conv, u_break, lo, hi, u_form):
+        """One precursor-chemistry check with the caller's draws: per-gid
+        conversion and break uniforms, per-pair formation uniforms
+        (oc_chem_uniforms)."""
+        uc = np.ascontiguousarray(u_conv, dtype=np.float64)
+        ub = np.ascontiguousarray(u_break, dtype=np.float64)
+        lo = np.ascontiguousarray(lo, dtype=np.uint32)
+        hi = np.ascontiguousarray(hi, dtype=np.uint32)
+        uf = np.ascontiguousarray(u_form, dtype=np.float64)
+        P = C.POINTER
+        dp = lambda a: a.ctypes.data_as(P(C.c_double))
+        self._chk(self.lib.oc_chem_uniforms(self.h, int(step), dp(uc), dp(ub), lo.ctypes.data_as(P(C.c_uint32)),
+                                            hi.ctypes.data_as(P(C.c_uint32)), dp(uf), len(uf)))
+
     # ------------------------------------------------------------------ readouts
     def cells(self, r=0):
         n = C.c_int32()
