@@ -29,8 +29,12 @@ Fixtures (numpy .npz / JSON, small systems):
                     configurations (linear / pi / y / bola lipids, syn_nN,
                     syn_tries 1, feed cap, vent, nsub 2 / 3 / 4)
 
-`python tests/golden/make_golden.py templates` (or `rchem`, `catalysts`) writes
-that file only.
+  chemrates.json    the reference CPU engine's catalyst chemistry statistics
+                    in the two modes of cel/engine_gpu/tests/chemstat.sh (event
+                    totals, catalyst-checks, checks)
+
+`python tests/golden/make_golden.py templates` (or `rchem`, `catalysts`,
+`chemrates`) writes that file only.
 """
 import ctypes as C
 import json
@@ -211,6 +215,48 @@ def catalysts():
     np.savez_compressed(os.path.join(HERE, "catalysts.npz"), **out)
 
 
+# the reference's own chemistry statistics setup (cel/engine_gpu/tests/chemstat.sh):
+# well-mixed water with catalysts and nutrients, checks every 10 steps, two modes
+CHEMSTAT_BASE = dict(dt="0.03", L="48", f_lip="0", f_Ka="0.02", f_Kb="0.01", c_N="0.12", chem="1", n_chem="10",
+                     p_nd="0.001", feed="uniform", p_feed="0.01", feed_cap="0.12", steps="20000", snap_every="0",
+                     stats_every="100", log_every="0", ckpt_every="0", events="1", seed="11")
+CHEMSTAT_MODES = {
+    "syn": dict(p_syn_a="0.02", p_syn_b="0.06", p_rep_a="0", p_rep_b="0", p_kd="0", p_ld="0.02"),
+    "rep": dict(p_syn_a="0", p_syn_b="0", p_rep_a="0.005", p_rep_b="0.005", mu="0.3", p_kd="0.0025", p_ld="0"),
+}
+
+
+def chem_counts(events_csv, stats_csv, n_chem):
+    """Event totals per kind (feed: conversions), catalyst-checks (the mean
+    catalyst count of the stats rows times the checks) and checks, as
+    cel/engine_gpu/tests/compare.py:227-253 sums them."""
+    cnt = {}
+    for ln in events_csv.splitlines()[1:]:
+        f = ln.split(",")
+        cnt[f[1]] = cnt.get(f[1], 0) + (int(f[2]) if f[1] == "feed" else 1)
+    rows = [ln.split(",") for ln in stats_csv.splitlines()[1:]]
+    steps = max(int(r[0]) for r in rows)
+    nk = np.mean([int(r[7]) + int(r[8]) for r in rows])  # nKa + nKb
+    nc = steps // n_chem
+    return {"events": cnt, "catalyst_checks": float(nk * nc), "checks": nc}
+
+
+def chemrates():
+    """The reference CPU engine (its serial chemistry) in both chemstat modes:
+    event totals and catalyst-checks."""
+    out = {"base": CHEMSTAT_BASE, "modes": {}}
+    with tempfile.TemporaryDirectory() as tmp:
+        for mode, kw in CHEMSTAT_MODES.items():
+            ov = dict(CHEMSTAT_BASE, threads="4", **kw)
+            d = os.path.join(tmp, mode)
+            oracle.run_ref_cli(d, ov, fast=True, timeout=3000)
+            ev = open(os.path.join(d, "events.csv")).read()
+            st = open(os.path.join(d, "stats.csv")).read()
+            out["modes"][mode] = dict(chem_counts(ev, st, 10), overrides=kw)
+    with open(os.path.join(HERE, "chemrates.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 def main():
     if not oracle.ref_available():
         sys.exit("oracle/_ref is not built: make -C oracle ref")
@@ -291,13 +337,14 @@ def main():
     templates()
     rchem()
     catalysts()
+    chemrates()
     print("golden fixtures written to", HERE)
 
 
 if __name__ == "__main__":
-    if sys.argv[1:] in (["templates"], ["rchem"], ["catalysts"]):
+    if sys.argv[1:] in (["templates"], ["rchem"], ["catalysts"], ["chemrates"]):
         if not oracle.ref_available():
             sys.exit("oracle/_ref is not built: make -C oracle ref")
-        {"templates": templates, "rchem": rchem, "catalysts": catalysts}[sys.argv[1]]()
+        {"templates": templates, "rchem": rchem, "catalysts": catalysts, "chemrates": chemrates}[sys.argv[1]]()
     else:
         main()
