@@ -791,6 +791,10 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
         delete h;
         return fail(OC_EINVAL, "too many sort tiles for 32-bit indices");
     }
+    if (h->Ntot >= kPairSlotLimit) {  // the pair kernel packs a staged bead's slot in 28 bits
+        delete h;
+        return fail(OC_EINVAL, "more than 2^28 beads in one engine");
+    }
     h->evq.resize(nrep);
     h->evcap = std::min<int64_t>(std::max<int64_t>(1 << 16, 4 * h->Ntot), 8 << 20);
     if (const char *ec = getenv("OC_EVENT_CAP")) {  // a smaller event buffer (tests of the overflow path)

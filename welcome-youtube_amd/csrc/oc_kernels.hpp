@@ -1019,25 +1019,29 @@ struct PairArgs {
 
 // Full-shell pair forces of the i-beads of one tile (a column, rows [k0,k1)).
 // The candidate beads -- columns cx-1..cx+1, rows k0-M..k1-1+M -- are staged
-// in LDS once per tile; each lane then runs its own y-window per column, the
-// window expressions of cel/engine/oercel.c:1073-1074.  Forces are summed in
-// FP64 per bead, no atomics: the result is deterministic.
+// in LDS once per tile, row-major: staged row r holds the beads of its three
+// columns (column cx-1, cx, cx+1), each at the position of its periodic image
+// next to the tile.  A lane's y-window (the window expressions of
+// cel/engine/oercel.c:1073-1074) over the three columns is then one
+// contiguous LDS range.  Forces are summed in FP64 per bead, no atomics: the
+// result is deterministic.
 // rows per staged column (tile rows + 2 * margin + 1) of a pair tile
 __host__ __device__ constexpr int pair_max_rows(int block) { return 2 * block; }
+constexpr int64_t kPairSlotLimit = int64_t(1) << 28;  // slots the staging packs (bits 0-27)
 
 template <int BLOCK, int CAP, int CAPL, bool ENERGY, bool ANG, bool INJ = false>
 __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     const InjTheta *inj = INJ ? &A.inj : nullptr;  // caller-supplied pair noise (single pass only)
-    static_assert(CAP <= 65536, "queue entries pack two staged indices in 16 bits each");
+    static_assert(CAP < 65536, "staged indices are 16-bit");
     constexpr int kMaxRows = pair_max_rows(BLOCK);
-    __shared__ double2 s_xy[CAP];
+    static_assert(kMaxRows <= 2 * BLOCK, "two staged rows per thread");
+    __shared__ double2 s_xy[CAP];             // candidate position (its image next to the tile)
     __shared__ float2 s_vt[CAP];
-    __shared__ uint32_t s_gt[CAP];
-    __shared__ int32_t s_row[3][kMaxRows + 1];
+    __shared__ uint32_t s_gt[CAP];            // before the staging loads: the candidate's source (slot | column | image)
+    __shared__ uint16_t s_rs[kMaxRows + 1];   // LDS index of the first candidate of staged row r (+ the end)
+    __shared__ uint16_t s_ib[CAP];            // LDS index of the tile's q-th i-bead
     __shared__ float s_a[64];
-    __shared__ int32_t s_piece[3][3][3];  // [col][piece A/B/C] = {first slot, length, lds start}
-    __shared__ int32_t s_colbase[4];
-    __shared__ int32_t s_col[3];
+    __shared__ int32_t s_ws[BLOCK / 64][2];
 
     // XCD-aware tile order: consecutive workgroups go to different XCDs
     // round-robin, so give each XCD a contiguous run of tiles (neighbouring
@@ -1050,142 +1054,133 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     const int ncy = R.ncy, ncx = R.ncx;
     const int M = (int)ceil(R.icsy) + 1;  // staging margin in rows
     const int kb0 = T.k0 - M;             // first staged (unwrapped) row
-    const int nrow = T.k1 - T.k0 + 2 * M + 1;
+    const int nr = T.k1 - T.k0 + 2 * M;   // staged rows kb0 .. T.k1 - 1 + M
     const int32_t *cs = A.cs;
+    const double L = R.L;
+    const int tid = (int)threadIdx.x;
+    if (nr + 1 > kMaxRows) {  // the host sizes tiles so that this cannot happen
+        if (tid == 0) atomicOr(A.err, ERR_TILE);
+        return;
+    }
+    // the three columns (cell base, x image shift)
+    int colb[3];
+    double sxc[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        int col = T.cx - 1 + c;
+        sxc[c] = col < 0 ? -L : (col >= ncx ? L : 0.0);
+        col = col < 0 ? col + ncx : (col >= ncx ? col - ncx : col);
+        colb[c] = R.coff + col * ncy;
+    }
+    // the tile's i-beads: slots [slot0, slot0 + ni) of column cx
+    const int slot0 = cs[colb[1] + T.k0], ni = cs[colb[1] + T.k1] - slot0;
+    if (tid < 64) s_a[tid] = S.a_eff[tid];
 
     // The prologue costs two rounds of memory latency: the cell starts of the
-    // nine staged pieces, then every row offset and staged bead.
-    // Pieces: lane q < 9 of wavefront 0 takes column q / 3 (T.cx - 1 + q / 3)
-    // and piece q % 3 of its staged rows lo..hi: A = rows lo..-1 (wrapped to
-    // the top of the column), B = rows max(lo,0)..min(hi,ncy-1), C = rows
-    // ncy..hi (wrapped to the bottom); LDS holds them in q order.
-    if (threadIdx.x < 64) {
-        const float av = S.a_eff[threadIdx.x];
-        const int q = threadIdx.x;
-        const int lo = kb0, hi = T.k1 - 1 + M;  // staged rows, inclusive
-        int f = 0, n = 0;
-        if (q < 9) {
-            const int c = q / 3, pc = q - 3 * c;
-            int col = T.cx - 1 + c;
-            col = col < 0 ? col + ncx : (col >= ncx ? col - ncx : col);
-            const int cb = R.coff + col * ncy;
-            int ra = 0, rb = -1;  // rows of the piece within the column
-            if (pc == 0) {
-                if (lo < 0) { ra = lo + ncy; rb = ncy - 1; }
-            } else if (pc == 1) {
-                ra = max(lo, 0); rb = min(hi, ncy - 1);
-            } else if (hi >= ncy) {
-                ra = 0; rb = hi - ncy;
+    // staged rows (thread t: rows t and t + BLOCK, three columns), then the
+    // staged beads.  Row r's beads start at the exclusive prefix of the row
+    // totals (a block scan of both rows' totals).
+    int f[2][3], n[2][3], tot[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int r = tid + h * BLOCK;
+        const int k = kb0 + r;
+        const int w = k < 0 ? k + ncy : (k < ncy ? k : k - ncy);
+        tot[h] = 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            f[h][c] = 0;
+            n[h][c] = 0;
+            if (r < nr) {
+                f[h][c] = cs[colb[c] + w];
+                n[h][c] = cs[colb[c] + w + 1] - f[h][c];
             }
-            if (rb >= ra) {
-                f = cs[cb + ra];
-                n = cs[cb + rb + 1] - f;
-            }
-            if (pc == 0) s_col[c] = col;
+            tot[h] += n[h][c];
         }
-        int incl = n;  // inclusive prefix of the lengths over q
-        for (int o = 1; o < 16; o <<= 1) {
-            const int v = __shfl_up(incl, o, 64);
-            if (q >= o) incl += v;
-        }
-        if (q < 9) {
-            const int c = q / 3;
-            s_piece[c][q - 3 * c][0] = f;
-            s_piece[c][q - 3 * c][1] = n;
-            s_piece[c][q - 3 * c][2] = incl - n;
-            if (q == 3 * c) s_colbase[c] = incl - n;
-            if (q == 8) s_colbase[3] = incl;
-        }
-        s_a[q] = av;
+    }
+    const int lane = tid & 63, wv = tid >> 6;
+    const int inc0 = wave_incl_scan(tot[0]), inc1 = wave_incl_scan(tot[1]);
+    if (lane == 63) {
+        s_ws[wv][0] = inc0;
+        s_ws[wv][1] = inc1;
     }
     __syncthreads();
-    const int total = s_colbase[3];
-    const bool staged = total <= CAP && nrow <= kMaxRows;
-
-    // row offsets: s_row[c][k - kb0] = LDS (staged) or piece-relative index of
-    // the first bead of unwrapped row k of column c; staging of the pieces.
-    // Every load of a thread is issued before its first LDS store.
-    {
-        constexpr int kRowPer = (3 * (kMaxRows + 1) + BLOCK - 1) / BLOCK;
-        constexpr int kStgPer = (CAP + BLOCK - 1) / BLOCK;
-        const int nrow3 = nrow <= kMaxRows ? 3 * nrow : 0;
-        int pb[9], pf[9];
+    int off0 = 0, off1 = 0, sum0 = 0, sum1 = 0;
 #pragma unroll
-        for (int w = 0; w < 9; w++) {
-            pb[w] = s_piece[w / 3][w % 3][2];
-            pf[w] = s_piece[w / 3][w % 3][0];
-        }
-        int rcs[kRowPer];
+    for (int q = 0; q < BLOCK / 64; q++) {
+        const int a0 = s_ws[q][0], a1 = s_ws[q][1];
+        off0 += q < wv ? a0 : 0;
+        off1 += q < wv ? a1 : 0;
+        sum0 += a0;
+        sum1 += a1;
+    }
+    const int total = sum0 + sum1;
+    const bool staged = total <= CAP;
+    const int rs[2] = {off0 + inc0 - tot[0], sum0 + off1 + inc1 - tot[1]};
+    if (staged) {
+        // row starts; each candidate's source: slot (bits 0-27), column (28-29),
+        // y image (30-31: 0 none, 1 -L, 2 +L); the i-bead table
 #pragma unroll
-        for (int u = 0; u < kRowPer; u++) {
-            const int e = (int)threadIdx.x + u * BLOCK;
-            int idx = R.coff;  // a valid index for threads without an entry
-            if (e < nrow3) {
-                const int c = e >= 2 * nrow ? 2 : (e >= nrow ? 1 : 0);
-                const int k = kb0 + (e - c * nrow);
-                idx = R.coff + s_col[c] * ncy + (k < 0 ? k + ncy : (k < ncy ? k : k - ncy));
+        for (int h = 0; h < 2; h++) {
+            const int r = tid + h * BLOCK;
+            if (r >= nr) continue;
+            s_rs[r] = (uint16_t)rs[h];
+            const int k = kb0 + r;
+            const uint32_t syc = k < 0 ? 1u : (k < ncy ? 0u : 2u);
+            const bool irow = k >= T.k0 && k < T.k1;
+            int d = rs[h];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const uint32_t code = ((uint32_t)c << 28) | (syc << 30);
+                for (int b = 0; b < n[h][c]; b++) {
+                    s_gt[d + b] = (uint32_t)(f[h][c] + b) | code;
+                    if (c == 1 && irow) s_ib[f[h][c] + b - slot0] = (uint16_t)(d + b);
+                }
+                d += n[h][c];
             }
-            rcs[u] = cs[idx];
         }
+        if (tid == 0) s_rs[nr] = (uint16_t)total;
+    }
+    __syncthreads();
+    if (staged) {
+        // every load of a thread is issued before its first LDS store
+        constexpr int kStgPer = (CAP + BLOCK - 1) / BLOCK;
         double2 sx[kStgPer];
         float2 sv[kStgPer];
         uint32_t sg[kStgPer];
-        int sd[kStgPer];
 #pragma unroll
         for (int u = 0; u < kStgPer; u++) {
-            const int e = (int)threadIdx.x + u * BLOCK;
-            int src = R.off;  // a valid slot for threads without a bead
-            sd[u] = -1;
-            if (staged && e < total) {
-                int q = 0;  // the last piece starting at or before e (empty pieces skipped)
-#pragma unroll
-                for (int w = 1; w < 9; w++) q = e >= pb[w] ? w : q;
-                int fq = pf[0], bq = pb[0];
-#pragma unroll
-                for (int w = 1; w < 9; w++) {
-                    fq = q == w ? pf[w] : fq;
-                    bq = q == w ? pb[w] : bq;
-                }
-                src = fq + (e - bq);
-                sd[u] = e;
-            }
-            sx[u] = A.xy[src];
-            sv[u] = A.vt[src];
-            sg[u] = A.gt[src];
-        }
-#pragma unroll
-        for (int u = 0; u < kRowPer; u++) {
-            const int e = (int)threadIdx.x + u * BLOCK;
-            if (e < nrow3) {
-                const int c = e >= 2 * nrow ? 2 : (e >= nrow ? 1 : 0);
-                const int k = kb0 + (e - c * nrow);
-                const int pc = k < 0 ? 0 : (k < ncy ? 1 : 2);
-                s_row[c][e - c * nrow] = pb[3 * c + pc] + (rcs[u] - pf[3 * c + pc]);
+            if (u * BLOCK >= total) break;
+            const int e = tid + u * BLOCK;
+            if (e < total) {
+                const uint32_t code = s_gt[e];
+                const int src = (int)(code & 0x0fffffffu), c = (int)((code >> 28) & 3u), syc = (int)(code >> 30);
+                const double2 p = A.xy[src];
+                sv[u] = A.vt[src];
+                sg[u] = A.gt[src];
+                const double shx = c == 0 ? sxc[0] : (c == 1 ? 0.0 : sxc[2]);
+                const double shy = syc == 1 ? -L : (syc == 2 ? L : 0.0);
+                sx[u] = make_double2(p.x + shx, p.y + shy);
             }
         }
+        __syncthreads();  // the sources are read: s_gt takes the gid words
 #pragma unroll
-        for (int u = 0; u < kStgPer; u++)
-            if (sd[u] >= 0) {
-                s_xy[sd[u]] = sx[u];
-                s_vt[sd[u]] = sv[u];
-                s_gt[sd[u]] = sg[u];
+        for (int u = 0; u < kStgPer; u++) {
+            if (u * BLOCK >= total) break;
+            const int e = tid + u * BLOCK;
+            if (e < total) {
+                s_xy[e] = sx[u];
+                s_vt[e] = sv[u];
+                s_gt[e] = sg[u];
             }
-    }
-    __syncthreads();
-    if (nrow > kMaxRows) {  // the host sizes tiles so that this cannot happen
-        if (threadIdx.x == 0) atomicOr(A.err, ERR_TILE);
-        return;
+        }
+        __syncthreads();
     }
 
-    const double L = R.L;
-    const double sxc[3] = {T.cx == 0 ? -L : 0.0, 0.0, T.cx == ncx - 1 ? L : 0.0};
     const float gam = S.gamma, sdtk = S.sdt * kTheta32;
     const uint64_t sseed = S.sseed;
-    const int ibeg = s_row[1][T.k0 - kb0], iend = s_row[1][T.k1 - kb0];
-    const int slot0 = s_piece[1][1][0] + (ibeg - s_piece[1][1][2]);  // T.k0 lies in piece B of column 1
     // A tile whose candidates overflow the LDS buffer (a dense clump) runs the
-    // same loop on global memory; s_row offsets are then rebased per piece.
-    // Each lane owns one i-bead at a time.
+    // per-column loop on global memory.  Each lane owns one i-bead at a time.
     double mb = 0.0;  // largest bond length seen by this lane
     // per-bead epilogue: energy, or force (+ stirring, + bonded terms)
     auto finish = [&](int slot, double2 pi, uint32_t gti, const BondPre &B, double ax, double ay, float2 e,
@@ -1213,15 +1208,14 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     // LDS, each first an (i, j) pair of staged indices, then its force.
     __shared__ uint2 s_q[CAPL == 2 ? (BLOCK / 64) * kPairQueue : 1];
     auto run = [&](const double2 *XY, const float2 *VT, const uint32_t *GT, const bool stg) {
-        const int lane = (int)threadIdx.x & 63;
         // block-uniform bead loop: every lane of a wavefront takes part in the
         // wavefront-wide steps, lanes past the tile's last bead (act false)
         // with an empty window
-        for (int ib = ibeg; ib < iend; ib += BLOCK) {
-            const int ii = ib + (int)threadIdx.x;
-            const bool act = ii < iend;
-            const int slot = slot0 + ((act ? ii : ibeg) - ibeg);
-            const int is = stg ? (act ? ii : ibeg) : slot;
+        for (int ib = 0; ib < ni; ib += BLOCK) {
+            const int q = ib + tid;
+            const bool act = q < ni;
+            const int slot = slot0 + (act ? q : 0);
+            const int is = stg ? (int)s_ib[act ? q : 0] : slot;  // an i-bead for every lane (ni > 0)
             BondPre B;
             const bool bond = !ENERGY && A.bonded && act;
             if (bond) bond_links(slot, A.mp, A.lnk, A.prv, B);
@@ -1240,74 +1234,47 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             }
             double ax = 0.0, ay = 0.0;
             float2 e = make_float2(0.0f, 0.0f);
-            // Flat path (staged, no periodic image anywhere in the wavefront's
-            // windows): a column's window rows are one contiguous LDS range
-            // (its staged pieces are concatenated in unwrapped row order), and
-            // the lane runs through its three column ranges as one sequence
-            // (stepping past the end of one range lands on the next by two
-            // selects), so the wavefront iterates ~max over lanes of the whole
-            // window rather than the sum of per-column maxima.
-            //   CAPL == 1: two phases per lane: the cutoff test over the
-            // sequence marks the interacting candidates among the first 64 in
-            // a 64-bit register mask, then the force and noise arithmetic runs
-            // over the marked ones only (the wavefront still executes it ~max
-            // over lanes of the pair count).
-            //   CAPL == 2: the marked pairs of all 64 lanes go into one queue
-            // in lane order, the whole wavefront evaluates it 64 pairs at a
-            // time (~mean rather than ~max of the pair count), and each lane
-            // sums its own pairs' forces back in sequence order -- the same
-            // FP32 terms added in the same order as the single pass, so the
-            // forces are bit-identical to it.
-            const bool img = act && (sxc[0] != 0.0 || sxc[2] != 0.0 || klo < 0 || khi >= ncy);
-            if (stg && !__any(img)) {
-                const int st0 = s_row[0][klo - kb0], en0 = s_row[0][khi + 1 - kb0];
-                const int st1 = s_row[1][klo - kb0], en1 = s_row[1][khi + 1 - kb0];
-                const int st2 = s_row[2][klo - kb0], en2 = s_row[2][khi + 1 - kb0];
-                int j = st0 == en0 ? st1 : st0;
-                j = j == en1 ? st2 : j;
-                // the iteration count bounds the walk even for inconsistent
-                // cell starts (flagged)
-                const int nj = act ? max(en0 - st0, 0) + max(en1 - st1, 0) + max(en2 - st2, 0) : 0;
-                if (act && (st1 < en0 || st2 < en1)) atomicOr(A.err, ERR_TILE);
-                const int c0 = max(en0 - st0, 0), c01 = c0 + max(en1 - st1, 0);
+            if (stg) {
+                // the window: staged rows klo..khi of the three columns, one LDS range
+                const int st = s_rs[klo - kb0], en = s_rs[khi + 1 - kb0];
+                const int nj = act ? en - st : 0;
+                int j = st;
                 if (CAPL == 2 && !ENERGY && !__any(nj > 64)) {
-                    // phase 1: bit t of mk marks an interacting candidate t
+                    //   CAPL == 2: the marked pairs of all 64 lanes go into one
+                    // queue in lane order, the whole wavefront evaluates it 64
+                    // pairs at a time (~mean rather than ~max of the pair
+                    // count), and each lane sums its own pairs' forces back in
+                    // sequence order -- the same FP32 terms added in the same
+                    // order as the single pass, so the forces are bit-identical
+                    // to it.  Phase 1: bit t of mk marks an interacting candidate t.
                     uint64_t mk = 0;
-                    for (int it = 0; it < nj; it++) {
+                    for (int it = 0; it < nj; it++, j++) {
                         const double2 pj = XY[j];
                         const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
                         const float r2 = __builtin_fmaf(dx, dx, dy * dy);  // the test of pair_one
                         if (r2 < 1.0f && r2 > 1e-24f) mk |= 1ull << it;
-                        ++j;
-                        j = j == en0 ? st1 : j;
-                        j = j == en1 ? st2 : j;
                     }
                     // this lane's first entry in the wavefront's queue
                     const int cnt = __popcll(mk);
-                    int incl = cnt;
-                    for (int o = 1; o < 64; o <<= 1) {
-                        const int v = __shfl_up(incl, o, 64);
-                        if (lane >= o) incl += v;
-                    }
-                    const int off = incl - cnt, total = __shfl(incl, 63, 64);
-                    uint2 *qb = s_q + ((int)threadIdx.x >> 6) * kPairQueue;
+                    const int incl = wave_incl_scan(cnt);
+                    const int off = incl - cnt, qtot = __shfl(incl, 63, 64);
+                    uint2 *qb = s_q + (tid >> 6) * kPairQueue;
                     uint64_t rem = mk;
                     int pos = off;
                     // rounds of kPairQueue entries (wavefront-uniform)
-                    for (int w0 = 0; w0 < total; w0 += kPairQueue) {
-                        const int w1 = min(w0 + kPairQueue, total);
+                    for (int w0 = 0; w0 < qtot; w0 += kPairQueue) {
+                        const int w1 = min(w0 + kPairQueue, qtot);
                         // phase 2: this lane's pairs that fall in the round
                         while (rem != 0 && pos < w1) {
                             const int tk = __builtin_ctzll(rem);
                             rem &= rem - 1;
-                            const int jk = tk < c0 ? st0 + tk : (tk < c01 ? st1 + (tk - c0) : st2 + (tk - c01));
-                            qb[pos - w0] = make_uint2((uint32_t)ii | ((uint32_t)jk << 16), 0u);
+                            qb[pos - w0] = make_uint2((uint32_t)is | ((uint32_t)(st + tk) << 16), 0u);
                             ++pos;
                         }
                         __syncwarp();
                         // phase 3: the round's pairs, one per lane
-                        for (int q = lane; q < w1 - w0; q += 64) {
-                            const uint32_t u = qb[q].x;
+                        for (int q3 = (tid & 63); q3 < w1 - w0; q3 += 64) {
+                            const uint32_t u = qb[q3].x;
                             const int i1 = (int)(u & 0xffffu), j1 = (int)(u >> 16);
                             const double2 p1 = XY[i1], p2 = XY[j1];
                             const float2 v1 = VT[i1], v2 = VT[j1];
@@ -1317,7 +1284,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                             float w;
                             const float fm = pair_core(dx, dy, r2, v1.x, v1.y, g1 & kGidMask,
                                                        s_a + 8 * (g1 >> kGidBits), v2, g2, gam, sdtk, sseed, w);
-                            qb[q] = make_uint2(__float_as_uint(fm * dx), __float_as_uint(fm * dy));
+                            qb[q3] = make_uint2(__float_as_uint(fm * dx), __float_as_uint(fm * dy));
                         }
                         __syncwarp();
                         // phase 4: this lane's forces of the round, in sequence order
@@ -1330,18 +1297,18 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                         __syncwarp();
                     }
                 } else if (CAPL != 1) {
-                    for (int it = 0; it < nj; it++) {
+                    // the single pass
+                    for (; j < st + nj; j++)
                         pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], 0.0, 0.0, gam, sdtk, sseed,
                                  ax, ay, e, ENERGY, inj);
-                        ++j;
-                        j = j == en0 ? st1 : j;
-                        j = j == en1 ? st2 : j;
-                    }
                 } else {
-                    // pass 1: bit t of mk marks an interacting candidate t < 64
-                    // of the sequence (later ones are evaluated in place)
+                    //   CAPL == 1: two phases per lane: the cutoff test over
+                    // the window marks the interacting candidates among the
+                    // first 64 in a 64-bit register mask (later ones are
+                    // evaluated in place), then the force and noise arithmetic
+                    // runs over the marked ones only.
                     uint64_t mk = 0;
-                    for (int it = 0; it < nj; it++) {
+                    for (int it = 0; it < nj; it++, j++) {
                         const double2 pj = XY[j];
                         const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
                         const float r2 = __builtin_fmaf(dx, dx, dy * dy);  // the test of pair_one
@@ -1352,36 +1319,30 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                                 pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, pj, VT[j], GT[j], 0.0, 0.0, gam, sdtk,
                                          sseed, ax, ay, e, ENERGY, inj);
                         }
-                        ++j;
-                        j = j == en0 ? st1 : j;
-                        j = j == en1 ? st2 : j;
                     }
-                    // pass 2: the marked candidates in sequence order
+                    // pass 2: the marked candidates in window order
                     while (mk) {
-                        const int tk = __builtin_ctzll(mk);
+                        const int jk = st + __builtin_ctzll(mk);
                         mk &= mk - 1;
-                        const int jk = tk < c0 ? st0 + tk : (tk < c01 ? st1 + (tk - c0) : st2 + (tk - c01));
                         pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jk], VT[jk], GT[jk], 0.0, 0.0, gam, sdtk, sseed,
                                  ax, ay, e, ENERGY, inj);
                     }
                 }
             } else if (act) {
-                // general path: per column and periodic piece (image shifts), single pass
+                // global memory: per column and periodic piece (image shifts)
                 for (int c = 0; c < 3; c++) {
-                    // window split at the periodic seams: rows < 0 (image -L, staged piece 0),
-                    // 0..ncy-1 (piece 1), >= ncy (image +L, piece 2)
-                    for (int q = 0; q < 3; q++) {
-                        int pa, pb;
+                    // window split at the periodic seams: rows < 0 (image -L),
+                    // 0..ncy-1, >= ncy (image +L)
+                    for (int pq = 0; pq < 3; pq++) {
+                        int pa, pb, wrap;
                         double sy;
-                        if (q == 0) { pa = klo; pb = min(khi, -1); sy = -L; }
-                        else if (q == 1) { pa = max(klo, 0); pb = min(khi, ncy - 1); sy = 0.0; }
-                        else { pa = max(klo, ncy); pb = khi; sy = L; }
+                        if (pq == 0) { pa = klo; pb = min(khi, -1); sy = -L; wrap = ncy; }
+                        else if (pq == 1) { pa = max(klo, 0); pb = min(khi, ncy - 1); sy = 0.0; wrap = 0; }
+                        else { pa = max(klo, ncy); pb = khi; sy = L; wrap = -ncy; }
                         if (pa > pb) continue;
-                        const double sx = sxc[c];
-                        const int adj = stg ? 0 : s_piece[c][q][0] - s_piece[c][q][2];
-                        const int ja = s_row[c][pa - kb0] + adj, jb = s_row[c][pb + 1 - kb0] + adj;
+                        const int ja = cs[colb[c] + pa + wrap], jb = cs[colb[c] + pb + wrap + 1];
                         for (int jj = ja; jj < jb; jj++)
-                            pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jj], VT[jj], GT[jj], sx, sy, gam, sdtk,
+                            pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jj], VT[jj], GT[jj], sxc[c], sy, gam, sdtk,
                                      sseed, ax, ay, e, ENERGY, inj);
                     }
                 }
