@@ -1036,8 +1036,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     constexpr int kMaxRows = pair_max_rows(BLOCK);
     static_assert(kMaxRows <= 2 * BLOCK, "two staged rows per thread");
     __shared__ double2 s_xy[CAP];             // candidate position (its image next to the tile)
-    __shared__ float2 s_vt[CAP];
-    __shared__ uint32_t s_gt[CAP];            // before the staging loads: the candidate's source (slot | column | image)
+    __shared__ uint4 s_vg[CAP];               // predicted velocity (x, y bits) and gid word (z) of the candidate
+    // before the staging loads: each candidate's source (slot | column | image)
+    uint32_t *s_src = reinterpret_cast<uint32_t *>(s_vg);
     __shared__ uint16_t s_rs[kMaxRows + 1];   // LDS index of the first candidate of staged row r (+ the end)
     __shared__ uint16_t s_ib[CAP];            // LDS index of the tile's q-th i-bead
     __shared__ float s_a[64];
@@ -1133,7 +1134,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             for (int c = 0; c < 3; c++) {
                 const uint32_t code = ((uint32_t)c << 28) | (syc << 30);
                 for (int b = 0; b < n[h][c]; b++) {
-                    s_gt[d + b] = (uint32_t)(f[h][c] + b) | code;
+                    s_src[d + b] = (uint32_t)(f[h][c] + b) | code;
                     if (c == 1 && irow) s_ib[f[h][c] + b - slot0] = (uint16_t)(d + b);
                 }
                 d += n[h][c];
@@ -1153,7 +1154,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             if (u * BLOCK >= total) break;
             const int e = tid + u * BLOCK;
             if (e < total) {
-                const uint32_t code = s_gt[e];
+                const uint32_t code = s_src[e];
                 const int src = (int)(code & 0x0fffffffu), c = (int)((code >> 28) & 3u), syc = (int)(code >> 30);
                 const double2 p = A.xy[src];
                 sv[u] = A.vt[src];
@@ -1163,15 +1164,14 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                 sx[u] = make_double2(p.x + shx, p.y + shy);
             }
         }
-        __syncthreads();  // the sources are read: s_gt takes the gid words
+        __syncthreads();  // the sources are read: s_vg takes the candidates
 #pragma unroll
         for (int u = 0; u < kStgPer; u++) {
             if (u * BLOCK >= total) break;
             const int e = tid + u * BLOCK;
             if (e < total) {
                 s_xy[e] = sx[u];
-                s_vt[e] = sv[u];
-                s_gt[e] = sg[u];
+                s_vg[e] = make_uint4(__float_as_uint(sv[u].x), __float_as_uint(sv[u].y), sg[u], 0u);
             }
         }
         __syncthreads();
@@ -1207,7 +1207,19 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     // Cross-lane pair queue (CAPL == 2): per wavefront, kPairQueue entries of
     // LDS, each first an (i, j) pair of staged indices, then its force.
     __shared__ uint2 s_q[CAPL == 2 ? (BLOCK / 64) * kPairQueue : 1];
-    auto run = [&](const double2 *XY, const float2 *VT, const uint32_t *GT, const bool stg) {
+    // candidate j's predicted velocity and gid word: one 16-byte LDS read
+    // when staged
+    auto run = [&](const double2 *XY, const float2 *VT, const uint32_t *GT, const uint4 *VG, const bool stg) {
+        auto cand = [&](int j, float2 &v, uint32_t &g) {
+            if (stg) {
+                const uint4 w = VG[j];
+                v = make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
+                g = w.z;
+            } else {
+                v = VT[j];
+                g = GT[j];
+            }
+        };
         // block-uniform bead loop: every lane of a wavefront takes part in the
         // wavefront-wide steps, lanes past the tile's last bead (act false)
         // with an empty window
@@ -1221,8 +1233,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             if (bond) bond_links(slot, A.mp, A.lnk, A.prv, B);
             const double2 vh = !ENERGY && A.key && act ? A.v[slot] : make_double2(0.0, 0.0);
             const double2 pi = XY[is];
-            const float2 vi = VT[is];
-            const uint32_t gti = GT[is];
+            float2 vi;
+            uint32_t gti;
+            cand(is, vi, gti);
             const uint32_t gi = gti & kGidMask;
             const float *arow = s_a + 8 * (gti >> kGidBits);
             int khi = (int)((pi.y + 1.0) * R.icsy);
@@ -1277,8 +1290,10 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                             const uint32_t u = qb[q3].x;
                             const int i1 = (int)(u & 0xffffu), j1 = (int)(u >> 16);
                             const double2 p1 = XY[i1], p2 = XY[j1];
-                            const float2 v1 = VT[i1], v2 = VT[j1];
-                            const uint32_t g1 = GT[i1], g2 = GT[j1];
+                            float2 v1, v2;
+                            uint32_t g1, g2;
+                            cand(i1, v1, g1);
+                            cand(j1, v2, g2);
                             const float dx = (float)(p1.x - p2.x), dy = (float)(p1.y - p2.y);
                             const float r2 = __builtin_fmaf(dx, dx, dy * dy);
                             float w;
@@ -1298,9 +1313,13 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                     }
                 } else if (CAPL != 1) {
                     // the single pass
-                    for (; j < st + nj; j++)
-                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], VT[j], GT[j], 0.0, 0.0, gam, sdtk, sseed,
-                                 ax, ay, e, ENERGY, inj);
+                    for (; j < st + nj; j++) {
+                        float2 vj;
+                        uint32_t gj;
+                        cand(j, vj, gj);
+                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], vj, gj, 0.0, 0.0, gam, sdtk, sseed, ax, ay, e,
+                                 ENERGY, inj);
+                    }
                 } else {
                     //   CAPL == 1: two phases per lane: the cutoff test over
                     // the window marks the interacting candidates among the
@@ -1313,19 +1332,26 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                         const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
                         const float r2 = __builtin_fmaf(dx, dx, dy * dy);  // the test of pair_one
                         if (r2 < 1.0f && r2 > 1e-24f) {
-                            if (it < 64)
+                            if (it < 64) {
                                 mk |= 1ull << it;
-                            else
-                                pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, pj, VT[j], GT[j], 0.0, 0.0, gam, sdtk,
-                                         sseed, ax, ay, e, ENERGY, inj);
+                            } else {
+                                float2 vj;
+                                uint32_t gj;
+                                cand(j, vj, gj);
+                                pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, pj, vj, gj, 0.0, 0.0, gam, sdtk, sseed, ax,
+                                         ay, e, ENERGY, inj);
+                            }
                         }
                     }
                     // pass 2: the marked candidates in window order
                     while (mk) {
                         const int jk = st + __builtin_ctzll(mk);
                         mk &= mk - 1;
-                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jk], VT[jk], GT[jk], 0.0, 0.0, gam, sdtk, sseed,
-                                 ax, ay, e, ENERGY, inj);
+                        float2 vj;
+                        uint32_t gj;
+                        cand(jk, vj, gj);
+                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jk], vj, gj, 0.0, 0.0, gam, sdtk, sseed, ax, ay,
+                                 e, ENERGY, inj);
                     }
                 }
             } else if (act) {
@@ -1342,8 +1368,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                         if (pa > pb) continue;
                         const int ja = cs[colb[c] + pa + wrap], jb = cs[colb[c] + pb + wrap + 1];
                         for (int jj = ja; jj < jb; jj++)
-                            pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jj], VT[jj], GT[jj], sxc[c], sy, gam, sdtk,
-                                     sseed, ax, ay, e, ENERGY, inj);
+                            pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jj], VT[jj], GT[jj], sxc[c], sy, gam,
+                                     sdtk, sseed, ax, ay, e, ENERGY, inj);
                     }
                 }
             }
@@ -1353,8 +1379,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             }
         }
     };
-    if (staged) run(s_xy, s_vt, s_gt, true);
-    else run(A.xy, A.vt, A.gt, false);
+    if (staged) run(s_xy, nullptr, nullptr, s_vg, true);
+    else run(A.xy, A.vt, A.gt, nullptr, false);
     if (ENERGY || !A.bonded) return;
     // running max bond length of the replica: block max, then an atomic only
     // when it exceeds the stored value (positive doubles order as their bits)
