@@ -800,18 +800,21 @@ __device__ __forceinline__ int64_t inj_find(const uint64_t *k, int64_t n, uint32
     return lo < n && k[lo] == key ? lo : -1;
 }
 
-// Force factor fm of an interacting pair (0 < r2 < 1): F_ij = fm * (dx, dy),
-// fm = w (a_ij - gamma w r.v + sigma/sqrt(dt) theta_ij) / r, w = 1 - r.
-// sdtk = sigma / sqrt(dt) * sqrt(12) / 2^32 (the draw scale folded in).
-__device__ __forceinline__ float pair_core(float dx, float dy, float r2, float vxi, float vyi, uint32_t gi,
-                                           const float *arow, float2 vj, uint32_t gtj, float gam, float sdtk,
-                                           uint64_t sseed, float &w, const InjTheta *inj = nullptr) {
+// Force of an interacting pair (0 < r2 < 1); sdtk = sigma / sqrt(dt) *
+// sqrt(12) / 2^32 (the draw scale folded in).
+// Returns s with F_ij = s * (ux, uy), (ux, uy) = (dx, dy) / r: s = w (a_ij -
+// gamma w r.v + sigma/sqrt(dt) theta_ij), w = 1 - r.  gj: the candidate's
+// gid, a: a_ij.
+__device__ __forceinline__ float pair_core(float dx, float dy, float r2, float vxi, float vyi, uint32_t gi, float a,
+                                           float2 vj, uint32_t gj, float gam, float sdtk, uint64_t sseed, float &w,
+                                           float &ux, float &uy, const InjTheta *inj = nullptr) {
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * ir;
     w = 1.0f - r;
+    ux = dx * ir;
+    uy = dy * ir;
     const float dvx = vxi - vj.x, dvy = vyi - vj.y;
-    const float rv = __builtin_fmaf(dx, dvx, dy * dvy) * ir;
-    const uint32_t gj = gtj & kGidMask;
+    const float rv = __builtin_fmaf(ux, dvx, uy * dvy);
     float th;
     if (inj) {  // the caller's theta, in the units of the hash word below
         const int64_t q = inj_find(inj->k, inj->n, gi, gj);
@@ -822,12 +825,12 @@ __device__ __forceinline__ float pair_core(float dx, float dy, float r2, float v
         const uint64_t h = pair_hash(sseed, gi, gj);
         th = (float)(int32_t)(uint32_t)(h >> 32);
     }
-    const float a = arow[gtj >> kGidBits];
-    return w * __builtin_fmaf(sdtk, th, __builtin_fmaf(-gam * w, rv, a)) * ir;
+    return w * __builtin_fmaf(sdtk, th, __builtin_fmaf(-gam * w, rv, a));
 }
 
-__device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float vyi, uint32_t gi, const float *arow,
-                                         double2 pj, float2 vj, uint32_t gtj, double sxs, double sys, float gam,
+// gj: the candidate's gid, ap: its a_ij (read when the pair interacts)
+__device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float vyi, uint32_t gi, double2 pj,
+                                         float2 vj, uint32_t gj, const float *ap, double sxs, double sys, float gam,
                                          float sdtk, uint64_t sseed, double &ax, double &ay, float2 &e, bool energy,
                                          const InjTheta *inj = nullptr) {
     // FP64 image difference, then FP32 force arithmetic (fused multiply-adds).
@@ -836,14 +839,14 @@ __device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float 
     const float dx = (float)((xi - pj.x) - sxs), dy = (float)((yi - pj.y) - sys);
     const float r2 = __builtin_fmaf(dx, dx, dy * dy);
     if (r2 < 1.0f && r2 > 1e-24f) {
-        float w;
-        const float fm = pair_core(dx, dy, r2, vxi, vyi, gi, arow, vj, gtj, gam, sdtk, sseed, w, inj);
-        ax += (double)(fm * dx);
-        ay += (double)(fm * dy);
+        float w, ux, uy;
+        const float a = *ap;
+        const float fs = pair_core(dx, dy, r2, vxi, vyi, gi, a, vj, gj, gam, sdtk, sseed, w, ux, uy, inj);
+        ax += (double)(fs * ux);
+        ay += (double)(fs * uy);
         if (energy) {
             // e.x: half the pair's energy a/2 (1-r)^2; e.y: half its
             // conservative virial r . F^C = a (1-r) r (both orientations add)
-            const float a = arow[gtj >> kGidBits];
             e.x += 0.25f * a * w * w;
             e.y += 0.5f * a * w * (1.0f - w);
         }
@@ -1036,7 +1039,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     constexpr int kMaxRows = pair_max_rows(BLOCK);
     static_assert(kMaxRows <= 2 * BLOCK, "two staged rows per thread");
     __shared__ double2 s_xy[CAP];             // candidate position (its image next to the tile)
-    __shared__ uint4 s_vg[CAP];               // predicted velocity (x, y bits) and gid word (z) of the candidate
+    // the candidate's predicted velocity (x, y bits), gid (z) and the byte
+    // offset of its type in a row of the a_ij table (w)
+    __shared__ uint4 s_vg[CAP];
     // before the staging loads: each candidate's source (slot | column | image)
     uint32_t *s_src = reinterpret_cast<uint32_t *>(s_vg);
     __shared__ uint16_t s_rs[kMaxRows + 1];   // LDS index of the first candidate of staged row r (+ the end)
@@ -1171,7 +1176,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             const int e = tid + u * BLOCK;
             if (e < total) {
                 s_xy[e] = sx[u];
-                s_vg[e] = make_uint4(__float_as_uint(sv[u].x), __float_as_uint(sv[u].y), sg[u], 0u);
+                s_vg[e] = make_uint4(__float_as_uint(sv[u].x), __float_as_uint(sv[u].y), sg[u] & kGidMask,
+                                     (sg[u] >> kGidBits) * 4u);
             }
         }
         __syncthreads();
@@ -1210,14 +1216,18 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     // candidate j's predicted velocity and gid word: one 16-byte LDS read
     // when staged
     auto run = [&](const double2 *XY, const float2 *VT, const uint32_t *GT, const uint4 *VG, const bool stg) {
-        auto cand = [&](int j, float2 &v, uint32_t &g) {
+        // candidate j: velocity, gid and the address of a_ij in row arow
+        auto cand = [&](int j, const float *arow, float2 &v, uint32_t &g, const float *&ap) {
             if (stg) {
                 const uint4 w = VG[j];
                 v = make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
                 g = w.z;
+                ap = (const float *)((const char *)arow + w.w);
             } else {
+                const uint32_t gt = GT[j];
                 v = VT[j];
-                g = GT[j];
+                g = gt & kGidMask;
+                ap = arow + (gt >> kGidBits);
             }
         };
         // block-uniform bead loop: every lane of a wavefront takes part in the
@@ -1234,10 +1244,10 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             const double2 vh = !ENERGY && A.key && act ? A.v[slot] : make_double2(0.0, 0.0);
             const double2 pi = XY[is];
             float2 vi;
-            uint32_t gti;
-            cand(is, vi, gti);
-            const uint32_t gi = gti & kGidMask;
-            const float *arow = s_a + 8 * (gti >> kGidBits);
+            uint32_t gi;
+            const float *ai;  // s_a + bead i's type
+            cand(is, s_a, vi, gi, ai);
+            const float *arow = s_a + 8 * (ai - s_a);  // the a_ij row of i's type
             int khi = (int)((pi.y + 1.0) * R.icsy);
             int klo = (int)((pi.y - 1.0) * R.icsy + ncy) - ncy;
             if (klo < kb0 || khi > T.k1 - 1 + M) {  // cannot happen for rows k0..k1-1 (margin M); guard
@@ -1292,14 +1302,15 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                             const double2 p1 = XY[i1], p2 = XY[j1];
                             float2 v1, v2;
                             uint32_t g1, g2;
-                            cand(i1, v1, g1);
-                            cand(j1, v2, g2);
+                            const float *a1r, *ap;
+                            cand(i1, s_a, v1, g1, a1r);
+                            cand(j1, s_a + 8 * (a1r - s_a), v2, g2, ap);
                             const float dx = (float)(p1.x - p2.x), dy = (float)(p1.y - p2.y);
                             const float r2 = __builtin_fmaf(dx, dx, dy * dy);
-                            float w;
-                            const float fm = pair_core(dx, dy, r2, v1.x, v1.y, g1 & kGidMask,
-                                                       s_a + 8 * (g1 >> kGidBits), v2, g2, gam, sdtk, sseed, w);
-                            qb[q3] = make_uint2(__float_as_uint(fm * dx), __float_as_uint(fm * dy));
+                            float w, ux, uy;
+                            const float fs = pair_core(dx, dy, r2, v1.x, v1.y, g1, *ap, v2, g2, gam, sdtk, sseed, w, ux,
+                                                       uy);
+                            qb[q3] = make_uint2(__float_as_uint(fs * ux), __float_as_uint(fs * uy));
                         }
                         __syncwarp();
                         // phase 4: this lane's forces of the round, in sequence order
@@ -1316,8 +1327,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                     for (; j < st + nj; j++) {
                         float2 vj;
                         uint32_t gj;
-                        cand(j, vj, gj);
-                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[j], vj, gj, 0.0, 0.0, gam, sdtk, sseed, ax, ay, e,
+                        const float *ap;
+                        cand(j, arow, vj, gj, ap);
+                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, XY[j], vj, gj, ap, 0.0, 0.0, gam, sdtk, sseed, ax, ay, e,
                                  ENERGY, inj);
                     }
                 } else {
@@ -1337,8 +1349,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                             } else {
                                 float2 vj;
                                 uint32_t gj;
-                                cand(j, vj, gj);
-                                pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, pj, vj, gj, 0.0, 0.0, gam, sdtk, sseed, ax,
+                                const float *ap;
+                                cand(j, arow, vj, gj, ap);
+                                pair_one(pi.x, pi.y, vi.x, vi.y, gi, pj, vj, gj, ap, 0.0, 0.0, gam, sdtk, sseed, ax,
                                          ay, e, ENERGY, inj);
                             }
                         }
@@ -1349,9 +1362,10 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                         mk &= mk - 1;
                         float2 vj;
                         uint32_t gj;
-                        cand(jk, vj, gj);
-                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jk], vj, gj, 0.0, 0.0, gam, sdtk, sseed, ax, ay,
-                                 e, ENERGY, inj);
+                        const float *ap;
+                        cand(jk, arow, vj, gj, ap);
+                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, XY[jk], vj, gj, ap, 0.0, 0.0, gam, sdtk, sseed, ax, ay, e,
+                                 ENERGY, inj);
                     }
                 }
             } else if (act) {
@@ -1368,14 +1382,20 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                         if (pa > pb) continue;
                         const int ja = cs[colb[c] + pa + wrap], jb = cs[colb[c] + pb + wrap + 1];
                         for (int jj = ja; jj < jb; jj++)
-                            pair_one(pi.x, pi.y, vi.x, vi.y, gi, arow, XY[jj], VT[jj], GT[jj], sxc[c], sy, gam,
-                                     sdtk, sseed, ax, ay, e, ENERGY, inj);
+                        {
+                            float2 vj;
+                            uint32_t gj;
+                            const float *ap;
+                            cand(jj, arow, vj, gj, ap);
+                            pair_one(pi.x, pi.y, vi.x, vi.y, gi, XY[jj], vj, gj, ap, sxc[c], sy, gam, sdtk, sseed, ax,
+                                     ay, e, ENERGY, inj);
+                        }
                     }
                 }
             }
             if (act) {
                 if (bond) bond_partners(A.xy, A.gt, B);
-                finish(slot, pi, gti, B, ax, ay, e, vh);
+                finish(slot, pi, gi, B, ax, ay, e, vh);
             }
         }
     };
