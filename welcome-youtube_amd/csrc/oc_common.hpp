@@ -82,6 +82,9 @@ struct RepStep {
     uint64_t maxbond_bits;    // running max bond length (bits of a positive double)
 };
 
+static_assert(offsetof(RepStep, kcap) - offsetof(RepStep, sseed) == 12 * sizeof(uint64_t),
+              "the step keys sseed .. kcap are consecutive");
+
 struct Event {
     int64_t step;
     int32_t kind;

@@ -81,31 +81,36 @@ __device__ __forceinline__ void step_params_wave(const RepInfo &R, RepStep &S, i
         ae = 25.0 + (a - 25.0) * (fr < 0 ? 0 : fr);
     }
     S.a_eff[k] = (float)ae;
+    // the step's keys, one per lane: lane 1 the pair-noise key, lanes 2-13
+    // the keys sseed + 1 .. kcap of RepStep (consecutive fields)
+    if (k >= 1 && k <= 13) {
+        uint64_t salt = 0;
+        switch (k) {
+        case 2: salt = SALT_CONV; break;
+        case 3: salt = SALT_FORM; break;
+        case 4: salt = SALT_BREAK; break;
+        case 5: salt = 11; break;  // decay and feed keys, cel/engine/oercel.c:1664-1665, 1724
+        case 6: salt = 13; break;
+        case 7: salt = 14; break;
+        case 8: salt = 12; break;  // catalyst decay and reactions, feed cap: cel/engine_gpu/oercel_gpu.cu:1765-1772
+        case 9: salt = 21; break;
+        case 10: salt = 22; break;
+        case 11: salt = 23; break;
+        case 12: salt = 24; break;
+        default: salt = 25; break;
+        }
+        (&S.sseed)[k - 1] = k == 1 ? step_seed(R.seed, st) : hash_key(R.seed, st, salt);
+    }
     if (k != 0) return;
     S.step = st;
     const double t = R.t_anchor + (double)(st - R.step_anchor) * R.dt;
     S.t = t;
-    S.sseed = step_seed(R.seed, st);
-    S.kconv = hash_key(R.seed, st, SALT_CONV);
-    S.kform = hash_key(R.seed, st, SALT_FORM);
-    S.kbreak = hash_key(R.seed, st, SALT_BREAK);
     // kT_at, cel/engine/oercel.c:622-624; sdt as in pair_forces, :1204
     const double kTt = R.kT_amp == 0.0 ? R.kT : R.kT + R.kT_amp * sin(2.0 * M_PI * t / R.kT_period);
     S.sdt = (float)(sqrt(2.0 * R.gamma * kTt) / sqrt(R.dt));
     S.gamma = (float)R.gamma;
     S.chem_now = (R.chem && R.n_chem > 0 && (st % R.n_chem) == 0) ? 1 : 0;
     S.rchem_now = (R.rchem && R.n_chem > 0 && (st % R.n_chem) == 0) ? 1 : 0;
-    // decay and feed keys, cel/engine/oercel.c:1664-1665, 1724
-    S.kld = hash_key(R.seed, st, 11);
-    S.knd = hash_key(R.seed, st, 13);
-    S.kfeed = hash_key(R.seed, st, 14);
-    // catalyst decay and reactions, feed cap: cel/engine_gpu/oercel_gpu.cu:1765-1772
-    S.kkd = hash_key(R.seed, st, 12);
-    S.ksyn = hash_key(R.seed, st, 21);
-    S.krep = hash_key(R.seed, st, 22);
-    S.kmu = hash_key(R.seed, st, 23);
-    S.kpri = hash_key(R.seed, st, 24);
-    S.kcap = hash_key(R.seed, st, 25);
     S.disp = 0;
 }
 
