@@ -805,19 +805,14 @@ __device__ __forceinline__ int64_t inj_find(const uint64_t *k, int64_t n, uint32
     return lo < n && k[lo] == key ? lo : -1;
 }
 
-// Force of an interacting pair (0 < r2 < 1); sdtk = sigma / sqrt(dt) *
-// sqrt(12) / 2^32 (the draw scale folded in).
-// Returns s with F_ij = s * (ux, uy), (ux, uy) = (dx, dy) / r: s = w (a_ij -
-// gamma w r.v + sigma/sqrt(dt) theta_ij), w = 1 - r.  gj: the candidate's
-// gid, a: a_ij.
-__device__ __forceinline__ float pair_core(float dx, float dy, float r2, float vxi, float vyi, uint32_t gi, float a,
+// Force of an interacting pair (0 < r < 1) from its unit vector (ux, uy) =
+// (dx, dy) / r: F_ij = s * (ux, uy), s = w (a_ij - gamma w r.v + sigma/sqrt(dt)
+// theta_ij), w = 1 - r.  gj: the candidate's gid, a: a_ij; sdtk = sigma /
+// sqrt(dt) * sqrt(12) / 2^32 (the draw scale folded in).
+__device__ __forceinline__ float pair_core(float ux, float uy, float r, float vxi, float vyi, uint32_t gi, float a,
                                            float2 vj, uint32_t gj, float gam, float sdtk, uint64_t sseed, float &w,
-                                           float &ux, float &uy, const InjTheta *inj = nullptr) {
-    const float ir = __builtin_amdgcn_rsqf(r2);
-    const float r = r2 * ir;
+                                           const InjTheta *inj = nullptr) {
     w = 1.0f - r;
-    ux = dx * ir;
-    uy = dy * ir;
     const float dvx = vxi - vj.x, dvy = vyi - vj.y;
     const float rv = __builtin_fmaf(ux, dvx, uy * dvy);
     float th;
@@ -844,14 +839,53 @@ __device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float 
     const float dx = (float)((xi - pj.x) - sxs), dy = (float)((yi - pj.y) - sys);
     const float r2 = __builtin_fmaf(dx, dx, dy * dy);
     if (r2 < 1.0f && r2 > 1e-24f) {
-        float w, ux, uy;
+        const float ir = __builtin_amdgcn_rsqf(r2);
+        const float ux = dx * ir, uy = dy * ir;
+        float w;
         const float a = *ap;
-        const float fs = pair_core(dx, dy, r2, vxi, vyi, gi, a, vj, gj, gam, sdtk, sseed, w, ux, uy, inj);
+        const float fs = pair_core(ux, uy, r2 * ir, vxi, vyi, gi, a, vj, gj, gam, sdtk, sseed, w, inj);
         ax += (double)(fs * ux);
         ay += (double)(fs * uy);
         if (energy) {
             // e.x: half the pair's energy a/2 (1-r)^2; e.y: half its
             // conservative virial r . F^C = a (1-r) r (both orientations add)
+            e.x += 0.25f * a * w * w;
+            e.y += 0.5f * a * w * (1.0f - w);
+        }
+    }
+}
+
+// Staged (fixed-point) coordinates: the low 32 bits of round(x * 2^29) plus
+// the periodic image shift (a multiple of round(L * 2^29)), so a difference
+// of two staged coordinates is the exact integer difference for |dx| < 4 and
+// the two orientations of a pair negate exactly.  The FP32 cutoff test and
+// arithmetic run in units of 2^-29 (r2 scaled by 2^58: the same FP32 values,
+// scaled by a power of two).
+constexpr double kFixScale = 536870912.0;  // 2^29
+constexpr float kFixR = 0x1p-29f;           // grid unit
+constexpr float kFixR2Max = 0x1p58f;         // r2 = 1
+constexpr float kFixR2Min = 1e-24f * 0x1p58f;
+__device__ __forceinline__ uint32_t fix_lo(double v) {  // v >= 0
+    const double t = __builtin_rint(v * kFixScale);
+    return (uint32_t)(t - floor(t * 0x1p-32) * 4294967296.0);
+}
+
+// pair_one on staged coordinates: xq, qj the fixed-point coordinates of i and j
+__device__ __forceinline__ void pair_one_q(uint2 xq, float vxi, float vyi, uint32_t gi, uint2 qj, float2 vj,
+                                           uint32_t gj, const float *ap, float gam, float sdtk, uint64_t sseed,
+                                           double &ax, double &ay, float2 &e, bool energy,
+                                           const InjTheta *inj = nullptr) {
+    const float dx = (float)(int32_t)(xq.x - qj.x), dy = (float)(int32_t)(xq.y - qj.y);
+    const float r2 = __builtin_fmaf(dx, dx, dy * dy);
+    if (r2 < kFixR2Max && r2 > kFixR2Min) {
+        const float ir = __builtin_amdgcn_rsqf(r2);
+        const float ux = dx * ir, uy = dy * ir;
+        float w;
+        const float a = *ap;
+        const float fs = pair_core(ux, uy, (r2 * ir) * kFixR, vxi, vyi, gi, a, vj, gj, gam, sdtk, sseed, w, inj);
+        ax += (double)(fs * ux);
+        ay += (double)(fs * uy);
+        if (energy) {
             e.x += 0.25f * a * w * w;
             e.y += 0.5f * a * w * (1.0f - w);
         }
@@ -1043,7 +1077,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     static_assert(CAP < 65536, "staged indices are 16-bit");
     constexpr int kMaxRows = pair_max_rows(BLOCK);
     static_assert(kMaxRows <= 2 * BLOCK, "two staged rows per thread");
-    __shared__ double2 s_xy[CAP];             // candidate position (its image next to the tile)
+    __shared__ uint2 s_xq[CAP];               // candidate's fixed-point coordinates (its image next to the tile)
     // the candidate's predicted velocity (x, y bits), gid (z) and the byte
     // offset of its type in a row of the a_ij table (w)
     __shared__ uint4 s_vg[CAP];
@@ -1156,9 +1190,14 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     if (staged) {
         // every load of a thread is issued before its first LDS store
         constexpr int kStgPer = (CAP + BLOCK - 1) / BLOCK;
+        // the x / y image shifts in fixed point: columns 0 and 2, rows below 0 and past ncy - 1
+        const uint32_t fL = fix_lo(L);
+        const uint32_t fx0 = sxc[0] < 0.0 ? 0u - fL : (sxc[0] > 0.0 ? fL : 0u);
+        const uint32_t fx2 = sxc[2] < 0.0 ? 0u - fL : (sxc[2] > 0.0 ? fL : 0u);
         double2 sx[kStgPer];
         float2 sv[kStgPer];
         uint32_t sg[kStgPer];
+        uint32_t sc[kStgPer];
 #pragma unroll
         for (int u = 0; u < kStgPer; u++) {
             if (u * BLOCK >= total) break;
@@ -1166,12 +1205,10 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             if (e < total) {
                 const uint32_t code = s_src[e];
                 const int src = (int)(code & 0x0fffffffu), c = (int)((code >> 28) & 3u), syc = (int)(code >> 30);
-                const double2 p = A.xy[src];
+                sx[u] = A.xy[src];
                 sv[u] = A.vt[src];
                 sg[u] = A.gt[src];
-                const double shx = c == 0 ? sxc[0] : (c == 1 ? 0.0 : sxc[2]);
-                const double shy = syc == 1 ? -L : (syc == 2 ? L : 0.0);
-                sx[u] = make_double2(p.x + shx, p.y + shy);
+                sc[u] = (uint32_t)c | ((uint32_t)syc << 2);
             }
         }
         __syncthreads();  // the sources are read: s_vg takes the candidates
@@ -1180,7 +1217,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             if (u * BLOCK >= total) break;
             const int e = tid + u * BLOCK;
             if (e < total) {
-                s_xy[e] = sx[u];
+                const uint32_t c = sc[u] & 3u, syc = sc[u] >> 2;
+                s_xq[e] = make_uint2(fix_lo(sx[u].x) + (c == 0 ? fx0 : (c == 2 ? fx2 : 0u)),
+                                     fix_lo(sx[u].y) + (syc == 1 ? 0u - fL : (syc == 2 ? fL : 0u)));
                 s_vg[e] = make_uint4(__float_as_uint(sv[u].x), __float_as_uint(sv[u].y), sg[u] & kGidMask,
                                      (sg[u] >> kGidBits) * 4u);
             }
@@ -1220,7 +1259,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     __shared__ uint2 s_q[CAPL == 2 ? (BLOCK / 64) * kPairQueue : 1];
     // candidate j's predicted velocity and gid word: one 16-byte LDS read
     // when staged
-    auto run = [&](const double2 *XY, const float2 *VT, const uint32_t *GT, const uint4 *VG, const bool stg) {
+    auto run = [&](const uint2 *XQ, const float2 *VT, const uint32_t *GT, const uint4 *VG, const bool stg) {
+        const double2 *XY = A.xy;
         // candidate j: velocity, gid and the address of a_ij in row arow
         auto cand = [&](int j, const float *arow, float2 &v, uint32_t &g, const float *&ap) {
             if (stg) {
@@ -1247,7 +1287,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             const bool bond = !ENERGY && A.bonded && act;
             if (bond) bond_links(slot, A.mp, A.lnk, A.prv, B);
             const double2 vh = !ENERGY && A.key && act ? A.v[slot] : make_double2(0.0, 0.0);
-            const double2 pi = XY[is];
+            const double2 pi = XY[slot];
+            const uint2 xq = stg ? XQ[is] : make_uint2(0u, 0u);
             float2 vi;
             uint32_t gi;
             const float *ai;  // s_a + bead i's type
@@ -1277,10 +1318,10 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                     // to it.  Phase 1: bit t of mk marks an interacting candidate t.
                     uint64_t mk = 0;
                     for (int it = 0; it < nj; it++, j++) {
-                        const double2 pj = XY[j];
-                        const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
-                        const float r2 = __builtin_fmaf(dx, dx, dy * dy);  // the test of pair_one
-                        if (r2 < 1.0f && r2 > 1e-24f) mk |= 1ull << it;
+                        const uint2 qj = XQ[j];
+                        const float dx = (float)(int32_t)(xq.x - qj.x), dy = (float)(int32_t)(xq.y - qj.y);
+                        const float r2 = __builtin_fmaf(dx, dx, dy * dy);  // the test of pair_one_q
+                        if (r2 < kFixR2Max && r2 > kFixR2Min) mk |= 1ull << it;
                     }
                     // this lane's first entry in the wavefront's queue
                     const int cnt = __popcll(mk);
@@ -1304,17 +1345,19 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                         for (int q3 = (tid & 63); q3 < w1 - w0; q3 += 64) {
                             const uint32_t u = qb[q3].x;
                             const int i1 = (int)(u & 0xffffu), j1 = (int)(u >> 16);
-                            const double2 p1 = XY[i1], p2 = XY[j1];
+                            const uint2 p1 = XQ[i1], p2 = XQ[j1];
                             float2 v1, v2;
                             uint32_t g1, g2;
                             const float *a1r, *ap;
                             cand(i1, s_a, v1, g1, a1r);
                             cand(j1, s_a + 8 * (a1r - s_a), v2, g2, ap);
-                            const float dx = (float)(p1.x - p2.x), dy = (float)(p1.y - p2.y);
+                            const float dx = (float)(int32_t)(p1.x - p2.x), dy = (float)(int32_t)(p1.y - p2.y);
                             const float r2 = __builtin_fmaf(dx, dx, dy * dy);
-                            float w, ux, uy;
-                            const float fs = pair_core(dx, dy, r2, v1.x, v1.y, g1, *ap, v2, g2, gam, sdtk, sseed, w, ux,
-                                                       uy);
+                            const float ir = __builtin_amdgcn_rsqf(r2);
+                            const float ux = dx * ir, uy = dy * ir;
+                            float w;
+                            const float fs = pair_core(ux, uy, (r2 * ir) * kFixR, v1.x, v1.y, g1, *ap, v2, g2, gam,
+                                                       sdtk, sseed, w);
                             qb[q3] = make_uint2(__float_as_uint(fs * ux), __float_as_uint(fs * uy));
                         }
                         __syncwarp();
@@ -1334,8 +1377,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                         uint32_t gj;
                         const float *ap;
                         cand(j, arow, vj, gj, ap);
-                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, XY[j], vj, gj, ap, 0.0, 0.0, gam, sdtk, sseed, ax, ay, e,
-                                 ENERGY, inj);
+                        pair_one_q(xq, vi.x, vi.y, gi, XQ[j], vj, gj, ap, gam, sdtk, sseed, ax, ay, e, ENERGY, inj);
                     }
                 } else {
                     //   CAPL == 1: two phases per lane: the cutoff test over
@@ -1345,10 +1387,10 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                     // runs over the marked ones only.
                     uint64_t mk = 0;
                     for (int it = 0; it < nj; it++, j++) {
-                        const double2 pj = XY[j];
-                        const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
-                        const float r2 = __builtin_fmaf(dx, dx, dy * dy);  // the test of pair_one
-                        if (r2 < 1.0f && r2 > 1e-24f) {
+                        const uint2 qj = XQ[j];
+                        const float dx = (float)(int32_t)(xq.x - qj.x), dy = (float)(int32_t)(xq.y - qj.y);
+                        const float r2 = __builtin_fmaf(dx, dx, dy * dy);  // the test of pair_one_q
+                        if (r2 < kFixR2Max && r2 > kFixR2Min) {
                             if (it < 64) {
                                 mk |= 1ull << it;
                             } else {
@@ -1356,8 +1398,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                                 uint32_t gj;
                                 const float *ap;
                                 cand(j, arow, vj, gj, ap);
-                                pair_one(pi.x, pi.y, vi.x, vi.y, gi, pj, vj, gj, ap, 0.0, 0.0, gam, sdtk, sseed, ax,
-                                         ay, e, ENERGY, inj);
+                                pair_one_q(xq, vi.x, vi.y, gi, qj, vj, gj, ap, gam, sdtk, sseed, ax, ay, e, ENERGY,
+                                           inj);
                             }
                         }
                     }
@@ -1369,8 +1411,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                         uint32_t gj;
                         const float *ap;
                         cand(jk, arow, vj, gj, ap);
-                        pair_one(pi.x, pi.y, vi.x, vi.y, gi, XY[jk], vj, gj, ap, 0.0, 0.0, gam, sdtk, sseed, ax, ay, e,
-                                 ENERGY, inj);
+                        pair_one_q(xq, vi.x, vi.y, gi, XQ[jk], vj, gj, ap, gam, sdtk, sseed, ax, ay, e, ENERGY, inj);
                     }
                 }
             } else if (act) {
@@ -1404,8 +1445,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             }
         }
     };
-    if (staged) run(s_xy, nullptr, nullptr, s_vg, true);
-    else run(A.xy, A.vt, A.gt, nullptr, false);
+    if (staged) run(s_xq, nullptr, nullptr, s_vg, true);
+    else run(nullptr, A.vt, A.gt, nullptr, false);
     if (ENERGY || !A.bonded) return;
     // running max bond length of the replica: block max, then an atomic only
     // when it exceeds the stored value (positive doubles order as their bits)
