@@ -243,8 +243,8 @@ __device__ __forceinline__ int win_pieces(int c, int R, int n, int *a, int *b) {
 // into the new order.  No global atomics, no histogram to clear.
 constexpr int kSortRows = 256;                // rows per sort tile
 constexpr int kSortThr = 128;                 // threads per sort tile (two rows each)
-constexpr int kSortCand = 1024;               // staged candidates
-constexpr int kSortMv = 256;                  // listed beads that changed cells (more: ranked in place)
+constexpr int kSortCand = 768;                // staged candidates (~580 at rho 3, nsub 4; more: the general path)
+constexpr int kSortMv = 64;                   // listed beads that changed cells (~10; more: ranked in place)
 constexpr int kSortRowTab = kSortRows + 3;    // staged rows per column (+ end)
 constexpr int kSortCapLog = 10;
 constexpr int kSortCap = 1 << kSortCapLog;    // beads per sort tile
@@ -296,10 +296,13 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
                                                     int32_t *__restrict__ tsrc, int32_t *__restrict__ tslot,
                                                     uint32_t *__restrict__ err, int nt) {
     __shared__ int32_t s_key[kSortCand];
+    __shared__ int32_t s_slot[kSortCand];     // old slot of staged candidate j
+    __shared__ uint16_t s_cst[kSortRows];     // LDS index of the first staged bead of tile row r (old cell)
     __shared__ int32_t s_cs[3][kSortRowTab];  // cs_old of the staged rows (slot order); extras: piece ends
     __shared__ int32_t s_e[3][2];
     __shared__ int32_t s_pc[4][3];  // per staged column: LDS base, piece counts; [3][0]: total
-    __shared__ int32_t s_cnt[kSortRows], s_alow[kSortRows], s_start[kSortRows];
+    __shared__ int32_t s_cnt[kSortRows], s_alow[kSortRows];
+    __shared__ uint16_t s_start[kSortRows];  // < kSortCap
     __shared__ int32_t s_mv[kSortMv];  // staged ids of the tile's beads that changed cells
     __shared__ int s_nmv;
     __shared__ int s_tmp[2];
@@ -395,6 +398,7 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
                                  : (u + 1 == nr ? s_pc[q][0] + s_pc[q][1] + s_pc[q][2] : lds_row(q, u + 1));
         };
         auto u_of = [&](int row) { return row >= a0 && row <= b0 ? row - a0 : len0 + (row - a1); };
+        const int ra = 2 * t, rb = 2 * t + 1;  // this thread's rows (local)
         auto slot_of = [&](int j) {  // old slot of staged candidate j
             const int q = j < s_pc[1][0] ? 0 : (j < s_pc[2][0] ? 1 : 2);
             const int o = j - s_pc[q][0];
@@ -404,18 +408,26 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
             // keys in LDS order; slot of LDS index j from the six pieces
             constexpr int kPer = kSortCand / kSortThr;
             const int nper = (nst + kSortThr - 1) / kSortThr;
-            int v[kPer];
+            int v[kPer], sl[kPer];
     #pragma unroll
             for (int i = 0; i < kPer; i++)
                 if (i < nper) {
                     const int j = t + i * kSortThr;
-                    if (j < nst) v[i] = key[slot_of(j)];
+                    if (j < nst) {
+                        sl[i] = slot_of(j);
+                        v[i] = key[sl[i]];
+                    }
                 }
+            if (ra < nrows) s_cst[ra] = (uint16_t)lds_row(qo, u_of(T.k0 + ra));
+            if (rb < nrows) s_cst[rb] = (uint16_t)lds_row(qo, u_of(T.k0 + rb));
     #pragma unroll
             for (int i = 0; i < kPer; i++)
                 if (i < nper) {
                     const int j = t + i * kSortThr;
-                    if (j < nst) s_key[j] = v[i];
+                    if (j < nst) {
+                        s_key[j] = v[i];
+                        s_slot[j] = sl[i];
+                    }
                 }
         }
         __syncthreads();
@@ -444,9 +456,8 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
             return rank;
         };
         int total;
-        const int ra = 2 * t, rb = 2 * t + 1;  // this thread's rows (local)
         if (fast) {
-            auto cell_start = [&](int r) { return lds_row(qo, u_of(T.k0 + r)); };
+            auto cell_start = [&](int r) { return (int)s_cst[r]; };
             for (int j = t; j < nst; j += kSortThr) {
                 const int k = s_key[j];
                 const int r = (k & 0x7fffffff) - c0;
@@ -481,18 +492,18 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
                 if (k < 0) {  // changed cells: ranked below, one lane per bead
                     const int m = atomicAdd(&s_nmv, 1);
                     if (m < kSortMv) s_mv[m] = j;
-                    else put(s_start[r] + mover_rank(j), slot_of(j));  // list full: rank in place
+                    else put(s_start[r] + mover_rank(j), s_slot[j]);  // list full: rank in place
                     continue;
                 }
                 const int o = cell_start(r);
                 int d = 0;
                 for (int jj = o; jj < j; jj++) d += s_key[jj] < 0 ? 1 : 0;
-                put(s_start[r] + s_alow[r] + (j - o) - d, slot_of(j));
+                put(s_start[r] + s_alow[r] + (j - o) - d, s_slot[j]);
             }
             __syncthreads();
             for (int m = t; m < min(s_nmv, kSortMv); m += kSortThr) {
                 const int j = s_mv[m];
-                put(s_start[(s_key[j] & 0x7fffffff) - c0] + mover_rank(j), slot_of(j));
+                put(s_start[(s_key[j] & 0x7fffffff) - c0] + mover_rank(j), s_slot[j]);
             }
         } else {
             // general path: old cells within (Rx, Ry) of each row's cell, from global memory
