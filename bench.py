@@ -545,11 +545,16 @@ def main():
     e = eng.Engine(p, pair_block=a.pair_block, tile_beads=a.tile_beads, pair_variant=a.pair_variant)
     e.start()
     # untimed equilibration from the random soup (no warm-up ramp), so the
-    # timed steps run at the steady state (kT ~ 1), then the user's warm-up
-    e.step(a.equil, graph=not a.no_graph)
+    # timed steps run at the steady state (kT ~ 1), then the user's warm-up.
+    # The event drain and the temperature read (host work, the device idle)
+    # come before the equilibration's last 100 steps, which run in one call
+    # with the warm-up: the device is busy without a pause up to the timed
+    # steps (a pause lets the clocks drop for the first milliseconds after it)
+    run_in = min(100, a.equil)
+    e.step(a.equil - run_in, graph=not a.no_graph)
     e.events()  # drain
-    st0 = e.stats()  # T_kin at the start of the warm-up steps
-    e.step(a.warmup, graph=not a.no_graph)
+    st0 = e.stats()  # T_kin 100 steps before the warm-up steps
+    e.step(run_in + a.warmup, graph=not a.no_graph)
     s0 = e.step_count()  # device synchronised; events of steps > s0 are the timed region's
     log(f"plain N={N}: {a.equil} + {a.warmup} untimed steps, T_kin {st0['T_kin']:.4f}")
     t0 = time.perf_counter()
