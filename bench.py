@@ -616,7 +616,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(N),
             "avg_launch_us": pair_s * 1e6, "algorithmic_bytes_per_launch": bytes_alg,
             "layout_bytes_per_launch": bytes_layout,
-            "limiter": "VALU issue (hash and force arithmetic per candidate pair), not HBM",
+            "limiter": "issue and latency of the per-candidate FP32 pair arithmetic and hash on LDS-staged "
+                       "candidates, not HBM",
         },
         "cpu_baseline": cpu,
     }
