@@ -65,11 +65,10 @@ struct oc_engine {
     bool keys_fresh = false;     // d_key holds the keys of the current state (written by the force kernel)
     double *d_stir = nullptr, *d_epot = nullptr, *d_evir = nullptr, *d_ebond = nullptr, *d_part = nullptr,
            *d_statout = nullptr;
-    uint8_t *d_flag = nullptr;
-    int32_t *d_best = nullptr;
+    uint8_t *d_ccls = nullptr;  // chemistry class byte of every bead at a check
     // precursor chemistry: the check's list of beads with a role, in segments
     // (one per placement tile, or per bead block of k_chem_list)
-    int32_t *d_clist = nullptr;
+    ChemEnt *d_cinl = nullptr, *d_cspill = nullptr;  // the check's list: inline entries, spill by slot
     ChemSeg *d_cseg = nullptr;
     int cseg_n = 0;  // segments of the last list
     Event *d_ev = nullptr;
@@ -335,6 +334,11 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false, const In
     A.rmax = h->d_rmax;
     A.moved = h->d_moved;
     if (!energy) h->keys_fresh = !h->nokeys;
+    A.lnkw = h->st[c].lnk;
+    A.ev = h->d_ev;
+    A.evn = h->d_evn;
+    A.evcap = h->evcap;
+    A.record = h->record;
     const dim3 grid(8 * h->per_xcd);
     // CAP (staged candidates) is 3.5x the block; the variant picks the single
     // pass, the per-lane two phases or the cross-lane pair queue
@@ -460,23 +464,32 @@ static void launch_rchem(oc_engine *h, bool list) {
                                               h->record, h->d_err);
 }
 
-// the precursor chemistry (after the sort, on the fresh cell list)
-static void launch_pchem(oc_engine *h, bool listed, const ChemInj *inj = nullptr) {
+// The check's list of the engine's buffers.
+static ChemList chem_list(oc_engine *h) { return ChemList{h->d_cseg, h->d_cinl, h->d_cspill}; }
+
+// the precursor chemistry (after the sort, on the fresh cell list): one pass
+// decides and applies the check; a formation proposal waits in the proposer's
+// link word for the force pass of the step (settled: settle = false), or is
+// settled by k_chem_resolve (a check on its own).
+static void launch_pchem(oc_engine *h, bool listed, const ChemInj *inj = nullptr, bool settle = true) {
     if (!h->any_chem) return;
     const int c = h->cur;
     const ChemInj ci = inj ? *inj : ChemInj{nullptr, nullptr, nullptr, nullptr, 0, nullptr};
+    const ChemList CL = chem_list(h);
     if (!listed) {
-        // a state that was not just placed: list the roles of every bead
+        // a state that was not just placed: list the roles (and classes) of every bead
         h->cseg_n = (int)h->bblk.size();
-        k_chem_list<<<h->cseg_n, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].gt, h->st[c].mp,
-                                                             h->st[c].lnk, h->d_clist, h->d_cseg, ci);
+        k_chem_list<<<h->cseg_n, kBeadBlock, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_bblk, h->st[c].xy, h->st[c].gt,
+                                                             h->st[c].mp, h->st[c].lnk, CL, h->d_ccls, ci);
     }
     // one wavefront per segment (XCD bands: a multiple of 8 workgroups)
     const int ng = h->cseg_n >= 8 ? (std::min(h->cseg_n, kChemGrid) + 7) & ~7 : h->cseg_n;
-    k_chem_decide<<<ng, 64, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_cs[c], h->st[c].xy, h->st[c].gt, h->st[c].mp,
-                                            h->st[c].lnk, h->d_clist, h->d_cseg, h->cseg_n, h->d_flag, h->d_best, ci);
-    k_chem_apply<<<ng, 64, 0, h->stream>>>(h->d_rs, h->st[c].gt, h->st[c].lnk, h->d_flag, h->d_best, h->d_clist,
-                                           h->d_cseg, h->cseg_n, h->d_ev, h->d_evn, h->evcap, h->record, h->d_err);
+    k_chem_check<<<ng, 64, 0, h->stream>>>(h->d_ri, h->d_rs, h->d_cs[c], h->st[c].xy, h->st[c].gt, h->st[c].lnk,
+                                           h->d_ccls, CL, h->cseg_n, h->d_ev, h->d_evn, h->evcap, h->record,
+                                           h->d_err, ci);
+    if (settle)
+        k_chem_resolve<<<ng, 64, 0, h->stream>>>(h->d_rs, h->st[c].gt, h->st[c].lnk, CL, h->cseg_n, h->d_ev,
+                                                 h->d_evn, h->evcap, h->record, h->d_err);
 }
 
 // one chemistry check on the current (sorted) state; the reference
@@ -519,18 +532,20 @@ static void launch_sort(oc_engine *h, bool integ, int step_delta, bool keys_read
     if (nch > 1) k_tsum<<<nch, 1024, 0, h->stream>>>(h->d_tcount, nt, per, h->d_tpart);
     k_tscan<<<nch, 1024, 0, h->stream>>>(h->d_ri, h->d_rs, h->nrep, h->d_tcount, nt, per,
                                          nch > 1 ? h->d_tpart : nullptr, h->d_toff, h->d_moved, h->d_rmax, step_delta);
-    // the placement lists the precursor chemistry's roles of the replicas whose check it is
-    int32_t *cl = h->any_chem ? h->d_clist : nullptr;
-    if (cl) h->cseg_n = nt;
+    // the placement lists the precursor chemistry's roles of the replicas whose
+    // check it is, and clears every bead's decision flag
+    const ChemList CL = h->any_chem ? chem_list(h) : ChemList{nullptr, nullptr, nullptr};
+    uint8_t *cf = h->any_chem ? h->d_ccls : nullptr;
+    if (h->any_chem) h->cseg_n = nt;
     int32_t *src = h->rec_src;
     if (integ)
         k_place<true><<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,
                                                        h->d_tsrc, h->d_tslot, h->d_cs[n], src, h->st[c],
-                                                       h->st[n], h->d_f, h->d_err, nt, h->d_rs, cl, h->d_cseg);
+                                                       h->st[n], h->d_f, h->d_err, nt, h->d_rs, CL, cf);
     else
         k_place<false><<<ng, kSortThr, 0, h->stream>>>(h->d_ri, h->d_stiles, h->d_tcount, h->d_toff, h->d_locst,
                                                         h->d_tsrc, h->d_tslot, h->d_cs[n], src, h->st[c],
-                                                        h->st[n], h->d_f, h->d_err, nt, h->d_rs, cl, h->d_cseg);
+                                                        h->st[n], h->d_f, h->d_err, nt, h->d_rs, CL, cf);
     h->cur = n;
     h->keys_fresh = false;
 }
@@ -557,11 +572,12 @@ static void enqueue_step(oc_engine *h, bool chem = true) {
                                                                     h->d_catn);
         if (chem) launch_rchem(h, false);
         launch_sort(h, false, 0, true);
-        if (chem) launch_pchem(h, true);
+        if (chem) launch_pchem(h, true, nullptr, false);
     } else {
         launch_sort(h, true, 1);
-        launch_pchem(h, true);
+        launch_pchem(h, true, nullptr, false);
     }
+    // the force pass settles the check's formation proposals
     launch_forces(h, true);
 }
 
@@ -834,10 +850,13 @@ OC_API int oc_create(const oc_params *p, const int32_t *N, int32_t nrep, oc_engi
     rc = rc ? rc : dalloc(h, &h->d_ebond, Nt);
     rc = rc ? rc : dalloc(h, &h->d_part, h->rblk.size() * 16);
     rc = rc ? rc : dalloc(h, &h->d_statout, (size_t)nrep * 16);
-    rc = rc ? rc : dalloc(h, &h->d_flag, Nt);
-    rc = rc ? rc : dalloc(h, &h->d_best, Nt);
-    rc = rc ? rc : dalloc(h, &h->d_clist, h->any_chem ? Nt : 1);
-    rc = rc ? rc : dalloc(h, &h->d_cseg, h->any_chem ? std::max(h->stiles.size(), h->bblk.size()) : 1);
+    rc = rc ? rc : dalloc(h, &h->d_ccls, h->any_chem ? Nt : 1);
+    {
+        const size_t nseg = h->any_chem ? std::max(h->stiles.size(), h->bblk.size()) : 1;
+        rc = rc ? rc : dalloc(h, &h->d_cseg, nseg);
+        rc = rc ? rc : dalloc(h, &h->d_cinl, nseg * kChemInl);
+        rc = rc ? rc : dalloc(h, &h->d_cspill, h->any_chem ? Nt : 1);
+    }
     rc = rc ? rc : dalloc(h, &h->d_ev, (size_t)h->evcap);
     rc = rc ? rc : dalloc(h, &h->d_evn, 1);
     rc = rc ? rc : dalloc(h, &h->d_err, 1);

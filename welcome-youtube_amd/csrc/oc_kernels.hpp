@@ -66,6 +66,20 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
+// append an event (a dropped one when the buffer is full raises ERR_EVOVF)
+__device__ __forceinline__ void push_event(Event *ev, unsigned long long *evn, int64_t cap, uint32_t *err, int64_t step,
+                                           int kind, uint32_t a, uint32_t b, int rep, double x = 0.0,
+                                           double y = 0.0) {
+    const unsigned long long k = atomicAdd(evn, 1ull);
+    if ((int64_t)k < cap) {
+        Event e;
+        e.step = step; e.kind = kind; e.gid_a = a; e.gid_b = b; e.rep = rep; e.x = x; e.y = y;
+        ev[k] = e;
+    } else {
+        atomicOr(err, ERR_EVOVF);
+    }
+}
+
 // --------------------------------------------------------------------------- per-step parameters
 
 // Per-step values of replica r, computed by one wavefront (lane k: a_eff[k]).
@@ -123,6 +137,15 @@ __global__ __launch_bounds__(64) void k_step_params(const RepInfo *__restrict__ 
 
 // --------------------------------------------------------------------------- sort
 
+// Sort tile k of a column split into ntc tiles: rows [k ncy / ntc, (k + 1)
+// ncy / ntc) hold row.
+__device__ __forceinline__ int sort_tile_row(const RepInfo &R, int row) {
+    int k = (int)(((int64_t)row * R.ntc) / R.ncy);
+    while (k + 1 < R.ntc && ((int64_t)(k + 1) * R.ncy) / R.ntc <= row) k++;
+    while (k > 0 && ((int64_t)k * R.ncy) / R.ntc > row) k--;
+    return k;
+}
+
 // New cell key of every bead (of its integrated position when INTEG; bit 31
 // set when the bead changed cells).  A bead that moved by more than one cell
 // (a rare event) sets *moved and raises its replica's largest periodic cell
@@ -145,11 +168,7 @@ __device__ __forceinline__ void new_key_from(const RepInfo &R, int r, int s, int
         atomicMax(&rmax[r].x, dx);
         atomicMax(&rmax[r].y, dy);
         atomicOr(moved, 1u);
-        // the tile of column cx whose rows [k ncy / ntc, (k + 1) ncy / ntc) hold cy
-        int k = (int)(((int64_t)cy * R.ntc) / R.ncy);
-        while (k + 1 < R.ntc && ((int64_t)(k + 1) * R.ncy) / R.ntc <= cy) k++;
-        while (k > 0 && ((int64_t)k * R.ncy) / R.ntc > cy) k--;
-        atomicOr(&moved[1 + R.stile0 + cx * R.ntc + k], 1u);
+        atomicOr(&moved[1 + R.stile0 + cx * R.ntc + sort_tile_row(R, cy)], 1u);
     }
 }
 __device__ __forceinline__ void new_key(const RepInfo &R, int r, int s, double2 p, double2 q, int32_t *key,
@@ -205,14 +224,92 @@ __device__ __forceinline__ int chem_role(const RepInfo &R, const RepStep &S, uin
     return (brk ? 1 : 0) | (conv || form ? 2 : 0);
 }
 
+// Class byte of every bead at a check (written with the state of the check's
+// start: by the placement, or by k_chem_list for a loaded state): a lipid tail
+// (type T in a lipid) or a free T (type T, in no lipid, unbonded) -- the only
+// candidates a search reads further.
+enum : uint8_t { CC_LIPT = 1, CC_FREET = 2 };
+__device__ __forceinline__ uint8_t chem_class(uint32_t w, int m, int l) {
+    if ((w >> kGidBits) != OC_T) return 0;
+    return m >= 0 ? CC_LIPT : (l < 0 ? CC_FREET : 0);
+}
+// A free T's formation proposal to slot b is held in its link word as
+// -2 - b until the force pass of the step (or k_chem_resolve) settles it: the
+// bond forms when the proposal is mutual.
+__device__ __forceinline__ int32_t chem_prop_code(int b) { return -2 - b; }
+// The partner of free T s whose link word l holds a proposal (l < -1): b =
+// -2 - l when b's link word proposes s, or already holds s (b settled
+// first); else -1 (no bond).
+__device__ __forceinline__ int32_t chem_settle(int s, int32_t l, const int32_t *lnk) {
+    const int b = -2 - l;
+    const int32_t lb = lnk[b];
+    return lb == chem_prop_code(s) || lb == s ? b : -1;
+}
+
 // The check's list is kept in segments: a producer block (a placement tile, or
-// a bead block of k_chem_list) writes the entries of its beads with a role
-// (slot | role << 30, in any order: every decision is per bead) to
-// list[base .. base + cnt), base = its first slot, and seg = (base, cnt,
-// replica); LDS counters only, no global atomics.
+// a bead block of k_chem_list) numbers the entries of its beads with a role
+// (an LDS counter, in any order: every decision is per bead) and writes entry
+// e to the segment's inline record when e < kChemInl (the decision loads it
+// together with the header) or to spill[base + e], base = the block's first
+// slot; the header seg = (base, cnt, replica).  No global atomics.
+constexpr int kChemInl = 16;
 struct ChemSeg {
     int32_t base, cnt, rep, pad;
 };
+// One listed bead: slot | role << kRoleShift, gid word and position.
+struct ChemEnt {
+    int32_t sr;
+    uint32_t w;
+    double x, y;
+};
+struct ChemList {
+    ChemSeg *seg;
+    ChemEnt *inl;    // kChemInl entries per segment
+    ChemEnt *spill;  // indexed by slot
+};
+__device__ __forceinline__ ChemEnt *chem_ent(const ChemList &C, int k, int base, int e) {
+    return e < kChemInl ? C.inl + (size_t)k * kChemInl + e : C.spill + base + e;
+}
+
+// The search window of a point p (rw <= 1 + 1e-6 and ncy >= 8 rows: at most
+// one seam) as up to six contiguous slot ranges: per column x-1, x, x+1 the
+// rows of [y - rw, y + rw], split at the periodic seam; csf(col, row) is the
+// first slot of cell (col, row) for row in [0, ncy] (row ncy: the end of the
+// column).  Every cell-start load is issued before the first is used.
+struct Win {
+    int first[6], cnt[6], total;
+};
+template <class CS>
+__device__ __forceinline__ Win chem_window(const RepInfo &R, double2 p, double rw, CS csf) {
+    const int cx = cell_x(R, p.x);
+    const int khi = (int)((p.y + rw) * R.icsy);
+    const int klo = (int)((p.y - rw) * R.icsy + R.ncy) - R.ncy;
+    int a0, b0, a1 = 0, b1 = -1;
+    if (klo < 0) { a0 = klo + R.ncy; b0 = R.ncy - 1; a1 = 0; b1 = khi; }
+    else if (khi >= R.ncy) { a0 = klo; b0 = R.ncy - 1; a1 = 0; b1 = khi - R.ncy; }
+    else { a0 = klo; b0 = khi; }
+    int v[12];
+#pragma unroll
+    for (int dc = 0; dc < 3; dc++) {
+        int col = cx + dc - 1;
+        col = col < 0 ? col + R.ncx : (col >= R.ncx ? col - R.ncx : col);
+        v[4 * dc] = csf(col, a0);
+        v[4 * dc + 1] = csf(col, b0 + 1);
+        v[4 * dc + 2] = b1 >= a1 ? csf(col, a1) : 0;
+        v[4 * dc + 3] = b1 >= a1 ? csf(col, b1 + 1) : 0;
+    }
+    Win W;
+    W.total = 0;
+#pragma unroll
+    for (int dc = 0; dc < 3; dc++) {
+        W.first[2 * dc] = v[4 * dc];
+        W.cnt[2 * dc] = v[4 * dc + 1] - v[4 * dc];
+        W.first[2 * dc + 1] = v[4 * dc + 2];
+        W.cnt[2 * dc + 1] = v[4 * dc + 3] - v[4 * dc + 2];
+        W.total += W.cnt[2 * dc] + W.cnt[2 * dc + 1];
+    }
+    return W;
+}
 
 // Piece list of the periodic window [c - R, c + R] of 0..n-1 in ascending order.
 __device__ __forceinline__ int win_pieces(int c, int R, int n, int *a, int *b) {
@@ -650,8 +747,8 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                                                      const int32_t *__restrict__ tslot, int32_t *__restrict__ cs_new,
                                                      int32_t *__restrict__ src, StateP in, StateP out,
                                                      const double2 *__restrict__ f, uint32_t *__restrict__ err,
-                                                     int nt, const RepStep *__restrict__ rs,
-                                                     int32_t *__restrict__ clist, ChemSeg *__restrict__ cseg) {
+                                                     int nt, const RepStep *__restrict__ rs, ChemList CL,
+                                                     uint8_t *__restrict__ cflag) {
     // Each workgroup walks tiles blockIdx.x, + gridDim.x, ...; the next tile's
     // header and bead list are loaded while the current one is placed.
     constexpr int kPl = 2;  // beads per thread per round
@@ -716,7 +813,7 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                 }
             // (tile, rank) of the partners' old slots; the chemistry partner's
             // gid word at a precursor-chemistry check
-            const bool chk = clist && rs[T.rep].chem_now;
+            const bool chk = CL.seg && rs[T.rep].chem_now;
             int32_t ts[kPl], tp[kPl];
             uint32_t wp[kPl];
 #pragma unroll
@@ -748,16 +845,24 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                 const int nl = ts[k] >= 0 ? toff[ts[k] >> kSortCapLog] + (ts[k] & (kSortCap - 1)) : -1;
                 out.lnk[d] = nl;
                 out.prv[d] = tp[k] >= 0 ? toff[tp[k] >> kSortCapLog] + (tp[k] & (kSortCap - 1)) : -1;
+                if (chk) cflag[d] = chem_class(gq[k], m, nl);
                 if (chk) {
                     const int role = chem_role(R, rs[T.rep], gq[k], m, nl, wp[k]);
-                    if (role) clist[base + atomicAdd(&s_cn, 1)] = d | (role << kRoleShift);
+                    if (role) {
+                        ChemEnt E;
+                        E.sr = d | (role << kRoleShift);
+                        E.w = gq[k];
+                        E.x = q.x;
+                        E.y = q.y;
+                        *chem_ent(CL, tix, base, atomicAdd(&s_cn, 1)) = E;
+                    }
                 }
             }
         }
-        if (clist) {
+        if (CL.seg) {
             __syncthreads();
             if (t == 0) {
-                cseg[tix] = ChemSeg{base, s_cn, T.rep, 0};
+                CL.seg[tix] = ChemSeg{base, s_cn, T.rep, 0};
                 s_cn = 0;
             }
             __syncthreads();
@@ -1068,7 +1173,30 @@ struct PairArgs {
     int32_t *key;
     int2 *rmax;
     uint32_t *moved;
+    // the state's link words (lnk, writable): a free T's pending formation
+    // proposal of this step's check is settled by its bead (chem_settle),
+    // with the event
+    int32_t *lnkw;
+    Event *ev;
+    unsigned long long *evn;
+    int64_t evcap;
+    int record;
 };
+
+// A pending proposal of bead s settled before its bonded terms: the link word
+// becomes the partner (mutual) or -1; the bond's smaller gid records the
+// event once the partner's gid is loaded (chem_formed).
+__device__ __forceinline__ bool chem_pending(const PairArgs &A, int slot, BondPre &B) {
+    if (B.n1 >= -1) return false;
+    B.n1 = chem_settle(slot, B.n1, A.lnk);
+    A.lnkw[slot] = B.n1;
+    return B.n1 >= 0;
+}
+__device__ __forceinline__ void chem_formed(const PairArgs &A, const RepStep &S, int rep, uint32_t gi,
+                                            const BondPre &B) {
+    const uint32_t gn = B.gn & kGidMask;
+    if (A.record && gi < gn) push_event(A.ev, A.evn, A.evcap, A.err, S.step, OC_EV_FORM, gi, gn, rep);
+}
 
 // Full-shell pair forces of the i-beads of one tile (a column, rows [k0,k1)).
 // The candidate beads -- columns cx-1..cx+1, rows k0-M..k1-1+M -- are staged
@@ -1451,7 +1579,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                 }
             }
             if (act) {
+                const bool formed = bond && chem_pending(A, slot, B);
                 if (bond) bond_partners(A.xy, A.gt, B);
+                if (formed) chem_formed(A, S, T.rep, gi & kGidMask, B);
                 finish(slot, pi, gi, B, ax, ay, e, vh);
             }
         }
@@ -1689,60 +1819,20 @@ __device__ __forceinline__ double dist2(const RepInfo &R, double2 p, double2 q) 
     return dx * dx + dy * dy;
 }
 
-// The cell window of a point (cutoff <= 1) as up to six contiguous slot
-// ranges: per column (x-1, x, x+1) the rows of [y-rw, y+rw] (rw <= 1 + 1e-6),
-// split at the periodic seam.  slot(k) is the k-th candidate in that order.
-struct Win {
-    int first[6], cnt[6], total;
-    __device__ __forceinline__ int slot(int k) const {
-#pragma unroll
-        for (int q = 0; q < 6; q++) {
-            if (k < cnt[q]) return first[q] + k;
-            k -= cnt[q];
-        }
-        return -1;
-    }
-};
-
-__device__ __forceinline__ Win window_of(const RepInfo &R, const int32_t *cs, double2 p, double rw) {
-    Win W;
-    const int cx = cell_x(R, p.x);
-    const int khi = (int)((p.y + rw) * R.icsy);
-    const int klo = (int)((p.y - rw) * R.icsy + R.ncy) - R.ncy;
-    W.total = 0;
-#pragma unroll
-    for (int dc = 0; dc < 3; dc++) {
-        int col = cx + dc - 1;
-        col = col < 0 ? col + R.ncx : (col >= R.ncx ? col - R.ncx : col);
-        const int cb = R.coff + col * R.ncy;
-        // the window spans fewer rows than the column (L >= 8): at most one seam
-        int a0, b0, a1 = 0, b1 = -1;
-        if (klo < 0) { a0 = klo + R.ncy; b0 = R.ncy - 1; a1 = 0; b1 = khi; }
-        else if (khi >= R.ncy) { a0 = klo; b0 = R.ncy - 1; a1 = 0; b1 = khi - R.ncy; }
-        else { a0 = klo; b0 = khi; }
-        W.first[2 * dc] = cs[cb + a0];
-        W.cnt[2 * dc] = cs[cb + b0 + 1] - W.first[2 * dc];
-        W.first[2 * dc + 1] = cs[cb + a1];
-        W.cnt[2 * dc + 1] = b1 >= a1 ? cs[cb + b1 + 1] - W.first[2 * dc + 1] : 0;
-        W.total += W.cnt[2 * dc] + W.cnt[2 * dc + 1];
-    }
-    return W;
-}
-
-enum : uint8_t { CF_BREAK = 1, CF_CONV = 2 };
-constexpr int kChemGroup = 8;  // lanes per search
-constexpr int kChemK = 2;      // candidates per lane per chunk
+constexpr int kChemGroup = 4;  // lanes per entry
+constexpr int kChemK = 2;      // candidates per lane per round
 constexpr int kChemGrid = 1 << 16;  // wavefronts walking the segments of the check's list
 
 // The check's list for a state that was not just placed (a chemistry check
-// on a loaded state): one segment per bead block.
+// on a loaded state): one segment per bead block, and every bead's class.
 __global__ __launch_bounds__(kBeadBlock) void k_chem_list(const RepInfo *__restrict__ ri,
                                                           const RepStep *__restrict__ rs,
                                                           const int32_t *__restrict__ bblk,
+                                                          const double2 *__restrict__ xy,
                                                           const uint32_t *__restrict__ gt,
                                                           const int8_t *__restrict__ mp,
-                                                          const int32_t *__restrict__ lnk, int32_t *__restrict__ list,
-                                                          ChemSeg *__restrict__ seg, ChemInj inj) {
+                                                          const int32_t *__restrict__ lnk, ChemList CL,
+                                                          uint8_t *__restrict__ cls, ChemInj inj) {
     __shared__ int s_n;
     const int r = bblk[blockIdx.x];
     const RepInfo &R = ri[r];
@@ -1755,134 +1845,168 @@ __global__ __launch_bounds__(kBeadBlock) void k_chem_list(const RepInfo *__restr
         const int s = R.off + loc;
         const uint32_t w = gt[s];
         const int m = mp[s], l = lnk[s];
+        cls[s] = chem_class(w, m, l);
         const int role = chem_role(R, S, w, m, l, m < 0 && l >= 0 ? gt[l] : 0u, inj.uconv ? &inj : nullptr);
-        if (role) list[base + atomicAdd(&s_n, 1)] = s | (role << kRoleShift);
+        if (role) {
+            ChemEnt E;
+            E.sr = s | (role << kRoleShift);
+            E.w = w;
+            const double2 p = xy[s];
+            E.x = p.x;
+            E.y = p.y;
+            *chem_ent(CL, blockIdx.x, base, atomicAdd(&s_n, 1)) = E;
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) seg[blockIdx.x] = ChemSeg{base, s_n, r, 0};
+    if (threadIdx.x == 0) CL.seg[blockIdx.x] = ChemSeg{base, s_n, r, 0};
 }
 
-// Decisions from the state at the start of the check for the listed beads:
-// a searching P: any lipid tail within r_conv; a searching free T: the free T
-// within r_form with the smallest symmetric draw pair_u01(kform) below p_form
-// (ties: the smaller gid).  kChemGroup lanes per entry (the window's
-// candidates dealt round robin, then reduced over the group).  flag / best
-// are written for the listed beads only; k_chem_apply reads nothing else.
-__global__ __launch_bounds__(64) void k_chem_decide(const RepInfo *__restrict__ ri, const RepStep *__restrict__ rs,
-                                                    const int32_t *__restrict__ cs, const double2 *__restrict__ xy,
-                                                    const uint32_t *__restrict__ gt, const int8_t *__restrict__ mp,
-                                                    const int32_t *__restrict__ lnk, const int32_t *__restrict__ list,
-                                                    const ChemSeg *__restrict__ seg, int nseg,
-                                                    uint8_t *__restrict__ flag, int32_t *__restrict__ best,
-                                                    ChemInj inj) {
-    const int gl = threadIdx.x & (kChemGroup - 1);
-    constexpr int per = 64 / kChemGroup;
-    // one wavefront per segment, kChemGroup lanes per entry; segments walked
-    // in the placement's XCD bands (their beads are in this XCD's L2)
-    const TileWalk Wk = tile_walk(nseg);
-    for (int k = Wk.first; k < Wk.end; k += Wk.stride) {
-        const ChemSeg G = seg[k];
-        const RepStep &S = rs[G.rep];
-        if (!S.chem_now) continue;
-        const RepInfo &R = ri[G.rep];
-        for (int e = (int)threadIdx.x / kChemGroup; e < G.cnt; e += per) {
-        const int32_t ent = list[G.base + e];
-        const int s = ent & ((1 << kRoleShift) - 1), role = (int)((uint32_t)ent >> kRoleShift);
-        if (!(role & 2)) {  // a break only
-            if (gl == 0) {
-                flag[s] = CF_BREAK;
-                best[s] = -1;
+// Search of a listed bead s (gid g) at p by the kChemGroup lanes of its group
+// (gl: the lane in the group; every lane returns the group's result).  conv:
+// any lipid tail within r_conv.  Otherwise (a free T): the free T within
+// r_form with the smallest symmetric draw pair_u01(kform) below p_form (ties:
+// the smaller gid), as its slot (-1: none) and gid.  The window's candidates
+// are dealt round robin; a lane reads each candidate's class byte, and the
+// position (and gid) only of a candidate of the class searched for.
+__device__ __forceinline__ void chem_search(const RepInfo &R, const RepStep &S, const int32_t *__restrict__ cs,
+                                            const double2 *__restrict__ xy, const uint32_t *__restrict__ gt,
+                                            const uint8_t *__restrict__ cls, const ChemInj &inj, int s, uint32_t g,
+                                            double2 p, bool conv, int gl, bool &found, int32_t &bsel,
+                                            uint32_t &bg) {
+    const Win W = chem_window(R, p, conv ? R.rwc : R.rwf,
+                              [&](int col, int row) { return cs[R.coff + col * R.ncy + row]; });
+    const uint8_t want = conv ? CC_LIPT : CC_FREET;
+    found = false;
+    uint64_t bkey = ~0ull;  // (bits of u) of the best accepted draw
+    bg = 0xffffffffu;
+    bsel = -1;
+    for (int k0 = gl; k0 < W.total; k0 += kChemGroup * kChemK) {
+        int jc[kChemK];
+        uint8_t fc[kChemK];
+#pragma unroll
+        for (int i = 0; i < kChemK; i++) {
+            int kk = k0 + i * kChemGroup, j = -1;
+            if (kk < W.total) {
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    if (j < 0 && kk < W.cnt[q]) j = W.first[q] + kk;
+                    kk -= W.cnt[q];
+                }
             }
-            continue;
+            jc[i] = j == s ? -1 : j;
+            fc[i] = jc[i] >= 0 ? cls[jc[i]] : 0;
         }
-        const uint32_t w = gt[s], g = w & kGidMask, ty = w >> kGidBits;
-        const double2 p = xy[s];
-        const Win W = window_of(R, cs, p, ty == OC_N ? R.rwc : R.rwf);
-        bool found = false;
-        uint64_t bkey = ~0ull;  // (bits of u) of the best accepted draw
-        uint32_t bg = 0xffffffffu;
-        int32_t bsel = -1;
-        // kChemK candidates per lane per chunk (one chunk covers most
-        // windows), every field of a candidate loaded in one round
-        for (int k0 = gl; k0 < W.total; k0 += kChemGroup * kChemK) {
-            int jc[kChemK];
-            uint32_t gc[kChemK];
-            int8_t mc[kChemK];
-            int32_t lc[kChemK];
-            double2 pc[kChemK];
+        double2 pc[kChemK];
+        uint32_t gc[kChemK];
 #pragma unroll
-            for (int i = 0; i < kChemK; i++) {
-                const int k = k0 + i * kChemGroup;
-                jc[i] = k < W.total ? W.slot(k) : -1;
-                if (jc[i] == s) jc[i] = -1;
-                const int jj = jc[i] >= 0 ? jc[i] : s;  // a valid slot for the loads
-                gc[i] = gt[jj];
-                mc[i] = mp[jj];
-                lc[i] = lnk[jj];
-                pc[i] = xy[jj];
+        for (int i = 0; i < kChemK; i++)  // every load of the round issued before the first is used
+            if (fc[i] & want) {
+                pc[i] = xy[jc[i]];
+                if (!conv) gc[i] = gt[jc[i]] & kGidMask;
             }
 #pragma unroll
-            for (int i = 0; i < kChemK; i++)
-                if (jc[i] >= 0 && (gc[i] >> kGidBits) != OC_T) jc[i] = -1;
-#pragma unroll
-            for (int i = 0; i < kChemK; i++) {
-                if (jc[i] < 0) continue;
-                if (ty == OC_N) {
-                    // a lipid tail bead within r_conv
-                    if (mc[i] >= 0 && dist2(R, p, pc[i]) < R.rc2) found = true;
-                } else if (mc[i] < 0 && lc[i] < 0 && dist2(R, p, pc[i]) < R.rf2) {
-                    // a free T within r_form; draws in [0,1) order as their bits
-                    const uint32_t gg = gc[i] & kGidMask;
-                    double u;
-                    if (inj.fk) {  // the caller's formation uniform of the pair
-                        const int64_t q = inj_find(inj.fk, inj.nf, g, gg);
-                        if (q < 0) atomicOr(inj.err, ERR_INJ);
-                        u = q < 0 ? 2.0 : inj.fv[q];
-                    } else {
-                        u = pair_u01(S.kform, g, gg);
-                    }
-                    const uint64_t ku = (uint64_t)__double_as_longlong(u);
-                    if (u < R.p_form && (ku < bkey || (ku == bkey && gg < bg))) {
-                        bkey = ku;
-                        bg = gg;
-                        bsel = jc[i];
-                    }
+        for (int i = 0; i < kChemK; i++) {
+            if (!(fc[i] & want)) continue;
+            if (conv) {
+                if (dist2(R, p, pc[i]) < R.rc2) found = true;
+            } else if (dist2(R, p, pc[i]) < R.rf2) {
+                // draws in [0,1) order as their bits
+                const uint32_t gg = gc[i];
+                double u;
+                if (inj.fk) {  // the caller's formation uniform of the pair
+                    const int64_t q = inj_find(inj.fk, inj.nf, g, gg);
+                    if (q < 0) atomicOr(inj.err, ERR_INJ);
+                    u = q < 0 ? 2.0 : inj.fv[q];
+                } else {
+                    u = pair_u01(S.kform, g, gg);
+                }
+                const uint64_t ku = (uint64_t)__double_as_longlong(u);
+                if (u < R.p_form && (ku < bkey || (ku == bkey && gg < bg))) {
+                    bkey = ku;
+                    bg = gg;
+                    bsel = jc[i];
                 }
             }
         }
-        for (int o = kChemGroup / 2; o > 0; o >>= 1) {
-            found |= __shfl_xor((int)found, o, kChemGroup) != 0;
-            const uint64_t ok = __shfl_xor(bkey, o, kChemGroup);
-            const uint32_t og = __shfl_xor(bg, o, kChemGroup);
-            const int32_t oj = __shfl_xor(bsel, o, kChemGroup);
-            if (ok < bkey || (ok == bkey && og < bg)) {
-                bkey = ok;
-                bg = og;
-                bsel = oj;
+    }
+    for (int o = kChemGroup / 2; o > 0; o >>= 1) {
+        found |= __shfl_xor((int)found, o, kChemGroup) != 0;
+        const uint64_t ok = __shfl_xor(bkey, o, kChemGroup);
+        const uint32_t og = __shfl_xor(bg, o, kChemGroup);
+        const int32_t oj = __shfl_xor(bsel, o, kChemGroup);
+        if (ok < bkey || (ok == bkey && og < bg)) {
+            bkey = ok;
+            bg = og;
+            bsel = oj;
+        }
+    }
+}
+
+// The check (own definition), decided from the state at its start for the
+// listed beads and applied at once, one wavefront per segment, kChemGroup
+// lanes per entry (the segment header and its inline entries are loaded
+// together):
+//   - a break (the bond's smaller gid drew it): both link words cleared;
+//   - a searching P with a lipid tail within r_conv becomes T;
+//   - a searching free T with a proposal writes it into its link word
+//     (chem_prop_code); the force pass that follows (or k_chem_resolve)
+//     bonds the mutual ones.
+// The searches read only what the check does not change -- the class bytes,
+// positions and gids -- so the writes do not race with them.  Every write has
+// one writer; events are sorted when read.
+__global__ __launch_bounds__(64) void k_chem_check(const RepInfo *__restrict__ ri, const RepStep *__restrict__ rs,
+                                                   const int32_t *__restrict__ cs, const double2 *__restrict__ xy,
+                                                   uint32_t *__restrict__ gt, int32_t *__restrict__ lnk,
+                                                   const uint8_t *__restrict__ cls, ChemList CL, int nseg,
+                                                   Event *__restrict__ ev, unsigned long long *__restrict__ evn,
+                                                   int64_t evcap, int record, uint32_t *__restrict__ err,
+                                                   ChemInj inj) {
+    const int gl = threadIdx.x & (kChemGroup - 1);
+    const int ge = (int)threadIdx.x / kChemGroup;
+    constexpr int per = 64 / kChemGroup;
+    static_assert(per <= kChemInl, "a wavefront's first entries are inline");
+    // segments walked in the placement's XCD bands (their beads are in this
+    // XCD's L2)
+    const TileWalk Wk = tile_walk(nseg);
+    for (int k = Wk.first; k < Wk.end; k += Wk.stride) {
+        const ChemSeg G = CL.seg[k];
+        ChemEnt E = CL.inl[(size_t)k * kChemInl + ge];  // read with the header (used when ge < cnt)
+        if (G.cnt <= 0) continue;
+        const RepInfo &R = ri[G.rep];
+        const RepStep &S = rs[G.rep];
+        for (int e = ge; e < G.cnt; e += per) {
+            if (e != ge) E = *chem_ent(CL, k, G.base, e);  // the wavefront's later rounds
+            const int s = E.sr & ((1 << kRoleShift) - 1), role = (int)((uint32_t)E.sr >> kRoleShift);
+            const uint32_t g = E.w & kGidMask, ty = E.w >> kGidBits;
+            if (!(role & 2)) {  // a break (of a bead in no search)
+                if (gl == 0) {
+                    const int pq = lnk[s];
+                    const uint32_t gp = gt[pq] & kGidMask;
+                    lnk[s] = -1;
+                    lnk[pq] = -1;
+                    if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_BREAK, min(g, gp), max(g, gp), G.rep);
+                }
+                continue;
             }
-        }
-        if (gl == 0) {
-            flag[s] = (role & 1 ? CF_BREAK : 0) | (found ? CF_CONV : 0);
-            best[s] = bsel;
-        }
+            bool found;
+            int32_t b;
+            uint32_t bg;
+            chem_search(R, S, cs, xy, gt, cls, inj, s, g, make_double2(E.x, E.y), ty == OC_N, gl, found, b, bg);
+            if (gl != 0) continue;
+            if (ty == OC_N) {
+                if (found) {
+                    gt[s] = g | ((uint32_t)OC_T << kGidBits);
+                    if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_CONV, g, g, G.rep);
+                }
+            } else if (b >= 0) {
+                lnk[s] = chem_prop_code(b);
+            }
         }
     }
 }
 
 __device__ __forceinline__ bool R_react(const RepInfo &R) { return R.react != 0; }
 
-__device__ __forceinline__ void push_event(Event *ev, unsigned long long *evn, int64_t cap, uint32_t *err, int64_t step,
-                                           int kind, uint32_t a, uint32_t b, int rep, double x = 0.0,
-                                           double y = 0.0) {
-    const unsigned long long k = atomicAdd(evn, 1ull);
-    if ((int64_t)k < cap) {
-        Event e;
-        e.step = step; e.kind = kind; e.gid_a = a; e.gid_b = b; e.rep = rep; e.x = x; e.y = y;
-        ev[k] = e;
-    } else {
-        atomicOr(err, ERR_EVOVF);
-    }
-}
 
 // The decay pass and the food supply of the reference's chemistry check,
 // cel/engine/oercel.c:1656-1753, as per-bead passes over every bead of the
@@ -1987,49 +2111,30 @@ __global__ __launch_bounds__(kBeadBlock) void k_rfeed(const RepInfo *__restrict_
     if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_FEED, g, g, r, p.x, p.y);
 }
 
-// Applies the decisions of the listed beads of one chunk.  A proposal forms a
-// bond when it is mutual (best[best[s]] == s; best of a free T is always
-// written by k_chem_decide of the same check).
-// Apply the check's decisions to the listed beads: breaks (the bond's smaller
-// gid), conversions, and bonds of mutual best proposals (written by the
-// smaller gid).  Every write has one writer: the order is irrelevant, and
-// the events are sorted when read.
-__global__ __launch_bounds__(64) void k_chem_apply(const RepStep *__restrict__ rs, uint32_t *__restrict__ gt,
-                                                   int32_t *__restrict__ lnk, const uint8_t *__restrict__ flag,
-                                                   const int32_t *__restrict__ best, const int32_t *__restrict__ list,
-                                                   const ChemSeg *__restrict__ seg, int nseg, Event *__restrict__ ev,
-                                                   unsigned long long *__restrict__ evn, int64_t evcap, int record,
-                                                   uint32_t *__restrict__ err) {
+// A listed free T's pending proposal settled (the rule the force pass applies
+// to each bead, chem_settle): the bond forms when the proposal is mutual.
+// For a check on its own (oc_chem), after k_chem_check.
+__global__ __launch_bounds__(64) void k_chem_resolve(const RepStep *__restrict__ rs, const uint32_t *__restrict__ gt,
+                                                     int32_t *__restrict__ lnk, ChemList CL, int nseg,
+                                                     Event *__restrict__ ev, unsigned long long *__restrict__ evn,
+                                                     int64_t evcap, int record, uint32_t *__restrict__ err) {
     const TileWalk Wk = tile_walk(nseg);
     for (int k = Wk.first; k < Wk.end; k += Wk.stride) {
-        const ChemSeg G = seg[k];
+        const ChemSeg G = CL.seg[k];
+        if (G.cnt <= 0) continue;
         const RepStep &S = rs[G.rep];
-        if (!S.chem_now) continue;
-        const int r = G.rep;
         for (int e = threadIdx.x; e < G.cnt; e += 64) {
-        const int s = list[G.base + e] & ((1 << kRoleShift) - 1);
-        const uint32_t g = gt[s] & kGidMask;
-        const uint8_t fl = flag[s];
-        if (fl & CF_BREAK) {
-            const int p = lnk[s];
-            const uint32_t gp = gt[p] & kGidMask;
-            lnk[s] = -1;
-            lnk[p] = -1;
-            if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_BREAK, min(g, gp), max(g, gp), r);
-        }
-        if (fl & CF_CONV) {
-            gt[s] = g | ((uint32_t)OC_T << kGidBits);
-            if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_CONV, g, g, r);
-        }
-        const int b = best[s];
-        if (b >= 0 && best[b] == s) {
-            const uint32_t gb = gt[b] & kGidMask;
-            if (g < gb) {
-                lnk[s] = b;
-                lnk[b] = s;
-                if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_FORM, g, gb, r);
+            const ChemEnt &E = *chem_ent(CL, k, G.base, e);
+            const int s = E.sr & ((1 << kRoleShift) - 1);
+            const int l = lnk[s];
+            if (l >= -1) continue;
+            const uint32_t g = E.w & kGidMask;
+            const int n = chem_settle(s, l, lnk);
+            lnk[s] = n;
+            if (n >= 0 && record) {
+                const uint32_t gb = gt[n] & kGidMask;
+                if (g < gb) push_event(ev, evn, evcap, err, S.step, OC_EV_FORM, g, gb, G.rep);
             }
-        }
         }
     }
 }

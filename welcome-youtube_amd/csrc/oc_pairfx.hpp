@@ -350,7 +350,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair_fx(PairArgs A) {
                     }
             }
             if (act) {
+                const bool formed = bond && chem_pending(A, slot, Bd);
                 if (bond) bond_partners(A.xy, A.gt, Bd);
+                if (formed) chem_formed(A, S, T.rep, gi, Bd);
                 double bx = 0.0;
                 if (A.stir) bx = A.stir[slot] - S.stir_mean;
                 double fxs = bx + ax, fys = ay;
