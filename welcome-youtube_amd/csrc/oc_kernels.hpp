@@ -925,10 +925,12 @@ __device__ __forceinline__ int64_t inj_find(const uint64_t *k, int64_t n, uint32
 // (dx, dy) / r: F_ij = s * (ux, uy), s = w (a_ij - gamma w r.v + sigma/sqrt(dt)
 // theta_ij), w = 1 - r.  gj: the candidate's gid, a: a_ij; sdtk = sigma /
 // sqrt(dt) * sqrt(12) / 2^32 (the draw scale folded in).
-__device__ __forceinline__ float pair_core(float ux, float uy, float r, float vxi, float vyi, uint32_t gi, float a,
-                                           float2 vj, uint32_t gj, float gam, float sdtk, uint64_t sseed, float &w,
-                                           const InjTheta *inj = nullptr) {
-    w = 1.0f - r;
+// r * rscale is the distance: rscale a power of two, so w = 1 - r rscale in
+// one fused multiply-add rounds exactly as the product then the difference.
+__device__ __forceinline__ float pair_core(float ux, float uy, float r, float rscale, float vxi, float vyi, uint32_t gi,
+                                           float a, float2 vj, uint32_t gj, float gam, float sdtk, uint64_t sseed,
+                                           float &w, const InjTheta *inj = nullptr) {
+    w = __builtin_fmaf(r, -rscale, 1.0f);
     const float dvx = vxi - vj.x, dvy = vyi - vj.y;
     const float rv = __builtin_fmaf(ux, dvx, uy * dvy);
     float th;
@@ -959,7 +961,7 @@ __device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float 
         const float ux = dx * ir, uy = dy * ir;
         float w;
         const float a = *ap;
-        const float fs = pair_core(ux, uy, r2 * ir, vxi, vyi, gi, a, vj, gj, gam, sdtk, sseed, w, inj);
+        const float fs = pair_core(ux, uy, r2 * ir, 1.0f, vxi, vyi, gi, a, vj, gj, gam, sdtk, sseed, w, inj);
         ax += (double)(fs * ux);
         ay += (double)(fs * uy);
         if (energy) {
@@ -998,7 +1000,7 @@ __device__ __forceinline__ void pair_one_q(uint2 xq, float vxi, float vyi, uint3
         const float ux = dx * ir, uy = dy * ir;
         float w;
         const float a = *ap;
-        const float fs = pair_core(ux, uy, (r2 * ir) * kFixR, vxi, vyi, gi, a, vj, gj, gam, sdtk, sseed, w, inj);
+        const float fs = pair_core(ux, uy, r2 * ir, kFixR, vxi, vyi, gi, a, vj, gj, gam, sdtk, sseed, w, inj);
         ax += (double)(fs * ux);
         ay += (double)(fs * uy);
         if (energy) {
@@ -1216,12 +1218,15 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     static_assert(CAP < 65536, "staged indices are 16-bit");
     constexpr int kMaxRows = pair_max_rows(BLOCK);
     static_assert(kMaxRows <= 2 * BLOCK, "two staged rows per thread");
-    __shared__ uint2 s_xq[CAP];               // candidate's fixed-point coordinates (its image next to the tile)
-    // the candidate's predicted velocity (x, y bits), gid (z) and the byte
-    // offset of its type in a row of the a_ij table (w)
-    __shared__ uint4 s_vg[CAP];
+    // Staged candidate j, three 8-byte words at the same stride (one address
+    // for all three in the candidate loop): s_st[j] its fixed-point coordinates
+    // (its image next to the tile), s_st[CAP + j] its predicted velocity (x, y
+    // bits), s_st[2 CAP + j] its gid and the byte offset of its type in a row
+    // of the a_ij table.
+    __shared__ uint2 s_st[3 * CAP];
+    uint2 *const s_xq = s_st;
     // before the staging loads: each candidate's source (slot | column | image)
-    uint32_t *s_src = reinterpret_cast<uint32_t *>(s_vg);
+    uint32_t *s_src = reinterpret_cast<uint32_t *>(s_st + CAP);
     __shared__ uint16_t s_rs[kMaxRows + 1];   // LDS index of the first candidate of staged row r (+ the end)
     __shared__ uint16_t s_ib[CAP];            // LDS index of the tile's q-th i-bead
     __shared__ float s_a[64];
@@ -1350,7 +1355,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                 sc[u] = (uint32_t)c | ((uint32_t)syc << 2);
             }
         }
-        __syncthreads();  // the sources are read: s_vg takes the candidates
+        __syncthreads();  // the sources are read: the velocity words take the candidates
 #pragma unroll
         for (int u = 0; u < kStgPer; u++) {
             if (u * BLOCK >= total) break;
@@ -1359,8 +1364,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                 const uint32_t c = sc[u] & 3u, syc = sc[u] >> 2;
                 s_xq[e] = make_uint2(fix_lo(sx[u].x) + (c == 0 ? fx0 : (c == 2 ? fx2 : 0u)),
                                      fix_lo(sx[u].y) + (syc == 1 ? 0u - fL : (syc == 2 ? fL : 0u)));
-                s_vg[e] = make_uint4(__float_as_uint(sv[u].x), __float_as_uint(sv[u].y), sg[u] & kGidMask,
-                                     (sg[u] >> kGidBits) * 4u);
+                s_st[CAP + e] = make_uint2(__float_as_uint(sv[u].x), __float_as_uint(sv[u].y));
+                s_st[2 * CAP + e] = make_uint2(sg[u] & kGidMask, (sg[u] >> kGidBits) * 4u);
             }
         }
         __syncthreads();
@@ -1396,17 +1401,17 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     // Cross-lane pair queue (CAPL == 2): per wavefront, kPairQueue entries of
     // LDS, each first an (i, j) pair of staged indices, then its force.
     __shared__ uint2 s_q[CAPL == 2 ? (BLOCK / 64) * kPairQueue : 1];
-    // candidate j's predicted velocity and gid word: one 16-byte LDS read
-    // when staged
-    auto run = [&](const uint2 *XQ, const float2 *VT, const uint32_t *GT, const uint4 *VG, const bool stg) {
+    // candidate j's predicted velocity and gid word: two 8-byte LDS reads at
+    // the stride of its coordinates when staged
+    auto run = [&](const uint2 *XQ, const float2 *VT, const uint32_t *GT, const bool stg) {
         const double2 *XY = A.xy;
         // candidate j: velocity, gid and the address of a_ij in row arow
         auto cand = [&](int j, const float *arow, float2 &v, uint32_t &g, const float *&ap) {
             if (stg) {
-                const uint4 w = VG[j];
-                v = make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
-                g = w.z;
-                ap = (const float *)((const char *)arow + w.w);
+                const uint2 wv = XQ[CAP + j], wg = XQ[2 * CAP + j];
+                v = make_float2(__uint_as_float(wv.x), __uint_as_float(wv.y));
+                g = wg.x;
+                ap = (const float *)((const char *)arow + wg.y);
             } else {
                 const uint32_t gt = GT[j];
                 v = VT[j];
@@ -1495,7 +1500,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                             const float ir = __builtin_amdgcn_rsqf(r2);
                             const float ux = dx * ir, uy = dy * ir;
                             float w;
-                            const float fs = pair_core(ux, uy, (r2 * ir) * kFixR, v1.x, v1.y, g1, *ap, v2, g2, gam,
+                            const float fs = pair_core(ux, uy, r2 * ir, kFixR, v1.x, v1.y, g1, *ap, v2, g2, gam,
                                                        sdtk, sseed, w);
                             qb[q3] = make_uint2(__float_as_uint(fs * ux), __float_as_uint(fs * uy));
                         }
@@ -1510,13 +1515,13 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                         __syncwarp();
                     }
                 } else if (CAPL != 1) {
-                    // the single pass
-                    for (; j < st + nj; j++) {
-                        float2 vj;
-                        uint32_t gj;
-                        const float *ap;
-                        cand(j, arow, vj, gj, ap);
-                        pair_one_q(xq, vi.x, vi.y, gi, XQ[j], vj, gj, ap, gam, sdtk, sseed, ax, ay, e, ENERGY, inj);
+                    // the single pass, the staged words addressed through
+                    // one pointer (one induction variable for the loop)
+                    for (const uint2 *pj = XQ + st, *pe = XQ + st + nj; pj < pe; pj++) {
+                        const uint2 wv = pj[CAP], wg = pj[2 * CAP];
+                        const float2 vj = make_float2(__uint_as_float(wv.x), __uint_as_float(wv.y));
+                        const float *ap = (const float *)((const char *)arow + wg.y);
+                        pair_one_q(xq, vi.x, vi.y, gi, *pj, vj, wg.x, ap, gam, sdtk, sseed, ax, ay, e, ENERGY, inj);
                     }
                 } else {
                     //   CAPL == 1: two phases per lane: the cutoff test over
@@ -1586,8 +1591,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             }
         }
     };
-    if (staged) run(s_xq, nullptr, nullptr, s_vg, true);
-    else run(nullptr, A.vt, A.gt, nullptr, false);
+    if (staged) run(s_st, nullptr, nullptr, true);
+    else run(nullptr, A.vt, A.gt, false);
     if (ENERGY || !A.bonded) return;
     // running max bond length of the replica: block max, then an atomic only
     // when it exceeds the stored value (positive doubles order as their bits)
