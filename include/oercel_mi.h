@@ -193,13 +193,14 @@ void oc_destroy(oc_engine *h);
 int oc_set_tuning(oc_engine *h, int32_t pair_block, int32_t tile_beads);
 
 /* Pair-kernel variant (no counterpart in the reference, whose pair loop has
- * one form): 0 evaluates every candidate in one pass per lane; 1 tests every
- * candidate first and evaluates each lane's marked interacting pairs in a
- * second pass; 2 queues the interacting pairs of all lanes of a wavefront and
- * evaluates them 64 at a time, each lane summing its own pairs back in
- * candidate order.  0 and 2 give bit-identical forces; 1 the same up to the
- * FP64 rounding of the per-bead sums (pairs past a full mask are summed in
- * another order).  Other values: OC_EINVAL. */
+ * one form): 0 evaluates every candidate in one pass per lane; 1 (the
+ * default) tests every candidate first and evaluates each lane's marked
+ * interacting pairs in a second pass; 2 queues the interacting pairs of all
+ * lanes of a wavefront and evaluates them 64 at a time, each lane summing its
+ * own pairs back in candidate order; 3 lists each lane's interacting pairs in
+ * LDS.  0, 1 and 2 give bit-identical forces for windows of up to 64
+ * candidates (1 sums the pairs past a full mask first, in another FP64
+ * rounding order).  Other values: OC_EINVAL. */
 int oc_set_pair_variant(oc_engine *h, int32_t lists);
 
 /* Host-side soup of replica r in gid order, exactly as init_soup +

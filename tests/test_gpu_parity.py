@@ -100,9 +100,10 @@ def test_pair_forces_dense_clump(oracle, eng, rng, variant):
 def test_pair_queue_bit_identical_to_single_pass(oracle, eng, rng):
     """The cross-lane pair queue adds the same FP32 pair terms in the same
     order as the single pass: forces (with bonded terms) and a 30-step
-    trajectory with chemistry are bitwise equal.  The second system has a
-    clump of 300 beads, whose wavefronts exceed 64 candidates per lane and fall
-    back to the single pass."""
+    trajectory with chemistry are bitwise equal; so does the two-phase kernel
+    (the library's default) while every window fits its 64-bit mask.  The
+    second system has a clump of 300 beads, whose wavefronts exceed 64
+    candidates per lane and fall back to the single pass."""
     for L, clump in ((61.0, 0), (40.0, 300)):
         say(f"case: queue vs single pass L={L} clump={clump}")
         p = mkparams(L, nsub=4, f_lip=0.3, c_N=0.2, seed=31, pchem=1, p_conv=0.2, p_form=0.2, p_break=0.05)
@@ -114,16 +115,23 @@ def test_pair_queue_bit_identical_to_single_pass(oracle, eng, rng):
         a = device_forces(eng, p, s, 777, bonded=True, variant=0)
         b = device_forces(eng, p, s, 777, bonded=True, variant=2)
         assert np.array_equal(a["fx"], b["fx"]) and np.array_equal(a["fy"], b["fy"])
+        # the two-phase kernel (the default) sums the same pairs in the same
+        # order while every window fits its 64-bit mask (no clump)
+        vs = (0, 2) if clump else (0, 1, 2)
+        if not clump:
+            c = device_forces(eng, p, s, 777, bonded=True, variant=1)
+            assert np.array_equal(a["fx"], c["fx"]) and np.array_equal(a["fy"], c["fy"])
         st = {k: s[k] for k in ("x", "y", "vx", "vy", "type", "mpos", "nxt")}
         outs = []
-        for v in (0, 2):
+        for v in vs:
             e = eng.Engine(p, pair_variant=v)
             e.start([st])
             e.step(30)
             outs.append(e.get_state())
             e.close()
-        for k in outs[0]:
-            assert np.array_equal(outs[0][k], outs[1][k]), k
+        for o in outs[1:]:
+            for k in outs[0]:
+                assert np.array_equal(outs[0][k], o[k]), k
 
 
 def test_bonded_forces_vs_oracle(oracle, eng, rng):

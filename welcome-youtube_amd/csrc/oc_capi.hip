@@ -41,7 +41,7 @@ struct oc_engine {
     std::vector<Tile> tiles;   // pair-force tiles
     std::vector<Tile> stiles;  // sort tiles (rows of new cells, one thread each)
     int pair_block = 256, tile_beads = 0;
-    int pair_variant = 0;  // k_pair: 0 single pass, 1 per-lane two phases, 2 cross-lane pair queue
+    int pair_variant = 1;  // k_pair: 0 single pass, 1 per-lane two phases (default), 2 cross-lane pair queue
     int per_xcd = 0;
     // device
     RepInfo *d_ri = nullptr;

@@ -989,13 +989,15 @@ __device__ __forceinline__ uint32_t fix_lo(double v) {  // v >= 0
 }
 
 // pair_one on staged coordinates: xq, qj the fixed-point coordinates of i and j
+// TEST: the cutoff test (false: the caller has made it, the pair interacts)
+template <bool TEST = true>
 __device__ __forceinline__ void pair_one_q(uint2 xq, float vxi, float vyi, uint32_t gi, uint2 qj, float2 vj,
                                            uint32_t gj, const float *ap, float gam, float sdtk, uint64_t sseed,
                                            double &ax, double &ay, float2 &e, bool energy,
                                            const InjTheta *inj = nullptr) {
     const float dx = (float)(int32_t)(xq.x - qj.x), dy = (float)(int32_t)(xq.y - qj.y);
     const float r2 = __builtin_fmaf(dx, dx, dy * dy);
-    if (r2 < kFixR2Max && r2 > kFixR2Min) {
+    if (!TEST || (r2 < kFixR2Max && r2 > kFixR2Min)) {
         const float ir = __builtin_amdgcn_rsqf(r2);
         const float ux = dx * ir, uy = dy * ir;
         float w;
@@ -1547,15 +1549,21 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                             }
                         }
                     }
-                    // pass 2: the marked candidates in window order
-                    while (mk) {
-                        const int jk = st + __builtin_ctzll(mk);
-                        mk &= mk - 1;
-                        float2 vj;
-                        uint32_t gj;
-                        const float *ap;
-                        cand(jk, arow, vj, gj, ap);
-                        pair_one_q(xq, vi.x, vi.y, gi, XQ[jk], vj, gj, ap, gam, sdtk, sseed, ax, ay, e, ENERGY, inj);
+                    // pass 2: the marked candidates in window order, the
+                    // mask as two 32-bit words (most windows fit the first)
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        uint32_t m = h ? (uint32_t)(mk >> 32) : (uint32_t)mk;
+                        const uint2 *pb = XQ + st + 32 * h;
+                        while (m) {
+                            const uint2 *pj = pb + __builtin_ctz(m);
+                            m &= m - 1;
+                            const uint2 wv = pj[CAP], wg = pj[2 * CAP];
+                            const float2 vj = make_float2(__uint_as_float(wv.x), __uint_as_float(wv.y));
+                            const float *ap = (const float *)((const char *)arow + wg.y);
+                            pair_one_q<false>(xq, vi.x, vi.y, gi, *pj, vj, wg.x, ap, gam, sdtk, sseed, ax, ay, e,
+                                              ENERGY, inj);
+                        }
                     }
                 }
             } else if (act) {
