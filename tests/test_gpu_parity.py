@@ -418,17 +418,20 @@ def test_momentum_and_temperature_config1(eng, f_lip):
 
 def test_tail_tail_rdf_vs_oracle(oracle, eng):
     """Stochastic run: the tail-tail RDF (64 bins on [0, 3), triweight width
-    one bin) averaged over 400 steps after 1,500 steps from the same soup,
-    device against the FP64 oracle.  The two trajectories have decorrelated by
-    then (FP32 forces), so the difference is sampling noise; its scale is set
-    by an oracle run from another seed.  Stated bound: relative L2 error
-    ||g_dev - g_oracle|| / ||g_oracle|| < 0.03, and below twice the distance
-    between the two oracle runs."""
+    one bin) averaged over 400 steps after 1,500 steps from the soup, device
+    against the FP64 oracle.  The trajectories decorrelate long before (FP32
+    forces), so one run against one run differs by sampling noise of the size
+    of the seed-to-seed scatter (about 0.02 relative L2 per run).  The test
+    compares the mean over 8 seeds on the device (one ensemble launch) with
+    the mean over 3 seeds of the oracle; the expected distance of two such
+    means is sigma sqrt(1/8 + 1/3), with sigma^2 the per-run variance summed
+    over bins, estimated from the 8 device runs.  Stated bound: relative L2
+    distance of the means < 0.03 and < 3 times that expected distance, and
+    the oracle's own runs lie within 3 sigma of their mean."""
     abi = importlib.import_module("welcome-youtube_amd.abi")
-    p = mkparams(40.0, f_lip=0.3, seed=11)
-    cp = p.to_c()
     spec, ospec = abi.rdf_spec(2, 2, 64, 3.0), oracle.rdf_spec(2, 2, 64, 3.0)
-    nwin, eq = 400, 1500
+    nwin, eq, kd, ko = 400, 1500, 8, 3
+    ps = [mkparams(40.0, f_lip=0.3, seed=11 + k) for k in range(kd)]
 
     def oracle_run(pp):
         o = oracle.System(pp.to_c(), pp.N)
@@ -440,18 +443,22 @@ def test_tail_tail_rdf_vs_oracle(oracle, eng):
             g += o.rdf(ospec)
         return g / nwin
 
-    s0 = oracle.soup(cp, p.N)
-    e = eng.Engine(p)
-    e.start([s0])
+    e = eng.Engine(ps)
+    e.start([oracle.soup(pp.to_c(), pp.N) for pp in ps])
     e.step(eq)
-    gd = e.window_forward(nwin, spec)[0]
+    gd = e.window_forward(nwin, spec)
     e.close()
     say("device done; oracle runs")
-    go = oracle_run(p)
-    go2 = oracle_run(mkparams(40.0, f_lip=0.3, seed=12))
-    err = np.linalg.norm(gd - go) / np.linalg.norm(go)
-    noise = np.linalg.norm(go2 - go) / np.linalg.norm(go)
-    say(f"tail-tail RDF: rel L2 device-oracle {err:.4f}, oracle seed 11 vs 12 {noise:.4f}, "
-        f"peak g {go.max():.3f} at r = {(np.argmax(go) + 0.5) * 3 / 64:.3f}")
+    go = np.array([oracle_run(pp) for pp in ps[:ko]])
+    md, mo = gd.mean(0), go.mean(0)
+    nrm = np.linalg.norm(mo)
+    sigma = np.sqrt(gd.var(0, ddof=1).sum()) / nrm
+    expect = sigma * np.sqrt(1.0 / kd + 1.0 / ko)
+    err = np.linalg.norm(md - mo) / nrm
+    spread = max(np.linalg.norm(g - mo) for g in go) / nrm
+    say(f"tail-tail RDF: rel L2 device mean - oracle mean {err:.4f}, expected {expect:.4f} "
+        f"(per-run sigma {sigma:.4f}); oracle runs to their mean <= {spread:.4f}; "
+        f"peak g {mo.max():.3f} at r = {(np.argmax(mo) + 0.5) * 3 / 64:.3f}")
     assert err < 0.03
-    assert err < 2.0 * noise
+    assert err < 3.0 * expect
+    assert spread < 3.0 * sigma

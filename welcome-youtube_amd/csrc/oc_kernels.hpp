@@ -981,8 +981,15 @@ __device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float 
 // scaled by a power of two).
 constexpr double kFixScale = 536870912.0;  // 2^29
 constexpr float kFixR = 0x1p-29f;           // grid unit
-constexpr float kFixR2Max = 0x1p58f;         // r2 = 1
-constexpr float kFixR2Min = 1e-24f * 0x1p58f;
+// The cutoff test on staged coordinates, in exact integer arithmetic: the
+// squared separation in grid units (|dx|, |dy| < 2^31, within a tile's
+// window < 2^31) lies in (0, 2^58), i.e. 0 < r < 1 -- the same for both
+// orientations of a pair, and for every kernel variant.
+__device__ __forceinline__ bool in_cutoff_q(uint2 xi, uint2 xj) {
+    const int32_t dx = (int32_t)(xi.x - xj.x), dy = (int32_t)(xi.y - xj.y);
+    const int64_t r2 = (int64_t)dx * dx + (int64_t)dy * dy;
+    return (uint64_t)(r2 - 1) < (1ull << 58) - 1ull;
+}
 __device__ __forceinline__ uint32_t fix_lo(double v) {  // v >= 0
     const double t = __builtin_rint(v * kFixScale);
     return (uint32_t)(t - floor(t * 0x1p-32) * 4294967296.0);
@@ -995,9 +1002,9 @@ __device__ __forceinline__ void pair_one_q(uint2 xq, float vxi, float vyi, uint3
                                            uint32_t gj, const float *ap, float gam, float sdtk, uint64_t sseed,
                                            double &ax, double &ay, float2 &e, bool energy,
                                            const InjTheta *inj = nullptr) {
-    const float dx = (float)(int32_t)(xq.x - qj.x), dy = (float)(int32_t)(xq.y - qj.y);
-    const float r2 = __builtin_fmaf(dx, dx, dy * dy);
-    if (!TEST || (r2 < kFixR2Max && r2 > kFixR2Min)) {
+    if (!TEST || in_cutoff_q(xq, qj)) {
+        const float dx = (float)(int32_t)(xq.x - qj.x), dy = (float)(int32_t)(xq.y - qj.y);
+        const float r2 = __builtin_fmaf(dx, dx, dy * dy);
         const float ir = __builtin_amdgcn_rsqf(r2);
         const float ux = dx * ir, uy = dy * ir;
         float w;
@@ -1463,12 +1470,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                     // order as the single pass, so the forces are bit-identical
                     // to it.  Phase 1: bit t of mk marks an interacting candidate t.
                     uint64_t mk = 0;
-                    for (int it = 0; it < nj; it++, j++) {
-                        const uint2 qj = XQ[j];
-                        const float dx = (float)(int32_t)(xq.x - qj.x), dy = (float)(int32_t)(xq.y - qj.y);
-                        const float r2 = __builtin_fmaf(dx, dx, dy * dy);  // the test of pair_one_q
-                        if (r2 < kFixR2Max && r2 > kFixR2Min) mk |= 1ull << it;
-                    }
+                    for (int it = 0; it < nj; it++, j++)
+                        if (in_cutoff_q(xq, XQ[j])) mk |= 1ull << it;  // the test of pair_one_q
                     // this lane's first entry in the wavefront's queue
                     const int cnt = __popcll(mk);
                     const int incl = wave_incl_scan(cnt);
@@ -1528,24 +1531,25 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                 } else {
                     //   CAPL == 1: two phases per lane: the cutoff test over
                     // the window marks the interacting candidates among the
-                    // first 64 in a 64-bit register mask (later ones are
+                    // first 64 in two 32-bit register masks (later ones are
                     // evaluated in place), then the force and noise arithmetic
-                    // runs over the marked ones only.
-                    uint64_t mk = 0;
-                    for (int it = 0; it < nj; it++, j++) {
-                        const uint2 qj = XQ[j];
-                        const float dx = (float)(int32_t)(xq.x - qj.x), dy = (float)(int32_t)(xq.y - qj.y);
-                        const float r2 = __builtin_fmaf(dx, dx, dy * dy);  // the test of pair_one_q
-                        if (r2 < kFixR2Max && r2 > kFixR2Min) {
-                            if (it < 64) {
-                                mk |= 1ull << it;
+                    // runs over the marked ones only.  The candidate counter is
+                    // the same in every lane (uniform), so a mark is one OR.
+                    uint32_t mw[2] = {0u, 0u};
+                    {
+                        const uint2 *pj = XQ + st, *const pe = XQ + st + nj;
+                        for (int it = 0; pj < pe; pj++, it++) {
+                            if (!in_cutoff_q(xq, *pj)) continue;
+                            if (it < 32) {
+                                mw[0] |= 1u << it;
+                            } else if (it < 64) {
+                                mw[1] |= 1u << (it - 32);
                             } else {
-                                float2 vj;
-                                uint32_t gj;
-                                const float *ap;
-                                cand(j, arow, vj, gj, ap);
-                                pair_one_q(xq, vi.x, vi.y, gi, qj, vj, gj, ap, gam, sdtk, sseed, ax, ay, e, ENERGY,
-                                           inj);
+                                const uint2 wv = pj[CAP], wg = pj[2 * CAP];
+                                const float2 vj = make_float2(__uint_as_float(wv.x), __uint_as_float(wv.y));
+                                const float *ap = (const float *)((const char *)arow + wg.y);
+                                pair_one_q<false>(xq, vi.x, vi.y, gi, *pj, vj, wg.x, ap, gam, sdtk, sseed, ax, ay, e,
+                                                  ENERGY, inj);
                             }
                         }
                     }
@@ -1553,7 +1557,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                     // mask as two 32-bit words (most windows fit the first)
 #pragma unroll
                     for (int h = 0; h < 2; h++) {
-                        uint32_t m = h ? (uint32_t)(mk >> 32) : (uint32_t)mk;
+                        uint32_t m = mw[h];
                         const uint2 *pb = XQ + st + 32 * h;
                         while (m) {
                             const uint2 *pj = pb + __builtin_ctz(m);
