@@ -109,11 +109,12 @@ class HipEvents:
 
 def time_pair_kernel(e, reps=50):
     """Average duration (s) of one pair-kernel launch, from HIP events recorded
-    on the stream the kernel runs on."""
+    on the stream the kernel runs on around one replay of reps back-to-back
+    launches (a graph, so no host gaps between them)."""
     ev = HipEvents()
     a, b = ev.new(), ev.new()
     stream = C.c_void_p(e.stream)
-    e.lib.oc_launch_pair(e.h, 3)
+    e.lib.oc_launch_pair(e.h, reps)  # builds the graph of reps launches, untimed
     ev.hip.hipEventRecord(a, stream)
     rc = e.lib.oc_launch_pair(e.h, reps)
     ev.hip.hipEventRecord(b, stream)

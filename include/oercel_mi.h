@@ -324,7 +324,10 @@ int oc_window_grad(oc_engine *h, int32_t nsteps, int32_t ckpt_every, const oc_rd
 /* Launch only the force kernel (pair forces with the bonded terms, as the
  * step runs it) reps times on the handle's stream, for the current state and
  * step, without synchronising: lets a caller time that kernel alone with
- * events on oc_stream(). */
+ * events on oc_stream().  The reps launches run as one captured graph, built
+ * at the first call with that count (and after any change of the engine's
+ * buffers), so a timing caller makes one untimed call with the same count
+ * first; between the launches are only the graph's own gaps. */
 int oc_launch_pair(oc_engine *h, int32_t reps);
 
 /* sizeof the ABI structs, for binding checks: out[0..3] = oc_params,

@@ -327,6 +327,31 @@ def test_graph_equals_eager_and_deterministic(eng):
     assert np.array_equal(runs[0][1], runs[1][1])
 
 
+def test_pair_timing_launches_leave_the_run_unchanged(eng):
+    """oc_launch_pair (the benchmark's standalone timing of the force kernel,
+    a replayed graph of repeated launches) between steps: the run continues
+    bit for bit as one without it, with the same events, for two counts (a
+    rebuilt graph) and after the state buffer has flipped."""
+    p = mkparams(24.0, f_lip=0.3, c_N=0.2, pchem=1, n_chem=1, seed=8)
+    runs = []
+    for timed in (False, True):
+        e = eng.Engine(p)
+        e.start()
+        e.step(7)
+        if timed:
+            for reps in (3, 3, 5):
+                e._chk(e.lib.oc_launch_pair(e.h, reps))
+        e.step(1)
+        if timed:
+            e._chk(e.lib.oc_launch_pair(e.h, 4))
+        e.step(12)
+        runs.append((e.get_state(), e.events()))
+        e.close()
+    for k in runs[0][0]:
+        assert np.array_equal(runs[0][0][k], runs[1][0][k]), k
+    assert np.array_equal(runs[0][1], runs[1][1])
+
+
 def test_ensemble_equals_single(eng):
     """Replicas of one handle are bit-identical to separate single runs."""
     ps = [mkparams(L, nsub=ns, f_lip=fl, a_W_T=aw, c_N=0.2, pchem=1, n_chem=1, seed=sd)

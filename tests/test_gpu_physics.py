@@ -110,33 +110,48 @@ def test_water_temperature_rdf_pressure(eng):
 
 
 def test_stirring_profile(eng):
-    A = 0.0015
-    p = water(L=48, stir_A=A)
-    say(f"case: stirring water L=48 A={A} N={p.N}, 600 t.u.")
-    e = eng.Engine(p)
+    """Kolmogorov flow: water at L = 48 driven by A sin(2 pi y / L).  The
+    velocity profile's sin amplitude u0 against the reference's 0.266 +- 0.005
+    (cel/engine/README.md:253), its cos part against 0, T_kin - u0^2/4
+    against 1.  Both coefficients are amplitudes of the slowest shear mode,
+    whose thermal fluctuations (variance 2 kT / N per replica) relax on
+    tau = 1 / (nu k^2) ~ 180 time units, so a time average over T_w carries a
+    standard error sqrt(2 kT / N * 2 tau / T_w); the test averages 8
+    independent replicas (seeds 7-14) over 600 time units after 300 and
+    states its bounds in that error (or the replicas' own scatter, if larger)."""
+    A, nrep = 0.0015, 8
+    ps = [water(L=48, stir_A=A, seed=7 + r) for r in range(nrep)]
+    p = ps[0]
+    say(f"case: stirring water L=48 A={A} N={p.N} x {nrep} replicas, 300 + 600 t.u.")
+    e = eng.Engine(ps)
     e.start()
     L = p.L
     k = 2 * np.pi / L
     e.step(10000)  # 300 t.u.: the flow develops on 1/(nu k^2) ~ 180 t.u.
-    fits, T, ys, vxs = [], [], [], []
-    for _ in range(100):
+    fits, T = [[] for _ in range(nrep)], []
+    for _ in range(200):
         e.step(100)
-        s = e.get_state()
-        st = e.stats()
-        assert abs(st["px"]) < 1e-8 and abs(st["py"]) < 1e-8
-        T.append(st["T_kin"])
-        vx = s["vx"] + 0.5 * p.dt * s["fx"]  # full velocity
-        M = np.c_[np.sin(k * s["y"]), np.cos(k * s["y"]), np.ones(p.N)]
-        fits.append(np.linalg.lstsq(M, vx, rcond=None)[0])
+        for r in range(nrep):
+            s = e.get_state(r)
+            st = e.stats(r)
+            assert abs(st["px"]) < 1e-8 and abs(st["py"]) < 1e-8
+            T.append(st["T_kin"])
+            vx = s["vx"] + 0.5 * p.dt * s["fx"]  # full velocity
+            M = np.c_[np.sin(k * s["y"]), np.cos(k * s["y"]), np.ones(p.N)]
+            fits[r].append(np.linalg.lstsq(M, vx, rcond=None)[0])
     e.close()
-    fits = np.array(fits)
-    u0, c1 = fits[:, 0].mean(), fits[:, 1].mean()
-    eu, ec = block_err(fits[:, 0], 10), block_err(fits[:, 1], 10)
+    per = np.array([np.mean(f, axis=0) for f in fits])  # time mean per replica
+    u0, c1 = per[:, 0].mean(), per[:, 1].mean()
+    tau, tw = 1.0 / (0.33 * k * k), 600.0
+    e_th = np.sqrt(2.0 / p.N * 2.0 * tau / tw / nrep)
+    eu = max(e_th, per[:, 0].std(ddof=1) / np.sqrt(nrep))
+    ec = max(e_th, per[:, 1].std(ddof=1) / np.sqrt(nrep))
     Tm = np.mean(T)
-    say(f"  u0 = {u0:.4f} +- {eu:.4f} (reference 0.266 +- 0.005), cos part {c1:+.4f} +- {ec:.4f}, "
-        f"nu = {A / (u0 * k * k):.3f} (reference 0.33), T_kin - u0^2/4 = {Tm - u0 ** 2 / 4:.4f} (reference 0.9963)")
+    say(f"  u0 = {u0:.4f} +- {eu:.4f} (reference 0.266 +- 0.005), cos part {c1:+.4f} +- {ec:.4f} "
+        f"(shear-mode error {e_th:.4f}), nu = {A / (u0 * k * k):.3f} (reference 0.33), "
+        f"T_kin - u0^2/4 = {Tm - u0 ** 2 / 4:.4f} (reference 0.9963)")
     assert abs(u0 - 0.266) < 4 * np.hypot(eu, 0.005)
-    assert abs(c1) < 4 * ec + 1e-3
+    assert abs(c1) < 4 * ec
     assert abs(Tm - u0 ** 2 / 4 - 1) < 0.03
 
 

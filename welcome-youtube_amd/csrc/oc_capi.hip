@@ -83,6 +83,10 @@ struct oc_engine {
     hipGraphExec_t pexec[2] = {nullptr, nullptr};
     hipGraph_t pgraph[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
+    // oc_launch_pair: the last count's launches as one graph, per state buffer
+    hipGraphExec_t texec[2] = {nullptr, nullptr};
+    hipGraph_t tgraph[2] = {nullptr, nullptr};
+    int32_t treps[2] = {0, 0};
     bool have_state = false, any_stir = false, any_chem = false, any_angle = false;
     bool any_tmpl = false;  // some replica has branched lipids (k_mol; the force kernel leaves no keys)
     bool any_rchem = false; // some replica runs the reference chemistry: the step integrates, runs it, then sorts
@@ -299,6 +303,11 @@ static void drop_graphs(oc_engine *h) {
         if (h->pgraph[k]) (void)hipGraphDestroy(h->pgraph[k]);
         h->pexec[k] = nullptr;
         h->pgraph[k] = nullptr;
+        if (h->texec[k]) (void)hipGraphExecDestroy(h->texec[k]);
+        if (h->tgraph[k]) (void)hipGraphDestroy(h->tgraph[k]);
+        h->texec[k] = nullptr;
+        h->tgraph[k] = nullptr;
+        h->treps[k] = 0;
     }
 }
 
@@ -340,7 +349,9 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false, const In
     A.evcap = h->evcap;
     A.record = h->record;
     const dim3 grid(8 * h->per_xcd);
-    // CAP (staged candidates) is 3.5x the block; the variant picks the single
+    // CAP: staged candidates (3.5x the block; 812 at 256 threads, so that seven
+    // workgroups fit a CU's LDS: tiles of ~230 beads stage ~710 at rho 3, and
+    // a tile with more runs on global memory); the variant picks the single
     // pass, the per-lane two phases or the cross-lane pair queue
     const int V = h->pair_variant;
 #define OC_PAIR(B, C, CL)                                                                        \
@@ -364,7 +375,7 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false, const In
         switch (h->pair_block) {
         case 64: OC_PAIRINJ(64, 224); break;
         case 128: OC_PAIRINJ(128, 448); break;
-        default: OC_PAIRINJ(256, 896); break;
+        default: OC_PAIRINJ(256, 812); break;
         }
 #undef OC_PAIRINJ
         return;
@@ -386,7 +397,7 @@ static void launch_pair(oc_engine *h, bool energy, bool bonded = false, const In
         if (V == 2) OC_PAIR(128, 448, 2); else if (V == 1) OC_PAIR(128, 448, 1); else OC_PAIR(128, 448, 0);
         break;
     default:
-        if (V == 2) OC_PAIR(256, 896, 2); else if (V == 1) OC_PAIR(256, 896, 1); else OC_PAIR(256, 896, 0);
+        if (V == 2) OC_PAIR(256, 812, 2); else if (V == 1) OC_PAIR(256, 812, 1); else OC_PAIR(256, 812, 0);
         break;
     }
 #undef OC_PAIR
@@ -1227,8 +1238,26 @@ OC_API int oc_forces(oc_engine *h, int64_t step, int32_t with_bonded) {
 
 OC_API int oc_launch_pair(oc_engine *h, int32_t reps) {
     if (!h || !h->have_state) return fail(OC_ESTATE, "no state");
-    for (int k = 0; k < reps; k++) launch_pair(h, false, true);
-    HIPCHK(hipGetLastError());
+    if (reps <= 0) return 0;
+    const int c = h->cur;
+    if (h->treps[c] != reps) {
+        // the launches as one graph, so that back-to-back launches are timed
+        // without the host's per-launch gaps; rebuilt when the count changes
+        if (h->texec[c]) (void)hipGraphExecDestroy(h->texec[c]);
+        if (h->tgraph[c]) (void)hipGraphDestroy(h->tgraph[c]);
+        h->texec[c] = nullptr;
+        h->tgraph[c] = nullptr;
+        h->treps[c] = 0;
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < reps; k++) launch_pair(h, false, true);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(h->stream, &g);
+        if (e != hipSuccess) return fail(OC_EHIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+        h->tgraph[c] = g;
+        HIPCHK(hipGraphInstantiate(&h->texec[c], g, nullptr, nullptr, 0));
+        h->treps[c] = reps;
+    }
+    HIPCHK(hipGraphLaunch(h->texec[c], h->stream));
     return 0;
 }
 

@@ -946,33 +946,6 @@ __device__ __forceinline__ float pair_core(float ux, float uy, float r, float rs
     return w * __builtin_fmaf(sdtk, th, __builtin_fmaf(-gam * w, rv, a));
 }
 
-// gj: the candidate's gid, ap: its a_ij (read when the pair interacts)
-__device__ __forceinline__ void pair_one(double xi, double yi, float vxi, float vyi, uint32_t gi, double2 pj,
-                                         float2 vj, uint32_t gj, const float *ap, double sxs, double sys, float gam,
-                                         float sdtk, uint64_t sseed, double &ax, double &ay, float2 &e, bool energy,
-                                         const InjTheta *inj = nullptr) {
-    // FP64 image difference, then FP32 force arithmetic (fused multiply-adds).
-    // Both orientations of a pair run the same operations on negated
-    // operands, so f_ji = -f_ij exactly.
-    const float dx = (float)((xi - pj.x) - sxs), dy = (float)((yi - pj.y) - sys);
-    const float r2 = __builtin_fmaf(dx, dx, dy * dy);
-    if (r2 < 1.0f && r2 > 1e-24f) {
-        const float ir = __builtin_amdgcn_rsqf(r2);
-        const float ux = dx * ir, uy = dy * ir;
-        float w;
-        const float a = *ap;
-        const float fs = pair_core(ux, uy, r2 * ir, 1.0f, vxi, vyi, gi, a, vj, gj, gam, sdtk, sseed, w, inj);
-        ax += (double)(fs * ux);
-        ay += (double)(fs * uy);
-        if (energy) {
-            // e.x: half the pair's energy a/2 (1-r)^2; e.y: half its
-            // conservative virial r . F^C = a (1-r) r (both orientations add)
-            e.x += 0.25f * a * w * w;
-            e.y += 0.5f * a * w * (1.0f - w);
-        }
-    }
-}
-
 // Staged (fixed-point) coordinates: the low 32 bits of round(x * 2^29) plus
 // the periodic image shift (a multiple of round(L * 2^29)), so a difference
 // of two staged coordinates is the exact integer difference for |dx| < 4 and
@@ -995,8 +968,11 @@ __device__ __forceinline__ uint32_t fix_lo(double v) {  // v >= 0
     return (uint32_t)(t - floor(t * 0x1p-32) * 4294967296.0);
 }
 
-// pair_one on staged coordinates: xq, qj the fixed-point coordinates of i and j
-// TEST: the cutoff test (false: the caller has made it, the pair interacts)
+// Pair term of i and j, added to (ax, ay) in FP64, from their fixed-point
+// coordinates xq, qj (FP32 arithmetic, fused multiply-adds; both orientations
+// of a pair run the same operations on negated operands, so f_ji = -f_ij
+// exactly).  TEST: the cutoff test (false: the caller has made it, the pair
+// interacts)
 template <bool TEST = true>
 __device__ __forceinline__ void pair_one_q(uint2 xq, float vxi, float vyi, uint32_t gi, uint2 qj, float2 vj,
                                            uint32_t gj, const float *ap, float gam, float sdtk, uint64_t sseed,
@@ -1221,8 +1197,16 @@ __device__ __forceinline__ void chem_formed(const PairArgs &A, const RepStep &S,
 __host__ __device__ constexpr int pair_max_rows(int block) { return 2 * block; }
 constexpr int64_t kPairSlotLimit = int64_t(1) << 28;  // slots the staging packs (bits 0-27)
 
+// Occupancy: the kernel issues VALU work at nearly every cycle, and one
+// wavefront per SIMD more (7 against 6 at BLOCK 256: at most 72 VGPRs and
+// 22.8 KB of LDS a workgroup) hides more of the LDS latency of the candidate
+// loop (k_pair 60.3 -> 58.6 us at N = 1M, rho 3); the angle terms and the
+// injected noise would spill at 72 VGPRs and the pair queue's LDS allows five,
+// so those kernels keep the compiler's choice.
 template <int BLOCK, int CAP, int CAPL, bool ENERGY, bool ANG, bool INJ = false>
-__global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ANG || ENERGY || INJ || CAPL == 2 ? 1 : 7)))
+void k_pair(
+    PairArgs A) {
     const InjTheta *inj = INJ ? &A.inj : nullptr;  // caller-supplied pair noise (single pass only)
     static_assert(CAP < 65536, "staged indices are 16-bit");
     constexpr int kMaxRows = pair_max_rows(BLOCK);
@@ -1340,13 +1324,14 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
         if (tid == 0) s_rs[nr] = (uint16_t)total;
     }
     __syncthreads();
+    // the x / y image shifts in fixed point: columns 0 and 2, rows below 0 and
+    // past ncy - 1 (the staged coordinates, and the global-memory path's)
+    const uint32_t fL = fix_lo(L);
+    const uint32_t fx0 = sxc[0] < 0.0 ? 0u - fL : (sxc[0] > 0.0 ? fL : 0u);
+    const uint32_t fx2 = sxc[2] < 0.0 ? 0u - fL : (sxc[2] > 0.0 ? fL : 0u);
     if (staged) {
         // every load of a thread is issued before its first LDS store
         constexpr int kStgPer = (CAP + BLOCK - 1) / BLOCK;
-        // the x / y image shifts in fixed point: columns 0 and 2, rows below 0 and past ncy - 1
-        const uint32_t fL = fix_lo(L);
-        const uint32_t fx0 = sxc[0] < 0.0 ? 0u - fL : (sxc[0] > 0.0 ? fL : 0u);
-        const uint32_t fx2 = sxc[2] < 0.0 ? 0u - fL : (sxc[2] > 0.0 ? fL : 0u);
         double2 sx[kStgPer];
         float2 sv[kStgPer];
         uint32_t sg[kStgPer];
@@ -1441,7 +1426,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             if (bond) bond_links(slot, A.mp, A.lnk, A.prv, B);
             const double2 vh = !ENERGY && A.key && act ? A.v[slot] : make_double2(0.0, 0.0);
             const double2 pi = XY[slot];
-            const uint2 xq = stg ? XQ[is] : make_uint2(0u, 0u);
+            // fixed-point coordinates of the bead: staged, or from its position
+            // (the global-memory path runs the same arithmetic as the staged one)
+            const uint2 xq = stg ? XQ[is] : make_uint2(fix_lo(pi.x), fix_lo(pi.y));
             float2 vi;
             uint32_t gi;
             const float *ai;  // s_a + bead i's type
@@ -1571,16 +1558,20 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                     }
                 }
             } else if (act) {
-                // global memory: per column and periodic piece (image shifts)
+                // global memory: per column and periodic piece (image shifts),
+                // candidates in fixed point as the staging forms them, so a
+                // pair's two orientations cancel exactly whichever path each
+                // side of it takes
                 for (int c = 0; c < 3; c++) {
+                    const uint32_t sxq = c == 0 ? fx0 : (c == 2 ? fx2 : 0u);
                     // window split at the periodic seams: rows < 0 (image -L),
                     // 0..ncy-1, >= ncy (image +L)
                     for (int pq = 0; pq < 3; pq++) {
                         int pa, pb, wrap;
-                        double sy;
-                        if (pq == 0) { pa = klo; pb = min(khi, -1); sy = -L; wrap = ncy; }
-                        else if (pq == 1) { pa = max(klo, 0); pb = min(khi, ncy - 1); sy = 0.0; wrap = 0; }
-                        else { pa = max(klo, ncy); pb = khi; sy = L; wrap = -ncy; }
+                        uint32_t syq;
+                        if (pq == 0) { pa = klo; pb = min(khi, -1); syq = 0u - fL; wrap = ncy; }
+                        else if (pq == 1) { pa = max(klo, 0); pb = min(khi, ncy - 1); syq = 0u; wrap = 0; }
+                        else { pa = max(klo, ncy); pb = khi; syq = fL; wrap = -ncy; }
                         if (pa > pb) continue;
                         const int ja = cs[colb[c] + pa + wrap], jb = cs[colb[c] + pb + wrap + 1];
                         for (int jj = ja; jj < jb; jj++)
@@ -1589,8 +1580,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
                             uint32_t gj;
                             const float *ap;
                             cand(jj, arow, vj, gj, ap);
-                            pair_one(pi.x, pi.y, vi.x, vi.y, gi, XY[jj], vj, gj, ap, sxc[c], sy, gam, sdtk, sseed, ax,
-                                     ay, e, ENERGY, inj);
+                            const double2 pj = XY[jj];
+                            const uint2 qj = make_uint2(fix_lo(pj.x) + sxq, fix_lo(pj.y) + syq);
+                            pair_one_q(xq, vi.x, vi.y, gi, qj, vj, gj, ap, gam, sdtk, sseed, ax, ay, e, ENERGY, inj);
                         }
                     }
                 }
