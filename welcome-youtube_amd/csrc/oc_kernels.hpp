@@ -958,10 +958,13 @@ constexpr float kFixR = 0x1p-29f;           // grid unit
 // squared separation in grid units (|dx|, |dy| < 2^31, within a tile's
 // window < 2^31) lies in (0, 2^58), i.e. 0 < r < 1 -- the same for both
 // orientations of a pair, and for every kernel variant.
-__device__ __forceinline__ bool in_cutoff_q(uint2 xi, uint2 xj) {
+// m1 = -1, passed in a register by the loops that test many candidates, so
+// that it is the addend of the first 64-bit multiply-add (two instructions
+// for r^2 - 1 instead of three).
+__device__ __forceinline__ bool in_cutoff_q(uint2 xi, uint2 xj, int64_t m1 = -1) {
     const int32_t dx = (int32_t)(xi.x - xj.x), dy = (int32_t)(xi.y - xj.y);
-    const int64_t r2 = (int64_t)dx * dx + (int64_t)dy * dy;
-    return (uint64_t)(r2 - 1) < (1ull << 58) - 1ull;
+    const int64_t r2m1 = (int64_t)dy * dy + ((int64_t)dx * dx + m1);
+    return (uint64_t)r2m1 < (1ull << 58) - 1ull;
 }
 __device__ __forceinline__ uint32_t fix_lo(double v) {  // v >= 0
     const double t = __builtin_rint(v * kFixScale);
@@ -1524,20 +1527,27 @@ void k_pair(
                     // the same in every lane (uniform), so a mark is one OR.
                     uint32_t mw[2] = {0u, 0u};
                     {
-                        const uint2 *pj = XQ + st, *const pe = XQ + st + nj;
-                        for (int it = 0; pj < pe; pj++, it++) {
+                        // the marks without branches: one loop per mask word,
+                        // candidates past the 64th evaluated in place
+                        const uint2 *pj = XQ + st;
+                        const int n0 = min(nj, 32), n1 = min(nj, 64);
+                        int64_t m1 = -1;  // in a register (an empty asm hides the constant)
+                        asm volatile("" : "+v"(m1));
+                        int it = 0;
+                        for (; it + 1 < n0; it += 2, pj += 2)
+                            mw[0] |= ((uint32_t)in_cutoff_q(xq, pj[0], m1) | (uint32_t)in_cutoff_q(xq, pj[1], m1) << 1) << it;
+                        for (; it < n0; it++, pj++) mw[0] |= (uint32_t)in_cutoff_q(xq, *pj, m1) << it;
+                        for (; it + 1 < n1; it += 2, pj += 2)
+                            mw[1] |= ((uint32_t)in_cutoff_q(xq, pj[0], m1) | (uint32_t)in_cutoff_q(xq, pj[1], m1) << 1)
+                                     << (it - 32);
+                        for (; it < n1; it++, pj++) mw[1] |= (uint32_t)in_cutoff_q(xq, *pj, m1) << (it - 32);
+                        for (; it < nj; it++, pj++) {
                             if (!in_cutoff_q(xq, *pj)) continue;
-                            if (it < 32) {
-                                mw[0] |= 1u << it;
-                            } else if (it < 64) {
-                                mw[1] |= 1u << (it - 32);
-                            } else {
-                                const uint2 wv = pj[CAP], wg = pj[2 * CAP];
-                                const float2 vj = make_float2(__uint_as_float(wv.x), __uint_as_float(wv.y));
-                                const float *ap = (const float *)((const char *)arow + wg.y);
-                                pair_one_q<false>(xq, vi.x, vi.y, gi, *pj, vj, wg.x, ap, gam, sdtk, sseed, ax, ay, e,
-                                                  ENERGY, inj);
-                            }
+                            const uint2 wv = pj[CAP], wg = pj[2 * CAP];
+                            const float2 vj = make_float2(__uint_as_float(wv.x), __uint_as_float(wv.y));
+                            const float *ap = (const float *)((const char *)arow + wg.y);
+                            pair_one_q<false>(xq, vi.x, vi.y, gi, *pj, vj, wg.x, ap, gam, sdtk, sseed, ax, ay, e,
+                                              ENERGY, inj);
                         }
                     }
                     // pass 2: the marked candidates in window order, the
