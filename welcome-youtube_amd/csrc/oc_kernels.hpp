@@ -966,9 +966,12 @@ __device__ __forceinline__ bool in_cutoff_q(uint2 xi, uint2 xj, int64_t m1 = -1)
     const int64_t r2m1 = (int64_t)dy * dy + ((int64_t)dx * dx + m1);
     return (uint64_t)r2m1 < (1ull << 58) - 1ull;
 }
-__device__ __forceinline__ uint32_t fix_lo(double v) {  // v >= 0
-    const double t = __builtin_rint(v * kFixScale);
-    return (uint32_t)(t - floor(t * 0x1p-32) * 4294967296.0);
+// Low 32 bits of round(v * 2^29) (ties to even), 0 <= v * 2^29 < 2^52: the
+// fused multiply-add v * 2^29 + 2^52 rounds once, to the integer grid of
+// [2^52, 2^53), so its mantissa field is exactly round(v * 2^29), and its low
+// word the result -- one instruction.
+__device__ __forceinline__ uint32_t fix_lo(double v) {
+    return (uint32_t)__double_as_longlong(__builtin_fma(v, kFixScale, 0x1p52));
 }
 
 // Pair term of i and j, added to (ax, ay) in FP64, from their fixed-point
@@ -998,13 +1001,23 @@ __device__ __forceinline__ void pair_one_q(uint2 xq, float vxi, float vyi, uint3
     }
 }
 
-// harmonic bond i -> j (d = minimum image of p_j - p_i), the arithmetic of
-// cel/engine/oercel.c:1280-1289: force on j is +fb*d, on i -fb*d
-__device__ __forceinline__ void bond_vec(const RepInfo &R, double2 pi, double2 pj, double &dx, double &dy, double &r) {
+// harmonic bond i -> j (d = minimum image of p_j - p_i), the force of
+// cel/engine/oercel.c:1280-1289, on j +fb*d and on i -fb*d with fb = -k_b
+// (r - r0) / r.  The length comes from the hardware's FP64 reciprocal square
+// root refined by two Newton steps (relative error ~1e-15, against a sqrt and
+// a division of ~25 instructions): ir = 1 / r, r = r^2 ir.
+__device__ __forceinline__ void bond_vec(const RepInfo &R, double2 pi, double2 pj, double &dx, double &dy, double &r,
+                                         double &ir) {
     dx = minimg(pj.x - pi.x, R.L, R.halfL);
     dy = minimg(pj.y - pi.y, R.L, R.halfL);
-    r = sqrt(dx * dx + dy * dy);
+    const double r2 = dx * dx + dy * dy, h = 0.5 * r2;
+    double y = __builtin_amdgcn_rsq(r2);
+    y = y * __builtin_fma(-h * y, y, 1.5);
+    y = y * __builtin_fma(-h * y, y, 1.5);
+    ir = y;
+    r = r2 * y;
 }
+__device__ __forceinline__ double bond_fb(const RepInfo &R, double r, double ir) { return -R.kb * (r - R.r0) * ir; }
 
 // Links and partner data of bead s for its bonded terms, loaded in two
 // rounds: bond_links at the start of the bead (the pair loop hides their
@@ -1030,11 +1043,11 @@ __device__ __forceinline__ void bond_partners(const double2 *__restrict__ xy, co
 }
 
 // Bonded force on bead s in gather form: every bond and angle term that s
-// takes part in is evaluated whole, with the arithmetic of
-// chain_forces_linear (cel/engine/oercel.c:1267-1308), and s adds its share,
-// bonds along the chain first, then angles -- the order in which a per-chain
-// pass adds them to each bead, so the result is the same bit for bit and the
-// two beads of a bond receive exactly opposite forces.  A chemistry bond is
+// takes part in is evaluated whole, with the terms of chain_forces_linear
+// (cel/engine/oercel.c:1267-1308; bond lengths as bond_vec forms them), and s
+// adds its share, bonds along the chain first, then angles -- the order in
+// which a per-chain pass adds them to each bead, so both beads of a bond
+// evaluate it on the same operands and receive exactly opposite forces.  A chemistry bond is
 // the bond from the smaller gid to the larger.  mb: largest bond length at s.
 // The angle terms (ANG: some replica has k_angle != 0; each replica's own
 // k_angle applies) load the beads two links away here.
@@ -1046,10 +1059,10 @@ __device__ __forceinline__ void bonded_gather(const RepInfo &R, double2 ps, uint
     if (m < 0) {
         if (n1 < 0) return;
         const bool lo = (gs & kGidMask) < (B.gn & kGidMask);
-        double dx, dy, r;
-        bond_vec(R, lo ? ps : B.pn, lo ? B.pn : ps, dx, dy, r);
+        double dx, dy, r, ir;
+        bond_vec(R, lo ? ps : B.pn, lo ? B.pn : ps, dx, dy, r, ir);
         mb = fmax(mb, r);
-        const double fb = -R.kb * (r - R.r0) / r;
+        const double fb = bond_fb(R, r, ir);
         if (lo) {
             fx -= fb * dx;
             fy -= fb * dy;
@@ -1062,18 +1075,18 @@ __device__ __forceinline__ void bonded_gather(const RepInfo &R, double2 ps, uint
     if (!ANG) {
         // bonds to the previous and the next chain bead
         if (m > 0 && B.p1 >= 0) {
-            double dx, dy, r;
-            bond_vec(R, B.pp, ps, dx, dy, r);
+            double dx, dy, r, ir;
+            bond_vec(R, B.pp, ps, dx, dy, r, ir);
             mb = fmax(mb, r);
-            const double fb = -R.kb * (r - R.r0) / r;
+            const double fb = bond_fb(R, r, ir);
             fx += fb * dx;
             fy += fb * dy;
         }
         if (n1 >= 0) {
-            double dx, dy, r;
-            bond_vec(R, ps, B.pn, dx, dy, r);
+            double dx, dy, r, ir;
+            bond_vec(R, ps, B.pn, dx, dy, r, ir);
             mb = fmax(mb, r);
-            const double fb = -R.kb * (r - R.r0) / r;
+            const double fb = bond_fb(R, r, ir);
             fx -= fb * dx;
             fy -= fb * dy;
         }
@@ -1093,22 +1106,22 @@ __device__ __forceinline__ void bonded_gather(const RepInfo &R, double2 ps, uint
     c[4] = (ang && c[3] >= 0) ? lnk[c[3]] : -1;
     if (c[0] >= 0) p[0] = xy[c[0]];
     if (c[4] >= 0) p[4] = xy[c[4]];
-    double bx[4], by[4], bl[4];
+    double bx[4], by[4], bl[4], bi[4];
     bool has[4];
 #pragma unroll
     for (int b = 0; b < 4; b++) {
         has[b] = c[b] >= 0 && c[b + 1] >= 0;
-        if (has[b]) bond_vec(R, p[b], p[b + 1], bx[b], by[b], bl[b]);
+        if (has[b]) bond_vec(R, p[b], p[b + 1], bx[b], by[b], bl[b], bi[b]);
     }
     if (has[1]) {  // s is the second bead of bond 1
         mb = fmax(mb, bl[1]);
-        const double fb = -R.kb * (bl[1] - R.r0) / bl[1];
+        const double fb = bond_fb(R, bl[1], bi[1]);
         fx += fb * bx[1];
         fy += fb * by[1];
     }
     if (has[2]) {  // s is the first bead of bond 2
         mb = fmax(mb, bl[2]);
-        const double fb = -R.kb * (bl[2] - R.r0) / bl[2];
+        const double fb = bond_fb(R, bl[2], bi[2]);
         fx -= fb * bx[2];
         fy -= fb * by[2];
     }
