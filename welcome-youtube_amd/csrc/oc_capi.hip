@@ -541,7 +541,7 @@ static void launch_sort(oc_engine *h, bool integ, int step_delta, bool keys_read
                                              h->d_tcount, h->d_locst, h->d_tsrc, h->d_tslot, h->d_err, nt);
     const int nch = (nt + kTScanChunk - 1) / kTScanChunk, per = nch > 1 ? 8 : (nt + 1023) / 1024;
     if (nch > 1) k_tsum<<<nch, 1024, 0, h->stream>>>(h->d_tcount, nt, per, h->d_tpart);
-    k_tscan<<<nch, 1024, 0, h->stream>>>(h->d_ri, h->d_rs, h->nrep, h->d_tcount, nt, per,
+    k_tscan<<<nch + (step_delta ? 1 : 0), 1024, 0, h->stream>>>(h->d_ri, h->d_rs, h->nrep, h->d_tcount, nt, per,
                                          nch > 1 ? h->d_tpart : nullptr, h->d_toff, h->d_moved, h->d_rmax, step_delta);
     // the placement lists the precursor chemistry's roles of the replicas whose
     // check it is, and clears every bead's decision flag

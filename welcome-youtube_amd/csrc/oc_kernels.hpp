@@ -663,9 +663,10 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
 // consecutive counts a thread, loaded together).  With more than one chunk,
 // part holds the chunk totals (k_tsum) and each block adds those of the
 // chunks before it.
-// Block 0 also clears the displacement flag and maxima and, when
-// step_delta > 0, sets the next step's per-replica values (the step counter
-// advance of the step sequence).
+// Block 0 also clears the displacement flag and maxima; when step_delta > 0
+// one more block (blockIdx.x == number of chunks) sets the next step's
+// per-replica values (the step counter advance of the step sequence) at the
+// same time as the scan runs.
 constexpr int kTScanChunk = 1024 * 8;  // largest chunk
 __global__ __launch_bounds__(1024) void k_tsum(const int32_t *__restrict__ tcount, int nt, int per,
                                                int32_t *__restrict__ part) {
@@ -690,6 +691,13 @@ __global__ __launch_bounds__(1024) void k_tscan(const RepInfo *__restrict__ ri, 
                                                 uint32_t *__restrict__ moved, int2 *__restrict__ rmax,
                                                 int step_delta) {
     __shared__ int wsum[16];
+    const int nch = (nt + 1024 * per - 1) / (1024 * max(per, 1));  // scan blocks
+    if ((int)blockIdx.x == nch) {  // the step's per-replica values
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        if (step_delta)
+            for (int r = wv; r < nrep; r += 16) step_params_wave(ri[r], rs[r], r, lane, step_delta, -1, nullptr);
+        return;
+    }
     const int b0 = (blockIdx.x * 1024 + threadIdx.x) * per;
     int c8[8];
     int loc = 0;
@@ -717,10 +725,6 @@ __global__ __launch_bounds__(1024) void k_tscan(const RepInfo *__restrict__ ri, 
     if (blockIdx.x != 0) return;
     if (threadIdx.x == 0) *moved = 0u;
     for (int r = threadIdx.x; r < nrep; r += blockDim.x) rmax[r] = make_int2(0, 0);
-    if (step_delta) {
-        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int r = wv; r < nrep; r += 16) step_params_wave(ri[r], rs[r], r, lane, step_delta, -1, nullptr);
-    }
 }
 
 struct StateP {
