@@ -107,7 +107,7 @@ class HipEvents:
         return t.value
 
 
-def time_pair_kernel(e, reps=50):
+def time_pair_kernel(e, reps=200):
     """Average duration (s) of one pair-kernel launch, from HIP events recorded
     on the stream the kernel runs on around one replay of reps back-to-back
     launches (a graph, so no host gaps between them)."""
