@@ -1856,7 +1856,8 @@ __device__ __forceinline__ double dist2(const RepInfo &R, double2 p, double2 q) 
 }
 
 constexpr int kChemGroup = 4;  // lanes per entry
-constexpr int kChemK = 2;      // candidates per lane per round
+constexpr int kChemCls = 8;    // class bytes a lane loads per round
+constexpr int kChemPos = 2;    // positions of matching candidates a lane loads per round
 constexpr int kChemGrid = 1 << 16;  // wavefronts walking the segments of the check's list
 
 // The check's list for a state that was not just placed (a chemistry check
@@ -1916,51 +1917,71 @@ __device__ __forceinline__ void chem_search(const RepInfo &R, const RepStep &S, 
     uint64_t bkey = ~0ull;  // (bits of u) of the best accepted draw
     bg = 0xffffffffu;
     bsel = -1;
-    for (int k0 = gl; k0 < W.total; k0 += kChemGroup * kChemK) {
-        int jc[kChemK];
-        uint8_t fc[kChemK];
+    // candidate kk of the window (its six pieces in order)
+    auto win_at = [&](int kk) {
+        int j = -1;
 #pragma unroll
-        for (int i = 0; i < kChemK; i++) {
-            int kk = k0 + i * kChemGroup, j = -1;
-            if (kk < W.total) {
-#pragma unroll
-                for (int q = 0; q < 6; q++) {
-                    if (j < 0 && kk < W.cnt[q]) j = W.first[q] + kk;
-                    kk -= W.cnt[q];
-                }
-            }
-            jc[i] = j == s ? -1 : j;
-            fc[i] = jc[i] >= 0 ? cls[jc[i]] : 0;
+        for (int q = 0; q < 6; q++) {
+            if (j < 0 && kk < W.cnt[q]) j = W.first[q] + kk;
+            kk -= W.cnt[q];
         }
-        double2 pc[kChemK];
-        uint32_t gc[kChemK];
+        return j;
+    };
+    // A round: the class bytes of up to kChemCls candidates of this lane
+    // (loaded together), then the positions (and gids) of the ones of the
+    // searched class, kChemPos at a time -- one load round each for a window
+    // of up to kChemGroup * kChemCls candidates.
+    for (int k0 = gl; k0 < W.total; k0 += kChemGroup * kChemCls) {
+        uint8_t fc[kChemCls];
 #pragma unroll
-        for (int i = 0; i < kChemK; i++)  // every load of the round issued before the first is used
-            if (fc[i] & want) {
-                pc[i] = xy[jc[i]];
-                if (!conv) gc[i] = gt[jc[i]] & kGidMask;
+        for (int i = 0; i < kChemCls; i++) {
+            const int kk = k0 + i * kChemGroup;
+            const int j = kk < W.total ? win_at(kk) : -1;
+            fc[i] = j >= 0 && j != s ? cls[j] : 0;
+        }
+        uint32_t mk = 0;
+#pragma unroll
+        for (int i = 0; i < kChemCls; i++) mk |= (fc[i] & want) ? 1u << i : 0u;
+        while (mk) {
+            int jm[kChemPos];
+            double2 pc[kChemPos];
+            uint32_t gc[kChemPos];
+#pragma unroll
+            for (int m = 0; m < kChemPos; m++) {
+                jm[m] = -1;
+                if (mk) {
+                    jm[m] = win_at(k0 + __builtin_ctz(mk) * kChemGroup);
+                    mk &= mk - 1;
+                }
             }
 #pragma unroll
-        for (int i = 0; i < kChemK; i++) {
-            if (!(fc[i] & want)) continue;
-            if (conv) {
-                if (dist2(R, p, pc[i]) < R.rc2) found = true;
-            } else if (dist2(R, p, pc[i]) < R.rf2) {
-                // draws in [0,1) order as their bits
-                const uint32_t gg = gc[i];
-                double u;
-                if (inj.fk) {  // the caller's formation uniform of the pair
-                    const int64_t q = inj_find(inj.fk, inj.nf, g, gg);
-                    if (q < 0) atomicOr(inj.err, ERR_INJ);
-                    u = q < 0 ? 2.0 : inj.fv[q];
-                } else {
-                    u = pair_u01(S.kform, g, gg);
+            for (int m = 0; m < kChemPos; m++)  // every load of the round issued before the first is used
+                if (jm[m] >= 0) {
+                    pc[m] = xy[jm[m]];
+                    if (!conv) gc[m] = gt[jm[m]] & kGidMask;
                 }
-                const uint64_t ku = (uint64_t)__double_as_longlong(u);
-                if (u < R.p_form && (ku < bkey || (ku == bkey && gg < bg))) {
-                    bkey = ku;
-                    bg = gg;
-                    bsel = jc[i];
+#pragma unroll
+            for (int m = 0; m < kChemPos; m++) {
+                if (jm[m] < 0) continue;
+                if (conv) {
+                    if (dist2(R, p, pc[m]) < R.rc2) found = true;
+                } else if (dist2(R, p, pc[m]) < R.rf2) {
+                    // draws in [0,1) order as their bits
+                    const uint32_t gg = gc[m];
+                    double u;
+                    if (inj.fk) {  // the caller's formation uniform of the pair
+                        const int64_t q = inj_find(inj.fk, inj.nf, g, gg);
+                        if (q < 0) atomicOr(inj.err, ERR_INJ);
+                        u = q < 0 ? 2.0 : inj.fv[q];
+                    } else {
+                        u = pair_u01(S.kform, g, gg);
+                    }
+                    const uint64_t ku = (uint64_t)__double_as_longlong(u);
+                    if (u < R.p_form && (ku < bkey || (ku == bkey && gg < bg))) {
+                        bkey = ku;
+                        bg = gg;
+                        bsel = jm[m];
+                    }
                 }
             }
         }
