@@ -1856,6 +1856,7 @@ __device__ __forceinline__ double dist2(const RepInfo &R, double2 p, double2 q) 
 }
 
 constexpr int kChemGroup = 4;  // lanes per entry
+constexpr int kChemGroupWide = 2;  // lanes per entry in a segment of more than 64 / kChemGroup entries
 constexpr int kChemCls = 8;    // class bytes a lane loads per round
 constexpr int kChemPos = 2;    // positions of matching candidates a lane loads per round
 constexpr int kChemGrid = 1 << 16;  // wavefronts walking the segments of the check's list
@@ -1898,13 +1899,14 @@ __global__ __launch_bounds__(kBeadBlock) void k_chem_list(const RepInfo *__restr
     if (threadIdx.x == 0) CL.seg[blockIdx.x] = ChemSeg{base, s_n, r, 0};
 }
 
-// Search of a listed bead s (gid g) at p by the kChemGroup lanes of its group
+// Search of a listed bead s (gid g) at p by the GRP lanes of its group
 // (gl: the lane in the group; every lane returns the group's result).  conv:
 // any lipid tail within r_conv.  Otherwise (a free T): the free T within
 // r_form with the smallest symmetric draw pair_u01(kform) below p_form (ties:
 // the smaller gid), as its slot (-1: none) and gid.  The window's candidates
 // are dealt round robin; a lane reads each candidate's class byte, and the
 // position (and gid) only of a candidate of the class searched for.
+template <int GRP>
 __device__ __forceinline__ void chem_search(const RepInfo &R, const RepStep &S, const int32_t *__restrict__ cs,
                                             const double2 *__restrict__ xy, const uint32_t *__restrict__ gt,
                                             const uint8_t *__restrict__ cls, const ChemInj &inj, int s, uint32_t g,
@@ -1930,12 +1932,12 @@ __device__ __forceinline__ void chem_search(const RepInfo &R, const RepStep &S, 
     // A round: the class bytes of up to kChemCls candidates of this lane
     // (loaded together), then the positions (and gids) of the ones of the
     // searched class, kChemPos at a time -- one load round each for a window
-    // of up to kChemGroup * kChemCls candidates.
-    for (int k0 = gl; k0 < W.total; k0 += kChemGroup * kChemCls) {
+    // of up to GRP * kChemCls candidates.
+    for (int k0 = gl; k0 < W.total; k0 += GRP * kChemCls) {
         uint8_t fc[kChemCls];
 #pragma unroll
         for (int i = 0; i < kChemCls; i++) {
-            const int kk = k0 + i * kChemGroup;
+            const int kk = k0 + i * GRP;
             const int j = kk < W.total ? win_at(kk) : -1;
             fc[i] = j >= 0 && j != s ? cls[j] : 0;
         }
@@ -1950,7 +1952,7 @@ __device__ __forceinline__ void chem_search(const RepInfo &R, const RepStep &S, 
             for (int m = 0; m < kChemPos; m++) {
                 jm[m] = -1;
                 if (mk) {
-                    jm[m] = win_at(k0 + __builtin_ctz(mk) * kChemGroup);
+                    jm[m] = win_at(k0 + __builtin_ctz(mk) * GRP);
                     mk &= mk - 1;
                 }
             }
@@ -1986,11 +1988,11 @@ __device__ __forceinline__ void chem_search(const RepInfo &R, const RepStep &S, 
             }
         }
     }
-    for (int o = kChemGroup / 2; o > 0; o >>= 1) {
-        found |= __shfl_xor((int)found, o, kChemGroup) != 0;
-        const uint64_t ok = __shfl_xor(bkey, o, kChemGroup);
-        const uint32_t og = __shfl_xor(bg, o, kChemGroup);
-        const int32_t oj = __shfl_xor(bsel, o, kChemGroup);
+    for (int o = GRP / 2; o > 0; o >>= 1) {
+        found |= __shfl_xor((int)found, o, GRP) != 0;
+        const uint64_t ok = __shfl_xor(bkey, o, GRP);
+        const uint32_t og = __shfl_xor(bg, o, GRP);
+        const int32_t oj = __shfl_xor(bsel, o, GRP);
         if (ok < bkey || (ok == bkey && og < bg)) {
             bkey = ok;
             bg = og;
@@ -2001,8 +2003,8 @@ __device__ __forceinline__ void chem_search(const RepInfo &R, const RepStep &S, 
 
 // The check (own definition), decided from the state at its start for the
 // listed beads and applied at once, one wavefront per segment, kChemGroup
-// lanes per entry (the segment header and its inline entries are loaded
-// together):
+// lanes per entry (kChemGroupWide in a large segment; the segment header and
+// its inline entries are loaded together):
 //   - a break (the bond's smaller gid drew it): both link words cleared;
 //   - a searching P with a lipid tail within r_conv becomes T;
 //   - a searching free T with a proposal writes it into its link word
@@ -2011,6 +2013,48 @@ __device__ __forceinline__ void chem_search(const RepInfo &R, const RepStep &S, 
 // The searches read only what the check does not change -- the class bytes,
 // positions and gids -- so the writes do not race with them.  Every write has
 // one writer; events are sorted when read.
+// The entries of segment k, GRP lanes each (E: entry lane / GRP of the
+// first round when preloaded).
+template <int GRP>
+__device__ __forceinline__ void chem_segment(const RepInfo &R, const RepStep &S, const ChemSeg &G, int k, ChemList CL,
+                                             bool preloaded, ChemEnt E, const int32_t *__restrict__ cs,
+                                             const double2 *__restrict__ xy, uint32_t *__restrict__ gt,
+                                             int32_t *__restrict__ lnk, const uint8_t *__restrict__ cls,
+                                             Event *__restrict__ ev, unsigned long long *__restrict__ evn,
+                                             int64_t evcap, int record, uint32_t *__restrict__ err,
+                                             const ChemInj &inj) {
+    constexpr int per = 64 / GRP;
+    const int lane = (int)threadIdx.x & 63, gl = lane & (GRP - 1), ge = lane / GRP;
+    for (int e = ge; e < G.cnt; e += per) {
+        if (!preloaded || e != ge) E = *chem_ent(CL, k, G.base, e);
+        const int s = E.sr & ((1 << kRoleShift) - 1), role = (int)((uint32_t)E.sr >> kRoleShift);
+        const uint32_t g = E.w & kGidMask, ty = E.w >> kGidBits;
+        if (!(role & 2)) {  // a break (of a bead in no search)
+            if (gl == 0) {
+                const int pq = lnk[s];
+                const uint32_t gp = gt[pq] & kGidMask;
+                lnk[s] = -1;
+                lnk[pq] = -1;
+                if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_BREAK, min(g, gp), max(g, gp), G.rep);
+            }
+            continue;
+        }
+        bool found;
+        int32_t b;
+        uint32_t bg;
+        chem_search<GRP>(R, S, cs, xy, gt, cls, inj, s, g, make_double2(E.x, E.y), ty == OC_N, gl, found, b, bg);
+        if (gl != 0) continue;
+        if (ty == OC_N) {
+            if (found) {
+                gt[s] = g | ((uint32_t)OC_T << kGidBits);
+                if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_CONV, g, g, G.rep);
+            }
+        } else if (b >= 0) {
+            lnk[s] = chem_prop_code(b);
+        }
+    }
+}
+
 __global__ __launch_bounds__(64) void k_chem_check(const RepInfo *__restrict__ ri, const RepStep *__restrict__ rs,
                                                    const int32_t *__restrict__ cs, const double2 *__restrict__ xy,
                                                    uint32_t *__restrict__ gt, int32_t *__restrict__ lnk,
@@ -2018,8 +2062,6 @@ __global__ __launch_bounds__(64) void k_chem_check(const RepInfo *__restrict__ r
                                                    Event *__restrict__ ev, unsigned long long *__restrict__ evn,
                                                    int64_t evcap, int record, uint32_t *__restrict__ err,
                                                    ChemInj inj) {
-    const int gl = threadIdx.x & (kChemGroup - 1);
-    const int ge = (int)threadIdx.x / kChemGroup;
     constexpr int per = 64 / kChemGroup;
     static_assert(per <= kChemInl, "a wavefront's first entries are inline");
     // segments walked in the placement's XCD bands (their beads are in this
@@ -2027,38 +2069,19 @@ __global__ __launch_bounds__(64) void k_chem_check(const RepInfo *__restrict__ r
     const TileWalk Wk = tile_walk(nseg);
     for (int k = Wk.first; k < Wk.end; k += Wk.stride) {
         const ChemSeg G = CL.seg[k];
-        ChemEnt E = CL.inl[(size_t)k * kChemInl + ge];  // read with the header (used when ge < cnt)
+        // read with the header (used when the segment has at most per entries)
+        const ChemEnt E = CL.inl[(size_t)k * kChemInl + (int)(threadIdx.x & 63) / kChemGroup];
         if (G.cnt <= 0) continue;
         const RepInfo &R = ri[G.rep];
         const RepStep &S = rs[G.rep];
-        for (int e = ge; e < G.cnt; e += per) {
-            if (e != ge) E = *chem_ent(CL, k, G.base, e);  // the wavefront's later rounds
-            const int s = E.sr & ((1 << kRoleShift) - 1), role = (int)((uint32_t)E.sr >> kRoleShift);
-            const uint32_t g = E.w & kGidMask, ty = E.w >> kGidBits;
-            if (!(role & 2)) {  // a break (of a bead in no search)
-                if (gl == 0) {
-                    const int pq = lnk[s];
-                    const uint32_t gp = gt[pq] & kGidMask;
-                    lnk[s] = -1;
-                    lnk[pq] = -1;
-                    if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_BREAK, min(g, gp), max(g, gp), G.rep);
-                }
-                continue;
-            }
-            bool found;
-            int32_t b;
-            uint32_t bg;
-            chem_search(R, S, cs, xy, gt, cls, inj, s, g, make_double2(E.x, E.y), ty == OC_N, gl, found, b, bg);
-            if (gl != 0) continue;
-            if (ty == OC_N) {
-                if (found) {
-                    gt[s] = g | ((uint32_t)OC_T << kGidBits);
-                    if (record) push_event(ev, evn, evcap, err, S.step, OC_EV_CONV, g, g, G.rep);
-                }
-            } else if (b >= 0) {
-                lnk[s] = chem_prop_code(b);
-            }
-        }
+        // a segment of more entries than a round of kChemGroup lanes each
+        // takes is searched kChemGroupWide lanes an entry (fewer rounds; the
+        // few such segments would otherwise end last)
+        if (G.cnt <= per)
+            chem_segment<kChemGroup>(R, S, G, k, CL, true, E, cs, xy, gt, lnk, cls, ev, evn, evcap, record, err, inj);
+        else
+            chem_segment<kChemGroupWide>(R, S, G, k, CL, false, E, cs, xy, gt, lnk, cls, ev, evn, evcap, record, err,
+                                         inj);
     }
 }
 
