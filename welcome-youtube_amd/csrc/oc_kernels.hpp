@@ -441,14 +441,13 @@ __global__ __launch_bounds__(kSortThr) void k_tsort(const RepInfo *__restrict__ 
         if (fast) {
             constexpr int kPer = (kSortRowTab + kSortThr - 1) / kSortThr;
             int v[3][kPer];
+            // every load unconditional (the row index clamped into the staged
+            // rows), so that all of them are in flight together
     #pragma unroll
             for (int q = 0; q < 3; q++) {
                 const int cbq = R.coff + col[q] * ncy;
     #pragma unroll
-                for (int i = 0; i < kPer; i++) {
-                    const int u = t + i * kSortThr;
-                    if (u < nr) v[q][i] = cs_old[cbq + row_of(u)];
-                }
+                for (int i = 0; i < kPer; i++) v[q][i] = cs_old[cbq + row_of(min(t + i * kSortThr, nr - 1))];
             }
             int ve = 0;
             if (t < 6) {
@@ -803,6 +802,12 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
             int8_t mq[kPl];
             int32_t lq[kPl], pq[kPl];
 #pragma unroll
+            for (int k = 0; k < kPl; k++) {
+                mq[k] = 0;
+                lq[k] = -1;
+                pq[k] = -1;
+            }
+#pragma unroll
             for (int k = 0; k < kPl; k++)
                 if (ok[k]) {
                     const int s = sv[k];
@@ -816,15 +821,31 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                     pq[k] = in.prv[s];
                 }
             // (tile, rank) of the partners' old slots; the chemistry partner's
-            // gid word at a precursor-chemistry check
+            // gid word at a precursor-chemistry check.  The loads are
+            // unconditional (index 0 where there is no partner; lanes without a
+            // bead have lq = pq = -1) so that they are in flight together, and
+            // so are the tile offsets after them.
             const bool chk = CL.seg && rs[T.rep].chem_now;
-            int32_t ts[kPl], tp[kPl];
+            int32_t ts[kPl], tp[kPl], os[kPl], op[kPl];
             uint32_t wp[kPl];
 #pragma unroll
             for (int k = 0; k < kPl; k++) {
-                ts[k] = ok[k] && lq[k] >= 0 ? tslot[lq[k]] : -1;
-                tp[k] = ok[k] && mq[k] > 0 && pq[k] >= 0 ? tslot[pq[k]] : -1;
-                wp[k] = chk && ok[k] && mq[k] < 0 && lq[k] >= 0 ? in.gt[lq[k]] : 0u;
+                const int a = tslot[max(lq[k], 0)], b = tslot[max(pq[k], 0)];
+                ts[k] = lq[k] >= 0 ? a : -1;
+                tp[k] = mq[k] > 0 && pq[k] >= 0 ? b : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < kPl; k++) {
+                wp[k] = 0u;
+                if (chk) {
+                    const uint32_t w = in.gt[max(lq[k], 0)];
+                    wp[k] = mq[k] < 0 && lq[k] >= 0 ? w : 0u;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kPl; k++) {
+                os[k] = toff[max(ts[k], 0) >> kSortCapLog];
+                op[k] = toff[max(tp[k], 0) >> kSortCapLog];
             }
 #pragma unroll
             for (int k = 0; k < kPl; k++) {
@@ -846,9 +867,9 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
                 out.mp[d] = m;
                 // the partners' new slots: next / chemistry partner, and the
                 // chain predecessor of a lipid bead past the head
-                const int nl = ts[k] >= 0 ? toff[ts[k] >> kSortCapLog] + (ts[k] & (kSortCap - 1)) : -1;
+                const int nl = ts[k] >= 0 ? os[k] + (ts[k] & (kSortCap - 1)) : -1;
                 out.lnk[d] = nl;
-                out.prv[d] = tp[k] >= 0 ? toff[tp[k] >> kSortCapLog] + (tp[k] & (kSortCap - 1)) : -1;
+                out.prv[d] = tp[k] >= 0 ? op[k] + (tp[k] & (kSortCap - 1)) : -1;
                 if (chk) cflag[d] = chem_class(gq[k], m, nl);
                 if (chk) {
                     const int role = chem_role(R, rs[T.rep], gq[k], m, nl, wp[k]);
