@@ -288,15 +288,23 @@ __device__ __forceinline__ Win chem_window(const RepInfo &R, double2 p, double r
     if (klo < 0) { a0 = klo + R.ncy; b0 = R.ncy - 1; a1 = 0; b1 = khi; }
     else if (khi >= R.ncy) { a0 = klo; b0 = R.ncy - 1; a1 = 0; b1 = khi - R.ncy; }
     else { a0 = klo; b0 = khi; }
+    // all twelve loads unconditional (without a second piece a1 = 0, b1 = -1:
+    // row 0, in bounds), so that they are in flight together
     int v[12];
+    const bool two = b1 >= a1;
 #pragma unroll
     for (int dc = 0; dc < 3; dc++) {
         int col = cx + dc - 1;
         col = col < 0 ? col + R.ncx : (col >= R.ncx ? col - R.ncx : col);
         v[4 * dc] = csf(col, a0);
         v[4 * dc + 1] = csf(col, b0 + 1);
-        v[4 * dc + 2] = b1 >= a1 ? csf(col, a1) : 0;
-        v[4 * dc + 3] = b1 >= a1 ? csf(col, b1 + 1) : 0;
+        v[4 * dc + 2] = csf(col, a1);
+        v[4 * dc + 3] = csf(col, b1 + 1);
+    }
+#pragma unroll
+    for (int dc = 0; dc < 3; dc++) {
+        v[4 * dc + 2] = two ? v[4 * dc + 2] : 0;
+        v[4 * dc + 3] = two ? v[4 * dc + 3] : 0;
     }
     Win W;
     W.total = 0;
@@ -1956,12 +1964,15 @@ __device__ __forceinline__ void chem_search(const RepInfo &R, const RepStep &S, 
     // of up to GRP * kChemCls candidates.
     for (int k0 = gl; k0 < W.total; k0 += GRP * kChemCls) {
         uint8_t fc[kChemCls];
+        int jc[kChemCls];
 #pragma unroll
         for (int i = 0; i < kChemCls; i++) {
             const int kk = k0 + i * GRP;
-            const int j = kk < W.total ? win_at(kk) : -1;
-            fc[i] = j >= 0 && j != s ? cls[j] : 0;
+            jc[i] = kk < W.total ? win_at(kk) : -1;
+            fc[i] = cls[max(jc[i], 0)];  // unconditional: the loads in flight together
         }
+#pragma unroll
+        for (int i = 0; i < kChemCls; i++) fc[i] = jc[i] >= 0 && jc[i] != s ? fc[i] : 0;
         uint32_t mk = 0;
 #pragma unroll
         for (int i = 0; i < kChemCls; i++) mk |= (fc[i] & want) ? 1u << i : 0u;
@@ -1978,11 +1989,10 @@ __device__ __forceinline__ void chem_search(const RepInfo &R, const RepStep &S, 
                 }
             }
 #pragma unroll
-            for (int m = 0; m < kChemPos; m++)  // every load of the round issued before the first is used
-                if (jm[m] >= 0) {
-                    pc[m] = xy[jm[m]];
-                    if (!conv) gc[m] = gt[jm[m]] & kGidMask;
-                }
+            for (int m = 0; m < kChemPos; m++) {  // every load of the round issued before the first is used
+                pc[m] = xy[max(jm[m], 0)];
+                gc[m] = gt[max(jm[m], 0)] & kGidMask;
+            }
 #pragma unroll
             for (int m = 0; m < kChemPos; m++) {
                 if (jm[m] < 0) continue;
