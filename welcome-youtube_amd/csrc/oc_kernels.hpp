@@ -810,24 +810,19 @@ __global__ __launch_bounds__(kSortThr) void k_place(const RepInfo *__restrict__ 
             int8_t mq[kPl];
             int32_t lq[kPl], pq[kPl];
 #pragma unroll
-            for (int k = 0; k < kPl; k++) {
-                mq[k] = 0;
-                lq[k] = -1;
-                pq[k] = -1;
+            for (int k = 0; k < kPl; k++) {  // unconditional (slot 0 for a lane without a bead): in flight together
+                const int s = ok[k] ? sv[k] : 0;
+                xq[k] = in.xy[s];
+                vq[k] = in.v[s];
+                if (INTEG) fq[k] = f[s];
+                else vt[k] = in.vt[s];
+                gq[k] = in.gt[s];
+                const int8_t m = in.mp[s];
+                const int32_t l = in.lnk[s], pv = in.prv[s];
+                mq[k] = ok[k] ? m : (int8_t)0;
+                lq[k] = ok[k] ? l : -1;
+                pq[k] = ok[k] ? pv : -1;
             }
-#pragma unroll
-            for (int k = 0; k < kPl; k++)
-                if (ok[k]) {
-                    const int s = sv[k];
-                    xq[k] = in.xy[s];
-                    vq[k] = in.v[s];
-                    if (INTEG) fq[k] = f[s];
-                    else vt[k] = in.vt[s];
-                    gq[k] = in.gt[s];
-                    mq[k] = in.mp[s];
-                    lq[k] = in.lnk[s];
-                    pq[k] = in.prv[s];
-                }
             // (tile, rank) of the partners' old slots; the chemistry partner's
             // gid word at a precursor-chemistry check.  The loads are
             // unconditional (index 0 where there is no partner; lanes without a
