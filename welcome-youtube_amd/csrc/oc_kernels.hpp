@@ -1306,21 +1306,27 @@ void k_pair(
     // staged rows (thread t: rows t and t + BLOCK, three columns), then the
     // staged beads.  Row r's beads start at the exclusive prefix of the row
     // totals (a block scan of both rows' totals).
+    // (loads unconditional, rows past the staged ones clamped to the last, so
+    // that all twelve are in flight together)
     int f[2][3], n[2][3], tot[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-        const int r = tid + h * BLOCK;
-        const int k = kb0 + r;
+        const int k = kb0 + min(tid + h * BLOCK, nr - 1);
         const int w = k < 0 ? k + ncy : (k < ncy ? k : k - ncy);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            f[h][c] = cs[colb[c] + w];
+            n[h][c] = cs[colb[c] + w + 1];
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const bool in = tid + h * BLOCK < nr;
         tot[h] = 0;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            f[h][c] = 0;
-            n[h][c] = 0;
-            if (r < nr) {
-                f[h][c] = cs[colb[c] + w];
-                n[h][c] = cs[colb[c] + w + 1] - f[h][c];
-            }
+            n[h][c] = in ? n[h][c] - f[h][c] : 0;
+            f[h][c] = in ? f[h][c] : 0;
             tot[h] += n[h][c];
         }
     }
