@@ -44,6 +44,31 @@ struct TapeRef {
     int32_t *lidx, *lcs;
 };
 
+// The tape's list arrays as global-memory pointers for the kernels that read
+// them: TapeRef's pointers are loaded from memory, so the compiler cannot tell
+// their address space and emits flat loads, which also count against the LDS
+// counter, so that every wait for a histogram atomic waits for the loads too.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T *;
+typedef double dvec2 __attribute__((ext_vector_type(2)));  // a position as a plain vector (one 16-byte load)
+struct TapeG {
+    gptr<int32_t> lcs, lidx;
+    gptr<dvec2> lxy;
+    gptr<uint32_t> lgt;
+};
+__device__ __forceinline__ double2 ldxy(gptr<dvec2> p, int i) {
+    const dvec2 v = p[i];
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ TapeG tape_g(const TapeRef &S) {
+    TapeG G;
+    G.lcs = (gptr<int32_t>)S.lcs;
+    G.lidx = (gptr<int32_t>)S.lidx;
+    G.lxy = (gptr<dvec2>)S.lxy;
+    G.lgt = (gptr<uint32_t>)S.lgt;
+    return G;
+}
+
 __device__ __forceinline__ int block_excl_scan256(int v, int *tmp, int &total) {
     // exclusive scan over a 256-thread block; tmp: 4 ints of LDS
     const int inc = wave_incl_scan(v);
@@ -116,6 +141,7 @@ __global__ __launch_bounds__(256) void k_tlist(const RepInfo *__restrict__ ri, c
 template <typename Pre, typename Fn>
 __device__ __forceinline__ void visit_list(const RepInfo &R, const TapeRef &S, int i, double2 p, float cut, int Rc,
                                            int gl, Pre pre, Fn fn) {
+    const TapeG G = tape_g(S);
     const int cx = (int)(p.x * R.icsx) >= R.ncx ? R.ncx - 1 : (int)(p.x * R.icsx);
     const int khi = (int)((p.y + cut) * R.icsy);
     const int klo = (int)((p.y - cut) * R.icsy + 4 * R.ncy) - 4 * R.ncy;
@@ -129,8 +155,8 @@ __device__ __forceinline__ void visit_list(const RepInfo &R, const TapeRef &S, i
         sxq[q] = colu < 0 ? -R.L : (colu >= R.ncx ? R.L : 0.0);
         cbs[q] = R.coff + R.col0 + col * (R.ncy + 1);
         const bool act = q <= 2 * Rc;  // columns past 2 Rc + 1 are empty
-        j0[q] = act ? S.lcs[cbs[q] + pa1] : 0;
-        j1[q] = act ? S.lcs[cbs[q] + pb1 + 1] : 0;
+        j0[q] = act ? G.lcs[cbs[q] + pa1] : 0;
+        j1[q] = act ? G.lcs[cbs[q] + pb1 + 1] : 0;
     }
     int lmax = 0;
 #pragma unroll
@@ -142,13 +168,13 @@ __device__ __forceinline__ void visit_list(const RepInfo &R, const TapeRef &S, i
         for (int q = 0; q < kRdfMaxCol; q++) {
             const int j = j0[q] + off;
             jq[q] = j < j1[q] ? j : i;
-            gq[q] = S.lgt[jq[q]];
+            gq[q] = G.lgt[jq[q]];
         }
         double2 pq[kRdfMaxCol];
 #pragma unroll
         for (int q = 0; q < kRdfMaxCol; q++) {
             if (jq[q] == i || !pre(gq[q])) jq[q] = -1;
-            pq[q] = S.lxy[jq[q] >= 0 ? jq[q] : i];
+            pq[q] = ldxy(G.lxy, jq[q] >= 0 ? jq[q] : i);
         }
 #pragma unroll
         for (int q = 0; q < kRdfMaxCol; q++)
@@ -164,14 +190,14 @@ __device__ __forceinline__ void visit_list(const RepInfo &R, const TapeRef &S, i
             double sy;
             if (piece == 0) {
                 if (klo >= 0) continue;
-                a = S.lcs[cbs[q] + klo + R.ncy]; e = S.lcs[cbs[q] + R.ncy]; sy = -R.L;  // rows wrapped from the top
+                a = G.lcs[cbs[q] + klo + R.ncy]; e = G.lcs[cbs[q] + R.ncy]; sy = -R.L;  // rows wrapped from the top
             } else {
                 if (khi < R.ncy) continue;
-                a = S.lcs[cbs[q]]; e = S.lcs[cbs[q] + khi - R.ncy + 1]; sy = R.L;       // rows wrapped from the bottom
+                a = G.lcs[cbs[q]]; e = G.lcs[cbs[q] + khi - R.ncy + 1]; sy = R.L;       // rows wrapped from the bottom
             }
             for (int j = a + gl; j < e; j += kTlG)
-                if (j != i && pre(S.lgt[j])) {
-                    const double2 qq = S.lxy[j];
+                if (j != i && pre(G.lgt[j])) {
+                    const double2 qq = ldxy(G.lxy, j);
                     fn((float)((p.x - qq.x) - sxq[q]), (float)((p.y - qq.y) - sy));
                 }
         }
@@ -185,7 +211,7 @@ __device__ __forceinline__ int list_block(const RepInfo *ri, const int32_t *bblk
     const int r = bblk[blockIdx.x];
     const RepInfo &R = ri[r];
     const int loc = (blockIdx.x - R.bblk0) * kBeadBlock + threadIdx.x;
-    if (loc < R.N && S.lidx[R.off + loc] >= 0) s_list[atomicAdd(s_n, 1)] = R.off + loc;
+    if (loc < R.N && tape_g(S).lidx[R.off + loc] >= 0) s_list[atomicAdd(s_n, 1)] = R.off + loc;
     __syncthreads();
     return r;
 }
@@ -212,9 +238,9 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdf(const RepInfo *__restrict__ 
     const float pref = 35.0f / (32.0f * A.width), iw = 1.0f / A.width;
     for (int e = threadIdx.x / kTlG; e < s_n; e += kBeadBlock / kTlG) {
         const int i = s_list[e];
-        const uint32_t gti = S.lgt[i], ti = gti >> kGidBits, gi = gti & kGidMask;
+        const uint32_t gti = tape_g(S).lgt[i], ti = gti >> kGidBits, gi = gti & kGidMask;
         visit_list(
-            R, S, i, S.lxy[i], A.cut, A.Rc, gl,
+            R, S, i, ldxy(tape_g(S).lxy, i), A.cut, A.Rc, gl,
             [&](uint32_t gtj) {
                 const uint32_t tj = gtj >> kGidBits;
                 if (A.ta == A.tb) return gi < (gtj & kGidMask);  // each pair once
@@ -293,9 +319,9 @@ __global__ __launch_bounds__(kBeadBlock) void k_rdf_adj(const RepInfo *__restric
         int i = -1;
         if (e < s_n) {
             i = s_list[e];
-            const uint32_t ti = S.lgt[i] >> kGidBits;
+            const uint32_t ti = tape_g(S).lgt[i] >> kGidBits;
             visit_list(
-                R, S, i, S.lxy[i], A.cut, A.Rc, gl,
+                R, S, i, ldxy(tape_g(S).lxy, i), A.cut, A.Rc, gl,
                 [&](uint32_t gtj) {
                     const uint32_t tj = gtj >> kGidBits;
                     return ((int)ti == A.ta && (int)tj == A.tb) || ((int)ti == A.tb && (int)tj == A.ta);
