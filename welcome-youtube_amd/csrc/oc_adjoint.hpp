@@ -50,6 +50,8 @@ struct TapeRef {
 // counter, so that every wait for a histogram atomic waits for the loads too.
 template <class T>
 using gptr = const __attribute__((address_space(1))) T *;
+template <class T>
+using gwptr = __attribute__((address_space(1))) T *;
 typedef double dvec2 __attribute__((ext_vector_type(2)));  // a position as a plain vector (one 16-byte load)
 struct TapeG {
     gptr<int32_t> lcs, lidx;
@@ -90,44 +92,51 @@ __global__ __launch_bounds__(256) void k_tlist(const RepInfo *__restrict__ ri, c
                                                const TapeRef *__restrict__ tape, RdfArgs A, int zero_rg) {
     __shared__ int s_tmp[4];
     const TapeRef S = tape[blockIdx.y];
+    // the tape's arrays as global-memory pointers (no flat accesses)
+    const gptr<int32_t> cs = (gptr<int32_t>)S.cs;
+    const gptr<uint32_t> gt = (gptr<uint32_t>)S.gt;
+    const gptr<dvec2> xy = (gptr<dvec2>)S.xy;
+    const gwptr<int32_t> lcs = (gwptr<int32_t>)S.lcs, lidx = (gwptr<int32_t>)S.lidx;
+    const gwptr<uint32_t> lgt = (gwptr<uint32_t>)S.lgt;
+    const gwptr<dvec2> lxy = (gwptr<dvec2>)S.lxy, rg = (gwptr<dvec2>)S.rg;
     const int2 rc = cols[blockIdx.x];  // (replica, column)
     const RepInfo &R = ri[rc.x];
     const int cb = R.coff + rc.y * R.ncy, lb = R.coff + R.col0 + rc.y * (R.ncy + 1);
-    const int s1 = S.cs[cb + R.ncy];
-    int run = S.cs[cb];
+    const int s1 = cs[cb + R.ncy];
+    int run = cs[cb];
     for (int k0 = 0; k0 < R.ncy; k0 += 256) {
         const int k = k0 + (int)threadIdx.x;
         int a = 0, e = 0, cnt = 0;
         if (k < R.ncy) {
-            a = S.cs[cb + k];
-            e = S.cs[cb + k + 1];
+            a = cs[cb + k];
+            e = cs[cb + k + 1];
             for (int d = a; d < e; d++) {
-                const int t = (int)(S.gt[d] >> kGidBits);
+                const int t = (int)(gt[d] >> kGidBits);
                 cnt += t == A.ta || t == A.tb;
             }
         }
         int total;
         const int base = run + block_excl_scan256(cnt, s_tmp, total);
         if (k < R.ncy) {
-            S.lcs[lb + k] = base;
+            lcs[lb + k] = base;
             int q = base;
             for (int d = a; d < e; d++) {
-                const uint32_t g = S.gt[d];
+                const uint32_t g = gt[d];
                 const int t = (int)(g >> kGidBits);
                 if (t == A.ta || t == A.tb) {
-                    S.lxy[q] = S.xy[d];
-                    S.lgt[q] = g;
-                    S.lidx[q] = d;
+                    lxy[q] = xy[d];
+                    lgt[q] = g;
+                    lidx[q] = d;
                     q++;
                 } else if (zero_rg) {
-                    S.rg[d] = make_double2(0.0, 0.0);
+                    rg[d] = dvec2{0.0, 0.0};
                 }
             }
         }
         run += total;
     }
-    if (threadIdx.x == 0) S.lcs[lb + R.ncy] = run;
-    for (int q = run + (int)threadIdx.x; q < s1; q += 256) S.lidx[q] = -1;
+    if (threadIdx.x == 0) lcs[lb + R.ncy] = run;
+    for (int q = run + (int)threadIdx.x; q < s1; q += 256) lidx[q] = -1;
 }
 
 // The listed beads within `cut` of listed bead i (cut < L/2 - 1): columns
